@@ -1,0 +1,59 @@
+"""Copy the reference's own test data for the hot path into small fixtures.
+
+Reads (as data, never as code) from the reference checkout and writes
+tests/golden/*.npz and *.txt.  Run once in the build container:
+
+    python tests/golden/make_golden.py /root/reference
+
+Contents
+  gk_reference.npz
+    log          gk_integral.dat as float64 [20000, 7] (tests/data/gk_integral.dat)
+    headers      its column names
+    gk_2comp     tests/cpp_regression_data/gk/gk_2comp, float64 [5000, 14]
+                 (GK c_flux[1] c_vcm[1][1], reset(5000), test_gk.cpp:8-35)
+    test_gk      tests/test_gk/test_gk.csv as float64 [2000, 14]
+                 (test_gk.py:4-12: lmax 2000, n_seg 4, reset(T-2000))
+  cli_GK_heat_single.txt, cli_GK_heat_double.txt
+                 tests/data/cli/GK_heat_* (stdout of the CLI, test_cli.sh:31-32)
+  gofr_reference.npz
+    test_gofr    tests/test_gofrt/test_gofr.csv as float64 [10, 12, 200]
+                 (test_gofrt.py:4-13; its input positions.npy is not in the
+                 snapshot, so only structural facts of it are usable)
+"""
+import os
+import shutil
+import sys
+
+import numpy as np
+
+
+def read_csv_column(path):
+    vals = []
+    with open(path) as f:
+        next(f)
+        for line in f:
+            cell = line.split(",")[1].strip()
+            # pandas writes NaN as an empty cell (k_einst at lag 1 is NaN)
+            vals.append(float(cell) if cell else float("nan"))
+    return np.array(vals, dtype=np.float64)
+
+
+def main(ref):
+    out = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(ref, "tests/data/gk_integral.dat")) as f:
+        headers = f.readline().split()
+        rows = [[float(x) for x in line.split()] for line in f if line.strip()]
+    log = np.array(rows, dtype=np.float64)
+    raw = np.fromfile(os.path.join(ref, "tests/cpp_regression_data/gk/gk_2comp"), dtype=np.float64)
+    gk_2comp = raw.reshape(5000, 14)
+    test_gk = read_csv_column(os.path.join(ref, "tests/test_gk/test_gk.csv")).reshape(2000, 14)
+    np.savez_compressed(os.path.join(out, "gk_reference.npz"), log=log, headers=np.array(headers),
+                        gk_2comp=gk_2comp, test_gk=test_gk)
+    for name in ("GK_heat_single", "GK_heat_double"):
+        shutil.copyfile(os.path.join(ref, "tests/data/cli", name), os.path.join(out, "cli_" + name + ".txt"))
+    gofr = read_csv_column(os.path.join(ref, "tests/test_gofrt/test_gofr.csv")).reshape(10, 12, 200)
+    np.savez_compressed(os.path.join(out, "gofr_reference.npz"), test_gofr=gofr)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
