@@ -1,0 +1,199 @@
+"""ctypes binding of the C ABI in include/analisi_hip.h (libanalisi_hip.so).
+
+This is the Python-side binding a maintainer would add next to pyanalisi's
+pybind11 module; the pybind11 classes in ``analisi_amd._pyanalisi`` go
+through the same C ABI from C++.  Loading fails loudly when the library is
+missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libanalisi_hip.so")
+
+ANL_OK = 0
+ANL_ERR_ARG = 1
+ANL_ERR_TOO_SHORT = 2
+ANL_ERR_NOT_LOADED = 3
+ANL_ERR_HIP = 4
+ANL_ERR_NO_DEVICE = 5
+ANL_ERR_HEADER = 6
+
+GOFRT_FP64 = 0
+GOFRT_SCREENED = 1
+GOFRT_AUTO = 2
+
+# (name, restype, argtypes) of every exported entry point
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i = ctypes.c_int
+_d = ctypes.c_double
+_pp = ctypes.POINTER(ctypes.c_void_p)
+SIGNATURES = [
+    ("anl_last_error", ctypes.c_char_p, []),
+    ("anl_init", _i, [_i]),
+    ("anl_info", ctypes.c_char_p, []),
+    ("anl_traj_create", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _u64, _pp]),
+    ("anl_traj_destroy", _i, [_vp]),
+    ("anl_gofrt_create", _i, [_vp, _d, _d, _u32, _u32, _u32, _u32, _i, _pp]),
+    ("anl_gofrt_destroy", _i, [_vp]),
+    ("anl_gofrt_reset", _i, [_vp, _u32, ctypes.POINTER(_u32)]),
+    ("anl_gofrt_calculate", _i, [_vp, _u64, _vp]),
+    ("anl_gofrt_calculate_async", _i, [_vp, _u64, _vp, _vp]),
+    ("anl_gofrt_calc_single_th", _i, [_vp, _i, _i, _i, _i, _u64, _vp]),
+    ("anl_gofrt_calculate_blocks_async", _i, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    ("anl_gofrt_last_stats", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_u64)]),
+    ("anl_msd_calculate", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp]),
+    ("anl_msd_calculate_async", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
+    ("anl_series_create", _i, [_vp, _u64, _u32, _pp]),
+    ("anl_series_destroy", _i, [_vp]),
+    ("anl_gk_calculate", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
+    ("anl_mediavar_update_async", _i, [_vp, _vp, _vp, _u64, _u32, _vp]),
+    ("anl_mediavar_end_async", _i, [_vp, _u64, _u32, _vp]),
+    ("anl_counts_to_double_async", _i, [_vp, _d, _u64, _vp, _vp]),
+]
+
+_lib = None
+
+
+class AnlError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libanalisi_hip.so is not built (python -m analisi_amd.build); "
+                              "analisi_amd has no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != ANL_OK:
+        raise AnlError(rc, lib().anl_last_error().decode())
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class DeviceTrajectory:
+    """anl_traj: device mirror of positions (internal layout), boxes (internal
+    format), type ids and per-type centres of mass."""
+
+    def __init__(self, pos, box_internal, type_id, ntypes, pos_cm=None):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        F, N, _ = self.pos.shape
+        self.box = np.ascontiguousarray(box_internal, dtype=np.float64).reshape(F, -1)
+        self.type_id = np.ascontiguousarray(type_id, dtype=np.int32)
+        self.cm = None if pos_cm is None else np.ascontiguousarray(pos_cm, dtype=np.float64)
+        self.ntypes = int(ntypes)
+        self.nframes, self.natoms = F, N
+        h = ctypes.c_void_p()
+        check(lib().anl_traj_create(_ptr(self.pos), _ptr(self.box), self.box.shape[1], _ptr(self.type_id),
+                                    self.ntypes, _ptr(self.cm), F, N, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().anl_traj_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGofrt:
+    def __init__(self, traj, rmin, rmax, nbin, tmax=0, skip=1, every=1, form=GOFRT_AUTO):
+        self.traj = traj
+        self.nbin = int(nbin)
+        self.nch = traj.ntypes * (traj.ntypes + 1)
+        h = ctypes.c_void_p()
+        check(lib().anl_gofrt_create(traj.handle, rmin, rmax, nbin, tmax, skip, every, form, ctypes.byref(h)))
+        self.handle = h
+        self.leff = 0
+
+    def reset(self, n):
+        leff = _u32()
+        check(lib().anl_gofrt_reset(self.handle, n, ctypes.byref(leff)))
+        self.leff = leff.value
+        return self.leff
+
+    def calculate(self, primo):
+        out = np.zeros((self.leff, self.nch, self.nbin), dtype=np.uint64)
+        check(lib().anl_gofrt_calculate(self.handle, primo, _ptr(out)))
+        return out
+
+    def calc_single_th(self, t, imedia, a0, a1, primo, counts=None):
+        if counts is None:
+            counts = np.zeros((self.leff, self.nch, self.nbin), dtype=np.uint64)
+        check(lib().anl_gofrt_calc_single_th(self.handle, t, imedia, a0, a1, primo, _ptr(counts)))
+        return counts
+
+    def stats(self):
+        f = _i()
+        c = _u64()
+        check(lib().anl_gofrt_last_stats(self.handle, ctypes.byref(f), ctypes.byref(c)))
+        return f.value, c.value
+
+    def close(self):
+        if self.handle:
+            lib().anl_gofrt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def msd(traj, skip, tmax, cm_msd, cm_self, n, primo):
+    leff = n if (n < tmax or tmax == 0) else tmax
+    out = np.zeros((leff, 2 if cm_msd else 1, traj.ntypes), dtype=np.float64)
+    check(lib().anl_msd_calculate(traj.handle, skip, tmax, int(bool(cm_msd)), int(bool(cm_self)), n, primo,
+                                  _ptr(out)))
+    return out
+
+
+class DeviceSeries:
+    def __init__(self, data):
+        self.data = np.ascontiguousarray(data, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().anl_series_create(_ptr(self.data), self.data.shape[0], self.data.shape[1], ctypes.byref(h)))
+        self.handle = h
+
+    def gk(self, idx_j, skip, tmax, n, n_seg, subtract_mean, start_mean, primo):
+        idx = np.ascontiguousarray(idx_j, dtype=np.uint32)
+        ncur = len(idx)
+        leff = n if (n < tmax or tmax == 0) else tmax
+        out = np.zeros((leff, 3 * ncur * ncur + 2), dtype=np.float64)
+        check(lib().anl_gk_calculate(self.handle, _ptr(idx), ncur, skip, tmax, n, n_seg, int(bool(subtract_mean)),
+                                     start_mean, primo, _ptr(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().anl_series_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,98 @@
+// Internal declarations shared by the HIP translation units of libanalisi_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/analisi_hip.h"
+
+namespace anl {
+
+// Thread-local message behind anl_last_error().
+void set_error(const std::string &msg);
+void clear_error();
+
+// Status helpers: every C-ABI body runs inside guard(), which converts
+// exceptions into status codes and the message.
+struct Error {
+    int code;
+    std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+void check_hip(hipError_t e, const char *what);
+#define ANL_HIP(call) ::anl::check_hip((call), #call)
+
+template <class F>
+int guard(F &&f) {
+    try {
+        clear_error();
+        f();
+        return ANL_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return ANL_ERR_ARG;
+    }
+}
+
+void ensure_device();
+
+// Cell parameters of one frame in the internal format (basetrajectory.h:21-43):
+// origin, half lengths, full lengths (2*half, exact) and tilts xy, xz, yz.
+struct Cell {
+    double lo[3];
+    double lh[3];
+    double L[3];
+    double xy, xz, yz;
+};
+
+// g(r,t) screen parameters (16-bit fractional fixed point; see gofrt.hip).
+struct Screen {
+    bool valid;            // preconditions hold for the whole trajectory
+    uint32_t threshold;    // |q|^2 <= threshold passes
+    std::string why_not;   // reason when !valid
+};
+
+}  // namespace anl
+
+// Device-resident trajectory.  Atoms are stored in "slots": sorted by type id,
+// then along a Morton curve of the frame-0 fractional position, so that a
+// wavefront's 64 atoms are spatial neighbours (coherent branches in the pair
+// kernels).  Counts and per-type sums do not depend on this order.
+struct anl_traj {
+    int device = 0;
+    uint64_t F = 0, N = 0;
+    uint32_t nt = 0;
+    uint32_t stride = 6;
+    bool triclinic = false;
+    bool box_const = false;             // every frame has bitwise the same box
+    double4 *pos = nullptr;             // [F][N] slots, w = 0
+    double *box = nullptr;              // [F][9] internal format, tilts 0 if ortho
+    int32_t *type_slot = nullptr;       // [N]
+    int32_t *orig_slot = nullptr;       // [N] original atom index of a slot
+    double *cm = nullptr;               // [F][nt][3]
+    uint2 *frac16 = nullptr;            // [F][N] {a | b<<16, c}, lazily built
+    std::vector<double> box_host;       // [F][9]
+    std::vector<int32_t> type_of_slot;  // host copy
+    std::vector<uint64_t> type_count;   // atoms per type
+    anl::Cell cell0;                    // cell of frame 0
+    anl::Screen screen_cache;
+    double screen_rmax = -1.0;
+};
+
+struct anl_series {
+    int device = 0;
+    uint64_t nrows = 0;
+    uint32_t ncols = 0;
+    double *data = nullptr;
+};
+
+namespace anl {
+Cell cell_of(const double *box9);
+void build_frac16(anl_traj *tr, hipStream_t s);
+}  // namespace anl

@@ -1,0 +1,804 @@
+// g(r,t): van Hove histogram of the minimum-image distance between atom i at
+// frame primo+t0 and atom j at frame primo+t0+t, for every lag t (step
+// every), every t0 (step skip) and every ordered pair (i, j), i == j included
+// (gofrt.cpp:96-122).  Counts are exact integers.
+//
+// Two arithmetic forms produce the same counts:
+//  * FP64 plain  -- every pair goes through the reference's FP64 sequence:
+//    x = xi - xj, iterative min image z->y->x with tilt carry
+//    (basetrajectory.h:224-268), d2 = 0 + x^2 + y^2 + z^2 without FMA
+//    (:200-219), gate rmin2 <= d2 <= rmax2, bin floorf((float)((sqrt(d2) -
+//    rmin)/dr)) (gofrt.cpp:104-118).
+//  * SCREENED   -- every pair is first evaluated in 16-bit fixed-point
+//    fractional coordinates (packed 2-wide integer ops); only pairs whose
+//    screened distance can be <= rmax (plus a proven margin) become
+//    candidates, which are queued per wavefront and evaluated with the exact
+//    FP64 sequence above.  Rejecting a pair in the screen is proven safe when
+//    the cell is constant and rmax < half of every cell width.
+//
+// The bin is never computed with device sqrt/division: the host tabulates
+// T[k] = smallest double d2 whose reference bin is >= k (the reference bin is a
+// monotone step function of d2), so on the device
+//   accept  <=>  lo <= d2 <= rmax2  and  d2 < T[nbin]
+//   bin     =    max{k : T[k] <= d2}
+// which reproduces the reference's counts bit for bit.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "anl_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace anl {
+
+// ---------------------------------------------------------------------------
+// host: exact bin edges
+// ---------------------------------------------------------------------------
+
+// gofrt.cpp:114-115, evaluated on the host with IEEE double sqrt/sub/div and
+// the double->float round-to-nearest cast before floorf.
+static int ref_bin(double d2, double rmin, double dr) {
+    volatile double d = std::sqrt(d2);
+    volatile double q = (d - rmin) / dr;
+    float f = (float)q;
+    return (int)std::floor(f);
+}
+
+static uint64_t dbits(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return u;
+}
+static double bitsd(uint64_t u) {
+    double x;
+    std::memcpy(&x, &u, 8);
+    return x;
+}
+
+// T[k] for k = 0..nbin: smallest non-negative double with ref_bin >= k, by
+// bisection on the (monotone) bit patterns of non-negative doubles.
+static std::vector<double> bin_edges(double rmin, double rmax, uint32_t nbin, double dr) {
+    std::vector<double> T(nbin + 1);
+    double top = (std::fabs(rmax) + std::fabs(rmin) + 4.0 * std::fabs(dr) + 1.0);
+    top = top * top;
+    for (uint32_t k = 0; k <= nbin; k++) {
+        if (ref_bin(0.0, rmin, dr) >= (int)k) {
+            T[k] = 0.0;
+            continue;
+        }
+        if (ref_bin(top, rmin, dr) < (int)k) {
+            T[k] = HUGE_VAL;
+            continue;
+        }
+        uint64_t lo = 0, hi = dbits(top);  // ref_bin(lo) < k <= ref_bin(hi)
+        while (hi - lo > 1) {
+            uint64_t mid = lo + (hi - lo) / 2;
+            if (ref_bin(bitsd(mid), rmin, dr) >= (int)k) hi = mid;
+            else lo = mid;
+        }
+        T[k] = bitsd(hi);
+    }
+    return T;
+}
+
+// ---------------------------------------------------------------------------
+// host: screen preconditions and threshold
+// ---------------------------------------------------------------------------
+
+// Smallest eigenvalue of the symmetric 3x3 metric G = A^T A (cyclic Jacobi).
+static double min_eig_sym3(double G[3][3]) {
+    double a[3][3];
+    std::memcpy(a, G, sizeof(a));
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = std::fabs(a[0][1]) + std::fabs(a[0][2]) + std::fabs(a[1][2]);
+        if (off < 1e-300) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (a[p][q] == 0.0) continue;
+                double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; k++) {
+                    double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+            }
+    }
+    return std::min(a[0][0], std::min(a[1][1], a[2][2]));
+}
+
+// The screen works on q = 16-bit wrapped differences of fractional
+// coordinates.  It may reject a pair only if the reference cannot count it:
+//  (1) constant cell, so both frames share one fractional frame;
+//  (2) rmax < w_k (1/2 - 2^-15) for every cell width w_k: then any image with
+//      d <= rmax is the unique shortest one, the reference's image equals it,
+//      and its fractional components lie strictly inside (-1/2, 1/2), where
+//      16-bit wrapping picks the same image;
+//  (3) |A delta| >= sqrt(lambda_min(A^T A)) |delta|, and rounding each 16-bit
+//      coordinate moves each component of q by at most 1 unit, so
+//      d <= rmax  =>  |q| <= rmax 2^16 / sqrt(lambda_min) + sqrt(3).
+static Screen make_screen(const anl_traj *tr, double rmax) {
+    Screen s{false, 0, ""};
+    if (!tr->box_const) {
+        s.why_not = "the cell changes along the trajectory";
+        return s;
+    }
+    const Cell &c = tr->cell0;
+    double A[3][3] = {{c.L[0], c.xy, c.xz}, {0, c.L[1], c.yz}, {0, 0, c.L[2]}};
+    // inverse of the upper-triangular A
+    double Ai[3][3] = {{0}};
+    Ai[0][0] = 1 / A[0][0];
+    Ai[1][1] = 1 / A[1][1];
+    Ai[2][2] = 1 / A[2][2];
+    Ai[0][1] = -A[0][1] / (A[0][0] * A[1][1]);
+    Ai[1][2] = -A[1][2] / (A[1][1] * A[2][2]);
+    Ai[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / (A[0][0] * A[1][1] * A[2][2]);
+    double wmin = HUGE_VAL;
+    for (int k = 0; k < 3; k++) {
+        double n = std::sqrt(Ai[k][0] * Ai[k][0] + Ai[k][1] * Ai[k][1] + Ai[k][2] * Ai[k][2]);
+        wmin = std::min(wmin, 1.0 / n);
+    }
+    double r = std::fabs(rmax) * (1 + 1e-9) + 1e-12;
+    if (!(r < wmin * (0.5 - 4.0 / 65536.0))) {
+        s.why_not = "rmax is not below half of the smallest cell width";
+        return s;
+    }
+    double G[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) G[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
+    double lam = min_eig_sym3(G) * (1 - 1e-9);
+    if (!(lam > 0)) {
+        s.why_not = "degenerate cell";
+        return s;
+    }
+    double R = r * 65536.0 / std::sqrt(lam) + std::sqrt(3.0) * 1.0001 + 1e-3;
+    double R2 = std::ceil(R * R * (1 + 1e-9));
+    if (!(R2 < 4294967295.0)) {
+        s.why_not = "screen threshold out of range";
+        return s;
+    }
+    s.valid = true;
+    s.threshold = (uint32_t)R2;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+
+constexpr int WG = 256;       // threads per workgroup = i atoms per tile
+constexpr int QCAP = 192;     // per-wavefront candidate queue entries
+
+struct BlockInfo {
+    uint64_t primo;
+    uint32_t leff;
+    uint32_t pad;
+};
+
+struct WorkItem {
+    uint32_t block;
+    uint32_t k0, k1;  // t0 indices [k0, k1): t0 = k * skip
+    uint32_t pad;
+};
+
+struct GofrtArgs {
+    const double4 *pos;
+    const uint2 *frac;
+    const double *box;
+    const int32_t *type_slot;
+    const int32_t *orig_slot;
+    unsigned long long *out;     // [nblocks][leff_max][nch][nbin]
+    unsigned long long *ncand;   // screened: candidate count
+    const double *thr;           // [nbin+1]
+    const int32_t *chan;         // [nt*nt]
+    const WorkItem *work;
+    const BlockInfo *blocks;
+    uint64_t N;
+    uint64_t out_block_stride;   // leff_max * nch * nbin
+    uint32_t nt, nbin, nch, half, skip, every;
+    double lo, rmax2, Tn;
+    float rminf, inv_drf;
+    uint32_t threshold;
+    int32_t a0, a1;              // original-index range of i atoms
+    uint32_t lag0;               // first lag (lag = lag0 + blockIdx.y * every)
+};
+
+struct DevCell {
+    double lh0, lh1, lh2, L0, L1, L2, xy, xz, yz;
+};
+
+__device__ __forceinline__ DevCell load_cell(const double *b) {
+    DevCell c;
+    c.lh0 = b[3];
+    c.lh1 = b[4];
+    c.lh2 = b[5];
+    c.L0 = b[3] * 2;
+    c.L1 = b[4] * 2;
+    c.L2 = b[5] * 2;
+    c.xy = b[6];
+    c.xz = b[7];
+    c.yz = b[8];
+    return c;
+}
+
+// basetrajectory.h:200-268 in the reference's operation order.  A step
+// "if (d<0) d += 2l else d -= 2l" is written d -= s*2l with s = +-1 (exact
+// products; x - (-y) == x + y in IEEE arithmetic).
+template <bool TRI>
+__device__ __forceinline__ double d2_ref(double xi0, double xi1, double xi2, double xj0, double xj1, double xj2,
+                                         const DevCell &c) {
+    double x0 = xi0 - xj0;
+    double x1 = xi1 - xj1;
+    double x2 = xi2 - xj2;
+    // Inputs are validated on the host (finite, within 10^4 cell lengths), so
+    // each loop ends after a few steps; the counters only guard the device.
+    int guard = 0;
+    while (fabs(x2) > c.lh2 && ++guard < (1 << 20)) {
+        double s = x2 < 0.0 ? -1.0 : 1.0;
+        x2 = x2 - s * c.L2;
+        if (TRI) {
+            x1 = x1 - s * c.yz;
+            x0 = x0 - s * c.xz;
+        }
+    }
+    while (fabs(x1) > c.lh1 && ++guard < (2 << 20)) {
+        double s = x1 < 0.0 ? -1.0 : 1.0;
+        x1 = x1 - s * c.L1;
+        if (TRI) x0 = x0 - s * c.xy;
+    }
+    while (fabs(x0) > c.lh0 && ++guard < (3 << 20)) {
+        double s = x0 < 0.0 ? -1.0 : 1.0;
+        x0 = x0 - s * c.L0;
+    }
+    double d2 = __dmul_rn(x0, x0);
+    d2 = __dadd_rn(d2, __dmul_rn(x1, x1));
+    d2 = __dadd_rn(d2, __dmul_rn(x2, x2));
+    return d2;
+}
+
+__device__ __forceinline__ int bin_of(double d2, const double *T, int nbin, float rminf, float inv_drf) {
+    float u = (sqrtf((float)d2) - rminf) * inv_drf;
+    int b = (int)u;
+    b = b < 0 ? 0 : (b > nbin - 1 ? nbin - 1 : b);
+    while (b > 0 && d2 < T[b]) --b;
+    while (b < nbin - 1 && d2 >= T[b + 1]) ++b;
+    return b;
+}
+
+__device__ __forceinline__ void lds_setup(const GofrtArgs &a, unsigned *hist, double *thr, int *chan) {
+    const int nh = (int)(a.nch * a.nbin);
+    for (int k = threadIdx.x; k < nh; k += WG) hist[k] = 0u;
+    for (int k = threadIdx.x; k <= (int)a.nbin; k += WG) thr[k] = a.thr[k];
+    for (int k = threadIdx.x; k < (int)(a.nt * a.nt); k += WG) chan[k] = a.chan[k];
+}
+
+__device__ __forceinline__ void lds_flush(const GofrtArgs &a, const unsigned *hist, uint32_t block, uint32_t t) {
+    const int nh = (int)(a.nch * a.nbin);
+    unsigned long long *o = a.out + (uint64_t)block * a.out_block_stride + (uint64_t)t * nh;
+    for (int k = threadIdx.x; k < nh; k += WG) {
+        unsigned v = hist[k];
+        if (v) atomicAdd(o + k, (unsigned long long)v);
+    }
+}
+
+// grid: x = i tile (256 slots), y = lag index, z = work item (block, t0 range)
+template <bool TRI>
+__global__ __launch_bounds__(WG) void gofrt_fp64_kernel(const GofrtArgs a) {
+    extern __shared__ unsigned char smem[];
+    const WorkItem w = a.work[blockIdx.z];
+    const BlockInfo b = a.blocks[w.block];
+    const uint32_t t = a.lag0 + blockIdx.y * a.every;
+    if (t >= b.leff) return;
+    double *thr = (double *)smem;
+    int *chan = (int *)(thr + a.nbin + 1);
+    unsigned *hist = (unsigned *)(chan + a.nt * a.nt);
+    lds_setup(a, hist, thr, chan);
+    __syncthreads();
+
+    const uint64_t N = a.N;
+    const uint64_t slot = (uint64_t)blockIdx.x * WG + threadIdx.x;
+    bool active = slot < N;
+    int ti = 0;
+    if (active) {
+        int o = a.orig_slot[slot];
+        active = o >= a.a0 && o < a.a1;
+        ti = a.type_slot[slot];
+    }
+    const int nbin = (int)a.nbin;
+    for (uint32_t k = w.k0; k < w.k1; k++) {
+        const uint64_t fi = b.primo + (uint64_t)k * a.skip;
+        const uint64_t fj = fi + t;
+        const DevCell c = load_cell(a.box + fi * 9);
+        const double4 *__restrict__ pj = a.pos + fj * N;
+        if (active) {
+            const double4 xi = a.pos[fi * N + slot];
+            for (uint64_t j = 0; j < N; j++) {
+                const double4 xj = pj[j];
+                double d2 = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
+                if (d2 >= a.lo && d2 <= a.rmax2 && d2 < a.Tn) {
+                    int bb = bin_of(d2, thr, nbin, a.rminf, a.inv_drf);
+                    int tj = a.type_slot[j];
+                    int ch = chan[ti * a.nt + tj] + (slot == j ? (int)a.half : 0);
+                    atomicAdd(&hist[ch * nbin + bb], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    lds_flush(a, hist, w.block, t);
+}
+
+typedef short short2_t __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort2_t __attribute__((ext_vector_type(2)));
+
+// |q|^2 of the 16-bit wrapped fractional difference of (a,b | c) words:
+// packed 2-wide subtractions wrap each component to [-2^15, 2^15), and the
+// sum of the three squares (< 3*2^30) is exact as a 32-bit unsigned value.
+// The c word's high half is 0 - 0 = 0.  The first dot product takes the
+// inline constant 0 as accumulator (VOP3P form), the second accumulates in
+// place, so a pair costs 2 subtractions and 2 dot products.
+__device__ __forceinline__ uint32_t sq16(uint32_t ab_i, uint32_t ab_j, uint32_t c_i, uint32_t c_j) {
+    ushort2_t dab = __builtin_bit_cast(ushort2_t, ab_i) - __builtin_bit_cast(ushort2_t, ab_j);
+    ushort2_t dc = __builtin_bit_cast(ushort2_t, c_i) - __builtin_bit_cast(ushort2_t, c_j);
+    int s;
+    asm("v_dot2_i32_i16 %0, %1, %1, 0" : "=v"(s) : "v"(__builtin_bit_cast(uint32_t, dc)));
+    s = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, dab), __builtin_bit_cast(short2_t, dab), s, false);
+    return (uint32_t)s;
+}
+
+// Loads through the constant address space: uniform addresses become scalar
+// (SMEM) loads, so the j atom's data arrives in SGPRs.
+typedef const __attribute__((address_space(4))) unsigned long long cu64_t;
+
+__device__ __forceinline__ uint32_t lane_prefix(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Exact evaluation of one queued candidate (entry = j << 8 | i_local).
+template <bool TRI>
+__device__ __forceinline__ void eval_candidate(const GofrtArgs &a, uint32_t e, const double4 *xi_lds,
+                                               const int *ti_lds, const double4 *__restrict__ pj, uint64_t slot0,
+                                               const DevCell &c, const double *thr, const int *chan, unsigned *hist) {
+    const uint32_t il = e & 0xFFu;
+    const uint64_t j = e >> 8;
+    const double4 xi = xi_lds[il];
+    const double4 xj = pj[j];
+    double d2 = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
+    if (d2 >= a.lo && d2 <= a.rmax2 && d2 < a.Tn) {
+        const int nbin = (int)a.nbin;
+        int bb = bin_of(d2, thr, nbin, a.rminf, a.inv_drf);
+        int tj = a.type_slot[j];
+        int ch = chan[ti_lds[il] * a.nt + tj] + (slot0 + il == j ? (int)a.half : 0);
+        atomicAdd(&hist[ch * nbin + bb], 1u);
+    }
+}
+
+template <bool TRI>
+__global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
+    extern __shared__ unsigned char smem[];
+    const WorkItem w = a.work[blockIdx.z];
+    const BlockInfo b = a.blocks[w.block];
+    const uint32_t t = a.lag0 + blockIdx.y * a.every;
+    if (t >= b.leff) return;
+    double4 *xi_lds = (double4 *)smem;                     // [WG]
+    uint32_t *queue = (uint32_t *)(xi_lds + WG);           // [WG/64][QCAP]
+    int *ti_lds = (int *)(queue + (WG / 64) * QCAP);       // [WG]
+    double *thr = (double *)(ti_lds + WG);                 // [nbin+1]
+    int *chan = (int *)(thr + a.nbin + 1);                 // [nt*nt]
+    unsigned *hist = (unsigned *)(chan + a.nt * a.nt);     // [nch*nbin]
+    lds_setup(a, hist, thr, chan);
+
+    const uint64_t N = a.N;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t *q = queue + (threadIdx.x >> 6) * QCAP;
+    const uint64_t slot0 = (uint64_t)blockIdx.x * WG;
+    const uint64_t slot = slot0 + threadIdx.x;
+    bool active = slot < N;
+    int ti = 0;
+    if (active) {
+        int o = a.orig_slot[slot];
+        active = o >= a.a0 && o < a.a1;
+        ti = a.type_slot[slot];
+    }
+    ti_lds[threadIdx.x] = ti;
+    const uint32_t il = threadIdx.x;
+    const uint64_t N2 = N & ~(uint64_t)1;
+    // strict comparison against threshold+1; inactive lanes compare against 0
+    // and never pass
+    const uint32_t th = active ? a.threshold + 1u : 0u;
+    unsigned long long ncand = 0;
+
+    for (uint32_t k = w.k0; k < w.k1; k++) {
+        const uint64_t fi = b.primo + (uint64_t)k * a.skip;
+        const uint64_t fj = fi + t;
+        const DevCell c = load_cell(a.box + fi * 9);
+        const double4 *__restrict__ pj = a.pos + fj * N;
+        const uint2 *__restrict__ fjp = a.frac + fj * N;
+        __syncthreads();  // previous t0's candidates are done with xi_lds
+        uint32_t abi = 0, ci = 0;
+        if (slot < N) {
+            xi_lds[threadIdx.x] = a.pos[fi * N + slot];
+            uint2 f = a.frac[fi * N + slot];
+            abi = f.x;
+            ci = f.y;
+        }
+        __syncthreads();
+        uint32_t qn = 0;
+        cu64_t *fjc = (cu64_t *)fjp;
+        const uint32_t N2u = (uint32_t)N2;
+        for (uint32_t j = 0; j < N2u; j += 2) {
+            const unsigned long long g0 = fjc[j];
+            const unsigned long long g1 = fjc[j + 1];
+            const uint32_t s0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32));
+            const uint32_t s1 = sq16(abi, (uint32_t)g1, ci, (uint32_t)(g1 >> 32));
+            const bool p0 = s0 < th;
+            const bool p1 = s1 < th;
+            const unsigned long long m0 = __ballot(p0);
+            const unsigned long long m1 = __ballot(p1);
+            if (m0) {
+                if (p0) q[qn + lane_prefix(m0)] = (j << 8) | il;
+                qn += (uint32_t)__popcll(m0);
+            }
+            if (m1) {
+                if (p1) q[qn + lane_prefix(m1)] = ((j + 1) << 8) | il;
+                qn += (uint32_t)__popcll(m1);
+            }
+            while (qn >= 64) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                qn -= 64;
+                const uint32_t e = q[qn + lane];
+                ncand += 1;
+                eval_candidate<TRI>(a, e, xi_lds, ti_lds, pj, slot0, c, thr, chan, hist);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        if (N2 != N) {  // odd N: last j alone
+            const uint64_t j = N2;
+            const uint2 f0 = fjp[j];
+            const bool p0 = sq16(abi, f0.x, ci, f0.y) < th;
+            const unsigned long long m0 = __ballot(p0);
+            if (m0) {
+                if (p0) q[qn + lane_prefix(m0)] = ((uint32_t)j << 8) | il;
+                qn += (uint32_t)__popcll(m0);
+            }
+        }
+        // drain the queue: up to QCAP-1 entries, 64 at a time
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t base = 0; base < qn; base += 64) {
+            if (base + lane < qn) {
+                const uint32_t e = q[base + lane];
+                ncand += 1;
+                eval_candidate<TRI>(a, e, xi_lds, ti_lds, pj, slot0, c, thr, chan, hist);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // per-wavefront candidate total, one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) ncand += __shfl_down(ncand, off, 64);
+    if (lane == 0 && a.ncand) atomicAdd(a.ncand, ncand);
+    __syncthreads();
+    lds_flush(a, hist, w.block, t);
+}
+
+}  // namespace anl
+
+// ---------------------------------------------------------------------------
+// host state and C ABI
+// ---------------------------------------------------------------------------
+using namespace anl;
+
+struct anl_gofrt {
+    anl_traj *tr = nullptr;
+    double rmin = 0, rmax = 0, dr = 0, rmin2 = 0, rmax2 = 0;
+    uint32_t nbin = 0, lmax = 0, skip = 1, every = 1;
+    int form = ANL_GOFRT_AUTO;
+    uint32_t n = 0, leff = 0;     // from reset()
+    std::vector<double> thr;      // T[0..nbin]
+    double lo = 0;
+    double *d_thr = nullptr;
+    int32_t *d_chan = nullptr;
+    unsigned long long *d_ncand = nullptr;
+    int last_form = -1;
+    uint64_t last_candidates = 0;
+    // scratch for the work list
+    BlockInfo *d_blocks = nullptr;
+    WorkItem *d_work = nullptr;
+    size_t cap_blocks = 0, cap_work = 0;
+};
+
+namespace {
+
+uint32_t nch_of(const anl_traj *tr) { return tr->nt * (tr->nt + 1); }
+
+int choose_form(anl_gofrt *g) {
+    if (g->form == ANL_GOFRT_FP64) return ANL_GOFRT_FP64;
+    if (g->tr->screen_rmax != g->rmax) {
+        g->tr->screen_cache = make_screen(g->tr, g->rmax);
+        g->tr->screen_rmax = g->rmax;
+    }
+    if (g->tr->screen_cache.valid) return ANL_GOFRT_SCREENED;
+    if (g->form == ANL_GOFRT_SCREENED)
+        fail(ANL_ERR_ARG, "the screened g(r,t) form cannot be used: " + g->tr->screen_cache.why_not);
+    return ANL_GOFRT_FP64;
+}
+
+struct Launch {
+    std::vector<BlockInfo> blocks;
+    std::vector<WorkItem> work;
+    uint32_t leff_max = 0;
+};
+
+// Splits every block's t0 range into chunks: a chunk bounds the LDS counters
+// (each t0 adds at most 256*N to one bin: 256*N*chunk < 2^32) and sets the
+// work per workgroup (about 2^31 pair evaluations).
+Launch plan(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, int single_k) {
+    Launch L;
+    const uint64_t N = g->tr->N;
+    uint64_t cap = (0xFFFFFFFFull / ((uint64_t)WG * N));
+    if (cap < 1) cap = 1;
+    uint64_t target = (1ull << 31) / ((uint64_t)WG * N);
+    if (target < 1) target = 1;
+    uint64_t chunk = std::min(cap, target);
+    for (uint32_t bi = 0; bi < nblocks; bi++) {
+        uint32_t leff = (n[bi] < g->lmax || g->lmax == 0) ? n[bi] : g->lmax;
+        L.blocks.push_back(BlockInfo{primo[bi], leff, 0});
+        L.leff_max = std::max(L.leff_max, leff);
+        if (single_k >= 0) {
+            L.work.push_back(WorkItem{bi, (uint32_t)single_k, (uint32_t)single_k + 1, 0});
+            continue;
+        }
+        uint32_t n0 = (n[bi] + g->skip - 1) / g->skip;  // t0 = 0, skip, ... < n
+        for (uint64_t k = 0; k < n0; k += chunk)
+            L.work.push_back(WorkItem{bi, (uint32_t)k, (uint32_t)std::min<uint64_t>(n0, k + chunk), 0});
+    }
+    return L;
+}
+
+void check_lengths(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nblocks) {
+    for (uint32_t bi = 0; bi < nblocks; bi++) {
+        uint32_t leff = (n[bi] < g->lmax || g->lmax == 0) ? n[bi] : g->lmax;
+        // gofrt.cpp:81-83
+        if ((uint64_t)leff + n[bi] + primo[bi] > g->tr->F + 1)
+            fail(ANL_ERR_TOO_SHORT,
+                 "trajectory is too short for this kind of calculation. Select a different starting timestep or "
+                 "lower the size of the average or the lenght of the time lag");
+    }
+}
+
+void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out_len, int a0, int a1,
+            hipStream_t s, uint32_t lag0 = 0, uint32_t nlag_force = 0) {
+    anl_traj *tr = g->tr;
+    const int form = choose_form(g);
+    if (form == ANL_GOFRT_SCREENED) build_frac16(tr, s);
+    if (L.blocks.size() > g->cap_blocks) {
+        (void)hipFree(g->d_blocks);
+        ANL_HIP(hipMalloc(&g->d_blocks, L.blocks.size() * sizeof(BlockInfo)));
+        g->cap_blocks = L.blocks.size();
+    }
+    if (L.work.size() > g->cap_work) {
+        (void)hipFree(g->d_work);
+        ANL_HIP(hipMalloc(&g->d_work, L.work.size() * sizeof(WorkItem)));
+        g->cap_work = L.work.size();
+    }
+    ANL_HIP(hipMemcpyAsync(g->d_blocks, L.blocks.data(), L.blocks.size() * sizeof(BlockInfo), hipMemcpyHostToDevice, s));
+    ANL_HIP(hipMemcpyAsync(g->d_work, L.work.data(), L.work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    ANL_HIP(hipMemsetAsync(out, 0, out_len * sizeof(unsigned long long), s));
+    ANL_HIP(hipMemsetAsync(g->d_ncand, 0, sizeof(unsigned long long), s));
+
+    GofrtArgs a{};
+    a.pos = tr->pos;
+    a.frac = tr->frac16;
+    a.box = tr->box;
+    a.type_slot = tr->type_slot;
+    a.orig_slot = tr->orig_slot;
+    a.out = out;
+    a.ncand = g->d_ncand;
+    a.thr = g->d_thr;
+    a.chan = g->d_chan;
+    a.work = g->d_work;
+    a.blocks = g->d_blocks;
+    a.N = tr->N;
+    a.nt = tr->nt;
+    a.nbin = g->nbin;
+    a.nch = nch_of(tr);
+    a.half = tr->nt * (tr->nt + 1) / 2;
+    a.out_block_stride = (uint64_t)L.leff_max * a.nch * g->nbin;
+    a.skip = g->skip;
+    a.every = g->every;
+    a.lo = g->lo;
+    a.rmax2 = g->rmax2;
+    a.Tn = g->thr[g->nbin];
+    a.rminf = (float)g->rmin;
+    a.inv_drf = (float)(1.0 / g->dr);
+    a.threshold = form == ANL_GOFRT_SCREENED ? tr->screen_cache.threshold : 0;
+    a.a0 = a0;
+    a.a1 = a1;
+
+    const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
+    const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;
+    a.lag0 = lag0;
+    if (L.work.size() > 65535u * 16u) fail(ANL_ERR_ARG, "too many work items");
+    dim3 grid(ntiles, nlag, (uint32_t)L.work.size());
+    size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
+                    (size_t)a.nch * g->nbin * sizeof(unsigned);
+    if (form == ANL_GOFRT_FP64) {
+        size_t lds = common;
+        if (lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
+        if (tr->triclinic) hipLaunchKernelGGL(gofrt_fp64_kernel<true>, grid, dim3(WG), lds, s, a);
+        else hipLaunchKernelGGL(gofrt_fp64_kernel<false>, grid, dim3(WG), lds, s, a);
+    } else {
+        size_t lds = WG * sizeof(double4) + (WG / 64) * QCAP * sizeof(uint32_t) + WG * sizeof(int) + common;
+        if (lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
+        if (tr->triclinic) hipLaunchKernelGGL(gofrt_screened_kernel<true>, grid, dim3(WG), lds, s, a);
+        else hipLaunchKernelGGL(gofrt_screened_kernel<false>, grid, dim3(WG), lds, s, a);
+    }
+    ANL_HIP(hipGetLastError());
+    g->last_form = form;
+}
+
+}  // namespace
+
+extern "C" {
+
+int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint32_t tmax, uint32_t skip,
+                     uint32_t every, int form, anl_gofrt **out) {
+    return guard([&] {
+        if (!tr || !out) fail(ANL_ERR_ARG, "null pointer passed to anl_gofrt_create");
+        if (nbin == 0) fail(ANL_ERR_ARG, "nbin must be positive");
+        if (!(rmax > rmin) || rmin < 0) fail(ANL_ERR_ARG, "need 0 <= rmin < rmax");
+        if (form < ANL_GOFRT_FP64 || form > ANL_GOFRT_AUTO) fail(ANL_ERR_ARG, "unknown g(r,t) form");
+        ensure_device();
+        auto *g = new anl_gofrt();
+        g->tr = tr;
+        g->rmin = rmin;
+        g->rmax = rmax;
+        g->nbin = nbin;
+        g->lmax = tmax;
+        g->skip = skip == 0 ? 1 : skip;     // calculatemultithread.h:43-46
+        g->every = every == 0 ? 1 : every;
+        g->form = form;
+        g->dr = (rmax - rmin) / (double)nbin;  // gofrt.cpp:27-29
+        g->rmax2 = rmax * rmax;
+        g->rmin2 = rmin * rmin;
+        g->thr = bin_edges(rmin, rmax, nbin, g->dr);
+        g->lo = std::max(g->rmin2, g->thr[0]);
+        ANL_HIP(hipMalloc(&g->d_thr, (nbin + 1) * sizeof(double)));
+        ANL_HIP(hipMemcpy(g->d_thr, g->thr.data(), (nbin + 1) * sizeof(double), hipMemcpyHostToDevice));
+        // gofrt.h:86-104: channel of (type1, type2)
+        std::vector<int32_t> chan(tr->nt * tr->nt);
+        for (uint32_t a = 0; a < tr->nt; a++)
+            for (uint32_t b = 0; b < tr->nt; b++) {
+                uint32_t t1 = std::min(a, b), t2 = std::max(a, b);
+                chan[a * tr->nt + b] = (int32_t)(tr->nt * (tr->nt + 1) / 2 - (t2 + 1) * (t2 + 2) / 2 + t1);
+            }
+        ANL_HIP(hipMalloc(&g->d_chan, chan.size() * sizeof(int32_t)));
+        ANL_HIP(hipMemcpy(g->d_chan, chan.data(), chan.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        ANL_HIP(hipMalloc(&g->d_ncand, sizeof(unsigned long long)));
+        *out = g;
+    });
+}
+
+int anl_gofrt_destroy(anl_gofrt *g) {
+    return guard([&] {
+        if (!g) return;
+        (void)hipFree(g->d_thr);
+        (void)hipFree(g->d_chan);
+        (void)hipFree(g->d_ncand);
+        (void)hipFree(g->d_blocks);
+        (void)hipFree(g->d_work);
+        delete g;
+    });
+}
+
+int anl_gofrt_reset(anl_gofrt *g, uint32_t n, uint32_t *leff_out) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        g->n = n;
+        g->leff = (n < g->lmax || g->lmax == 0) ? n : g->lmax;  // gofrt.cpp:56
+        if (leff_out) *leff_out = g->leff;
+    });
+}
+
+int anl_gofrt_calculate_async(anl_gofrt *g, uint64_t primo, uint64_t *counts_dev, void *stream) {
+    return guard([&] {
+        if (!g || !counts_dev) fail(ANL_ERR_ARG, "null pointer passed to anl_gofrt_calculate_async");
+        ensure_device();
+        check_lengths(g, &primo, &g->n, 1);
+        Launch L = plan(g, &primo, &g->n, 1, -1);
+        if (L.leff_max == 0) return;
+        uint64_t len = (uint64_t)L.leff_max * nch_of(g->tr) * g->nbin;
+        launch(g, L, (unsigned long long *)counts_dev, len, INT32_MIN, INT32_MAX, (hipStream_t)stream);
+    });
+}
+
+int anl_gofrt_calculate(anl_gofrt *g, uint64_t primo, uint64_t *counts_host) {
+    return guard([&] {
+        if (!g || !counts_host) fail(ANL_ERR_ARG, "null pointer passed to anl_gofrt_calculate");
+        ensure_device();
+        check_lengths(g, &primo, &g->n, 1);
+        uint64_t len = (uint64_t)g->leff * nch_of(g->tr) * g->nbin;
+        if (len == 0) return;
+        unsigned long long *d = nullptr;
+        ANL_HIP(hipMalloc(&d, len * sizeof(unsigned long long)));
+        try {
+            Launch L = plan(g, &primo, &g->n, 1, -1);
+            launch(g, L, d, len, INT32_MIN, INT32_MAX, 0);
+            ANL_HIP(hipMemcpy(counts_host, d, len * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            ANL_HIP(hipMemcpy(&g->last_candidates, g->d_ncand, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipFree(d);
+    });
+}
+
+int anl_gofrt_calc_single_th(anl_gofrt *g, int t, int imedia, int a0, int a1, uint64_t primo,
+                             uint64_t *counts_host) {
+    return guard([&] {
+        if (!g || !counts_host) fail(ANL_ERR_ARG, "null pointer passed to anl_gofrt_calc_single_th");
+        if (t < 0 || (uint32_t)t >= g->leff || imedia < 0) fail(ANL_ERR_ARG, "lag or t0 out of range");
+        if (primo + (uint64_t)imedia + (uint64_t)t >= g->tr->F) fail(ANL_ERR_TOO_SHORT, "frame out of range");
+        ensure_device();
+        const uint64_t len = (uint64_t)g->leff * nch_of(g->tr) * g->nbin;
+        // one t0 (frame primo+imedia) and one lag t
+        Launch L;
+        L.leff_max = g->leff;
+        L.blocks.push_back(BlockInfo{primo + (uint64_t)imedia, g->leff, 0});
+        L.work.push_back(WorkItem{0, 0, 1, 0});
+        unsigned long long *d = nullptr;
+        ANL_HIP(hipMalloc(&d, len * sizeof(unsigned long long)));
+        std::vector<uint64_t> h(len);
+        try {
+            launch(g, L, d, len, a0, a1, 0, (uint32_t)t, 1);
+            ANL_HIP(hipMemcpy(h.data(), d, len * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipFree(d);
+        for (uint64_t k = 0; k < len; k++) counts_host[k] += h[k];
+    });
+}
+
+int anl_gofrt_calculate_blocks_async(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nblocks,
+                                     uint64_t *counts_dev, void *stream) {
+    return guard([&] {
+        if (!g || !primo || !n || !counts_dev || nblocks == 0)
+            fail(ANL_ERR_ARG, "bad arguments to anl_gofrt_calculate_blocks_async");
+        ensure_device();
+        check_lengths(g, primo, n, nblocks);
+        Launch L = plan(g, primo, n, nblocks, -1);
+        uint64_t len = (uint64_t)nblocks * L.leff_max * nch_of(g->tr) * g->nbin;
+        launch(g, L, (unsigned long long *)counts_dev, len, INT32_MIN, INT32_MAX, (hipStream_t)stream);
+    });
+}
+
+int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        if (form_used) *form_used = g->last_form;
+        if (candidates) {
+            unsigned long long c = 0;
+            ANL_HIP(hipMemcpy(&c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
+            *candidates = c;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,310 @@
+// Runtime of libanalisi_hip: errors, device selection, trajectory and time
+// series mirrors, block statistics kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "anl_internal.h"
+
+namespace anl {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+void check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) fail(ANL_ERR_HIP, std::string("HIP error '") + hipGetErrorString(e) + "' in " + what);
+}
+
+static int g_device = -1;
+
+void ensure_device() {
+    if (g_device >= 0) {
+        ANL_HIP(hipSetDevice(g_device));
+        return;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        fail(ANL_ERR_NO_DEVICE, "no HIP device visible: the analisi_amd kernels need an MI355X (gfx950)");
+    hipDeviceProp_t p;
+    ANL_HIP(hipGetDeviceProperties(&p, 0));
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        fail(ANL_ERR_NO_DEVICE, std::string("device 0 is ") + p.gcnArchName + ", the kernels are built for gfx950");
+    g_device = 0;
+    ANL_HIP(hipSetDevice(0));
+}
+
+Cell cell_of(const double *b) {
+    Cell c;
+    for (int k = 0; k < 3; k++) {
+        c.lo[k] = b[k];
+        c.lh[k] = b[3 + k];
+        c.L[k] = b[3 + k] * 2;
+    }
+    c.xy = b[6];
+    c.xz = b[7];
+    c.yz = b[8];
+    return c;
+}
+
+// Fractional coordinates of x in the cell (x = lo + A s with columns
+// a=(Lx,0,0), b=(xy,Ly,0), c=(xz,yz,Lz)).
+__host__ __device__ static inline void to_fractional(const Cell &c, double x, double y, double z, double *s) {
+    double sc = (z - c.lo[2]) / c.L[2];
+    double sb = (y - c.lo[1] - c.yz * sc) / c.L[1];
+    double sa = (x - c.lo[0] - c.xy * sb - c.xz * sc) / c.L[0];
+    s[0] = sa;
+    s[1] = sb;
+    s[2] = sc;
+}
+
+// 16-bit fixed point of the fractional part: round(frac(s) * 65536) mod 65536.
+__device__ static inline uint32_t fix16(double s) {
+    double f = s - floor(s);
+    double u = rint(f * 65536.0);
+    return (uint32_t)(int64_t)u & 0xFFFFu;
+}
+
+__global__ void frac16_kernel(const double4 *__restrict__ pos, uint2 *__restrict__ out, uint64_t total, Cell c) {
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    double4 x = pos[k];
+    double s[3];
+    to_fractional(c, x.x, x.y, x.z, s);
+    out[k] = make_uint2(fix16(s[0]) | (fix16(s[1]) << 16), fix16(s[2]));
+}
+
+void build_frac16(anl_traj *tr, hipStream_t s) {
+    if (tr->frac16) return;
+    uint64_t total = tr->F * tr->N;
+    ANL_HIP(hipMalloc(&tr->frac16, total * sizeof(uint2)));
+    unsigned blocks = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(frac16_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->frac16, total, tr->cell0);
+    ANL_HIP(hipGetLastError());
+    ANL_HIP(hipStreamSynchronize(s));
+}
+
+static uint32_t spread10(uint32_t v) {
+    v &= 0x3FF;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+__global__ void mediavar_update_kernel(double *__restrict__ mean, double *__restrict__ var,
+                                       const double *__restrict__ x, uint64_t len, double bp1) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    // calcoliblocchi.h:35-58, element-wise, same operation order
+    double delta = x[i] - mean[i];
+    double tmp = delta / bp1;
+    double m = mean[i] + tmp;
+    mean[i] = m;
+    tmp = x[i] - m;
+    tmp = tmp * delta;
+    var[i] = var[i] + tmp;
+}
+
+__global__ void div_kernel(double *__restrict__ v, uint64_t len, double d) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) v[i] = v[i] / d;
+}
+
+__global__ void counts_to_double_kernel(const unsigned long long *__restrict__ c, double incr, uint64_t len,
+                                        double *__restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) out[i] = (double)c[i] * incr;
+}
+
+}  // namespace anl
+
+using namespace anl;
+
+extern "C" {
+
+const char *anl_last_error(void) { return g_last_error.c_str(); }
+
+int anl_init(int device) {
+    return guard([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+            fail(ANL_ERR_NO_DEVICE, "no HIP device visible: the analisi_amd kernels need an MI355X (gfx950)");
+        if (device < 0 || device >= n) fail(ANL_ERR_ARG, "device index out of range");
+        hipDeviceProp_t p;
+        ANL_HIP(hipGetDeviceProperties(&p, device));
+        if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+            fail(ANL_ERR_NO_DEVICE, std::string("device is ") + p.gcnArchName + ", the kernels are built for gfx950");
+        g_device = device;
+        ANL_HIP(hipSetDevice(device));
+    });
+}
+
+const char *anl_info(void) {
+    return "analisi_amd: MI355X (gfx950) g(r,t) / MSD / Green-Kubo lag-correlation path; "
+           "restates rikigigi/analisi v0.5.2 behaviour (Gofrt, MSD, GreenKuboNComponentIonicFluid, MediaVar)";
+}
+
+int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, const int32_t *type_id,
+                    uint32_t ntypes, const double *pos_cm, uint64_t nframes, uint64_t natoms, anl_traj **out) {
+    return guard([&] {
+        if (!pos || !box || !type_id || !out) fail(ANL_ERR_ARG, "null pointer passed to anl_traj_create");
+        if (box_stride != 6 && box_stride != 9) fail(ANL_ERR_ARG, "box stride must be 6 or 9");
+        if (nframes == 0 || natoms == 0 || ntypes == 0) fail(ANL_ERR_ARG, "empty trajectory");
+        if (natoms >= (1u << 24)) fail(ANL_ERR_ARG, "at most 2^24-1 atoms are supported");
+        for (uint64_t i = 0; i < natoms; i++)
+            if (type_id[i] < 0 || (uint32_t)type_id[i] >= ntypes) fail(ANL_ERR_ARG, "type id out of range");
+        // The device min-image loops step by one cell length per iteration;
+        // bounded, finite inputs keep every loop short and finite.
+        double lmin = HUGE_VAL;
+        for (uint64_t f = 0; f < nframes; f++)
+            for (int k = 3; k < 6; k++) {
+                double h = box[f * box_stride + k];
+                if (!std::isfinite(h) || !(h > 0)) fail(ANL_ERR_ARG, "cell half-lengths must be finite and positive");
+                lmin = std::min(lmin, h);
+            }
+        for (uint64_t f = 0; f < nframes; f++)
+            for (uint32_t k = 6; k < box_stride; k++)
+                if (!std::isfinite(box[f * box_stride + k])) fail(ANL_ERR_ARG, "cell tilts must be finite");
+        const double bound = 1e4 * 2 * lmin;
+        for (uint64_t i = 0; i < nframes * natoms * 3; i++)
+            if (!(std::fabs(pos[i]) <= bound))
+                fail(ANL_ERR_ARG, "positions must be finite and within 10^4 (shortest) cell lengths of the origin");
+        ensure_device();
+        auto *tr = new anl_traj();
+        tr->F = nframes;
+        tr->N = natoms;
+        tr->nt = ntypes;
+        tr->stride = box_stride;
+        tr->triclinic = box_stride == 9;
+        tr->box_host.assign(nframes * 9, 0.0);
+        for (uint64_t f = 0; f < nframes; f++)
+            for (uint32_t k = 0; k < box_stride; k++) tr->box_host[f * 9 + k] = box[f * box_stride + k];
+        tr->box_const = true;
+        for (uint64_t f = 1; f < nframes && tr->box_const; f++)
+            tr->box_const = std::memcmp(&tr->box_host[f * 9], &tr->box_host[0], 9 * sizeof(double)) == 0;
+        tr->cell0 = cell_of(&tr->box_host[0]);
+
+        // slot order: type id, then Morton key of the frame-0 fractional position
+        std::vector<uint64_t> key(natoms);
+        for (uint64_t i = 0; i < natoms; i++) {
+            double s[3];
+            to_fractional(tr->cell0, pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2], s);
+            uint32_t q[3];
+            for (int k = 0; k < 3; k++) {
+                double f = s[k] - std::floor(s[k]);
+                q[k] = std::min<uint32_t>(1023u, (uint32_t)(f * 1024.0));
+            }
+            uint64_t m = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+            key[i] = ((uint64_t)type_id[i] << 32) | m;
+        }
+        std::vector<int32_t> order(natoms);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+        tr->type_of_slot.resize(natoms);
+        tr->type_count.assign(ntypes, 0);
+        for (uint64_t s = 0; s < natoms; s++) {
+            tr->type_of_slot[s] = type_id[order[s]];
+            tr->type_count[type_id[order[s]]]++;
+        }
+        // permuted, padded positions, uploaded frame by frame
+        ANL_HIP(hipMalloc(&tr->pos, nframes * natoms * sizeof(double4)));
+        std::vector<double4> buf(natoms);
+        for (uint64_t f = 0; f < nframes; f++) {
+            const double *p = pos + f * natoms * 3;
+            for (uint64_t s = 0; s < natoms; s++) {
+                const double *x = p + (uint64_t)order[s] * 3;
+                buf[s] = make_double4(x[0], x[1], x[2], 0.0);
+            }
+            ANL_HIP(hipMemcpy(tr->pos + f * natoms, buf.data(), natoms * sizeof(double4), hipMemcpyHostToDevice));
+        }
+        ANL_HIP(hipMalloc(&tr->box, nframes * 9 * sizeof(double)));
+        ANL_HIP(hipMemcpy(tr->box, tr->box_host.data(), nframes * 9 * sizeof(double), hipMemcpyHostToDevice));
+        ANL_HIP(hipMalloc(&tr->type_slot, natoms * sizeof(int32_t)));
+        ANL_HIP(hipMemcpy(tr->type_slot, tr->type_of_slot.data(), natoms * sizeof(int32_t), hipMemcpyHostToDevice));
+        ANL_HIP(hipMalloc(&tr->orig_slot, natoms * sizeof(int32_t)));
+        ANL_HIP(hipMemcpy(tr->orig_slot, order.data(), natoms * sizeof(int32_t), hipMemcpyHostToDevice));
+        if (pos_cm) {
+            ANL_HIP(hipMalloc(&tr->cm, nframes * ntypes * 3 * sizeof(double)));
+            ANL_HIP(hipMemcpy(tr->cm, pos_cm, nframes * ntypes * 3 * sizeof(double), hipMemcpyHostToDevice));
+        }
+        *out = tr;
+    });
+}
+
+int anl_traj_destroy(anl_traj *tr) {
+    return guard([&] {
+        if (!tr) return;
+        (void)hipFree(tr->pos);
+        (void)hipFree(tr->box);
+        (void)hipFree(tr->type_slot);
+        (void)hipFree(tr->orig_slot);
+        (void)hipFree(tr->cm);
+        (void)hipFree(tr->frac16);
+        delete tr;
+    });
+}
+
+int anl_series_create(const double *data, uint64_t nrows, uint32_t ncols, anl_series **out) {
+    return guard([&] {
+        if (!data || !out) fail(ANL_ERR_ARG, "null pointer passed to anl_series_create");
+        if (nrows == 0 || ncols == 0) fail(ANL_ERR_ARG, "empty time series");
+        ensure_device();
+        auto *s = new anl_series();
+        s->nrows = nrows;
+        s->ncols = ncols;
+        ANL_HIP(hipMalloc(&s->data, nrows * ncols * sizeof(double)));
+        ANL_HIP(hipMemcpy(s->data, data, nrows * ncols * sizeof(double), hipMemcpyHostToDevice));
+        *out = s;
+    });
+}
+
+int anl_series_destroy(anl_series *s) {
+    return guard([&] {
+        if (!s) return;
+        (void)hipFree(s->data);
+        delete s;
+    });
+}
+
+int anl_mediavar_update_async(double *mean_dev, double *var_dev, const double *x_dev, uint64_t len, uint32_t iblock,
+                              void *stream) {
+    return guard([&] {
+        if (!mean_dev || !var_dev || !x_dev) fail(ANL_ERR_ARG, "null pointer passed to anl_mediavar_update_async");
+        if (len == 0) return;
+        ensure_device();
+        hipLaunchKernelGGL(mediavar_update_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, mean_dev, var_dev, x_dev, len, (double)(iblock + 1));
+        ANL_HIP(hipGetLastError());
+    });
+}
+
+int anl_mediavar_end_async(double *var_dev, uint64_t len, uint32_t n_b, void *stream) {
+    return guard([&] {
+        if (!var_dev) fail(ANL_ERR_ARG, "null pointer passed to anl_mediavar_end_async");
+        if (len == 0) return;
+        ensure_device();
+        double d = (double)((n_b - 1) * n_b);  // unsigned product, then to double (calcoliblocchi.h:60)
+        hipLaunchKernelGGL(div_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, (hipStream_t)stream, var_dev,
+                           len, d);
+        ANL_HIP(hipGetLastError());
+    });
+}
+
+int anl_counts_to_double_async(const uint64_t *counts_dev, double incr, uint64_t len, double *out_dev, void *stream) {
+    return guard([&] {
+        if (!counts_dev || !out_dev) fail(ANL_ERR_ARG, "null pointer passed to anl_counts_to_double_async");
+        if (len == 0) return;
+        ensure_device();
+        hipLaunchKernelGGL(counts_to_double_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, (const unsigned long long *)counts_dev, incr, len, out_dev);
+        ANL_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

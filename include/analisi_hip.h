@@ -1,0 +1,139 @@
+/*
+ * analisi_hip.h -- C ABI of the MI355X (gfx950) time-lagged pair-correlation
+ * path: g(r,t) histograms, mean square displacement, Green-Kubo lag
+ * correlations and block statistics.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Every call returns an
+ * ANL_* status; on failure anl_last_error() returns the message, which the
+ * C++ operator classes rethrow as std::runtime_error with the reference's
+ * wording (gofrt.cpp:81-83, msd.cpp:54-60, greenkuboNcomponentionicfluid.cpp:
+ * 168-169, :67).  Calls are synchronous unless named *_async.  A handle is
+ * not re-entrant: one calculation at a time per handle, as in the reference
+ * (th_data is a member, calculatemultithread.h:106-162).
+ *
+ * Each entry point names the reference interface it replaces (file:line of
+ * rikigigi/analisi v0.5.2).
+ */
+#ifndef ANALISI_HIP_H
+#define ANALISI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ANL_OK 0
+#define ANL_ERR_ARG 1          /* invalid argument (shape, null pointer, sizes) */
+#define ANL_ERR_TOO_SHORT 2    /* trajectory / series too short for (primo, n, lags) */
+#define ANL_ERR_NOT_LOADED 3   /* MSD: not enough loaded timesteps (msd.cpp:56-59) */
+#define ANL_ERR_HIP 4          /* HIP runtime error */
+#define ANL_ERR_NO_DEVICE 5    /* no gfx950 device visible */
+#define ANL_ERR_HEADER 6       /* GK header not found (greenkubo...cpp:62-69) */
+
+/* g(r,t) arithmetic forms */
+#define ANL_GOFRT_FP64 0       /* plain FP64 evaluation of every ordered pair */
+#define ANL_GOFRT_SCREENED 1   /* 16-bit fixed-point screen of every pair, exact FP64
+                                  evaluation of every candidate (same counts) */
+#define ANL_GOFRT_AUTO 2       /* SCREENED when its preconditions hold, else FP64 */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *anl_last_error(void);
+
+/* Select and initialise the HIP device (default 0).  Fails with
+ * ANL_ERR_NO_DEVICE when no gfx950 device is present. */
+int anl_init(int device);
+
+/* Build/version string (replaces pyanalisi.info(), config.h.in:49-64). */
+const char *anl_info(void);
+
+/* ---- trajectory mirror ------------------------------------------------- */
+/* Device-resident copy of what BaseTrajectory serves to the calculations:
+ * positions<false>(t,i) (trajectory_numpy.h:18-19), box<false>(t) in the
+ * internal format (basetrajectory.h:21-43, :94-105), get_type(i) ids
+ * (basetrajectory.h:71-77, basetrajectory.cpp:51-89) and positions_cm(t,type)
+ * (trajectory_numpy.cpp:201-225).  The host arrays are read once; the mirror
+ * is immutable afterwards (Trajectory_numpy data never changes after construction).
+ *   pos      [nframes][natoms][3] FP64
+ *   box      [nframes][box_stride] FP64, internal format, box_stride 6 or 9
+ *   type_id  [natoms] int32, dense ids 0..ntypes-1
+ *   pos_cm   [nframes][ntypes][3] FP64 (may be NULL when MSD is not used) */
+typedef struct anl_traj anl_traj;
+int anl_traj_create(const double *pos, const double *box, uint32_t box_stride,
+                    const int32_t *type_id, uint32_t ntypes, const double *pos_cm,
+                    uint64_t nframes, uint64_t natoms, anl_traj **out);
+int anl_traj_destroy(anl_traj *tr);
+
+/* ---- g(r,t) ------------------------------------------------------------ */
+/* Replaces Gofrt<double,T> (gofrt.h:37-45, gofrt.cpp:22-155) driven by
+ * CalculateMultiThread::calculate (calculatemultithread.h:106-162).
+ * Counts are exact integers: the reference's vdata[k] is counts[k]*incr with
+ * incr = 1/int(n/skip) (gofrt.cpp:91-92).  Output layout
+ * [leff][ntypes*(ntypes+1)][nbin] (gofrt.cpp:65-71). */
+typedef struct anl_gofrt anl_gofrt;
+int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint32_t tmax,
+                     uint32_t skip, uint32_t every, int form, anl_gofrt **out);
+int anl_gofrt_destroy(anl_gofrt *g);
+/* Gofrt::reset(n) (gofrt.cpp:41-63): leff = min(n, tmax) (tmax 0 = n). */
+int anl_gofrt_reset(anl_gofrt *g, uint32_t n, uint32_t *leff_out);
+/* Gofrt::calculate(primo): whole (lags x t0 x ordered pairs) nest in one
+ * launch; counts_host receives [leff][nch][nbin] uint64. */
+int anl_gofrt_calculate(anl_gofrt *g, uint64_t primo, uint64_t *counts_host);
+/* Same, device output (counts_dev zeroed and filled), enqueued on stream
+ * (a hipStream_t, NULL = default stream), no host synchronisation. */
+int anl_gofrt_calculate_async(anl_gofrt *g, uint64_t primo, uint64_t *counts_dev, void *stream);
+/* Gofrt::calc_single_th(t, imedia, a0, a1, primo, ith) (gofrt.cpp:96-122):
+ * one lag t, one t0 offset imedia, atoms a0 <= i < a1 against all j;
+ * ADDS into counts_host [leff][nch][nbin]. */
+int anl_gofrt_calc_single_th(anl_gofrt *g, int t, int imedia, int a0, int a1, uint64_t primo,
+                             uint64_t *counts_host);
+/* Uneven blocks in one launch (replaces the per-block loop of
+ * BlockAverageG::calcola_custom, blockaverage.h:128-144, for Gofrt): block b
+ * covers t0 in [0, n_b) from frame primo_b with leff_b = min(n_b, tmax) lags;
+ * counts_dev is [nblocks][leff_max][nch][nbin], leff_max = max leff_b. */
+int anl_gofrt_calculate_blocks_async(anl_gofrt *g, const uint64_t *primo, const uint32_t *n,
+                                     uint32_t nblocks, uint64_t *counts_dev, void *stream);
+/* Form actually used by the last calculate (ANL_GOFRT_FP64/SCREENED) and the
+ * number of candidate pairs the screen passed to the exact evaluation. */
+int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates);
+
+/* ---- mean square displacement ----------------------------------------- */
+/* Replaces MSD<T> (msd.h:30-47, msd.cpp:40-146): out_host [leff][f_cm][ntypes]
+ * FP64, f_cm = 2 when cm_msd, per-type mean over (t0 in 0,skip,..<n, atoms);
+ * cm_self subtracts the per-type centre-of-mass displacement. */
+int anl_msd_calculate(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self,
+                      uint32_t n, uint64_t primo, double *out_host);
+int anl_msd_calculate_async(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self,
+                            uint32_t n, uint64_t primo, double *out_dev, void *stream);
+
+/* ---- time series and Green-Kubo --------------------------------------- */
+/* Device mirror of ReadLog_numpy (readlog_numpy.h:6-26): row-major
+ * [nrows][ncols] FP64. */
+typedef struct anl_series anl_series;
+int anl_series_create(const double *data, uint64_t nrows, uint32_t ncols, anl_series **out);
+int anl_series_destroy(anl_series *s);
+/* Replaces GreenKuboNComponentIonicFluid::calculate (greenkubo...cpp:156-380)
+ * with reset(n) semantics (:122-131): idx_j[a] is the first column of current
+ * a's (x,y,z) triplet; out_host [leff][3*ncur^2+2] FP64 in the reference
+ * column order (C, (I,E) pairs, k_gk, k_einst). */
+int anl_gk_calculate(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
+                     uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
+                     uint32_t start_mean, uint64_t primo, double *out_host);
+
+/* ---- block statistics --------------------------------------------------- */
+/* MediaVar::calculate (calcoliblocchi.h:35-58) on device vectors of length
+ * len: one Welford step with x = block iblock (0-based); mean/var updated in
+ * place.  anl_mediavar_end divides var by (n_b-1)*n_b (:59-61). */
+int anl_mediavar_update_async(double *mean_dev, double *var_dev, const double *x_dev, uint64_t len,
+                              uint32_t iblock, void *stream);
+int anl_mediavar_end_async(double *var_dev, uint64_t len, uint32_t n_b, void *stream);
+/* Converts counts_dev (uint64) to the reference FP64 layout: out = count*incr. */
+int anl_counts_to_double_async(const uint64_t *counts_dev, double incr, uint64_t len, double *out_dev,
+                               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANALISI_HIP_H */
