@@ -45,7 +45,7 @@ SIGNATURES = [
     ("anl_gofrt_calculate_async", _i, [_vp, _u64, _vp, _vp]),
     ("anl_gofrt_calc_single_th", _i, [_vp, _i, _i, _i, _i, _u64, _vp]),
     ("anl_gofrt_calculate_blocks_async", _i, [_vp, _vp, _vp, _u32, _vp, _vp]),
-    ("anl_gofrt_last_stats", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_u64)]),
+    ("anl_gofrt_last_stats", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     ("anl_msd_calculate", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp]),
     ("anl_msd_calculate_async", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
     ("anl_series_create", _i, [_vp, _u64, _u32, _pp]),
@@ -146,10 +146,12 @@ class DeviceGofrt:
         return counts
 
     def stats(self):
+        """(form used, candidates, FP64 rechecks) of the last calculate."""
         f = _i()
         c = _u64()
-        check(lib().anl_gofrt_last_stats(self.handle, ctypes.byref(f), ctypes.byref(c)))
-        return f.value, c.value
+        r = _u64()
+        check(lib().anl_gofrt_last_stats(self.handle, ctypes.byref(f), ctypes.byref(c), ctypes.byref(r)))
+        return f.value, c.value, r.value
 
     def close(self):
         if self.handle:

@@ -56,14 +56,16 @@ struct Screen {
     bool valid;            // preconditions hold for the whole trajectory
     uint32_t threshold;    // |q|^2 <= threshold passes
     std::string why_not;   // reason when !valid
+    double frac_max;       // bound of |fractional component| of a candidate
+    double row_sum;        // max over rows of the cell matrix of sum |A_ik|
 };
 
 }  // namespace anl
 
-// Device-resident trajectory.  Atoms are stored in "slots": sorted by type id,
-// then along a Morton curve of the frame-0 fractional position, so that a
-// wavefront's 64 atoms are spatial neighbours (coherent branches in the pair
-// kernels).  Counts and per-type sums do not depend on this order.
+// Device-resident trajectory.  Atoms are stored in "slots" sorted along a
+// Morton curve of the frame-0 fractional position, so that a wavefront's 64
+// atoms are spatial neighbours (coherent branches in the pair kernels).
+// Counts and per-type sums do not depend on this order.
 struct anl_traj {
     int device = 0;
     uint64_t F = 0, N = 0;
@@ -76,7 +78,8 @@ struct anl_traj {
     int32_t *type_slot = nullptr;       // [N]
     int32_t *orig_slot = nullptr;       // [N] original atom index of a slot
     double *cm = nullptr;               // [F][nt][3]
-    uint2 *frac16 = nullptr;            // [F][N] {a | b<<16, c}, lazily built
+    uint2 *frac16 = nullptr;            // [F][N] {a | b<<16, c} 16-bit fractions, lazily built
+    uint4 *frac32 = nullptr;            // [F][N] {a, b, c, 0} 32-bit fractions, lazily built
     std::vector<double> box_host;       // [F][9]
     std::vector<int32_t> type_of_slot;  // host copy
     std::vector<uint64_t> type_count;   // atoms per type

@@ -125,7 +125,7 @@ static double min_eig_sym3(double G[3][3]) {
 //      coordinate moves each component of q by at most 1 unit, so
 //      d <= rmax  =>  |q| <= rmax 2^16 / sqrt(lambda_min) + sqrt(3).
 static Screen make_screen(const anl_traj *tr, double rmax) {
-    Screen s{false, 0, ""};
+    Screen s{false, 0, "", 0.5, 0.0};
     if (!tr->box_const) {
         s.why_not = "the cell changes along the trajectory";
         return s;
@@ -166,7 +166,31 @@ static Screen make_screen(const anl_traj *tr, double rmax) {
     }
     s.valid = true;
     s.threshold = (uint32_t)R2;
+    s.frac_max = std::min(0.5, std::sqrt(R2) / 65536.0 + 2.0 / 65536.0);
+    s.row_sum = 0;
+    for (int i = 0; i < 3; i++)
+        s.row_sum = std::max(s.row_sum, std::fabs(A[i][0]) + std::fabs(A[i][1]) + std::fabs(A[i][2]));
     return s;
+}
+
+// Window (in bins) of the FP32 candidate evaluation.  Error sources, per
+// Cartesian component of a candidate (fractional components <= frac_max):
+// 32-bit quantisation of both atoms (2^-32 of each cell row), int->float
+// conversion, float cell constants and two FMAs (<= 2^-24 relative each, five
+// in all, bounded by 10 ulps of the largest term).  On d: sqrt(3) times the
+// component error plus the rounding of the three-term sum and the square
+// root.  On u = (d - rmin)/dr: that over dr plus the subtraction/product and
+// the reference's own double->float rounding of u.  The window is four times
+// the bound.
+static float du_window(const Screen &s, double rmin, double dr, uint32_t nbin) {
+    const double e24 = std::ldexp(1.0, -24);
+    const double term = s.row_sum * s.frac_max;
+    const double e_comp = s.row_sum * std::ldexp(1.0, -31) + 10.0 * e24 * term;
+    const double dmax = std::sqrt(3.0) * term + std::fabs(rmin) + 1.0;
+    const double e_d = std::sqrt(3.0) * e_comp + 8.0 * e24 * dmax;
+    const double umax = dmax / dr + nbin + 1.0;
+    const double du = e_d / dr + 8.0 * e24 * umax;
+    return (float)(4.0 * du + 1e-6);
 }
 
 // ---------------------------------------------------------------------------
@@ -208,6 +232,11 @@ struct GofrtArgs {
     uint32_t threshold;
     int32_t a0, a1;              // original-index range of i atoms
     uint32_t lag0;               // first lag (lag = lag0 + blockIdx.y * every)
+    // screened form: 32-bit fractions and the FP32 evaluation of candidates
+    const uint4 *frac32;
+    unsigned long long *nrecheck;
+    float cx, cxy, cxz, cy, cyz, cz;  // cell matrix entries times 2^-32
+    float du_margin;                  // bins: |u - nearest integer| below this -> FP64
 };
 
 struct DevCell {
@@ -361,24 +390,73 @@ __device__ __forceinline__ uint32_t lane_prefix(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Exact evaluation of one queued candidate (entry = j << 8 | i_local).
+// One queued candidate (entry = j << 8 | i_local).  The distance is first
+// evaluated in FP32 from the 32-bit fractional difference (the wrapped
+// difference is the minimum image for every pair the reference can count);
+// if u = (d - rmin)/dr lies within du_margin of an integer (a bin edge, rmin
+// or rmax) the pair is re-evaluated with the reference's exact FP64 sequence.
+// du_margin is four times a worst-case bound of |u_fp32 - u_reference|, so
+// outside the window both floor to the same bin.
 template <bool TRI>
-__device__ __forceinline__ void eval_candidate(const GofrtArgs &a, uint32_t e, const double4 *xi_lds,
-                                               const int *ti_lds, const double4 *__restrict__ pj, uint64_t slot0,
-                                               const DevCell &c, const double *thr, const int *chan, unsigned *hist) {
+__device__ __forceinline__ void eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const int *ti_lds,
+                                               const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
+                                               const double4 *__restrict__ pj, uint64_t slot0, const DevCell &c,
+                                               const double *thr, const int *chan, unsigned *hist,
+                                               unsigned long long &nre) {
     const uint32_t il = e & 0xFFu;
-    const uint64_t j = e >> 8;
-    const double4 xi = xi_lds[il];
-    const double4 xj = pj[j];
-    double d2 = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
-    if (d2 >= a.lo && d2 <= a.rmax2 && d2 < a.Tn) {
-        const int nbin = (int)a.nbin;
-        int bb = bin_of(d2, thr, nbin, a.rminf, a.inv_drf);
-        int tj = a.type_slot[j];
-        int ch = chan[ti_lds[il] * a.nt + tj] + (slot0 + il == j ? (int)a.half : 0);
-        atomicAdd(&hist[ch * nbin + bb], 1u);
+    const uint32_t j = e >> 8;
+    const uint4 ui = fi_lds[il];
+    const uint4 uj = fj32[j];
+    const float fa = (float)(int)(ui.x - uj.x);
+    const float fb = (float)(int)(ui.y - uj.y);
+    const float fc = (float)(int)(ui.z - uj.z);
+    float dx, dy, dz;
+    if (TRI) {
+        dz = a.cz * fc;
+        dy = fmaf(a.cyz, fc, a.cy * fb);
+        dx = fmaf(a.cxz, fc, fmaf(a.cxy, fb, a.cx * fa));
+    } else {
+        dx = a.cx * fa;
+        dy = a.cy * fb;
+        dz = a.cz * fc;
+    }
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
+    const float k = rintf(u);
+    const int nbin = (int)a.nbin;
+    int bb;
+    bool ok;
+    if (fabsf(u - k) < a.du_margin) {
+        nre += 1;
+        const double4 xi = pi[slot0 + il];
+        const double4 xj = pj[j];
+        const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
+        ok = d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn;
+        bb = ok ? bin_of(d2r, thr, nbin, a.rminf, a.inv_drf) : 0;
+    } else {
+        bb = (int)floorf(u);
+        ok = bb >= 0 && bb < nbin;
+    }
+    if (ok) {
+        const int tj = a.type_slot[j];
+        const int ch = chan[ti_lds[il] * (int)a.nt + tj] + (slot0 + il == j ? (int)a.half : 0);
+        atomicAdd(&hist[__umul24((unsigned)ch, (unsigned)nbin) + (unsigned)bb], 1u);
     }
 }
+
+constexpr int SQCAP = 320;   // per-wavefront queue: < 64 left + 4 masks of 64
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#define ANL_APPEND(P, M, J)                                                 \
+    if (M) {                                                                \
+        if (P) q[qn + lane_prefix(M)] = ((uint32_t)(J) << 8) | il;          \
+        qn += (uint32_t)__popcll(M);                                        \
+    }
 
 template <bool TRI>
 __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
@@ -387,17 +465,17 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     const BlockInfo b = a.blocks[w.block];
     const uint32_t t = a.lag0 + blockIdx.y * a.every;
     if (t >= b.leff) return;
-    double4 *xi_lds = (double4 *)smem;                     // [WG]
-    uint32_t *queue = (uint32_t *)(xi_lds + WG);           // [WG/64][QCAP]
-    int *ti_lds = (int *)(queue + (WG / 64) * QCAP);       // [WG]
-    double *thr = (double *)(ti_lds + WG);                 // [nbin+1]
-    int *chan = (int *)(thr + a.nbin + 1);                 // [nt*nt]
-    unsigned *hist = (unsigned *)(chan + a.nt * a.nt);     // [nch*nbin]
+    uint4 *fi_lds = (uint4 *)smem;                          // [WG]
+    uint32_t *queue = (uint32_t *)(fi_lds + WG);            // [WG/64][SQCAP]
+    int *ti_lds = (int *)(queue + (WG / 64) * SQCAP);       // [WG]
+    double *thr = (double *)(ti_lds + WG);                  // [nbin+1]
+    int *chan = (int *)(thr + a.nbin + 1);                  // [nt*nt]
+    unsigned *hist = (unsigned *)(chan + a.nt * a.nt);      // [nch*nbin]
     lds_setup(a, hist, thr, chan);
 
     const uint64_t N = a.N;
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t *q = queue + (threadIdx.x >> 6) * QCAP;
+    uint32_t *q = queue + (threadIdx.x >> 6) * SQCAP;
     const uint64_t slot0 = (uint64_t)blockIdx.x * WG;
     const uint64_t slot = slot0 + threadIdx.x;
     bool active = slot < N;
@@ -409,91 +487,88 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     }
     ti_lds[threadIdx.x] = ti;
     const uint32_t il = threadIdx.x;
-    const uint64_t N2 = N & ~(uint64_t)1;
+    const uint32_t N4 = (uint32_t)(N & ~(uint64_t)3);
     // strict comparison against threshold+1; inactive lanes compare against 0
     // and never pass
     const uint32_t th = active ? a.threshold + 1u : 0u;
-    unsigned long long ncand = 0;
+    unsigned long long ncand = 0, nre = 0;
 
     for (uint32_t k = w.k0; k < w.k1; k++) {
         const uint64_t fi = b.primo + (uint64_t)k * a.skip;
         const uint64_t fj = fi + t;
         const DevCell c = load_cell(a.box + fi * 9);
+        const double4 *__restrict__ pi = a.pos + fi * N;
         const double4 *__restrict__ pj = a.pos + fj * N;
-        const uint2 *__restrict__ fjp = a.frac + fj * N;
-        __syncthreads();  // previous t0's candidates are done with xi_lds
+        const uint4 *__restrict__ fj32 = a.frac32 + fj * N;
+        cu64_t *fjc = (cu64_t *)(a.frac + fj * N);
+        __syncthreads();  // previous t0's candidates are done with fi_lds
         uint32_t abi = 0, ci = 0;
         if (slot < N) {
-            xi_lds[threadIdx.x] = a.pos[fi * N + slot];
-            uint2 f = a.frac[fi * N + slot];
+            fi_lds[threadIdx.x] = a.frac32[fi * N + slot];
+            const uint2 f = a.frac[fi * N + slot];
             abi = f.x;
             ci = f.y;
         }
         __syncthreads();
         uint32_t qn = 0;
-        cu64_t *fjc = (cu64_t *)fjp;
-        const uint32_t N2u = (uint32_t)N2;
-        for (uint32_t j = 0; j < N2u; j += 2) {
+        for (uint32_t j = 0; j < N4; j += 4) {
             const unsigned long long g0 = fjc[j];
             const unsigned long long g1 = fjc[j + 1];
-            const uint32_t s0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32));
-            const uint32_t s1 = sq16(abi, (uint32_t)g1, ci, (uint32_t)(g1 >> 32));
-            const bool p0 = s0 < th;
-            const bool p1 = s1 < th;
+            const unsigned long long g2 = fjc[j + 2];
+            const unsigned long long g3 = fjc[j + 3];
+            const bool p0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32)) < th;
+            const bool p1 = sq16(abi, (uint32_t)g1, ci, (uint32_t)(g1 >> 32)) < th;
+            const bool p2 = sq16(abi, (uint32_t)g2, ci, (uint32_t)(g2 >> 32)) < th;
+            const bool p3 = sq16(abi, (uint32_t)g3, ci, (uint32_t)(g3 >> 32)) < th;
             const unsigned long long m0 = __ballot(p0);
             const unsigned long long m1 = __ballot(p1);
-            if (m0) {
-                if (p0) q[qn + lane_prefix(m0)] = (j << 8) | il;
-                qn += (uint32_t)__popcll(m0);
-            }
-            if (m1) {
-                if (p1) q[qn + lane_prefix(m1)] = ((j + 1) << 8) | il;
-                qn += (uint32_t)__popcll(m1);
-            }
-            while (qn >= 64) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                qn -= 64;
-                const uint32_t e = q[qn + lane];
-                ncand += 1;
-                eval_candidate<TRI>(a, e, xi_lds, ti_lds, pj, slot0, c, thr, chan, hist);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const unsigned long long m2 = __ballot(p2);
+            const unsigned long long m3 = __ballot(p3);
+            if (m0 | m1 | m2 | m3) {
+                ANL_APPEND(p0, m0, j)
+                ANL_APPEND(p1, m1, j + 1)
+                ANL_APPEND(p2, m2, j + 2)
+                ANL_APPEND(p3, m3, j + 3)
+                while (qn >= 64) {
+                    wave_sync_lds();
+                    qn -= 64;
+                    const uint32_t e = q[qn + lane];
+                    ncand += 1;
+                    eval_candidate<TRI>(a, e, fi_lds, ti_lds, fj32, pi, pj, slot0, c, thr, chan, hist, nre);
+                    wave_sync_lds();
+                }
             }
         }
-        if (N2 != N) {  // odd N: last j alone
-            const uint64_t j = N2;
-            const uint2 f0 = fjp[j];
-            const bool p0 = sq16(abi, f0.x, ci, f0.y) < th;
+        for (uint32_t j = N4; j < (uint32_t)N; j++) {  // N % 4 tail
+            const unsigned long long g0 = fjc[j];
+            const bool p0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32)) < th;
             const unsigned long long m0 = __ballot(p0);
-            if (m0) {
-                if (p0) q[qn + lane_prefix(m0)] = ((uint32_t)j << 8) | il;
-                qn += (uint32_t)__popcll(m0);
-            }
+            ANL_APPEND(p0, m0, j)
         }
-        // drain the queue: up to QCAP-1 entries, 64 at a time
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // drain: fewer than 64 + 3*64 entries, 64 at a time
+        wave_sync_lds();
         for (uint32_t base = 0; base < qn; base += 64) {
             if (base + lane < qn) {
                 const uint32_t e = q[base + lane];
                 ncand += 1;
-                eval_candidate<TRI>(a, e, xi_lds, ti_lds, pj, slot0, c, thr, chan, hist);
+                eval_candidate<TRI>(a, e, fi_lds, ti_lds, fj32, pi, pj, slot0, c, thr, chan, hist, nre);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync_lds();
     }
-    // per-wavefront candidate total, one atomic per wavefront
-    for (int off = 32; off > 0; off >>= 1) ncand += __shfl_down(ncand, off, 64);
-    if (lane == 0 && a.ncand) atomicAdd(a.ncand, ncand);
+    // per-wavefront totals, one atomic each per wavefront
+    for (int off = 32; off > 0; off >>= 1) {
+        ncand += __shfl_down(ncand, off, 64);
+        nre += __shfl_down(nre, off, 64);
+    }
+    if (lane == 0 && a.ncand) {
+        atomicAdd(a.ncand, ncand);
+        atomicAdd(a.nrecheck, nre);
+    }
     __syncthreads();
     lds_flush(a, hist, w.block, t);
 }
+#undef ANL_APPEND
 
 }  // namespace anl
 
@@ -512,9 +587,8 @@ struct anl_gofrt {
     double lo = 0;
     double *d_thr = nullptr;
     int32_t *d_chan = nullptr;
-    unsigned long long *d_ncand = nullptr;
+    unsigned long long *d_ncand = nullptr;   // [2]: candidates, FP64 rechecks
     int last_form = -1;
-    uint64_t last_candidates = 0;
     // scratch for the work list
     BlockInfo *d_blocks = nullptr;
     WorkItem *d_work = nullptr;
@@ -598,7 +672,7 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
     ANL_HIP(hipMemcpyAsync(g->d_blocks, L.blocks.data(), L.blocks.size() * sizeof(BlockInfo), hipMemcpyHostToDevice, s));
     ANL_HIP(hipMemcpyAsync(g->d_work, L.work.data(), L.work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
     ANL_HIP(hipMemsetAsync(out, 0, out_len * sizeof(unsigned long long), s));
-    ANL_HIP(hipMemsetAsync(g->d_ncand, 0, sizeof(unsigned long long), s));
+    ANL_HIP(hipMemsetAsync(g->d_ncand, 0, 2 * sizeof(unsigned long long), s));
 
     GofrtArgs a{};
     a.pos = tr->pos;
@@ -628,6 +702,19 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
     a.threshold = form == ANL_GOFRT_SCREENED ? tr->screen_cache.threshold : 0;
     a.a0 = a0;
     a.a1 = a1;
+    a.frac32 = tr->frac32;
+    a.nrecheck = g->d_ncand + 1;
+    {
+        const double sc = std::ldexp(1.0, -32);
+        const Cell &cl = tr->cell0;
+        a.cx = (float)(cl.L[0] * sc);
+        a.cxy = (float)(cl.xy * sc);
+        a.cxz = (float)(cl.xz * sc);
+        a.cy = (float)(cl.L[1] * sc);
+        a.cyz = (float)(cl.yz * sc);
+        a.cz = (float)(cl.L[2] * sc);
+        a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->dr, g->nbin) : 0.0f;
+    }
 
     const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
     const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;
@@ -642,7 +729,7 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
         if (tr->triclinic) hipLaunchKernelGGL(gofrt_fp64_kernel<true>, grid, dim3(WG), lds, s, a);
         else hipLaunchKernelGGL(gofrt_fp64_kernel<false>, grid, dim3(WG), lds, s, a);
     } else {
-        size_t lds = WG * sizeof(double4) + (WG / 64) * QCAP * sizeof(uint32_t) + WG * sizeof(int) + common;
+        size_t lds = WG * sizeof(uint4) + (WG / 64) * SQCAP * sizeof(uint32_t) + WG * sizeof(int) + common;
         if (lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
         if (tr->triclinic) hipLaunchKernelGGL(gofrt_screened_kernel<true>, grid, dim3(WG), lds, s, a);
         else hipLaunchKernelGGL(gofrt_screened_kernel<false>, grid, dim3(WG), lds, s, a);
@@ -688,7 +775,7 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
             }
         ANL_HIP(hipMalloc(&g->d_chan, chan.size() * sizeof(int32_t)));
         ANL_HIP(hipMemcpy(g->d_chan, chan.data(), chan.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        ANL_HIP(hipMalloc(&g->d_ncand, sizeof(unsigned long long)));
+        ANL_HIP(hipMalloc(&g->d_ncand, 2 * sizeof(unsigned long long)));
         *out = g;
     });
 }
@@ -739,7 +826,7 @@ int anl_gofrt_calculate(anl_gofrt *g, uint64_t primo, uint64_t *counts_host) {
             Launch L = plan(g, &primo, &g->n, 1, -1);
             launch(g, L, d, len, INT32_MIN, INT32_MAX, 0);
             ANL_HIP(hipMemcpy(counts_host, d, len * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-            ANL_HIP(hipMemcpy(&g->last_candidates, g->d_ncand, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+
         } catch (...) {
             (void)hipFree(d);
             throw;
@@ -789,15 +876,14 @@ int anl_gofrt_calculate_blocks_async(anl_gofrt *g, const uint64_t *primo, const 
     });
 }
 
-int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates) {
+int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uint64_t *rechecks) {
     return guard([&] {
         if (!g) fail(ANL_ERR_ARG, "null handle");
         if (form_used) *form_used = g->last_form;
-        if (candidates) {
-            unsigned long long c = 0;
-            ANL_HIP(hipMemcpy(&c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
-            *candidates = c;
-        }
+        unsigned long long c[2] = {0, 0};
+        ANL_HIP(hipMemcpy(c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
+        if (candidates) *candidates = c[0];
+        if (rechecks) *rechecks = c[1];
     });
 }
 

@@ -1,0 +1,792 @@
+// C++ operator API of the time-lagged pair-correlation path, kept as in
+// rikigigi/analisi v0.5.2 (same class names, constructor arguments, reset /
+// calculate / calc_single_th / calc_end, buffer layouts and error messages) so
+// that code written against the reference compiles against this header; the
+// work is done on the GPU through the C ABI in analisi_hip.h.
+//
+//   VectorOp<T,TFLOAT>                operazionisulista.h:25-148
+//   Trajectory_numpy                  trajectory_numpy.{h,cpp} (plain arrays in
+//                                     place of pybind11 buffers)
+//   ReadLog_numpy<TFLOAT>             readlog_numpy.{h,cpp}
+//   Gofrt<TFLOAT,T>                   gofrt.{h,cpp}
+//   MSD<T,FPE>                        msd.{h,cpp}
+//   GreenKuboNComponentIonicFluid<..> greenkuboNcomponentionicfluid.{h,cpp}
+//   MediaVar<T>, MediaVarCovar<T>     calcoliblocchi.h
+//   BlockAverageG<TR,T,Args...>       blockaverage.h:83-218
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "analisi_hip.h"
+
+namespace analisi {
+
+inline void anl_check(int rc) {
+    if (rc != ANL_OK) throw std::runtime_error(anl_last_error());
+}
+
+// ---------------------------------------------------------------------------
+// VectorOp (operazionisulista.h:25-148): owner of vdata, element-wise ops
+// ---------------------------------------------------------------------------
+template <class T, class TFLOAT = double>
+class VectorOp {
+   public:
+    VectorOp &operator=(const VectorOp &destra) {
+        if (data_length != destra.data_length) {
+            delete[] vdata;
+            data_length = destra.data_length;
+            vdata = new TFLOAT[data_length];
+        }
+        for (unsigned int i = 0; i < data_length; i++) vdata[i] = destra.vdata[i];
+        return *this;
+    }
+#define ANL_VOP(OP)                                                                         \
+    T &operator OP(const T &destra) {                                                       \
+        if (destra.lunghezza() != data_length)                                              \
+            throw std::runtime_error("Trying to operate on VectorOp of different sizes!");  \
+        for (unsigned int i = 0; i < data_length; i++) vdata[i] OP destra.elemento(i);      \
+        return static_cast<T &>(*this);                                                     \
+    }                                                                                       \
+    T &operator OP(const TFLOAT &destra) {                                                  \
+        for (unsigned int i = 0; i < data_length; i++) vdata[i] OP destra;                  \
+        return static_cast<T &>(*this);                                                     \
+    }
+    ANL_VOP(+=)
+    ANL_VOP(-=)
+    ANL_VOP(*=)
+    ANL_VOP(/=)
+#undef ANL_VOP
+    unsigned int lunghezza() const { return data_length; }
+    TFLOAT elemento(unsigned int i) const {
+        if (i < data_length) return vdata[i];
+        throw std::runtime_error("Out of range index");
+    }
+    TFLOAT *access_vdata() { return vdata; }
+    void azzera() {
+        for (unsigned int i = 0; i < data_length; i++) vdata[i] = 0;
+    }
+
+   protected:
+    VectorOp() : vdata(nullptr), data_length(0) {}
+    VectorOp(const VectorOp &c) : vdata(nullptr), data_length(0) { operator=(c); }
+    ~VectorOp() { delete[] vdata; }
+    TFLOAT *vdata;
+    unsigned int data_length;
+};
+
+// ---------------------------------------------------------------------------
+// Trajectory_numpy (trajectory_numpy.cpp:7-199)
+// ---------------------------------------------------------------------------
+class Trajectory_numpy {
+   public:
+    enum class BoxFormat { Lammps_ortho, Lammps_triclinic, Cell_vectors, Invalid };
+
+    // pos/vel [nsteps][natoms][3], types [natoms], box [nsteps][6|9] (lammps)
+    // or [nsteps][3][3] (cell vectors, row-major as numpy gives it).
+    Trajectory_numpy(const double *pos, const double *vel, const int *types, const double *box, size_t nsteps,
+                     size_t natoms, BoxFormat fmt, bool wrap_pbc = false, bool save_rotation_matrix = false)
+        : n_timesteps(nsteps), natoms(natoms), wrap_pbc(wrap_pbc), box_format(fmt) {
+        if (fmt == BoxFormat::Invalid) throw std::runtime_error("Invalid input cell format");
+        triclinic = false;
+        buffer_boxes_stride = 6;
+        std::vector<double> rot;
+        if (fmt == BoxFormat::Lammps_ortho || fmt == BoxFormat::Lammps_triclinic) {
+            if (fmt == BoxFormat::Lammps_triclinic) {
+                buffer_boxes_stride = 9;
+                triclinic = true;
+            }
+            boxes.assign(box, box + nsteps * buffer_boxes_stride);
+            for (size_t i = 0; i < nsteps; ++i) lammps_to_internal(&boxes[i * buffer_boxes_stride]);
+        } else {
+            // triclinic.h:10-64: lammps cell from the QR of the cell matrix
+            std::vector<Lcell> cells(nsteps);
+            for (size_t i = 0; i < nsteps; i++) {
+                cells[i] = lammps_cell(box + 9 * i);
+                if (!cells[i].diagonal) triclinic = true;
+            }
+            buffer_boxes_stride = triclinic ? 9 : 6;
+            boxes.assign(nsteps * buffer_boxes_stride, 0.0);
+            for (size_t i = 0; i < nsteps; i++) {
+                double *c = &boxes[i * buffer_boxes_stride];
+                c[3] = cells[i].r[0][0] / 2;
+                c[4] = cells[i].r[1][1] / 2;
+                c[5] = cells[i].r[2][2] / 2;
+                if (triclinic) {
+                    c[6] = cells[i].r[1][0];
+                    c[7] = cells[i].r[2][0];
+                    c[8] = cells[i].r[2][1];
+                }
+            }
+            if (triclinic) {
+                rot.resize(nsteps * 9);
+                for (size_t i = 0; i < nsteps; i++) std::memcpy(&rot[9 * i], cells[i].q, sizeof(double) * 9);
+                rotations = cells;
+            }
+        }
+        pos_.assign(pos, pos + nsteps * natoms * 3);
+        vel_.assign(vel, vel + nsteps * natoms * 3);
+        if (triclinic && fmt == BoxFormat::Cell_vectors) {
+            for (size_t i = 0; i < nsteps; i++)
+                for (size_t n = 0; n < natoms; n++) {
+                    rotate(rotations[i], &pos_[(i * natoms + n) * 3]);
+                    rotate(rotations[i], &vel_[(i * natoms + n) * 3]);
+                }
+            if (save_rotation_matrix) rotation_matrix = rot;
+        }
+        raw_types.assign(types, types + natoms);
+        compute_type_ids();
+        calc_cm(pos, positions_cm_);
+        calc_cm(vel, velocities_cm_);
+        if (wrap_pbc)
+            for (size_t i = 0; i < nsteps; ++i) pbc_wrap(i);
+    }
+    ~Trajectory_numpy() {
+        if (dev) anl_traj_destroy(dev);
+    }
+    Trajectory_numpy(const Trajectory_numpy &) = delete;
+    Trajectory_numpy &operator=(const Trajectory_numpy &) = delete;
+
+    double *positions(int t, int i) { return &pos_[(size_t)t * natoms * 3 + (size_t)i * 3]; }
+    double *velocity(int t, int i) { return &vel_[(size_t)t * natoms * 3 + (size_t)i * 3]; }
+    double *box(int t) { return &boxes[(size_t)t * buffer_boxes_stride]; }
+    double *positions_cm(int t, int type) { return &positions_cm_[((size_t)t * ntypes + type) * 3]; }
+    double *velocity_cm(int t, int type) { return &velocities_cm_[((size_t)t * ntypes + type) * 3]; }
+    double *get_rotation_matrix(size_t t) { return rotation_matrix.empty() ? nullptr : &rotation_matrix[9 * t]; }
+    unsigned int get_type(unsigned int i) const {
+        if (i < natoms) return type_id[i];
+        throw std::runtime_error("Atom index out of range\n");
+    }
+    std::vector<unsigned int> get_types() const { return types; }
+    size_t get_ntypes() const { return ntypes; }
+    size_t get_natoms() const { return natoms; }
+    size_t get_ntimesteps() const { return n_timesteps; }
+    size_t get_nloaded_timesteps() const { return n_timesteps; }
+    ssize_t get_current_timestep() const { return 0; }
+    bool get_pbc_wrap() const { return wrap_pbc; }
+    bool is_triclinic() const { return triclinic; }
+    size_t get_box_stride() const { return buffer_boxes_stride; }
+
+    // basetrajectory.h:94-105
+    static void lammps_to_internal(double *c) {
+        double t = c[4];
+        c[5] = (c[5] - c[4]) / 2;
+        c[4] = (c[3] - c[2]) / 2;
+        c[3] = (c[1] - c[0]) / 2;
+        c[1] = c[2];
+        c[2] = t;
+    }
+    // basetrajectory.h:224-268
+    static void minImage_triclinic(double *d, const double *lh, const double *t, bool tri) {
+        double xy = tri ? t[0] : 0, xz = tri ? t[1] : 0, yz = tri ? t[2] : 0;
+        while (std::fabs(d[2]) > lh[2]) {
+            if (d[2] < 0.0) { d[2] += lh[2] * 2; if (tri) { d[1] += yz; d[0] += xz; } }
+            else { d[2] -= lh[2] * 2; if (tri) { d[1] -= yz; d[0] -= xz; } }
+        }
+        while (std::fabs(d[1]) > lh[1]) {
+            if (d[1] < 0.0) { d[1] += lh[1] * 2; if (tri) d[0] += xy; }
+            else { d[1] -= lh[1] * 2; if (tri) d[0] -= xy; }
+        }
+        while (std::fabs(d[0]) > lh[0]) {
+            if (d[0] < 0.0) d[0] += lh[0] * 2;
+            else d[0] -= lh[0] * 2;
+        }
+    }
+    // basetrajectory.h:181-219, on the host (pyanalisi's Trajectory.minImage)
+    double d2_minImage(size_t i, size_t j, size_t ti, size_t tj, double *x) {
+        const double *c = box((int)ti);
+        for (int k = 0; k < 3; k++) x[k] = positions((int)ti, (int)i)[k] - positions((int)tj, (int)j)[k];
+        minImage_triclinic(x, c + 3, c + 6, triclinic);
+        double d2 = 0.0;
+        for (int k = 0; k < 3; k++) d2 += x[k] * x[k];
+        return d2;
+    }
+
+    // device mirror, built on first use and immutable afterwards
+    anl_traj *device() {
+        if (!dev) {
+            std::vector<int32_t> tid(type_id.begin(), type_id.end());
+            anl_check(anl_traj_create(pos_.data(), boxes.data(), (uint32_t)buffer_boxes_stride, tid.data(),
+                                      (uint32_t)ntypes, positions_cm_.data(), n_timesteps, natoms, &dev));
+        }
+        return dev;
+    }
+
+   private:
+    struct Lcell {
+        double q[9];     // column-major Q
+        double r[3][3];  // lower-triangular lammps cell (rows: a, b, c)
+        bool diagonal;
+    };
+    std::vector<Lcell> rotations;
+    // triclinic.h:10-40: cell is the row-major numpy [3][3] seen column-major,
+    // QR of its transpose by Householder reflections, then signs flipped so
+    // that the diagonal is positive.
+    static Lcell lammps_cell(const double *m) {
+        Lcell L{};
+        double cell[3][3];  // cell(i,j) = m[j*3+i]
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) cell[i][j] = m[j * 3 + i];
+        if (cell[0][1] == 0 && cell[0][2] == 0 && cell[1][0] == 0 && cell[1][2] == 0 && cell[2][0] == 0 &&
+            cell[2][1] == 0) {
+            L.diagonal = true;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    L.r[i][j] = cell[i][j];
+                    L.q[j * 3 + i] = i == j ? 1.0 : 0.0;
+                }
+            return L;
+        }
+        L.diagonal = false;
+        double A[3][3], Q[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                A[i][j] = cell[j][i];
+                Q[i][j] = i == j ? 1.0 : 0.0;
+            }
+        for (int k = 0; k < 2; k++) {
+            double norm = 0;
+            for (int i = k; i < 3; i++) norm += A[i][k] * A[i][k];
+            norm = std::sqrt(norm);
+            if (norm == 0) continue;
+            double alpha = A[k][k] > 0 ? -norm : norm;
+            double v[3] = {0, 0, 0};
+            for (int i = k; i < 3; i++) v[i] = A[i][k];
+            v[k] -= alpha;
+            double vv = 0;
+            for (int i = k; i < 3; i++) vv += v[i] * v[i];
+            if (vv == 0) continue;
+            for (int j = 0; j < 3; j++) {
+                double s = 0;
+                for (int i = k; i < 3; i++) s += v[i] * A[i][j];
+                for (int i = k; i < 3; i++) A[i][j] -= 2 * v[i] * s / vv;
+            }
+            for (int j = 0; j < 3; j++) {  // Q = Q H
+                double s = 0;
+                for (int i = k; i < 3; i++) s += Q[j][i] * v[i];
+                for (int i = k; i < 3; i++) Q[j][i] -= 2 * s * v[i] / vv;
+            }
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < i; j++) A[i][j] = 0;
+        for (int i = 0; i < 3; i++)  // triclinic.h:30-38 reflect
+            if (A[i][i] < 0)
+                for (int j = 0; j < 3; j++) {
+                    A[i][j] = -A[i][j];
+                    Q[j][i] = -Q[j][i];
+                }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                L.r[i][j] = A[j][i];
+                L.q[j * 3 + i] = Q[i][j];
+            }
+        return L;
+    }
+    // triclinic.h:53-56: row vector times Q
+    static void rotate(const Lcell &c, double *v) {
+        double o[3];
+        for (int j = 0; j < 3; j++) o[j] = v[0] * c.q[j * 3 + 0] + v[1] * c.q[j * 3 + 1] + v[2] * c.q[j * 3 + 2];
+        std::memcpy(v, o, sizeof(o));
+    }
+    // basetrajectory.cpp:51-89
+    void compute_type_ids() {
+        std::vector<int> distinct;
+        for (size_t i = 0; i < natoms; i++) {
+            bool seen = false;
+            for (int d : distinct) seen |= d == raw_types[i];
+            if (!seen) distinct.push_back(raw_types[i]);
+        }
+        std::sort(distinct.begin(), distinct.end());
+        std::map<int, unsigned int> m;
+        for (unsigned int k = 0; k < distinct.size(); k++) {
+            m[distinct[k]] = k;
+            types.push_back(distinct[k]);
+        }
+        ntypes = distinct.size();
+        type_id.resize(natoms);
+        for (size_t i = 0; i < natoms; i++) type_id[i] = m.at(raw_types[i]);
+    }
+    // trajectory_numpy.cpp:201-225: running mean of the raw input per type
+    void calc_cm(const double *a, std::vector<double> &cm) {
+        cm.assign(n_timesteps * ntypes * 3, 0.0);
+        std::vector<int> cont(ntypes);
+        for (size_t i = 0; i < n_timesteps; ++i) {
+            std::fill(cont.begin(), cont.end(), 0);
+            for (size_t ia = 0; ia < natoms; ++ia) {
+                unsigned int it = type_id[ia];
+                cont[it]++;
+                for (int k = 0; k < 3; k++)
+                    cm[(i * ntypes + it) * 3 + k] +=
+                        (a[(i * natoms + ia) * 3 + k] - cm[(i * ntypes + it) * 3 + k]) / double(cont[it]);
+            }
+        }
+    }
+    // basetrajectory.h:145-161
+    void pbc_wrap(size_t idx) {
+        double *c = box((int)idx);
+        double x0[3] = {c[3], c[4], c[5]};
+        for (size_t ia = 0; ia < natoms; ++ia) {
+            double *xa = positions((int)idx, (int)ia);
+            for (int k = 0; k < 3; ++k) xa[k] = xa[k] - x0[k];
+            minImage_triclinic(xa, c + 3, c + 6, triclinic);
+            for (int k = 0; k < 3; ++k) xa[k] = xa[k] + x0[k];
+        }
+    }
+
+    size_t n_timesteps, natoms, ntypes = 0, buffer_boxes_stride;
+    bool wrap_pbc, triclinic;
+    BoxFormat box_format;
+    std::vector<double> pos_, vel_, boxes, positions_cm_, velocities_cm_, rotation_matrix;
+    std::vector<int> raw_types;
+    std::vector<unsigned int> type_id, types;
+    anl_traj *dev = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// ReadLog_numpy (readlog_numpy.{h,cpp})
+// ---------------------------------------------------------------------------
+template <class TFLOAT = double>
+class ReadLog_numpy {
+   public:
+    ReadLog_numpy(const TFLOAT *data, size_t nrows, size_t ncols, std::vector<std::string> headers)
+        : headers(std::move(headers)), ntimestep((unsigned)nrows), ncols((unsigned)ncols), data_(data) {
+        if (this->headers.size() != ncols)
+            throw std::runtime_error("Number of headers is not equal to the number of columns");
+    }
+    ~ReadLog_numpy() {
+        if (dev) anl_series_destroy(dev);
+    }
+    TFLOAT *line(unsigned int index) { return const_cast<TFLOAT *>(data_) + (size_t)index * ncols; }
+    unsigned int n_timestep() { return ntimestep; }
+    std::pair<unsigned int, bool> get_index_of(const std::string &header) {
+        for (unsigned int i = 0; i < ncols; ++i)
+            if (header == headers.at(i)) return {i, true};
+        return {0, false};
+    }
+    anl_series *device() {
+        if (!dev) anl_check(anl_series_create(data_, ntimestep, ncols, &dev));
+        return dev;
+    }
+
+   private:
+    std::vector<std::string> headers;
+    unsigned int ntimestep, ncols;
+    const TFLOAT *data_;
+    anl_series *dev = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// Gofrt (gofrt.h:28-107, gofrt.cpp)
+// ---------------------------------------------------------------------------
+template <class TFLOAT, class T>
+class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
+    using VOp = VectorOp<Gofrt<TFLOAT, T>, TFLOAT>;
+    using VOp::data_length;
+    using VOp::vdata;
+
+   public:
+    Gofrt(T *t, TFLOAT rmin, TFLOAT rmax, unsigned int nbin, unsigned int tmax = 0, unsigned int nthreads = 0,
+          unsigned int skip = 1, unsigned int every = 1, bool debug = false, int form = ANL_GOFRT_AUTO)
+        : traiettoria(t), rmin(rmin), rmax(rmax), nbin(nbin), lmax(tmax), nthreads(nthreads == 0 ? 1 : nthreads),
+          skip(skip == 0 ? 1 : skip), every(every == 0 ? 1 : every), debug(debug) {
+        anl_check(anl_gofrt_create(t->device(), rmin, rmax, nbin, tmax, this->skip, this->every, form, &g));
+    }
+    ~Gofrt() {
+        if (g) anl_gofrt_destroy(g);
+    }
+    Gofrt &operator=(const Gofrt &destra) {
+        VOp::operator=(destra);
+        return *this;
+    }
+    unsigned int nExtraTimesteps(unsigned int n_b) {  // gofrt.cpp:37-39
+        unsigned int v = (unsigned int)(traiettoria->get_ntimesteps() / (n_b + 1) + 1);
+        return (v < lmax || lmax == 0) ? v : lmax;
+    }
+    void reset(const unsigned int n) {  // gofrt.cpp:41-63
+        leff = (n < lmax || lmax == 0) ? n : lmax;
+        ntimesteps = n;
+        data_length = leff * nch() * nbin;
+        delete[] vdata;
+        vdata = new TFLOAT[data_length];
+        anl_check(anl_gofrt_reset(g, n, nullptr));
+    }
+    std::vector<ssize_t> get_shape() { return {(ssize_t)leff, (ssize_t)nch(), (ssize_t)nbin}; }
+    std::vector<ssize_t> get_stride() {
+        return {(ssize_t)(nch() * nbin * sizeof(TFLOAT)), (ssize_t)(nbin * sizeof(TFLOAT)), (ssize_t)sizeof(TFLOAT)};
+    }
+    // CalculateMultiThread::calculate(primo) with Gofrt's flags: calc_init,
+    // the whole (lag, t0, pair) nest in one device call, calc_end.
+    void calculate(size_t primo) {
+        calc_init((int)primo);
+        std::vector<uint64_t> counts(data_length);
+        anl_check(anl_gofrt_calculate(g, primo, counts.data()));
+        for (unsigned int k = 0; k < data_length; k++) vdata[k] = (TFLOAT)counts[k] * incr;
+        calc_end();
+    }
+    void calc_init(int primo) {  // gofrt.cpp:74-93
+        if ((uint64_t)leff + ntimesteps + primo > traiettoria->get_ntimesteps() + 1)
+            throw std::runtime_error(
+                "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
+                "the size of the average or the lenght of the time lag");
+        VOp::azzera();
+        incr = (ntimesteps / skip > 0) ? 1.0 / int(ntimesteps / skip) : 1;
+    }
+    // gofrt.cpp:96-122: one lag t and t0 index imedia, atoms [a0,a1); adds
+    // incr per hit into vdata (the ith argument is kept for the signature).
+    void calc_single_th(int t, int imedia, int a0, int a1, int primo, int /*ith*/) {
+        std::vector<uint64_t> counts(data_length, 0);
+        anl_check(anl_gofrt_calc_single_th(g, t, imedia, a0, a1, (uint64_t)primo, counts.data()));
+        for (unsigned int k = 0; k < data_length; k++)
+            if (counts[k]) vdata[k] += (TFLOAT)counts[k] * incr;
+    }
+    void calc_end() {  // gofrt.cpp:127-155 (debug dump)
+        if (!debug) return;
+        std::ofstream out("gofrt.dump", std::ios::app);
+        for (unsigned int ts = 0; ts < leff; ts++) {
+            for (unsigned int r = 0; r < nbin; r++) {
+                out << ts << " " << r;
+                for (unsigned int it = 0; it < nch(); it++) out << " " << vdata[(size_t)ts * nch() * nbin + it * nbin + r];
+                out << "\n";
+            }
+        }
+        out << "\n\n";
+    }
+    anl_gofrt *handle() { return g; }
+
+   private:
+    unsigned int nch() const { return (unsigned int)(traiettoria->get_ntypes() * (traiettoria->get_ntypes() + 1)); }
+    T *traiettoria;
+    TFLOAT rmin, rmax, incr = 1;
+    unsigned int nbin, lmax, nthreads, skip, every;
+    unsigned int leff = 0, ntimesteps = 0;
+    bool debug;
+    anl_gofrt *g = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// MSD (msd.h:26-60, msd.cpp)
+// ---------------------------------------------------------------------------
+template <class T, bool FPE = false>
+class MSD : public VectorOp<MSD<T, FPE>> {
+    using VOp = VectorOp<MSD<T, FPE>>;
+    using VOp::data_length;
+    using VOp::vdata;
+
+   public:
+    MSD(T *t, unsigned int skip = 1, unsigned int tmax = 0, unsigned int nthreads = 0, bool cm_msd = false,
+        bool cm_self = false, bool debug = false)
+        : traiettoria(t), lmax(tmax), f_cm(cm_msd ? 2 : 1), skip(skip == 0 ? 1 : skip), cm_msd(cm_msd),
+          cm_self(cm_self), debug(debug) {}
+    std::vector<ssize_t> get_shape() const { return {(ssize_t)leff, (ssize_t)f_cm, (ssize_t)ntypes}; }
+    std::vector<ssize_t> get_stride() const {
+        return {(ssize_t)(ntypes * f_cm * sizeof(double)), (ssize_t)(ntypes * sizeof(double)), (ssize_t)sizeof(double)};
+    }
+    void reset(const unsigned int n) {  // msd.cpp:40-49
+        leff = (n < lmax || lmax == 0) ? n : lmax;
+        ntypes = traiettoria->get_ntypes();
+        data_length = leff * ntypes * f_cm;
+        ntimesteps = n;
+        delete[] vdata;
+        vdata = new double[data_length];
+    }
+    void calculate(size_t primo) {
+        calc_init(primo);
+        anl_check(anl_msd_calculate(traiettoria->device(), skip, lmax, cm_msd, cm_self, ntimesteps, primo, vdata));
+        calc_end();
+    }
+    void calc_init(size_t primo) {  // msd.cpp:52-61
+        if (leff + ntimesteps + primo > traiettoria->get_ntimesteps())
+            throw std::runtime_error(
+                "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
+                "the size of the average or the lenght of the time lag");
+        if (leff + ntimesteps > traiettoria->get_nloaded_timesteps()) {
+            std::stringstream ss;
+            ss << "there are not enough loaded timesteps inside the trajectory object. I need at least "
+               << leff + ntimesteps << " timesteps to do the requested calculation";
+            throw std::runtime_error(ss.str());
+        }
+    }
+    void calc_end() {  // msd.cpp:129-146
+        if (!debug) return;
+        std::ofstream out("msd.dump", std::ios::app);
+        for (size_t ts = 0; ts < leff; ts++) {
+            out << ts;
+            for (size_t it = 0; it < ntypes * f_cm; it++) out << " " << vdata[ntypes * ts * f_cm + it];
+            out << "\n";
+        }
+        out << "\n\n";
+    }
+    MSD &operator=(const MSD &destra) {
+        VOp::operator=(destra);
+        return *this;
+    }
+    unsigned int nExtraTimesteps(unsigned int n_b) {  // msd.cpp:36-38
+        unsigned int v = (unsigned int)(traiettoria->get_ntimesteps() / (n_b + 1) + 1);
+        return (v < lmax || lmax == 0) ? v : lmax;
+    }
+
+   private:
+    T *traiettoria;
+    size_t lmax, f_cm, ntypes = 0, leff = 0, ntimesteps = 0;
+    unsigned int skip;
+    bool cm_msd, cm_self, debug;
+};
+
+// ---------------------------------------------------------------------------
+// GreenKuboNComponentIonicFluid (greenkuboNcomponentionicfluid.h:22-65)
+// ---------------------------------------------------------------------------
+template <class READLOG, class TFLOAT = double, class TFLOAT_READ = double>
+class GreenKuboNComponentIonicFluid
+    : public VectorOp<GreenKuboNComponentIonicFluid<READLOG, TFLOAT, TFLOAT_READ>, TFLOAT> {
+    using VOp = VectorOp<GreenKuboNComponentIonicFluid<READLOG, TFLOAT, TFLOAT_READ>, TFLOAT>;
+    using VOp::data_length;
+    using VOp::vdata;
+
+   public:
+    GreenKuboNComponentIonicFluid(READLOG *traiettoria, std::string log, unsigned int skip,
+                                  std::vector<std::string> headers, bool dump = false,
+                                  unsigned int lunghezza_funzione_max = 0, unsigned int nthreads = 0,
+                                  bool subtract_mean = false, unsigned int start_mean = 100, unsigned int n_seg = 1,
+                                  bool do_bench = false, unsigned int n_seg_start = 10, unsigned int n_seg_stop = 100)
+        : traiettoria(traiettoria), log(std::move(log)), skip(skip), scrivi_file(dump), lmax(lunghezza_funzione_max),
+          subtract_mean(subtract_mean), start_mean(start_mean), n_seg(n_seg < 1 ? 1 : n_seg) {
+        (void)nthreads;
+        (void)do_bench;
+        (void)n_seg_start;
+        (void)n_seg_stop;
+        for (const auto &h : headers) {  // :62-69
+            auto res = traiettoria->get_index_of(h);
+            if (res.second) idx_j.push_back(res.first);
+            else throw std::runtime_error("Errore: header '" + h + "' not found!\n");
+        }
+        N_corr = (unsigned int)(idx_j.size() * idx_j.size());
+        narr = 3 * N_corr + 2;
+        unsigned int c = 0;  // :78-102
+        std::stringstream d;
+        for (unsigned int i = 0; i < idx_j.size(); i++)
+            for (unsigned int j = 0; j < idx_j.size(); j++)
+                d << "#" << ++c << ": c(" << i << ", " << j << ")\n#" << ++c << ": var[c(" << i << ", " << j << ")]\n";
+        for (unsigned int i = 0; i < idx_j.size(); i++)
+            for (unsigned int j = 0; j < idx_j.size(); j++) {
+                d << "#" << ++c << ": L(" << i << ", " << j << ")\n#" << ++c << ": var[L(" << i << ", " << j << ")]\n";
+                d << "#" << ++c << ": L*(" << i << ", " << j << ")\n#" << ++c << ": var[L*(" << i << ", " << j
+                  << ")]\n";
+            }
+        d << "#" << ++c << ": k_gk\n#" << ++c << ": var[k_gk]\n";
+        d << "#" << ++c << ": k_einst\n#" << ++c << ": var[k_einst]\n";
+        c_descr = d.str();
+    }
+    unsigned int nExtraTimesteps(unsigned int n_b) {  // :117-119
+        unsigned int v = traiettoria->n_timestep() / (n_b + 1) + 1;
+        return (v < lmax || lmax == 0) ? v : lmax;
+    }
+    void reset(unsigned int n) {  // :122-131
+        leff = (n < lmax || lmax == 0) ? n : lmax;
+        data_length = leff * narr;
+        ntimesteps = n;
+        delete[] vdata;
+        vdata = new TFLOAT[data_length];
+        for (unsigned int i = 0; i < data_length; i++) vdata[i] = 0.0;
+    }
+    void calculate(unsigned int primo) {  // :156-380
+        if (leff + primo + ntimesteps > traiettoria->n_timestep())
+            throw std::runtime_error(
+                "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
+                "the size of the average or the lenght of the correlation function");
+        anl_check(anl_gk_calculate(traiettoria->device(), idx_j.data(), (uint32_t)idx_j.size(), skip, lmax,
+                                   ntimesteps, n_seg, subtract_mean, start_mean, primo, vdata));
+        if (scrivi_file) {
+            std::ofstream outfile(log + ".greekdump", std::ios::app);
+            for (unsigned int it = 0; it < leff; it++) {
+                for (unsigned int j = 0; j < narr; j++) outfile << vdata[it * narr + j] << " ";
+                outfile << "\n";
+            }
+            outfile << "\n\n";
+        }
+    }
+    GreenKuboNComponentIonicFluid &operator=(const GreenKuboNComponentIonicFluid &destra) {
+        VOp::operator=(destra);
+        return *this;
+    }
+    unsigned int get_narr() { return narr; }
+    unsigned int get_indexOfKappa() { return 3 * N_corr; }
+    std::string get_columns_description() { return c_descr; }
+    std::vector<ssize_t> get_shape() { return {(ssize_t)leff, (ssize_t)narr}; }
+    std::vector<ssize_t> get_stride() { return {(ssize_t)(narr * sizeof(TFLOAT)), (ssize_t)sizeof(TFLOAT)}; }
+
+   private:
+    READLOG *traiettoria;
+    std::string log, c_descr;
+    unsigned int skip, narr = 0, N_corr = 0;
+    bool scrivi_file;
+    unsigned int lmax, leff = 0, ntimesteps = 0;
+    bool subtract_mean;
+    unsigned int start_mean, n_seg;
+    std::vector<uint32_t> idx_j;
+};
+
+// ---------------------------------------------------------------------------
+// MediaVar / MediaVarCovar (calcoliblocchi.h:21-202)
+// ---------------------------------------------------------------------------
+template <class T>
+class MediaVar {
+   public:
+    MediaVar(T *Tmedio, T *Tvar, T *delta, T *tmp) : Tmedio(Tmedio), Tvar(Tvar), delta(delta), tmp(tmp) {}
+    void calcola_begin(unsigned int s, T *) {
+        Tmedio->reset(s);
+        Tmedio->azzera();
+        Tvar->reset(s);
+        Tvar->azzera();
+        delta->reset(s);
+        tmp->reset(s);
+        iblock = 0;
+    }
+    void calculate(T *calcolo) {
+        *delta = *calcolo;
+        *delta -= *Tmedio;
+        *tmp = *delta;
+        *tmp /= (iblock + 1);
+        *Tmedio += *tmp;
+        *tmp = *calcolo;
+        *tmp -= *Tmedio;
+        *tmp *= *delta;
+        *Tvar += *tmp;
+        iblock++;
+    }
+    void calcola_end(unsigned int n_b) { *Tvar /= ((n_b - 1) * n_b); }
+
+   private:
+    T *Tmedio, *Tvar, *delta, *tmp;
+    unsigned int iblock = 0;
+};
+
+template <class T>
+class MediaVarCovar {
+   public:
+    MediaVarCovar(unsigned int n_columns, std::vector<std::pair<unsigned int, unsigned int>> cvar)
+        : n_columns(n_columns), cvar(std::move(cvar)) {
+        for (auto &p : this->cvar)
+            if (p.first >= n_columns || p.second >= n_columns)
+                throw std::runtime_error("Errore: indice della covarianza fuori dal range!");
+    }
+    void calcola_begin(unsigned int, T *c) {
+        size_block = c->lunghezza() / n_columns;
+        iblock = 0;
+        medie.assign(n_columns, std::vector<double>(size_block, 0.0));
+        varianze.assign(n_columns, std::vector<double>(size_block, 0.0));
+        covarianze.assign(cvar.size(), std::vector<double>(size_block, 0.0));
+        dx.assign(n_columns, 0.0);
+        dx2.assign(n_columns, 0.0);
+    }
+    void calculate(T *calcolo) {
+        iblock++;
+        for (unsigned int ib = 0; ib < size_block; ib++) {
+            for (unsigned int ic = 0; ic < n_columns; ic++) {
+                double x = calcolo->elemento(ib * n_columns + ic);
+                dx[ic] = x - medie[ic][ib];
+                medie[ic][ib] += dx[ic] / iblock;
+                dx2[ic] = x - medie[ic][ib];
+                varianze[ic][ib] += dx[ic] * dx2[ic];
+            }
+            for (unsigned int k = 0; k < cvar.size(); k++) covarianze[k][ib] += dx[cvar[k].first] * dx2[cvar[k].second];
+        }
+    }
+    void calcola_end(unsigned int) {
+        for (unsigned int ib = 0; ib < size_block; ib++) {
+            for (unsigned int ic = 0; ic < n_columns; ic++) varianze[ic][ib] /= (iblock * (iblock - 1));
+            for (unsigned int k = 0; k < cvar.size(); k++) covarianze[k][ib] /= (iblock * (iblock - 1));
+        }
+    }
+    double *media(unsigned int c) { return medie.at(c).data(); }
+    double *varianza(unsigned int c) { return varianze.at(c).data(); }
+    double *covarianza(unsigned int c) { return covarianze.at(c).data(); }
+    unsigned int size() { return size_block; }
+    unsigned int n_column() { return (unsigned int)medie.size(); }
+    unsigned int n_cvar() { return (unsigned int)covarianze.size(); }
+
+   private:
+    unsigned int n_columns, size_block = 0, iblock = 0;
+    std::vector<std::pair<unsigned int, unsigned int>> cvar;
+    std::vector<std::vector<double>> medie, varianze, covarianze;
+    std::vector<double> dx, dx2;
+};
+
+// ---------------------------------------------------------------------------
+// BlockAverageG (blockaverage.h:83-218), single process
+// ---------------------------------------------------------------------------
+template <class TR>
+inline unsigned int n_timesteps_of(TR *t) {
+    return (unsigned int)t->get_ntimesteps();
+}
+template <>
+inline unsigned int n_timesteps_of<ReadLog_numpy<double>>(ReadLog_numpy<double> *t) {
+    return t->n_timestep();
+}
+
+template <class TR, class T, typename... Args>
+class BlockAverageG {
+   public:
+    BlockAverageG(TR *t, const unsigned int &numero_blocchi) : traiettoria(t), n_b(numero_blocchi) {}
+    ~BlockAverageG() {
+        delete Tmedio;
+        delete Tvar;
+        delete delta;
+        delete tmp;
+        delete calcolo;
+    }
+    template <class Calcolo>
+    void calcola_custom(Calcolo *calc, Args... arg) {  // :107-190
+        calcolo = new T(traiettoria, arg...);
+        int per_block = ((int)n_timesteps_of(traiettoria) - (int)calcolo->nExtraTimesteps(n_b)) / (int)n_b;
+        if (per_block <= 0) throw std::runtime_error("Cannot divide the trajectory in the requested blocks");
+        s = (unsigned int)per_block;
+        ok = true;
+        calcolo->reset(s);
+        calc->calcola_begin(s, calcolo);
+        for (unsigned int iblock = 0; iblock < n_b; iblock++) {
+            calcolo->reset(s);
+            calcolo->calculate(iblock * s);
+            calc->calculate(calcolo);
+        }
+        calc->calcola_end(n_b);
+    }
+    void calculate(Args... arg) {  // :194-203
+        Tmedio = new T(traiettoria, arg...);
+        Tvar = new T(traiettoria, arg...);
+        delta = new T(traiettoria, arg...);
+        tmp = new T(traiettoria, arg...);
+        MediaVar<T> media_var(Tmedio, Tvar, delta, tmp);
+        calcola_custom<MediaVar<T>>(&media_var, arg...);
+    }
+    T *media() {
+        if (ok) return Tmedio;
+        throw std::runtime_error("block average not computed");
+    }
+    T *varianza() {
+        if (ok) return Tvar;
+        throw std::runtime_error("block average not computed");
+    }
+    T *puntatoreCalcolo() {
+        if (ok) return calcolo;
+        throw std::runtime_error("block average not computed");
+    }
+
+   private:
+    TR *traiettoria;
+    unsigned int n_b, s = 0;
+    T *Tmedio = nullptr, *Tvar = nullptr, *calcolo = nullptr, *delta = nullptr, *tmp = nullptr;
+    bool ok = false;
+};
+
+}  // namespace analisi

@@ -1,0 +1,236 @@
+// pybind11 module with the pyanalisi classes of the pair-correlation path
+// (pyanalisi/src/pyanalisi.cpp:65-153, :241-279, :281-373, :474-550 of
+// rikigigi/analisi v0.5.2): Trajectory, ReadLog, BoxFormat, Gofrt,
+// MeanSquareDisplacement, GreenKubo, info(), has_mmap().  Same constructor
+// arguments, methods and buffer protocol (FP64, zero copy on vdata); the
+// calculations run on the GPU through the C ABI.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "analisi.h"
+
+namespace py = pybind11;
+using namespace analisi;
+
+namespace {
+
+template <typename T>
+bool has_no_stride(const py::buffer_info &buf) {  // buffer_utils.h:7-15
+    ssize_t stride = sizeof(T);
+    for (int i = (int)buf.ndim - 1; i >= 0; --i) {
+        if (buf.strides[i] != stride) return false;
+        stride *= buf.shape[i];
+    }
+    return true;
+}
+
+// trajectory_numpy.cpp:7-80 argument checks, same messages
+Trajectory_numpy *make_trajectory(py::buffer bpos, py::buffer bvel, py::buffer btypes, py::buffer bbox,
+                                  Trajectory_numpy::BoxFormat fmt, bool pbc_wrap, bool save_rot) {
+    py::buffer_info ip = bpos.request(), iv = bvel.request(), it = btypes.request(), ib = bbox.request();
+    if (ip.ndim != 3) throw std::runtime_error("Wrong number of dimension of position array (must be 3)");
+    if (iv.ndim != 3) throw std::runtime_error("Wrong number of dimension of velocities array (must be 3)");
+    if (ip.shape[2] != 3)
+        throw std::runtime_error("Wrong number of cartesian components in the third dimension of positions array");
+    if (iv.shape[2] != 3)
+        throw std::runtime_error("Wrong number of cartesian components in the third dimension of velocities array");
+    for (size_t i = 0; i < ip.shape.size(); ++i)
+        if (ip.shape[i] != iv.shape[i]) throw std::runtime_error("Shape of positions and velocities array is different");
+    if (it.ndim != 1) throw std::runtime_error("Wrong number of dimension of types array (must be 1)");
+    if (it.shape[0] != ip.shape[1]) throw std::runtime_error("Wrong size of the type array");
+    if (fmt == Trajectory_numpy::BoxFormat::Cell_vectors) {
+        if (ib.ndim != 3)
+            throw std::runtime_error("Wrong number of dimensions of box array (must be 3) for cell matrix format");
+        if (ib.shape[0] != ip.shape[0] || ib.shape[1] != 3 || ib.shape[2] != 3)
+            throw std::runtime_error("Wrong shape of box array: must be (nsteps, 3, 3) for matrix format");
+    } else {
+        if (ib.ndim != 2)
+            throw std::runtime_error(
+                "Wrong number of dimensions of box array (must be 2 for lammps ortho/triclinic cell format)");
+        if (ib.shape[0] != ip.shape[0])
+            throw std::runtime_error("Wrong shape of box array: first dimension must be nsteps");
+        if (fmt == Trajectory_numpy::BoxFormat::Lammps_ortho && ib.shape[1] != 6)
+            throw std::runtime_error("Wrong shape of box array: must be (:, 6) for orthogonal lammps cell format");
+        if (fmt == Trajectory_numpy::BoxFormat::Lammps_triclinic && ib.shape[1] != 9)
+            throw std::runtime_error("Wrong shape of box array: must be (:, 9) for triclinic lammps cell format");
+    }
+    if (ib.format != py::format_descriptor<double>::format())
+        throw std::runtime_error("Format of box array should be double");
+    if (it.format != py::format_descriptor<int>::format())
+        throw std::runtime_error("Format of types array should be int (" + py::format_descriptor<int>::format() +
+                                 ") while it was " + it.format);
+    if (iv.format != py::format_descriptor<double>::format())
+        throw std::runtime_error("Format of velocities array should be double");
+    if (ip.format != py::format_descriptor<double>::format())
+        throw std::runtime_error("Format of positions array should be double");
+    if (!has_no_stride<double>(ib)) throw std::runtime_error("Unsupported stride in box array");
+    if (!has_no_stride<int>(it)) throw std::runtime_error("Unsupported stride in types array");
+    if (!has_no_stride<double>(iv)) throw std::runtime_error("Unsupported stride in vel array");
+    if (!has_no_stride<double>(ip)) throw std::runtime_error("Unsupported stride in pos array");
+    return new Trajectory_numpy((const double *)ip.ptr, (const double *)iv.ptr, (const int *)it.ptr,
+                                (const double *)ib.ptr, (size_t)ip.shape[0], (size_t)ip.shape[1], fmt, pbc_wrap,
+                                save_rot);
+}
+
+template <class C>
+py::buffer_info vbuffer(C &c) {
+    return py::buffer_info(c.access_vdata(), sizeof(double), py::format_descriptor<double>::format(),
+                           (ssize_t)c.get_shape().size(), c.get_shape(), c.get_stride());
+}
+
+py::array_t<double> copy_array(const double *p, std::vector<ssize_t> shape) {
+    py::array_t<double> a(shape);
+    std::memcpy(a.mutable_data(), p, a.size() * sizeof(double));
+    return a;
+}
+
+}  // namespace
+
+using GofrtT = Gofrt<double, Trajectory_numpy>;
+using MSDT = MSD<Trajectory_numpy>;
+using RL = ReadLog_numpy<double>;
+using GK = GreenKuboNComponentIonicFluid<RL, double, double>;
+
+PYBIND11_MODULE(_pyanalisi, m) {
+    m.doc() = "pyanalisi classes of the time-lagged pair-correlation path, computed on an MI355X";
+    py::enum_<Trajectory_numpy::BoxFormat>(m, "BoxFormat", py::arithmetic())
+        .value("Invalid", Trajectory_numpy::BoxFormat::Invalid)
+        .value("CellVectors", Trajectory_numpy::BoxFormat::Cell_vectors)
+        .value("LammpsOrtho", Trajectory_numpy::BoxFormat::Lammps_ortho)
+        .value("LammpsTriclinic", Trajectory_numpy::BoxFormat::Lammps_triclinic);
+
+    py::class_<Trajectory_numpy>(m, "Trajectory")
+        .def(py::init(&make_trajectory), py::keep_alive<1, 2>(), py::keep_alive<1, 3>(), py::keep_alive<1, 4>(),
+             py::keep_alive<1, 5>(), py::arg("positions"), py::arg("velocities"), py::arg("types"),
+             py::arg("lattice"), py::arg("format"), py::arg("pbc_wrap") = false,
+             py::arg("save_rotation_matrix") = false)
+        .def("get_positions_copy",
+             [](Trajectory_numpy &t) {
+                 return copy_array(t.positions(0, 0), {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_natoms(), 3});
+             })
+        .def("get_velocities_copy",
+             [](Trajectory_numpy &t) {
+                 return copy_array(t.velocity(0, 0), {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_natoms(), 3});
+             })
+        .def("get_box_copy",
+             [](Trajectory_numpy &t) {
+                 return copy_array(t.box(0), {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_box_stride()});
+             })
+        .def("get_rotation_matrix",
+             [](Trajectory_numpy &t) {
+                 double *r = t.get_rotation_matrix(0);
+                 if (!r) return py::array_t<double>();
+                 return copy_array(r, {(ssize_t)t.get_ntimesteps(), 3, 3});
+             })
+        .def("get_type", &Trajectory_numpy::get_type, "get the type used for the internal representation")
+        .def("get_nloaded_timesteps", &Trajectory_numpy::get_nloaded_timesteps)
+        .def("getNtimesteps", &Trajectory_numpy::get_ntimesteps)
+        .def("get_current_timestep", &Trajectory_numpy::get_current_timestep)
+        .def("getWrapPbc", &Trajectory_numpy::get_pbc_wrap)
+        .def("minImage", [](Trajectory_numpy &t, size_t i, size_t j, size_t it, size_t ij) {
+            py::array_t<double> a(4);
+            double *x = a.mutable_data();
+            x[3] = t.d2_minImage(i, j, it, ij, x);
+            return a;
+        });
+
+    py::class_<RL>(m, "ReadLog")
+        .def(py::init([](py::buffer data, std::vector<std::string> headers) {
+                 py::buffer_info info = data.request();
+                 if (info.ndim != 2) throw std::runtime_error("Wrong number of dimensions of the log array (must be 2)");
+                 if (info.format != py::format_descriptor<double>::format())
+                     throw std::runtime_error(
+                         "Format of log array is wrong (should be some kind of float, try to change it if it does not "
+                         "work)");
+                 if (!has_no_stride<double>(info)) throw std::runtime_error("Unsupported stride in log array");
+                 return new RL((const double *)info.ptr, (size_t)info.shape[0], (size_t)info.shape[1], headers);
+             }),
+             py::keep_alive<1, 2>())
+        .def("getNtimesteps", &RL::n_timestep);
+
+    py::class_<GofrtT>(m, "Gofrt", py::buffer_protocol())
+        .def(py::init<Trajectory_numpy *, double, double, unsigned int, unsigned int, unsigned int, unsigned int,
+                      unsigned int, bool, int>(),
+             py::arg("trajectory"), py::arg("rmin"), py::arg("rmax"), py::arg("nbin"), py::arg("tmax") = 0,
+             py::arg("nthreads") = 0, py::arg("skip") = 1, py::arg("every") = 1, py::arg("debug") = false,
+             py::arg("form") = ANL_GOFRT_AUTO)
+        .def("reset", &GofrtT::reset)
+        .def("getNumberOfExtraTimestepsNeeded", &GofrtT::nExtraTimesteps)
+        .def("calculate", &GofrtT::calculate)
+        .def("calc_single_th", &GofrtT::calc_single_th)
+        .def("last_stats",
+             [](GofrtT &g) {
+                 int f = -1;
+                 uint64_t c = 0, r = 0;
+                 anl_check(anl_gofrt_last_stats(g.handle(), &f, &c, &r));
+                 return py::make_tuple(f, c, r);
+             })
+        .def_buffer([](GofrtT &g) { return vbuffer(g); });
+
+    py::class_<MSDT>(m, "MeanSquareDisplacement", py::buffer_protocol())
+        .def(py::init<Trajectory_numpy *, unsigned int, unsigned int, unsigned int, bool, bool, bool>(),
+             py::arg("trajectory"), py::arg("skip") = 1, py::arg("tmax") = 0, py::arg("nthreads") = 0,
+             py::arg("cm") = false, py::arg("cm_self") = false, py::arg("debug") = false)
+        .def("reset", &MSDT::reset)
+        .def("getNumberOfExtraTimestepsNeeded", &MSDT::nExtraTimesteps)
+        .def("calculate", &MSDT::calculate)
+        .def_buffer([](MSDT &m) { return vbuffer(m); })
+        .def("__enter__", [](MSDT &m) -> MSDT & { return m; })
+        .def("__exit__", [](MSDT &, py::object, py::object, py::object) {});
+
+    py::class_<GK>(m, "GreenKubo", py::buffer_protocol())
+        .def(py::init<RL *, std::string, unsigned int, std::vector<std::string>, bool, unsigned int, unsigned int,
+                      bool, unsigned int, unsigned int, bool, unsigned int, unsigned int>())
+        .def("getNumberOfExtraTimestepsNeeded", &GK::nExtraTimesteps)
+        .def("reset", &GK::reset)
+        .def("calculate", &GK::calculate)
+        .def("get_columns_description", &GK::get_columns_description)
+        .def_buffer([](GK &g) { return vbuffer(g); });
+
+    // BlockAverageG + MediaVarCovar of GreenKubo, as the CLI's -H path runs it
+    // (main.cpp:456-488): returns (mean, variance) as [narr][block size].
+    m.def("gk_block_average",
+          [](RL &log, std::vector<std::string> headers, unsigned int nblocks, unsigned int skip, unsigned int lmax,
+             unsigned int n_seg, bool sub_mean, unsigned int start_mean) {
+              BlockAverageG<RL, GK, std::string, unsigned int, std::vector<std::string>, bool, unsigned int,
+                            unsigned int, bool, unsigned int, unsigned int, bool, unsigned int, unsigned int>
+                  ba(&log, nblocks);
+              unsigned int narr = (unsigned int)(headers.size() * headers.size() * 3 + 2);
+              MediaVarCovar<GK> mvc(narr, {});
+              ba.calcola_custom(&mvc, std::string(""), skip, headers, false, lmax, 1u, sub_mean, start_mean, n_seg,
+                                false, 10u, 100u);
+              py::array_t<double> mean({(ssize_t)narr, (ssize_t)mvc.size()}), var({(ssize_t)narr, (ssize_t)mvc.size()});
+              for (unsigned int c = 0; c < narr; c++) {
+                  std::memcpy(mean.mutable_data() + (size_t)c * mvc.size(), mvc.media(c), mvc.size() * sizeof(double));
+                  std::memcpy(var.mutable_data() + (size_t)c * mvc.size(), mvc.varianza(c), mvc.size() * sizeof(double));
+              }
+              return py::make_tuple(mean, var);
+          });
+    // BlockAverageG + MediaVar of Gofrt / MSD (blockaverage.h:194-203), the
+    // CLI's -g and -Q paths: returns (mean, variance) in the calculation's layout.
+    m.def("gofrt_block_average",
+          [](Trajectory_numpy &t, unsigned int nblocks, double rmin, double rmax, unsigned int nbin, unsigned int tmax,
+             unsigned int skip, unsigned int every) {
+              BlockAverageG<Trajectory_numpy, GofrtT, double, double, unsigned int, unsigned int, unsigned int,
+                            unsigned int, unsigned int, bool, int>
+                  ba(&t, nblocks);
+              ba.calculate(rmin, rmax, nbin, tmax, 1u, skip, every, false, (int)ANL_GOFRT_AUTO);
+              auto shape = ba.media()->get_shape();
+              return py::make_tuple(copy_array(ba.media()->access_vdata(), shape),
+                                    copy_array(ba.varianza()->access_vdata(), shape));
+          });
+    m.def("msd_block_average",
+          [](Trajectory_numpy &t, unsigned int nblocks, unsigned int skip, unsigned int tmax, bool cm, bool cm_self) {
+              BlockAverageG<Trajectory_numpy, MSDT, unsigned int, unsigned int, unsigned int, bool, bool, bool> ba(
+                  &t, nblocks);
+              ba.calculate(skip, tmax, 1u, cm, cm_self, false);
+              auto shape = ba.media()->get_shape();
+              std::vector<ssize_t> sh(shape.begin(), shape.end());
+              return py::make_tuple(copy_array(ba.media()->access_vdata(), sh),
+                                    copy_array(ba.varianza()->access_vdata(), sh));
+          });
+
+    m.def("info", []() { return std::string(anl_info()); });
+    m.def("has_mmap", []() { return false; });
+}

@@ -62,28 +62,38 @@ __host__ __device__ static inline void to_fractional(const Cell &c, double x, do
     s[2] = sc;
 }
 
-// 16-bit fixed point of the fractional part: round(frac(s) * 65536) mod 65536.
-__device__ static inline uint32_t fix16(double s) {
+// 32-bit fixed point of the fractional part: round(frac(s) * 2^32) mod 2^32
+// (s - floor(s) and the power-of-two scaling are exact in FP64).
+__device__ static inline uint32_t fix32(double s) {
     double f = s - floor(s);
-    double u = rint(f * 65536.0);
-    return (uint32_t)(int64_t)u & 0xFFFFu;
+    double u = rint(f * 4294967296.0);
+    return (uint32_t)(uint64_t)u;
 }
 
-__global__ void frac16_kernel(const double4 *__restrict__ pos, uint2 *__restrict__ out, uint64_t total, Cell c) {
+// Per atom and frame: {a, b, c} as 32-bit fixed-point fractions (frac32) and
+// the same rounded to 16 bits, packed {a | b << 16, c} (frac16).
+__global__ void frac_kernel(const double4 *__restrict__ pos, uint2 *__restrict__ out16, uint4 *__restrict__ out32,
+                            uint64_t total, Cell c) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
     double4 x = pos[k];
     double s[3];
     to_fractional(c, x.x, x.y, x.z, s);
-    out[k] = make_uint2(fix16(s[0]) | (fix16(s[1]) << 16), fix16(s[2]));
+    uint32_t a = fix32(s[0]), b = fix32(s[1]), cc = fix32(s[2]);
+    uint32_t a16 = ((a + 0x8000u) >> 16) & 0xFFFFu;
+    uint32_t b16 = ((b + 0x8000u) >> 16) & 0xFFFFu;
+    uint32_t c16 = ((cc + 0x8000u) >> 16) & 0xFFFFu;
+    out16[k] = make_uint2(a16 | (b16 << 16), c16);
+    out32[k] = make_uint4(a, b, cc, 0u);
 }
 
 void build_frac16(anl_traj *tr, hipStream_t s) {
     if (tr->frac16) return;
     uint64_t total = tr->F * tr->N;
     ANL_HIP(hipMalloc(&tr->frac16, total * sizeof(uint2)));
+    ANL_HIP(hipMalloc(&tr->frac32, total * sizeof(uint4)));
     unsigned blocks = (unsigned)((total + 255) / 256);
-    hipLaunchKernelGGL(frac16_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->frac16, total, tr->cell0);
+    hipLaunchKernelGGL(frac_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->frac16, tr->frac32, total, tr->cell0);
     ANL_HIP(hipGetLastError());
     ANL_HIP(hipStreamSynchronize(s));
 }
@@ -190,7 +200,7 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
             tr->box_const = std::memcmp(&tr->box_host[f * 9], &tr->box_host[0], 9 * sizeof(double)) == 0;
         tr->cell0 = cell_of(&tr->box_host[0]);
 
-        // slot order: type id, then Morton key of the frame-0 fractional position
+        // slot order: Morton key of the frame-0 fractional position (ties by index)
         std::vector<uint64_t> key(natoms);
         for (uint64_t i = 0; i < natoms; i++) {
             double s[3];
@@ -201,7 +211,7 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
                 q[k] = std::min<uint32_t>(1023u, (uint32_t)(f * 1024.0));
             }
             uint64_t m = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
-            key[i] = ((uint64_t)type_id[i] << 32) | m;
+            key[i] = m;
         }
         std::vector<int32_t> order(natoms);
         std::iota(order.begin(), order.end(), 0);
@@ -246,6 +256,7 @@ int anl_traj_destroy(anl_traj *tr) {
         (void)hipFree(tr->orig_slot);
         (void)hipFree(tr->cm);
         (void)hipFree(tr->frac16);
+        (void)hipFree(tr->frac32);
         delete tr;
     });
 }

@@ -95,9 +95,11 @@ int anl_gofrt_calc_single_th(anl_gofrt *g, int t, int imedia, int a0, int a1, ui
  * counts_dev is [nblocks][leff_max][nch][nbin], leff_max = max leff_b. */
 int anl_gofrt_calculate_blocks_async(anl_gofrt *g, const uint64_t *primo, const uint32_t *n,
                                      uint32_t nblocks, uint64_t *counts_dev, void *stream);
-/* Form actually used by the last calculate (ANL_GOFRT_FP64/SCREENED) and the
- * number of candidate pairs the screen passed to the exact evaluation. */
-int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates);
+/* Form actually used by the last calculate (ANL_GOFRT_FP64/SCREENED), the
+ * number of candidate pairs the screen passed to the FP32 evaluation, and how
+ * many of those were re-evaluated in FP64 (near a bin edge, rmin or rmax).
+ * Diagnostics of this implementation; no counterpart in the reference. */
+int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uint64_t *rechecks);
 
 /* ---- mean square displacement ----------------------------------------- */
 /* Replaces MSD<T> (msd.h:30-47, msd.cpp:40-146): out_host [leff][f_cm][ntypes]

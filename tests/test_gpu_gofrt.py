@@ -31,7 +31,7 @@ def test_counts_match_oracle(case, form):
     g.reset(n)
     got = g.calculate(1)
     assert first_diff(got, ref) is None, first_diff(got, ref)
-    used, cand = g.stats()
+    used, cand, nre = g.stats()
     assert used == form
     if form == capi.GOFRT_SCREENED:
         nin = int(ref.sum())

@@ -1,0 +1,175 @@
+"""The pyanalisi surface (pybind11 over the C++ operator classes over the C
+ABI) on the GPU, called the way the reference's own tests and the CLI call it:
+tests/test_gk.py:4-12, tests/test_gofrt.py:4-13, tests/test_msd.py:4-11,
+main.cpp:456-516 (block averages)."""
+import os
+
+import numpy as np
+import pytest
+
+from analisi_amd import synth
+from conftest import GOLDEN, is_same
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pa():
+    from analisi_amd import pyanalisi
+    return pyanalisi
+
+
+@pytest.fixture
+def analisi_log(pa, gk_ref):
+    return pa.ReadLog(np.array(gk_ref["log"], copy=True), list(gk_ref["headers"]))
+
+
+def test_gk_reference_test_call(pa, analisi_log, gk_ref):
+    gk = pa.GreenKubo(analisi_log, '', 1, ['c_flux[1]', 'c_vcm[1][1]'], False, 2000, 2, False, 0, 4, False, 1, 100)
+    gk.reset(analisi_log.getNtimesteps() - 2000)
+    gk.calculate(0)
+    m = np.array(gk, copy=False)
+    assert m.shape == (2000, 14)
+    assert is_same(m, gk_ref["test_gk"]).all()
+
+
+def test_gk_header_error(pa, analisi_log):
+    with pytest.raises(RuntimeError, match="not found"):
+        pa.GreenKubo(analisi_log, '', 1, ['nope'], False, 20, 1, False, 0, 1, False, 1, 100)
+
+
+@pytest.mark.parametrize("name,heads", [("single", ["c_flux[1]"]), ("double", ["c_flux[1]", "c_vcm[1][1]"])])
+def test_gk_block_average_matches_cli_text(pa, analisi_log, name, heads):
+    """BlockAverageG + MediaVarCovar on the GPU path, printed as the CLI prints
+    it (main.cpp:502-513, 6 significant digits): text-identical to the
+    reference's tests/data/cli/GK_heat_* outputs."""
+    mean, var = pa.gk_block_average(analisi_log, heads, 20, 1, 0, 1, False, 0)
+    narr, s = mean.shape
+    mine = [" ".join(sum([["%g" % mean[j, i], "%g" % var[j, i]] for j in range(narr)], [])) + " " for i in range(s)]
+    with open(os.path.join(GOLDEN, "cli_GK_heat_%s.txt" % name)) as f:
+        ref = [l.rstrip("\n") for l in f if l.strip() and not l.startswith("#")]
+    norm = lambda x: x.replace("-nan", "nan")
+    assert [norm(a) for a in mine] == [norm(b) for b in ref]
+
+
+def _cell_traj(F, N, seed):
+    """A trajectory in the reference's test layout (conftest.py:31-45): CellVectors
+    boxes, types 0 except the last 16 atoms (types 1 and 2)."""
+    cell = np.diag([4.14141307, 7.81544411, 6.44646321])   # tests/data/cells.npy[0]
+    box_l = synth.lammps_box(np.diag(cell))
+    pos = synth.random_walk(seed, N, F, box_l, 0.05)
+    vel = np.zeros_like(pos)
+    types = np.zeros(N, dtype="i")
+    types[-16:-8] = 1
+    types[-8:] = 2
+    boxes = np.tile(cell, (F, 1, 1))
+    return pos, vel, types, boxes, box_l
+
+
+def test_gofrt_reference_test_call(pa, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)                 # debug=1 appends ./gofrt.dump
+    pos, vel, types, boxes, box_l = _cell_traj(712, 56, 3)
+    traj = pa.Trajectory(pos, vel, types, boxes, pa.BoxFormat.CellVectors, False, False)
+    gofr = pa.Gofrt(traj, 0.0, 3.8, 200, 10, 4, 10, False, 1)
+    gofr.reset(700)
+    gofr.calculate(0)
+    gr = np.array(gofr, copy=True)
+    assert gr.shape == (10, 12, 200)
+    assert os.path.exists("gofrt.dump")
+    ot = O.Traj(pos, types, np.tile(box_l, (712, 1)), False)
+    counts, vref = O.gofrt(ot, 0.0, 3.8, 200, 10, 4, 10, 1, 700, 0)
+    np.testing.assert_array_equal(np.round(gr * 70).astype(np.int64), counts.astype(np.int64))
+    assert is_same(gr, vref).all()              # count*incr vs the reference's repeated sums
+    # the screen cannot be used (rmax > half the 4.14 A cell): AUTO picked FP64
+    assert gofr.last_stats()[0] == 0
+
+
+def test_gofrt_calc_single_th(pa):
+    pos, vel, types, boxes, box_l = _cell_traj(30, 56, 4)
+    traj = pa.Trajectory(pos, vel, types, boxes, pa.BoxFormat.CellVectors, False, False)
+    g = pa.Gofrt(traj, 0.5, 2.0, 30, 5, 1, 1, 1, False)
+    g.reset(20)
+    g.calculate(0)                               # sets incr and zeroes
+    full = np.array(g, copy=True)
+    ot = O.Traj(pos, types, np.tile(box_l, (30, 1)), False)
+    c = np.zeros((5, 12, 30), np.uint64)
+    for t in range(5):
+        for im in range(20):
+            O.gofrt_single(ot, 0.5, 2.0, 30, 5, t, im, 0, 56, 0, c)
+    np.testing.assert_array_equal(np.round(full * 20).astype(np.int64), c.astype(np.int64))
+
+
+def test_msd_reference_test_call(pa):
+    pos, vel, types, boxes, box_l = _cell_traj(700, 56, 5)
+    traj = pa.Trajectory(pos, vel, types, boxes, pa.BoxFormat.CellVectors, False, False)
+    msd = pa.MeanSquareDisplacement(traj, 10, 400, 4, True, False, False)
+    msd.reset(300)
+    msd.calculate(0)
+    m = np.array(msd, copy=False)
+    assert m.shape == (300, 2, 3)
+    ot = O.Traj(pos, types, np.tile(box_l, (700, 1)), False)
+    ref = O.msd(ot, 10, 400, 4, True, False, 300, 0)
+    assert is_same(m, ref).all()
+    with pytest.raises(RuntimeError, match="too short"):
+        msd.reset(400)
+        msd.calculate(0)
+
+
+def test_block_averages_match_oracle(pa):
+    F, N = 260, 120
+    box_l = synth.lammps_box(11.0, [0.6, -0.4, 0.5])
+    pos = synth.random_walk(8, N, F, box_l, 0.1)
+    types = synth.types_by_fraction(N, [0.6, 0.4])
+    traj = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsTriclinic,
+                         False, False)
+    ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), True)
+    nb = 5
+    # Gofrt: blocks of s = (F - nExtra)/nb, MediaVar over count*incr vectors
+    mean, var = pa.gofrt_block_average(traj, nb, 0.0, 5.0, 25, 8, 2, 1)
+    s = (F - O.n_extra_timesteps(F, nb, 8)) // nb
+    mv = O.MediaVar(mean.size)
+    for b in range(nb):
+        _, v = O.gofrt(ot, 0.0, 5.0, 25, 8, 3, 2, 1, s, b * s)
+        mv.add(v)
+    mv.end(nb)
+    assert is_same(mean.ravel(), mv.mean, 1e-10).all()
+    assert is_same(var.ravel(), mv.var, 1e-8).all()
+    # MSD
+    mean, var = pa.msd_block_average(traj, nb, 3, 10, True, True)
+    s = (F - O.n_extra_timesteps(F, nb, 10)) // nb
+    mv = O.MediaVar(mean.size)
+    for b in range(nb):
+        mv.add(O.msd(ot, 3, 10, 2, True, True, s, b * s))
+    mv.end(nb)
+    assert is_same(mean.ravel(), mv.mean, 1e-9).all()
+    np.testing.assert_allclose(var.ravel(), mv.var, rtol=1e-6, atol=1e-28)
+
+
+def test_triclinic_cell_vectors_rotation(pa):
+    """A triclinic cell given as rotated cell vectors: the trajectory recovers
+    the LAMMPS cell (positive diagonal, triclinic.h:30-38) and rotates
+    positions into it; g(r,t) equals the oracle on the rotated positions."""
+    F, N = 12, 90
+    Lc = np.array([[9.0, 0, 0], [1.1, 9.5, 0], [-0.8, 0.7, 10.0]])    # rows a, b, c (lower triangular)
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    vecs = Lc @ R.T                                                     # rotated lattice vectors (rows)
+    box_l = synth.lammps_box([9.0, 9.5, 10.0], [1.1, -0.8, 0.7])
+    posl = synth.random_walk(9, N, F, box_l, 0.1)
+    posr = posl @ R.T
+    cellv = np.tile(vecs.T, (F, 1, 1))                                  # numpy layout: columns = vectors
+    traj = pa.Trajectory(posr, np.zeros_like(posr), np.zeros(N, "i"), np.ascontiguousarray(cellv),
+                         pa.BoxFormat.CellVectors, False, True)
+    b = traj.get_box_copy()[0]
+    np.testing.assert_allclose(b[3:], [4.5, 4.75, 5.0, 1.1, -0.8, 0.7], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(traj.get_positions_copy(), posl, atol=1e-11)
+    g = pa.Gofrt(traj, 0.0, 4.0, 40, 3, 1, 1, 1, False)
+    g.reset(8)
+    g.calculate(0)
+    rot = traj.get_positions_copy()
+    ot = O.Traj(rot, np.zeros(N, np.int32), np.tile(box_l, (F, 1)), True)
+    ot.box[:] = traj.get_box_copy()
+    ot.pos[:] = rot
+    counts, _ = O.gofrt(ot, 0.0, 4.0, 40, 3, 1, 1, 1, 8, 0)
+    np.testing.assert_array_equal(np.round(np.array(g) * 8).astype(np.int64), counts.astype(np.int64))
