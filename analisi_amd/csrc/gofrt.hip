@@ -24,6 +24,7 @@
 // which reproduces the reference's counts bit for bit.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "anl_internal.h"
@@ -182,13 +183,15 @@ static Screen make_screen(const anl_traj *tr, double rmax) {
 // root.  On u = (d - rmin)/dr: that over dr plus the subtraction/product and
 // the reference's own double->float rounding of u.  The window is four times
 // the bound.
-static float du_window(const Screen &s, double rmin, double dr, uint32_t nbin) {
+static float du_window(const Screen &s, double rmin, double rmax, double dr, uint32_t nbin) {
     const double e24 = std::ldexp(1.0, -24);
     const double term = s.row_sum * s.frac_max;
     const double e_comp = s.row_sum * std::ldexp(1.0, -31) + 10.0 * e24 * term;
-    const double dmax = std::sqrt(3.0) * term + std::fabs(rmin) + 1.0;
+    // only distances up to rmax (u up to nbin) need the bound: beyond the
+    // window around nbin every pair is rejected, as the reference does
+    const double dmax = std::fabs(rmax) + 1.0;
     const double e_d = std::sqrt(3.0) * e_comp + 8.0 * e24 * dmax;
-    const double umax = dmax / dr + nbin + 1.0;
+    const double umax = nbin + 1.0;
     const double du = e_d / dr + 8.0 * e24 * umax;
     return (float)(4.0 * du + 1e-6);
 }
@@ -231,13 +234,35 @@ struct GofrtArgs {
     float rminf, inv_drf;
     uint32_t threshold;
     int32_t a0, a1;              // original-index range of i atoms
-    uint32_t lag0;               // first lag (lag = lag0 + blockIdx.y * every)
+    uint32_t lag0;               // first lag (lag = lag0 + lag index * every)
+    uint32_t ntiles, nlag, nwork, per_xcd;  // 1-D grid decoding, see decode_wg (per_xcd 0: no remap)
     // screened form: 32-bit fractions and the FP32 evaluation of candidates
     const uint4 *frac32;
     unsigned long long *nrecheck;
     float cx, cxy, cxz, cy, cyz, cz;  // cell matrix entries times 2^-32
     float du_margin;                  // bins: |u - nearest integer| below this -> FP64
 };
+
+// The grid is one-dimensional.  Workgroups are dispatched round-robin over
+// the 8 XCDs (hardware id h runs on XCD h % 8), so logical workgroup
+// L = (h % 8) * per_xcd + h / 8 puts consecutive logical ids -- the i tiles of
+// one (lag, t0 range), which read the same j frames -- on one XCD and its L2.
+// The map is a bijection on [0, 8 * per_xcd); ids >= the real count idle.
+struct WgId {
+    uint32_t tile, lagi, work;
+    bool valid;
+};
+__device__ __forceinline__ WgId decode_wg(uint32_t ntiles, uint32_t nlag, uint32_t nwork, uint32_t per_xcd) {
+    const uint32_t h = blockIdx.x;
+    const uint32_t L = per_xcd ? (h & 7u) * per_xcd + (h >> 3) : h;
+    WgId w;
+    w.valid = L < ntiles * nlag * nwork;
+    w.tile = L % ntiles;
+    const uint32_t rest = L / ntiles;
+    w.lagi = rest % nlag;
+    w.work = rest / nlag;
+    return w;
+}
 
 struct DevCell {
     double lh0, lh1, lh2, L0, L1, L2, xy, xz, yz;
@@ -321,9 +346,11 @@ __device__ __forceinline__ void lds_flush(const GofrtArgs &a, const unsigned *hi
 template <bool TRI>
 __global__ __launch_bounds__(WG) void gofrt_fp64_kernel(const GofrtArgs a) {
     extern __shared__ unsigned char smem[];
-    const WorkItem w = a.work[blockIdx.z];
+    const WgId id = decode_wg(a.ntiles, a.nlag, a.nwork, a.per_xcd);
+    if (!id.valid) return;
+    const WorkItem w = a.work[id.work];
     const BlockInfo b = a.blocks[w.block];
-    const uint32_t t = a.lag0 + blockIdx.y * a.every;
+    const uint32_t t = a.lag0 + id.lagi * a.every;
     if (t >= b.leff) return;
     double *thr = (double *)smem;
     int *chan = (int *)(thr + a.nbin + 1);
@@ -332,7 +359,7 @@ __global__ __launch_bounds__(WG) void gofrt_fp64_kernel(const GofrtArgs a) {
     __syncthreads();
 
     const uint64_t N = a.N;
-    const uint64_t slot = (uint64_t)blockIdx.x * WG + threadIdx.x;
+    const uint64_t slot = (uint64_t)id.tile * WG + threadIdx.x;
     bool active = slot < N;
     int ti = 0;
     if (active) {
@@ -389,6 +416,10 @@ typedef const __attribute__((address_space(4))) unsigned long long cu64_t;
 __device__ __forceinline__ uint32_t lane_prefix(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
+// base + number of set bits of m below this lane
+__device__ __forceinline__ uint32_t lane_slot(unsigned long long m, uint32_t base) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
+}
 
 // One queued candidate (entry = j << 8 | i_local).  The distance is first
 // evaluated in FP32 from the 32-bit fractional difference (the wrapped
@@ -398,11 +429,9 @@ __device__ __forceinline__ uint32_t lane_prefix(unsigned long long m) {
 // du_margin is four times a worst-case bound of |u_fp32 - u_reference|, so
 // outside the window both floor to the same bin.
 template <bool TRI>
-__device__ __forceinline__ void eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const int *ti_lds,
-                                               const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
-                                               const double4 *__restrict__ pj, uint64_t slot0, const DevCell &c,
-                                               const double *thr, const int *chan, unsigned *hist,
-                                               unsigned long long &nre) {
+__device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const int *ti_lds,
+                                               const uint4 *__restrict__ fj32, uint64_t slot0, const int *chan,
+                                               unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
     const uint4 ui = fi_lds[il];
@@ -422,29 +451,38 @@ __device__ __forceinline__ void eval_candidate(const GofrtArgs &a, uint32_t e, c
     }
     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
-    const float k = rintf(u);
-    const int nbin = (int)a.nbin;
-    int bb;
-    bool ok;
-    if (fabsf(u - k) < a.du_margin) {
-        nre += 1;
-        const double4 xi = pi[slot0 + il];
-        const double4 xj = pj[j];
-        const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
-        ok = d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn;
-        bb = ok ? bin_of(d2r, thr, nbin, a.rminf, a.inv_drf) : 0;
-    } else {
-        bb = (int)floorf(u);
-        ok = bb >= 0 && bb < nbin;
-    }
-    if (ok) {
+    if (fabsf(u - rintf(u)) < a.du_margin) return true;
+    const int bb = (int)floorf(u);
+    if (bb >= 0 && bb < (int)a.nbin) {
         const int tj = a.type_slot[j];
         const int ch = chan[ti_lds[il] * (int)a.nt + tj] + (slot0 + il == j ? (int)a.half : 0);
-        atomicAdd(&hist[__umul24((unsigned)ch, (unsigned)nbin) + (unsigned)bb], 1u);
+        atomicAdd(&hist[__umul24((unsigned)ch, a.nbin) + (unsigned)bb], 1u);
+    }
+    return false;
+}
+
+// Exact FP64 evaluation (the reference's sequence) of a candidate whose FP32
+// distance fell inside the window around a bin edge, rmin or rmax.
+template <bool TRI>
+__device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e, const int *ti_lds,
+                                                  const double4 *__restrict__ pi, const double4 *__restrict__ pj,
+                                                  uint64_t slot0, const DevCell &c, const double *thr, const int *chan,
+                                                  unsigned *hist) {
+    const uint32_t il = e & 0xFFu;
+    const uint32_t j = e >> 8;
+    const double4 xi = pi[slot0 + il];
+    const double4 xj = pj[j];
+    const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
+    if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) {
+        const int bb = bin_of(d2r, thr, (int)a.nbin, a.rminf, a.inv_drf);
+        const int tj = a.type_slot[j];
+        const int ch = chan[ti_lds[il] * (int)a.nt + tj] + (slot0 + il == j ? (int)a.half : 0);
+        atomicAdd(&hist[__umul24((unsigned)ch, a.nbin) + (unsigned)bb], 1u);
     }
 }
 
-constexpr int SQCAP = 320;   // per-wavefront queue: < 64 left + 4 masks of 64
+constexpr int SQCAP = 576;   // per-wavefront queue: < 64 left + up to 8 masks of 64
+constexpr int RQCAP = 128;   // per-wavefront recheck queue: < 64 left + one batch
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -454,20 +492,23 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 #define ANL_APPEND(P, M, J)                                                 \
     if (M) {                                                                \
-        if (P) q[qn + lane_prefix(M)] = ((uint32_t)(J) << 8) | il;          \
+        if (P) q[lane_slot(M, qn)] = ((uint32_t)(J) << 8) | il;             \
         qn += (uint32_t)__popcll(M);                                        \
     }
 
-template <bool TRI>
+template <bool TRI, int UNROLL>
 __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     extern __shared__ unsigned char smem[];
-    const WorkItem w = a.work[blockIdx.z];
+    const WgId id = decode_wg(a.ntiles, a.nlag, a.nwork, a.per_xcd);
+    if (!id.valid) return;
+    const WorkItem w = a.work[id.work];
     const BlockInfo b = a.blocks[w.block];
-    const uint32_t t = a.lag0 + blockIdx.y * a.every;
+    const uint32_t t = a.lag0 + id.lagi * a.every;
     if (t >= b.leff) return;
     uint4 *fi_lds = (uint4 *)smem;                          // [WG]
     uint32_t *queue = (uint32_t *)(fi_lds + WG);            // [WG/64][SQCAP]
-    int *ti_lds = (int *)(queue + (WG / 64) * SQCAP);       // [WG]
+    uint32_t *rqueue = queue + (WG / 64) * SQCAP;           // [WG/64][RQCAP]
+    int *ti_lds = (int *)(rqueue + (WG / 64) * RQCAP);      // [WG]
     double *thr = (double *)(ti_lds + WG);                  // [nbin+1]
     int *chan = (int *)(thr + a.nbin + 1);                  // [nt*nt]
     unsigned *hist = (unsigned *)(chan + a.nt * a.nt);      // [nch*nbin]
@@ -476,7 +517,8 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     const uint64_t N = a.N;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t *q = queue + (threadIdx.x >> 6) * SQCAP;
-    const uint64_t slot0 = (uint64_t)blockIdx.x * WG;
+    uint32_t *rq = rqueue + (threadIdx.x >> 6) * RQCAP;
+    const uint64_t slot0 = (uint64_t)id.tile * WG;
     const uint64_t slot = slot0 + threadIdx.x;
     bool active = slot < N;
     int ti = 0;
@@ -487,7 +529,7 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     }
     ti_lds[threadIdx.x] = ti;
     const uint32_t il = threadIdx.x;
-    const uint32_t N4 = (uint32_t)(N & ~(uint64_t)3);
+    const uint32_t NU = (uint32_t)(N - N % UNROLL);
     // strict comparison against threshold+1; inactive lanes compare against 0
     // and never pass
     const uint32_t th = active ? a.threshold + 1u : 0u;
@@ -510,49 +552,71 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
             ci = f.y;
         }
         __syncthreads();
-        uint32_t qn = 0;
-        for (uint32_t j = 0; j < N4; j += 4) {
-            const unsigned long long g0 = fjc[j];
-            const unsigned long long g1 = fjc[j + 1];
-            const unsigned long long g2 = fjc[j + 2];
-            const unsigned long long g3 = fjc[j + 3];
-            const bool p0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32)) < th;
-            const bool p1 = sq16(abi, (uint32_t)g1, ci, (uint32_t)(g1 >> 32)) < th;
-            const bool p2 = sq16(abi, (uint32_t)g2, ci, (uint32_t)(g2 >> 32)) < th;
-            const bool p3 = sq16(abi, (uint32_t)g3, ci, (uint32_t)(g3 >> 32)) < th;
-            const unsigned long long m0 = __ballot(p0);
-            const unsigned long long m1 = __ballot(p1);
-            const unsigned long long m2 = __ballot(p2);
-            const unsigned long long m3 = __ballot(p3);
-            if (m0 | m1 | m2 | m3) {
-                ANL_APPEND(p0, m0, j)
-                ANL_APPEND(p1, m1, j + 1)
-                ANL_APPEND(p2, m2, j + 2)
-                ANL_APPEND(p3, m3, j + 3)
+        uint32_t qn = 0, rn = 0;
+        // evaluate the queued entry of this lane; candidates inside the edge
+        // window go to the recheck queue, evaluated 64 at a time
+        auto eval_batch = [&](uint32_t e, bool valid) {
+            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, ti_lds, fj32, slot0, chan, hist);
+            const unsigned long long mr = __ballot(need);
+            if (mr) {
+                if (need) rq[rn + lane_prefix(mr)] = e;
+                rn += (uint32_t)__popcll(mr);
+                if (rn >= 64) {
+                    wave_sync_lds();
+                    rn -= 64;
+                    nre += 1;
+                    recheck_candidate<TRI>(a, rq[rn + lane], ti_lds, pi, pj, slot0, c, thr, chan, hist);
+                    wave_sync_lds();
+                }
+            }
+        };
+        for (uint32_t j = 0; j < NU; j += UNROLL) {
+            unsigned long long g[UNROLL];
+            bool p[UNROLL];
+            unsigned long long m[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) g[u] = fjc[j + u];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) p[u] = sq16(abi, (uint32_t)g[u], ci, (uint32_t)(g[u] >> 32)) < th;
+            unsigned long long any = 0;
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                m[u] = __ballot(p[u]);
+                any |= m[u];
+            }
+            if (any) {
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    ANL_APPEND(p[u], m[u], j + u)
+                }
                 while (qn >= 64) {
                     wave_sync_lds();
                     qn -= 64;
                     const uint32_t e = q[qn + lane];
                     ncand += 1;
-                    eval_candidate<TRI>(a, e, fi_lds, ti_lds, fj32, pi, pj, slot0, c, thr, chan, hist, nre);
+                    eval_batch(e, true);
                     wave_sync_lds();
                 }
             }
         }
-        for (uint32_t j = N4; j < (uint32_t)N; j++) {  // N % 4 tail
+        for (uint32_t j = NU; j < (uint32_t)N; j++) {  // N % UNROLL tail
             const unsigned long long g0 = fjc[j];
             const bool p0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32)) < th;
             const unsigned long long m0 = __ballot(p0);
             ANL_APPEND(p0, m0, j)
         }
-        // drain: fewer than 64 + 3*64 entries, 64 at a time
+        // drain: fewer than 64 + (UNROLL-1)*64 entries, 64 at a time, then the rechecks
         wave_sync_lds();
         for (uint32_t base = 0; base < qn; base += 64) {
-            if (base + lane < qn) {
-                const uint32_t e = q[base + lane];
-                ncand += 1;
-                eval_candidate<TRI>(a, e, fi_lds, ti_lds, fj32, pi, pj, slot0, c, thr, chan, hist, nre);
-            }
+            const bool valid = base + lane < qn;
+            const uint32_t e = valid ? q[base + lane] : 0u;
+            ncand += valid ? 1 : 0;
+            eval_batch(e, valid);
+        }
+        wave_sync_lds();
+        if (lane < rn) {
+            nre += 1;
+            recheck_candidate<TRI>(a, rq[lane], ti_lds, pi, pj, slot0, c, thr, chan, hist);
         }
         wave_sync_lds();
     }
@@ -577,8 +641,17 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
 // ---------------------------------------------------------------------------
 using namespace anl;
 
+// Launch shape of the screened kernel: unroll of the j loop, XCD-aware
+// workgroup order, pair evaluations per workgroup (sets the t0 chunk).
+struct GofrtTuning {
+    int unroll = 4;
+    bool xcd = false;
+    uint64_t pairs_per_wg = 1ull << 31;
+};
+
 struct anl_gofrt {
     anl_traj *tr = nullptr;
+    GofrtTuning tune;
     double rmin = 0, rmax = 0, dr = 0, rmin2 = 0, rmax2 = 0;
     uint32_t nbin = 0, lmax = 0, skip = 1, every = 1;
     int form = ANL_GOFRT_AUTO;
@@ -625,7 +698,7 @@ Launch plan(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nbl
     const uint64_t N = g->tr->N;
     uint64_t cap = (0xFFFFFFFFull / ((uint64_t)WG * N));
     if (cap < 1) cap = 1;
-    uint64_t target = (1ull << 31) / ((uint64_t)WG * N);
+    uint64_t target = g->tune.pairs_per_wg / ((uint64_t)WG * N);
     if (target < 1) target = 1;
     uint64_t chunk = std::min(cap, target);
     for (uint32_t bi = 0; bi < nblocks; bi++) {
@@ -713,14 +786,21 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
         a.cy = (float)(cl.L[1] * sc);
         a.cyz = (float)(cl.yz * sc);
         a.cz = (float)(cl.L[2] * sc);
-        a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->dr, g->nbin) : 0.0f;
+        a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->rmax, g->dr, g->nbin) : 0.0f;
     }
 
     const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
     const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;
     a.lag0 = lag0;
-    if (L.work.size() > 65535u * 16u) fail(ANL_ERR_ARG, "too many work items");
-    dim3 grid(ntiles, nlag, (uint32_t)L.work.size());
+    const uint64_t total = (uint64_t)ntiles * nlag * L.work.size();
+    const uint64_t per = (total + 7) / 8;
+    if (8 * per >= (1ull << 32)) fail(ANL_ERR_ARG, "too many workgroups (reduce the trajectory block)");
+    a.ntiles = ntiles;
+    a.nlag = nlag;
+    a.nwork = (uint32_t)L.work.size();
+    const bool remap = g->tune.xcd && form == ANL_GOFRT_SCREENED;
+    a.per_xcd = remap ? (uint32_t)per : 0u;
+    dim3 grid((uint32_t)(remap ? 8 * per : total));
     size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
                     (size_t)a.nch * g->nbin * sizeof(unsigned);
     if (form == ANL_GOFRT_FP64) {
@@ -729,10 +809,16 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
         if (tr->triclinic) hipLaunchKernelGGL(gofrt_fp64_kernel<true>, grid, dim3(WG), lds, s, a);
         else hipLaunchKernelGGL(gofrt_fp64_kernel<false>, grid, dim3(WG), lds, s, a);
     } else {
-        size_t lds = WG * sizeof(uint4) + (WG / 64) * SQCAP * sizeof(uint32_t) + WG * sizeof(int) + common;
+        size_t lds = WG * sizeof(uint4) + (WG / 64) * (SQCAP + RQCAP) * sizeof(uint32_t) + WG * sizeof(int) + common;
         if (lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
-        if (tr->triclinic) hipLaunchKernelGGL(gofrt_screened_kernel<true>, grid, dim3(WG), lds, s, a);
-        else hipLaunchKernelGGL(gofrt_screened_kernel<false>, grid, dim3(WG), lds, s, a);
+        const bool u8 = g->tune.unroll == 8;
+        if (tr->triclinic) {
+            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<true, 8>), grid, dim3(WG), lds, s, a);
+            else hipLaunchKernelGGL((gofrt_screened_kernel<true, 4>), grid, dim3(WG), lds, s, a);
+        } else {
+            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<false, 8>), grid, dim3(WG), lds, s, a);
+            else hipLaunchKernelGGL((gofrt_screened_kernel<false, 4>), grid, dim3(WG), lds, s, a);
+        }
     }
     ANL_HIP(hipGetLastError());
     g->last_form = form;
@@ -759,6 +845,12 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
         g->skip = skip == 0 ? 1 : skip;     // calculatemultithread.h:43-46
         g->every = every == 0 ? 1 : every;
         g->form = form;
+        if (const char *e = std::getenv("ANL_GOFRT_UNROLL")) g->tune.unroll = std::atoi(e) == 8 ? 8 : 4;
+        if (const char *e = std::getenv("ANL_GOFRT_XCD")) g->tune.xcd = std::atoi(e) != 0;
+        if (const char *e = std::getenv("ANL_GOFRT_PAIRS_PER_WG")) {
+            long long v = std::atoll(e);
+            if (v > 0) g->tune.pairs_per_wg = (uint64_t)v;
+        }
         g->dr = (rmax - rmin) / (double)nbin;  // gofrt.cpp:27-29
         g->rmax2 = rmax * rmax;
         g->rmin2 = rmin * rmin;

@@ -127,7 +127,19 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=8, help="reset(n) of the CPU baseline sample (n t0 x n lags)")
     ap.add_argument("--fp64-n", type=int, default=101, help="reset(n) of the plain-FP64 slice (n t0 x 101 lags)")
     ap.add_argument("--n", type=int, default=1900, help="reset(n) of the timed run (1900 = the full workload)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB mebibytes device-to-device (a plain copy kernel of known size, "
+                         "to scale the FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        hip = Hip()
+        nb = args.calibrate_copy << 20
+        src, dst = hip.malloc(nb), hip.malloc(nb)
+        for _ in range(3):
+            hip.h.hipMemcpyDtoD(dst, src, ctypes.c_size_t(nb))
+        hip.sync()
+        print(json.dumps({"copy_bytes": nb, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
