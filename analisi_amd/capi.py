@@ -46,6 +46,11 @@ SIGNATURES = [
     ("anl_gofrt_calc_single_th", _i, [_vp, _i, _i, _i, _i, _u64, _vp]),
     ("anl_gofrt_calculate_blocks_async", _i, [_vp, _vp, _vp, _u32, _vp, _vp]),
     ("anl_gofrt_last_stats", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
+    ("anl_blocks_create", _i, [_vp, _vp, _vp, _u32, _vp, _i, _u32, _u32, _i, _i, _i, _pp]),
+    ("anl_blocks_run", _i, [_vp, _vp]),
+    ("anl_blocks_gofrt_results", _i, [_vp, _vp, _vp, _vp, ctypes.POINTER(_u32)]),
+    ("anl_blocks_msd_results", _i, [_vp, _vp, _vp, _vp, ctypes.POINTER(_u32)]),
+    ("anl_blocks_destroy", _i, [_vp]),
     ("anl_msd_calculate", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp]),
     ("anl_msd_calculate_async", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
     ("anl_series_create", _i, [_vp, _u64, _u32, _pp]),
@@ -156,6 +161,59 @@ class DeviceGofrt:
     def close(self):
         if self.handle:
             lib().anl_gofrt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBlocks:
+    """anl_blocks: Gofrt (through a DeviceGofrt's parameters) and/or MSD of
+    uneven blocks in one launch each, with the block mean and variance."""
+
+    def __init__(self, traj, primo, n, gofrt=None, msd=False, msd_skip=1, msd_tmax=0, cm_msd=False,
+                 cm_self=False, use_graph=True):
+        self.traj = traj
+        self.primo = np.ascontiguousarray(primo, dtype=np.uint64)
+        self.n = np.ascontiguousarray(n, dtype=np.uint32)
+        self.nb = len(self.primo)
+        self.gofrt = gofrt
+        self.cm_msd = cm_msd
+        h = ctypes.c_void_p()
+        check(lib().anl_blocks_create(traj.handle, _ptr(self.primo), _ptr(self.n), self.nb,
+                                      gofrt.handle if gofrt is not None else None, int(bool(msd)), msd_skip,
+                                      msd_tmax, int(bool(cm_msd)), int(bool(cm_self)), int(bool(use_graph)),
+                                      ctypes.byref(h)))
+        self.handle = h
+
+    def run(self, stream=None):
+        check(lib().anl_blocks_run(self.handle, stream))
+
+    def gofrt_results(self):
+        leff = _u32()
+        check(lib().anl_blocks_gofrt_results(self.handle, None, None, None, ctypes.byref(leff)))
+        g = self.gofrt
+        shape = (leff.value, g.nch, g.nbin)
+        mean, var = np.zeros(shape), np.zeros(shape)
+        counts = np.zeros((self.nb,) + shape, dtype=np.uint64)
+        check(lib().anl_blocks_gofrt_results(self.handle, _ptr(mean), _ptr(var), _ptr(counts), ctypes.byref(leff)))
+        return mean, var, counts
+
+    def msd_results(self):
+        leff = _u32()
+        check(lib().anl_blocks_msd_results(self.handle, None, None, None, ctypes.byref(leff)))
+        shape = (leff.value, 2 if self.cm_msd else 1, self.traj.ntypes)
+        mean, var = np.zeros(shape), np.zeros(shape)
+        per = np.zeros((self.nb,) + shape)
+        check(lib().anl_blocks_msd_results(self.handle, _ptr(mean), _ptr(var), _ptr(per), ctypes.byref(leff)))
+        return mean, var, per
+
+    def close(self):
+        if self.handle:
+            lib().anl_blocks_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

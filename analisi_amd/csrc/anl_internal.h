@@ -98,4 +98,20 @@ struct anl_series {
 namespace anl {
 Cell cell_of(const double *box9);
 void build_frac16(anl_traj *tr, hipStream_t s);
+
+// Prepared launches for the block engine: all host work (checks, plans,
+// uploads, lazy data) happens in *_prepare_*; *_enqueue_prepared only
+// launches kernels (zero fills included), so it can be captured in a graph.
+void *gofrt_prepare_blocks(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nblocks,
+                           uint32_t *leff_max, hipStream_t s);
+void gofrt_enqueue_prepared(anl_gofrt *g, void *prep, unsigned long long *out, hipStream_t s);
+void gofrt_free_prepared(void *prep);
+int gofrt_form_of(void *prep);
+uint32_t gofrt_block_leff(const anl_gofrt *g, uint32_t n);
+uint32_t gofrt_nbin(const anl_gofrt *g);
+uint32_t gofrt_skip(const anl_gofrt *g);
+void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, uint32_t skip,
+                         uint32_t tmax, int cm_msd, int cm_self, uint32_t *leff_out);
+void msd_enqueue_prepared(void *prep, double *out, hipStream_t s);
+void msd_free_prepared(void *prep);
 }  // namespace anl

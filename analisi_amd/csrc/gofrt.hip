@@ -727,9 +727,37 @@ void check_lengths(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint3
     }
 }
 
-void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out_len, int a0, int a1,
-            hipStream_t s, uint32_t lag0 = 0, uint32_t nlag_force = 0) {
+}  // namespace
+
+// A launch whose host-side work (form choice, fractional coordinates, work
+// list upload, arguments) is done: enqueue_prepared() then only fills the
+// output and the counters with kernels and launches the pair kernel, so it
+// can be captured into a HIP graph and replayed.
+struct GofrtPrepared {
+    GofrtArgs a{};
+    dim3 grid;
+    size_t lds = 0;
+    int form = ANL_GOFRT_FP64;
+    uint64_t out_len = 0;
+    bool triclinic = false;
+    int unroll = 4;
+};
+
+__global__ void fill_u64_kernel(unsigned long long *__restrict__ p, uint64_t len) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) p[i] = 0ull;
+}
+
+static void zero_u64(unsigned long long *p, uint64_t len, hipStream_t s) {
+    if (len == 0) return;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, p, len);
+    ANL_HIP(hipGetLastError());
+}
+
+GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int a0, int a1, hipStream_t s,
+                            uint32_t lag0, uint32_t nlag_force) {
     anl_traj *tr = g->tr;
+    GofrtPrepared P;
     const int form = choose_form(g);
     if (form == ANL_GOFRT_SCREENED) build_frac16(tr, s);
     if (L.blocks.size() > g->cap_blocks) {
@@ -744,16 +772,15 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
     }
     ANL_HIP(hipMemcpyAsync(g->d_blocks, L.blocks.data(), L.blocks.size() * sizeof(BlockInfo), hipMemcpyHostToDevice, s));
     ANL_HIP(hipMemcpyAsync(g->d_work, L.work.data(), L.work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
-    ANL_HIP(hipMemsetAsync(out, 0, out_len * sizeof(unsigned long long), s));
-    ANL_HIP(hipMemsetAsync(g->d_ncand, 0, 2 * sizeof(unsigned long long), s));
+    ANL_HIP(hipStreamSynchronize(s));  // the host vectors die with L
 
-    GofrtArgs a{};
+    GofrtArgs &a = P.a;
     a.pos = tr->pos;
     a.frac = tr->frac16;
     a.box = tr->box;
     a.type_slot = tr->type_slot;
     a.orig_slot = tr->orig_slot;
-    a.out = out;
+    a.out = nullptr;
     a.ncand = g->d_ncand;
     a.thr = g->d_thr;
     a.chan = g->d_chan;
@@ -788,7 +815,6 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
         a.cz = (float)(cl.L[2] * sc);
         a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->rmax, g->dr, g->nbin) : 0.0f;
     }
-
     const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
     const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;
     a.lag0 = lag0;
@@ -800,28 +826,70 @@ void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out
     a.nwork = (uint32_t)L.work.size();
     const bool remap = g->tune.xcd && form == ANL_GOFRT_SCREENED;
     a.per_xcd = remap ? (uint32_t)per : 0u;
-    dim3 grid((uint32_t)(remap ? 8 * per : total));
-    size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
-                    (size_t)a.nch * g->nbin * sizeof(unsigned);
-    if (form == ANL_GOFRT_FP64) {
-        size_t lds = common;
-        if (lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
-        if (tr->triclinic) hipLaunchKernelGGL(gofrt_fp64_kernel<true>, grid, dim3(WG), lds, s, a);
-        else hipLaunchKernelGGL(gofrt_fp64_kernel<false>, grid, dim3(WG), lds, s, a);
+    P.grid = dim3((uint32_t)(remap ? 8 * per : total));
+    const size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
+                          (size_t)a.nch * g->nbin * sizeof(unsigned);
+    P.lds = form == ANL_GOFRT_FP64
+                ? common
+                : WG * sizeof(uint4) + (WG / 64) * (SQCAP + RQCAP) * sizeof(uint32_t) + WG * sizeof(int) + common;
+    if (P.lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
+    P.form = form;
+    P.out_len = out_len;
+    P.triclinic = tr->triclinic;
+    P.unroll = g->tune.unroll;
+    g->last_form = form;
+    return P;
+}
+
+void gofrt_enqueue(anl_gofrt *g, const GofrtPrepared &P, unsigned long long *out, hipStream_t s) {
+    GofrtArgs a = P.a;
+    a.out = out;
+    zero_u64(out, P.out_len, s);
+    zero_u64(g->d_ncand, 2, s);
+    if (P.grid.x == 0) return;
+    if (P.form == ANL_GOFRT_FP64) {
+        if (P.triclinic) hipLaunchKernelGGL(gofrt_fp64_kernel<true>, P.grid, dim3(WG), P.lds, s, a);
+        else hipLaunchKernelGGL(gofrt_fp64_kernel<false>, P.grid, dim3(WG), P.lds, s, a);
     } else {
-        size_t lds = WG * sizeof(uint4) + (WG / 64) * (SQCAP + RQCAP) * sizeof(uint32_t) + WG * sizeof(int) + common;
-        if (lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
-        const bool u8 = g->tune.unroll == 8;
-        if (tr->triclinic) {
-            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<true, 8>), grid, dim3(WG), lds, s, a);
-            else hipLaunchKernelGGL((gofrt_screened_kernel<true, 4>), grid, dim3(WG), lds, s, a);
+        const bool u8 = P.unroll == 8;
+        if (P.triclinic) {
+            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<true, 8>), P.grid, dim3(WG), P.lds, s, a);
+            else hipLaunchKernelGGL((gofrt_screened_kernel<true, 4>), P.grid, dim3(WG), P.lds, s, a);
         } else {
-            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<false, 8>), grid, dim3(WG), lds, s, a);
-            else hipLaunchKernelGGL((gofrt_screened_kernel<false, 4>), grid, dim3(WG), lds, s, a);
+            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<false, 8>), P.grid, dim3(WG), P.lds, s, a);
+            else hipLaunchKernelGGL((gofrt_screened_kernel<false, 4>), P.grid, dim3(WG), P.lds, s, a);
         }
     }
     ANL_HIP(hipGetLastError());
-    g->last_form = form;
+}
+
+// Internal API of the block engine (blocks.hip), declared in anl_internal.h.
+namespace anl {
+void *gofrt_prepare_blocks(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nblocks,
+                           uint32_t *leff_max, hipStream_t s) {
+    check_lengths(g, primo, n, nblocks);
+    Launch L = plan(g, primo, n, nblocks, -1);
+    const uint64_t len = (uint64_t)nblocks * L.leff_max * nch_of(g->tr) * g->nbin;
+    auto *P = new GofrtPrepared(gofrt_prepare(g, L, len, INT32_MIN, INT32_MAX, s, 0, 0));
+    if (leff_max) *leff_max = L.leff_max;
+    return P;
+}
+void gofrt_enqueue_prepared(anl_gofrt *g, void *prep, unsigned long long *out, hipStream_t s) {
+    gofrt_enqueue(g, *(GofrtPrepared *)prep, out, s);
+}
+void gofrt_free_prepared(void *prep) { delete (GofrtPrepared *)prep; }
+int gofrt_form_of(void *prep) { return ((GofrtPrepared *)prep)->form; }
+uint32_t gofrt_block_leff(const anl_gofrt *g, uint32_t n) { return (n < g->lmax || g->lmax == 0) ? n : g->lmax; }
+uint32_t gofrt_nbin(const anl_gofrt *g) { return g->nbin; }
+uint32_t gofrt_skip(const anl_gofrt *g) { return g->skip; }
+}  // namespace anl
+
+namespace {
+
+void launch(anl_gofrt *g, const Launch &L, unsigned long long *out, uint64_t out_len, int a0, int a1,
+            hipStream_t s, uint32_t lag0 = 0, uint32_t nlag_force = 0) {
+    GofrtPrepared P = gofrt_prepare(g, L, out_len, a0, a1, s, lag0, nlag_force);
+    gofrt_enqueue(g, P, out, s);
 }
 
 }  // namespace

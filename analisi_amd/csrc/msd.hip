@@ -10,6 +10,7 @@
 // partials in a fixed order and divides by the term count -- deterministic,
 // and within a few ulps of the reference's running mean (no cancellation).
 // The centre-of-mass channel replays the reference's running mean exactly.
+// Several blocks (primo_b, n_b) run in one launch (grid z = block).
 #include <algorithm>
 
 #include "anl_internal.h"
@@ -21,13 +22,22 @@ namespace anl {
 constexpr int MSD_WG = 256;
 constexpr int MSD_TL = 8;  // lags per thread
 
+struct MsdBlock {
+    uint64_t primo;
+    uint32_t n;
+    uint32_t pad;
+};
+
 template <bool CMSELF>
 __global__ __launch_bounds__(MSD_WG) void msd_atoms_kernel(const double4 *__restrict__ pos, const double *__restrict__ cm,
                                                           const int32_t *__restrict__ type_slot, uint64_t N, uint32_t nt,
-                                                          uint64_t primo, uint32_t n, uint32_t skip, uint32_t leff,
-                                                          uint32_t nchunks, double *__restrict__ partial) {
+                                                          const MsdBlock *__restrict__ blocks, uint32_t skip,
+                                                          uint32_t leff, uint32_t nchunks, double *__restrict__ partial) {
     __shared__ double acc_lds[MSD_TL][MSD_WG];
     __shared__ int type_lds[MSD_WG];
+    const MsdBlock blk = blocks[blockIdx.z];
+    const uint64_t primo = blk.primo;
+    const uint32_t n = blk.n;
     const uint64_t slot = (uint64_t)blockIdx.x * MSD_WG + threadIdx.x;
     const uint32_t tb = blockIdx.y * MSD_TL;
     double acc[MSD_TL];
@@ -76,37 +86,134 @@ __global__ __launch_bounds__(MSD_WG) void msd_atoms_kernel(const double4 *__rest
         double s = 0.0;
         for (int k = 0; k < MSD_WG; k++)
             if (type_lds[k] == (int)tt) s += acc_lds[l][k];
-        partial[((uint64_t)t * nchunks + blockIdx.x) * nt + tt] = s;
+        partial[(((uint64_t)blockIdx.z * leff + t) * nchunks + blockIdx.x) * nt + tt] = s;
     }
 }
 
 __global__ void msd_finish_kernel(const double *__restrict__ partial, uint32_t nchunks, uint32_t nt, uint32_t leff,
-                                  const uint64_t *__restrict__ count, uint32_t fcm, double *__restrict__ out) {
+                                  uint32_t nblocks, const uint64_t *__restrict__ count, uint32_t fcm,
+                                  double *__restrict__ out) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= leff * nt) return;
-    const uint32_t t = w / nt, tt = w % nt;
+    if (w >= nblocks * leff * nt) return;
+    const uint32_t b = w / (leff * nt), r = w % (leff * nt);
+    const uint32_t t = r / nt, tt = r % nt;
     double s = 0.0;
-    for (uint32_t c = 0; c < nchunks; c++) s += partial[((uint64_t)t * nchunks + c) * nt + tt];
-    out[(uint64_t)t * nt * fcm + tt] = count[tt] ? s / (double)count[tt] : 0.0;
+    for (uint32_t c = 0; c < nchunks; c++) s += partial[(((uint64_t)b * leff + t) * nchunks + c) * nt + tt];
+    const uint64_t cn = count[(uint64_t)b * nt + tt];
+    out[(uint64_t)b * leff * nt * fcm + (uint64_t)t * nt * fcm + tt] = cn ? s / (double)cn : 0.0;
 }
 
 // msd.cpp:113-123: |cm(t0) - cm(t0+t)|^2 running mean over t0, per type.
-__global__ void msd_cm_kernel(const double *__restrict__ cm, uint32_t nt, uint64_t primo, uint32_t n, uint32_t skip,
-                              uint32_t leff, double *__restrict__ out) {
+__global__ void msd_cm_kernel(const double *__restrict__ cm, uint32_t nt, const MsdBlock *__restrict__ blocks,
+                              uint32_t skip, uint32_t leff, uint32_t nblocks, double *__restrict__ out) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= leff * nt) return;
-    const uint32_t t = w / nt, tt = w % nt;
+    if (w >= nblocks * leff * nt) return;
+    const uint32_t b = w / (leff * nt), r = w % (leff * nt);
+    const uint32_t t = r / nt, tt = r % nt;
+    const MsdBlock blk = blocks[b];
     double m = 0.0;
     uint64_t cont = 0;
-    for (uint32_t t0 = 0; t0 < n; t0 += skip) {
-        const double *a = cm + ((primo + t0) * nt + tt) * 3;
-        const double *b = cm + ((primo + t0 + t) * nt + tt) * 3;
-        double e0 = a[0] - b[0], e1 = a[1] - b[1], e2 = a[2] - b[2];
+    for (uint32_t t0 = 0; t0 < blk.n; t0 += skip) {
+        const double *a = cm + ((blk.primo + t0) * nt + tt) * 3;
+        const double *c = cm + ((blk.primo + t0 + t) * nt + tt) * 3;
+        double e0 = a[0] - c[0], e1 = a[1] - c[1], e2 = a[2] - c[2];
         double delta = ((e0 * e0 + e1 * e1) + e2 * e2) - m;
         m = m + delta / (double)(++cont);
     }
-    out[(uint64_t)t * nt * 2 + nt + tt] = m;
+    out[(uint64_t)b * leff * nt * 2 + (uint64_t)t * nt * 2 + nt + tt] = m;
 }
+
+struct MsdPrepared {
+    anl_traj *tr = nullptr;
+    uint32_t skip = 1, leff = 0, nblocks = 0, nt = 0, fcm = 1, nchunks = 0;
+    bool cm_msd = false, cm_self = false;
+    MsdBlock *d_blocks = nullptr;
+    uint64_t *d_count = nullptr;
+    double *d_partial = nullptr;
+    ~MsdPrepared() {
+        (void)hipFree(d_blocks);
+        (void)hipFree(d_count);
+        (void)hipFree(d_partial);
+    }
+};
+
+// msd.cpp:52-61 checks (leff is common to all blocks: min(n_b, tmax) must agree)
+void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, uint32_t skip,
+                         uint32_t tmax, int cm_msd, int cm_self, uint32_t *leff_out) {
+    if (nblocks == 0) fail(ANL_ERR_ARG, "no blocks");
+    if (skip == 0) skip = 1;
+    const uint32_t leff = (n[0] < tmax || tmax == 0) ? n[0] : tmax;  // msd.cpp:42
+    for (uint32_t b = 0; b < nblocks; b++) {
+        const uint32_t lb = (n[b] < tmax || tmax == 0) ? n[b] : tmax;
+        if (lb != leff) fail(ANL_ERR_ARG, "blocks with different numbers of lags cannot be averaged together");
+        if ((uint64_t)leff + n[b] + primo[b] > tr->F)
+            fail(ANL_ERR_TOO_SHORT,
+                 "trajectory is too short for this kind of calculation. Select a different starting timestep or "
+                 "lower the size of the average or the lenght of the time lag");
+        if ((uint64_t)leff + n[b] > tr->F)
+            fail(ANL_ERR_NOT_LOADED, "there are not enough loaded timesteps inside the trajectory object. I need at "
+                                     "least " + std::to_string((uint64_t)leff + n[b]) +
+                                         " timesteps to do the requested calculation");
+    }
+    if ((cm_msd || cm_self) && !tr->cm) fail(ANL_ERR_ARG, "centre-of-mass series not provided");
+    ensure_device();
+    auto *P = new MsdPrepared();
+    try {
+        P->tr = tr;
+        P->skip = skip;
+        P->leff = leff;
+        P->nblocks = nblocks;
+        P->nt = tr->nt;
+        P->fcm = cm_msd ? 2 : 1;
+        P->cm_msd = cm_msd;
+        P->cm_self = cm_self;
+        P->nchunks = (uint32_t)((tr->N + MSD_WG - 1) / MSD_WG);
+        std::vector<MsdBlock> hb(nblocks);
+        std::vector<uint64_t> hc((size_t)nblocks * tr->nt);
+        for (uint32_t b = 0; b < nblocks; b++) {
+            hb[b] = MsdBlock{primo[b], n[b], 0};
+            const uint64_t n0 = (n[b] + skip - 1) / skip;
+            for (uint32_t k = 0; k < tr->nt; k++) hc[(size_t)b * tr->nt + k] = n0 * tr->type_count[k];
+        }
+        ANL_HIP(hipMalloc(&P->d_blocks, nblocks * sizeof(MsdBlock)));
+        ANL_HIP(hipMemcpy(P->d_blocks, hb.data(), nblocks * sizeof(MsdBlock), hipMemcpyHostToDevice));
+        ANL_HIP(hipMalloc(&P->d_count, hc.size() * sizeof(uint64_t)));
+        ANL_HIP(hipMemcpy(P->d_count, hc.data(), hc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+        ANL_HIP(hipMalloc(&P->d_partial, std::max<size_t>(1, (size_t)nblocks * leff * P->nchunks * P->nt) * sizeof(double)));
+    } catch (...) {
+        delete P;
+        throw;
+    }
+    if (leff_out) *leff_out = leff;
+    return P;
+}
+
+// out: [nblocks][leff][fcm][nt]
+void msd_enqueue_prepared(void *prep, double *out, hipStream_t s) {
+    const MsdPrepared &P = *(const MsdPrepared *)prep;
+    if (P.leff == 0) return;
+    const anl_traj *tr = P.tr;
+    const uint32_t nlb = (P.leff + MSD_TL - 1) / MSD_TL;
+    dim3 grid(P.nchunks, nlb, P.nblocks);
+    if (P.cm_self)
+        hipLaunchKernelGGL(msd_atoms_kernel<true>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N,
+                           P.nt, P.d_blocks, P.skip, P.leff, P.nchunks, P.d_partial);
+    else
+        hipLaunchKernelGGL(msd_atoms_kernel<false>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N,
+                           P.nt, P.d_blocks, P.skip, P.leff, P.nchunks, P.d_partial);
+    ANL_HIP(hipGetLastError());
+    const uint32_t nw = P.nblocks * P.leff * P.nt;
+    hipLaunchKernelGGL(msd_finish_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, P.d_partial, P.nchunks, P.nt,
+                       P.leff, P.nblocks, P.d_count, P.fcm, out);
+    ANL_HIP(hipGetLastError());
+    if (P.cm_msd) {
+        hipLaunchKernelGGL(msd_cm_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, tr->cm, P.nt, P.d_blocks, P.skip,
+                           P.leff, P.nblocks, out);
+        ANL_HIP(hipGetLastError());
+    }
+}
+
+void msd_free_prepared(void *prep) { delete (MsdPrepared *)prep; }
 
 }  // namespace anl
 
@@ -118,51 +225,15 @@ int anl_msd_calculate_async(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_m
                             uint64_t primo, double *out_dev, void *stream) {
     return guard([&] {
         if (!tr || !out_dev) fail(ANL_ERR_ARG, "null pointer passed to anl_msd_calculate_async");
-        if (skip == 0) skip = 1;
-        const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;  // msd.cpp:42
-        // msd.cpp:54-60
-        if ((uint64_t)leff + n + primo > tr->F)
-            fail(ANL_ERR_TOO_SHORT,
-                 "trajectory is too short for this kind of calculation. Select a different starting timestep or "
-                 "lower the size of the average or the lenght of the time lag");
-        if ((uint64_t)leff + n > tr->F)
-            fail(ANL_ERR_NOT_LOADED, "there are not enough loaded timesteps inside the trajectory object. I need at "
-                                     "least " + std::to_string((uint64_t)leff + n) +
-                                         " timesteps to do the requested calculation");
-        if ((cm_msd || cm_self) && !tr->cm) fail(ANL_ERR_ARG, "centre-of-mass series not provided");
-        if (leff == 0) return;
-        ensure_device();
-        hipStream_t s = (hipStream_t)stream;
-        const uint32_t nt = tr->nt, fcm = cm_msd ? 2 : 1;
-        const uint32_t nchunks = (uint32_t)((tr->N + MSD_WG - 1) / MSD_WG);
-        const uint32_t nlb = (leff + MSD_TL - 1) / MSD_TL;
-        double *partial = nullptr;
-        uint64_t *count = nullptr;
-        ANL_HIP(hipMallocAsync((void **)&partial, (size_t)leff * nchunks * nt * sizeof(double), s));
-        ANL_HIP(hipMallocAsync((void **)&count, nt * sizeof(uint64_t), s));
-        const uint64_t n0 = (n + skip - 1) / skip;
-        std::vector<uint64_t> hc(nt);
-        for (uint32_t k = 0; k < nt; k++) hc[k] = n0 * tr->type_count[k];
-        ANL_HIP(hipMemcpyAsync(count, hc.data(), nt * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-        dim3 grid(nchunks, nlb);
-        if (cm_self)
-            hipLaunchKernelGGL(msd_atoms_kernel<true>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N,
-                               nt, primo, n, skip, leff, nchunks, partial);
-        else
-            hipLaunchKernelGGL(msd_atoms_kernel<false>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot,
-                               tr->N, nt, primo, n, skip, leff, nchunks, partial);
-        ANL_HIP(hipGetLastError());
-        const uint32_t nw = leff * nt;
-        hipLaunchKernelGGL(msd_finish_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, partial, nchunks, nt, leff,
-                           count, fcm, out_dev);
-        ANL_HIP(hipGetLastError());
-        if (cm_msd) {
-            hipLaunchKernelGGL(msd_cm_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, tr->cm, nt, primo, n, skip, leff,
-                               out_dev);
-            ANL_HIP(hipGetLastError());
+        void *P = msd_prepare_blocks(tr, &primo, &n, 1, skip, tmax, cm_msd, cm_self, nullptr);
+        try {
+            msd_enqueue_prepared(P, out_dev, (hipStream_t)stream);
+            ANL_HIP(hipStreamSynchronize((hipStream_t)stream));  // the prepared buffers are freed below
+        } catch (...) {
+            msd_free_prepared(P);
+            throw;
         }
-        ANL_HIP(hipFreeAsync(partial, s));
-        ANL_HIP(hipFreeAsync(count, s));
+        msd_free_prepared(P);
     });
 }
 

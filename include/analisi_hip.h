@@ -101,6 +101,30 @@ int anl_gofrt_calculate_blocks_async(anl_gofrt *g, const uint64_t *primo, const 
  * Diagnostics of this implementation; no counterpart in the reference. */
 int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uint64_t *rechecks);
 
+/* ---- block engine ------------------------------------------------------ */
+/* Replaces the per-block loop of BlockAverageG::calcola_custom with MediaVar
+ * (blockaverage.h:107-203, calcoliblocchi.h:21-65) for Gofrt and MSD, with
+ * blocks of uneven lengths: block b starts at frame primo[b] and averages
+ * n[b] t0 values (reset(n[b]); every block must give the same number of
+ * lags).  g(r,t) of all blocks runs in one launch of the pair kernel with the
+ * parameters and form of `gofrt` (NULL: no g(r,t)); MSD (msd != 0) of all
+ * blocks in one launch; then mean and variance of the mean over blocks on
+ * the device, with the reference's Welford order.  use_graph != 0 captures
+ * the whole sequence once into a HIP graph, replayed by every anl_blocks_run.
+ * No counterpart in the reference beyond the loop it replaces. */
+typedef struct anl_blocks anl_blocks;
+int anl_blocks_create(anl_traj *tr, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, anl_gofrt *gofrt,
+                      int msd, uint32_t msd_skip, uint32_t msd_tmax, int cm_msd, int cm_self, int use_graph,
+                      anl_blocks **out);
+/* Enqueue one evaluation of every block on stream (NULL = default). */
+int anl_blocks_run(anl_blocks *b, void *stream);
+/* Synchronise and copy results: mean/var [leff][nt(nt+1)][nbin] FP64 (the
+ * reference's vdata layout), counts [nblocks][leff][nt(nt+1)][nbin] uint64
+ * (may be NULL); MSD mean/var [leff][f_cm][nt], per_block [nblocks][...]. */
+int anl_blocks_gofrt_results(anl_blocks *b, double *mean, double *var, uint64_t *counts, uint32_t *leff);
+int anl_blocks_msd_results(anl_blocks *b, double *mean, double *var, double *per_block, uint32_t *leff);
+int anl_blocks_destroy(anl_blocks *b);
+
 /* ---- mean square displacement ----------------------------------------- */
 /* Replaces MSD<T> (msd.h:30-47, msd.cpp:40-146): out_host [leff][f_cm][ntypes]
  * FP64, f_cm = 2 when cm_msd, per-type mean over (t0 in 0,skip,..<n, atoms);
