@@ -56,6 +56,7 @@ SIGNATURES = [
     ("anl_series_create", _i, [_vp, _u64, _u32, _pp]),
     ("anl_series_destroy", _i, [_vp]),
     ("anl_gk_calculate", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
+    ("anl_gk_calculate_mfma", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
     ("anl_mediavar_update_async", _i, [_vp, _vp, _vp, _u64, _u32, _vp]),
     ("anl_mediavar_end_async", _i, [_vp, _u64, _u32, _vp]),
     ("anl_counts_to_double_async", _i, [_vp, _d, _u64, _vp, _vp]),
@@ -238,13 +239,16 @@ class DeviceSeries:
         check(lib().anl_series_create(_ptr(self.data), self.data.shape[0], self.data.shape[1], ctypes.byref(h)))
         self.handle = h
 
-    def gk(self, idx_j, skip, tmax, n, n_seg, subtract_mean, start_mean, primo):
+    def gk(self, idx_j, skip, tmax, n, n_seg, subtract_mean, start_mean, primo, mfma=False):
+        """GreenKubo reset(n) + calculate(primo); mfma selects the split-BF16
+        matrix-core form of the lag products."""
         idx = np.ascontiguousarray(idx_j, dtype=np.uint32)
         ncur = len(idx)
         leff = n if (n < tmax or tmax == 0) else tmax
         out = np.zeros((leff, 3 * ncur * ncur + 2), dtype=np.float64)
-        check(lib().anl_gk_calculate(self.handle, _ptr(idx), ncur, skip, tmax, n, n_seg, int(bool(subtract_mean)),
-                                     start_mean, primo, _ptr(out)))
+        fn = lib().anl_gk_calculate_mfma if mfma else lib().anl_gk_calculate
+        check(fn(self.handle, _ptr(idx), ncur, skip, tmax, n, n_seg, int(bool(subtract_mean)), start_mean, primo,
+                 _ptr(out)))
         return out
 
     def close(self):

@@ -114,4 +114,8 @@ void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n,
                          uint32_t tmax, int cm_msd, int cm_self, uint32_t *leff_out);
 void msd_enqueue_prepared(void *prep, double *out, hipStream_t s);
 void msd_free_prepared(void *prep);
+// FP64 Green-Kubo epilogue (mean subtraction, running integrals, Schur
+// complement) over d[leff][narr] whose first ncur^2 columns hold C(l).
+void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
+                 hipStream_t s);
 }  // namespace anl

@@ -197,6 +197,19 @@ __global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_
     for (uint32_t j = 0; j < ncorr; j++) cur[ncorr + 2 * j + 1] = cur[ncorr + 2 * j + 1] / dit;
 }
 
+void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
+                 hipStream_t s) {
+    const uint32_t ncorr = ncur * ncur;
+    if (subtract_mean) {
+        hipLaunchKernelGGL(gk_mean_kernel, dim3(1), dim3(64), 0, s, d, narr, ncorr, leff, start_mean);
+        ANL_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(gk_integral_kernel, dim3(1), dim3(64), 0, s, d, narr, ncorr, leff);
+    ANL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256), dim3(256), 0, s, d, narr, ncur, leff);
+    ANL_HIP(hipGetLastError());
+}
+
 }  // namespace anl
 
 using namespace anl;
@@ -243,14 +256,7 @@ int anl_gk_calculate(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32
                 default: hipLaunchKernelGGL(gk_lag_kernel<4>, grid, dim3(256), 0, 0, a); break;
             }
             ANL_HIP(hipGetLastError());
-            if (subtract_mean) {
-                hipLaunchKernelGGL(gk_mean_kernel, dim3(1), dim3(64), 0, 0, d, narr, ncorr, leff, start_mean);
-                ANL_HIP(hipGetLastError());
-            }
-            hipLaunchKernelGGL(gk_integral_kernel, dim3(1), dim3(64), 0, 0, d, narr, ncorr, leff);
-            ANL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(gk_schur_kernel, grid, dim3(256), 0, 0, d, narr, ncur, leff);
-            ANL_HIP(hipGetLastError());
+            gk_epilogue(d, narr, ncur, leff, subtract_mean, start_mean, 0);
             ANL_HIP(hipMemcpy(out_host, d, (size_t)leff * narr * sizeof(double), hipMemcpyDeviceToHost));
         } catch (...) {
             (void)hipFree(d);

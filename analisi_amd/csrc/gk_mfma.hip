@@ -1,0 +1,275 @@
+// Green-Kubo lag correlations on the matrix cores (greenkuboNcomponentionicfluid.cpp:
+// 156-249): split-BF16 products on MFMA, FP32 accumulation within a K chunk,
+// FP64 across chunks, then the FP64 epilogue of gk.hip.
+//
+// For currents (a, b) and lag l the reference averages J_a(t0).J_b(t0 + l)/3
+// over the selected t0.  With the series cut into rows of 32 steps,
+//     S_q[r][s] = sum_{p, x} Xa_x(32p + r) * Yb_x(32(p + q) + s)
+// is a plain 32 x 32 x K GEMM (K = p), and every (t0, l) product lands in
+// exactly one tile: t0 = 32p + r, l = 32q + s - r.  Lag l is the sum of
+// diagonal s - r = l - 32q of tiles q = floor(l/32) and floor(l/32) + 1.
+// Xa carries the t0 selection of one segment (:207-209); Yb is unmasked.
+// Each FP64 value v is split into hi = bf16(v), lo = bf16(v - hi); a product
+// is hi*hi' + hi*lo' + lo*hi' (relative error ~2^-16 per product, which the
+// t0 average reduces further).
+//
+// Layouts in HBM (bf16 bit patterns, u16):
+//   X[cx][r][p]          = X_cx(32p + r),          p < PP        (cx = 3c + x)
+//   Y[c8][cx][s][p']     = Y_cx(32(p' + c8) + s),  p' < YP, c8 < 8
+// so that the 8 k-values one lane feeds to an MFMA are one aligned 16-byte
+// load for every tile offset q (the eight c8 copies absorb q mod 8).
+// A wavefront owns tiles q0 + 16j (j < GKM_Q) of one current pair and one K
+// chunk: tile q0 + 16j at step st reads the same Y vector as tile q0 at step
+// st + j, so Y operands live in a rolling register window and each step loads
+// one new Y vector and one X vector for GKM_Q tiles.
+#include <algorithm>
+#include <vector>
+
+#include "anl_internal.h"
+
+namespace anl {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t GKM_Q = 8;    // tiles per wavefront, 16 apart
+constexpr uint32_t GKM_CS = 16;  // MFMA steps (16 rows of 32 t0 each) per FP32 chunk
+
+__device__ __forceinline__ unsigned short bf16_rn(float f) {
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+
+// hi/lo split of one double: hi = RN_bf16((float)v), lo = RN_bf16((float)(v - hi)).
+__device__ __forceinline__ void split_bf16(double v, unsigned short &h, unsigned short &l) {
+    h = bf16_rn((float)v);
+    l = bf16_rn((float)(v - (double)bf16_to_f(h)));
+}
+
+// X side, one segment: element (cx, r, p) = data[primo + t][idx[c] + x] at
+// t = 32p + r when t is a selected t0 of the segment, else 0.
+__global__ void gkm_split_x_kernel(const double *__restrict__ data, uint32_t ncols, const uint32_t *__restrict__ idx,
+                                   uint64_t primo, uint32_t PP, uint32_t seg_lo, uint32_t seg_hi, uint32_t first,
+                                   uint32_t skip, unsigned short *__restrict__ xh, unsigned short *__restrict__ xl) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // e = r * PP + p within one cx
+    if (e >= 32ull * PP) return;
+    const uint32_t cx = blockIdx.y, c = cx / 3, x = cx % 3;
+    const uint32_t r = (uint32_t)(e / PP), p = (uint32_t)(e % PP);
+    const uint64_t t = 32ull * p + r;
+    double v = 0.0;
+    if (t >= seg_lo && t < seg_hi && t >= first && (t - first) % skip == 0) v = data[(primo + t) * ncols + idx[c] + x];
+    unsigned short h, l;
+    split_bf16(v, h, l);
+    const uint64_t o = (uint64_t)cx * 32 * PP + e;
+    xh[o] = h;
+    xl[o] = l;
+}
+
+// Y side: element (c8, cx, s, p') = data[primo + t][idx[c] + x] at
+// t = 32(p' + c8) + s when t < avail, else 0.
+__global__ void gkm_split_y_kernel(const double *__restrict__ data, uint32_t ncols, const uint32_t *__restrict__ idx,
+                                   uint64_t primo, uint64_t avail, uint32_t YP, uint32_t ncx,
+                                   unsigned short *__restrict__ yh, unsigned short *__restrict__ yl) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // s * YP + p'
+    if (e >= 32ull * YP) return;
+    const uint32_t cx = blockIdx.y % ncx, c8 = blockIdx.y / ncx, c = cx / 3, x = cx % 3;
+    const uint32_t s = (uint32_t)(e / YP), pp = (uint32_t)(e % YP);
+    const uint64_t t = 32ull * ((uint64_t)pp + c8) + s;
+    double v = 0.0;
+    if (t < avail) v = data[(primo + t) * ncols + idx[c] + x];
+    unsigned short h, l;
+    split_bf16(v, h, l);
+    const uint64_t o = ((uint64_t)c8 * ncx + cx) * 32 * YP + e;
+    yh[o] = h;
+    yl[o] = l;
+}
+
+struct GkmArgs {
+    const unsigned short *xh, *xl, *yh, *yl;
+    uint32_t ncur, PP, YP, ntiles, nchunks;
+    double *partial;  // [pair][ntiles][nchunks][63]
+};
+
+__device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
+}
+
+// Operand layouts of v_mfma_f32_32x32x16_bf16: lane l feeds row l % 32 of A
+// and column l % 32 of B, each with the 8 k-values of slot l / 32 (the same k
+// slots in A and B); result item e of lane l is row 8(e/4) + 4(l/32) + e%4,
+// column l % 32.
+__global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
+    __shared__ double tile[32][33];
+    const uint32_t g = blockIdx.x, pair = blockIdx.y, chunk = blockIdx.z;
+    const uint32_t qbase = (g / 16) * 16 * GKM_Q, q0 = g % 16;
+    const uint32_t ca = pair / a.ncur, cb = pair % a.ncur, ncx = 3 * a.ncur;
+    const uint32_t lane = threadIdx.x, r = lane & 31u, h = lane >> 5;
+    const uint32_t c8 = q0 & 7u, qoff = qbase + (q0 & ~7u);
+    const uint32_t st0 = chunk * GKM_CS;
+    f32x16 acc[GKM_Q];
+#pragma unroll
+    for (uint32_t j = 0; j < GKM_Q; j++) acc[j] = f32x16{};
+    for (uint32_t x = 0; x < 3; x++) {
+        const uint64_t xo = ((uint64_t)(3 * ca + x) * 32 + r) * a.PP + 8 * h;
+        const uint64_t yo = (((uint64_t)c8 * ncx + 3 * cb + x) * 32 + r) * a.YP + qoff + 8 * h;
+        const unsigned short *XH = a.xh + xo, *XL = a.xl + xo, *YH = a.yh + yo, *YL = a.yl + yo;
+        bf16x8 wh[GKM_Q], wl[GKM_Q];
+#pragma unroll
+        for (uint32_t j = 0; j + 1 < GKM_Q; j++) {
+            wh[j] = ld8(YH + 16 * (st0 + j));
+            wl[j] = ld8(YL + 16 * (st0 + j));
+        }
+        for (uint32_t st = st0; st < st0 + GKM_CS; st += GKM_Q) {
+#pragma unroll
+            for (uint32_t u = 0; u < GKM_Q; u++) {
+                const uint32_t sn = st + u;
+                wh[(u + GKM_Q - 1) % GKM_Q] = ld8(YH + 16 * (sn + GKM_Q - 1));
+                wl[(u + GKM_Q - 1) % GKM_Q] = ld8(YL + 16 * (sn + GKM_Q - 1));
+                const bf16x8 ah = ld8(XH + 16 * sn), al = ld8(XL + 16 * sn);
+#pragma unroll
+                for (uint32_t j = 0; j < GKM_Q; j++) {
+                    const uint32_t w = (u + j) % GKM_Q;
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wh[w], acc[j], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wl[w], acc[j], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wh[w], acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // diagonal sums of each tile, in FP64, into this chunk's partial
+#pragma unroll
+    for (uint32_t j = 0; j < GKM_Q; j++) {
+        const uint32_t q = qbase + q0 + 16 * j;
+        if (q >= a.ntiles) break;  // uniform across the wavefront
+#pragma unroll
+        for (int e = 0; e < 16; e++) tile[8 * (e / 4) + 4 * h + (e % 4)][r] = (double)acc[j][e];
+        __syncthreads();
+        if (lane < 63) {
+            const int d = (int)lane - 31;  // s - r
+            double sum = 0.0;
+            const int r0 = d < 0 ? -d : 0, r1 = d < 0 ? 32 : 32 - d;
+            for (int rr = r0; rr < r1; rr++) sum += tile[rr][rr + d];
+            a.partial[(((uint64_t)pair * a.ntiles + q) * a.nchunks + chunk) * 63 + lane] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// Segment mean of product/3 for lag l and pair (diagonal l - 32q of tiles
+// q = floor(l/32), floor(l/32) + 1; chunks in order), folded into the running
+// mean over segments (:230-237).
+__global__ void gkm_reduce_kernel(const double *__restrict__ partial, uint32_t ntiles, uint32_t nchunks,
+                                  uint32_t npairs, uint32_t leff, uint32_t narr, double cont, uint32_t iseg,
+                                  double *__restrict__ out) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= leff * npairs) return;
+    const uint32_t l = w / npairs, pair = w % npairs;
+    double s = 0.0;
+    for (uint32_t q = l / 32; q <= l / 32 + 1 && q < ntiles; q++) {
+        const int d = (int)l - 32 * (int)q;
+        if (d < -31 || d > 31) continue;
+        const double *p = partial + ((uint64_t)pair * ntiles + q) * nchunks * 63 + (d + 31);
+        for (uint32_t ch = 0; ch < nchunks; ch++) s += p[(uint64_t)ch * 63];
+    }
+    const double jj = cont > 0 ? (s / 3.0) / cont : 0.0;
+    double *v = out + (uint64_t)l * narr + pair;
+    *v = *v + (jj - *v) / (double)(iseg + 1);
+}
+
+__global__ void gkm_zero_kernel(double *__restrict__ p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+}  // namespace anl
+
+using namespace anl;
+
+extern "C" {
+
+int anl_gk_calculate_mfma(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                          uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
+                          double *out_host) {
+    return guard([&] {
+        if (!sr || !idx_j || !out_host) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate_mfma");
+        if (ncur == 0 || ncur > 4) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
+        for (uint32_t c = 0; c < ncur; c++)
+            if (idx_j[c] + 3 > sr->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
+        if (n_seg < 1) n_seg = 1;  // greenkubo...cpp:50-53
+        if (skip == 0) skip = 1;
+        const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;
+        if ((uint64_t)leff + primo + n > sr->nrows)
+            fail(ANL_ERR_TOO_SHORT,
+                 "trajectory is too short for this kind of calculation. Select a different starting timestep or "
+                 "lower the size of the average or the lenght of the correlation function");
+        const uint32_t ncorr = ncur * ncur, narr = 3 * ncorr + 2, ncx = 3 * ncur;
+        if (leff == 0) return;
+        ensure_device();
+        // t0 axis: rows of 32, MFMA steps of 16 rows, chunks of GKM_CS steps
+        const uint64_t nsteps = ((uint64_t)n + 511) / 512;
+        const uint64_t nchunks = (nsteps + GKM_CS - 1) / GKM_CS;
+        const uint64_t PP = nchunks * GKM_CS * 16;
+        // lag axis: tiles of 32 (plus one for the negative diagonals)
+        const uint64_t ntiles = (leff + 30) / 32 + 1;
+        const uint64_t ngroups = (ntiles + 16 * GKM_Q - 1) / (16 * GKM_Q) * 16;
+        const uint64_t qmax = ngroups / 16 * 16 * GKM_Q;  // tiles covered, >= ntiles
+        // highest Y row read: 16 (st + Q - 1) + 8 + 7 + qoff, st < nchunks * CS, qoff < qmax
+        const uint64_t YP = (PP + 16 * GKM_Q + qmax + 16 + 7) / 8 * 8;
+        if (32 * YP * 8 > (1ull << 31) || 32 * PP > (1ull << 31) || nchunks > 65535)
+            fail(ANL_ERR_ARG, "series too long for the MFMA form (use the FP64 form)");
+        const uint64_t avail = sr->nrows - primo;
+
+        std::vector<void *> bufs;
+        auto dmalloc = [&](size_t bytes) {
+            void *p = nullptr;
+            ANL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+            bufs.push_back(p);
+            return p;
+        };
+        auto grid1 = [](uint64_t n_, uint32_t y) { return dim3((unsigned)((n_ + 255) / 256), y); };
+        try {
+            uint32_t *d_idx = (uint32_t *)dmalloc(ncur * sizeof(uint32_t));
+            ANL_HIP(hipMemcpy(d_idx, idx_j, ncur * sizeof(uint32_t), hipMemcpyHostToDevice));
+            const size_t xn = (size_t)ncx * 32 * PP, yn = (size_t)8 * ncx * 32 * YP;
+            auto *xh = (unsigned short *)dmalloc(xn * 2);
+            auto *xl = (unsigned short *)dmalloc(xn * 2);
+            auto *yh = (unsigned short *)dmalloc(yn * 2);
+            auto *yl = (unsigned short *)dmalloc(yn * 2);
+            auto *partial = (double *)dmalloc((size_t)ncorr * ntiles * nchunks * 63 * sizeof(double));
+            const uint64_t outn = (uint64_t)leff * narr;
+            auto *d = (double *)dmalloc(outn * sizeof(double));
+            hipLaunchKernelGGL(gkm_zero_kernel, grid1(outn, 1), dim3(256), 0, 0, d, outn);
+            ANL_HIP(hipGetLastError());
+            hipLaunchKernelGGL(gkm_split_y_kernel, grid1(32 * YP, 8 * ncx), dim3(256), 0, 0, sr->data, sr->ncols,
+                               d_idx, primo, avail, (uint32_t)YP, ncx, yh, yl);
+            ANL_HIP(hipGetLastError());
+            GkmArgs ka{xh, xl, yh, yl, ncur, (uint32_t)PP, (uint32_t)YP, (uint32_t)ntiles, (uint32_t)nchunks, partial};
+            const uint32_t seg = n / n_seg;
+            for (uint32_t iseg = 0; iseg < n_seg; iseg++) {
+                const uint32_t lo = seg * iseg, hi = seg * (iseg + 1);
+                const uint32_t first = lo + lo % skip;  // greenkubo...cpp:208
+                const uint32_t cont = first < hi ? (hi - first + skip - 1) / skip : 0;
+                hipLaunchKernelGGL(gkm_split_x_kernel, grid1(32 * PP, ncx), dim3(256), 0, 0, sr->data, sr->ncols,
+                                   d_idx, primo, (uint32_t)PP, lo, hi, first, skip, xh, xl);
+                ANL_HIP(hipGetLastError());
+                hipLaunchKernelGGL(gkm_tile_kernel, dim3((unsigned)ngroups, ncorr, (unsigned)nchunks), dim3(64), 0, 0,
+                                   ka);
+                ANL_HIP(hipGetLastError());
+                const uint64_t nw = (uint64_t)leff * ncorr;
+                hipLaunchKernelGGL(gkm_reduce_kernel, grid1(nw, 1), dim3(256), 0, 0, partial, (uint32_t)ntiles,
+                                   (uint32_t)nchunks, ncorr, leff, narr, (double)cont, iseg, d);
+                ANL_HIP(hipGetLastError());
+            }
+            gk_epilogue(d, narr, ncur, leff, subtract_mean, start_mean, 0);
+            ANL_HIP(hipMemcpy(out_host, d, outn * sizeof(double), hipMemcpyDeviceToHost));
+        } catch (...) {
+            for (void *p : bufs) (void)hipFree(p);
+            throw;
+        }
+        for (void *p : bufs) (void)hipFree(p);
+    });
+}
+
+}  // extern "C"

@@ -558,9 +558,10 @@ class GreenKuboNComponentIonicFluid
                                   std::vector<std::string> headers, bool dump = false,
                                   unsigned int lunghezza_funzione_max = 0, unsigned int nthreads = 0,
                                   bool subtract_mean = false, unsigned int start_mean = 100, unsigned int n_seg = 1,
-                                  bool do_bench = false, unsigned int n_seg_start = 10, unsigned int n_seg_stop = 100)
+                                  bool do_bench = false, unsigned int n_seg_start = 10, unsigned int n_seg_stop = 100,
+                                  bool mfma = false)
         : traiettoria(traiettoria), log(std::move(log)), skip(skip), scrivi_file(dump), lmax(lunghezza_funzione_max),
-          subtract_mean(subtract_mean), start_mean(start_mean), n_seg(n_seg < 1 ? 1 : n_seg) {
+          subtract_mean(subtract_mean), start_mean(start_mean), n_seg(n_seg < 1 ? 1 : n_seg), mfma(mfma) {
         (void)nthreads;
         (void)do_bench;
         (void)n_seg_start;
@@ -604,8 +605,11 @@ class GreenKuboNComponentIonicFluid
             throw std::runtime_error(
                 "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
                 "the size of the average or the lenght of the correlation function");
-        anl_check(anl_gk_calculate(traiettoria->device(), idx_j.data(), (uint32_t)idx_j.size(), skip, lmax,
-                                   ntimesteps, n_seg, subtract_mean, start_mean, primo, vdata));
+        // mfma: lag products as split-BF16 MFMA products (C(l) within ~1e-6 of
+        // its scale); otherwise the FP64 form, the reference's values.
+        anl_check((mfma ? anl_gk_calculate_mfma : anl_gk_calculate)(traiettoria->device(), idx_j.data(),
+                                                                    (uint32_t)idx_j.size(), skip, lmax, ntimesteps,
+                                                                    n_seg, subtract_mean, start_mean, primo, vdata));
         if (scrivi_file) {
             std::ofstream outfile(log + ".greekdump", std::ios::app);
             for (unsigned int it = 0; it < leff; it++) {
@@ -633,6 +637,7 @@ class GreenKuboNComponentIonicFluid
     unsigned int lmax, leff = 0, ntimesteps = 0;
     bool subtract_mean;
     unsigned int start_mean, n_seg;
+    bool mfma;
     std::vector<uint32_t> idx_j;
 };
 

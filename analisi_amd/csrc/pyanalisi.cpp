@@ -181,7 +181,11 @@ PYBIND11_MODULE(_pyanalisi, m) {
 
     py::class_<GK>(m, "GreenKubo", py::buffer_protocol())
         .def(py::init<RL *, std::string, unsigned int, std::vector<std::string>, bool, unsigned int, unsigned int,
-                      bool, unsigned int, unsigned int, bool, unsigned int, unsigned int>())
+                      bool, unsigned int, unsigned int, bool, unsigned int, unsigned int, bool>(),
+             py::arg("log"), py::arg("dump"), py::arg("skip"), py::arg("headers"), py::arg("dump_files") = false,
+             py::arg("lmax") = 0, py::arg("nthreads") = 0, py::arg("subtract_mean") = false,
+             py::arg("start_mean") = 100, py::arg("n_seg") = 1, py::arg("do_bench") = false,
+             py::arg("n_seg_start") = 10, py::arg("n_seg_stop") = 100, py::arg("mfma") = false)
         .def("getNumberOfExtraTimestepsNeeded", &GK::nExtraTimesteps)
         .def("reset", &GK::reset)
         .def("calculate", &GK::calculate)
@@ -192,21 +196,23 @@ PYBIND11_MODULE(_pyanalisi, m) {
     // (main.cpp:456-488): returns (mean, variance) as [narr][block size].
     m.def("gk_block_average",
           [](RL &log, std::vector<std::string> headers, unsigned int nblocks, unsigned int skip, unsigned int lmax,
-             unsigned int n_seg, bool sub_mean, unsigned int start_mean) {
+             unsigned int n_seg, bool sub_mean, unsigned int start_mean, bool mfma) {
               BlockAverageG<RL, GK, std::string, unsigned int, std::vector<std::string>, bool, unsigned int,
-                            unsigned int, bool, unsigned int, unsigned int, bool, unsigned int, unsigned int>
+                            unsigned int, bool, unsigned int, unsigned int, bool, unsigned int, unsigned int, bool>
                   ba(&log, nblocks);
               unsigned int narr = (unsigned int)(headers.size() * headers.size() * 3 + 2);
               MediaVarCovar<GK> mvc(narr, {});
               ba.calcola_custom(&mvc, std::string(""), skip, headers, false, lmax, 1u, sub_mean, start_mean, n_seg,
-                                false, 10u, 100u);
+                                false, 10u, 100u, mfma);
               py::array_t<double> mean({(ssize_t)narr, (ssize_t)mvc.size()}), var({(ssize_t)narr, (ssize_t)mvc.size()});
               for (unsigned int c = 0; c < narr; c++) {
                   std::memcpy(mean.mutable_data() + (size_t)c * mvc.size(), mvc.media(c), mvc.size() * sizeof(double));
                   std::memcpy(var.mutable_data() + (size_t)c * mvc.size(), mvc.varianza(c), mvc.size() * sizeof(double));
               }
               return py::make_tuple(mean, var);
-          });
+          },
+          py::arg("log"), py::arg("headers"), py::arg("nblocks"), py::arg("skip"), py::arg("lmax"), py::arg("n_seg"),
+          py::arg("subtract_mean"), py::arg("start_mean"), py::arg("mfma") = false);
     // BlockAverageG + MediaVar of Gofrt / MSD (blockaverage.h:194-203), the
     // CLI's -g and -Q paths: returns (mean, variance) in the calculation's layout.
     m.def("gofrt_block_average",

@@ -147,6 +147,15 @@ int anl_series_destroy(anl_series *s);
 int anl_gk_calculate(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
                      uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
                      uint32_t start_mean, uint64_t primo, double *out_host);
+/* Same interface and output as anl_gk_calculate, with the lag products
+ * J_a(t0).J_b(t0+l) formed on the matrix cores as three split-BF16 MFMA
+ * products (hi*hi + hi*lo + lo*hi), FP32 accumulation within chunks of 8192
+ * t0 and FP64 across chunks; the integrals and the Schur complement are the
+ * FP64 ones.  Not bit-exact: C(l) is within ~1e-6 of max|C| of the FP64
+ * form.  Replaces the same loop (greenkubo...cpp:156-249). */
+int anl_gk_calculate_mfma(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
+                          uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
+                          uint32_t start_mean, uint64_t primo, double *out_host);
 
 /* ---- block statistics --------------------------------------------------- */
 /* MediaVar::calculate (calcoliblocchi.h:35-58) on device vectors of length

@@ -1,0 +1,96 @@
+"""Green-Kubo with the lag products on the matrix cores (split-BF16 MFMA)
+against the FP64 oracle (greenkuboNcomponentionicfluid.cpp:156-356).
+
+Tolerances, per correlation column ab with scale
+sigma_ab = sqrt(C_aa(0) C_bb(0)) (the Cauchy-Schwarz bound of |C_ab(l)|):
+  * AR(2) currents: |dC_ab(l)| <= 1e-6 sigma_ab, and the running integrals
+    |dI_ab(l)| <= 1e-6 sigma_ab (l + 1), the bound their sum of C gives;
+  * integer currents |J| <= 8: every value is exact in bf16 (lo = 0) and
+    every FP32 chunk sum is an exact integer, so C agrees with the FP64 form
+    to 1e-12 sigma_ab -- any misplaced tile row, column or lag shows up here.
+"""
+import numpy as np
+import pytest
+
+from analisi_amd import capi, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# ncur, skip, n_seg, tmax, n, primo
+CASES = [
+    (1, 1, 1, 0, 700, 0),        # leff = n, one chunk, one tile group
+    (2, 1, 1, 300, 9000, 5),     # two FP32 chunks of 8192 t0
+    (2, 3, 5, 2100, 4000, 17),   # skip and segments
+    (3, 2, 2, 4500, 5000, 1),    # 142 lag tiles: two tile groups
+    (4, 1, 3, 37, 1000, 3),      # four currents, short lags
+]
+
+
+def _scales(ref, ncur):
+    c0 = ref[0, :ncur * ncur].reshape(ncur, ncur)
+    d = np.sqrt(np.abs(np.diag(c0)))
+    return np.outer(d, d).reshape(-1)
+
+
+def _series(kind, ncur, rows, seed):
+    if kind == "int":
+        rng = np.random.default_rng(seed)
+        J = rng.integers(-8, 9, size=(rows, 3 * ncur)).astype(np.float64)
+    else:
+        J = synth.ar2_currents(seed, rows, ncur=ncur)
+    return np.ascontiguousarray(np.concatenate([np.full((rows, 1), 7.0), J], axis=1))
+
+
+@pytest.mark.parametrize("kind", ["int", "ar2"])
+@pytest.mark.parametrize("case", CASES, ids=[str(c) for c in CASES])
+def test_gk_mfma_matches_fp64_oracle(kind, case):
+    ncur, skip, n_seg, tmax, n, primo = case
+    leff = n if (tmax == 0 or n < tmax) else tmax
+    rows = leff + primo + n
+    data = _series(kind, ncur, rows, 31 + ncur)
+    idx = [1 + 3 * k for k in range(ncur)]
+    sub = n_seg == 3
+    s = capi.DeviceSeries(data)
+    got = s.gk(idx, skip, tmax, n, n_seg, sub, 10, primo, mfma=True)
+    ref = O.gk(data, idx, skip, tmax, sub, 10, n_seg, n, primo)
+    assert got.shape == ref.shape
+    nc = ncur * ncur
+    sig = _scales(ref, ncur)
+    dC = np.abs(got[:, :nc] - ref[:, :nc]) / sig
+    tol = 1e-12 if kind == "int" else 1e-6
+    assert dC.max() <= tol, (dC.max(), np.unravel_index(dC.argmax(), dC.shape))
+    I_got, I_ref = got[:, nc:3 * nc:2], ref[:, nc:3 * nc:2]
+    dI = np.abs(I_got - I_ref) / sig / (np.arange(leff)[:, None] + 1.0)
+    assert dI.max() <= tol, dI.max()
+    # the FP64 form of the same call is the oracle's, bit for bit (ncur <= 2)
+    if ncur <= 2:
+        f64 = s.gk(idx, skip, tmax, n, n_seg, sub, 10, primo)
+        assert np.array_equal(np.nan_to_num(f64, nan=1.5), np.nan_to_num(ref, nan=1.5))
+
+
+def test_gk_mfma_errors():
+    s = capi.DeviceSeries(np.zeros((100, 3)))
+    with pytest.raises(capi.AnlError) as e:
+        s.gk([0], 1, 50, 60, 1, False, 0, 0, mfma=True)
+    assert e.value.code == capi.ANL_ERR_TOO_SHORT
+    with pytest.raises(capi.AnlError) as e:
+        s.gk([1], 1, 5, 20, 1, False, 0, 0, mfma=True)
+    assert e.value.code == capi.ANL_ERR_HEADER
+
+
+def test_gk_mfma_pyanalisi_reference_call(gk_ref):
+    """The reference's test call (tests/test_gk.py:4-12) with mfma=True,
+    against the golden FP64 result (tests/test_gk.py snapshot)."""
+    from analisi_amd import pyanalisi as pa
+    log = pa.ReadLog(np.array(gk_ref["log"], copy=True), list(gk_ref["headers"]))
+    gk = pa.GreenKubo(log, '', 1, ['c_flux[1]', 'c_vcm[1][1]'], False, 2000, 2, False, 0, 4, False, 1, 100,
+                      mfma=True)
+    gk.reset(log.getNtimesteps() - 2000)
+    gk.calculate(0)
+    m = np.array(gk, copy=True)
+    ref = gk_ref["test_gk"]
+    assert m.shape == ref.shape
+    sig = _scales(ref, 2)
+    dC = np.abs(m[:, :4] - ref[:, :4]) / sig
+    assert dC.max() <= 1e-6, dC.max()
