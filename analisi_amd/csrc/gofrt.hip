@@ -1047,4 +1047,92 @@ int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uin
     });
 }
 
+int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_wg) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        if (unroll != 4 && unroll != 8) fail(ANL_ERR_ARG, "unroll must be 4 or 8");
+        if (pairs_per_wg == 0) fail(ANL_ERR_ARG, "pairs_per_wg must be positive");
+        g->tune.unroll = unroll;
+        g->tune.xcd = xcd != 0;
+        g->tune.pairs_per_wg = pairs_per_wg;
+    });
+}
+
+int anl_gofrt_get_tuning(anl_gofrt *g, int *unroll, int *xcd, uint64_t *pairs_per_wg) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        if (unroll) *unroll = g->tune.unroll;
+        if (xcd) *xcd = g->tune.xcd ? 1 : 0;
+        if (pairs_per_wg) *pairs_per_wg = g->tune.pairs_per_wg;
+    });
+}
+
+// Times every candidate launch shape on the first nlag_probe lags of the
+// calculation set by reset(n) at primo (one untimed launch, then one timed
+// launch per candidate, HIP events on the null stream) and keeps the fastest.
+int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const int32_t *cand, uint32_t ncand,
+                       float *ms_out, uint32_t *best_out) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        ensure_device();
+        check_lengths(g, &primo, &g->n, 1);
+        static const int32_t defaults[] = {4, 0, 31, 4, 1, 31, 8, 0, 31, 8, 1, 31,
+                                           4, 0, 30, 4, 0, 32, 8, 0, 30, 8, 0, 32};
+        if (!cand) {
+            cand = defaults;
+            ncand = sizeof(defaults) / sizeof(defaults[0]) / 3;
+        }
+        if (ncand == 0) fail(ANL_ERR_ARG, "no candidates");
+        for (uint32_t c = 0; c < ncand; c++)
+            if ((cand[3 * c] != 4 && cand[3 * c] != 8) || cand[3 * c + 2] < 10 || cand[3 * c + 2] > 40)
+                fail(ANL_ERR_ARG, "candidate " + std::to_string(c) + ": unroll 4|8, log2(pairs per workgroup) in [10, 40]");
+        const uint32_t nlag_all = (g->leff + g->every - 1) / g->every;
+        const uint32_t nlag = std::max(1u, std::min(nlag_probe ? nlag_probe : 1u, nlag_all));
+        const uint64_t len = (uint64_t)g->leff * nch_of(g->tr) * g->nbin;
+        if (len == 0) return;
+        const GofrtTuning saved = g->tune;
+        unsigned long long *d = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ANL_HIP(hipMalloc(&d, len * sizeof(unsigned long long)));
+        float best = -1.0f;
+        uint32_t besti = 0;
+        try {
+            ANL_HIP(hipEventCreate(&e0));
+            ANL_HIP(hipEventCreate(&e1));
+            for (uint32_t c = 0; c < ncand; c++) {
+                g->tune.unroll = cand[3 * c];
+                g->tune.xcd = cand[3 * c + 1] != 0;
+                g->tune.pairs_per_wg = 1ull << cand[3 * c + 2];
+                Launch L = plan(g, &primo, &g->n, 1, -1);
+                GofrtPrepared P = gofrt_prepare(g, L, len, INT32_MIN, INT32_MAX, 0, 0, nlag);
+                gofrt_enqueue(g, P, d, 0);
+                ANL_HIP(hipEventRecord(e0, 0));
+                gofrt_enqueue(g, P, d, 0);
+                ANL_HIP(hipEventRecord(e1, 0));
+                ANL_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                ANL_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms_out) ms_out[c] = ms;
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    besti = c;
+                }
+            }
+        } catch (...) {
+            g->tune = saved;
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(d);
+        g->tune.unroll = cand[3 * besti];
+        g->tune.xcd = cand[3 * besti + 1] != 0;
+        g->tune.pairs_per_wg = 1ull << cand[3 * besti + 2];
+        if (best_out) *best_out = besti;
+    });
+}
+
 }  // extern "C"

@@ -115,6 +115,30 @@ def test_screened_refuses_large_rmax():
     assert g2.stats()[0] == capi.GOFRT_FP64
 
 
+@pytest.mark.parametrize("form", [capi.GOFRT_FP64, capi.GOFRT_SCREENED])
+def test_launch_shapes_and_autotune_keep_counts(form):
+    """Every launch shape (unroll, XCD order, t0 chunk down to one t0 per
+    workgroup) gives the oracle's counts; the autotuner times the built-in
+    candidates, keeps one of them and the counts stay the same."""
+    ot, dt = make_case(12, 700, 30, 15.0, [0.7, 0.3], 0.15, [1.1, -0.9, 0.6])
+    ref, _ = O.gofrt(ot, 0.0, 6.0, 48, 9, 4, 1, 1, 20, 0, want_vdata=False)
+    g = capi.DeviceGofrt(dt, 0.0, 6.0, 48, 9, 1, 1, form)
+    g.reset(20)
+    for unroll, xcd, lp in [(4, 0, 31), (8, 1, 31), (4, 1, 12), (8, 0, 16), (4, 0, 40)]:
+        g.set_tuning(unroll, xcd, 1 << lp)
+        assert g.tuning() == (unroll, xcd, 1 << lp)
+        got = g.calculate(0)
+        assert first_diff(got, ref) is None, (unroll, xcd, lp, first_diff(got, ref))
+    ms, best = g.autotune(0, 3)
+    assert len(ms) == 8 and (ms > 0).all() and 0 <= best < 8
+    assert first_diff(g.calculate(0), ref) is None
+    ms, best = g.autotune(0, 2, [(8, 1, 20), (4, 0, 22)])
+    assert g.tuning() == [(8, 1, 1 << 20), (4, 0, 1 << 22)][best]
+    assert first_diff(g.calculate(0), ref) is None
+    with pytest.raises(capi.AnlError):
+        g.autotune(0, 2, [(5, 0, 20)])
+
+
 def test_full_size_triclinic_invariants():
     """16,384 atoms (80/10/10), triclinic L=56 with tilts, lags 0..100, 256
     bins to 12 A, a few t0: both forms agree bit for bit; lag-0 self channels
