@@ -60,6 +60,8 @@ SIGNATURES = [
     ("anl_series_destroy", _i, [_vp]),
     ("anl_gk_calculate", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
     ("anl_gk_calculate_mfma", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
+    ("anl_gk_calculate_async", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp, _vp]),
+    ("anl_gk_calculate_mfma_async", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp, _vp]),
     ("anl_mediavar_update_async", _i, [_vp, _vp, _vp, _u64, _u32, _vp]),
     ("anl_mediavar_end_async", _i, [_vp, _u64, _u32, _vp]),
     ("anl_counts_to_double_async", _i, [_vp, _d, _u64, _vp, _vp]),

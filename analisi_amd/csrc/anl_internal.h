@@ -93,6 +93,10 @@ struct anl_series {
     uint64_t nrows = 0;
     uint32_t ncols = 0;
     double *data = nullptr;
+    // Green-Kubo scratch (splits, partial sums, output staging), grow-only;
+    // calls on one series are stream-ordered on the caller's stream.
+    void *work = nullptr;
+    size_t work_cap = 0;
 };
 
 namespace anl {
@@ -114,6 +118,9 @@ void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n,
                          uint32_t tmax, int cm_msd, int cm_self, uint32_t *leff_out);
 void msd_enqueue_prepared(void *prep, double *out, hipStream_t s);
 void msd_free_prepared(void *prep);
+// Scratch of at least bytes in s->work (256-byte aligned sub-buffers are
+// carved by the caller); growing synchronises the device first.
+void *series_workspace(anl_series *s, size_t bytes);
 // FP64 Green-Kubo epilogue (mean subtraction, running integrals, Schur
 // complement) over d[leff][narr] whose first ncur^2 columns hold C(l).
 void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,

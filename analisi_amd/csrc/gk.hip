@@ -214,55 +214,81 @@ void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int sub
 
 using namespace anl;
 
+namespace {
+
+uint32_t gk_fp64_checks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t tmax, uint32_t n,
+                        uint64_t primo) {
+    if (!s || !idx_j) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate");
+    if (ncur == 0 || ncur > GK_MAXC) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
+    for (uint32_t c = 0; c < ncur; c++)
+        if (idx_j[c] + 3 > s->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
+    const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;  // :123
+    if ((uint64_t)leff + primo + n > s->nrows)                 // :168-169
+        fail(ANL_ERR_TOO_SHORT,
+             "trajectory is too short for this kind of calculation. Select a different starting timestep or "
+             "lower the size of the average or the lenght of the correlation function");
+    return leff;
+}
+
+void gk_fp64_enqueue(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff, uint32_t n,
+                     uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo, double *d,
+                     hipStream_t st) {
+    const uint32_t narr = 3 * ncur * ncur + 2;
+    GkArgs a{};
+    a.data = s->data;
+    a.ncols = s->ncols;
+    a.ncur = ncur;
+    for (uint32_t c = 0; c < ncur; c++) a.idx[c] = idx_j[c];
+    a.primo = primo;
+    a.n = n;
+    a.n_seg = n_seg;
+    a.skip = skip;
+    a.leff = leff;
+    a.narr = narr;
+    a.out = d;
+    dim3 grid((leff + 255) / 256);
+    switch (ncur) {
+        case 1: hipLaunchKernelGGL(gk_lag_kernel<1>, grid, dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL(gk_lag_kernel<2>, grid, dim3(256), 0, st, a); break;
+        case 3: hipLaunchKernelGGL(gk_lag_kernel<3>, grid, dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL(gk_lag_kernel<4>, grid, dim3(256), 0, st, a); break;
+    }
+    ANL_HIP(hipGetLastError());
+    gk_epilogue(d, narr, ncur, leff, subtract_mean, start_mean, st);
+}
+
+}  // namespace
+
 extern "C" {
+
+int anl_gk_calculate_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                           uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
+                           double *out_dev, void *stream) {
+    return guard([&] {
+        const uint32_t leff = gk_fp64_checks(s, idx_j, ncur, tmax, n, primo);
+        if (!out_dev) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate_async");
+        if (n_seg < 1) n_seg = 1;  // greenkubo...cpp:50-53
+        if (skip == 0) skip = 1;
+        if (leff == 0) return;
+        ensure_device();
+        gk_fp64_enqueue(s, idx_j, ncur, skip, leff, n, n_seg, subtract_mean, start_mean, primo, out_dev,
+                        (hipStream_t)stream);
+    });
+}
 
 int anl_gk_calculate(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax, uint32_t n,
                      uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo, double *out_host) {
     return guard([&] {
-        if (!s || !idx_j || !out_host) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate");
-        if (ncur == 0 || ncur > GK_MAXC) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
-        for (uint32_t c = 0; c < ncur; c++)
-            if (idx_j[c] + 3 > s->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
-        if (n_seg < 1) n_seg = 1;  // greenkubo...cpp:50-53
+        const uint32_t leff = gk_fp64_checks(s, idx_j, ncur, tmax, n, primo);
+        if (!out_host) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate");
+        if (n_seg < 1) n_seg = 1;
         if (skip == 0) skip = 1;
-        const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;  // :123
-        if ((uint64_t)leff + primo + n > s->nrows)                 // :168-169
-            fail(ANL_ERR_TOO_SHORT,
-                 "trajectory is too short for this kind of calculation. Select a different starting timestep or "
-                 "lower the size of the average or the lenght of the correlation function");
-        const uint32_t ncorr = ncur * ncur, narr = 3 * ncorr + 2;
         if (leff == 0) return;
         ensure_device();
-        double *d = nullptr;
-        ANL_HIP(hipMalloc(&d, (size_t)leff * narr * sizeof(double)));
-        try {
-            GkArgs a{};
-            a.data = s->data;
-            a.ncols = s->ncols;
-            a.ncur = ncur;
-            for (uint32_t c = 0; c < ncur; c++) a.idx[c] = idx_j[c];
-            a.primo = primo;
-            a.n = n;
-            a.n_seg = n_seg;
-            a.skip = skip;
-            a.leff = leff;
-            a.narr = narr;
-            a.out = d;
-            dim3 grid((leff + 255) / 256);
-            switch (ncur) {
-                case 1: hipLaunchKernelGGL(gk_lag_kernel<1>, grid, dim3(256), 0, 0, a); break;
-                case 2: hipLaunchKernelGGL(gk_lag_kernel<2>, grid, dim3(256), 0, 0, a); break;
-                case 3: hipLaunchKernelGGL(gk_lag_kernel<3>, grid, dim3(256), 0, 0, a); break;
-                default: hipLaunchKernelGGL(gk_lag_kernel<4>, grid, dim3(256), 0, 0, a); break;
-            }
-            ANL_HIP(hipGetLastError());
-            gk_epilogue(d, narr, ncur, leff, subtract_mean, start_mean, 0);
-            ANL_HIP(hipMemcpy(out_host, d, (size_t)leff * narr * sizeof(double), hipMemcpyDeviceToHost));
-        } catch (...) {
-            (void)hipFree(d);
-            throw;
-        }
-        (void)hipFree(d);
+        const size_t bytes = (size_t)leff * (3 * ncur * ncur + 2) * sizeof(double);
+        double *d = (double *)series_workspace(s, bytes);
+        gk_fp64_enqueue(s, idx_j, ncur, skip, leff, n, n_seg, subtract_mean, start_mean, primo, d, 0);
+        ANL_HIP(hipMemcpy(out_host, d, bytes, hipMemcpyDeviceToHost));
     });
 }
 

@@ -50,8 +50,11 @@ __device__ __forceinline__ void split_bf16(double v, unsigned short &h, unsigned
 
 // X side, one segment: element (cx, r, p) = data[primo + t][idx[c] + x] at
 // t = 32p + r when t is a selected t0 of the segment, else 0.
-__global__ void gkm_split_x_kernel(const double *__restrict__ data, uint32_t ncols, const uint32_t *__restrict__ idx,
-                                   uint64_t primo, uint32_t PP, uint32_t seg_lo, uint32_t seg_hi, uint32_t first,
+struct GkIdx {
+    uint32_t v[4];
+};
+
+__global__ void gkm_split_x_kernel(const double *__restrict__ data, uint32_t ncols, const GkIdx idx, uint64_t primo, uint32_t PP, uint32_t seg_lo, uint32_t seg_hi, uint32_t first,
                                    uint32_t skip, unsigned short *__restrict__ xh, unsigned short *__restrict__ xl) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // e = r * PP + p within one cx
     if (e >= 32ull * PP) return;
@@ -59,7 +62,7 @@ __global__ void gkm_split_x_kernel(const double *__restrict__ data, uint32_t nco
     const uint32_t r = (uint32_t)(e / PP), p = (uint32_t)(e % PP);
     const uint64_t t = 32ull * p + r;
     double v = 0.0;
-    if (t >= seg_lo && t < seg_hi && t >= first && (t - first) % skip == 0) v = data[(primo + t) * ncols + idx[c] + x];
+    if (t >= seg_lo && t < seg_hi && t >= first && (t - first) % skip == 0) v = data[(primo + t) * ncols + idx.v[c] + x];
     unsigned short h, l;
     split_bf16(v, h, l);
     const uint64_t o = (uint64_t)cx * 32 * PP + e;
@@ -69,8 +72,7 @@ __global__ void gkm_split_x_kernel(const double *__restrict__ data, uint32_t nco
 
 // Y side: element (c8, cx, s, p') = data[primo + t][idx[c] + x] at
 // t = 32(p' + c8) + s when t < avail, else 0.
-__global__ void gkm_split_y_kernel(const double *__restrict__ data, uint32_t ncols, const uint32_t *__restrict__ idx,
-                                   uint64_t primo, uint64_t avail, uint32_t YP, uint32_t ncx,
+__global__ void gkm_split_y_kernel(const double *__restrict__ data, uint32_t ncols, const GkIdx idx, uint64_t primo, uint64_t avail, uint32_t YP, uint32_t ncx,
                                    unsigned short *__restrict__ yh, unsigned short *__restrict__ yl) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // s * YP + p'
     if (e >= 32ull * YP) return;
@@ -78,7 +80,7 @@ __global__ void gkm_split_y_kernel(const double *__restrict__ data, uint32_t nco
     const uint32_t s = (uint32_t)(e / YP), pp = (uint32_t)(e % YP);
     const uint64_t t = 32ull * ((uint64_t)pp + c8) + s;
     double v = 0.0;
-    if (t < avail) v = data[(primo + t) * ncols + idx[c] + x];
+    if (t < avail) v = data[(primo + t) * ncols + idx.v[c] + x];
     unsigned short h, l;
     split_bf16(v, h, l);
     const uint64_t o = ((uint64_t)c8 * ncx + cx) * 32 * YP + e;
@@ -187,88 +189,134 @@ __global__ void gkm_zero_kernel(double *__restrict__ p, uint64_t n) {
 
 using namespace anl;
 
+namespace {
+
+struct GkmPlan {
+    uint32_t leff, ncorr, narr, ncx;
+    uint64_t nchunks, PP, ntiles, ngroups, YP;
+    size_t off_xh, off_xl, off_yh, off_yl, off_partial, bytes;
+};
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+GkmPlan gkm_plan(uint32_t ncur, uint32_t leff, uint32_t n) {
+    GkmPlan p{};
+    p.leff = leff;
+    p.ncorr = ncur * ncur;
+    p.narr = 3 * p.ncorr + 2;
+    p.ncx = 3 * ncur;
+    // t0 axis: rows of 32, MFMA steps of 16 rows, chunks of GKM_CS steps
+    const uint64_t nsteps = ((uint64_t)n + 511) / 512;
+    p.nchunks = (nsteps + GKM_CS - 1) / GKM_CS;
+    p.PP = p.nchunks * GKM_CS * 16;
+    // lag axis: tiles of 32 (plus one for the negative diagonals)
+    p.ntiles = (leff + 30) / 32 + 1;
+    p.ngroups = (p.ntiles + 16 * GKM_Q - 1) / (16 * GKM_Q) * 16;
+    const uint64_t qmax = p.ngroups / 16 * 16 * GKM_Q;  // tiles covered, >= ntiles
+    // highest Y row read: 16 (st + Q - 1) + 8 + 7 + qoff, st < nchunks * CS, qoff <= qmax - 16 Q + 8
+    p.YP = (p.PP + 16 * GKM_Q + qmax + 16 + 7) / 8 * 8;
+    if (p.YP >= (1ull << 31) || p.nchunks > 65535)
+        fail(ANL_ERR_ARG, "series too long for the MFMA form (use the FP64 form)");
+    const size_t xn = (size_t)p.ncx * 32 * p.PP, yn = (size_t)8 * p.ncx * 32 * p.YP;
+    p.off_xh = 0;
+    p.off_xl = p.off_xh + align256(xn * 2);
+    p.off_yh = p.off_xl + align256(xn * 2);
+    p.off_yl = p.off_yh + align256(yn * 2);
+    p.off_partial = p.off_yl + align256(yn * 2);
+    p.bytes = p.off_partial + align256((size_t)p.ncorr * p.ntiles * p.nchunks * 63 * sizeof(double));
+    return p;
+}
+
+uint32_t gk_checks(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t tmax, uint32_t n,
+                   uint64_t primo) {
+    if (!sr || !idx_j) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate");
+    if (ncur == 0 || ncur > 4) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
+    for (uint32_t c = 0; c < ncur; c++)
+        if (idx_j[c] + 3 > sr->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
+    const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;  // greenkubo...cpp:123
+    if ((uint64_t)leff + primo + n > sr->nrows)                 // :168-169
+        fail(ANL_ERR_TOO_SHORT,
+             "trajectory is too short for this kind of calculation. Select a different starting timestep or "
+             "lower the size of the average or the lenght of the correlation function");
+    return leff;
+}
+
+// Enqueues the whole MFMA form on s, output d [leff][narr] (device).
+void gkm_enqueue(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t n, uint32_t n_seg,
+                 int subtract_mean, uint32_t start_mean, uint64_t primo, const GkmPlan &p, char *work, double *d,
+                 hipStream_t s) {
+    auto *xh = (unsigned short *)(work + p.off_xh);
+    auto *xl = (unsigned short *)(work + p.off_xl);
+    auto *yh = (unsigned short *)(work + p.off_yh);
+    auto *yl = (unsigned short *)(work + p.off_yl);
+    auto *partial = (double *)(work + p.off_partial);
+    GkIdx idx{};
+    for (uint32_t c = 0; c < ncur; c++) idx.v[c] = idx_j[c];
+    auto grid1 = [](uint64_t n_, uint32_t y) { return dim3((unsigned)((n_ + 255) / 256), y); };
+    const uint64_t outn = (uint64_t)p.leff * p.narr;
+    const uint64_t avail = sr->nrows - primo;
+    hipLaunchKernelGGL(gkm_zero_kernel, grid1(outn, 1), dim3(256), 0, s, d, outn);
+    ANL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gkm_split_y_kernel, grid1(32 * p.YP, 8 * p.ncx), dim3(256), 0, s, sr->data, sr->ncols, idx,
+                       primo, avail, (uint32_t)p.YP, p.ncx, yh, yl);
+    ANL_HIP(hipGetLastError());
+    GkmArgs ka{xh, xl, yh, yl, ncur, (uint32_t)p.PP, (uint32_t)p.YP, (uint32_t)p.ntiles, (uint32_t)p.nchunks, partial};
+    const uint32_t seg = n / n_seg;
+    for (uint32_t iseg = 0; iseg < n_seg; iseg++) {
+        const uint32_t lo = seg * iseg, hi = seg * (iseg + 1);
+        const uint32_t first = lo + lo % skip;  // greenkubo...cpp:208
+        const uint32_t cont = first < hi ? (hi - first + skip - 1) / skip : 0;
+        hipLaunchKernelGGL(gkm_split_x_kernel, grid1(32 * p.PP, p.ncx), dim3(256), 0, s, sr->data, sr->ncols, idx,
+                           primo, (uint32_t)p.PP, lo, hi, first, skip, xh, xl);
+        ANL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gkm_tile_kernel, dim3((unsigned)p.ngroups, p.ncorr, (unsigned)p.nchunks), dim3(64), 0, s,
+                           ka);
+        ANL_HIP(hipGetLastError());
+        const uint64_t nw = (uint64_t)p.leff * p.ncorr;
+        hipLaunchKernelGGL(gkm_reduce_kernel, grid1(nw, 1), dim3(256), 0, s, partial, (uint32_t)p.ntiles,
+                           (uint32_t)p.nchunks, p.ncorr, p.leff, p.narr, (double)cont, iseg, d);
+        ANL_HIP(hipGetLastError());
+    }
+    gk_epilogue(d, p.narr, ncur, p.leff, subtract_mean, start_mean, s);
+}
+
+}  // namespace
+
 extern "C" {
+
+int anl_gk_calculate_mfma_async(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                                uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
+                                double *out_dev, void *stream) {
+    return guard([&] {
+        const uint32_t leff = gk_checks(sr, idx_j, ncur, tmax, n, primo);
+        if (!out_dev) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate_mfma_async");
+        if (n_seg < 1) n_seg = 1;  // greenkubo...cpp:50-53
+        if (skip == 0) skip = 1;
+        if (leff == 0) return;
+        ensure_device();
+        const GkmPlan p = gkm_plan(ncur, leff, n);
+        char *work = (char *)series_workspace(sr, p.bytes);
+        gkm_enqueue(sr, idx_j, ncur, skip, n, n_seg, subtract_mean, start_mean, primo, p, work, out_dev,
+                    (hipStream_t)stream);
+    });
+}
 
 int anl_gk_calculate_mfma(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
                           uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
                           double *out_host) {
     return guard([&] {
-        if (!sr || !idx_j || !out_host) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate_mfma");
-        if (ncur == 0 || ncur > 4) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
-        for (uint32_t c = 0; c < ncur; c++)
-            if (idx_j[c] + 3 > sr->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
-        if (n_seg < 1) n_seg = 1;  // greenkubo...cpp:50-53
+        const uint32_t leff = gk_checks(sr, idx_j, ncur, tmax, n, primo);
+        if (!out_host) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate_mfma");
+        if (n_seg < 1) n_seg = 1;
         if (skip == 0) skip = 1;
-        const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;
-        if ((uint64_t)leff + primo + n > sr->nrows)
-            fail(ANL_ERR_TOO_SHORT,
-                 "trajectory is too short for this kind of calculation. Select a different starting timestep or "
-                 "lower the size of the average or the lenght of the correlation function");
-        const uint32_t ncorr = ncur * ncur, narr = 3 * ncorr + 2, ncx = 3 * ncur;
         if (leff == 0) return;
         ensure_device();
-        // t0 axis: rows of 32, MFMA steps of 16 rows, chunks of GKM_CS steps
-        const uint64_t nsteps = ((uint64_t)n + 511) / 512;
-        const uint64_t nchunks = (nsteps + GKM_CS - 1) / GKM_CS;
-        const uint64_t PP = nchunks * GKM_CS * 16;
-        // lag axis: tiles of 32 (plus one for the negative diagonals)
-        const uint64_t ntiles = (leff + 30) / 32 + 1;
-        const uint64_t ngroups = (ntiles + 16 * GKM_Q - 1) / (16 * GKM_Q) * 16;
-        const uint64_t qmax = ngroups / 16 * 16 * GKM_Q;  // tiles covered, >= ntiles
-        // highest Y row read: 16 (st + Q - 1) + 8 + 7 + qoff, st < nchunks * CS, qoff < qmax
-        const uint64_t YP = (PP + 16 * GKM_Q + qmax + 16 + 7) / 8 * 8;
-        if (32 * YP * 8 > (1ull << 31) || 32 * PP > (1ull << 31) || nchunks > 65535)
-            fail(ANL_ERR_ARG, "series too long for the MFMA form (use the FP64 form)");
-        const uint64_t avail = sr->nrows - primo;
-
-        std::vector<void *> bufs;
-        auto dmalloc = [&](size_t bytes) {
-            void *p = nullptr;
-            ANL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
-            bufs.push_back(p);
-            return p;
-        };
-        auto grid1 = [](uint64_t n_, uint32_t y) { return dim3((unsigned)((n_ + 255) / 256), y); };
-        try {
-            uint32_t *d_idx = (uint32_t *)dmalloc(ncur * sizeof(uint32_t));
-            ANL_HIP(hipMemcpy(d_idx, idx_j, ncur * sizeof(uint32_t), hipMemcpyHostToDevice));
-            const size_t xn = (size_t)ncx * 32 * PP, yn = (size_t)8 * ncx * 32 * YP;
-            auto *xh = (unsigned short *)dmalloc(xn * 2);
-            auto *xl = (unsigned short *)dmalloc(xn * 2);
-            auto *yh = (unsigned short *)dmalloc(yn * 2);
-            auto *yl = (unsigned short *)dmalloc(yn * 2);
-            auto *partial = (double *)dmalloc((size_t)ncorr * ntiles * nchunks * 63 * sizeof(double));
-            const uint64_t outn = (uint64_t)leff * narr;
-            auto *d = (double *)dmalloc(outn * sizeof(double));
-            hipLaunchKernelGGL(gkm_zero_kernel, grid1(outn, 1), dim3(256), 0, 0, d, outn);
-            ANL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(gkm_split_y_kernel, grid1(32 * YP, 8 * ncx), dim3(256), 0, 0, sr->data, sr->ncols,
-                               d_idx, primo, avail, (uint32_t)YP, ncx, yh, yl);
-            ANL_HIP(hipGetLastError());
-            GkmArgs ka{xh, xl, yh, yl, ncur, (uint32_t)PP, (uint32_t)YP, (uint32_t)ntiles, (uint32_t)nchunks, partial};
-            const uint32_t seg = n / n_seg;
-            for (uint32_t iseg = 0; iseg < n_seg; iseg++) {
-                const uint32_t lo = seg * iseg, hi = seg * (iseg + 1);
-                const uint32_t first = lo + lo % skip;  // greenkubo...cpp:208
-                const uint32_t cont = first < hi ? (hi - first + skip - 1) / skip : 0;
-                hipLaunchKernelGGL(gkm_split_x_kernel, grid1(32 * PP, ncx), dim3(256), 0, 0, sr->data, sr->ncols,
-                                   d_idx, primo, (uint32_t)PP, lo, hi, first, skip, xh, xl);
-                ANL_HIP(hipGetLastError());
-                hipLaunchKernelGGL(gkm_tile_kernel, dim3((unsigned)ngroups, ncorr, (unsigned)nchunks), dim3(64), 0, 0,
-                                   ka);
-                ANL_HIP(hipGetLastError());
-                const uint64_t nw = (uint64_t)leff * ncorr;
-                hipLaunchKernelGGL(gkm_reduce_kernel, grid1(nw, 1), dim3(256), 0, 0, partial, (uint32_t)ntiles,
-                                   (uint32_t)nchunks, ncorr, leff, narr, (double)cont, iseg, d);
-                ANL_HIP(hipGetLastError());
-            }
-            gk_epilogue(d, narr, ncur, leff, subtract_mean, start_mean, 0);
-            ANL_HIP(hipMemcpy(out_host, d, outn * sizeof(double), hipMemcpyDeviceToHost));
-        } catch (...) {
-            for (void *p : bufs) (void)hipFree(p);
-            throw;
-        }
-        for (void *p : bufs) (void)hipFree(p);
+        const GkmPlan p = gkm_plan(ncur, leff, n);
+        const size_t outb = align256((size_t)leff * p.narr * sizeof(double));
+        char *work = (char *)series_workspace(sr, p.bytes + outb);
+        double *d = (double *)(work + p.bytes);
+        gkm_enqueue(sr, idx_j, ncur, skip, n, n_seg, subtract_mean, start_mean, primo, p, work, d, 0);
+        ANL_HIP(hipMemcpy(out_host, d, (size_t)leff * p.narr * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 

@@ -136,6 +136,19 @@ __global__ void counts_to_double_kernel(const unsigned long long *__restrict__ c
 
 using namespace anl;
 
+namespace anl {
+void *series_workspace(anl_series *s, size_t bytes) {
+    if (bytes <= s->work_cap) return s->work;
+    ANL_HIP(hipDeviceSynchronize());  // earlier async calls may still read the old buffer
+    (void)hipFree(s->work);
+    s->work = nullptr;
+    s->work_cap = 0;
+    ANL_HIP(hipMalloc(&s->work, bytes));
+    s->work_cap = bytes;
+    return s->work;
+}
+}  // namespace anl
+
 extern "C" {
 
 const char *anl_last_error(void) { return g_last_error.c_str(); }
@@ -261,6 +274,7 @@ int anl_traj_destroy(anl_traj *tr) {
     });
 }
 
+
 int anl_series_create(const double *data, uint64_t nrows, uint32_t ncols, anl_series **out) {
     return guard([&] {
         if (!data || !out) fail(ANL_ERR_ARG, "null pointer passed to anl_series_create");
@@ -278,6 +292,8 @@ int anl_series_create(const double *data, uint64_t nrows, uint32_t ncols, anl_se
 int anl_series_destroy(anl_series *s) {
     return guard([&] {
         if (!s) return;
+        (void)hipDeviceSynchronize();
+        (void)hipFree(s->work);
         (void)hipFree(s->data);
         delete s;
     });

@@ -1,20 +1,36 @@
-"""Benchmark of the g(r,t) pair-lag path on one MI355X.
+"""Benchmark of the time-lagged pair-correlation path on one MI355X.
 
-Workload (default): 16,384 atoms of 3 types (80/10/10 %), 2,000 frames of a
-seeded Gaussian random walk (sigma 0.05 A per axis) in a triclinic cell with
-diagonal 56 A and tilts drawn uniformly in +-0.1 L (fixed per run); Gofrt with
-rmin 0, rmax 12 A, 256 bins, lags 0..100, t0 stride 1, reset(1900),
-calculate(0).  One step = one whole Gofrt::calculate: every ordered atom pair
-for every (t0, lag) = 16384^2 * 1900 * 101 = 5.15e13 pair-lag updates.
+Default workload (the measured line): 16,384 atoms of 3 types (80/10/10 %),
+2,000 frames of a seeded Gaussian random walk (sigma 0.05 A per axis) in a
+triclinic cell with diagonal 56 A and tilts drawn uniformly in +-0.1 L (fixed
+per run); Gofrt with rmin 0, rmax 12 A, 256 bins, lags 0..100, t0 stride 1,
+reset(1900), calculate(0).  One step = one whole Gofrt::calculate: every
+ordered atom pair for every (t0, lag) = 16384^2 * 1900 * 101 = 5.15e13
+pair-lag updates.
 
-The timed value uses the screened form (16-bit fixed-point screen of every
-pair, exact FP64 evaluation of every candidate); the plain FP64 form is timed
-on a t0 slice of the same workload, and both forms' counts are compared on
-that slice.  Inputs are resident in HBM before timing starts.  The CPU
-baseline is the oracle's std::thread-style restatement on the host cores, on a
-bounded t0 slice of the same workload.
+The timed value uses the FP32 + FP64-recheck form (16-bit fixed-point screen
+of every pair, FP32 evaluation of every candidate, FP64 re-evaluation near a
+bin edge); the plain FP64 form is timed on a t0 slice of the same workload and
+both forms' counts are compared on that slice.  Inputs are resident in HBM
+before timing starts; the launch shape is autotuned (untimed) first.  The CPU
+baseline is the oracle's restatement of the reference loop nest on the host
+cores, on a bounded t0 slice of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Other workloads (--config), each printing one JSON line of its own:
+  c1  1,000 atoms 500/500, cubic 32.2 A, 200 frames: Gofrt rmax 10, 200 bins,
+      lags 0..50, reset(150); the CPU baseline runs it whole
+  c3  65,536 atoms 50/50, cubic 90 A, sigma 0.08, 1,000 frames: Gofrt rmax 15,
+      512 bins, lags 0..64, reset(936); MSD lags 0..500 with the
+      centre-of-mass channel and drift removal, reset(499)
+  c4  8,192 atoms 80/10/10, cubic 45 A, 60,000 frames; 64 seeded windows of
+      500..4000 frames: Gofrt lags 0..40 and MSD lags 0..400 of every window in
+      one graph launch, with the mean and variance over windows
+  c5  Green-Kubo, 2 vector currents (6 FP64 columns), 2,000,000 steps of
+      seeded AR(2) processes with cross-coupling 0.3, 32 uneven blocks
+      partitioning t0 in [0, 1,980,000), 20,000 lags per block, lag products
+      on the matrix cores (split-BF16 MFMA) vs the FP64 form
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config c2]
 """
 import argparse
 import ctypes
@@ -28,9 +44,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FLOP_PER_UPDATE_TRICLINIC = 19   # 3 sub + 6 min image + 3 tilt + 5 d2 + 2 gates
+FLOP_PER_UPDATE = {True: 19, False: 16}   # triclinic / orthorhombic pair-lag update
 FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+MSD_FLOP = 9                               # per atom-lag-t0 update, +3 with drift removal
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 class Hip:
@@ -50,7 +72,7 @@ class Hip:
 
     def malloc(self, nbytes):
         p = ctypes.c_void_p()
-        self._c(self.h.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)), "hipMalloc")
+        self._c(self.h.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(nbytes, 8))), "hipMalloc")
         return p
 
     def free(self, p):
@@ -77,26 +99,74 @@ class Hip:
         self._c(self.h.hipMemcpy(ctypes.c_void_p(host.ctypes.data), dev, ctypes.c_size_t(nbytes), 2), "hipMemcpy")
 
 
-def c2_workload(seed=2024):
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+def gofrt_workload(name):
     from analisi_amd import synth
-    N, F, L = 16384, 2000, 56.0
-    tilt = (2.0 * synth.uniform(seed, 7, 3) - 1.0) * 0.1 * L
+    if name == "c1":
+        seed, N, F, L, fr, sigma, tilt = 11, 1000, 200, 32.2, [0.5, 0.5], 0.05, None
+        g = dict(rmin=0.0, rmax=10.0, nbin=200, tmax=51, n=150)
+    elif name == "c2":
+        seed, N, F, L, fr, sigma = 2024, 16384, 2000, 56.0, [0.8, 0.1, 0.1], 0.05
+        tilt = (2.0 * synth.uniform(seed, 7, 3) - 1.0) * 0.1 * L
+        g = dict(rmin=0.0, rmax=12.0, nbin=256, tmax=101, n=1900)
+    elif name == "c3":
+        seed, N, F, L, fr, sigma, tilt = 33, 65536, 1000, 90.0, [0.5, 0.5], 0.08, None
+        g = dict(rmin=0.0, rmax=15.0, nbin=512, tmax=65, n=936, msd_tmax=501, msd_n=499)
+    elif name == "c4":
+        seed, N, F, L, fr, sigma, tilt = 44, 8192, 60000, 45.0, [0.8, 0.1, 0.1], 0.05, None
+        g = dict(rmin=0.0, rmax=12.0, nbin=256, tmax=41, n=0, msd_tmax=401, nblocks=64)
+    else:
+        raise ValueError(name)
     box_l = synth.lammps_box(L, tilt)
-    pos = synth.random_walk(seed, N, F, box_l, 0.05)
-    types = synth.types_by_fraction(N, [0.8, 0.1, 0.1])
-    return dict(pos=pos, types=types, box_l=box_l, tilt=tilt, N=N, F=F, L=L,
-                rmin=0.0, rmax=12.0, nbin=256, tmax=101, skip=1, every=1, n=1900, primo=0, seed=seed)
+    log("generating %s: N=%d F=%d" % (name, N, F))
+    pos = synth.random_walk(seed, N, F, box_l, sigma)
+    types = synth.types_by_fraction(N, fr)
+    w = dict(name=name, pos=pos, types=types, box_l=box_l, tilt=tilt, N=N, F=F, L=L, sigma=sigma, seed=seed,
+             triclinic=tilt is not None, skip=1, every=1, primo=0, fractions=fr)
+    w.update(g)
+    return w
+
+
+def describe(w):
+    cell = "triclinic, diagonal %g A, tilts %s" % (w["L"], np.round(w["tilt"], 4).tolist()) if w["triclinic"] \
+        else "cubic %g A" % w["L"]
+    return ("%d atoms %s, %d frames, %s, sigma %g A (seed %d)"
+            % (w["N"], "/".join("%g" % (100 * f) for f in w["fractions"]), w["F"], cell, w["sigma"], w["seed"]))
 
 
 def internal_box(box_l, F):
+    """LAMMPS box -> the internal [lo(3), half-length(3), tilt(3)] format."""
     b = np.array(box_l, dtype=np.float64).copy()
-    t = b[4]
-    b[5] = (b[5] - b[4]) / 2
-    b[4] = (b[3] - b[2]) / 2
-    b[3] = (b[1] - b[0]) / 2
-    b[1] = b[2]
-    b[2] = t
-    return np.tile(b, (F, 1))
+    if b.size == 6:
+        out = np.array([b[0], b[2], b[4], (b[1] - b[0]) / 2, (b[3] - b[2]) / 2, (b[5] - b[4]) / 2])
+    else:
+        out = np.array([b[0], b[2], b[4], (b[1] - b[0]) / 2, (b[3] - b[2]) / 2, (b[5] - b[4]) / 2,
+                        b[6], b[7], b[8]])
+    return np.tile(out, (F, 1))
+
+
+def type_ids(types):
+    """Dense type ids in sorted order of the type values (basetrajectory.cpp:51-89)."""
+    u, inv = np.unique(np.asarray(types), return_inverse=True)
+    return inv.astype(np.int32), len(u)
+
+
+def centres_of_mass(pos, tid, nt):
+    """Per-type centre of mass per frame, unit masses: [F][nt][3]."""
+    F = pos.shape[0]
+    cm = np.zeros((F, nt, 3))
+    for t in range(nt):
+        cm[:, t, :] = pos[:, tid == t, :].mean(axis=1)
+    return cm
+
+
+def self_channel(nt, t):
+    """Channel of the (t, t) self part: get_itype (gofrt.h:86-104) + nt(nt+1)/2."""
+    half = nt * (nt + 1) // 2
+    return half + half - (t + 1) * (t + 2) // 2 + t
 
 
 def n_updates(N, n, tmax, skip=1, every=1):
@@ -106,68 +176,64 @@ def n_updates(N, n, tmax, skip=1, every=1):
     return float(N) * N * ((n + skip - 1) // skip) * ((leff + every - 1) // every)
 
 
-def cpu_baseline(w, n, nthreads):
-    """The oracle (reference loop nest, one std::thread-style round per
-    (lag, t0)) on reset(n) of the same trajectory: n t0 values x min(n, 101) lags."""
+def device_traj(w, with_cm=False):
+    from analisi_amd import capi
+    tid, nt = type_ids(w["types"])
+    cm = centres_of_mass(w["pos"], tid, nt) if with_cm else None
+    boxi = internal_box(w["box_l"], w["F"])
+    return capi.DeviceTrajectory(w["pos"], boxi, tid, nt, cm), tid, nt
+
+
+def cpu_gofrt(w, n, nthreads, F_used=None):
+    """The oracle (reference loop nest) on reset(n) of the same trajectory."""
     from oracle import oracle as O
-    F = w["tmax"] + n
-    tr = O.Traj(w["pos"][:F], w["types"], np.tile(w["box_l"], (F, 1)), True)
+    F = F_used or (min(n, w["tmax"]) + n)
+    tr = O.Traj(w["pos"][:F], w["types"], np.tile(w["box_l"], (F, 1)), w["triclinic"])
     t = time.perf_counter()
     counts, _ = O.gofrt(tr, w["rmin"], w["rmax"], w["nbin"], w["tmax"], nthreads, 1, 1, n, 0, want_vdata=True)
     dt = time.perf_counter() - t
     return n_updates(w["N"], n, w["tmax"]) / dt, dt, counts
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=8, help="reset(n) of the CPU baseline sample (n t0 x n lags)")
-    ap.add_argument("--fp64-n", type=int, default=101, help="reset(n) of the plain-FP64 slice (n t0 x 101 lags)")
-    ap.add_argument("--n", type=int, default=1900, help="reset(n) of the timed run (1900 = the full workload)")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only copy MIB mebibytes device-to-device (a plain copy kernel of known size, "
-                         "to scale the FETCH_SIZE/WRITE_SIZE counters) and exit")
-    args = ap.parse_args()
-    if args.calibrate_copy:
-        hip = Hip()
-        nb = args.calibrate_copy << 20
-        src, dst = hip.malloc(nb), hip.malloc(nb)
-        for _ in range(3):
-            hip.h.hipMemcpyDtoD(dst, src, ctypes.c_size_t(nb))
-        hip.sync()
-        print(json.dumps({"copy_bytes": nb, "copies": 3}))
-        return
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
-        sys.exit(2)
+def traffic_from_profile(kernel_prefix, workload):
+    """HBM bytes per launch of a kernel, from a committed rocprofv3 counter
+    summary under profiles/ (lines 'kernel workload bytes_per_launch')."""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and not line.startswith("#") and parts[0].startswith(kernel_prefix) \
+                    and parts[1] == workload:
+                return float(parts[2])
+    return None
 
+
+# ---------------------------------------------------------------------------
+# g(r,t) (c1, c2, c3)
+# ---------------------------------------------------------------------------
+
+def run_gofrt(w, args, hip, res_extra=None):
     from analisi_amd import capi
-    w = c2_workload()
-    w["n"] = args.n
-    N, F = w["N"], w["F"]
-    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
-    log("workload generated: N=%d F=%d tilt=%s" % (N, F, np.round(w["tilt"], 4).tolist()))
-
-    hip = Hip()
     check = capi.check
     L = capi.lib()
-    check(L.anl_init(0))
-    from oracle.oracle import lib as olib, _p, _ip
-    tid = np.zeros(N, np.int32)
-    nt = int(olib().orc_type_ids(_p(w["types"], _ip), N, _p(tid, _ip)))
-    boxi = internal_box(w["box_l"], F)
-    traj = capi.DeviceTrajectory(w["pos"], boxi, tid, nt, None)
+    traj, tid, nt = device_traj(w, with_cm=("msd_tmax" in w))
+    N, F = w["N"], w["F"]
     g = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], w["skip"], w["every"], capi.GOFRT_SCREENED)
     leff = g.reset(w["n"])
+    tune = None
+    if not args.no_autotune:
+        nl = max(1, min(leff, args.tune_lags))
+        ms, best = g.autotune(w["primo"], nl)
+        tune = {"probe_lags": nl, "candidates_ms": [round(float(x), 2) for x in ms], "best": int(best),
+                "chosen": list(g.tuning())}
+        log("autotune on %d lags: %s -> %s" % (nl, tune["candidates_ms"], tune["chosen"]))
     nch = nt * (nt + 1)
     out_len = leff * nch * w["nbin"]
     d_out = hip.malloc(out_len * 8)
     stream = hip.stream()
     updates = n_updates(N, w["n"], w["tmax"], w["skip"], w["every"])
-
     for i in range(args.warmup):
         check(L.anl_gofrt_calculate_async(g.handle, w["primo"], d_out, stream))
         hip.sync()
@@ -187,20 +253,22 @@ def main():
     counts = np.zeros((leff, nch, w["nbin"]), dtype=np.uint64)
     hip.d2h(counts, d_out, out_len * 8)
     _, ncand, nre = g.stats()
-    # invariants of the full run: lag-0 self channels hold N_type per t0 in bin 0
+    # invariants of the full run: lag-0 self channels hold N_type per t0 in
+    # bin 0, the pair channels are symmetric (even counts at lag 0)
     ntype = np.bincount(tid, minlength=nt)
     half = nt * (nt + 1) // 2
     inv_ok = True
     for t in range(nt):
-        ch = int(olib().orc_gofrt_channel(nt, t, t, 1))
+        ch = self_channel(nt, t)
         inv_ok &= int(counts[0, ch, 0]) == int(ntype[t]) * w["n"] and int(counts[0, ch, 1:].sum()) == 0
     inv_ok &= bool(np.all(counts[0, :half] % 2 == 0))
 
     # plain FP64 form on a t0 slice of the same workload + bit-exact check
-    n_sl = args.fp64_n
+    n_sl = min(args.fp64_n, w["n"])
     g64 = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_FP64)
     g64.reset(n_sl)
     gsc = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_SCREENED)
+    gsc.set_tuning(*g.tuning())
     gsc.reset(n_sl)
     leff_sl = g64.leff
     len_sl = leff_sl * nch * w["nbin"]
@@ -221,7 +289,8 @@ def main():
     log("fp64 slice: %d t0, %.1f ms, %.3e upd/s, identical=%s" % (n_sl, ms64, rate64, slice_identical))
 
     rate = updates / (ms * 1e-3)
-    achieved = updates * FLOP_PER_UPDATE_TRICLINIC / (ms * 1e-3) / 1e12
+    fl = FLOP_PER_UPDATE[w["triclinic"]]
+    achieved = updates * fl / (ms * 1e-3) / 1e12
     res = {
         "metric": "g(r,t) pair-lag updates/s, FP32+recheck vs plain FP64, bit-exact counts",
         "value": rate,
@@ -233,39 +302,315 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i16 screen + f64 candidates",
+        "dtype": "i16 screen + f32 candidates + f64 recheck",
         "data": "synthetic (seeded SplitMix64/Box-Muller random walk, seed %d)" % w["seed"],
-        "config": {"workload": "Gofrt g(r,t) triclinic: 16384 atoms 80/10/10, 2000 frames, L=56 A, tilts U(+-0.1L), "
-                               "sigma 0.05 A, rmax 12 A, 256 bins, lags 0..100, t0 stride 1, reset(%d)" % w["n"],
-                   "form": "screened", "natoms": N, "frames": F, "lags": leff, "t0": w["n"], "nbin": w["nbin"],
-                   "updates_per_step": updates},
+        "config": {"workload": "Gofrt g(r,t): %s; rmax %g A, %d bins, lags 0..%d, t0 stride 1, reset(%d)"
+                               % (describe(w), w["rmax"], w["nbin"], leff - 1, w["n"]),
+                   "form": "fp32+recheck", "natoms": N, "frames": F, "lags": leff, "t0": w["n"], "nbin": w["nbin"],
+                   "updates_per_step": updates, "launch_shape": list(g.tuning())},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "note": "VALU-bound pair kernel: 19 algorithmic FLOP per triclinic pair-lag update "
-                             "(3 sub, 6 min image, 3 tilt, 5 d2, 2 gates) over the event-timed kernel; "
-                             "peak = MI355X FP32 157.3 TFLOP/s"},
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic_from_profile("gofrt_screened", w["name"]),
+                     "note": "VALU-bound pair kernel (no MFMA work): %d algorithmic FLOP per pair-lag update "
+                             "(3 sub, 6 min image, %s5 d2, 2 gates) over the event-timed kernel; peak = MI355X "
+                             "vector FP32 157.3 TFLOP/s" % (fl, "3 tilt, " if w["triclinic"] else "")},
         "fp64_plain": {"value": rate64, "unit": "pair-lag updates/s",
                        "sample": "same workload, reset(%d): %d t0 x %d lags" % (n_sl, n_sl, leff_sl), "ms": ms64,
-                       "speedup_screened_vs_fp64": rate / rate64, "counts_identical_on_slice": slice_identical},
+                       "roofline_frac": upd64 * fl / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                       "speedup_fp32_recheck_vs_fp64": rate / rate64, "counts_identical_on_slice": slice_identical},
         "screen": {"candidates_per_step": int(ncand), "candidate_fraction": ncand / updates,
                    "fp64_rechecks_per_step": int(nre), "recheck_fraction_of_candidates": nre / max(ncand, 1)},
         "checks": {"lag0_self_channels_and_pair_symmetry": bool(inv_ok)},
+        "autotune": tune,
         "wall_s_timed": t_wall,
     }
+    if w["name"] != "c2":
+        res["metric"] = "g(r,t) pair-lag updates/s (%s)" % w["name"]
     if not args.no_cpu:
         nthr = min(os.cpu_count() or 1, 16)
-        cr, cdt, ccounts = cpu_baseline(w, args.cpu_n, nthr)
-        # the oracle sample must equal the device result on the same reset(n)
-        gsl = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_SCREENED)
-        gsl.reset(args.cpu_n)
-        same = bool(np.array_equal(gsl.calculate(0), ccounts))
+        cn = w["n"] if w["name"] == "c1" else args.cpu_n
+        cr, cdt, ccounts = cpu_gofrt(w, cn, nthr, F_used=F if w["name"] == "c1" else None)
+        if cn == w["n"]:
+            same = bool(np.array_equal(counts, ccounts))
+        else:   # the oracle sample must equal the device result on the same reset(n)
+            gsl = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_SCREENED)
+            gsl.reset(cn)
+            same = bool(np.array_equal(gsl.calculate(0), ccounts))
         res["cpu_baseline"] = {"value": cr, "unit": "pair-lag updates/s", "cores": nthr, "kind": "port",
                                "sample": "same trajectory and histogram, reset(%d): %d t0 x %d lags, %.1f s"
-                                         % (args.cpu_n, args.cpu_n, min(args.cpu_n, w["tmax"]), cdt),
+                                         % (cn, cn, min(cn, w["tmax"]), cdt),
                                "counts_identical_to_gpu": same}
         log("cpu baseline %.3e upd/s on %d threads (%.1f s), identical=%s" % (cr, nthr, cdt, same))
     for p in (d_out, d64):
         hip.free(p)
+    if "msd_tmax" in w:
+        res["msd"] = run_msd(w, args, hip, traj, nt)
+    return res
+
+
+def run_msd(w, args, hip, traj, nt):
+    """MSD with the centre-of-mass channel and drift removal, timed on the
+    device; the FP64 oracle error on a t0 slice."""
+    from analisi_amd import capi
+    check = capi.check
+    L = capi.lib()
+    n, tmax = w["msd_n"], w["msd_tmax"]
+    leff = n if n < tmax else tmax
+    out_len = leff * 2 * nt
+    d = hip.malloc(out_len * 8)
+    stream = hip.stream()
+    check(L.anl_msd_calculate_async(traj.handle, 1, tmax, 1, 1, n, 0, d, stream))
+    hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    reps = max(1, args.steps)
+    hip.record(e0, stream)
+    for _ in range(reps):
+        check(L.anl_msd_calculate_async(traj.handle, 1, tmax, 1, 1, n, 0, d, stream))
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1) / reps
+    upd = float(w["N"]) * n * leff
+    flop = MSD_FLOP + 3
+    res = {"metric": "MSD atom-lag-t0 updates/s", "value": upd / (ms * 1e-3), "ms_per_step": ms,
+           "config": "MSD lags 0..%d, reset(%d), centre-of-mass channel + drift removal, skip 1" % (leff - 1, n),
+           "roofline": {"bound": "fp64 valu", "achieved": upd * flop / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": upd * flop / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                        "note": "%d FLOP per atom-lag-t0 update (3 sub, 3 mul, 2 add, 1 accumulate, "
+                                "3 drift)" % flop}}
+    hip.free(d)
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ns = args.msd_cpu_n
+        F = ns + min(ns, tmax)
+        tr = O.Traj(w["pos"][:F], w["types"], np.tile(w["box_l"], (F, 1)), w["triclinic"])
+        nthr = min(os.cpu_count() or 1, 16)
+        t = time.perf_counter()
+        ref = O.msd(tr, 1, tmax, nthr, True, True, ns, 0)
+        dt = time.perf_counter() - t
+        got = capi.msd(traj, 1, tmax, True, True, ns, 0)
+        err = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))
+        res["cpu_baseline"] = {"value": float(w["N"]) * ns * min(ns, tmax) / dt, "unit": "atom-lag-t0 updates/s",
+                               "cores": nthr, "kind": "port",
+                               "sample": "same trajectory, reset(%d), %.1f s" % (ns, dt),
+                               "max_rel_error_vs_fp64_oracle": err}
+        log("msd: %.1f ms, oracle slice max rel error %.2e" % (ms, err))
+    return res
+
+
+# ---------------------------------------------------------------------------
+# block engine (c4)
+# ---------------------------------------------------------------------------
+
+def c4_windows(w):
+    from analisi_amd import synth
+    nb, F = w["nblocks"], w["F"]
+    u1 = synth.uniform(w["seed"], 21, nb)
+    u2 = synth.uniform(w["seed"], 22, nb)
+    lens = (500 + np.floor(u1 * 3501)).astype(np.int64).clip(500, 4000)
+    starts = np.floor(u2 * (F - lens - 401)).astype(np.int64)
+    return starts.astype(np.uint64), lens.astype(np.uint32)
+
+
+def run_blocks(w, args, hip):
+    from analisi_amd import capi
+    traj, tid, nt = device_traj(w, with_cm=False)
+    starts, lens = c4_windows(w)
+    g = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_SCREENED)
+    tune = None
+    if not args.no_autotune:
+        g.reset(int(lens.min()))
+        ms_c, best = g.autotune(int(starts[0]), 2)
+        tune = {"candidates_ms": [round(float(x), 2) for x in ms_c], "chosen": list(g.tuning())}
+        log("autotune: %s -> %s" % (tune["candidates_ms"], tune["chosen"]))
+    t = time.perf_counter()
+    B = capi.DeviceBlocks(traj, starts, lens, gofrt=g, msd=True, msd_skip=1, msd_tmax=w["msd_tmax"], use_graph=True)
+    t_create = time.perf_counter() - t
+    stream = hip.stream()
+    B.run(stream)
+    hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    reps = max(1, args.steps)
+    hip.record(e0, stream)
+    for _ in range(reps):
+        B.run(stream)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1) / reps
+    gmean, gvar, gcounts = B.gofrt_results()
+    mmean, mvar, mper = B.msd_results()
+    N = w["N"]
+    upd = sum(n_updates(N, int(n), w["tmax"]) for n in lens)
+    mupd = sum(float(N) * int(n) * min(int(n), w["msd_tmax"]) for n in lens)
+    # one window recomputed alone: FP64 g(r,t) and MSD, bit-identical
+    b = int(np.argmin(lens))
+    g64 = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_FP64)
+    g64.reset(int(lens[b]))
+    same_g = bool(np.array_equal(g64.calculate(int(starts[b])), gcounts[b]))
+    m1 = capi.msd(traj, 1, w["msd_tmax"], False, False, int(lens[b]), int(starts[b]))
+    same_m = bool(np.array_equal(m1, mper[b]))
+    res = {"metric": "block-averaged g(r,t) + MSD, pair-lag updates/s over all windows",
+           "value": upd / (ms * 1e-3), "unit": "pair-lag updates/s", "n_gpus": 1, "ms_per_step": ms,
+           "steps": reps, "dtype": "i16 screen + f32 + f64 recheck; MSD f64",
+           "config": {"workload": "block engine: %s; %d windows of 500..4000 frames; Gofrt rmax %g, %d bins, "
+                                  "lags 0..%d; MSD lags 0..%d; one HIP graph per step"
+                                  % (describe(w), len(lens), w["rmax"], w["nbin"], w["tmax"] - 1, w["msd_tmax"] - 1),
+                      "sum_window_frames": int(lens.sum()), "gofrt_updates": upd, "msd_updates": mupd},
+           "setup_s": t_create, "autotune": tune,
+           "checks": {"window_%d_gofrt_fp64_identical" % b: same_g, "window_%d_msd_identical" % b: same_m}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        mv = O.MediaVar(gmean.size)
+        incr = [1.0 / int(n) for n in lens]
+        for k in range(len(lens)):
+            mv.add(gcounts[k].ravel().astype(np.float64) * incr[k])
+        mv.end(len(lens))
+        mv2 = O.MediaVar(mmean.size)
+        for k in range(len(lens)):
+            mv2.add(mper[k].ravel())
+        mv2.end(len(lens))
+        res["cpu_baseline"] = None
+        res["checks"]["mediavar_identical_to_oracle"] = bool(
+            np.array_equal(gmean.ravel(), mv.mean) and np.array_equal(gvar.ravel(), mv.var)
+            and np.array_equal(mmean.ravel(), mv2.mean) and np.array_equal(mvar.ravel(), mv2.var))
+    log("blocks: %.1f ms/step, checks %s" % (ms, res["checks"]))
+    B.close()
+    return res
+
+
+# ---------------------------------------------------------------------------
+# Green-Kubo (c5)
+# ---------------------------------------------------------------------------
+
+def c5_blocks(seed, T, lmax, nb):
+    from analisi_amd import synth
+    span = T - lmax
+    u = synth.uniform(seed, 31, nb)
+    extra = span - nb * lmax
+    lens = lmax + np.floor(extra * u / u.sum()).astype(np.int64)
+    lens[-1] += span - lens.sum()
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    return starts, lens
+
+
+def run_gk(args, hip):
+    from analisi_amd import capi, synth
+    check = capi.check
+    L = capi.lib()
+    T, lmax, nb, seed = 2_000_000, 20_000, 32, 55
+    log("generating AR(2) currents, %d steps" % T)
+    J = synth.ar2_currents(seed, T, ncur=2, coupling=0.3)
+    starts, lens = c5_blocks(seed, T, lmax, nb)
+    s = capi.DeviceSeries(J)
+    idx = np.array([0, 3], dtype=np.uint32)
+    pidx = ctypes.c_void_p(idx.ctypes.data)
+    narr = 14
+    outs_m = [hip.malloc(lmax * narr * 8) for _ in range(nb)]
+    outs_f = [hip.malloc(lmax * narr * 8) for _ in range(nb)]
+    stream = hip.stream()
+
+    def enqueue(fn, outs):
+        for b in range(nb):
+            check(fn(s.handle, pidx, 2, 1, lmax, int(lens[b]), 1, 0, 0, int(starts[b]), outs[b], stream))
+
+    big = int(np.argmax(lens))   # sizes the series scratch once, outside the timed region
+    check(L.anl_gk_calculate_mfma_async(s.handle, pidx, 2, 1, lmax, int(lens[big]), 1, 0, 0, int(starts[big]),
+                                        outs_m[big], stream))
+    check(L.anl_gk_calculate_async(s.handle, pidx, 2, 1, lmax, int(lens[big]), 1, 0, 0, int(starts[big]),
+                                   outs_f[big], stream))
+    hip.sync()
+    e0, e1, e2 = hip.event(), hip.event(), hip.event()
+    reps = max(1, args.steps)
+    hip.record(e0, stream)
+    for _ in range(reps):
+        enqueue(L.anl_gk_calculate_mfma_async, outs_m)
+    hip.record(e1, stream)
+    ms_m = hip.elapsed_ms(e0, e1) / reps
+    hip.record(e1, stream)
+    enqueue(L.anl_gk_calculate_async, outs_f)
+    hip.record(e2, stream)
+    ms_f = hip.elapsed_ms(e1, e2)
+    errC, errI, errK = 0.0, 0.0, 0.0
+    a = np.zeros((lmax, narr))
+    b_ = np.zeros((lmax, narr))
+    for b in range(nb):
+        hip.d2h(a, outs_m[b], a.nbytes)
+        hip.d2h(b_, outs_f[b], b_.nbytes)
+        c0 = np.sqrt(np.abs(np.diag(b_[0, :4].reshape(2, 2))))
+        sig = np.outer(c0, c0).reshape(-1)
+        errC = max(errC, float(np.max(np.abs(a[:, :4] - b_[:, :4]) / sig)))
+        li = np.arange(lmax)[:, None] + 1.0
+        errI = max(errI, float(np.max(np.abs(a[:, 4:12:2] - b_[:, 4:12:2]) / sig / li)))
+        k = np.abs(b_[1:, 12])
+        errK = max(errK, float(np.max(np.abs(a[1:, 12] - b_[1:, 12]) / np.maximum(k, k.max() * 1e-3))))
+    for p in outs_m + outs_f:
+        hip.free(p)
+    flop = 2.0 * 3 * 4 * float(lens.sum()) * lmax
+    res = {"metric": "Green-Kubo lag contraction FLOP/s (algorithmic 2*3*ncur^2*sum(n0)*lags)",
+           "value": flop / (ms_m * 1e-3), "unit": "FLOP/s", "n_gpus": 1, "steps": reps, "ms_per_step": ms_m,
+           "dtype": "split-bf16 mfma, f32 chunk, f64 reduce",
+           "config": {"workload": "GK 2 vector currents (6 FP64 columns), %d AR(2) steps, coupling 0.3, %d uneven "
+                                  "blocks, %d lags, skip 1, n_seg 1" % (T, nb, lmax),
+                      "block_lengths_min_max": [int(lens.min()), int(lens.max())], "algorithmic_flop": flop},
+           "ms_all_blocks_fp64": ms_f, "speedup_mfma_vs_fp64": ms_f / ms_m,
+           "roofline": {"bound": "mfma", "achieved": 3 * flop / (ms_m * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": 3 * flop / (ms_m * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+                        "note": "3 bf16 MFMA products per algorithmic product; event-timed over all 32 blocks "
+                                "(splits, tiles, reduction and the FP64 epilogue)"},
+           "error_vs_fp64": {"max_dC_over_sigma": errC, "max_dI_over_sigma_l": errI,
+                             "max_rel_dk_gk": errK,
+                             "note": "sigma_ab = sqrt(C_aa(0) C_bb(0)) of the FP64 form (itself identical to the "
+                                     "oracle); dk_gk relative to max(|k|, 1e-3 max|k|)"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        n_s = 400
+        t = time.perf_counter()
+        O.gk(J, [0, 3], 1, lmax, False, 0, 1, n_s, 0)
+        dt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": 2.0 * 3 * 4 * n_s * lmax / dt, "unit": "FLOP/s", "cores": 1, "kind": "port",
+                               "sample": "GreenKubo reset(%d) with 20,000 lags on the same series, %.1f s"
+                                         % (n_s, dt)}
+    log("gk: mfma %.2f ms, fp64 %.2f ms, max dC/sigma %.2e" % (ms_m, ms_f, errC))
+    s.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune-lags", type=int, default=4, help="lags per autotune probe launch")
+    ap.add_argument("--cpu-n", type=int, default=8, help="reset(n) of the CPU baseline sample (n t0 x n lags)")
+    ap.add_argument("--msd-cpu-n", type=int, default=20, help="reset(n) of the MSD oracle sample")
+    ap.add_argument("--fp64-n", type=int, default=101, help="reset(n) of the plain-FP64 slice")
+    ap.add_argument("--n", type=int, default=0, help="reset(n) of the timed run (0 = the workload's own)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB mebibytes device-to-device (a plain copy kernel of known size, "
+                         "to scale the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        hip = Hip()
+        nb = args.calibrate_copy << 20
+        src, dst = hip.malloc(nb), hip.malloc(nb)
+        for _ in range(3):
+            hip.h.hipMemcpyDtoD(dst, src, ctypes.c_size_t(nb))
+        hip.sync()
+        print(json.dumps({"copy_bytes": nb, "copies": 3}))
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    from analisi_amd import capi
+    hip = Hip()
+    capi.check(capi.lib().anl_init(0))
+    if args.config == "c5":
+        res = run_gk(args, hip)
+    else:
+        w = gofrt_workload(args.config)
+        if args.n:
+            w["n"] = args.n
+        res = run_blocks(w, args, hip) if args.config == "c4" else run_gofrt(w, args, hip)
     print(json.dumps(res), flush=True)
 
 

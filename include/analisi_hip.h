@@ -169,6 +169,15 @@ int anl_gk_calculate(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32
 int anl_gk_calculate_mfma(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
                           uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
                           uint32_t start_mean, uint64_t primo, double *out_host);
+/* Stream-ordered variants writing [leff][3*ncur^2+2] FP64 to device memory.
+ * Scratch lives in the series (grow-only; growing synchronises the device),
+ * so the calls on one series must be ordered on one stream. */
+int anl_gk_calculate_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
+                           uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
+                           uint32_t start_mean, uint64_t primo, double *out_dev, void *stream);
+int anl_gk_calculate_mfma_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
+                                uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
+                                uint32_t start_mean, uint64_t primo, double *out_dev, void *stream);
 
 /* ---- block statistics --------------------------------------------------- */
 /* MediaVar::calculate (calcoliblocchi.h:35-58) on device vectors of length
