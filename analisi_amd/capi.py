@@ -24,6 +24,8 @@ ANL_ERR_HEADER = 6
 GOFRT_FP64 = 0
 GOFRT_SCREENED = 1
 GOFRT_AUTO = 2
+GK_FP64 = 0
+GK_MFMA = 1
 
 # (name, restype, argtypes) of every exported entry point
 _vp = ctypes.c_void_p
@@ -60,6 +62,8 @@ SIGNATURES = [
     ("anl_series_destroy", _i, [_vp]),
     ("anl_gk_calculate", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
     ("anl_gk_calculate_mfma", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
+    ("anl_gk_calculate_blocks_async", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _i, _u32, _vp, _vp, _u32, _i, _vp, _vp,
+                                          _vp, _vp]),
     ("anl_gk_calculate_async", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp, _vp]),
     ("anl_gk_calculate_mfma_async", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp, _vp]),
     ("anl_mediavar_update_async", _i, [_vp, _vp, _vp, _u64, _u32, _vp]),
@@ -261,6 +265,38 @@ def msd(traj, skip, tmax, cm_msd, cm_self, n, primo):
     return out
 
 
+class DeviceBuffer:
+    """Plain device memory (hipMalloc) for the stream-ordered entry points."""
+    _hip = None
+
+    def __init__(self, nbytes):
+        if DeviceBuffer._hip is None:
+            DeviceBuffer._hip = ctypes.CDLL("libamdhip64.so")
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        if DeviceBuffer._hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(self.nbytes, 8))) != 0:
+            raise AnlError(ANL_ERR_HIP, "hipMalloc of %d bytes failed" % self.nbytes)
+
+    def to_host(self, shape, dtype=np.float64):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        if DeviceBuffer._hip.hipDeviceSynchronize() != 0 or DeviceBuffer._hip.hipMemcpy(
+                ctypes.c_void_p(out.ctypes.data), self.ptr, ctypes.c_size_t(out.nbytes), 2) != 0:
+            raise AnlError(ANL_ERR_HIP, "device to host copy failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            DeviceBuffer._hip.hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class DeviceSeries:
     def __init__(self, data):
         self.data = np.ascontiguousarray(data, dtype=np.float64)
@@ -279,6 +315,28 @@ class DeviceSeries:
         check(fn(self.handle, _ptr(idx), ncur, skip, tmax, n, n_seg, int(bool(subtract_mean)), start_mean, primo,
                  _ptr(out)))
         return out
+
+    def gk_blocks(self, idx_j, skip, tmax, n_seg, subtract_mean, start_mean, primo, n, mfma=False, stats=True):
+        """GreenKubo of every block b = reset(n[b]) + calculate(primo[b]) in one
+        call; returns (per-block [nb][leff][narr], mean, variance of the mean)."""
+        idx = np.ascontiguousarray(idx_j, dtype=np.uint32)
+        ncur = len(idx)
+        pr = np.ascontiguousarray(primo, dtype=np.uint64)
+        nn = np.ascontiguousarray(n, dtype=np.uint32)
+        nb = len(pr)
+        leff = int(nn[0]) if (int(nn[0]) < tmax or tmax == 0) else tmax
+        narr = 3 * ncur * ncur + 2
+        out = DeviceBuffer(nb * leff * narr * 8)
+        mean = DeviceBuffer(leff * narr * 8) if stats else None
+        var = DeviceBuffer(leff * narr * 8) if stats else None
+        check(lib().anl_gk_calculate_blocks_async(self.handle, _ptr(idx), ncur, skip, tmax, n_seg,
+                                                  int(bool(subtract_mean)), start_mean, _ptr(pr), _ptr(nn), nb,
+                                                  GK_MFMA if mfma else GK_FP64, out.ptr,
+                                                  mean.ptr if stats else None, var.ptr if stats else None, None))
+        res = out.to_host((nb, leff, narr))
+        if not stats:
+            return res, None, None
+        return res, mean.to_host((leff, narr)), var.to_host((leff, narr))
 
     def close(self):
         if self.handle:

@@ -124,5 +124,15 @@ void *series_workspace(anl_series *s, size_t bytes);
 // FP64 Green-Kubo epilogue (mean subtraction, running integrals, Schur
 // complement) over d[leff][narr] whose first ncur^2 columns hold C(l).
 void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
-                 hipStream_t s);
+                 hipStream_t s, uint32_t nblocks, uint64_t stride);
+// Lag products of every block (split-BF16 MFMA) into d + b * leff * narr:
+// C(l) in the first ncur^2 columns, zeros elsewhere; scratch from the series.
+void gk_mfma_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
+                            uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, double *d,
+                            hipStream_t st);
+size_t gk_mfma_scratch_bytes(uint32_t ncur, uint32_t leff, uint32_t n);
+// MediaVar over blocks (calcoliblocchi.h:21-65) of x_b = counts_b * incr_b or
+// values_b, element-parallel, Welford steps in block order.
+void mediavar_blocks(const unsigned long long *counts, const double *values, const double *incr, uint32_t nblocks,
+                     uint64_t len, double *mean, double *var, hipStream_t s);
 }  // namespace anl

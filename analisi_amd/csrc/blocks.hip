@@ -36,6 +36,14 @@ __global__ void mediavar_blocks_kernel(const unsigned long long *__restrict__ co
     var[k] = v / (double)((nblocks - 1) * nblocks);
 }
 
+void mediavar_blocks(const unsigned long long *counts, const double *values, const double *incr, uint32_t nblocks,
+                     uint64_t len, double *mean, double *var, hipStream_t s) {
+    if (len == 0) return;
+    hipLaunchKernelGGL(mediavar_blocks_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, counts, values,
+                       incr, nblocks, len, mean, var);
+    ANL_HIP(hipGetLastError());
+}
+
 }  // namespace anl
 
 using namespace anl;
@@ -62,15 +70,11 @@ struct anl_blocks {
     void enqueue(hipStream_t s) {
         if (gprep) {
             gofrt_enqueue_prepared(g, gprep, d_counts, s);
-            hipLaunchKernelGGL(mediavar_blocks_kernel, dim3((unsigned)((g_len + 255) / 256)), dim3(256), 0, s,
-                               d_counts, nullptr, d_incr, nblocks, g_len, d_gmean, d_gvar);
-            ANL_HIP(hipGetLastError());
+            mediavar_blocks(d_counts, nullptr, d_incr, nblocks, g_len, d_gmean, d_gvar, s);
         }
         if (mprep) {
             msd_enqueue_prepared(mprep, d_msd, s);
-            hipLaunchKernelGGL(mediavar_blocks_kernel, dim3((unsigned)((m_len + 255) / 256)), dim3(256), 0, s,
-                               nullptr, d_msd, nullptr, nblocks, m_len, d_mmean, d_mvar);
-            ANL_HIP(hipGetLastError());
+            mediavar_blocks(nullptr, d_msd, nullptr, nblocks, m_len, d_mmean, d_mvar, s);
         }
     }
     ~anl_blocks() {

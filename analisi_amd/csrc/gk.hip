@@ -21,25 +21,38 @@ namespace anl {
 
 constexpr int GK_MAXC = 4;  // up to 4 vector currents (16 correlation pairs)
 
+constexpr uint32_t GK_MAXB = 64;  // blocks per launch
+
+// Per-block t0 ranges of one launch: block b starts at row primo[b] and
+// averages over n[b] t0 values; its output is out + b * out_stride.
+struct GkBlocks {
+    uint64_t primo[GK_MAXB];
+    uint32_t n[GK_MAXB];
+};
+
 struct GkArgs {
     const double *data;
     uint32_t ncols;
     uint32_t ncur;
     uint32_t idx[GK_MAXC];
-    uint64_t primo;
-    uint32_t n, n_seg, skip, leff, narr;
-    double *out;  // [leff][narr]
+    uint32_t n_seg, skip, leff, narr;
+    uint64_t out_stride;
+    double *out;  // [block][leff][narr]
+    GkBlocks bl;
 };
 
 template <int NC>
 __global__ __launch_bounds__(256) void gk_lag_kernel(const GkArgs a) {
     const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= a.leff) return;
+    const uint32_t blk = blockIdx.y;
+    const uint64_t primo = a.bl.primo[blk];
+    const uint32_t n = a.bl.n[blk];
     constexpr int NP = NC * NC;
     double v[NP], JJ[NP];
 #pragma unroll
     for (int k = 0; k < NP; k++) v[k] = 0.0;
-    const uint32_t seg = a.n / a.n_seg;
+    const uint32_t seg = n / a.n_seg;
     uint32_t cont_JJ = 0;
     for (uint32_t iseg = 0; iseg < a.n_seg; iseg++) {
         ++cont_JJ;
@@ -49,8 +62,8 @@ __global__ __launch_bounds__(256) void gk_lag_kernel(const GkArgs a) {
         const uint32_t allinea = (seg * iseg) + (seg * iseg) % a.skip;
         for (uint32_t jm = allinea; jm < seg * (iseg + 1); jm += a.skip) {
             cont++;
-            const double *la = a.data + (a.primo + jm) * a.ncols;
-            const double *lb = a.data + (a.primo + jm + it) * a.ncols;
+            const double *la = a.data + (primo + jm) * a.ncols;
+            const double *lb = a.data + (primo + jm + it) * a.ncols;
             double A[NC][3], B[NC][3];
 #pragma unroll
             for (int c = 0; c < NC; c++)
@@ -76,39 +89,75 @@ __global__ __launch_bounds__(256) void gk_lag_kernel(const GkArgs a) {
 #pragma unroll
         for (int k = 0; k < NP; k++) v[k] = v[k] + (JJ[k] - v[k]) / dcj;
     }
-    double *o = a.out + (uint64_t)it * a.narr;
+    double *o = a.out + blk * a.out_stride + (uint64_t)it * a.narr;
 #pragma unroll
     for (int k = 0; k < NP; k++) o[k] = v[k];
     for (uint32_t k = NP; k < a.narr; k++) o[k] = 0.0;
 }
 
-// :259-277 subtract the mean of C over lags [start_mean, leff)
+// :259-277 subtract the mean of C over lags [start_mean, leff).  One thread per
+// column; the serial running mean keeps the reference's order, loads run
+// ahead in batches of 16 lags.
 __global__ void gk_mean_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr, uint32_t leff,
-                               uint32_t start_mean) {
+                               uint32_t start_mean, uint64_t stride) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ncorr) return;
+    out += blockIdx.y * stride;
     double m = 0.0;
     uint32_t c = 0;
-    for (uint32_t it = start_mean; it < leff; it++) {
-        c++;
-        const double d = out[(uint64_t)it * narr + j] - m;
-        m = m + d / (double)c;
+    constexpr uint32_t B = 16;
+    for (uint32_t it0 = start_mean; it0 < leff; it0 += B) {
+        double v[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) v[k] = it0 + k < leff ? out[(uint64_t)(it0 + k) * narr + j] : 0.0;
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) {
+            if (it0 + k >= leff) break;
+            c++;
+            const double d = v[k] - m;
+            m = m + d / (double)c;
+        }
     }
     for (uint32_t it = 0; it < leff; it++) out[(uint64_t)it * narr + j] = out[(uint64_t)it * narr + j] - m;
 }
 
-// :279-288 trapezoid I and Einstein E running integrals, per correlation column
-__global__ void gk_integral_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr, uint32_t leff) {
+// :279-288 trapezoid I and Einstein E running integrals, per correlation
+// column: I(it) = (I(it-1) + C(it-1)/2) + C(it)/2, E(it) = (E(it-1) +
+// C(it-1) it/2) + C(it) it/2, with I(0) = E(0) = 0 as left by the lag loop.
+// The running values stay in registers (the reference's order); the C(it)
+// loads of the next 32 lags are in flight while the current 32 are summed.
+__global__ void gk_integral_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr, uint32_t leff,
+                                   uint64_t stride) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncorr) return;
-    for (uint32_t it = 1; it < leff; it++) {
-        double *cur = out + (uint64_t)it * narr;
-        const double *prev = out + (uint64_t)(it - 1) * narr;
-        const double dit = (double)it;
-        double I = prev[ncorr + 2 * j] + prev[j] / 2.0;
-        cur[ncorr + 2 * j] = I + cur[j] / 2.0;
-        double E = prev[ncorr + 2 * j + 1] + prev[j] * dit / 2.0;
-        cur[ncorr + 2 * j + 1] = E + cur[j] * dit / 2.0;
+    if (j >= ncorr || leff == 0) return;
+    out += blockIdx.y * stride;
+    double I = out[ncorr + 2 * j], E = out[ncorr + 2 * j + 1], cprev = out[j];
+    constexpr uint32_t B = 32;
+    double v[B], w[B];
+#pragma unroll
+    for (uint32_t k = 0; k < B; k++) v[k] = 1 + k < leff ? out[(uint64_t)(1 + k) * narr + j] : 0.0;
+    for (uint32_t it0 = 1; it0 < leff; it0 += B) {
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) {
+            const uint32_t it = it0 + B + k;
+            w[k] = it < leff ? out[(uint64_t)it * narr + j] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) {
+            const uint32_t it = it0 + k;
+            if (it >= leff) break;
+            const double dit = (double)it;
+            double *cur = out + (uint64_t)it * narr;
+            const double In = I + cprev / 2.0;
+            I = In + v[k] / 2.0;
+            const double En = E + cprev * dit / 2.0;
+            E = En + v[k] * dit / 2.0;
+            cur[ncorr + 2 * j] = I;
+            cur[ncorr + 2 * j + 1] = E;
+            cprev = v[k];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) v[k] = w[k];
     }
 }
 
@@ -166,9 +215,11 @@ __device__ double schur00(const double *c, int n) {
 }
 
 // :289-346 k_gk and k_einst per lag, then :352-356 E /= l
-__global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncur, uint32_t leff) {
+__global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncur, uint32_t leff,
+                                uint64_t stride) {
     const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it == 0 || it >= leff) return;
+    out += blockIdx.y * stride;
     const uint32_t ncorr = ncur * ncur;
     double *cur = out + (uint64_t)it * narr;
     double matr[GK_MAXC * GK_MAXC];
@@ -198,98 +249,150 @@ __global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_
 }
 
 void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
-                 hipStream_t s) {
+                 hipStream_t s, uint32_t nblocks, uint64_t stride) {
     const uint32_t ncorr = ncur * ncur;
     if (subtract_mean) {
-        hipLaunchKernelGGL(gk_mean_kernel, dim3(1), dim3(64), 0, s, d, narr, ncorr, leff, start_mean);
+        hipLaunchKernelGGL(gk_mean_kernel, dim3(1, nblocks), dim3(64), 0, s, d, narr, ncorr, leff, start_mean, stride);
         ANL_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(gk_integral_kernel, dim3(1), dim3(64), 0, s, d, narr, ncorr, leff);
+    hipLaunchKernelGGL(gk_integral_kernel, dim3(1, nblocks), dim3(64), 0, s, d, narr, ncorr, leff, stride);
     ANL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256), dim3(256), 0, s, d, narr, ncur, leff);
+    hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256, nblocks), dim3(256), 0, s, d, narr, ncur, leff,
+                       stride);
     ANL_HIP(hipGetLastError());
+}
+
+void gk_fp64_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
+                            uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, double *d,
+                            hipStream_t st) {
+    const uint32_t narr = 3 * ncur * ncur + 2;
+    const uint64_t stride = (uint64_t)leff * narr;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += GK_MAXB) {
+        const uint32_t nb = std::min(GK_MAXB, nblocks - b0);
+        GkArgs a{};
+        a.data = s->data;
+        a.ncols = s->ncols;
+        a.ncur = ncur;
+        for (uint32_t c = 0; c < ncur; c++) a.idx[c] = idx_j[c];
+        a.n_seg = n_seg;
+        a.skip = skip;
+        a.leff = leff;
+        a.narr = narr;
+        a.out_stride = stride;
+        a.out = d + b0 * stride;
+        for (uint32_t b = 0; b < nb; b++) {
+            a.bl.primo[b] = primo[b0 + b];
+            a.bl.n[b] = n[b0 + b];
+        }
+        dim3 grid((leff + 255) / 256, nb);
+        switch (ncur) {
+            case 1: hipLaunchKernelGGL(gk_lag_kernel<1>, grid, dim3(256), 0, st, a); break;
+            case 2: hipLaunchKernelGGL(gk_lag_kernel<2>, grid, dim3(256), 0, st, a); break;
+            case 3: hipLaunchKernelGGL(gk_lag_kernel<3>, grid, dim3(256), 0, st, a); break;
+            default: hipLaunchKernelGGL(gk_lag_kernel<4>, grid, dim3(256), 0, st, a); break;
+        }
+        ANL_HIP(hipGetLastError());
+    }
+}
+
+uint32_t gk_block_checks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t tmax, const uint64_t *primo,
+                         const uint32_t *n, uint32_t nblocks) {
+    if (!s || !idx_j || !primo || !n) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate");
+    if (ncur == 0 || ncur > GK_MAXC) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
+    if (nblocks == 0) fail(ANL_ERR_ARG, "no blocks");
+    for (uint32_t c = 0; c < ncur; c++)
+        if (idx_j[c] + 3 > s->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
+    uint32_t leff0 = 0;
+    for (uint32_t b = 0; b < nblocks; b++) {
+        const uint32_t leff = (n[b] < tmax || tmax == 0) ? n[b] : tmax;  // greenkubo...cpp:123
+        if ((uint64_t)leff + primo[b] + n[b] > s->nrows)                 // :168-169
+            fail(ANL_ERR_TOO_SHORT,
+                 "trajectory is too short for this kind of calculation. Select a different starting timestep or "
+                 "lower the size of the average or the lenght of the correlation function");
+        if (b == 0) leff0 = leff;
+        else if (leff != leff0)
+            fail(ANL_ERR_ARG, "blocks with different numbers of lags cannot be averaged together");
+    }
+    return leff0;
 }
 
 }  // namespace anl
 
 using namespace anl;
 
-namespace {
-
-uint32_t gk_fp64_checks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t tmax, uint32_t n,
-                        uint64_t primo) {
-    if (!s || !idx_j) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate");
-    if (ncur == 0 || ncur > GK_MAXC) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
-    for (uint32_t c = 0; c < ncur; c++)
-        if (idx_j[c] + 3 > s->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
-    const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;  // :123
-    if ((uint64_t)leff + primo + n > s->nrows)                 // :168-169
-        fail(ANL_ERR_TOO_SHORT,
-             "trajectory is too short for this kind of calculation. Select a different starting timestep or "
-             "lower the size of the average or the lenght of the correlation function");
-    return leff;
-}
-
-void gk_fp64_enqueue(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff, uint32_t n,
-                     uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo, double *d,
-                     hipStream_t st) {
-    const uint32_t narr = 3 * ncur * ncur + 2;
-    GkArgs a{};
-    a.data = s->data;
-    a.ncols = s->ncols;
-    a.ncur = ncur;
-    for (uint32_t c = 0; c < ncur; c++) a.idx[c] = idx_j[c];
-    a.primo = primo;
-    a.n = n;
-    a.n_seg = n_seg;
-    a.skip = skip;
-    a.leff = leff;
-    a.narr = narr;
-    a.out = d;
-    dim3 grid((leff + 255) / 256);
-    switch (ncur) {
-        case 1: hipLaunchKernelGGL(gk_lag_kernel<1>, grid, dim3(256), 0, st, a); break;
-        case 2: hipLaunchKernelGGL(gk_lag_kernel<2>, grid, dim3(256), 0, st, a); break;
-        case 3: hipLaunchKernelGGL(gk_lag_kernel<3>, grid, dim3(256), 0, st, a); break;
-        default: hipLaunchKernelGGL(gk_lag_kernel<4>, grid, dim3(256), 0, st, a); break;
-    }
-    ANL_HIP(hipGetLastError());
-    gk_epilogue(d, narr, ncur, leff, subtract_mean, start_mean, st);
-}
-
-}  // namespace
-
 extern "C" {
 
-int anl_gk_calculate_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
-                           uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
-                           double *out_dev, void *stream) {
+int anl_gk_calculate_blocks_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                                  uint32_t n_seg, int subtract_mean, uint32_t start_mean, const uint64_t *primo,
+                                  const uint32_t *n, uint32_t nblocks, int form, double *out_dev, double *mean_dev,
+                                  double *var_dev, void *stream) {
     return guard([&] {
-        const uint32_t leff = gk_fp64_checks(s, idx_j, ncur, tmax, n, primo);
-        if (!out_dev) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate_async");
+        const uint32_t leff = gk_block_checks(s, idx_j, ncur, tmax, primo, n, nblocks);
+        if (!out_dev) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate_blocks_async");
+        if ((mean_dev || var_dev) && (!mean_dev || !var_dev || nblocks < 2))
+            fail(ANL_ERR_ARG, "block mean and variance need both outputs and at least two blocks");
+        if (form != ANL_GK_FP64 && form != ANL_GK_MFMA) fail(ANL_ERR_ARG, "unknown Green-Kubo form");
         if (n_seg < 1) n_seg = 1;  // greenkubo...cpp:50-53
         if (skip == 0) skip = 1;
         if (leff == 0) return;
         ensure_device();
-        gk_fp64_enqueue(s, idx_j, ncur, skip, leff, n, n_seg, subtract_mean, start_mean, primo, out_dev,
-                        (hipStream_t)stream);
+        const uint32_t narr = 3 * ncur * ncur + 2;
+        const uint64_t stride = (uint64_t)leff * narr;
+        hipStream_t st = (hipStream_t)stream;
+        if (form == ANL_GK_MFMA) gk_mfma_enqueue_blocks(s, idx_j, ncur, skip, leff, n_seg, primo, n, nblocks, out_dev, st);
+        else gk_fp64_enqueue_blocks(s, idx_j, ncur, skip, leff, n_seg, primo, n, nblocks, out_dev, st);
+        gk_epilogue(out_dev, narr, ncur, leff, subtract_mean, start_mean, st, nblocks, stride);
+        if (mean_dev) mediavar_blocks(nullptr, out_dev, nullptr, nblocks, stride, mean_dev, var_dev, st);
+    });
+}
+
+int anl_gk_calculate_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                           uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
+                           double *out_dev, void *stream) {
+    return anl_gk_calculate_blocks_async(s, idx_j, ncur, skip, tmax, n_seg, subtract_mean, start_mean, &primo, &n, 1,
+                                         ANL_GK_FP64, out_dev, nullptr, nullptr, stream);
+}
+
+int anl_gk_calculate_mfma_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                                uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
+                                double *out_dev, void *stream) {
+    return anl_gk_calculate_blocks_async(s, idx_j, ncur, skip, tmax, n_seg, subtract_mean, start_mean, &primo, &n, 1,
+                                         ANL_GK_MFMA, out_dev, nullptr, nullptr, stream);
+}
+
+// Synchronous host-output form: the output is staged in the series scratch,
+// after the space the MFMA form needs.
+static int gk_calculate_host(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                             uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
+                             double *out_host, int form) {
+    return guard([&] {
+        const uint32_t leff = gk_block_checks(s, idx_j, ncur, tmax, &primo, &n, 1);
+        if (!out_host) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate");
+        if (leff == 0) return;
+        ensure_device();
+        const size_t bytes = (size_t)leff * (3 * ncur * ncur + 2) * sizeof(double);
+        const size_t front = form == ANL_GK_MFMA ? gk_mfma_scratch_bytes(ncur, leff, n) : 0;
+        char *w = (char *)series_workspace(s, front + bytes);
+        double *d = (double *)(w + front);
+        // the MFMA form takes its scratch from the front of the series scratch
+        int rc = anl_gk_calculate_blocks_async(s, idx_j, ncur, skip, tmax, n_seg, subtract_mean, start_mean, &primo,
+                                               &n, 1, form, d, nullptr, nullptr, 0);
+        if (rc != ANL_OK) throw Error{rc, anl_last_error()};
+        ANL_HIP(hipMemcpy(out_host, d, bytes, hipMemcpyDeviceToHost));
     });
 }
 
 int anl_gk_calculate(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax, uint32_t n,
                      uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo, double *out_host) {
-    return guard([&] {
-        const uint32_t leff = gk_fp64_checks(s, idx_j, ncur, tmax, n, primo);
-        if (!out_host) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate");
-        if (n_seg < 1) n_seg = 1;
-        if (skip == 0) skip = 1;
-        if (leff == 0) return;
-        ensure_device();
-        const size_t bytes = (size_t)leff * (3 * ncur * ncur + 2) * sizeof(double);
-        double *d = (double *)series_workspace(s, bytes);
-        gk_fp64_enqueue(s, idx_j, ncur, skip, leff, n, n_seg, subtract_mean, start_mean, primo, d, 0);
-        ANL_HIP(hipMemcpy(out_host, d, bytes, hipMemcpyDeviceToHost));
-    });
+    return gk_calculate_host(s, idx_j, ncur, skip, tmax, n, n_seg, subtract_mean, start_mean, primo, out_host,
+                             ANL_GK_FP64);
+}
+
+int anl_gk_calculate_mfma(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
+                          uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
+                          double *out_host) {
+    return gk_calculate_host(s, idx_j, ncur, skip, tmax, n, n_seg, subtract_mean, start_mean, primo, out_host,
+                             ANL_GK_MFMA);
 }
 
 }  // extern "C"

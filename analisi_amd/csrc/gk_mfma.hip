@@ -1,6 +1,7 @@
 // Green-Kubo lag correlations on the matrix cores (greenkuboNcomponentionicfluid.cpp:
 // 156-249): split-BF16 products on MFMA, FP32 accumulation within a K chunk,
-// FP64 across chunks, then the FP64 epilogue of gk.hip.
+// FP64 across chunks; the epilogue (integrals, Schur complement) is the FP64
+// one of gk.hip.
 //
 // For currents (a, b) and lag l the reference averages J_a(t0).J_b(t0 + l)/3
 // over the selected t0.  With the series cut into rows of 32 steps,
@@ -10,8 +11,10 @@
 // diagonal s - r = l - 32q of tiles q = floor(l/32) and floor(l/32) + 1.
 // Xa carries the t0 selection of one segment (:207-209); Yb is unmasked.
 // Each FP64 value v is split into hi = bf16(v), lo = bf16(v - hi); a product
-// is hi*hi' + hi*lo' + lo*hi' (relative error ~2^-16 per product, which the
-// t0 average reduces further).
+// is hi*hi' + hi*lo' + lo*hi' (relative error ~2^-16 per product, with random
+// sign across t0, so the t0 average reduces it further).  The dropped lo*lo'
+// is the one term whose sign is fixed, at lag 0 of a == b (lo^2): the tile
+// holding lag 0 adds it as a fourth product.
 //
 // Layouts in HBM (bf16 bit patterns, u16):
 //   X[cx][r][p]          = X_cx(32p + r),          p < PP        (cx = 3c + x)
@@ -136,6 +139,10 @@ __global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
                     acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wh[w], acc[j], 0, 0, 0);
                     acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wl[w], acc[j], 0, 0, 0);
                     acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wh[w], acc[j], 0, 0, 0);
+                    // tile 0 holds lag 0, where lo*lo' = lo^2 >= 0 for a == b
+                    // does not average out over t0: add it there
+                    if (j == 0 && g == 0)
+                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wl[w], acc[0], 0, 0, 0);
                 }
             }
         }
@@ -227,24 +234,10 @@ GkmPlan gkm_plan(uint32_t ncur, uint32_t leff, uint32_t n) {
     return p;
 }
 
-uint32_t gk_checks(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t tmax, uint32_t n,
-                   uint64_t primo) {
-    if (!sr || !idx_j) fail(ANL_ERR_ARG, "null pointer passed to anl_gk_calculate");
-    if (ncur == 0 || ncur > 4) fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
-    for (uint32_t c = 0; c < ncur; c++)
-        if (idx_j[c] + 3 > sr->ncols) fail(ANL_ERR_HEADER, "current column triplet out of range");
-    const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;  // greenkubo...cpp:123
-    if ((uint64_t)leff + primo + n > sr->nrows)                 // :168-169
-        fail(ANL_ERR_TOO_SHORT,
-             "trajectory is too short for this kind of calculation. Select a different starting timestep or "
-             "lower the size of the average or the lenght of the correlation function");
-    return leff;
-}
-
-// Enqueues the whole MFMA form on s, output d [leff][narr] (device).
+// Enqueues the lag products of one block on s: d [leff][narr] (device) gets
+// C(l) in its first ncur^2 columns and zeros elsewhere.
 void gkm_enqueue(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t n, uint32_t n_seg,
-                 int subtract_mean, uint32_t start_mean, uint64_t primo, const GkmPlan &p, char *work, double *d,
-                 hipStream_t s) {
+                 uint64_t primo, const GkmPlan &p, char *work, double *d, hipStream_t s) {
     auto *xh = (unsigned short *)(work + p.off_xh);
     auto *xl = (unsigned short *)(work + p.off_xl);
     auto *yh = (unsigned short *)(work + p.off_yh);
@@ -277,47 +270,24 @@ void gkm_enqueue(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t 
                            (uint32_t)p.nchunks, p.ncorr, p.leff, p.narr, (double)cont, iseg, d);
         ANL_HIP(hipGetLastError());
     }
-    gk_epilogue(d, p.narr, ncur, p.leff, subtract_mean, start_mean, s);
 }
 
 }  // namespace
 
-extern "C" {
+namespace anl {
 
-int anl_gk_calculate_mfma_async(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
-                                uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
-                                double *out_dev, void *stream) {
-    return guard([&] {
-        const uint32_t leff = gk_checks(sr, idx_j, ncur, tmax, n, primo);
-        if (!out_dev) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate_mfma_async");
-        if (n_seg < 1) n_seg = 1;  // greenkubo...cpp:50-53
-        if (skip == 0) skip = 1;
-        if (leff == 0) return;
-        ensure_device();
-        const GkmPlan p = gkm_plan(ncur, leff, n);
-        char *work = (char *)series_workspace(sr, p.bytes);
-        gkm_enqueue(sr, idx_j, ncur, skip, n, n_seg, subtract_mean, start_mean, primo, p, work, out_dev,
-                    (hipStream_t)stream);
-    });
+size_t gk_mfma_scratch_bytes(uint32_t ncur, uint32_t leff, uint32_t n) { return gkm_plan(ncur, leff, n).bytes; }
+
+void gk_mfma_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
+                            uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, double *d,
+                            hipStream_t st) {
+    size_t need = 0;
+    for (uint32_t b = 0; b < nblocks; b++) need = std::max(need, gkm_plan(ncur, leff, n[b]).bytes);
+    char *work = (char *)series_workspace(s, need);
+    const uint64_t stride = (uint64_t)leff * (3 * ncur * ncur + 2);
+    // blocks run one after the other on st, sharing the scratch
+    for (uint32_t b = 0; b < nblocks; b++)
+        gkm_enqueue(s, idx_j, ncur, skip, n[b], n_seg, primo[b], gkm_plan(ncur, leff, n[b]), work, d + b * stride, st);
 }
 
-int anl_gk_calculate_mfma(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax,
-                          uint32_t n, uint32_t n_seg, int subtract_mean, uint32_t start_mean, uint64_t primo,
-                          double *out_host) {
-    return guard([&] {
-        const uint32_t leff = gk_checks(sr, idx_j, ncur, tmax, n, primo);
-        if (!out_host) fail(ANL_ERR_ARG, "null output passed to anl_gk_calculate_mfma");
-        if (n_seg < 1) n_seg = 1;
-        if (skip == 0) skip = 1;
-        if (leff == 0) return;
-        ensure_device();
-        const GkmPlan p = gkm_plan(ncur, leff, n);
-        const size_t outb = align256((size_t)leff * p.narr * sizeof(double));
-        char *work = (char *)series_workspace(sr, p.bytes + outb);
-        double *d = (double *)(work + p.bytes);
-        gkm_enqueue(sr, idx_j, ncur, skip, n, n_seg, subtract_mean, start_mean, primo, p, work, d, 0);
-        ANL_HIP(hipMemcpy(out_host, d, (size_t)leff * p.narr * sizeof(double), hipMemcpyDeviceToHost));
-    });
-}
-
-}  // extern "C"
+}  // namespace anl

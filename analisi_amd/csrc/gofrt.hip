@@ -644,9 +644,9 @@ using namespace anl;
 // Launch shape of the screened kernel: unroll of the j loop, XCD-aware
 // workgroup order, pair evaluations per workgroup (sets the t0 chunk).
 struct GofrtTuning {
-    int unroll = 4;
+    int unroll = 8;
     bool xcd = false;
-    uint64_t pairs_per_wg = 1ull << 31;
+    uint64_t pairs_per_wg = 1ull << 30;
 };
 
 struct anl_gofrt {

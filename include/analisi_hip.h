@@ -169,6 +169,22 @@ int anl_gk_calculate(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32
 int anl_gk_calculate_mfma(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
                           uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
                           uint32_t start_mean, uint64_t primo, double *out_host);
+/* Green-Kubo of nblocks blocks in one call (the block loop of
+ * BlockAverageG::calcola_custom, blockaverage.h:107-190, with
+ * MediaVarCovar, calcoliblocchi.h:70-176, and no covariance pairs): block b
+ * is reset(n[b]) + calculate(primo[b]); all blocks must have the same
+ * leff = min(n[b], tmax).  out_dev [nblocks][leff][3*ncur^2+2]; mean_dev and
+ * var_dev ([leff][3*ncur^2+2], both or neither, nblocks >= 2) receive the
+ * mean over blocks and the variance of that mean.  form: ANL_GK_FP64 (the
+ * reference's values) or ANL_GK_MFMA (split-BF16 lag products).  The FP64
+ * lag loop of all blocks is one launch, and so is the epilogue of all
+ * blocks. */
+#define ANL_GK_FP64 0
+#define ANL_GK_MFMA 1
+int anl_gk_calculate_blocks_async(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip,
+                                  uint32_t tmax, uint32_t n_seg, int subtract_mean, uint32_t start_mean,
+                                  const uint64_t *primo, const uint32_t *n, uint32_t nblocks, int form,
+                                  double *out_dev, double *mean_dev, double *var_dev, void *stream);
 /* Stream-ordered variants writing [leff][3*ncur^2+2] FP64 to device memory.
  * Scratch lives in the series (grow-only; growing synchronises the device),
  * so the calls on one series must be ordered on one stream. */

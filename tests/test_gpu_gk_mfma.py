@@ -94,3 +94,40 @@ def test_gk_mfma_pyanalisi_reference_call(gk_ref):
     sig = _scales(ref, 2)
     dC = np.abs(m[:, :4] - ref[:, :4]) / sig
     assert dC.max() <= 1e-6, dC.max()
+
+
+@pytest.mark.parametrize("mfma", [False, True])
+def test_gk_blocks_match_single_calls_and_mediavar(mfma):
+    """Uneven blocks in one call: every block equals its own single call (same
+    kernels, bit for bit), and the block mean / variance equal the oracle's
+    MediaVar over those blocks (calcoliblocchi.h:21-65, the arithmetic of
+    MediaVarCovar without covariance pairs)."""
+    J = synth.ar2_currents(77, 9000, ncur=2)
+    s = capi.DeviceSeries(J)
+    primo = [0, 1500, 2200, 4100, 7000]
+    n = [1400, 700, 1900, 2800, 900]
+    tmax = 350
+    per, mean, var = s.gk_blocks([0, 3], 2, tmax, 2, True, 20, primo, n, mfma=mfma)
+    assert per.shape == (5, tmax, 14)
+    for b in range(5):
+        one = s.gk([0, 3], 2, tmax, n[b], 2, True, 20, primo[b], mfma=mfma)
+        assert np.array_equal(np.nan_to_num(per[b], nan=0.25), np.nan_to_num(one, nan=0.25)), b
+        if not mfma:
+            ref = O.gk(J, [0, 3], 2, tmax, True, 20, 2, n[b], primo[b])
+            assert np.array_equal(np.nan_to_num(one, nan=0.25), np.nan_to_num(ref, nan=0.25)), b
+    mv = O.MediaVar(per[0].size)
+    for b in range(5):
+        mv.add(per[b].ravel())
+    mv.end(5)
+    same = lambda a, b_: np.array_equal(np.nan_to_num(a, nan=0.25), np.nan_to_num(b_, nan=0.25))
+    assert same(mean.ravel(), mv.mean) and same(var.ravel(), mv.var)
+
+
+def test_gk_blocks_errors():
+    s = capi.DeviceSeries(np.zeros((1000, 3)))
+    with pytest.raises(capi.AnlError) as e:      # different leff
+        s.gk_blocks([0], 1, 100, 1, False, 0, [0, 10], [200, 50])
+    assert e.value.code == capi.ANL_ERR_ARG
+    with pytest.raises(capi.AnlError) as e:      # block 1 runs past the end
+        s.gk_blocks([0], 1, 100, 1, False, 0, [0, 800], [200, 200])
+    assert e.value.code == capi.ANL_ERR_TOO_SHORT
