@@ -47,8 +47,9 @@ SIGNATURES = [
     ("anl_gofrt_calculate_async", _i, [_vp, _u64, _vp, _vp]),
     ("anl_gofrt_calc_single_th", _i, [_vp, _i, _i, _i, _i, _u64, _vp]),
     ("anl_gofrt_calculate_blocks_async", _i, [_vp, _vp, _vp, _u32, _vp, _vp]),
-    ("anl_gofrt_set_tuning", _i, [_vp, _i, _i, _u64]),
-    ("anl_gofrt_get_tuning", _i, [_vp, _vp, _vp, _vp]),
+    ("anl_gofrt_cluster_stats", _i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
+    ("anl_gofrt_set_tuning", _i, [_vp, _i, _i, _u64, _i]),
+    ("anl_gofrt_get_tuning", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("anl_gofrt_autotune", _i, [_vp, _u64, _u32, _vp, _u32, _vp, _vp]),
     ("anl_gofrt_last_stats", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     ("anl_blocks_create", _i, [_vp, _vp, _vp, _u32, _vp, _i, _u32, _u32, _i, _i, _i, _pp]),
@@ -168,23 +169,30 @@ class DeviceGofrt:
         check(lib().anl_gofrt_last_stats(self.handle, ctypes.byref(f), ctypes.byref(c), ctypes.byref(r)))
         return f.value, c.value, r.value
 
-    def set_tuning(self, unroll, xcd, pairs_per_wg):
-        check(lib().anl_gofrt_set_tuning(self.handle, unroll, int(bool(xcd)), pairs_per_wg))
+    def cluster_stats(self):
+        """(clustered, all) (wavefront, t0) units of the last screened calculate."""
+        c, t = _u64(), _u64()
+        check(lib().anl_gofrt_cluster_stats(self.handle, ctypes.byref(c), ctypes.byref(t)))
+        return c.value, t.value
+
+    def set_tuning(self, unroll, xcd, pairs_per_wg, cluster=True):
+        check(lib().anl_gofrt_set_tuning(self.handle, unroll, int(bool(xcd)), pairs_per_wg, int(bool(cluster))))
 
     def tuning(self):
-        """(unroll, xcd, pairs per workgroup) of the pair kernel's launch shape."""
-        u, x, p = _i(), _i(), _u64()
-        check(lib().anl_gofrt_get_tuning(self.handle, ctypes.byref(u), ctypes.byref(x), ctypes.byref(p)))
-        return u.value, x.value, p.value
+        """(unroll, xcd, pairs per workgroup, cluster) of the pair kernel's launch shape."""
+        u, x, p, c = _i(), _i(), _u64(), _i()
+        check(lib().anl_gofrt_get_tuning(self.handle, ctypes.byref(u), ctypes.byref(x), ctypes.byref(p),
+                                         ctypes.byref(c)))
+        return u.value, x.value, p.value, c.value
 
     def autotune(self, primo, nlag_probe, candidates=None):
-        """Times candidate launch shapes [(unroll, xcd, log2 pairs), ...] on the
-        first nlag_probe lags of reset()'s calculation at primo; keeps the
-        fastest.  Returns (times in ms per candidate, index of the best)."""
+        """Times candidate launch shapes [(unroll, xcd, log2 pairs, cluster), ...]
+        on the first nlag_probe lags of reset()'s calculation at primo; keeps
+        the fastest.  Returns (times in ms per candidate, index of the best)."""
         if candidates is None:
             cand, nc = None, 8
         else:
-            cand = np.ascontiguousarray(candidates, dtype=np.int32).reshape(-1, 3)
+            cand = np.ascontiguousarray(candidates, dtype=np.int32).reshape(-1, 4)
             nc = cand.shape[0]
         ms = np.zeros(nc, dtype=np.float32)
         best = _u32()

@@ -58,6 +58,8 @@ struct Screen {
     std::string why_not;   // reason when !valid
     double frac_max;       // bound of |fractional component| of a candidate
     double row_sum;        // max over rows of the cell matrix of sum |A_ik|
+    double wmin;           // smallest cell width (distance between opposite faces)
+    double fro;            // Frobenius norm of the cell matrix (>= its spectral norm)
 };
 
 }  // namespace anl

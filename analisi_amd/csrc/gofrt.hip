@@ -126,7 +126,7 @@ static double min_eig_sym3(double G[3][3]) {
 //      coordinate moves each component of q by at most 1 unit, so
 //      d <= rmax  =>  |q| <= rmax 2^16 / sqrt(lambda_min) + sqrt(3).
 static Screen make_screen(const anl_traj *tr, double rmax) {
-    Screen s{false, 0, "", 0.5, 0.0};
+    Screen s{false, 0, "", 0.5, 0.0, 0.0, 0.0};
     if (!tr->box_const) {
         s.why_not = "the cell changes along the trajectory";
         return s;
@@ -171,6 +171,11 @@ static Screen make_screen(const anl_traj *tr, double rmax) {
     s.row_sum = 0;
     for (int i = 0; i < 3; i++)
         s.row_sum = std::max(s.row_sum, std::fabs(A[i][0]) + std::fabs(A[i][1]) + std::fabs(A[i][2]));
+    s.wmin = wmin;
+    s.fro = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) s.fro += A[i][j] * A[i][j];
+    s.fro = std::sqrt(s.fro);
     return s;
 }
 
@@ -241,6 +246,10 @@ struct GofrtArgs {
     unsigned long long *nrecheck;
     float cx, cxy, cxz, cy, cyz, cz;  // cell matrix entries times 2^-32
     float du_margin;                  // bins: |u - nearest integer| below this -> FP64
+    // cluster screen (see cluster_threshold): cell entries times 2^-16,
+    // largest usable cluster radius, quantisation bound, rmax
+    float hx, hxy, hxz, hy, hyz, hz;
+    float rho_cap, dq, rmaxf;
 };
 
 // The grid is one-dimensional.  Workgroups are dispatched round-robin over
@@ -482,6 +491,7 @@ __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e
 }
 
 constexpr int SQCAP = 576;   // per-wavefront queue: < 64 left + up to 8 masks of 64
+constexpr int SQCAP_CL = 2112;  // cluster screen: < 64 left + 32 masks of 64 (one drain per 32 j)
 constexpr int RQCAP = 128;   // per-wavefront recheck queue: < 64 left + one batch
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -490,13 +500,57 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int s16(uint32_t x) { return (int)(short)(x & 0xFFFFu); }
+
+// Wavefront sum / max over all 64 lanes (butterfly, every lane gets the result).
+__device__ __forceinline__ int wave_sum_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Cluster screen.  A wavefront's 64 i atoms in frame fi get a common centre
+// c (16-bit fractional) and Cartesian offsets p_i = A wrap16(f_i - c); j atoms
+// get q_j = A wrap16(f_j - c).  If rho = max |p_i| satisfies
+//   rho + rmax + dq < w_min (1/2 - 2^-15)            (rho < rho_cap)
+// then for every pair the reference can count (d <= rmax) the vector
+// q_j - p_i is the reference's minimum-image vector up to the 16-bit
+// quantisation (|error| <= dq): the fractional components of
+// (f_j + n - c) stay strictly inside (-1/2, 1/2), so wrapping relative to c
+// picks the reference's image.  The squared distance
+//   |q - p|^2 = |p|^2 + |q|^2 - 2 p.q
+// of 32 x 32 pairs is then one MFMA with rows (-2p, |p|^2, 1) and columns
+// (q, 1, |q|^2) in FP16, accumulated in FP32.  Rounding p, q, |p|^2, |q|^2 to
+// FP16 (2^-11 relative) and the FP32 sum change the result by at most
+// 2^-10 (|p| + |q|)^2, and |q| <= rho + rmax + dq for counted pairs, so
+//   D < ((rmax + dq)^2 + 2^-10 (2 rho + rmax + dq)^2) (1 + 2^-12) + 1e-4
+// holds for every pair the reference counts.  Passing pairs go to the same
+// candidate queue as the 16-bit screen's.
+__device__ __forceinline__ float cluster_threshold(float rho, float rmax, float dq) {
+    const float a = rmax + dq, b = 2.0f * rho + rmax + dq;
+    return (a * a + b * b * 0x1p-10f) * (1.0f + 0x1p-12f) + 1e-4f;
+}
+
+#define ANL_APPEND_E(P, M, ENTRY)                                           \
+    if (M) {                                                                \
+        if (P) q[lane_slot(M, qn)] = (ENTRY);                               \
+        qn += (uint32_t)__popcll(M);                                        \
+    }
+
 #define ANL_APPEND(P, M, J)                                                 \
     if (M) {                                                                \
         if (P) q[lane_slot(M, qn)] = ((uint32_t)(J) << 8) | il;             \
         qn += (uint32_t)__popcll(M);                                        \
     }
 
-template <bool TRI, int UNROLL>
+template <bool TRI, int UNROLL, bool CLUSTER>
 __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     extern __shared__ unsigned char smem[];
     const WgId id = decode_wg(a.ntiles, a.nlag, a.nwork, a.per_xcd);
@@ -506,8 +560,9 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     const uint32_t t = a.lag0 + id.lagi * a.every;
     if (t >= b.leff) return;
     uint4 *fi_lds = (uint4 *)smem;                          // [WG]
-    uint32_t *queue = (uint32_t *)(fi_lds + WG);            // [WG/64][SQCAP]
-    uint32_t *rqueue = queue + (WG / 64) * SQCAP;           // [WG/64][RQCAP]
+    constexpr int QC = CLUSTER ? SQCAP_CL : SQCAP;
+    uint32_t *queue = (uint32_t *)(fi_lds + WG);            // [WG/64][QC]
+    uint32_t *rqueue = queue + (WG / 64) * QC;              // [WG/64][RQCAP]
     int *ti_lds = (int *)(rqueue + (WG / 64) * RQCAP);      // [WG]
     double *thr = (double *)(ti_lds + WG);                  // [nbin+1]
     int *chan = (int *)(thr + a.nbin + 1);                  // [nt*nt]
@@ -516,7 +571,7 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
 
     const uint64_t N = a.N;
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t *q = queue + (threadIdx.x >> 6) * SQCAP;
+    uint32_t *q = queue + (threadIdx.x >> 6) * QC;
     uint32_t *rq = rqueue + (threadIdx.x >> 6) * RQCAP;
     const uint64_t slot0 = (uint64_t)id.tile * WG;
     const uint64_t slot = slot0 + threadIdx.x;
@@ -529,11 +584,12 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     }
     ti_lds[threadIdx.x] = ti;
     const uint32_t il = threadIdx.x;
-    const uint32_t NU = (uint32_t)(N - N % UNROLL);
+    constexpr int UR = UNROLL;
+    const uint32_t NU = (uint32_t)(N - N % UR);
     // strict comparison against threshold+1; inactive lanes compare against 0
     // and never pass
     const uint32_t th = active ? a.threshold + 1u : 0u;
-    unsigned long long ncand = 0, nre = 0;
+    unsigned long long ncand = 0, nre = 0, ncl = 0, nwt = 0;
 
     for (uint32_t k = w.k0; k < w.k1; k++) {
         const uint64_t fi = b.primo + (uint64_t)k * a.skip;
@@ -570,40 +626,169 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
                 }
             }
         };
-        for (uint32_t j = 0; j < NU; j += UNROLL) {
-            unsigned long long g[UNROLL];
-            bool p[UNROLL];
-            unsigned long long m[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; u++) g[u] = fjc[j + u];
-#pragma unroll
-            for (int u = 0; u < UNROLL; u++) p[u] = sq16(abi, (uint32_t)g[u], ci, (uint32_t)(g[u] >> 32)) < th;
-            unsigned long long any = 0;
-#pragma unroll
-            for (int u = 0; u < UNROLL; u++) {
-                m[u] = __ballot(p[u]);
-                any |= m[u];
-            }
-            if (any) {
-#pragma unroll
-                for (int u = 0; u < UNROLL; u++) {
-                    ANL_APPEND(p[u], m[u], j + u)
+        bool clustered = false;
+        if constexpr (CLUSTER) {
+            nwt += lane == 0 ? 1 : 0;
+            const unsigned long long act = __ballot(active);
+            if (act) {
+                // centre: reference atom + mean wrapped offset of the active lanes
+                const int ref = __builtin_ctzll(act);
+                const uint32_t ab_r = __shfl(abi, ref, 64), c_r = __shfl(ci, ref, 64);
+                const int oa = active ? s16(abi - ab_r) : 0;
+                const int ob = active ? s16((abi >> 16) - (ab_r >> 16)) : 0;
+                const int oc = active ? s16(ci - c_r) : 0;
+                const float inv_n = 1.0f / (float)__popcll(act);
+                const uint32_t ca = (ab_r + (uint32_t)__float2int_rn((float)wave_sum_i(oa) * inv_n)) & 0xFFFFu;
+                const uint32_t cb = ((ab_r >> 16) + (uint32_t)__float2int_rn((float)wave_sum_i(ob) * inv_n)) & 0xFFFFu;
+                const uint32_t cc = (c_r + (uint32_t)__float2int_rn((float)wave_sum_i(oc) * inv_n)) & 0xFFFFu;
+                const float fa = (float)s16(abi - ca), fb = (float)s16((abi >> 16) - cb), fc = (float)s16(ci - cc);
+                float px, py, pz;
+                if (TRI) {
+                    pz = a.hz * fc;
+                    py = fmaf(a.hyz, fc, a.hy * fb);
+                    px = fmaf(a.hxz, fc, fmaf(a.hxy, fb, a.hx * fa));
+                } else {
+                    px = a.hx * fa;
+                    py = a.hy * fb;
+                    pz = a.hz * fc;
                 }
-                while (qn >= 64) {
-                    wave_sync_lds();
-                    qn -= 64;
-                    const uint32_t e = q[qn + lane];
-                    ncand += 1;
-                    eval_batch(e, true);
-                    wave_sync_lds();
+                const float p2 = fmaf(pz, pz, fmaf(py, py, px * px));
+                const float rho = sqrtf(wave_max_f(active ? p2 : 0.0f));
+                if (rho < a.rho_cap) {
+                    clustered = true;
+                    ncl += lane == 0 ? 1 : 0;
+                    const float thc = cluster_threshold(rho, a.rmaxf, a.dq);
+                    // rows: (-2p, |p|^2, 1) for active lanes, (0, 0, 0, 60000, 1) otherwise
+                    f16x8_t row = {};
+                    if (active) {
+                        row[0] = (_Float16)(-2.0f * px);
+                        row[1] = (_Float16)(-2.0f * py);
+                        row[2] = (_Float16)(-2.0f * pz);
+                        row[3] = (_Float16)p2;
+                    } else {
+                        row[3] = (_Float16)60000.0f;
+                    }
+                    row[4] = (_Float16)1.0f;
+                    // A of i rows 0..31 (lanes 0-31 own them) and 32..63 (moved
+                    // down from lanes 32-63); k slots 8..15 (lanes 32-63) are 0
+                    const uint4 rw = __builtin_bit_cast(uint4, row);
+                    uint4 up;
+                    up.x = __shfl(rw.x, (int)(lane | 32u), 64);
+                    up.y = __shfl(rw.y, (int)(lane | 32u), 64);
+                    up.z = __shfl(rw.z, (int)(lane | 32u), 64);
+                    up.w = __shfl(rw.w, (int)(lane | 32u), 64);
+                    const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+                    const f16x8_t A0 = __builtin_bit_cast(f16x8_t, lane < 32 ? rw : z4);
+                    const f16x8_t A1 = __builtin_bit_cast(f16x8_t, lane < 32 ? up : z4);
+                    const uint2 *__restrict__ fj16 = a.frac + fj * N;
+                    const uint32_t wbase = threadIdx.x & ~63u;
+                    const uint32_t half4 = 4u * (lane >> 5);
+                    // lanes 0-31 build column lane of the 32 j atoms j0..j0+31;
+                    // the next block's 16-bit fractions are loaded one
+                    // iteration ahead
+                    const bool colane = lane < 32;
+                    uint2 gn = make_uint2(0u, 0u);
+                    if (colane && lane < (uint32_t)N) gn = fj16[lane];
+                    for (uint32_t j0 = 0; j0 < (uint32_t)N; j0 += 32) {
+                        const uint2 g = gn;
+                        const bool jin = colane && j0 + lane < (uint32_t)N;
+                        if (colane && j0 + 32 + lane < (uint32_t)N) gn = fj16[j0 + 32 + lane];
+                        const float ga = (float)s16(g.x - ca), gb = (float)s16((g.x >> 16) - cb);
+                        const float gc = (float)s16(g.y - cc);
+                        float qx, qy, qz;
+                        if (TRI) {
+                            qz = a.hz * gc;
+                            qy = fmaf(a.hyz, gc, a.hy * gb);
+                            qx = fmaf(a.hxz, gc, fmaf(a.hxy, gb, a.hx * ga));
+                        } else {
+                            qx = a.hx * ga;
+                            qy = a.hy * gb;
+                            qz = a.hz * gc;
+                        }
+                        const float q2 = fmaf(qz, qz, fmaf(qy, qy, qx * qx));
+                        // column (q, 1, |q|^2); past N: (0, 0, 0, 1, 60000); lanes 32-63: 0
+                        f16x8_t B = {};
+                        B[0] = (_Float16)(jin ? qx : 0.0f);
+                        B[1] = (_Float16)(jin ? qy : 0.0f);
+                        B[2] = (_Float16)(jin ? qz : 0.0f);
+                        B[3] = (_Float16)(colane ? 1.0f : 0.0f);
+                        B[4] = (_Float16)(jin ? q2 : (colane ? 60000.0f : 0.0f));
+                        const f32x16_t D0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A0, B, f32x16_t{}, 0, 0, 0);
+                        const f32x16_t D1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, B, f32x16_t{}, 0, 0, 0);
+                        // entry = j << 8 | local i; item e of tile T is row
+                        // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
+                        const uint32_t eb = ((j0 + (lane & 31u)) << 8) | (wbase + half4);
+#pragma unroll
+                        for (int grp = 0; grp < 4; grp++) {
+                            bool pc[8];
+                            unsigned long long mk[8];
+                            unsigned long long any = 0;
+#pragma unroll
+                            for (int u = 0; u < 8; u++) {
+                                const int e = 8 * (grp & 1) + u;
+                                pc[u] = (grp < 2 ? D0[e] : D1[e]) < thc;
+                                mk[u] = __ballot(pc[u]);
+                                any |= mk[u];
+                            }
+                            if (any) {
+#pragma unroll
+                                for (int u = 0; u < 8; u++) {
+                                    const int e = 8 * (grp & 1) + u;
+                                    const uint32_t roff = 32u * (uint32_t)(grp >> 1) + 8u * (uint32_t)(e >> 2) +
+                                                          (uint32_t)(e & 3);
+                                    ANL_APPEND_E(pc[u], mk[u], eb + roff)
+                                }
+                            }
+                        }
+                        // one drain site per 32 j (queue holds < 64 + 32 masks)
+                        while (qn >= 64) {
+                            wave_sync_lds();
+                            qn -= 64;
+                            const uint32_t e = q[qn + lane];
+                            ncand += 1;
+                            eval_batch(e, true);
+                            wave_sync_lds();
+                        }
+                    }
                 }
             }
         }
-        for (uint32_t j = NU; j < (uint32_t)N; j++) {  // N % UNROLL tail
-            const unsigned long long g0 = fjc[j];
-            const bool p0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32)) < th;
-            const unsigned long long m0 = __ballot(p0);
-            ANL_APPEND(p0, m0, j)
+        if (!clustered) {
+            for (uint32_t j = 0; j < NU; j += UR) {
+                unsigned long long g[UR];
+                bool p[UR];
+                unsigned long long m[UR];
+    #pragma unroll
+                for (int u = 0; u < UR; u++) g[u] = fjc[j + u];
+    #pragma unroll
+                for (int u = 0; u < UR; u++) p[u] = sq16(abi, (uint32_t)g[u], ci, (uint32_t)(g[u] >> 32)) < th;
+                unsigned long long any = 0;
+    #pragma unroll
+                for (int u = 0; u < UR; u++) {
+                    m[u] = __ballot(p[u]);
+                    any |= m[u];
+                }
+                if (any) {
+    #pragma unroll
+                    for (int u = 0; u < UR; u++) {
+                        ANL_APPEND(p[u], m[u], j + u)
+                    }
+                    while (qn >= 64) {
+                        wave_sync_lds();
+                        qn -= 64;
+                        const uint32_t e = q[qn + lane];
+                        ncand += 1;
+                        eval_batch(e, true);
+                        wave_sync_lds();
+                    }
+                }
+            }
+            for (uint32_t j = NU; j < (uint32_t)N; j++) {  // N % UR tail
+                const unsigned long long g0 = fjc[j];
+                const bool p0 = sq16(abi, (uint32_t)g0, ci, (uint32_t)(g0 >> 32)) < th;
+                const unsigned long long m0 = __ballot(p0);
+                ANL_APPEND(p0, m0, j)
+            }
         }
         // drain: fewer than 64 + (UNROLL-1)*64 entries, 64 at a time, then the rechecks
         wave_sync_lds();
@@ -628,6 +813,10 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     if (lane == 0 && a.ncand) {
         atomicAdd(a.ncand, ncand);
         atomicAdd(a.nrecheck, nre);
+        if (CLUSTER) {
+            atomicAdd(a.ncand + 2, ncl);
+            atomicAdd(a.ncand + 3, nwt);
+        }
     }
     __syncthreads();
     lds_flush(a, hist, w.block, t);
@@ -642,11 +831,13 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
 using namespace anl;
 
 // Launch shape of the screened kernel: unroll of the j loop, XCD-aware
-// workgroup order, pair evaluations per workgroup (sets the t0 chunk).
+// workgroup order, pair evaluations per workgroup (sets the t0 chunk), and
+// whether compact wavefronts use the cluster (MFMA) screen.
 struct GofrtTuning {
     int unroll = 8;
     bool xcd = false;
     uint64_t pairs_per_wg = 1ull << 30;
+    bool cluster = false;  // cluster (MFMA) screen where a wavefront's atoms are compact enough
 };
 
 struct anl_gofrt {
@@ -660,7 +851,7 @@ struct anl_gofrt {
     double lo = 0;
     double *d_thr = nullptr;
     int32_t *d_chan = nullptr;
-    unsigned long long *d_ncand = nullptr;   // [2]: candidates, FP64 rechecks
+    unsigned long long *d_ncand = nullptr;   // [4]: candidates, FP64 rechecks, clustered and all wavefront-t0 units
     int last_form = -1;
     // scratch for the work list
     BlockInfo *d_blocks = nullptr;
@@ -741,6 +932,7 @@ struct GofrtPrepared {
     uint64_t out_len = 0;
     bool triclinic = false;
     int unroll = 4;
+    bool cluster = false;
 };
 
 __global__ void fill_u64_kernel(unsigned long long *__restrict__ p, uint64_t len) {
@@ -814,6 +1006,24 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         a.cyz = (float)(cl.yz * sc);
         a.cz = (float)(cl.L[2] * sc);
         a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->rmax, g->dr, g->nbin) : 0.0f;
+        // cluster screen: cell entries times 2^-16; dq bounds the error of a
+        // pair vector from 16-bit quantisation of both atoms (each fractional
+        // component within 2^-17 + 2^-33, times |A|_2 <= |A|_F, sqrt(3)
+        // components); rho_cap from rho + rmax + dq < w_min (1/2 - 2^-15),
+        // less 1e-3 A for the FP32 evaluation of rho
+        const double sh = std::ldexp(1.0, -16);
+        a.hx = (float)(cl.L[0] * sh);
+        a.hxy = (float)(cl.xy * sh);
+        a.hxz = (float)(cl.xz * sh);
+        a.hy = (float)(cl.L[1] * sh);
+        a.hyz = (float)(cl.yz * sh);
+        a.hz = (float)(cl.L[2] * sh);
+        const Screen &scr = tr->screen_cache;
+        const double dq = 2.0 * std::sqrt(3.0) * scr.fro * (std::ldexp(1.0, -17) + std::ldexp(1.0, -33)) * 1.001;
+        a.dq = (float)(dq * 1.0001);
+        a.rmaxf = (float)(std::fabs(g->rmax) * (1 + 1e-6));
+        const double cap = scr.wmin * (0.5 - std::ldexp(1.0, -15)) - std::fabs(g->rmax) - dq - 1e-3;
+        a.rho_cap = form == ANL_GOFRT_SCREENED && cap > 0 ? (float)cap : 0.0f;
     }
     const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
     const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;
@@ -829,14 +1039,19 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
     P.grid = dim3((uint32_t)(remap ? 8 * per : total));
     const size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
                           (size_t)a.nch * g->nbin * sizeof(unsigned);
-    P.lds = form == ANL_GOFRT_FP64
-                ? common
-                : WG * sizeof(uint4) + (WG / 64) * (SQCAP + RQCAP) * sizeof(uint32_t) + WG * sizeof(int) + common;
+    auto screened_lds = [&](int qcap) {
+        return WG * sizeof(uint4) + (WG / 64) * (qcap + RQCAP) * sizeof(uint32_t) + WG * sizeof(int) + common;
+    };
+    // the cluster screen needs the larger queue; without room it is not used
+    const bool cluster_ok = g->tune.cluster && form == ANL_GOFRT_SCREENED && a.rho_cap > 0 &&
+                            screened_lds(SQCAP_CL) <= 160 * 1024;
+    P.lds = form == ANL_GOFRT_FP64 ? common : screened_lds(cluster_ok ? SQCAP_CL : SQCAP);
     if (P.lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
     P.form = form;
     P.out_len = out_len;
     P.triclinic = tr->triclinic;
     P.unroll = g->tune.unroll;
+    P.cluster = cluster_ok;
     g->last_form = form;
     return P;
 }
@@ -845,20 +1060,27 @@ void gofrt_enqueue(anl_gofrt *g, const GofrtPrepared &P, unsigned long long *out
     GofrtArgs a = P.a;
     a.out = out;
     zero_u64(out, P.out_len, s);
-    zero_u64(g->d_ncand, 2, s);
+    zero_u64(g->d_ncand, 4, s);
     if (P.grid.x == 0) return;
     if (P.form == ANL_GOFRT_FP64) {
         if (P.triclinic) hipLaunchKernelGGL(gofrt_fp64_kernel<true>, P.grid, dim3(WG), P.lds, s, a);
         else hipLaunchKernelGGL(gofrt_fp64_kernel<false>, P.grid, dim3(WG), P.lds, s, a);
     } else {
         const bool u8 = P.unroll == 8;
-        if (P.triclinic) {
-            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<true, 8>), P.grid, dim3(WG), P.lds, s, a);
-            else hipLaunchKernelGGL((gofrt_screened_kernel<true, 4>), P.grid, dim3(WG), P.lds, s, a);
+#define ANL_SCREENED(T, U, C) hipLaunchKernelGGL((gofrt_screened_kernel<T, U, C>), P.grid, dim3(WG), P.lds, s, a)
+        if (P.cluster) {  // the 16-bit fallback of non-compact wavefronts takes 8 j per step
+            if (P.triclinic) ANL_SCREENED(true, 8, true);
+            else ANL_SCREENED(false, 8, true);
         } else {
-            if (u8) hipLaunchKernelGGL((gofrt_screened_kernel<false, 8>), P.grid, dim3(WG), P.lds, s, a);
-            else hipLaunchKernelGGL((gofrt_screened_kernel<false, 4>), P.grid, dim3(WG), P.lds, s, a);
+            if (P.triclinic) {
+                if (u8) ANL_SCREENED(true, 8, false);
+                else ANL_SCREENED(true, 4, false);
+            } else {
+                if (u8) ANL_SCREENED(false, 8, false);
+                else ANL_SCREENED(false, 4, false);
+            }
         }
+#undef ANL_SCREENED
     }
     ANL_HIP(hipGetLastError());
 }
@@ -915,6 +1137,7 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
         g->form = form;
         if (const char *e = std::getenv("ANL_GOFRT_UNROLL")) g->tune.unroll = std::atoi(e) == 8 ? 8 : 4;
         if (const char *e = std::getenv("ANL_GOFRT_XCD")) g->tune.xcd = std::atoi(e) != 0;
+        if (const char *e = std::getenv("ANL_GOFRT_CLUSTER")) g->tune.cluster = std::atoi(e) != 0;
         if (const char *e = std::getenv("ANL_GOFRT_PAIRS_PER_WG")) {
             long long v = std::atoll(e);
             if (v > 0) g->tune.pairs_per_wg = (uint64_t)v;
@@ -935,7 +1158,7 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
             }
         ANL_HIP(hipMalloc(&g->d_chan, chan.size() * sizeof(int32_t)));
         ANL_HIP(hipMemcpy(g->d_chan, chan.data(), chan.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        ANL_HIP(hipMalloc(&g->d_ncand, 2 * sizeof(unsigned long long)));
+        ANL_HIP(hipMalloc(&g->d_ncand, 4 * sizeof(unsigned long long)));
         *out = g;
     });
 }
@@ -1047,7 +1270,17 @@ int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uin
     });
 }
 
-int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_wg) {
+int anl_gofrt_cluster_stats(anl_gofrt *g, uint64_t *clustered, uint64_t *units) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        unsigned long long c[4] = {0, 0, 0, 0};
+        ANL_HIP(hipMemcpy(c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
+        if (clustered) *clustered = c[2];
+        if (units) *units = c[3];
+    });
+}
+
+int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_wg, int cluster) {
     return guard([&] {
         if (!g) fail(ANL_ERR_ARG, "null handle");
         if (unroll != 4 && unroll != 8) fail(ANL_ERR_ARG, "unroll must be 4 or 8");
@@ -1055,15 +1288,17 @@ int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_w
         g->tune.unroll = unroll;
         g->tune.xcd = xcd != 0;
         g->tune.pairs_per_wg = pairs_per_wg;
+        g->tune.cluster = cluster != 0;
     });
 }
 
-int anl_gofrt_get_tuning(anl_gofrt *g, int *unroll, int *xcd, uint64_t *pairs_per_wg) {
+int anl_gofrt_get_tuning(anl_gofrt *g, int *unroll, int *xcd, uint64_t *pairs_per_wg, int *cluster) {
     return guard([&] {
         if (!g) fail(ANL_ERR_ARG, "null handle");
         if (unroll) *unroll = g->tune.unroll;
         if (xcd) *xcd = g->tune.xcd ? 1 : 0;
         if (pairs_per_wg) *pairs_per_wg = g->tune.pairs_per_wg;
+        if (cluster) *cluster = g->tune.cluster ? 1 : 0;
     });
 }
 
@@ -1076,15 +1311,16 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         if (!g) fail(ANL_ERR_ARG, "null handle");
         ensure_device();
         check_lengths(g, &primo, &g->n, 1);
-        static const int32_t defaults[] = {4, 0, 31, 4, 1, 31, 8, 0, 31, 8, 1, 31,
-                                           4, 0, 30, 4, 0, 32, 8, 0, 30, 8, 0, 32};
+        // unroll, xcd, log2 pairs per workgroup, cluster
+        static const int32_t defaults[] = {8, 0, 30, 1, 8, 0, 31, 1, 4, 0, 30, 1, 8, 1, 30, 1,
+                                           8, 0, 29, 1, 8, 0, 30, 0, 4, 0, 31, 0, 8, 1, 31, 0};
         if (!cand) {
             cand = defaults;
-            ncand = sizeof(defaults) / sizeof(defaults[0]) / 3;
+            ncand = sizeof(defaults) / sizeof(defaults[0]) / 4;
         }
         if (ncand == 0) fail(ANL_ERR_ARG, "no candidates");
         for (uint32_t c = 0; c < ncand; c++)
-            if ((cand[3 * c] != 4 && cand[3 * c] != 8) || cand[3 * c + 2] < 10 || cand[3 * c + 2] > 40)
+            if ((cand[4 * c] != 4 && cand[4 * c] != 8) || cand[4 * c + 2] < 10 || cand[4 * c + 2] > 40)
                 fail(ANL_ERR_ARG, "candidate " + std::to_string(c) + ": unroll 4|8, log2(pairs per workgroup) in [10, 40]");
         const uint32_t nlag_all = (g->leff + g->every - 1) / g->every;
         const uint32_t nlag = std::max(1u, std::min(nlag_probe ? nlag_probe : 1u, nlag_all));
@@ -1096,13 +1332,17 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         ANL_HIP(hipMalloc(&d, len * sizeof(unsigned long long)));
         float best = -1.0f;
         uint32_t besti = 0;
+        auto set = [&](uint32_t c) {
+            g->tune.unroll = cand[4 * c];
+            g->tune.xcd = cand[4 * c + 1] != 0;
+            g->tune.pairs_per_wg = 1ull << cand[4 * c + 2];
+            g->tune.cluster = cand[4 * c + 3] != 0;
+        };
         try {
             ANL_HIP(hipEventCreate(&e0));
             ANL_HIP(hipEventCreate(&e1));
             for (uint32_t c = 0; c < ncand; c++) {
-                g->tune.unroll = cand[3 * c];
-                g->tune.xcd = cand[3 * c + 1] != 0;
-                g->tune.pairs_per_wg = 1ull << cand[3 * c + 2];
+                set(c);
                 Launch L = plan(g, &primo, &g->n, 1, -1);
                 GofrtPrepared P = gofrt_prepare(g, L, len, INT32_MIN, INT32_MAX, 0, 0, nlag);
                 gofrt_enqueue(g, P, d, 0);
@@ -1128,9 +1368,7 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipFree(d);
-        g->tune.unroll = cand[3 * besti];
-        g->tune.xcd = cand[3 * besti + 1] != 0;
-        g->tune.pairs_per_wg = 1ull << cand[3 * besti + 2];
+        set(besti);
         if (best_out) *best_out = besti;
     });
 }

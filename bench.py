@@ -155,11 +155,16 @@ def type_ids(types):
 
 
 def centres_of_mass(pos, tid, nt):
-    """Per-type centre of mass per frame, unit masses: [F][nt][3]."""
+    """Per-type centre of mass per frame, unit masses, [F][nt][3], as the
+    reference's running mean over atoms in index order (cm += (x - cm)/n,
+    trajectory_numpy.cpp:201-225), vectorised over frames."""
     F = pos.shape[0]
     cm = np.zeros((F, nt, 3))
-    for t in range(nt):
-        cm[:, t, :] = pos[:, tid == t, :].mean(axis=1)
+    cont = np.zeros(nt, dtype=np.int64)
+    for ia in range(pos.shape[1]):
+        t = tid[ia]
+        cont[t] += 1
+        cm[:, t, :] += (pos[:, ia, :] - cm[:, t, :]) / float(cont[t])
     return cm
 
 

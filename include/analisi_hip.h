@@ -100,17 +100,22 @@ int anl_gofrt_calculate_blocks_async(anl_gofrt *g, const uint64_t *primo, const 
  * many of those were re-evaluated in FP64 (near a bin edge, rmin or rmax).
  * Diagnostics of this implementation; no counterpart in the reference. */
 int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uint64_t *rechecks);
-/* Launch shape of the pair kernels: unroll of the FP32 pair loop (4 or 8),
- * XCD-aware workgroup order (0/1), pair evaluations per workgroup (sets the
- * t0 chunk of one workgroup).  Shapes change speed only, never results.
- * anl_gofrt_autotune times each candidate (cand[3*c] = unroll, cand[3*c+1] =
- * xcd, cand[3*c+2] = log2 pairs per workgroup; NULL = a built-in set of 8) on
- * the first nlag_probe lags of the calculation set by reset() at primo, keeps
- * the fastest and reports ms_out[c] (may be NULL) and its index.  The
- * reference's equivalent knob is nthreads (calculatemultithread.h:80-104);
- * no counterpart for the rest. */
-int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_wg);
-int anl_gofrt_get_tuning(anl_gofrt *g, int *unroll, int *xcd, uint64_t *pairs_per_wg);
+/* Of the last screened calculate: how many (wavefront, t0) units used the
+ * cluster (MFMA) screen, out of how many (0 of 0 when it was off). */
+int anl_gofrt_cluster_stats(anl_gofrt *g, uint64_t *clustered, uint64_t *units);
+/* Launch shape of the pair kernels: unroll of the 16-bit screen loop (4 or
+ * 8), XCD-aware workgroup order (0/1), pair evaluations per workgroup (sets
+ * the t0 chunk of one workgroup), and the cluster screen (0/1: wavefronts
+ * whose 64 atoms are compact screen their pairs with FP16 MFMA distance
+ * tiles).  Shapes change speed only, never results.  anl_gofrt_autotune
+ * times each candidate (cand[4*c] = unroll, cand[4*c+1] = xcd, cand[4*c+2] =
+ * log2 pairs per workgroup, cand[4*c+3] = cluster; NULL = a built-in set of
+ * 8) on the first nlag_probe lags of the calculation set by reset() at
+ * primo, keeps the fastest and reports ms_out[c] (may be NULL) and its
+ * index.  The reference's equivalent knob is nthreads
+ * (calculatemultithread.h:80-104); no counterpart for the rest. */
+int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_wg, int cluster);
+int anl_gofrt_get_tuning(anl_gofrt *g, int *unroll, int *xcd, uint64_t *pairs_per_wg, int *cluster);
 int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const int32_t *cand, uint32_t ncand,
                        float *ms_out, uint32_t *best_index);
 

@@ -124,49 +124,62 @@ def test_launch_shapes_and_autotune_keep_counts(form):
     ref, _ = O.gofrt(ot, 0.0, 6.0, 48, 9, 4, 1, 1, 20, 0, want_vdata=False)
     g = capi.DeviceGofrt(dt, 0.0, 6.0, 48, 9, 1, 1, form)
     g.reset(20)
-    for unroll, xcd, lp in [(4, 0, 31), (8, 1, 31), (4, 1, 12), (8, 0, 16), (4, 0, 40)]:
-        g.set_tuning(unroll, xcd, 1 << lp)
-        assert g.tuning() == (unroll, xcd, 1 << lp)
+    for unroll, xcd, lp, cl in [(4, 0, 31, 1), (8, 1, 31, 0), (4, 1, 12, 1), (8, 0, 16, 0), (4, 0, 40, 1)]:
+        g.set_tuning(unroll, xcd, 1 << lp, cl)
+        assert g.tuning() == (unroll, xcd, 1 << lp, cl)
         got = g.calculate(0)
-        assert first_diff(got, ref) is None, (unroll, xcd, lp, first_diff(got, ref))
+        assert first_diff(got, ref) is None, (unroll, xcd, lp, cl, first_diff(got, ref))
     ms, best = g.autotune(0, 3)
     assert len(ms) == 8 and (ms > 0).all() and 0 <= best < 8
     assert first_diff(g.calculate(0), ref) is None
-    ms, best = g.autotune(0, 2, [(8, 1, 20), (4, 0, 22)])
-    assert g.tuning() == [(8, 1, 1 << 20), (4, 0, 1 << 22)][best]
+    ms, best = g.autotune(0, 2, [(8, 1, 20, 1), (4, 0, 22, 0)])
+    assert g.tuning() == [(8, 1, 1 << 20, 1), (4, 0, 1 << 22, 0)][best]
     assert first_diff(g.calculate(0), ref) is None
     with pytest.raises(capi.AnlError):
-        g.autotune(0, 2, [(5, 0, 20)])
+        g.autotune(0, 2, [(5, 0, 20, 1)])
 
 
-def test_full_size_triclinic_invariants():
-    """16,384 atoms (80/10/10), triclinic L=56 with tilts, lags 0..100, 256
-    bins to 12 A, a few t0: both forms agree bit for bit; lag-0 self channels
-    hold N_type per t0 in bin 0; lag-0 distinct counts are even."""
+# Boxes large against rmax, so that compact wavefronts take the cluster screen:
+# id, natoms, nframes, L, fractions, sigma, tilt, wrap, unwrapped shift, rmax,
+# first frame replaced by uniform random positions (None: none)
+CLUSTER_CASES = [
+    ("tri", 4000, 10, 40.0, [0.7, 0.3], 0.3, [3.1, -2.7, 2.2], False, 0.0, 7.0, None),
+    ("ortho-unwrapped", 3000, 9, 36.0, [0.5, 0.5], 0.4, None, False, 61.0, 6.5, None),
+    ("tri-wrapped", 2500, 8, 34.0, [1.0], 0.5, [-2.0, 1.5, -1.0], True, 0.0, 8.0, None),
+    ("tri-mixed", 4000, 12, 40.0, [0.6, 0.4], 0.3, [1.0, 1.0, -1.0], False, 0.0, 6.0, 6),
+]
+
+
+@pytest.mark.parametrize("case", CLUSTER_CASES, ids=[c[0] for c in CLUSTER_CASES])
+def test_cluster_screen_matches_oracle(case):
+    """The cluster (FP16 MFMA) screen gives the oracle's counts.  In the last
+    case the frames from 6 on are uniform random positions: their wavefronts
+    are not compact and fall back to the 16-bit screen in the same launch,
+    while the earlier frames' wavefronts use the cluster screen."""
     from analisi_amd import synth
-    N, F = 16384, 106
-    tilt = [3.1, -4.2, 2.7]
-    box_l = synth.lammps_box(56.0, tilt)
-    pos = synth.random_walk(42, N, F, box_l, 0.05)
-    types = synth.types_by_fraction(N, [0.8, 0.1, 0.1])
-    ot = O.Traj(pos[:1], types, box_l[None], True)           # type ids only
-    from oracle.oracle import lib, _p
-    boxi = np.tile(ot.box[0], (F, 1))
-    dt = capi.DeviceTrajectory(pos, boxi, ot.type_id, ot.ntypes, None)
-    n = 3
-    res = []
-    for form in (capi.GOFRT_FP64, capi.GOFRT_SCREENED):
-        g = capi.DeviceGofrt(dt, 0.0, 12.0, 256, 101, 1, 1, form)
-        g.reset(n)
-        res.append(g.calculate(2))
-    assert first_diff(res[0], res[1]) is None, first_diff(res[0], res[1])
-    c = res[0]
-    half = 6
-    counts = np.bincount(ot.type_id, minlength=3)
-    for t in range(3):
-        ch = lib().orc_gofrt_channel(3, t, t, 1)
-        assert c[0, ch, 0] == counts[t] * n
-        assert c[0, ch, 1:].sum() == 0
-    assert np.all(c[0, :half] % 2 == 0)
-    # every lag sees each ordered pair at most once per t0
-    assert c.sum(axis=(1, 2)).max() <= n * N * N
+    name, N, F, L, fr, sig, tilt, wrap, shift, rmax, scramble = case
+    if scramble is None:
+        ot, dt = make_case(21, N, F, L, fr, sig, tilt, wrap, shift)
+    else:
+        box_l = synth.lammps_box(L, tilt)
+        pos = synth.random_walk(21, N, F, box_l, sig)
+        for f in range(scramble, F):
+            pos[f] = synth.random_walk(1000 + f, N, 1, box_l, 0.0)[0]
+        types = synth.types_by_fraction(N, fr)
+        ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), tilt is not None)
+        dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    nbin, tmax, n = 64, 3, F - 3
+    ref, _ = O.gofrt(ot, 0.0, rmax, nbin, tmax, 8, 1, 1, n, 1, want_vdata=False)
+    g = capi.DeviceGofrt(dt, 0.0, rmax, nbin, tmax, 1, 1, capi.GOFRT_SCREENED)
+    g.reset(n)
+    for cl in (1, 0):
+        g.set_tuning(8, 0, 1 << 30, cl)
+        got = g.calculate(1)
+        assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
+        c, tot = g.cluster_stats()
+        if cl:
+            assert 0 < c <= tot, (c, tot)
+            if scramble is not None:
+                assert c < tot, (c, tot)
+        else:
+            assert tot == 0
