@@ -95,69 +95,102 @@ __global__ __launch_bounds__(256) void gk_lag_kernel(const GkArgs a) {
     for (uint32_t k = NP; k < a.narr; k++) o[k] = 0.0;
 }
 
-// :259-277 subtract the mean of C over lags [start_mean, leff).  One thread per
-// column; the serial running mean keeps the reference's order, loads run
-// ahead in batches of 16 lags.
-__global__ void gk_mean_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr, uint32_t leff,
-                               uint32_t start_mean, uint64_t stride) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncorr) return;
+// The serial scans over lags below read C(it) of every correlation column
+// through LDS: waves 1-3 of the workgroup load the next tile of lags (all
+// columns, coalesced) while the lanes of wave 0 that own a column scan the
+// current tile in the reference's order.  Tiles double-buffer; one barrier
+// per tile.
+__device__ __forceinline__ void gk_load_tile(double *__restrict__ dst, const double *__restrict__ out, uint32_t narr,
+                                             uint32_t ncorr, uint32_t it0, uint32_t leff, uint32_t tl, uint32_t t,
+                                             uint32_t nt) {
+    for (uint32_t e = t; e < tl * ncorr; e += nt) {
+        const uint32_t k = e / ncorr, jj = e - k * ncorr;
+        const uint32_t it = it0 + k;
+        dst[e] = it < leff ? out[(uint64_t)it * narr + jj] : 0.0;
+    }
+}
+
+// :259-277 subtract the mean of C over lags [start_mean, leff).
+__global__ __launch_bounds__(256) void gk_mean_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr,
+                                                      uint32_t leff, uint32_t start_mean, uint64_t stride,
+                                                      uint32_t tl) {
+    extern __shared__ double tile[];  // [2][tl][ncorr], then [ncorr] means
     out += blockIdx.y * stride;
+    const uint32_t t = threadIdx.x;
+    double *mean_lds = tile + 2 * tl * ncorr;
     double m = 0.0;
     uint32_t c = 0;
-    constexpr uint32_t B = 16;
-    for (uint32_t it0 = start_mean; it0 < leff; it0 += B) {
-        double v[B];
-#pragma unroll
-        for (uint32_t k = 0; k < B; k++) v[k] = it0 + k < leff ? out[(uint64_t)(it0 + k) * narr + j] : 0.0;
-#pragma unroll
-        for (uint32_t k = 0; k < B; k++) {
-            if (it0 + k >= leff) break;
-            c++;
-            const double d = v[k] - m;
-            m = m + d / (double)c;
+    if (start_mean < leff) {
+        if (t >= 64) gk_load_tile(tile, out, narr, ncorr, start_mean, leff, tl, t - 64, blockDim.x - 64);
+        __syncthreads();
+        uint32_t buf = 0;
+        for (uint32_t it0 = start_mean; it0 < leff; it0 += tl) {
+            if (t >= 64 && it0 + tl < leff)
+                gk_load_tile(tile + (buf ^ 1) * tl * ncorr, out, narr, ncorr, it0 + tl, leff, tl, t - 64,
+                             blockDim.x - 64);
+            if (t < ncorr) {
+                const double *col = tile + buf * tl * ncorr + t;
+                const uint32_t n = min(tl, leff - it0);
+                for (uint32_t k = 0; k < n; k++) {
+                    c++;
+                    const double d = col[k * ncorr] - m;
+                    m = m + d / (double)c;
+                }
+            }
+            __syncthreads();
+            buf ^= 1;
         }
     }
-    for (uint32_t it = 0; it < leff; it++) out[(uint64_t)it * narr + j] = out[(uint64_t)it * narr + j] - m;
+    if (t < ncorr) mean_lds[t] = m;
+    __syncthreads();
+    for (uint32_t e = t; e < leff * ncorr; e += blockDim.x) {
+        const uint32_t it = e / ncorr, jj = e - it * ncorr;
+        double *v = out + (uint64_t)it * narr + jj;
+        *v = *v - mean_lds[jj];
+    }
 }
 
 // :279-288 trapezoid I and Einstein E running integrals, per correlation
 // column: I(it) = (I(it-1) + C(it-1)/2) + C(it)/2, E(it) = (E(it-1) +
 // C(it-1) it/2) + C(it) it/2, with I(0) = E(0) = 0 as left by the lag loop.
-// The running values stay in registers (the reference's order); the C(it)
-// loads of the next 32 lags are in flight while the current 32 are summed.
-__global__ void gk_integral_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr, uint32_t leff,
-                                   uint64_t stride) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncorr || leff == 0) return;
+__global__ __launch_bounds__(256) void gk_integral_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr,
+                                                          uint32_t leff, uint64_t stride, uint32_t tl) {
+    extern __shared__ double tile[];  // [2][tl][ncorr]
+    if (leff < 2) return;
     out += blockIdx.y * stride;
-    double I = out[ncorr + 2 * j], E = out[ncorr + 2 * j + 1], cprev = out[j];
-    constexpr uint32_t B = 32;
-    double v[B], w[B];
-#pragma unroll
-    for (uint32_t k = 0; k < B; k++) v[k] = 1 + k < leff ? out[(uint64_t)(1 + k) * narr + j] : 0.0;
-    for (uint32_t it0 = 1; it0 < leff; it0 += B) {
-#pragma unroll
-        for (uint32_t k = 0; k < B; k++) {
-            const uint32_t it = it0 + B + k;
-            w[k] = it < leff ? out[(uint64_t)it * narr + j] : 0.0;
+    const uint32_t t = threadIdx.x;
+    double I = 0.0, E = 0.0, cprev = 0.0;
+    if (t < ncorr) {
+        I = out[ncorr + 2 * t];
+        E = out[ncorr + 2 * t + 1];
+        cprev = out[t];
+    }
+    if (t >= 64) gk_load_tile(tile, out, narr, ncorr, 1, leff, tl, t - 64, blockDim.x - 64);
+    __syncthreads();
+    uint32_t buf = 0;
+    for (uint32_t it0 = 1; it0 < leff; it0 += tl) {
+        if (t >= 64 && it0 + tl < leff)
+            gk_load_tile(tile + (buf ^ 1) * tl * ncorr, out, narr, ncorr, it0 + tl, leff, tl, t - 64,
+                         blockDim.x - 64);
+        if (t < ncorr) {
+            const double *col = tile + buf * tl * ncorr + t;
+            const uint32_t n = min(tl, leff - it0);
+            for (uint32_t k = 0; k < n; k++) {
+                const uint32_t it = it0 + k;
+                const double v = col[k * ncorr];
+                const double dit = (double)it;
+                const double In = I + cprev / 2.0;
+                I = In + v / 2.0;
+                const double En = E + cprev * dit / 2.0;
+                E = En + v * dit / 2.0;
+                double *cur = out + (uint64_t)it * narr;
+                cur[ncorr + 2 * t] = I;
+                cur[ncorr + 2 * t + 1] = E;
+                cprev = v;
+            }
         }
-#pragma unroll
-        for (uint32_t k = 0; k < B; k++) {
-            const uint32_t it = it0 + k;
-            if (it >= leff) break;
-            const double dit = (double)it;
-            double *cur = out + (uint64_t)it * narr;
-            const double In = I + cprev / 2.0;
-            I = In + v[k] / 2.0;
-            const double En = E + cprev * dit / 2.0;
-            E = En + v[k] * dit / 2.0;
-            cur[ncorr + 2 * j] = I;
-            cur[ncorr + 2 * j + 1] = E;
-            cprev = v[k];
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < B; k++) v[k] = w[k];
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
@@ -251,11 +284,15 @@ __global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_
 void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
                  hipStream_t s, uint32_t nblocks, uint64_t stride) {
     const uint32_t ncorr = ncur * ncur;
+    // lag tile of the scans: two buffers of tl x ncorr doubles, <= 64 KiB
+    const uint32_t tl = std::max(16u, 4096u / ncorr);
+    const size_t lds = (2 * (size_t)tl * ncorr + ncorr) * sizeof(double);
     if (subtract_mean) {
-        hipLaunchKernelGGL(gk_mean_kernel, dim3(1, nblocks), dim3(64), 0, s, d, narr, ncorr, leff, start_mean, stride);
+        hipLaunchKernelGGL(gk_mean_kernel, dim3(1, nblocks), dim3(256), lds, s, d, narr, ncorr, leff, start_mean,
+                           stride, tl);
         ANL_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(gk_integral_kernel, dim3(1, nblocks), dim3(64), 0, s, d, narr, ncorr, leff, stride);
+    hipLaunchKernelGGL(gk_integral_kernel, dim3(1, nblocks), dim3(256), lds, s, d, narr, ncorr, leff, stride, tl);
     ANL_HIP(hipGetLastError());
     hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256, nblocks), dim3(256), 0, s, d, narr, ncur, leff,
                        stride);

@@ -405,9 +405,9 @@ def run_msd(w, args, hip, traj, nt):
 # block engine (c4)
 # ---------------------------------------------------------------------------
 
-def c4_windows(w):
+def c4_windows(w, nb=None):
     from analisi_amd import synth
-    nb, F = w["nblocks"], w["F"]
+    nb, F = nb or w["nblocks"], w["F"]
     u1 = synth.uniform(w["seed"], 21, nb)
     u2 = synth.uniform(w["seed"], 22, nb)
     lens = (500 + np.floor(u1 * 3501)).astype(np.int64).clip(500, 4000)
@@ -416,9 +416,12 @@ def c4_windows(w):
 
 
 def run_blocks(w, args, hip):
+    """The block engine on the c4 windows.  With --c4-split K the windows are
+    cut into K consecutive groups, one engine (one graph) each, replayed one
+    after the other inside the timed step; K = 1 is one graph for all."""
     from analisi_amd import capi
     traj, tid, nt = device_traj(w, with_cm=False)
-    starts, lens = c4_windows(w)
+    starts, lens = c4_windows(w, args.c4_windows)
     g = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_SCREENED)
     tune = None
     if not args.no_autotune:
@@ -426,57 +429,75 @@ def run_blocks(w, args, hip):
         ms_c, best = g.autotune(int(starts[0]), 2)
         tune = {"candidates_ms": [round(float(x), 2) for x in ms_c], "chosen": list(g.tuning())}
         log("autotune: %s -> %s" % (tune["candidates_ms"], tune["chosen"]))
+    K = max(1, args.c4_split)
+    cuts = np.linspace(0, len(lens), K + 1).astype(int)
     t = time.perf_counter()
-    B = capi.DeviceBlocks(traj, starts, lens, gofrt=g, msd=True, msd_skip=1, msd_tmax=w["msd_tmax"], use_graph=True)
+    engines = [capi.DeviceBlocks(traj, starts[a:b], lens[a:b], gofrt=g, msd=True, msd_skip=1,
+                                 msd_tmax=w["msd_tmax"], use_graph=True) for a, b in zip(cuts[:-1], cuts[1:])]
     t_create = time.perf_counter() - t
+    log("block engine ready: %d windows in %d graph(s), %d frames in all, set up in %.1f s"
+        % (len(lens), K, int(lens.sum()), t_create))
     stream = hip.stream()
-    B.run(stream)
-    hip.sync()
-    e0, e1 = hip.event(), hip.event()
+    for i in range(args.warmup):
+        for B in engines:
+            B.run(stream)
+        hip.sync()
+        log("warmup %d done" % i)
     reps = max(1, args.steps)
-    hip.record(e0, stream)
-    for _ in range(reps):
-        B.run(stream)
-    hip.record(e1, stream)
-    ms = hip.elapsed_ms(e0, e1) / reps
-    gmean, gvar, gcounts = B.gofrt_results()
-    mmean, mvar, mper = B.msd_results()
+    ms = 0.0
+    for r in range(reps):
+        for k, B in enumerate(engines):
+            e0, e1 = hip.event(), hip.event()
+            hip.record(e0, stream)
+            B.run(stream)
+            hip.record(e1, stream)
+            dt = hip.elapsed_ms(e0, e1)
+            ms += dt
+            log("step %d graph %d/%d: %.1f ms" % (r, k + 1, K, dt))
+    ms /= reps
     N = w["N"]
     upd = sum(n_updates(N, int(n), w["tmax"]) for n in lens)
     mupd = sum(float(N) * int(n) * min(int(n), w["msd_tmax"]) for n in lens)
     # one window recomputed alone: FP64 g(r,t) and MSD, bit-identical
     b = int(np.argmin(lens))
+    k = int(np.searchsorted(cuts, b, side="right") - 1)
+    gmean, gvar, gcounts = engines[k].gofrt_results()
+    mmean, mvar, mper = engines[k].msd_results()
+    bl = b - cuts[k]
     g64 = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_FP64)
     g64.reset(int(lens[b]))
-    same_g = bool(np.array_equal(g64.calculate(int(starts[b])), gcounts[b]))
+    same_g = bool(np.array_equal(g64.calculate(int(starts[b])), gcounts[bl]))
     m1 = capi.msd(traj, 1, w["msd_tmax"], False, False, int(lens[b]), int(starts[b]))
-    same_m = bool(np.array_equal(m1, mper[b]))
+    same_m = bool(np.array_equal(m1, mper[bl]))
     res = {"metric": "block-averaged g(r,t) + MSD, pair-lag updates/s over all windows",
            "value": upd / (ms * 1e-3), "unit": "pair-lag updates/s", "n_gpus": 1, "ms_per_step": ms,
            "steps": reps, "dtype": "i16 screen + f32 + f64 recheck; MSD f64",
            "config": {"workload": "block engine: %s; %d windows of 500..4000 frames; Gofrt rmax %g, %d bins, "
-                                  "lags 0..%d; MSD lags 0..%d; one HIP graph per step"
-                                  % (describe(w), len(lens), w["rmax"], w["nbin"], w["tmax"] - 1, w["msd_tmax"] - 1),
-                      "sum_window_frames": int(lens.sum()), "gofrt_updates": upd, "msd_updates": mupd},
+                                  "lags 0..%d; MSD lags 0..%d; %d HIP graph(s) per step"
+                                  % (describe(w), len(lens), w["rmax"], w["nbin"], w["tmax"] - 1,
+                                     w["msd_tmax"] - 1, K),
+                      "sum_window_frames": int(lens.sum()), "gofrt_updates": upd, "msd_updates": mupd,
+                      "launch_shape": list(g.tuning())},
            "setup_s": t_create, "autotune": tune,
            "checks": {"window_%d_gofrt_fp64_identical" % b: same_g, "window_%d_msd_identical" % b: same_m}}
     if not args.no_cpu:
         from oracle import oracle as O
+        nw = cuts[k + 1] - cuts[k]
         mv = O.MediaVar(gmean.size)
-        incr = [1.0 / int(n) for n in lens]
-        for k in range(len(lens)):
-            mv.add(gcounts[k].ravel().astype(np.float64) * incr[k])
-        mv.end(len(lens))
+        for q in range(nw):
+            mv.add(gcounts[q].ravel().astype(np.float64) * (1.0 / int(lens[cuts[k] + q])))
+        mv.end(nw)
         mv2 = O.MediaVar(mmean.size)
-        for k in range(len(lens)):
-            mv2.add(mper[k].ravel())
-        mv2.end(len(lens))
+        for q in range(nw):
+            mv2.add(mper[q].ravel())
+        mv2.end(nw)
         res["cpu_baseline"] = None
         res["checks"]["mediavar_identical_to_oracle"] = bool(
             np.array_equal(gmean.ravel(), mv.mean) and np.array_equal(gvar.ravel(), mv.var)
             and np.array_equal(mmean.ravel(), mv2.mean) and np.array_equal(mvar.ravel(), mv2.var))
     log("blocks: %.1f ms/step, checks %s" % (ms, res["checks"]))
-    B.close()
+    for B in engines:
+        B.close()
     return res
 
 
@@ -599,6 +620,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-lags", type=int, default=4, help="lags per autotune probe launch")
     ap.add_argument("--cpu-n", type=int, default=8, help="reset(n) of the CPU baseline sample (n t0 x n lags)")
+    ap.add_argument("--c4-windows", type=int, default=0, help="number of c4 windows (0 = the workload's 64)")
+    ap.add_argument("--c4-split", type=int, default=1, help="c4 windows cut into this many graphs per step")
     ap.add_argument("--gk-cpu-n", type=int, default=6000, help="reset(n), n lags, of the GK oracle sample")
     ap.add_argument("--msd-cpu-n", type=int, default=20, help="reset(n) of the MSD oracle sample")
     ap.add_argument("--fp64-n", type=int, default=101, help="reset(n) of the plain-FP64 slice")
