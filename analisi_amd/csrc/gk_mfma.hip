@@ -126,13 +126,19 @@ __global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
             wh[j] = ld8(YH + 16 * (st0 + j));
             wl[j] = ld8(YL + 16 * (st0 + j));
         }
+        // X operands of the next step are loaded while this step's MFMAs run
+        bf16x8 ahc = ld8(XH + 16 * st0), alc = ld8(XL + 16 * st0);
         for (uint32_t st = st0; st < st0 + GKM_CS; st += GKM_Q) {
 #pragma unroll
             for (uint32_t u = 0; u < GKM_Q; u++) {
                 const uint32_t sn = st + u;
                 wh[(u + GKM_Q - 1) % GKM_Q] = ld8(YH + 16 * (sn + GKM_Q - 1));
                 wl[(u + GKM_Q - 1) % GKM_Q] = ld8(YL + 16 * (sn + GKM_Q - 1));
-                const bf16x8 ah = ld8(XH + 16 * sn), al = ld8(XL + 16 * sn);
+                const bf16x8 ah = ahc, al = alc;
+                if (u + 1 < GKM_Q || st + GKM_Q < st0 + GKM_CS) {  // next step inside this chunk
+                    ahc = ld8(XH + 16 * (sn + 1));
+                    alc = ld8(XL + 16 * (sn + 1));
+                }
 #pragma unroll
                 for (uint32_t j = 0; j < GKM_Q; j++) {
                     const uint32_t w = (u + j) % GKM_Q;

@@ -933,6 +933,9 @@ struct GofrtPrepared {
     bool triclinic = false;
     int unroll = 4;
     bool cluster = false;
+    // device work list and block table owned by this launch (block engine)
+    BlockInfo *own_blocks = nullptr;
+    WorkItem *own_work = nullptr;
 };
 
 __global__ void fill_u64_kernel(unsigned long long *__restrict__ p, uint64_t len) {
@@ -946,24 +949,48 @@ static void zero_u64(unsigned long long *p, uint64_t len, hipStream_t s) {
     ANL_HIP(hipGetLastError());
 }
 
+// own = true: the work list and block table go to buffers owned by the
+// prepared launch (the block engine keeps several prepared launches of one
+// g alive and replays them later); otherwise to g's scratch, after the device
+// has finished any earlier launch that reads it.
 GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int a0, int a1, hipStream_t s,
-                            uint32_t lag0, uint32_t nlag_force) {
+                            uint32_t lag0, uint32_t nlag_force, bool own = false) {
     anl_traj *tr = g->tr;
     GofrtPrepared P;
     const int form = choose_form(g);
     if (form == ANL_GOFRT_SCREENED) build_frac16(tr, s);
-    if (L.blocks.size() > g->cap_blocks) {
-        (void)hipFree(g->d_blocks);
-        ANL_HIP(hipMalloc(&g->d_blocks, L.blocks.size() * sizeof(BlockInfo)));
-        g->cap_blocks = L.blocks.size();
+    BlockInfo *d_blocks = nullptr;
+    WorkItem *d_work = nullptr;
+    if (own) {
+        ANL_HIP(hipMalloc(&P.own_blocks, std::max<size_t>(1, L.blocks.size()) * sizeof(BlockInfo)));
+        d_blocks = P.own_blocks;
+        hipError_t e = hipMalloc(&P.own_work, std::max<size_t>(1, L.work.size()) * sizeof(WorkItem));
+        if (e != hipSuccess) {
+            (void)hipFree(P.own_blocks);
+            ANL_HIP(e);
+        }
+        d_work = P.own_work;
+    } else {
+        ANL_HIP(hipDeviceSynchronize());
+        if (L.blocks.size() > g->cap_blocks) {
+            (void)hipFree(g->d_blocks);
+            g->d_blocks = nullptr;
+            g->cap_blocks = 0;
+            ANL_HIP(hipMalloc(&g->d_blocks, L.blocks.size() * sizeof(BlockInfo)));
+            g->cap_blocks = L.blocks.size();
+        }
+        if (L.work.size() > g->cap_work) {
+            (void)hipFree(g->d_work);
+            g->d_work = nullptr;
+            g->cap_work = 0;
+            ANL_HIP(hipMalloc(&g->d_work, L.work.size() * sizeof(WorkItem)));
+            g->cap_work = L.work.size();
+        }
+        d_blocks = g->d_blocks;
+        d_work = g->d_work;
     }
-    if (L.work.size() > g->cap_work) {
-        (void)hipFree(g->d_work);
-        ANL_HIP(hipMalloc(&g->d_work, L.work.size() * sizeof(WorkItem)));
-        g->cap_work = L.work.size();
-    }
-    ANL_HIP(hipMemcpyAsync(g->d_blocks, L.blocks.data(), L.blocks.size() * sizeof(BlockInfo), hipMemcpyHostToDevice, s));
-    ANL_HIP(hipMemcpyAsync(g->d_work, L.work.data(), L.work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    ANL_HIP(hipMemcpyAsync(d_blocks, L.blocks.data(), L.blocks.size() * sizeof(BlockInfo), hipMemcpyHostToDevice, s));
+    ANL_HIP(hipMemcpyAsync(d_work, L.work.data(), L.work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
     ANL_HIP(hipStreamSynchronize(s));  // the host vectors die with L
 
     GofrtArgs &a = P.a;
@@ -976,8 +1003,8 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
     a.ncand = g->d_ncand;
     a.thr = g->d_thr;
     a.chan = g->d_chan;
-    a.work = g->d_work;
-    a.blocks = g->d_blocks;
+    a.work = d_work;
+    a.blocks = d_blocks;
     a.N = tr->N;
     a.nt = tr->nt;
     a.nbin = g->nbin;
@@ -1092,14 +1119,20 @@ void *gofrt_prepare_blocks(anl_gofrt *g, const uint64_t *primo, const uint32_t *
     check_lengths(g, primo, n, nblocks);
     Launch L = plan(g, primo, n, nblocks, -1);
     const uint64_t len = (uint64_t)nblocks * L.leff_max * nch_of(g->tr) * g->nbin;
-    auto *P = new GofrtPrepared(gofrt_prepare(g, L, len, INT32_MIN, INT32_MAX, s, 0, 0));
+    auto *P = new GofrtPrepared(gofrt_prepare(g, L, len, INT32_MIN, INT32_MAX, s, 0, 0, true));
     if (leff_max) *leff_max = L.leff_max;
     return P;
 }
 void gofrt_enqueue_prepared(anl_gofrt *g, void *prep, unsigned long long *out, hipStream_t s) {
     gofrt_enqueue(g, *(GofrtPrepared *)prep, out, s);
 }
-void gofrt_free_prepared(void *prep) { delete (GofrtPrepared *)prep; }
+void gofrt_free_prepared(void *prep) {
+    auto *P = (GofrtPrepared *)prep;
+    if (!P) return;
+    (void)hipFree(P->own_blocks);
+    (void)hipFree(P->own_work);
+    delete P;
+}
 int gofrt_form_of(void *prep) { return ((GofrtPrepared *)prep)->form; }
 uint32_t gofrt_block_leff(const anl_gofrt *g, uint32_t n) { return (n < g->lmax || g->lmax == 0) ? n : g->lmax; }
 uint32_t gofrt_nbin(const anl_gofrt *g) { return g->nbin; }
@@ -1322,8 +1355,13 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         for (uint32_t c = 0; c < ncand; c++)
             if ((cand[4 * c] != 4 && cand[4 * c] != 8) || cand[4 * c + 2] < 10 || cand[4 * c + 2] > 40)
                 fail(ANL_ERR_ARG, "candidate " + std::to_string(c) + ": unroll 4|8, log2(pairs per workgroup) in [10, 40]");
+        // probe lags: at least nlag_probe, and enough for >= 4096 workgroups
+        // (several waves of residency), so that the probe behaves like the run
         const uint32_t nlag_all = (g->leff + g->every - 1) / g->every;
-        const uint32_t nlag = std::max(1u, std::min(nlag_probe ? nlag_probe : 1u, nlag_all));
+        const uint64_t per_lag = (uint64_t)((g->tr->N + WG - 1) / WG) * plan(g, &primo, &g->n, 1, -1).work.size();
+        const uint64_t fill = (4096 + per_lag - 1) / std::max<uint64_t>(per_lag, 1);
+        const uint32_t nlag = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(std::max<uint64_t>(nlag_probe, fill), nlag_all));
         const uint64_t len = (uint64_t)g->leff * nch_of(g->tr) * g->nbin;
         if (len == 0) return;
         const GofrtTuning saved = g->tune;

@@ -110,9 +110,9 @@ int anl_gofrt_cluster_stats(anl_gofrt *g, uint64_t *clustered, uint64_t *units);
  * tiles).  Shapes change speed only, never results.  anl_gofrt_autotune
  * times each candidate (cand[4*c] = unroll, cand[4*c+1] = xcd, cand[4*c+2] =
  * log2 pairs per workgroup, cand[4*c+3] = cluster; NULL = a built-in set of
- * 8) on the first nlag_probe lags of the calculation set by reset() at
- * primo, keeps the fastest and reports ms_out[c] (may be NULL) and its
- * index.  The reference's equivalent knob is nthreads
+ * 8) on the first lags of the calculation set by reset() at primo (at least
+ * nlag_probe, raised until the probe launch has >= 4096 workgroups), keeps
+ * the fastest and reports ms_out[c] (may be NULL) and its index.  The reference's equivalent knob is nthreads
  * (calculatemultithread.h:80-104); no counterpart for the rest. */
 int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_wg, int cluster);
 int anl_gofrt_get_tuning(anl_gofrt *g, int *unroll, int *xcd, uint64_t *pairs_per_wg, int *cluster);

@@ -84,3 +84,25 @@ def test_blocks_reject_unequal_lags_and_short():
     with pytest.raises(capi.AnlError) as e:
         capi.DeviceBlocks(dt, [0, 90], [40, 40], gofrt=g)
     assert e.value.code == capi.ANL_ERR_TOO_SHORT
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_several_engines_share_one_gofrt(graph):
+    """Several block engines built from one Gofrt, replayed after all of them
+    exist and in an interleaved order: each window equals its own single call
+    (each engine keeps its own work list and block table on the device)."""
+    F = 500
+    ot, dt = make_case(23, 350, F, 15.0, [0.7, 0.3], 0.1, tilt=[0.5, -0.4, 0.3])
+    g = capi.DeviceGofrt(dt, 0.0, 5.0, 32, 12, 1, 1, capi.GOFRT_AUTO)
+    sets = [_blocks(7 + k, 3 + k, F, 40 + 10 * k, 90 + 20 * k, 20) for k in range(3)]
+    engines = [capi.DeviceBlocks(dt, p, n, gofrt=g, msd=False, use_graph=graph) for p, n in sets]
+    for order in ([2, 0, 1], [1, 2, 0]):
+        for k in order:
+            engines[k].run()
+        g1 = capi.DeviceGofrt(dt, 0.0, 5.0, 32, 12, 1, 1, capi.GOFRT_FP64)  # single calls between replays
+        for k, (p, n) in enumerate(sets):
+            _, _, counts = engines[k].gofrt_results()
+            for b in range(len(p)):
+                g1.reset(int(n[b]))
+                one = g1.calculate(int(p[b]))
+                assert first_diff(counts[b], one) is None, (order, k, b, first_diff(counts[b], one))
