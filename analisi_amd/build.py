@@ -34,13 +34,18 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+# per-file extra flags: the g(r,t) kernels take MFMA results in VGPRs (the
+# cluster screen compares them right away; AGPR results would need a copy)
+FILE_FLAGS = {"gofrt.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def build_hip():
     objs = []
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
     for src in HIP_SOURCES:
         obj = os.path.join(bdir, src.replace(".hip", ".o"))
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)

@@ -491,7 +491,7 @@ __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e
 }
 
 constexpr int SQCAP = 576;   // per-wavefront queue: < 64 left + up to 8 masks of 64
-constexpr int SQCAP_CL = 2112;  // cluster screen: < 64 left + 32 masks of 64 (one drain per 32 j)
+constexpr int SQCAP_CL = 1088;  // cluster screen: < 64 left + 16 masks of 64 (one drain per 32 x 32 tile)
 constexpr int RQCAP = 128;   // per-wavefront recheck queue: < 64 left + one batch
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -683,16 +683,17 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
                     const uint2 *__restrict__ fj16 = a.frac + fj * N;
                     const uint32_t wbase = threadIdx.x & ~63u;
                     const uint32_t half4 = 4u * (lane >> 5);
-                    // lanes 0-31 build column lane of the 32 j atoms j0..j0+31;
-                    // the next block's 16-bit fractions are loaded one
-                    // iteration ahead
+                    // every lane builds the column of j = j0 + lane; lanes 0-31's
+                    // columns form the B operand of the first 32 j, lanes 32-63's
+                    // (moved down) that of the next 32.  The next 64 j's 16-bit
+                    // fractions are loaded one step ahead.
                     const bool colane = lane < 32;
                     uint2 gn = make_uint2(0u, 0u);
-                    if (colane && lane < (uint32_t)N) gn = fj16[lane];
-                    for (uint32_t j0 = 0; j0 < (uint32_t)N; j0 += 32) {
+                    if (lane < (uint32_t)N) gn = fj16[lane];
+                    for (uint32_t j0 = 0; j0 < (uint32_t)N; j0 += 64) {
                         const uint2 g = gn;
-                        const bool jin = colane && j0 + lane < (uint32_t)N;
-                        if (colane && j0 + 32 + lane < (uint32_t)N) gn = fj16[j0 + 32 + lane];
+                        const bool jin = j0 + lane < (uint32_t)N;
+                        if (j0 + 64 + lane < (uint32_t)N) gn = fj16[j0 + 64 + lane];
                         const float ga = (float)s16(g.x - ca), gb = (float)s16((g.x >> 16) - cb);
                         const float gc = (float)s16(g.y - cc);
                         float qx, qy, qz;
@@ -706,48 +707,62 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
                             qz = a.hz * gc;
                         }
                         const float q2 = fmaf(qz, qz, fmaf(qy, qy, qx * qx));
-                        // column (q, 1, |q|^2); past N: (0, 0, 0, 1, 60000); lanes 32-63: 0
-                        f16x8_t B = {};
-                        B[0] = (_Float16)(jin ? qx : 0.0f);
-                        B[1] = (_Float16)(jin ? qy : 0.0f);
-                        B[2] = (_Float16)(jin ? qz : 0.0f);
-                        B[3] = (_Float16)(colane ? 1.0f : 0.0f);
-                        B[4] = (_Float16)(jin ? q2 : (colane ? 60000.0f : 0.0f));
-                        const f32x16_t D0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A0, B, f32x16_t{}, 0, 0, 0);
-                        const f32x16_t D1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, B, f32x16_t{}, 0, 0, 0);
-                        // entry = j << 8 | local i; item e of tile T is row
-                        // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
-                        const uint32_t eb = ((j0 + (lane & 31u)) << 8) | (wbase + half4);
+                        // column (q, 1, |q|^2); past N: (0, 0, 0, 1, 60000)
+                        f16x8_t col = {};
+                        col[0] = (_Float16)(jin ? qx : 0.0f);
+                        col[1] = (_Float16)(jin ? qy : 0.0f);
+                        col[2] = (_Float16)(jin ? qz : 0.0f);
+                        col[3] = (_Float16)1.0f;
+                        col[4] = (_Float16)(jin ? q2 : 60000.0f);
+                        const uint4 cw = __builtin_bit_cast(uint4, col);
+                        uint4 cdn;
+                        cdn.x = __shfl(cw.x, (int)(lane ^ 32u), 64);
+                        cdn.y = __shfl(cw.y, (int)(lane ^ 32u), 64);
+                        cdn.z = __shfl(cw.z, (int)(lane ^ 32u), 64);
+                        cdn.w = __shfl(cw.w, (int)(lane ^ 32u), 64);
+                        const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+                        const f16x8_t Bj[2] = {__builtin_bit_cast(f16x8_t, colane ? cw : z4),
+                                               __builtin_bit_cast(f16x8_t, colane ? cdn : z4)};
 #pragma unroll
-                        for (int grp = 0; grp < 4; grp++) {
-                            bool pc[8];
-                            unsigned long long mk[8];
-                            unsigned long long any = 0;
+                        for (int jb = 0; jb < 2; jb++) {
+                            // entry = j << 8 | local i; item e of tile T is row
+                            // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
+                            const uint32_t eb = ((j0 + 32u * (uint32_t)jb + (lane & 31u)) << 8) | (wbase + half4);
 #pragma unroll
-                            for (int u = 0; u < 8; u++) {
-                                const int e = 8 * (grp & 1) + u;
-                                pc[u] = (grp < 2 ? D0[e] : D1[e]) < thc;
-                                mk[u] = __ballot(pc[u]);
-                                any |= mk[u];
-                            }
-                            if (any) {
+                            for (int T = 0; T < 2; T++) {
+                                const f32x16_t D = __builtin_amdgcn_mfma_f32_32x32x16_f16(T ? A1 : A0, Bj[jb],
+                                                                                          f32x16_t{}, 0, 0, 0);
 #pragma unroll
-                                for (int u = 0; u < 8; u++) {
-                                    const int e = 8 * (grp & 1) + u;
-                                    const uint32_t roff = 32u * (uint32_t)(grp >> 1) + 8u * (uint32_t)(e >> 2) +
-                                                          (uint32_t)(e & 3);
-                                    ANL_APPEND_E(pc[u], mk[u], eb + roff)
+                                for (int grp = 0; grp < 2; grp++) {
+                                    bool pc[8];
+                                    unsigned long long mk[8];
+                                    unsigned long long any = 0;
+#pragma unroll
+                                    for (int u = 0; u < 8; u++) {
+                                        pc[u] = D[8 * grp + u] < thc;
+                                        mk[u] = __ballot(pc[u]);
+                                        any |= mk[u];
+                                    }
+                                    if (any) {
+#pragma unroll
+                                        for (int u = 0; u < 8; u++) {
+                                            const int e = 8 * grp + u;
+                                            const uint32_t roff = 32u * (uint32_t)T + 8u * (uint32_t)(e >> 2) +
+                                                                  (uint32_t)(e & 3);
+                                            ANL_APPEND_E(pc[u], mk[u], eb + roff)
+                                        }
+                                    }
+                                }
+                                // one drain site per 32 x 32 tile (queue holds < 64 + 16 masks)
+                                while (qn >= 64) {
+                                    wave_sync_lds();
+                                    qn -= 64;
+                                    const uint32_t e = q[qn + lane];
+                                    ncand += 1;
+                                    eval_batch(e, true);
+                                    wave_sync_lds();
                                 }
                             }
-                        }
-                        // one drain site per 32 j (queue holds < 64 + 32 masks)
-                        while (qn >= 64) {
-                            wave_sync_lds();
-                            qn -= 64;
-                            const uint32_t e = q[qn + lane];
-                            ncand += 1;
-                            eval_batch(e, true);
-                            wave_sync_lds();
                         }
                     }
                 }
@@ -837,7 +852,7 @@ struct GofrtTuning {
     int unroll = 8;
     bool xcd = false;
     uint64_t pairs_per_wg = 1ull << 30;
-    bool cluster = false;  // cluster (MFMA) screen where a wavefront's atoms are compact enough
+    bool cluster = true;  // cluster (MFMA) screen where a wavefront's atoms are compact enough
 };
 
 struct anl_gofrt {

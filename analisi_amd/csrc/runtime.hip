@@ -1,6 +1,7 @@
 // Runtime of libanalisi_hip: errors, device selection, trajectory and time
 // series mirrors, block statistics kernels.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -98,14 +99,28 @@ void build_frac16(anl_traj *tr, hipStream_t s) {
     ANL_HIP(hipStreamSynchronize(s));
 }
 
-static uint32_t spread10(uint32_t v) {
-    v &= 0x3FF;
-    v = (v | (v << 16)) & 0x030000FF;
-    v = (v | (v << 8)) & 0x0300F00F;
-    v = (v | (v << 4)) & 0x030C30C3;
-    v = (v | (v << 2)) & 0x09249249;
-    return v;
+// Recursive median bisection of order[lo, hi) (see the slot order below).
+static void kd_order(std::vector<int32_t> &order, const std::vector<std::array<float, 3>> &fr, size_t lo, size_t hi,
+                     const double len[3]) {
+    const size_t n = hi - lo;
+    if (n <= 32) return;
+    float mn[3] = {2.0f, 2.0f, 2.0f}, mx[3] = {-1.0f, -1.0f, -1.0f};
+    for (size_t i = lo; i < hi; i++)
+        for (int k = 0; k < 3; k++) {
+            mn[k] = std::min(mn[k], fr[order[i]][k]);
+            mx[k] = std::max(mx[k], fr[order[i]][k]);
+        }
+    int ax = 0;
+    for (int k = 1; k < 3; k++)
+        if ((mx[k] - mn[k]) * len[k] > (mx[ax] - mn[ax]) * len[ax]) ax = k;
+    const size_t half = (n + 63) / 64 * 32;  // a multiple of 32, < n
+    std::nth_element(order.begin() + lo, order.begin() + lo + half, order.begin() + hi, [&](int32_t a, int32_t b) {
+        return fr[a][ax] < fr[b][ax] || (fr[a][ax] == fr[b][ax] && a < b);
+    });
+    kd_order(order, fr, lo, lo + half, len);
+    kd_order(order, fr, lo + half, hi, len);
 }
+
 
 __global__ void mediavar_update_kernel(double *__restrict__ mean, double *__restrict__ var,
                                        const double *__restrict__ x, uint64_t len, double bp1) {
@@ -213,22 +228,20 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
             tr->box_const = std::memcmp(&tr->box_host[f * 9], &tr->box_host[0], 9 * sizeof(double)) == 0;
         tr->cell0 = cell_of(&tr->box_host[0]);
 
-        // slot order: Morton key of the frame-0 fractional position (ties by index)
-        std::vector<uint64_t> key(natoms);
+        // slot order: balanced k-d bisection of the frame-0 fractional
+        // positions (median cut along the widest cell-scaled extent, every
+        // cut at a multiple of 32 atoms), so that each run of 32 or 64
+        // consecutive slots is a compact box
+        std::vector<std::array<float, 3>> fr(natoms);
         for (uint64_t i = 0; i < natoms; i++) {
-            double s[3];
-            to_fractional(tr->cell0, pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2], s);
-            uint32_t q[3];
-            for (int k = 0; k < 3; k++) {
-                double f = s[k] - std::floor(s[k]);
-                q[k] = std::min<uint32_t>(1023u, (uint32_t)(f * 1024.0));
-            }
-            uint64_t m = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
-            key[i] = m;
+            double s3[3];
+            to_fractional(tr->cell0, pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2], s3);
+            for (int k = 0; k < 3; k++) fr[i][k] = (float)(s3[k] - std::floor(s3[k]));
         }
         std::vector<int32_t> order(natoms);
         std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+        const double len[3] = {tr->cell0.L[0], tr->cell0.L[1], tr->cell0.L[2]};
+        kd_order(order, fr, 0, natoms, len);
         tr->type_of_slot.resize(natoms);
         tr->type_count.assign(ntypes, 0);
         for (uint64_t s = 0; s < natoms; s++) {
