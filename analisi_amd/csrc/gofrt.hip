@@ -443,8 +443,8 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
                                                unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
-    const uint4 ui = fi_lds[il];
-    const uint4 uj = fj32[j];
+    const uint4 ui = fi_lds[il];  // .w = type of i
+    const uint4 uj = fj32[j];     // .w = type of j
     const float fa = (float)(int)(ui.x - uj.x);
     const float fb = (float)(int)(ui.y - uj.y);
     const float fc = (float)(int)(ui.z - uj.z);
@@ -462,9 +462,8 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
     if (fabsf(u - rintf(u)) < a.du_margin) return true;
     const int bb = (int)floorf(u);
-    if (bb >= 0 && bb < (int)a.nbin) {
-        const int tj = a.type_slot[j];
-        const int ch = chan[ti_lds[il] * (int)a.nt + tj] + (slot0 + il == j ? (int)a.half : 0);
+    if ((unsigned)bb < a.nbin) {
+        const int ch = chan[__umul24(ui.w, a.nt) + uj.w] + ((uint32_t)slot0 + il == j ? (int)a.half : 0);
         atomicAdd(&hist[__umul24((unsigned)ch, a.nbin) + (unsigned)bb], 1u);
     }
     return false;

@@ -73,8 +73,10 @@ __device__ static inline uint32_t fix32(double s) {
 
 // Per atom and frame: {a, b, c} as 32-bit fixed-point fractions (frac32) and
 // the same rounded to 16 bits, packed {a | b << 16, c} (frac16).
-__global__ void frac_kernel(const double4 *__restrict__ pos, uint2 *__restrict__ out16, uint4 *__restrict__ out32,
-                            uint64_t total, Cell c) {
+// out32.w carries the slot's type id, so that one 16-byte load gives a
+// candidate's position and type.
+__global__ void frac_kernel(const double4 *__restrict__ pos, const int32_t *__restrict__ type_slot, uint64_t natoms,
+                            uint2 *__restrict__ out16, uint4 *__restrict__ out32, uint64_t total, Cell c) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
     double4 x = pos[k];
@@ -85,7 +87,7 @@ __global__ void frac_kernel(const double4 *__restrict__ pos, uint2 *__restrict__
     uint32_t b16 = ((b + 0x8000u) >> 16) & 0xFFFFu;
     uint32_t c16 = ((cc + 0x8000u) >> 16) & 0xFFFFu;
     out16[k] = make_uint2(a16 | (b16 << 16), c16);
-    out32[k] = make_uint4(a, b, cc, 0u);
+    out32[k] = make_uint4(a, b, cc, (uint32_t)type_slot[k % natoms]);
 }
 
 void build_frac16(anl_traj *tr, hipStream_t s) {
@@ -94,7 +96,8 @@ void build_frac16(anl_traj *tr, hipStream_t s) {
     ANL_HIP(hipMalloc(&tr->frac16, total * sizeof(uint2)));
     ANL_HIP(hipMalloc(&tr->frac32, total * sizeof(uint4)));
     unsigned blocks = (unsigned)((total + 255) / 256);
-    hipLaunchKernelGGL(frac_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->frac16, tr->frac32, total, tr->cell0);
+    hipLaunchKernelGGL(frac_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->type_slot, tr->N, tr->frac16,
+                       tr->frac32, total, tr->cell0);
     ANL_HIP(hipGetLastError());
     ANL_HIP(hipStreamSynchronize(s));
 }
