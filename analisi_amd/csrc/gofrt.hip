@@ -474,12 +474,15 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
 template <bool TRI>
 __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e, const int *ti_lds,
                                                   const double4 *__restrict__ pi, const double4 *__restrict__ pj,
-                                                  uint64_t slot0, const DevCell &c, const double *thr, const int *chan,
-                                                  unsigned *hist) {
+                                                  uint64_t slot0, const double *cell, const double *thr,
+                                                  const int *chan, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
     const double4 xi = pi[slot0 + il];
     const double4 xj = pj[j];
+    // the t0 frame's cell is read here, on this rare path, rather than held
+    // in registers across the whole pair loop
+    const DevCell c = load_cell(cell);
     const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
     if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) {
         const int bb = bin_of(d2r, thr, (int)a.nbin, a.rminf, a.inv_drf);
@@ -539,7 +542,7 @@ __device__ __forceinline__ float cluster_threshold(float rho, float rmax, float 
 
 #define ANL_APPEND_E(P, M, ENTRY)                                           \
     if (M) {                                                                \
-        if (P) q[lane_slot(M, qn)] = (ENTRY);                               \
+        if (P) q[qn + lane_prefix(M)] = (ENTRY);                            \
         qn += (uint32_t)__popcll(M);                                        \
     }
 
@@ -570,7 +573,8 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
 
     const uint64_t N = a.N;
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t *q = queue + (threadIdx.x >> 6) * QC;
+    // per-wavefront queue; its base is wavefront-uniform (kept scalar)
+    uint32_t *q = queue + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * QC;
     uint32_t *rq = rqueue + (threadIdx.x >> 6) * RQCAP;
     const uint64_t slot0 = (uint64_t)id.tile * WG;
     const uint64_t slot = slot0 + threadIdx.x;
@@ -593,7 +597,7 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
     for (uint32_t k = w.k0; k < w.k1; k++) {
         const uint64_t fi = b.primo + (uint64_t)k * a.skip;
         const uint64_t fj = fi + t;
-        const DevCell c = load_cell(a.box + fi * 9);
+        const double *cellp = a.box + fi * 9;
         const double4 *__restrict__ pi = a.pos + fi * N;
         const double4 *__restrict__ pj = a.pos + fj * N;
         const uint4 *__restrict__ fj32 = a.frac32 + fj * N;
@@ -620,7 +624,7 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
                     wave_sync_lds();
                     rn -= 64;
                     nre += 1;
-                    recheck_candidate<TRI>(a, rq[rn + lane], ti_lds, pi, pj, slot0, c, thr, chan, hist);
+                    recheck_candidate<TRI>(a, rq[rn + lane], ti_lds, pi, pj, slot0, cellp, thr, chan, hist);
                     wave_sync_lds();
                 }
             }
@@ -727,10 +731,14 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
                             // entry = j << 8 | local i; item e of tile T is row
                             // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
                             const uint32_t eb = ((j0 + 32u * (uint32_t)jb + (lane & 31u)) << 8) | (wbase + half4);
+                            // both row tiles' products are issued before either is
+                            // compared, so one MFMA's latency hides behind the other's
+                            const f32x16_t DT[2] = {
+                                __builtin_amdgcn_mfma_f32_32x32x16_f16(A0, Bj[jb], f32x16_t{}, 0, 0, 0),
+                                __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, Bj[jb], f32x16_t{}, 0, 0, 0)};
 #pragma unroll
                             for (int T = 0; T < 2; T++) {
-                                const f32x16_t D = __builtin_amdgcn_mfma_f32_32x32x16_f16(T ? A1 : A0, Bj[jb],
-                                                                                          f32x16_t{}, 0, 0, 0);
+                                const f32x16_t D = DT[T];
 #pragma unroll
                                 for (int grp = 0; grp < 2; grp++) {
                                     bool pc[8];
@@ -815,7 +823,7 @@ __global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
         wave_sync_lds();
         if (lane < rn) {
             nre += 1;
-            recheck_candidate<TRI>(a, rq[lane], ti_lds, pi, pj, slot0, c, thr, chan, hist);
+            recheck_candidate<TRI>(a, rq[lane], ti_lds, pi, pj, slot0, cellp, thr, chan, hist);
         }
         wave_sync_lds();
     }
