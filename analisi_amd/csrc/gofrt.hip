@@ -553,7 +553,7 @@ __device__ __forceinline__ float cluster_threshold(float rho, float rmax, float 
     }
 
 template <bool TRI, int UNROLL, bool CLUSTER>
-__global__ __launch_bounds__(WG) void gofrt_screened_kernel(const GofrtArgs a) {
+__global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(const GofrtArgs a) {
     extern __shared__ unsigned char smem[];
     const WgId id = decode_wg(a.ntiles, a.nlag, a.nwork, a.per_xcd);
     if (!id.valid) return;
