@@ -153,44 +153,57 @@ __global__ __launch_bounds__(256) void gk_mean_kernel(double *__restrict__ out, 
 // :279-288 trapezoid I and Einstein E running integrals, per correlation
 // column: I(it) = (I(it-1) + C(it-1)/2) + C(it)/2, E(it) = (E(it-1) +
 // C(it-1) it/2) + C(it) it/2, with I(0) = E(0) = 0 as left by the lag loop.
+// The scan lanes write I and E of a tile into LDS; waves 1-3 store the
+// previous tile's results to HBM (coalesced) while they load the next tile's
+// C values, so the serial chain never waits on a global store.
 __global__ __launch_bounds__(256) void gk_integral_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr,
                                                           uint32_t leff, uint64_t stride, uint32_t tl) {
-    extern __shared__ double tile[];  // [2][tl][ncorr]
+    extern __shared__ double lds[];  // C in [2][tl][ncorr], then (I, E) out [2][tl][2 ncorr]
     if (leff < 2) return;
     out += blockIdx.y * stride;
-    const uint32_t t = threadIdx.x;
+    double *cin = lds, *res = lds + 2 * tl * ncorr;
+    const uint32_t t = threadIdx.x, nres = 2 * ncorr;
     double I = 0.0, E = 0.0, cprev = 0.0;
     if (t < ncorr) {
         I = out[ncorr + 2 * t];
         E = out[ncorr + 2 * t + 1];
         cprev = out[t];
     }
-    if (t >= 64) gk_load_tile(tile, out, narr, ncorr, 1, leff, tl, t - 64, blockDim.x - 64);
+    const uint32_t ntile = (leff - 1 + tl - 1) / tl;  // tiles of lags 1 .. leff-1
+    if (t >= 64) gk_load_tile(cin, out, narr, ncorr, 1, leff, tl, t - 64, blockDim.x - 64);
     __syncthreads();
-    uint32_t buf = 0;
-    for (uint32_t it0 = 1; it0 < leff; it0 += tl) {
-        if (t >= 64 && it0 + tl < leff)
-            gk_load_tile(tile + (buf ^ 1) * tl * ncorr, out, narr, ncorr, it0 + tl, leff, tl, t - 64,
-                         blockDim.x - 64);
-        if (t < ncorr) {
-            const double *col = tile + buf * tl * ncorr + t;
-            const uint32_t n = min(tl, leff - it0);
+    for (uint32_t i = 0; i <= ntile; i++) {
+        const uint32_t buf = i & 1u;
+        if (t >= 64) {
+            if (i + 1 < ntile)
+                gk_load_tile(cin + (buf ^ 1u) * tl * ncorr, out, narr, ncorr, 1 + (i + 1) * tl, leff, tl, t - 64,
+                             blockDim.x - 64);
+            if (i >= 1) {  // results of tile i - 1
+                const uint32_t it0 = 1 + (i - 1) * tl, n = min(tl, leff - it0);
+                const double *src = res + (buf ^ 1u) * tl * nres;
+                for (uint32_t e = t - 64; e < n * nres; e += blockDim.x - 64) {
+                    const uint32_t k = e / nres, jj = e - k * nres;
+                    out[(uint64_t)(it0 + k) * narr + ncorr + jj] = src[e];
+                }
+            }
+        } else if (t < ncorr && i < ntile) {
+            const uint32_t it0 = 1 + i * tl, n = min(tl, leff - it0);
+            const double *col = cin + buf * tl * ncorr + t;
+            double *dst = res + buf * tl * nres + 2 * t;
+#pragma unroll 4
             for (uint32_t k = 0; k < n; k++) {
-                const uint32_t it = it0 + k;
                 const double v = col[k * ncorr];
-                const double dit = (double)it;
+                const double dit = (double)(it0 + k);
                 const double In = I + cprev / 2.0;
                 I = In + v / 2.0;
                 const double En = E + cprev * dit / 2.0;
                 E = En + v * dit / 2.0;
-                double *cur = out + (uint64_t)it * narr;
-                cur[ncorr + 2 * t] = I;
-                cur[ncorr + 2 * t + 1] = E;
+                dst[k * nres] = I;
+                dst[k * nres + 1] = E;
                 cprev = v;
             }
         }
         __syncthreads();
-        buf ^= 1;
     }
 }
 
@@ -292,7 +305,10 @@ void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int sub
                            stride, tl);
         ANL_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(gk_integral_kernel, dim3(1, nblocks), dim3(256), lds, s, d, narr, ncorr, leff, stride, tl);
+    // integral: C tiles and (I, E) tiles, both double-buffered, <= 48 KiB
+    const uint32_t tli = std::max(16u, 1024u / ncorr);
+    const size_t ldsi = 6 * (size_t)tli * ncorr * sizeof(double);
+    hipLaunchKernelGGL(gk_integral_kernel, dim3(1, nblocks), dim3(256), ldsi, s, d, narr, ncorr, leff, stride, tli);
     ANL_HIP(hipGetLastError());
     hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256, nblocks), dim3(256), 0, s, d, narr, ncur, leff,
                        stride);

@@ -35,8 +35,10 @@ namespace anl {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t GKM_Q = 8;    // tiles per wavefront, 16 apart
-constexpr uint32_t GKM_CS = 16;  // MFMA steps (16 rows of 32 t0 each) per FP32 chunk
+constexpr uint32_t GKM_Q = 8;      // tiles per wavefront, 16 apart
+constexpr uint32_t GKM_CS = 16;    // MFMA steps (16 rows of 32 t0 each) per FP32 chunk
+constexpr uint32_t GKM_MAXB = 64;  // blocks per batch of launches
+constexpr uint32_t GKM_SP = 32;    // rows p of one split workgroup
 
 __device__ __forceinline__ unsigned short bf16_rn(float f) {
     unsigned u = __builtin_bit_cast(unsigned, f);
@@ -51,51 +53,102 @@ __device__ __forceinline__ void split_bf16(double v, unsigned short &h, unsigned
     l = bf16_rn((float)(v - (double)bf16_to_f(h)));
 }
 
-// X side, one segment: element (cx, r, p) = data[primo + t][idx[c] + x] at
-// t = 32p + r when t is a selected t0 of the segment, else 0.
-struct GkIdx {
-    uint32_t v[4];
-};
+__host__ __device__ inline uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
 
-__global__ void gkm_split_x_kernel(const double *__restrict__ data, uint32_t ncols, const GkIdx idx, uint64_t primo, uint32_t PP, uint32_t seg_lo, uint32_t seg_hi, uint32_t first,
-                                   uint32_t skip, unsigned short *__restrict__ xh, unsigned short *__restrict__ xl) {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // e = r * PP + p within one cx
-    if (e >= 32ull * PP) return;
-    const uint32_t cx = blockIdx.y, c = cx / 3, x = cx % 3;
-    const uint32_t r = (uint32_t)(e / PP), p = (uint32_t)(e % PP);
-    const uint64_t t = 32ull * p + r;
-    double v = 0.0;
-    if (t >= seg_lo && t < seg_hi && t >= first && (t - first) % skip == 0) v = data[(primo + t) * ncols + idx.v[c] + x];
-    unsigned short h, l;
-    split_bf16(v, h, l);
-    const uint64_t o = (uint64_t)cx * 32 * PP + e;
-    xh[o] = h;
-    xl[o] = l;
+// Byte offsets of the four operand arrays inside one block's region.
+struct GkmOff {
+    uint64_t xh, xl, yh, yl, end;
+};
+__host__ __device__ inline GkmOff gkm_off(uint32_t ncx, uint32_t PP, uint32_t YP) {
+    const uint64_t xn = (uint64_t)ncx * 32 * PP, yn = 8ull * ncx * 32 * YP;
+    GkmOff o;
+    o.xh = 0;
+    o.xl = align256(2 * xn);
+    o.yh = o.xl + align256(2 * xn);
+    o.yl = o.yh + align256(2 * yn);
+    o.end = o.yl + align256(2 * yn);
+    return o;
 }
 
-// Y side: element (c8, cx, s, p') = data[primo + t][idx[c] + x] at
-// t = 32(p' + c8) + s when t < avail, else 0.
-__global__ void gkm_split_y_kernel(const double *__restrict__ data, uint32_t ncols, const GkIdx idx, uint64_t primo, uint64_t avail, uint32_t YP, uint32_t ncx,
-                                   unsigned short *__restrict__ yh, unsigned short *__restrict__ yl) {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // s * YP + p'
-    if (e >= 32ull * YP) return;
-    const uint32_t cx = blockIdx.y % ncx, c8 = blockIdx.y / ncx, c = cx / 3, x = cx % 3;
-    const uint32_t s = (uint32_t)(e / YP), pp = (uint32_t)(e % YP);
-    const uint64_t t = 32ull * ((uint64_t)pp + c8) + s;
-    double v = 0.0;
-    if (t < avail) v = data[(primo + t) * ncols + idx.v[c] + x];
-    unsigned short h, l;
-    split_bf16(v, h, l);
-    const uint64_t o = ((uint64_t)c8 * ncx + cx) * 32 * YP + e;
-    yh[o] = h;
-    yl[o] = l;
-}
-
-struct GkmArgs {
-    const unsigned short *xh, *xl, *yh, *yl;
-    uint32_t ncur, PP, YP, ntiles, nchunks;
-    double *partial;  // [pair][ntiles][nchunks][63]
+// One batch of blocks: block b reads series rows primo[b] ..., averages over
+// n[b] t0, owns the operand region at work + base[b] (PP[b] t0 rows of 32 on
+// the X side, YP[b] on the Y side), the partial sums at partial + poff[b] and
+// MFMA chunks z0[b] .. z0[b+1] of the tile grid.
+struct GkmBatch {
+    const double *data;
+    char *work;
+    double *partial;
+    double *out;  // block b at out + b * stride, [leff][narr]
+    uint64_t stride;
+    uint32_t ncols, ncur, ncx, ntiles, leff, narr, skip, n_seg, nblk;
+    uint32_t idx[4];
+    uint64_t primo[GKM_MAXB], base[GKM_MAXB], poff[GKM_MAXB], nrows;
+    uint32_t n[GKM_MAXB], PP[GKM_MAXB], YP[GKM_MAXB], z0[GKM_MAXB + 1];
 };
+
+// Operand split of one block (blockIdx.z) and one group of GKM_SP rows p
+// (blockIdx.x): O[c8][cx][s][p] = V(32(p + c8) + s), c8 < nshift, where V is
+// the series value of column cx on the t0 side of segment iseg (X: zero
+// outside the selected t0 of the segment) or the lagged side (Y: zero past
+// the end of the series).  Rows of the series are read coalesced into LDS
+// (already split), then written out as 16-byte runs of 8 consecutive p.
+template <bool YSIDE>
+__global__ __launch_bounds__(256) void gkm_split_kernel(const GkmBatch a, uint32_t iseg) {
+    constexpr uint32_t NSH = YSIDE ? 8 : 1, NU = GKM_SP + NSH - 1, UP = NU + 1;
+    extern __shared__ unsigned short sh[];  // hi [cx][s][UP], then lo
+    const uint32_t b = blockIdx.z, P0 = blockIdx.x * GKM_SP, t = threadIdx.x;
+    const uint32_t P = YSIDE ? a.YP[b] : a.PP[b];
+    if (P0 >= P) return;  // uniform across the workgroup
+    const uint32_t ncx = a.ncx;
+    unsigned short *shl = sh + ncx * 32 * UP;
+    const uint64_t primo = a.primo[b];
+    uint64_t lo = 0, hi = 0, first = 0;
+    if (!YSIDE) {
+        const uint32_t seg = a.n[b] / a.n_seg;
+        lo = (uint64_t)seg * iseg;
+        hi = (uint64_t)seg * (iseg + 1);
+        first = lo + lo % a.skip;  // greenkubo...cpp:208
+    }
+    const uint64_t avail = a.nrows - primo;
+    for (uint32_t e = t; e < NU * 32 * ncx; e += blockDim.x) {
+        const uint32_t tl = e / ncx, cx = e - tl * ncx;
+        const uint64_t tt = 32ull * P0 + tl;
+        bool use;
+        if (YSIDE) use = tt < avail;
+        else use = tt >= first && tt < hi && (tt - first) % a.skip == 0;
+        double v = 0.0;
+        if (use) v = a.data[(primo + tt) * a.ncols + a.idx[cx / 3] + cx % 3];
+        unsigned short h, l;
+        split_bf16(v, h, l);
+        const uint32_t o = (cx * 32 + tl % 32) * UP + tl / 32;
+        sh[o] = h;
+        shl[o] = l;
+    }
+    __syncthreads();
+    const GkmOff off = gkm_off(ncx, a.PP[b], a.YP[b]);
+    unsigned short *dh = (unsigned short *)(a.work + a.base[b] + (YSIDE ? off.yh : off.xh));
+    unsigned short *dl = (unsigned short *)(a.work + a.base[b] + (YSIDE ? off.yl : off.xl));
+    const uint32_t nrun = NSH * ncx * 32 * (GKM_SP / 8);
+    for (uint32_t e = t; e < nrun; e += blockDim.x) {
+        const uint32_t k = e % (GKM_SP / 8), row = e / (GKM_SP / 8);  // row = (c8 * ncx + cx) * 32 + s
+        const uint32_t c8 = row / (ncx * 32), cs = row - c8 * ncx * 32;
+        const uint32_t p = P0 + 8 * k;
+        if (p >= P) continue;  // P is a multiple of 8
+        const unsigned short *sa = sh + cs * UP + 8 * k + c8, *sb = shl + cs * UP + 8 * k + c8;
+        uint4 wh, wl;
+        wh.x = sa[0] | (uint32_t)sa[1] << 16;
+        wh.y = sa[2] | (uint32_t)sa[3] << 16;
+        wh.z = sa[4] | (uint32_t)sa[5] << 16;
+        wh.w = sa[6] | (uint32_t)sa[7] << 16;
+        wl.x = sb[0] | (uint32_t)sb[1] << 16;
+        wl.y = sb[2] | (uint32_t)sb[3] << 16;
+        wl.z = sb[4] | (uint32_t)sb[5] << 16;
+        wl.w = sb[6] | (uint32_t)sb[7] << 16;
+        const uint64_t o = (uint64_t)row * P + p;
+        *reinterpret_cast<uint4 *>(dh + o) = wh;
+        *reinterpret_cast<uint4 *>(dl + o) = wl;
+    }
+}
 
 __device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
@@ -104,12 +157,20 @@ __device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
 // Operand layouts of v_mfma_f32_32x32x16_bf16: lane l feeds row l % 32 of A
 // and column l % 32 of B, each with the 8 k-values of slot l / 32 (the same k
 // slots in A and B); result item e of lane l is row 8(e/4) + 4(l/32) + e%4,
-// column l % 32.
-__global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
+// column l % 32.  Grid: x = tile group, y = current pair, z = (block, chunk).
+__global__ __launch_bounds__(64, 2) void gkm_tile_kernel(const GkmBatch a) {
     __shared__ double tile[32][33];
-    const uint32_t g = blockIdx.x, pair = blockIdx.y, chunk = blockIdx.z;
+    const uint32_t g = blockIdx.x, pair = blockIdx.y, z = blockIdx.z;
+    uint32_t b = 0;
+    while (z >= a.z0[b + 1]) b++;  // uniform
+    const uint32_t chunk = z - a.z0[b], nchunks = a.z0[b + 1] - a.z0[b];
+    const uint32_t PP = a.PP[b], YP = a.YP[b];
+    const GkmOff off = gkm_off(a.ncx, PP, YP);
+    const char *reg = a.work + a.base[b];
+    const unsigned short *xh = (const unsigned short *)(reg + off.xh), *xl = (const unsigned short *)(reg + off.xl);
+    const unsigned short *yh = (const unsigned short *)(reg + off.yh), *yl = (const unsigned short *)(reg + off.yl);
     const uint32_t qbase = (g / 16) * 16 * GKM_Q, q0 = g % 16;
-    const uint32_t ca = pair / a.ncur, cb = pair % a.ncur, ncx = 3 * a.ncur;
+    const uint32_t ca = pair / a.ncur, cb = pair % a.ncur, ncx = a.ncx;
     const uint32_t lane = threadIdx.x, r = lane & 31u, h = lane >> 5;
     const uint32_t c8 = q0 & 7u, qoff = qbase + (q0 & ~7u);
     const uint32_t st0 = chunk * GKM_CS;
@@ -117,9 +178,9 @@ __global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
 #pragma unroll
     for (uint32_t j = 0; j < GKM_Q; j++) acc[j] = f32x16{};
     for (uint32_t x = 0; x < 3; x++) {
-        const uint64_t xo = ((uint64_t)(3 * ca + x) * 32 + r) * a.PP + 8 * h;
-        const uint64_t yo = (((uint64_t)c8 * ncx + 3 * cb + x) * 32 + r) * a.YP + qoff + 8 * h;
-        const unsigned short *XH = a.xh + xo, *XL = a.xl + xo, *YH = a.yh + yo, *YL = a.yl + yo;
+        const uint64_t xo = ((uint64_t)(3 * ca + x) * 32 + r) * PP + 8 * h;
+        const uint64_t yo = (((uint64_t)c8 * ncx + 3 * cb + x) * 32 + r) * YP + qoff + 8 * h;
+        const unsigned short *XH = xh + xo, *XL = xl + xo, *YH = yh + yo, *YL = yl + yo;
         bf16x8 wh[GKM_Q], wl[GKM_Q];
 #pragma unroll
         for (uint32_t j = 0; j + 1 < GKM_Q; j++) {
@@ -154,6 +215,7 @@ __global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
         }
     }
     // diagonal sums of each tile, in FP64, into this chunk's partial
+    double *part = a.partial + a.poff[b];
 #pragma unroll
     for (uint32_t j = 0; j < GKM_Q; j++) {
         const uint32_t q = qbase + q0 + 16 * j;
@@ -166,7 +228,7 @@ __global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
             double sum = 0.0;
             const int r0 = d < 0 ? -d : 0, r1 = d < 0 ? 32 : 32 - d;
             for (int rr = r0; rr < r1; rr++) sum += tile[rr][rr + d];
-            a.partial[(((uint64_t)pair * a.ntiles + q) * a.nchunks + chunk) * 63 + lane] = sum;
+            part[(((uint64_t)pair * a.ntiles + q) * nchunks + chunk) * 63 + lane] = sum;
         }
         __syncthreads();
     }
@@ -174,28 +236,33 @@ __global__ __launch_bounds__(64) void gkm_tile_kernel(const GkmArgs a) {
 
 // Segment mean of product/3 for lag l and pair (diagonal l - 32q of tiles
 // q = floor(l/32), floor(l/32) + 1; chunks in order), folded into the running
-// mean over segments (:230-237).
-__global__ void gkm_reduce_kernel(const double *__restrict__ partial, uint32_t ntiles, uint32_t nchunks,
-                                  uint32_t npairs, uint32_t leff, uint32_t narr, double cont, uint32_t iseg,
-                                  double *__restrict__ out) {
+// mean over segments (:230-237); the first segment also clears the other
+// columns of the row.  Grid: x = (lag, column), y = block.
+__global__ __launch_bounds__(256) void gkm_reduce_kernel(const GkmBatch a, uint32_t iseg) {
+    const uint32_t b = blockIdx.y;
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= leff * npairs) return;
-    const uint32_t l = w / npairs, pair = w % npairs;
+    if (w >= a.leff * a.narr) return;
+    const uint32_t l = w / a.narr, col = w % a.narr, npairs = a.ncur * a.ncur;
+    double *v = a.out + b * a.stride + (uint64_t)l * a.narr + col;
+    if (col >= npairs) {
+        if (iseg == 0) *v = 0.0;
+        return;
+    }
+    const uint32_t nchunks = a.z0[b + 1] - a.z0[b];
     double s = 0.0;
-    for (uint32_t q = l / 32; q <= l / 32 + 1 && q < ntiles; q++) {
+    for (uint32_t q = l / 32; q <= l / 32 + 1 && q < a.ntiles; q++) {
         const int d = (int)l - 32 * (int)q;
         if (d < -31 || d > 31) continue;
-        const double *p = partial + ((uint64_t)pair * ntiles + q) * nchunks * 63 + (d + 31);
+        const double *p = a.partial + a.poff[b] + ((uint64_t)col * a.ntiles + q) * nchunks * 63 + (d + 31);
         for (uint32_t ch = 0; ch < nchunks; ch++) s += p[(uint64_t)ch * 63];
     }
-    const double jj = cont > 0 ? (s / 3.0) / cont : 0.0;
-    double *v = out + (uint64_t)l * narr + pair;
-    *v = *v + (jj - *v) / (double)(iseg + 1);
-}
-
-__global__ void gkm_zero_kernel(double *__restrict__ p, uint64_t n) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
+    const uint32_t seg = a.n[b] / a.n_seg;
+    const uint32_t lo = seg * iseg, hi = seg * (iseg + 1);
+    const uint32_t first = lo + lo % a.skip;  // greenkubo...cpp:208
+    const uint32_t cont = first < hi ? (hi - first + a.skip - 1) / a.skip : 0;
+    const double jj = cont > 0 ? (s / 3.0) / (double)cont : 0.0;
+    const double old = iseg == 0 ? 0.0 : *v;
+    *v = old + (jj - old) / (double)(iseg + 1);
 }
 
 }  // namespace anl
@@ -204,76 +271,111 @@ using namespace anl;
 
 namespace {
 
-struct GkmPlan {
-    uint32_t leff, ncorr, narr, ncx;
-    uint64_t nchunks, PP, ntiles, ngroups, YP;
-    size_t off_xh, off_xl, off_yh, off_yl, off_partial, bytes;
+// Sizes of one block's operands, from its t0 count n and the lag count leff.
+struct GkmSize {
+    uint32_t nchunks, PP, YP, ntiles, ngroups;
+    uint64_t region, partial;  // bytes of the operand region, doubles of partial sums
 };
 
-size_t align256(size_t b) { return (b + 255) / 256 * 256; }
-
-GkmPlan gkm_plan(uint32_t ncur, uint32_t leff, uint32_t n) {
-    GkmPlan p{};
-    p.leff = leff;
-    p.ncorr = ncur * ncur;
-    p.narr = 3 * p.ncorr + 2;
-    p.ncx = 3 * ncur;
+GkmSize gkm_size(uint32_t ncur, uint32_t leff, uint32_t n) {
+    GkmSize z{};
     // t0 axis: rows of 32, MFMA steps of 16 rows, chunks of GKM_CS steps
     const uint64_t nsteps = ((uint64_t)n + 511) / 512;
-    p.nchunks = (nsteps + GKM_CS - 1) / GKM_CS;
-    p.PP = p.nchunks * GKM_CS * 16;
+    const uint64_t nchunks = (nsteps + GKM_CS - 1) / GKM_CS;
+    const uint64_t PP = nchunks * GKM_CS * 16;
     // lag axis: tiles of 32 (plus one for the negative diagonals)
-    p.ntiles = (leff + 30) / 32 + 1;
-    p.ngroups = (p.ntiles + 16 * GKM_Q - 1) / (16 * GKM_Q) * 16;
-    const uint64_t qmax = p.ngroups / 16 * 16 * GKM_Q;  // tiles covered, >= ntiles
-    // highest Y row read: 16 (st + Q - 1) + 8 + 7 + qoff, st < nchunks * CS, qoff <= qmax - 16 Q + 8
-    p.YP = (p.PP + 16 * GKM_Q + qmax + 16 + 7) / 8 * 8;
-    if (p.YP >= (1ull << 31) || p.nchunks > 65535)
+    const uint64_t ntiles = (leff + 30) / 32 + 1;
+    const uint64_t ngroups = (ntiles + 16 * GKM_Q - 1) / (16 * GKM_Q) * 16;
+    const uint64_t qmax = ngroups / 16 * 16 * GKM_Q;  // tiles covered, >= ntiles
+    // highest Y row read: 16 (st + Q - 1) + 8 + 7 + qoff, st < nchunks * CS, qoff <= qmax - 16 Q + 8;
+    // rounded up to whole split workgroups
+    const uint64_t YP = (PP + 16 * GKM_Q + qmax + 16 + 7) / GKM_SP * GKM_SP;
+    if (YP >= (1ull << 31) || nchunks > 65535)
         fail(ANL_ERR_ARG, "series too long for the MFMA form (use the FP64 form)");
-    const size_t xn = (size_t)p.ncx * 32 * p.PP, yn = (size_t)8 * p.ncx * 32 * p.YP;
-    p.off_xh = 0;
-    p.off_xl = p.off_xh + align256(xn * 2);
-    p.off_yh = p.off_xl + align256(xn * 2);
-    p.off_yl = p.off_yh + align256(yn * 2);
-    p.off_partial = p.off_yl + align256(yn * 2);
-    p.bytes = p.off_partial + align256((size_t)p.ncorr * p.ntiles * p.nchunks * 63 * sizeof(double));
-    return p;
+    z.nchunks = (uint32_t)nchunks;
+    z.PP = (uint32_t)PP;
+    z.YP = (uint32_t)YP;
+    z.ntiles = (uint32_t)ntiles;
+    z.ngroups = (uint32_t)ngroups;
+    z.region = gkm_off(3 * ncur, z.PP, z.YP).end;
+    z.partial = (uint64_t)ncur * ncur * ntiles * nchunks * 63;
+    return z;
 }
 
-// Enqueues the lag products of one block on s: d [leff][narr] (device) gets
-// C(l) in its first ncur^2 columns and zeros elsewhere.
-void gkm_enqueue(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t n, uint32_t n_seg,
-                 uint64_t primo, const GkmPlan &p, char *work, double *d, hipStream_t s) {
-    auto *xh = (unsigned short *)(work + p.off_xh);
-    auto *xl = (unsigned short *)(work + p.off_xl);
-    auto *yh = (unsigned short *)(work + p.off_yh);
-    auto *yl = (unsigned short *)(work + p.off_yl);
-    auto *partial = (double *)(work + p.off_partial);
-    GkIdx idx{};
-    for (uint32_t c = 0; c < ncur; c++) idx.v[c] = idx_j[c];
-    auto grid1 = [](uint64_t n_, uint32_t y) { return dim3((unsigned)((n_ + 255) / 256), y); };
-    const uint64_t outn = (uint64_t)p.leff * p.narr;
-    const uint64_t avail = sr->nrows - primo;
-    hipLaunchKernelGGL(gkm_zero_kernel, grid1(outn, 1), dim3(256), 0, s, d, outn);
+// Scratch of a batch of blocks sharing one set of launches.
+constexpr uint64_t GKM_BATCH_BYTES = 4ull << 30;
+
+uint64_t gkm_need(const GkmSize &z) { return z.region + align256(z.partial * sizeof(double)); }
+
+// Blocks [b0, b1) of one batch: at most GKM_MAXB blocks, 65,535 chunks and
+// GKM_BATCH_BYTES of scratch (a single block may exceed the latter).
+uint32_t gkm_batch_end(uint32_t ncur, uint32_t leff, const uint32_t *n, uint32_t nblocks, uint32_t b0,
+                       uint64_t *bytes) {
+    uint64_t tot = 0, chunks = 0;
+    uint32_t b = b0;
+    for (; b < nblocks && b - b0 < GKM_MAXB; b++) {
+        const GkmSize z = gkm_size(ncur, leff, n[b]);
+        if (b > b0 && (tot + gkm_need(z) > GKM_BATCH_BYTES || chunks + z.nchunks > 65535)) break;
+        tot += gkm_need(z);
+        chunks += z.nchunks;
+    }
+    *bytes = tot;
+    return b;
+}
+
+// Enqueues the lag products of blocks [b0, b1) on s; block b's output
+// d + b * stride [leff][narr] gets C(l) in its first ncur^2 columns and zeros
+// elsewhere.
+void gkm_enqueue_batch(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
+                       uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t b0, uint32_t b1, char *work,
+                       double *d, uint64_t stride, hipStream_t s) {
+    GkmBatch a{};
+    a.data = sr->data;
+    a.work = work;
+    a.ncols = sr->ncols;
+    a.nrows = sr->nrows;
+    a.ncur = ncur;
+    a.ncx = 3 * ncur;
+    a.leff = leff;
+    a.narr = 3 * ncur * ncur + 2;
+    a.skip = skip;
+    a.n_seg = n_seg;
+    a.nblk = b1 - b0;
+    a.out = d + b0 * stride;
+    a.stride = stride;
+    for (uint32_t c = 0; c < ncur; c++) a.idx[c] = idx_j[c];
+    uint64_t region = 0, partial = 0;
+    uint32_t maxPP = 0, maxYP = 0, ngroups = 0;
+    a.z0[0] = 0;
+    for (uint32_t k = 0; k < a.nblk; k++) {
+        const GkmSize z = gkm_size(ncur, leff, n[b0 + k]);
+        a.primo[k] = primo[b0 + k];
+        a.n[k] = n[b0 + k];
+        a.PP[k] = z.PP;
+        a.YP[k] = z.YP;
+        a.base[k] = region;
+        a.poff[k] = partial;
+        a.z0[k + 1] = a.z0[k] + z.nchunks;
+        a.ntiles = z.ntiles;
+        ngroups = z.ngroups;
+        region += z.region;
+        partial += align256(z.partial * sizeof(double)) / sizeof(double);
+        maxPP = std::max(maxPP, z.PP);
+        maxYP = std::max(maxYP, z.YP);
+    }
+    a.partial = (double *)(work + region);
+    const uint32_t ncorr = ncur * ncur, ncx = 3 * ncur;
+    const size_t lds_x = 2 * (size_t)ncx * 32 * (GKM_SP + 1) * sizeof(unsigned short);
+    const size_t lds_y = 2 * (size_t)ncx * 32 * (GKM_SP + 8) * sizeof(unsigned short);
+    hipLaunchKernelGGL(gkm_split_kernel<true>, dim3(maxYP / GKM_SP, 1, a.nblk), dim3(256), lds_y, s, a, 0u);
     ANL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gkm_split_y_kernel, grid1(32 * p.YP, 8 * p.ncx), dim3(256), 0, s, sr->data, sr->ncols, idx,
-                       primo, avail, (uint32_t)p.YP, p.ncx, yh, yl);
-    ANL_HIP(hipGetLastError());
-    GkmArgs ka{xh, xl, yh, yl, ncur, (uint32_t)p.PP, (uint32_t)p.YP, (uint32_t)p.ntiles, (uint32_t)p.nchunks, partial};
-    const uint32_t seg = n / n_seg;
+    const dim3 rgrid((unsigned)(((uint64_t)leff * a.narr + 255) / 256), a.nblk);
     for (uint32_t iseg = 0; iseg < n_seg; iseg++) {
-        const uint32_t lo = seg * iseg, hi = seg * (iseg + 1);
-        const uint32_t first = lo + lo % skip;  // greenkubo...cpp:208
-        const uint32_t cont = first < hi ? (hi - first + skip - 1) / skip : 0;
-        hipLaunchKernelGGL(gkm_split_x_kernel, grid1(32 * p.PP, p.ncx), dim3(256), 0, s, sr->data, sr->ncols, idx,
-                           primo, (uint32_t)p.PP, lo, hi, first, skip, xh, xl);
+        hipLaunchKernelGGL(gkm_split_kernel<false>, dim3(maxPP / GKM_SP, 1, a.nblk), dim3(256), lds_x, s, a, iseg);
         ANL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gkm_tile_kernel, dim3((unsigned)p.ngroups, p.ncorr, (unsigned)p.nchunks), dim3(64), 0, s,
-                           ka);
+        hipLaunchKernelGGL(gkm_tile_kernel, dim3(ngroups, ncorr, a.z0[a.nblk]), dim3(64), 0, s, a);
         ANL_HIP(hipGetLastError());
-        const uint64_t nw = (uint64_t)p.leff * p.ncorr;
-        hipLaunchKernelGGL(gkm_reduce_kernel, grid1(nw, 1), dim3(256), 0, s, partial, (uint32_t)p.ntiles,
-                           (uint32_t)p.nchunks, p.ncorr, p.leff, p.narr, (double)cont, iseg, d);
+        hipLaunchKernelGGL(gkm_reduce_kernel, rgrid, dim3(256), 0, s, a, iseg);
         ANL_HIP(hipGetLastError());
     }
 }
@@ -282,18 +384,30 @@ void gkm_enqueue(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t 
 
 namespace anl {
 
-size_t gk_mfma_scratch_bytes(uint32_t ncur, uint32_t leff, uint32_t n) { return gkm_plan(ncur, leff, n).bytes; }
+size_t gk_mfma_scratch_bytes(uint32_t ncur, uint32_t leff, uint32_t n) {
+    uint64_t bytes = 0;
+    gkm_batch_end(ncur, leff, &n, 1, 0, &bytes);
+    return bytes;
+}
 
 void gk_mfma_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
                             uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, double *d,
                             hipStream_t st) {
-    size_t need = 0;
-    for (uint32_t b = 0; b < nblocks; b++) need = std::max(need, gkm_plan(ncur, leff, n[b]).bytes);
+    uint64_t need = 0;
+    for (uint32_t b0 = 0; b0 < nblocks;) {
+        uint64_t bytes = 0;
+        b0 = gkm_batch_end(ncur, leff, n, nblocks, b0, &bytes);
+        need = std::max(need, bytes);
+    }
     char *work = (char *)series_workspace(s, need);
     const uint64_t stride = (uint64_t)leff * (3 * ncur * ncur + 2);
-    // blocks run one after the other on st, sharing the scratch
-    for (uint32_t b = 0; b < nblocks; b++)
-        gkm_enqueue(s, idx_j, ncur, skip, n[b], n_seg, primo[b], gkm_plan(ncur, leff, n[b]), work, d + b * stride, st);
+    // batches run one after the other on st, sharing the scratch
+    for (uint32_t b0 = 0; b0 < nblocks;) {
+        uint64_t bytes = 0;
+        const uint32_t b1 = gkm_batch_end(ncur, leff, n, nblocks, b0, &bytes);
+        gkm_enqueue_batch(s, idx_j, ncur, skip, leff, n_seg, primo, n, b0, b1, work, d, stride, st);
+        b0 = b1;
+    }
 }
 
 }  // namespace anl

@@ -131,6 +131,10 @@ static Screen make_screen(const anl_traj *tr, double rmax) {
         s.why_not = "the cell changes along the trajectory";
         return s;
     }
+    if (tr->nt > 255) {  // a candidate's type rides in 8 bits
+        s.why_not = "more than 255 atom types";
+        return s;
+    }
     const Cell &c = tr->cell0;
     double A[3][3] = {{c.L[0], c.xy, c.xz}, {0, c.L[1], c.yz}, {0, 0, c.L[2]}};
     // inverse of the upper-triangular A
@@ -438,13 +442,12 @@ __device__ __forceinline__ uint32_t lane_slot(unsigned long long m, uint32_t bas
 // du_margin is four times a worst-case bound of |u_fp32 - u_reference|, so
 // outside the window both floor to the same bin.
 template <bool TRI>
-__device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const int *ti_lds,
-                                               const uint4 *__restrict__ fj32, uint64_t slot0, const int *chan,
-                                               unsigned *hist) {
+__device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds,
+                                               const uint4 *__restrict__ fj32, const int *chan, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
-    const uint4 ui = fi_lds[il];  // .w = type of i
-    const uint4 uj = fj32[j];     // .w = type of j
+    const uint4 ui = fi_lds[il];  // .w = type | slot << 8 of i
+    const uint4 uj = fj32[j];     // .w = type | slot << 8 of j
     const float fa = (float)(int)(ui.x - uj.x);
     const float fb = (float)(int)(ui.y - uj.y);
     const float fc = (float)(int)(ui.z - uj.z);
@@ -463,7 +466,7 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
     if (fabsf(u - rintf(u)) < a.du_margin) return true;
     const int bb = (int)floorf(u);
     if ((unsigned)bb < a.nbin) {
-        const int ch = chan[__umul24(ui.w, a.nt) + uj.w] + ((uint32_t)slot0 + il == j ? (int)a.half : 0);
+        const int ch = chan[__umul24(ui.w & 0xFFu, a.nt) + (uj.w & 0xFFu)] + (ui.w == uj.w ? (int)a.half : 0);
         atomicAdd(&hist[__umul24((unsigned)ch, a.nbin) + (unsigned)bb], 1u);
     }
     return false;
@@ -472,22 +475,22 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
 // Exact FP64 evaluation (the reference's sequence) of a candidate whose FP32
 // distance fell inside the window around a bin edge, rmin or rmax.
 template <bool TRI>
-__device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e, const int *ti_lds,
-                                                  const double4 *__restrict__ pi, const double4 *__restrict__ pj,
-                                                  uint64_t slot0, const double *cell, const double *thr,
-                                                  const int *chan, unsigned *hist) {
+__device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds,
+                                                  const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
+                                                  const double4 *__restrict__ pj, const double *cell,
+                                                  const double *thr, const int *chan, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
-    const double4 xi = pi[slot0 + il];
-    const double4 xj = pj[j];
+    const uint32_t wi = fi_lds[il].w, wj = fj32[j].w;  // type | slot << 8
+    const double4 xi = pi[wi >> 8];
+    const double4 xj = pj[wj >> 8];
     // the t0 frame's cell is read here, on this rare path, rather than held
     // in registers across the whole pair loop
     const DevCell c = load_cell(cell);
     const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
     if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) {
         const int bb = bin_of(d2r, thr, (int)a.nbin, a.rminf, a.inv_drf);
-        const int tj = a.type_slot[j];
-        const int ch = chan[ti_lds[il] * (int)a.nt + tj] + (slot0 + il == j ? (int)a.half : 0);
+        const int ch = chan[(wi & 0xFFu) * a.nt + (wj & 0xFFu)] + (wi == wj ? (int)a.half : 0);
         atomicAdd(&hist[__umul24((unsigned)ch, a.nbin) + (unsigned)bb], 1u);
     }
 }
@@ -565,8 +568,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     constexpr int QC = CLUSTER ? SQCAP_CL : SQCAP;
     uint32_t *queue = (uint32_t *)(fi_lds + WG);            // [WG/64][QC]
     uint32_t *rqueue = queue + (WG / 64) * QC;              // [WG/64][RQCAP]
-    int *ti_lds = (int *)(rqueue + (WG / 64) * RQCAP);      // [WG]
-    double *thr = (double *)(ti_lds + WG);                  // [nbin+1]
+    double *thr = (double *)(rqueue + (WG / 64) * RQCAP);   // [nbin+1]
     int *chan = (int *)(thr + a.nbin + 1);                  // [nt*nt]
     unsigned *hist = (unsigned *)(chan + a.nt * a.nt);      // [nch*nbin]
     lds_setup(a, hist, thr, chan);
@@ -578,20 +580,12 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     uint32_t *rq = rqueue + (threadIdx.x >> 6) * RQCAP;
     const uint64_t slot0 = (uint64_t)id.tile * WG;
     const uint64_t slot = slot0 + threadIdx.x;
-    bool active = slot < N;
-    int ti = 0;
-    if (active) {
-        int o = a.orig_slot[slot];
-        active = o >= a.a0 && o < a.a1;
-        ti = a.type_slot[slot];
-    }
-    ti_lds[threadIdx.x] = ti;
+    // an atom range of calc_single_th narrower than all atoms is tested per
+    // t0: the fractions of each frame are in that frame's own slot order
+    const bool all_atoms = a.a0 <= 0 && (int64_t)a.a1 >= (int64_t)N;
     const uint32_t il = threadIdx.x;
     constexpr int UR = UNROLL;
     const uint32_t NU = (uint32_t)(N - N % UR);
-    // strict comparison against threshold+1; inactive lanes compare against 0
-    // and never pass
-    const uint32_t th = active ? a.threshold + 1u : 0u;
     unsigned long long ncand = 0, nre = 0, ncl = 0, nwt = 0;
 
     for (uint32_t k = w.k0; k < w.k1; k++) {
@@ -604,18 +598,29 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         cu64_t *fjc = (cu64_t *)(a.frac + fj * N);
         __syncthreads();  // previous t0's candidates are done with fi_lds
         uint32_t abi = 0, ci = 0;
+        bool active = false;
         if (slot < N) {
-            fi_lds[threadIdx.x] = a.frac32[fi * N + slot];
+            const uint4 f32 = a.frac32[fi * N + slot];
+            fi_lds[threadIdx.x] = f32;
             const uint2 f = a.frac[fi * N + slot];
             abi = f.x;
             ci = f.y;
+            if (all_atoms) {
+                active = true;
+            } else {
+                const int o = a.orig_slot[f32.w >> 8];
+                active = o >= a.a0 && o < a.a1;
+            }
         }
+        // strict comparison against threshold+1; inactive lanes compare
+        // against 0 and never pass
+        const uint32_t th = active ? a.threshold + 1u : 0u;
         __syncthreads();
         uint32_t qn = 0, rn = 0;
         // evaluate the queued entry of this lane; candidates inside the edge
         // window go to the recheck queue, evaluated 64 at a time
         auto eval_batch = [&](uint32_t e, bool valid) {
-            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, ti_lds, fj32, slot0, chan, hist);
+            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, fj32, chan, hist);
             const unsigned long long mr = __ballot(need);
             if (mr) {
                 if (need) rq[rn + lane_prefix(mr)] = e;
@@ -624,7 +629,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     wave_sync_lds();
                     rn -= 64;
                     nre += 1;
-                    recheck_candidate<TRI>(a, rq[rn + lane], ti_lds, pi, pj, slot0, cellp, thr, chan, hist);
+                    recheck_candidate<TRI>(a, rq[rn + lane], fi_lds, fj32, pi, pj, cellp, thr, chan, hist);
                     wave_sync_lds();
                 }
             }
@@ -823,7 +828,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         wave_sync_lds();
         if (lane < rn) {
             nre += 1;
-            recheck_candidate<TRI>(a, rq[lane], ti_lds, pi, pj, slot0, cellp, thr, chan, hist);
+            recheck_candidate<TRI>(a, rq[lane], fi_lds, fj32, pi, pj, cellp, thr, chan, hist);
         }
         wave_sync_lds();
     }
@@ -1089,7 +1094,7 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
     const size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
                           (size_t)a.nch * g->nbin * sizeof(unsigned);
     auto screened_lds = [&](int qcap) {
-        return WG * sizeof(uint4) + (WG / 64) * (qcap + RQCAP) * sizeof(uint32_t) + WG * sizeof(int) + common;
+        return WG * sizeof(uint4) + (WG / 64) * (qcap + RQCAP) * sizeof(uint32_t) + common;
     };
     // the cluster screen needs the larger queue; without room it is not used
     const bool cluster_ok = g->tune.cluster && form == ANL_GOFRT_SCREENED && a.rho_cap > 0 &&

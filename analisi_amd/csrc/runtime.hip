@@ -2,6 +2,7 @@
 // series mirrors, block statistics kernels.
 #include <algorithm>
 #include <array>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -71,15 +72,19 @@ __device__ static inline uint32_t fix32(double s) {
     return (uint32_t)(uint64_t)u;
 }
 
-// Per atom and frame: {a, b, c} as 32-bit fixed-point fractions (frac32) and
-// the same rounded to 16 bits, packed {a | b << 16, c} (frac16).
-// out32.w carries the slot's type id, so that one 16-byte load gives a
-// candidate's position and type.
-__global__ void frac_kernel(const double4 *__restrict__ pos, const int32_t *__restrict__ type_slot, uint64_t natoms,
-                            uint2 *__restrict__ out16, uint4 *__restrict__ out32, uint64_t total, Cell c) {
+// Per screen slot k = (frame, s): the atom at slot order[k] of that frame
+// (slot s itself without an order) as {a, b, c} 32-bit fixed-point fractions
+// (frac32) and the same rounded to 16 bits, packed {a | b << 16, c} (frac16).
+// out32.w = type | slot << 8, so that one 16-byte load gives a candidate's
+// position, type and identity.
+__global__ void frac_kernel(const double4 *__restrict__ pos, const int32_t *__restrict__ type_slot,
+                            const int32_t *__restrict__ order, uint64_t natoms, uint2 *__restrict__ out16,
+                            uint4 *__restrict__ out32, uint64_t total, Cell c) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
-    double4 x = pos[k];
+    const uint64_t f = k / natoms;
+    const uint32_t sl = order ? (uint32_t)order[k] : (uint32_t)(k - f * natoms);
+    double4 x = pos[f * natoms + sl];
     double s[3];
     to_fractional(c, x.x, x.y, x.z, s);
     uint32_t a = fix32(s[0]), b = fix32(s[1]), cc = fix32(s[2]);
@@ -87,7 +92,7 @@ __global__ void frac_kernel(const double4 *__restrict__ pos, const int32_t *__re
     uint32_t b16 = ((b + 0x8000u) >> 16) & 0xFFFFu;
     uint32_t c16 = ((cc + 0x8000u) >> 16) & 0xFFFFu;
     out16[k] = make_uint2(a16 | (b16 << 16), c16);
-    out32[k] = make_uint4(a, b, cc, (uint32_t)type_slot[k % natoms]);
+    out32[k] = make_uint4(a, b, cc, (uint32_t)type_slot[sl] | (sl << 8));
 }
 
 void build_frac16(anl_traj *tr, hipStream_t s) {
@@ -96,15 +101,16 @@ void build_frac16(anl_traj *tr, hipStream_t s) {
     ANL_HIP(hipMalloc(&tr->frac16, total * sizeof(uint2)));
     ANL_HIP(hipMalloc(&tr->frac32, total * sizeof(uint4)));
     unsigned blocks = (unsigned)((total + 255) / 256);
-    hipLaunchKernelGGL(frac_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->type_slot, tr->N, tr->frac16,
-                       tr->frac32, total, tr->cell0);
+    hipLaunchKernelGGL(frac_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->type_slot, tr->frac_order, tr->N,
+                       tr->frac16, tr->frac32, total, tr->cell0);
     ANL_HIP(hipGetLastError());
     ANL_HIP(hipStreamSynchronize(s));
+    (void)hipFree(tr->frac_order);  // the fractions hold the order from now on
+    tr->frac_order = nullptr;
 }
 
 // Recursive median bisection of order[lo, hi) (see the slot order below).
-static void kd_order(std::vector<int32_t> &order, const std::vector<std::array<float, 3>> &fr, size_t lo, size_t hi,
-                     const double len[3]) {
+static void kd_order(int32_t *order, const std::array<float, 3> *fr, size_t lo, size_t hi, const double len[3]) {
     const size_t n = hi - lo;
     if (n <= 32) return;
     float mn[3] = {2.0f, 2.0f, 2.0f}, mx[3] = {-1.0f, -1.0f, -1.0f};
@@ -117,7 +123,7 @@ static void kd_order(std::vector<int32_t> &order, const std::vector<std::array<f
     for (int k = 1; k < 3; k++)
         if ((mx[k] - mn[k]) * len[k] > (mx[ax] - mn[ax]) * len[ax]) ax = k;
     const size_t half = (n + 63) / 64 * 32;  // a multiple of 32, < n
-    std::nth_element(order.begin() + lo, order.begin() + lo + half, order.begin() + hi, [&](int32_t a, int32_t b) {
+    std::nth_element(order + lo, order + lo + half, order + hi, [&](int32_t a, int32_t b) {
         return fr[a][ax] < fr[b][ax] || (fr[a][ax] == fr[b][ax] && a < b);
     });
     kd_order(order, fr, lo, lo + half, len);
@@ -244,7 +250,7 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
         std::vector<int32_t> order(natoms);
         std::iota(order.begin(), order.end(), 0);
         const double len[3] = {tr->cell0.L[0], tr->cell0.L[1], tr->cell0.L[2]};
-        kd_order(order, fr, 0, natoms, len);
+        kd_order(order.data(), fr.data(), 0, natoms, len);
         tr->type_of_slot.resize(natoms);
         tr->type_count.assign(ntypes, 0);
         for (uint64_t s = 0; s < natoms; s++) {
@@ -268,6 +274,34 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
         ANL_HIP(hipMemcpy(tr->type_slot, tr->type_of_slot.data(), natoms * sizeof(int32_t), hipMemcpyHostToDevice));
         ANL_HIP(hipMalloc(&tr->orig_slot, natoms * sizeof(int32_t)));
         ANL_HIP(hipMemcpy(tr->orig_slot, order.data(), natoms * sizeof(int32_t), hipMemcpyHostToDevice));
+        // screen order, per frame (the screened g(r,t) needs a constant
+        // cell): the same bisection of that frame's positions, so that the
+        // atoms of one wavefront stay compact in every frame while they
+        // diffuse.  frac_order[f][s] = slot of the atom at screen slot s.
+        if (tr->box_const) {
+            std::vector<int32_t> ford(nframes * natoms);
+            const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            for (unsigned k = 0; k < nth; k++)
+                pool.emplace_back([&, k] {
+                    std::vector<std::array<float, 3>> fs(natoms);
+                    for (uint64_t f = k; f < nframes; f += nth) {
+                        const double *p = pos + f * natoms * 3;
+                        for (uint64_t sl = 0; sl < natoms; sl++) {
+                            const double *x = p + (uint64_t)order[sl] * 3;
+                            double s3[3];
+                            to_fractional(tr->cell0, x[0], x[1], x[2], s3);
+                            for (int c = 0; c < 3; c++) fs[sl][c] = (float)(s3[c] - std::floor(s3[c]));
+                        }
+                        int32_t *o = ford.data() + f * natoms;
+                        std::iota(o, o + natoms, 0);
+                        kd_order(o, fs.data(), 0, natoms, len);
+                    }
+                });
+            for (auto &t : pool) t.join();
+            ANL_HIP(hipMalloc(&tr->frac_order, nframes * natoms * sizeof(int32_t)));
+            ANL_HIP(hipMemcpy(tr->frac_order, ford.data(), nframes * natoms * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
         if (pos_cm) {
             ANL_HIP(hipMalloc(&tr->cm, nframes * ntypes * 3 * sizeof(double)));
             ANL_HIP(hipMemcpy(tr->cm, pos_cm, nframes * ntypes * 3 * sizeof(double), hipMemcpyHostToDevice));
@@ -283,6 +317,7 @@ int anl_traj_destroy(anl_traj *tr) {
         (void)hipFree(tr->box);
         (void)hipFree(tr->type_slot);
         (void)hipFree(tr->orig_slot);
+        (void)hipFree(tr->frac_order);
         (void)hipFree(tr->cm);
         (void)hipFree(tr->frac16);
         (void)hipFree(tr->frac32);

@@ -131,3 +131,19 @@ def test_gk_blocks_errors():
     with pytest.raises(capi.AnlError) as e:      # block 1 runs past the end
         s.gk_blocks([0], 1, 100, 1, False, 0, [0, 800], [200, 200])
     assert e.value.code == capi.ANL_ERR_TOO_SHORT
+
+
+@pytest.mark.parametrize("mfma", [False, True])
+def test_gk_blocks_more_than_one_batch(mfma):
+    """70 blocks in one call run as two batches of launches (at most 64
+    blocks each); every block still equals its own single call."""
+    J = synth.ar2_currents(78, 7400, ncur=1)
+    s = capi.DeviceSeries(J)
+    primo = [100 * b for b in range(70)]
+    n = [100 + 20 * (b % 7) for b in range(70)]
+    tmax = 50
+    per, mean, var = s.gk_blocks([0], 1, tmax, 1, False, 0, primo, n, mfma=mfma)
+    assert per.shape == (70, tmax, 5)
+    for b in (0, 5, 63, 64, 69):
+        one = s.gk([0], 1, tmax, n[b], 1, False, 0, primo[b], mfma=mfma)
+        assert np.array_equal(np.nan_to_num(per[b], nan=0.25), np.nan_to_num(one, nan=0.25)), b

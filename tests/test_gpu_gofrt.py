@@ -49,10 +49,28 @@ def test_every_and_skip_and_lag_layout():
             assert first_diff(got, ref) is None, (skip, every, form, first_diff(got, ref))
 
 
-def test_calc_single_th_ranges():
-    ot, dt = make_case(5, 150, 12, 9.0, [0.3, 0.7], 0.2, [0.5, 0.4, -0.3])
+def _scrambled_case(seed, N, F, L, fractions, tilt):
+    """Every frame independent uniform random positions: each frame's slot
+    order of the screen differs from every other frame's."""
+    from analisi_amd import synth
+    box_l = synth.lammps_box(L, tilt)
+    pos = np.stack([synth.random_walk(seed + f, N, 1, box_l, 0.0)[0] for f in range(F)])
+    types = synth.types_by_fraction(N, fractions)
+    ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), tilt is not None)
+    return ot, capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+
+
+@pytest.mark.parametrize("form", [capi.GOFRT_FP64, capi.GOFRT_SCREENED])
+@pytest.mark.parametrize("scrambled", [False, True])
+def test_calc_single_th_ranges(form, scrambled):
+    """Atom ranges of calc_single_th select i atoms by their original index,
+    also when the screen holds each frame in its own slot order."""
+    if scrambled:
+        ot, dt = _scrambled_case(5, 150, 12, 9.0, [0.3, 0.7], [0.5, 0.4, -0.3])
+    else:
+        ot, dt = make_case(5, 150, 12, 9.0, [0.3, 0.7], 0.2, [0.5, 0.4, -0.3])
     rmin, rmax, nbin, leff = 0.0, 4.0, 33, 5
-    g = capi.DeviceGofrt(dt, rmin, rmax, nbin, leff, 1, 1, capi.GOFRT_FP64)
+    g = capi.DeviceGofrt(dt, rmin, rmax, nbin, leff, 1, 1, form)
     g.reset(6)
     acc = None
     ref = np.zeros((leff, 6, nbin), dtype=np.uint64)
@@ -60,6 +78,7 @@ def test_calc_single_th_ranges():
         acc = g.calc_single_th(t, im, a0, a1, 2, acc)
         O.gofrt_single(ot, rmin, rmax, nbin, leff, t, im, a0, a1, 2, ref)
     assert first_diff(acc, ref) is None, first_diff(acc, ref)
+    assert g.stats()[0] == form
 
 
 def test_bin_edges_adversarial():
@@ -141,28 +160,30 @@ def test_launch_shapes_and_autotune_keep_counts(form):
 
 # Boxes large against rmax, so that compact wavefronts take the cluster screen:
 # id, natoms, nframes, L, fractions, sigma, tilt, wrap, unwrapped shift, rmax,
-# first frame replaced by uniform random positions (None: none)
+# first frame of sparse uniform random positions (None: none)
 CLUSTER_CASES = [
     ("tri", 4000, 10, 40.0, [0.7, 0.3], 0.3, [3.1, -2.7, 2.2], False, 0.0, 7.0, None),
     ("ortho-unwrapped", 3000, 9, 36.0, [0.5, 0.5], 0.4, None, False, 61.0, 6.5, None),
     ("tri-wrapped", 2500, 8, 34.0, [1.0], 0.5, [-2.0, 1.5, -1.0], True, 0.0, 8.0, None),
-    ("tri-mixed", 4000, 12, 40.0, [0.6, 0.4], 0.3, [1.0, 1.0, -1.0], False, 0.0, 6.0, 6),
+    ("tri-mixed", 400, 12, 40.0, [0.6, 0.4], 0.3, [1.0, 1.0, -1.0], False, 0.0, 6.0, 6),
 ]
 
 
 @pytest.mark.parametrize("case", CLUSTER_CASES, ids=[c[0] for c in CLUSTER_CASES])
 def test_cluster_screen_matches_oracle(case):
     """The cluster (FP16 MFMA) screen gives the oracle's counts.  In the last
-    case the frames from 6 on are uniform random positions: their wavefronts
-    are not compact and fall back to the 16-bit screen in the same launch,
-    while the earlier frames' wavefronts use the cluster screen."""
+    case the first 6 frames hold all 400 atoms in an 8 A blob and the later
+    frames spread them uniformly over the 40 A cell: 64 atoms of a sparse
+    frame span about 20 A, their wavefronts fall back to the 16-bit screen
+    in the same launch, while the blob frames' wavefronts use the cluster
+    screen."""
     from analisi_amd import synth
     name, N, F, L, fr, sig, tilt, wrap, shift, rmax, scramble = case
     if scramble is None:
         ot, dt = make_case(21, N, F, L, fr, sig, tilt, wrap, shift)
     else:
         box_l = synth.lammps_box(L, tilt)
-        pos = synth.random_walk(21, N, F, box_l, sig)
+        pos = synth.random_walk(21, N, F, synth.lammps_box(8.0), sig * 0.1) + 16.0
         for f in range(scramble, F):
             pos[f] = synth.random_walk(1000 + f, N, 1, box_l, 0.0)[0]
         types = synth.types_by_fraction(N, fr)
