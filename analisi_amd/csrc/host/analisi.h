@@ -7,6 +7,8 @@
 //   VectorOp<T,TFLOAT>                operazionisulista.h:25-148
 //   Trajectory_numpy                  trajectory_numpy.{h,cpp} (plain arrays in
 //                                     place of pybind11 buffers)
+//   lammps_bin::dump                  basetrajectory.cpp:4-47 (LAMMPS-binary writer;
+//                                     the reader, Trajectory, is lammps_traj.h)
 //   ReadLog_numpy<TFLOAT>             readlog_numpy.{h,cpp}
 //   Gofrt<TFLOAT,T>                   gofrt.{h,cpp}
 //   MSD<T,FPE>                        msd.{h,cpp}
@@ -85,6 +87,71 @@ class VectorOp {
     TFLOAT *vdata;
     unsigned int data_length;
 };
+
+// ---------------------------------------------------------------------------
+// LAMMPS-binary writer (basetrajectory.cpp:4-47), shared by both trajectories
+// ---------------------------------------------------------------------------
+namespace lammps_bin {
+
+constexpr int NDOUBLE_ATOMO = 8;  // id, type, x, y, z, vx, vy, vz (lammps_struct.h:104-122)
+
+// internal box (xlo, ylo, zlo, lx/2, ly/2, lz/2) -> lammps (basetrajectory.h:110-119)
+inline void internal_to_lammps(double *c) {
+    double t = c[3];
+    c[3] = c[4] * 2 + c[1];
+    c[5] = c[5] * 2 + c[2];
+    c[4] = c[2];
+    c[2] = c[1];
+    c[1] = c[0] + t * 2;
+}
+
+// Timesteps [start_ts, stop_ts) (all of them when stop_ts <= 0), one chunk
+// each, the old header layout (lammps_struct.h:65-97), atom ids 0..N-1 and
+// the internal type index as the type.
+template <class T>
+void dump(T &t, const std::string &fname, int start_ts, int stop_ts) {
+    const int64_t nts = (int64_t)t.get_ntimesteps();
+    if (start_ts < 0 || start_ts >= nts)
+        throw std::runtime_error("You must provide a starting timestep between 0 and the number of timesteps!");
+    if (stop_ts <= 0) stop_ts = (int)nts;
+    std::ofstream out(fname, std::ofstream::binary);
+    const int64_t natoms = (int64_t)t.get_natoms();
+    const int tri = t.is_triclinic() ? 1 : 0;
+    for (int ts = start_ts; ts < stop_ts; ++ts) {
+        double box[6], tilt[3] = {0, 0, 0};
+        const double *b = t.box(ts);
+        for (int i = 0; i < 6; ++i) box[i] = b[i];
+        if (tri)
+            for (int i = 0; i < 3; ++i) tilt[i] = b[6 + i];
+        internal_to_lammps(box);
+        const int64_t step = ts;
+        const int zero6[6] = {0, 0, 0, 0, 0, 0};
+        const int size_one = NDOUBLE_ATOMO, nchunk = 1;
+        out.write((const char *)&step, sizeof(step));
+        out.write((const char *)&natoms, sizeof(natoms));
+        out.write((const char *)&tri, sizeof(tri));
+        out.write((const char *)zero6, sizeof(zero6));
+        out.write((const char *)box, sizeof(box));
+        if (tri) out.write((const char *)tilt, sizeof(tilt));
+        out.write((const char *)&size_one, sizeof(size_one));
+        out.write((const char *)&nchunk, sizeof(nchunk));
+        const int n_data = (int)(natoms * NDOUBLE_ATOMO);
+        out.write((const char *)&n_data, sizeof(n_data));
+        for (int64_t ia = 0; ia < natoms; ++ia) {
+            double data[NDOUBLE_ATOMO];
+            data[0] = (double)ia;
+            data[1] = (double)t.get_type((unsigned int)ia);
+            const double *x = t.positions(ts, (int)ia);
+            for (int k = 0; k < 3; ++k) data[2 + k] = x[k];
+            const double *v = t.velocity(ts, (int)ia);
+            for (int k = 0; k < 3; ++k) data[5 + k] = v[k];
+            out.write((const char *)data, sizeof(data));
+        }
+    }
+    if (!out) throw std::runtime_error("Error writing the trajectory \"" + fname + "\"");
+}
+
+}  // namespace lammps_bin
 
 // ---------------------------------------------------------------------------
 // Trajectory_numpy (trajectory_numpy.cpp:7-199)
@@ -177,6 +244,13 @@ class Trajectory_numpy {
     bool get_pbc_wrap() const { return wrap_pbc; }
     bool is_triclinic() const { return triclinic; }
     size_t get_box_stride() const { return buffer_boxes_stride; }
+    void dump_lammps_bin_traj(const std::string &fname, int start_ts, int stop_ts) {
+        lammps_bin::dump(*this, fname, start_ts, stop_ts);
+    }
+    // every timestep is resident: a calculation on [primo, primo + need)
+    // uses device timestep primo; the contents never change
+    size_t window_offset(size_t primo, size_t /*need*/) { return primo; }
+    uint64_t generation() const { return 0; }
 
     // basetrajectory.h:94-105
     static void lammps_to_internal(double *c) {
@@ -399,8 +473,8 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
     Gofrt(T *t, TFLOAT rmin, TFLOAT rmax, unsigned int nbin, unsigned int tmax = 0, unsigned int nthreads = 0,
           unsigned int skip = 1, unsigned int every = 1, bool debug = false, int form = ANL_GOFRT_AUTO)
         : traiettoria(t), rmin(rmin), rmax(rmax), nbin(nbin), lmax(tmax), nthreads(nthreads == 0 ? 1 : nthreads),
-          skip(skip == 0 ? 1 : skip), every(every == 0 ? 1 : every), debug(debug) {
-        anl_check(anl_gofrt_create(t->device(), rmin, rmax, nbin, tmax, this->skip, this->every, form, &g));
+          skip(skip == 0 ? 1 : skip), every(every == 0 ? 1 : every), debug(debug), form(form) {
+        if (t->get_nloaded_timesteps() > 0) h();  // argument errors surface here
     }
     ~Gofrt() {
         if (g) anl_gofrt_destroy(g);
@@ -419,7 +493,8 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
         data_length = leff * nch() * nbin;
         delete[] vdata;
         vdata = new TFLOAT[data_length];
-        anl_check(anl_gofrt_reset(g, n, nullptr));
+        reset_done = true;
+        if (g && gen == traiettoria->generation()) anl_check(anl_gofrt_reset(g, n, nullptr));
     }
     std::vector<ssize_t> get_shape() { return {(ssize_t)leff, (ssize_t)nch(), (ssize_t)nbin}; }
     std::vector<ssize_t> get_stride() {
@@ -429,8 +504,10 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
     // the whole (lag, t0, pair) nest in one device call, calc_end.
     void calculate(size_t primo) {
         calc_init((int)primo);
+        // timesteps primo .. primo + n - 1 + leff - 1 must be on the device
+        const size_t off = traiettoria->window_offset(primo, (size_t)ntimesteps + leff - 1);
         std::vector<uint64_t> counts(data_length);
-        anl_check(anl_gofrt_calculate(g, primo, counts.data()));
+        anl_check(anl_gofrt_calculate(h(), off, counts.data()));
         for (unsigned int k = 0; k < data_length; k++) vdata[k] = (TFLOAT)counts[k] * incr;
         calc_end();
     }
@@ -446,7 +523,8 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
     // incr per hit into vdata (the ith argument is kept for the signature).
     void calc_single_th(int t, int imedia, int a0, int a1, int primo, int /*ith*/) {
         std::vector<uint64_t> counts(data_length, 0);
-        anl_check(anl_gofrt_calc_single_th(g, t, imedia, a0, a1, (uint64_t)primo, counts.data()));
+        const size_t off = traiettoria->window_offset((size_t)primo, (size_t)imedia + (size_t)t + 1);
+        anl_check(anl_gofrt_calc_single_th(h(), t, imedia, a0, a1, (uint64_t)off, counts.data()));
         for (unsigned int k = 0; k < data_length; k++)
             if (counts[k]) vdata[k] += (TFLOAT)counts[k] * incr;
     }
@@ -465,12 +543,28 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
     anl_gofrt *handle() { return g; }
 
    private:
+    // device handle over the trajectory's current device mirror; rebuilt
+    // (with the same reset) when a windowed trajectory has moved its window
+    anl_gofrt *h() {
+        if (!g || gen != traiettoria->generation()) {
+            anl_traj *d = traiettoria->device();
+            if (g) anl_gofrt_destroy(g);
+            g = nullptr;
+            anl_check(anl_gofrt_create(d, rmin, rmax, nbin, lmax, skip, every, form, &g));
+            gen = traiettoria->generation();
+            if (reset_done) anl_check(anl_gofrt_reset(g, ntimesteps, nullptr));
+        }
+        return g;
+    }
     unsigned int nch() const { return (unsigned int)(traiettoria->get_ntypes() * (traiettoria->get_ntypes() + 1)); }
     T *traiettoria;
     TFLOAT rmin, rmax, incr = 1;
     unsigned int nbin, lmax, nthreads, skip, every;
     unsigned int leff = 0, ntimesteps = 0;
     bool debug;
+    int form;
+    bool reset_done = false;
+    uint64_t gen = 0;
     anl_gofrt *g = nullptr;
 };
 
@@ -502,7 +596,8 @@ class MSD : public VectorOp<MSD<T, FPE>> {
     }
     void calculate(size_t primo) {
         calc_init(primo);
-        anl_check(anl_msd_calculate(traiettoria->device(), skip, lmax, cm_msd, cm_self, ntimesteps, primo, vdata));
+        const size_t off = traiettoria->window_offset(primo, leff + ntimesteps);
+        anl_check(anl_msd_calculate(traiettoria->device(), skip, lmax, cm_msd, cm_self, ntimesteps, off, vdata));
         calc_end();
     }
     void calc_init(size_t primo) {  // msd.cpp:52-61

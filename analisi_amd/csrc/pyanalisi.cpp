@@ -1,14 +1,18 @@
 // pybind11 module with the pyanalisi classes of the pair-correlation path
-// (pyanalisi/src/pyanalisi.cpp:65-153, :241-279, :281-373, :474-550 of
-// rikigigi/analisi v0.5.2): Trajectory, ReadLog, BoxFormat, Gofrt,
-// MeanSquareDisplacement, GreenKubo, info(), has_mmap().  Same constructor
-// arguments, methods and buffer protocol (FP64, zero copy on vdata); the
-// calculations run on the GPU through the C ABI.
+// (pyanalisi/src/pyanalisi.cpp:65-153, :241-279, :281-373, :377-450,
+// :474-550 of rikigigi/analisi v0.5.2): Trajectory, Traj (LAMMPS binary),
+// ReadLog, ReadLog_file (text), BoxFormat, Gofrt / Gofrt_lammps,
+// MeanSquareDisplacement / MeanSquareDisplacement_lammps, GreenKubo /
+// GreenKubo_file, info(), has_mmap().  Same
+// constructor arguments, methods and buffer protocol (FP64, zero copy on
+// vdata); the calculations run on the GPU through the C ABI.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
 #include "analisi.h"
+#include "lammps_traj.h"
+#include "readlog.h"
 
 namespace py = pybind11;
 using namespace analisi;
@@ -85,6 +89,99 @@ py::array_t<double> copy_array(const double *p, std::vector<ssize_t> shape) {
     return a;
 }
 
+// trajectory_common_interfaces (pyanalisi.cpp:281-373): the methods both
+// trajectory classes share; the copies cover the loaded timesteps
+template <class Tk, class C>
+C &common_interfaces(C &c) {
+    c.def("write_lammps_binary", &Tk::dump_lammps_bin_traj, py::arg("filename"), py::arg("start"),
+          py::arg("stop"))
+        .def("get_positions_copy",
+             [](Tk &t) {
+                 if (t.get_nloaded_timesteps() == 0) return py::array_t<double>();
+                 return copy_array(t.positions(t.get_current_timestep(), 0),
+                                   {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_natoms(), 3});
+             })
+        .def("get_velocities_copy",
+             [](Tk &t) {
+                 if (t.get_nloaded_timesteps() == 0) return py::array_t<double>();
+                 return copy_array(t.velocity(t.get_current_timestep(), 0),
+                                   {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_natoms(), 3});
+             })
+        .def("get_box_copy",
+             [](Tk &t) {
+                 if (t.get_nloaded_timesteps() == 0) return py::array_t<double>();
+                 return copy_array(t.box(t.get_current_timestep()),
+                                   {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_box_stride()});
+             })
+        .def("get_type", &Tk::get_type, "get the type used for the internal representation")
+        .def("get_nloaded_timesteps", &Tk::get_nloaded_timesteps)
+        .def("getNtimesteps", &Tk::get_ntimesteps)
+        .def("get_current_timestep", &Tk::get_current_timestep)
+        .def("getWrapPbc", &Tk::get_pbc_wrap)
+        .def("minImage", [](Tk &t, size_t i, size_t j, size_t it, size_t ij) {
+            py::array_t<double> a(4);
+            double *x = a.mutable_data();
+            x[3] = t.d2_minImage(i, j, it, ij, x);
+            return a;
+        });
+    return c;
+}
+
+// define_atomic_traj (pyanalisi.cpp:65-153): the pair-correlation operators
+// over one trajectory class, named with its suffix
+template <class T>
+void define_atomic_traj(py::module &m, const std::string &suffix) {
+    using G = Gofrt<double, T>;
+    using M = MSD<T>;
+    py::class_<G>(m, ("Gofrt" + suffix).c_str(), py::buffer_protocol())
+        .def(py::init<T *, double, double, unsigned int, unsigned int, unsigned int, unsigned int, unsigned int, bool,
+                      int>(),
+             py::arg("trajectory"), py::arg("rmin"), py::arg("rmax"), py::arg("nbin"), py::arg("tmax") = 0,
+             py::arg("nthreads") = 0, py::arg("skip") = 1, py::arg("every") = 1, py::arg("debug") = false,
+             py::arg("form") = ANL_GOFRT_AUTO)
+        .def("reset", &G::reset)
+        .def("getNumberOfExtraTimestepsNeeded", &G::nExtraTimesteps)
+        .def("calculate", &G::calculate)
+        .def("calc_single_th", &G::calc_single_th)
+        .def("last_stats",
+             [](G &g) {
+                 int f = -1;
+                 uint64_t c = 0, r = 0;
+                 anl_check(anl_gofrt_last_stats(g.handle(), &f, &c, &r));
+                 return py::make_tuple(f, c, r);
+             })
+        .def_buffer([](G &g) { return vbuffer(g); });
+
+    py::class_<M>(m, ("MeanSquareDisplacement" + suffix).c_str(), py::buffer_protocol())
+        .def(py::init<T *, unsigned int, unsigned int, unsigned int, bool, bool, bool>(), py::arg("trajectory"),
+             py::arg("skip") = 1, py::arg("tmax") = 0, py::arg("nthreads") = 0, py::arg("cm") = false,
+             py::arg("cm_self") = false, py::arg("debug") = false)
+        .def("reset", &M::reset)
+        .def("getNumberOfExtraTimestepsNeeded", &M::nExtraTimesteps)
+        .def("calculate", &M::calculate)
+        .def_buffer([](M &x) { return vbuffer(x); })
+        .def("__enter__", [](M &x) -> M & { return x; })
+        .def("__exit__", [](M &, py::object, py::object, py::object) {});
+}
+
+// gk (pyanalisi.cpp:241-279): GreenKubo over one log class
+template <class RLT>
+void define_gk(py::module &m, const std::string &suffix) {
+    using G = GreenKuboNComponentIonicFluid<RLT, double, double>;
+    py::class_<G>(m, ("GreenKubo" + suffix).c_str(), py::buffer_protocol())
+        .def(py::init<RLT *, std::string, unsigned int, std::vector<std::string>, bool, unsigned int, unsigned int,
+                      bool, unsigned int, unsigned int, bool, unsigned int, unsigned int, bool>(),
+             py::arg("log"), py::arg("dump"), py::arg("skip"), py::arg("headers"), py::arg("dump_files") = false,
+             py::arg("lmax") = 0, py::arg("nthreads") = 0, py::arg("subtract_mean") = false,
+             py::arg("start_mean") = 100, py::arg("n_seg") = 1, py::arg("do_bench") = false,
+             py::arg("n_seg_start") = 10, py::arg("n_seg_stop") = 100, py::arg("mfma") = false)
+        .def("getNumberOfExtraTimestepsNeeded", &G::nExtraTimesteps)
+        .def("reset", &G::reset)
+        .def("calculate", &G::calculate)
+        .def("get_columns_description", &G::get_columns_description)
+        .def_buffer([](G &g) { return vbuffer(g); });
+}
+
 }  // namespace
 
 using GofrtT = Gofrt<double, Trajectory_numpy>;
@@ -100,40 +197,34 @@ PYBIND11_MODULE(_pyanalisi, m) {
         .value("LammpsOrtho", Trajectory_numpy::BoxFormat::Lammps_ortho)
         .value("LammpsTriclinic", Trajectory_numpy::BoxFormat::Lammps_triclinic);
 
-    py::class_<Trajectory_numpy>(m, "Trajectory")
-        .def(py::init(&make_trajectory), py::keep_alive<1, 2>(), py::keep_alive<1, 3>(), py::keep_alive<1, 4>(),
-             py::keep_alive<1, 5>(), py::arg("positions"), py::arg("velocities"), py::arg("types"),
-             py::arg("lattice"), py::arg("format"), py::arg("pbc_wrap") = false,
-             py::arg("save_rotation_matrix") = false)
-        .def("get_positions_copy",
-             [](Trajectory_numpy &t) {
-                 return copy_array(t.positions(0, 0), {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_natoms(), 3});
-             })
-        .def("get_velocities_copy",
-             [](Trajectory_numpy &t) {
-                 return copy_array(t.velocity(0, 0), {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_natoms(), 3});
-             })
-        .def("get_box_copy",
-             [](Trajectory_numpy &t) {
-                 return copy_array(t.box(0), {(ssize_t)t.get_nloaded_timesteps(), (ssize_t)t.get_box_stride()});
-             })
-        .def("get_rotation_matrix",
-             [](Trajectory_numpy &t) {
-                 double *r = t.get_rotation_matrix(0);
-                 if (!r) return py::array_t<double>();
-                 return copy_array(r, {(ssize_t)t.get_ntimesteps(), 3, 3});
-             })
-        .def("get_type", &Trajectory_numpy::get_type, "get the type used for the internal representation")
-        .def("get_nloaded_timesteps", &Trajectory_numpy::get_nloaded_timesteps)
-        .def("getNtimesteps", &Trajectory_numpy::get_ntimesteps)
-        .def("get_current_timestep", &Trajectory_numpy::get_current_timestep)
-        .def("getWrapPbc", &Trajectory_numpy::get_pbc_wrap)
-        .def("minImage", [](Trajectory_numpy &t, size_t i, size_t j, size_t it, size_t ij) {
-            py::array_t<double> a(4);
-            double *x = a.mutable_data();
-            x[3] = t.d2_minImage(i, j, it, ij, x);
-            return a;
+    // Traj: the memory-mapped LAMMPS-binary trajectory (pyanalisi.cpp:377-450)
+    py::class_<Trajectory> traj(m, "Traj", py::buffer_protocol());
+    traj.def(py::init<std::string>(), py::arg("filename"))
+        .def("setWrapPbc", &Trajectory::set_pbc_wrap)
+        .def("setAccessWindowSize", [](Trajectory &t, int ts) { return (int)t.set_data_access_block_size((size_t)ts); })
+        .def("setAccessStart", [](Trajectory &t, int ts) { return (int)t.set_access_at((size_t)ts); })
+        .def("serving_pos", &Trajectory::serving_pos)
+        .def("toggle_pos_vel", &Trajectory::toggle_pos_vel)
+        .def_buffer([](Trajectory &t) {
+            return py::buffer_info(t.serving_pos() ? t.positions_data() : t.velocity_data(), sizeof(double),
+                                   py::format_descriptor<double>::format(), 3, t.get_shape(), t.get_stride());
+        })
+        .def("get_lammps_id", [](Trajectory &t) { return py::array_t<int>(py::cast(t.get_lammps_id())); })
+        .def("get_lammps_type", [](Trajectory &t) { return py::array_t<int>(py::cast(t.get_lammps_type())); })
+        .def("get_timestep_lammps", &Trajectory::get_timestep_lammps);
+    common_interfaces<Trajectory>(traj);
+
+    py::class_<Trajectory_numpy> tnp(m, "Trajectory");
+    tnp.def(py::init(&make_trajectory), py::keep_alive<1, 2>(), py::keep_alive<1, 3>(), py::keep_alive<1, 4>(),
+            py::keep_alive<1, 5>(), py::arg("positions"), py::arg("velocities"), py::arg("types"),
+            py::arg("lattice"), py::arg("format"), py::arg("pbc_wrap") = false,
+            py::arg("save_rotation_matrix") = false)
+        .def("get_rotation_matrix", [](Trajectory_numpy &t) {
+            double *r = t.get_rotation_matrix(0);
+            if (!r) return py::array_t<double>();
+            return copy_array(r, {(ssize_t)t.get_ntimesteps(), 3, 3});
         });
+    common_interfaces<Trajectory_numpy>(tnp);
 
     py::class_<RL>(m, "ReadLog")
         .def(py::init([](py::buffer data, std::vector<std::string> headers) {
@@ -149,48 +240,25 @@ PYBIND11_MODULE(_pyanalisi, m) {
              py::keep_alive<1, 2>())
         .def("getNtimesteps", &RL::n_timestep);
 
-    py::class_<GofrtT>(m, "Gofrt", py::buffer_protocol())
-        .def(py::init<Trajectory_numpy *, double, double, unsigned int, unsigned int, unsigned int, unsigned int,
-                      unsigned int, bool, int>(),
-             py::arg("trajectory"), py::arg("rmin"), py::arg("rmax"), py::arg("nbin"), py::arg("tmax") = 0,
-             py::arg("nthreads") = 0, py::arg("skip") = 1, py::arg("every") = 1, py::arg("debug") = false,
-             py::arg("form") = ANL_GOFRT_AUTO)
-        .def("reset", &GofrtT::reset)
-        .def("getNumberOfExtraTimestepsNeeded", &GofrtT::nExtraTimesteps)
-        .def("calculate", &GofrtT::calculate)
-        .def("calc_single_th", &GofrtT::calc_single_th)
-        .def("last_stats",
-             [](GofrtT &g) {
-                 int f = -1;
-                 uint64_t c = 0, r = 0;
-                 anl_check(anl_gofrt_last_stats(g.handle(), &f, &c, &r));
-                 return py::make_tuple(f, c, r);
-             })
-        .def_buffer([](GofrtT &g) { return vbuffer(g); });
+    define_atomic_traj<Trajectory_numpy>(m, "");
+    define_atomic_traj<Trajectory>(m, "_lammps");
 
-    py::class_<MSDT>(m, "MeanSquareDisplacement", py::buffer_protocol())
-        .def(py::init<Trajectory_numpy *, unsigned int, unsigned int, unsigned int, bool, bool, bool>(),
-             py::arg("trajectory"), py::arg("skip") = 1, py::arg("tmax") = 0, py::arg("nthreads") = 0,
-             py::arg("cm") = false, py::arg("cm_self") = false, py::arg("debug") = false)
-        .def("reset", &MSDT::reset)
-        .def("getNumberOfExtraTimestepsNeeded", &MSDT::nExtraTimesteps)
-        .def("calculate", &MSDT::calculate)
-        .def_buffer([](MSDT &m) { return vbuffer(m); })
-        .def("__enter__", [](MSDT &m) -> MSDT & { return m; })
-        .def("__exit__", [](MSDT &, py::object, py::object, py::object) {});
+    // ReadLog_file: the text log reader (pyanalisi.cpp:455-471); calc_currents
+    // fills its "#traj:" columns from a Traj, as the CLI does (main.cpp)
+    using RLF = ReadLog<double>;
+    py::class_<RLF>(m, "ReadLog_file")
+        .def(py::init<std::string, Trajectory *, unsigned int, unsigned int, unsigned int, std::vector<std::string>>(),
+             py::arg("filename"), py::arg("traj") = nullptr, py::arg("skip") = 1, py::arg("nthreads") = 0,
+             py::arg("nbatch") = 200, py::arg("headers") = std::vector<std::string>())
+        .def("getNtimesteps", &RLF::n_timestep)
+        .def("calc_currents", &RLF::calc_currents, py::arg("traj"), py::arg("blocks"))
+        .def("get_data_copy", [](RLF &l) {
+            return copy_array(l.line(0), {(ssize_t)l.n_timestep(), (ssize_t)l.n_data()});
+        })
+        .def("timestep", &RLF::timestep);
 
-    py::class_<GK>(m, "GreenKubo", py::buffer_protocol())
-        .def(py::init<RL *, std::string, unsigned int, std::vector<std::string>, bool, unsigned int, unsigned int,
-                      bool, unsigned int, unsigned int, bool, unsigned int, unsigned int, bool>(),
-             py::arg("log"), py::arg("dump"), py::arg("skip"), py::arg("headers"), py::arg("dump_files") = false,
-             py::arg("lmax") = 0, py::arg("nthreads") = 0, py::arg("subtract_mean") = false,
-             py::arg("start_mean") = 100, py::arg("n_seg") = 1, py::arg("do_bench") = false,
-             py::arg("n_seg_start") = 10, py::arg("n_seg_stop") = 100, py::arg("mfma") = false)
-        .def("getNumberOfExtraTimestepsNeeded", &GK::nExtraTimesteps)
-        .def("reset", &GK::reset)
-        .def("calculate", &GK::calculate)
-        .def("get_columns_description", &GK::get_columns_description)
-        .def_buffer([](GK &g) { return vbuffer(g); });
+    define_gk<RL>(m, "");
+    define_gk<RLF>(m, "_file");
 
     // BlockAverageG + MediaVarCovar of GreenKubo, as the CLI's -H path runs it
     // (main.cpp:456-488): returns (mean, variance) as [narr][block size].
@@ -238,5 +306,5 @@ PYBIND11_MODULE(_pyanalisi, m) {
           });
 
     m.def("info", []() { return std::string(anl_info()); });
-    m.def("has_mmap", []() { return false; });
+    m.def("has_mmap", []() { return true; });
 }

@@ -19,6 +19,18 @@ Contents
     test_gofr    tests/test_gofrt/test_gofr.csv as float64 [10, 12, 200]
                  (test_gofrt.py:4-13; its input positions.npy is not in the
                  snapshot, so only structural facts of it are usable)
+  lammps/        the LAMMPS-binary trajectory data, copied byte for byte:
+    10steps.bin, .pos, .vel, .cel, .species
+                 tests/data/10steps.*: a binary trajectory (432 atoms, 10
+                 triclinic frames) and the cp.x text files it was made from
+    dump4.lammpstrj
+                 the first 4 timesteps of tools/lammps/dump.lammpstrj
+    water125.pos, .vel, .cel, .species
+                 tools/cp/water125_testanalisi.* (cp.x text trajectory)
+    test_lammps2analisi.csv, test_cp2analisi.csv
+                 tests/test_write_binary/*.csv: positions, boxes and
+                 velocities read back from the binaries the two converters
+                 write (test_write_binary.py:59-83)
 """
 import os
 import shutil
@@ -53,6 +65,19 @@ def main(ref):
         shutil.copyfile(os.path.join(ref, "tests/data/cli", name), os.path.join(out, "cli_" + name + ".txt"))
     gofr = read_csv_column(os.path.join(ref, "tests/test_gofrt/test_gofr.csv")).reshape(10, 12, 200)
     np.savez_compressed(os.path.join(out, "gofr_reference.npz"), test_gofr=gofr)
+    lmp = os.path.join(out, "lammps")
+    os.makedirs(lmp, exist_ok=True)
+    for ext in ("bin", "pos", "vel", "cel", "species"):
+        shutil.copyfile(os.path.join(ref, "tests/data/10steps." + ext), os.path.join(lmp, "10steps." + ext))
+    for ext in ("pos", "vel", "cel", "species"):
+        shutil.copyfile(os.path.join(ref, "tools/cp/water125_testanalisi." + ext), os.path.join(lmp, "water125." + ext))
+    with open(os.path.join(ref, "tools/lammps/dump.lammpstrj")) as f:
+        lines = f.readlines()
+    natoms = int(lines[3])
+    with open(os.path.join(lmp, "dump4.lammpstrj"), "w") as f:
+        f.writelines(lines[:4 * (natoms + 9)])
+    for name in ("test_lammps2analisi.csv", "test_cp2analisi.csv"):
+        shutil.copyfile(os.path.join(ref, "tests/test_write_binary", name), os.path.join(lmp, name))
 
 
 if __name__ == "__main__":

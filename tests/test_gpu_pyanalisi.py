@@ -173,3 +173,95 @@ def test_triclinic_cell_vectors_rotation(pa):
     ot.pos[:] = rot
     counts, _ = O.gofrt(ot, 0.0, 4.0, 40, 3, 1, 1, 1, 8, 0)
     np.testing.assert_array_equal(np.round(np.array(g) * 8).astype(np.int64), counts.astype(np.int64))
+
+
+def test_lammps_binary_operators_match_the_numpy_trajectory(pa, tmp_path):
+    """Gofrt_lammps / MeanSquareDisplacement_lammps over a Traj window give
+    what Gofrt / MeanSquareDisplacement give over the in-memory Trajectory of
+    the same data (written with write_lammps_binary), for a calculation
+    inside the loaded window (device offset primo - window start) and for one
+    outside it (the window moves to primo first): counts bit for bit; MSD to
+    the reference's is_same (1e-10), since the device sums atoms in a slot
+    order taken from the first frame on the device, which differs between a
+    window and the whole trajectory."""
+    N, F = 300, 30
+    tilt = [1.1, -0.7, 0.5]
+    box_l = synth.lammps_box(13.0, tilt)
+    pos = synth.random_walk(41, N, F, box_l, 0.3)
+    vel = np.random.default_rng(2).normal(size=(F, N, 3))
+    types = synth.types_by_fraction(N, [0.6, 0.3, 0.1]).astype(np.int32)
+    tnp = pa.Trajectory(pos, vel, types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsTriclinic, False, False)
+    path = str(tmp_path / "traj.bin")
+    tnp.write_lammps_binary(path, 0, -1)
+    tl = pa.Traj(path)
+    tl.setAccessWindowSize(16)
+    tl.setAccessStart(4)
+    n, tmax = 8, 6
+    for form in (0, 1):   # FP64 and screened forms
+        g0 = pa.Gofrt(tnp, 0.0, 5.5, 40, tmax, 1, 1, 1, False, form)
+        g1 = pa.Gofrt_lammps(tl, 0.0, 5.5, 40, tmax, 1, 1, 1, False, form)
+        g0.reset(n)
+        g1.reset(n)
+        for primo in (6, 0, 14):   # inside [4, 20), then two moves of the window
+            g0.calculate(primo)
+            g1.calculate(primo)
+            assert np.array_equal(np.array(g0), np.array(g1)), (form, primo)
+            assert tl.get_current_timestep() <= primo
+    m0 = pa.MeanSquareDisplacement(tnp, 1, 5, 1, True, True)
+    m1 = pa.MeanSquareDisplacement_lammps(tl, 1, 5, 1, True, True)
+    m0.reset(9)
+    m1.reset(9)
+    for primo in (tl.get_current_timestep() + 1, 2):
+        m0.calculate(primo)
+        m1.calculate(primo)
+        assert is_same(np.array(m0), np.array(m1)).all(), primo
+    big = pa.MeanSquareDisplacement_lammps(tl, 1, 10, 1, False, False)
+    big.reset(10)
+    with pytest.raises(RuntimeError, match="not enough loaded timesteps"):
+        big.calculate(0)
+
+
+def test_greenkubo_file_matches_greenkubo_on_the_same_log(pa, gk_ref, tmp_path):
+    """GreenKubo_file over ReadLog_file (the text log, as the CLI's -l reads
+    it) equals GreenKubo over ReadLog (numpy) bit for bit, and both equal the
+    reference's test_gk.csv golden to is_same; a "#traj:" charge-current
+    column of a LAMMPS binary feeds GreenKubo_file like any other column."""
+    log, headers = gk_ref["log"], list(gk_ref["headers"])
+    p = str(tmp_path / "gk_integral.dat")
+    with open(p, "w") as f:
+        f.write("    ".join(headers) + "\n")
+        for row in log:
+            f.write(repr(float(row[0])) + "\t" + " ".join(repr(float(x)) for x in row[1:]) + "\n")
+    rf = pa.ReadLog_file(p, None, 1, 1, 200, [])
+    rn = pa.ReadLog(np.array(log, copy=True), headers)
+    args = ('', 1, ['c_flux[1]', 'c_vcm[1][1]'], False, 2000, 2, False, 0, 4, False, 1, 100)
+    g0, g1 = pa.GreenKubo(rn, *args), pa.GreenKubo_file(rf, *args)
+    for g in (g0, g1):
+        g.reset(rn.getNtimesteps() - 2000)
+        g.calculate(0)
+    m0, m1 = np.array(g0, copy=True), np.array(g1, copy=True)
+    assert np.array_equal(np.nan_to_num(m0, nan=0.5), np.nan_to_num(m1, nan=0.5))
+    assert is_same(m1, gk_ref["test_gk"]).all()
+
+    N, F = 40, 300
+    box_l = synth.lammps_box(9.0)
+    pos = synth.random_walk(6, N, F, box_l, 0.1)
+    vel = np.random.default_rng(1).normal(size=(F, N, 3))
+    types = synth.types_by_fraction(N, [0.5, 0.5]).astype(np.int32)
+    b = str(tmp_path / "t.bin")
+    pa.Trajectory(pos, vel, types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False) \
+        .write_lammps_binary(b, 0, -1)
+    lp = str(tmp_path / "log")
+    with open(lp, "w") as f:
+        f.write("Step c_a\n" + "".join("%d %r\n" % (k + 1, 0.5 * k) for k in range(F)))
+    head = "#traj:JZ 2 1.0 -1.0"
+    t = pa.Traj(b)
+    r = pa.ReadLog_file(lp, t, 1, 1, 200, [head])
+    r.calc_currents(t, 3)
+    J = r.get_data_copy()[:, 1:4]
+    gk = pa.GreenKubo_file(r, '', 1, [head], False, 50, 1, False, 0, 1)
+    gk.reset(F - 50)
+    gk.calculate(0)
+    c = np.array(gk, copy=True)
+    ref = np.array([np.mean(np.sum(J[:F - 50] * J[l:l + F - 50], axis=1)) / 3.0 for l in range(50)])
+    np.testing.assert_allclose(c[:, 0], ref, rtol=1e-12, atol=1e-15)
