@@ -125,9 +125,11 @@ void msd_free_prepared(void *prep);
 // carved by the caller); growing synchronises the device first.
 void *series_workspace(anl_series *s, size_t bytes);
 // FP64 Green-Kubo epilogue (mean subtraction, running integrals, Schur
-// complement) over d[leff][narr] whose first ncur^2 columns hold C(l).
+// complement) over d[leff][narr] whose first ncur^2 columns hold C(l); the
+// mean and the integrals as the reference's serial scans, or as parallel
+// sums (different rounding) when serial_scans is false.
 void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
-                 hipStream_t s, uint32_t nblocks, uint64_t stride);
+                 hipStream_t s, uint32_t nblocks, uint64_t stride, bool serial_scans);
 // Lag products of every block (split-BF16 MFMA) into d + b * leff * narr:
 // C(l) in the first ncur^2 columns, zeros elsewhere; scratch from the series.
 void gk_mfma_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,

@@ -100,13 +100,25 @@ __global__ __launch_bounds__(256) void gk_lag_kernel(const GkArgs a) {
 // columns, coalesced) while the lanes of wave 0 that own a column scan the
 // current tile in the reference's order.  Tiles double-buffer; one barrier
 // per tile.
+// Each thread issues 8 global loads before the first LDS write, so a tile
+// costs about one memory latency rather than one per element.
 __device__ __forceinline__ void gk_load_tile(double *__restrict__ dst, const double *__restrict__ out, uint32_t narr,
                                              uint32_t ncorr, uint32_t it0, uint32_t leff, uint32_t tl, uint32_t t,
                                              uint32_t nt) {
-    for (uint32_t e = t; e < tl * ncorr; e += nt) {
-        const uint32_t k = e / ncorr, jj = e - k * ncorr;
-        const uint32_t it = it0 + k;
-        dst[e] = it < leff ? out[(uint64_t)it * narr + jj] : 0.0;
+    constexpr uint32_t B = 8;
+    const uint32_t total = tl * ncorr;
+    for (uint32_t e0 = t; e0 < total; e0 += B * nt) {
+        double v[B];
+#pragma unroll
+        for (uint32_t j = 0; j < B; j++) {
+            const uint32_t e = e0 + j * nt;
+            const uint32_t k = e / ncorr, jj = e - k * ncorr;
+            const uint32_t it = it0 + k;
+            v[j] = (e < total && it < leff) ? out[(uint64_t)it * narr + jj] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < B; j++)
+            if (e0 + j * nt < total) dst[e0 + j * nt] = v[j];
     }
 }
 
@@ -187,23 +199,124 @@ __global__ __launch_bounds__(256) void gk_integral_kernel(double *__restrict__ o
                 }
             }
         } else if (t < ncorr && i < ntile) {
+            // batches of 16 lags: 16 LDS reads, one wait, the serial chain in
+            // registers, 16 paired LDS writes (a read-use-write per lag would
+            // wait out the LDS latency at every lag)
             const uint32_t it0 = 1 + i * tl, n = min(tl, leff - it0);
             const double *col = cin + buf * tl * ncorr + t;
             double *dst = res + buf * tl * nres + 2 * t;
-#pragma unroll 4
-            for (uint32_t k = 0; k < n; k++) {
-                const double v = col[k * ncorr];
+            constexpr uint32_t B = 16;
+            auto step = [&](double v, uint32_t k, double &Io, double &Eo) {
                 const double dit = (double)(it0 + k);
                 const double In = I + cprev / 2.0;
                 I = In + v / 2.0;
                 const double En = E + cprev * dit / 2.0;
                 E = En + v * dit / 2.0;
-                dst[k * nres] = I;
-                dst[k * nres + 1] = E;
                 cprev = v;
+                Io = I;
+                Eo = E;
+            };
+            uint32_t k0 = 0;
+            for (; k0 + B <= n; k0 += B) {
+                double v[B], Ib[B], Eb[B];
+#pragma unroll
+                for (uint32_t j = 0; j < B; j++) v[j] = col[(k0 + j) * ncorr];
+#pragma unroll
+                for (uint32_t j = 0; j < B; j++) step(v[j], k0 + j, Ib[j], Eb[j]);
+#pragma unroll
+                for (uint32_t j = 0; j < B; j++) {
+                    dst[(k0 + j) * nres] = Ib[j];
+                    dst[(k0 + j) * nres + 1] = Eb[j];
+                }
+            }
+            for (; k0 < n; k0++) {  // tail of the last tile
+                double Io, Eo;
+                step(col[k0 * ncorr], k0, Io, Eo);
+                dst[k0 * nres] = Io;
+                dst[k0 * nres + 1] = Eo;
             }
         }
         __syncthreads();
+    }
+}
+
+// Parallel forms of the two scans, for the MFMA form (whose C(l) already
+// differs from the FP64 one by ~1e-7 of its scale): one workgroup of 1024
+// threads per block, each thread a contiguous run of lags, partial sums
+// combined in a fixed order, so the result is deterministic but rounds
+// differently from the serial reference order above.
+constexpr uint32_t GK_PAR = 1024;
+
+__device__ __forceinline__ void block_inclusive_scan2(double *a, double *b, uint32_t t) {
+    for (uint32_t off = 1; off < GK_PAR; off <<= 1) {
+        const double xa = a[t], xb = b[t];
+        const double ya = t >= off ? a[t - off] : 0.0, yb = t >= off ? b[t - off] : 0.0;
+        __syncthreads();
+        a[t] = ya + xa;
+        b[t] = yb + xb;
+        __syncthreads();
+    }
+}
+
+// :259-277 mean of C over lags [start_mean, leff), subtracted from every lag
+__global__ __launch_bounds__(GK_PAR) void gk_mean_par_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr,
+                                                             uint32_t leff, uint32_t start_mean, uint64_t stride) {
+    __shared__ double red[GK_PAR];
+    if (start_mean >= leff) return;
+    out += blockIdx.y * stride;
+    const uint32_t t = threadIdx.x, n = leff - start_mean, per = (n + GK_PAR - 1) / GK_PAR;
+    const uint32_t k0 = start_mean + min(n, t * per), k1 = start_mean + min(n, (t + 1) * per);
+    for (uint32_t j = 0; j < ncorr; j++) {
+        double s = 0.0;
+        for (uint32_t k = k0; k < k1; k++) s += out[(uint64_t)k * narr + j];
+        red[t] = s;
+        __syncthreads();
+        for (uint32_t w = GK_PAR / 2; w > 0; w >>= 1) {
+            if (t < w) red[t] += red[t + w];
+            __syncthreads();
+        }
+        const double m = red[0] / (double)n;
+        __syncthreads();
+        for (uint32_t k = t; k < leff; k += GK_PAR) out[(uint64_t)k * narr + j] -= m;
+    }
+}
+
+// :279-288 I(it) = I(0) + sum_{k<=it} (C(k-1)/2 + C(k)/2), E likewise with
+// weights k/2, as two prefix sums over the lags
+__global__ __launch_bounds__(GK_PAR) void gk_integral_par_kernel(double *__restrict__ out, uint32_t narr,
+                                                                 uint32_t ncorr, uint32_t leff, uint64_t stride) {
+    __shared__ double sa[GK_PAR], sb[GK_PAR];
+    if (leff < 2) return;
+    out += blockIdx.y * stride;
+    const uint32_t t = threadIdx.x, n = leff - 1, per = (n + GK_PAR - 1) / GK_PAR;
+    const uint32_t k0 = 1 + min(n, t * per), k1 = 1 + min(n, (t + 1) * per);
+    for (uint32_t j = 0; j < ncorr; j++) {
+        double a = 0.0, b = 0.0;
+        double cprev = k0 < k1 ? out[(uint64_t)(k0 - 1) * narr + j] : 0.0;
+#pragma unroll 8
+        for (uint32_t k = k0; k < k1; k++) {
+            const double v = out[(uint64_t)k * narr + j], dk = (double)k;
+            a += cprev / 2.0 + v / 2.0;
+            b += cprev * dk / 2.0 + v * dk / 2.0;
+            cprev = v;
+        }
+        sa[t] = a;
+        sb[t] = b;
+        __syncthreads();
+        block_inclusive_scan2(sa, sb, t);
+        double I = out[ncorr + 2 * j] + (t > 0 ? sa[t - 1] : 0.0);
+        double E = out[ncorr + 2 * j + 1] + (t > 0 ? sb[t - 1] : 0.0);
+        cprev = k0 < k1 ? out[(uint64_t)(k0 - 1) * narr + j] : 0.0;
+#pragma unroll 8
+        for (uint32_t k = k0; k < k1; k++) {
+            const double v = out[(uint64_t)k * narr + j], dk = (double)k;
+            I += cprev / 2.0 + v / 2.0;
+            E += cprev * dk / 2.0 + v * dk / 2.0;
+            cprev = v;
+            out[(uint64_t)k * narr + ncorr + 2 * j] = I;
+            out[(uint64_t)k * narr + ncorr + 2 * j + 1] = E;
+        }
+        __syncthreads();  // sa/sb are reused by the next column
     }
 }
 
@@ -295,8 +408,21 @@ __global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_
 }
 
 void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
-                 hipStream_t s, uint32_t nblocks, uint64_t stride) {
+                 hipStream_t s, uint32_t nblocks, uint64_t stride, bool serial_scans) {
     const uint32_t ncorr = ncur * ncur;
+    if (!serial_scans) {
+        if (subtract_mean) {
+            hipLaunchKernelGGL(gk_mean_par_kernel, dim3(1, nblocks), dim3(GK_PAR), 0, s, d, narr, ncorr, leff,
+                               start_mean, stride);
+            ANL_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(gk_integral_par_kernel, dim3(1, nblocks), dim3(GK_PAR), 0, s, d, narr, ncorr, leff, stride);
+        ANL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256, nblocks), dim3(256), 0, s, d, narr, ncur, leff,
+                           stride);
+        ANL_HIP(hipGetLastError());
+        return;
+    }
     // lag tile of the scans: two buffers of tl x ncorr doubles, <= 64 KiB
     const uint32_t tl = std::max(16u, 4096u / ncorr);
     const size_t lds = (2 * (size_t)tl * ncorr + ncorr) * sizeof(double);
@@ -394,7 +520,8 @@ int anl_gk_calculate_blocks_async(anl_series *s, const uint32_t *idx_j, uint32_t
         hipStream_t st = (hipStream_t)stream;
         if (form == ANL_GK_MFMA) gk_mfma_enqueue_blocks(s, idx_j, ncur, skip, leff, n_seg, primo, n, nblocks, out_dev, st);
         else gk_fp64_enqueue_blocks(s, idx_j, ncur, skip, leff, n_seg, primo, n, nblocks, out_dev, st);
-        gk_epilogue(out_dev, narr, ncur, leff, subtract_mean, start_mean, st, nblocks, stride);
+        // the FP64 form keeps the reference's serial scans (bit-identical results)
+        gk_epilogue(out_dev, narr, ncur, leff, subtract_mean, start_mean, st, nblocks, stride, form == ANL_GK_FP64);
         if (mean_dev) mediavar_blocks(nullptr, out_dev, nullptr, nblocks, stride, mean_dev, var_dev, st);
     });
 }

@@ -16,11 +16,13 @@
 // is the one term whose sign is fixed, at lag 0 of a == b (lo^2): the tile
 // holding lag 0 adds it as a fourth product.
 //
-// Layouts in HBM (bf16 bit patterns, u16):
-//   X[cx][r][p]          = X_cx(32p + r),          p < PP        (cx = 3c + x)
-//   Y[c8][cx][s][p']     = Y_cx(32(p' + c8) + s),  p' < YP, c8 < 8
+// Layouts in HBM (bf16 bit patterns, u16), in k-blocks of 8:
+//   X[cx][p/8][r][p%8]        = X_cx(32p + r),          p < PP      (cx = 3c + x)
+//   Y[c8][cx][p'/8][s][p'%8]  = Y_cx(32(p' + c8) + s),  p' < YP, c8 < 8
 // so that the 8 k-values one lane feeds to an MFMA are one aligned 16-byte
-// load for every tile offset q (the eight c8 copies absorb q mod 8).
+// load for every tile offset q (the eight c8 copies absorb q mod 8), and the
+// 32 rows of one k-block are 512 contiguous bytes: a wavefront's operand load
+// touches 8 whole cache lines.
 // A wavefront owns tiles q0 + 16j (j < GKM_Q) of one current pair and one K
 // chunk: tile q0 + 16j at step st reads the same Y vector as tile q0 at step
 // st + j, so Y operands live in a rolling register window and each step loads
@@ -37,6 +39,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t GKM_Q = 8;      // tiles per wavefront, 16 apart
 constexpr uint32_t GKM_CS = 16;    // MFMA steps (16 rows of 32 t0 each) per FP32 chunk
+constexpr uint32_t GKM_DY = 2;     // Y operands loaded this many steps before first use
+constexpr uint32_t GKM_DX = 1;     // X operands likewise
 constexpr uint32_t GKM_MAXB = 64;  // blocks per batch of launches
 constexpr uint32_t GKM_SP = 32;    // rows p of one split workgroup
 
@@ -130,8 +134,9 @@ __global__ __launch_bounds__(256) void gkm_split_kernel(const GkmBatch a, uint32
     unsigned short *dl = (unsigned short *)(a.work + a.base[b] + (YSIDE ? off.yl : off.xl));
     const uint32_t nrun = NSH * ncx * 32 * (GKM_SP / 8);
     for (uint32_t e = t; e < nrun; e += blockDim.x) {
-        const uint32_t k = e % (GKM_SP / 8), row = e / (GKM_SP / 8);  // row = (c8 * ncx + cx) * 32 + s
-        const uint32_t c8 = row / (ncx * 32), cs = row - c8 * ncx * 32;
+        // consecutive threads: consecutive rows s of one k-block (contiguous)
+        const uint32_t sr = e % 32, k = (e / 32) % (GKM_SP / 8), rb = e / (32 * (GKM_SP / 8));  // rb = c8 * ncx + cx
+        const uint32_t c8 = rb / ncx, cs = (rb - c8 * ncx) * 32 + sr;
         const uint32_t p = P0 + 8 * k;
         if (p >= P) continue;  // P is a multiple of 8
         const unsigned short *sa = sh + cs * UP + 8 * k + c8, *sb = shl + cs * UP + 8 * k + c8;
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void gkm_split_kernel(const GkmBatch a, uint32
         wl.y = sb[2] | (uint32_t)sb[3] << 16;
         wl.z = sb[4] | (uint32_t)sb[5] << 16;
         wl.w = sb[6] | (uint32_t)sb[7] << 16;
-        const uint64_t o = (uint64_t)row * P + p;
+        const uint64_t o = (((uint64_t)rb * (P / 8) + p / 8) * 32 + sr) * 8;
         *reinterpret_cast<uint4 *>(dh + o) = wh;
         *reinterpret_cast<uint4 *>(dl + o) = wl;
     }
@@ -154,13 +159,102 @@ __device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
 }
 
+constexpr uint32_t GKM_WG = 4;  // wavefronts per workgroup: consecutive tile groups sharing X
+
+// The K loop of one wavefront: 3 components x GKM_CS steps of GKM_Q tiles,
+// three split products each (four on tile 0 when LAG0).  The workgroup's
+// wavefronts share one X operand block per component: it is copied into LDS
+// (xs: [hi/lo][step][lane] 16-byte vectors, contiguous in HBM) while the Y
+// window preloads, and read from LDS one step ahead.
+template <bool LAG0>
+__device__ __forceinline__ void gkm_accumulate(f32x16 (&acc)[GKM_Q], uint4 *xs, const unsigned short *__restrict__ xh,
+                                               const unsigned short *__restrict__ xl,
+                                               const unsigned short *__restrict__ yh,
+                                               const unsigned short *__restrict__ yl, uint32_t PP, uint32_t YP,
+                                               uint32_t ca, uint32_t cb, uint32_t ncx, uint32_t r, uint32_t h,
+                                               uint32_t c8, uint32_t qoff, uint32_t st0) {
+    const uint32_t lane = h * 32 + r, tid = threadIdx.x;
+    constexpr uint32_t XN = GKM_CS * 64;  // vectors of one X block (hi or lo)
+    for (uint32_t x = 0; x < 3; x++) {
+        // k-block of step st: 2 st + h (X), 2 st + qoff / 8 + h (Y); 512 elements per step
+        const uint64_t xb = ((uint64_t)(3 * ca + x) * (PP / 8) + 2 * st0) * 32 * 8;
+        const uint64_t yo = ((((uint64_t)c8 * ncx + 3 * cb + x) * (YP / 8) + qoff / 8 + h) * 32 + r) * 8;
+        const unsigned short *YH = yh + yo, *YL = yl + yo;
+        // Y(v) (the operand vector of step v) lives in window slot v % YW and
+        // is loaded GKM_DY steps before its first use (by tile GKM_Q - 1);
+        // X(v) in register slot v % XW, read from LDS GKM_DX steps ahead
+        constexpr uint32_t YW = GKM_Q + GKM_DY, XW = GKM_DX + 1;
+        bf16x8 wh[YW], wl[YW], xhv[XW], xlv[XW];
+#pragma unroll
+        for (uint32_t v = 0; v + 1 < YW; v++) {
+            wh[v] = ld8(YH + 512 * (st0 + v));
+            wl[v] = ld8(YL + 512 * (st0 + v));
+        }
+        __syncthreads();  // every wavefront is done with the previous X block
+        {
+            const uint4 *XHb = reinterpret_cast<const uint4 *>(xh + xb), *XLb = reinterpret_cast<const uint4 *>(xl + xb);
+#pragma unroll
+            for (uint32_t k = 0; k < XN / (64 * GKM_WG); k++) {
+                const uint4 vh = XHb[tid + 64 * GKM_WG * k], vl = XLb[tid + 64 * GKM_WG * k];
+                xs[tid + 64 * GKM_WG * k] = vh;
+                xs[XN + tid + 64 * GKM_WG * k] = vl;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t v = 0; v + 1 < XW; v++) {
+            xhv[v] = __builtin_bit_cast(bf16x8, xs[v * 64 + lane]);
+            xlv[v] = __builtin_bit_cast(bf16x8, xs[XN + v * 64 + lane]);
+        }
+#pragma unroll
+        for (uint32_t sv = 0; sv < GKM_CS; sv++) {  // unrolled: every slot index is a constant
+            const uint32_t sn = st0 + sv;
+            if (sv + YW - 1 < GKM_CS + GKM_Q - 1) {  // Y rows up to step GKM_CS + GKM_Q - 2 are used
+                wh[(sv + YW - 1) % YW] = ld8(YH + 512 * (sn + YW - 1));
+                wl[(sv + YW - 1) % YW] = ld8(YL + 512 * (sn + YW - 1));
+            }
+            if (sv + XW - 1 < GKM_CS) {
+                xhv[(sv + XW - 1) % XW] = __builtin_bit_cast(bf16x8, xs[(sv + XW - 1) * 64 + lane]);
+                xlv[(sv + XW - 1) % XW] = __builtin_bit_cast(bf16x8, xs[XN + (sv + XW - 1) * 64 + lane]);
+            }
+            // keep this step's loads ahead of its MFMAs (the scheduler would
+            // otherwise sink them next to their first use)
+            __builtin_amdgcn_sched_barrier(0);
+            const bf16x8 ah = xhv[sv % XW], al = xlv[sv % XW];
+#pragma unroll
+            for (uint32_t j = 0; j < GKM_Q; j++) {
+                const uint32_t w = (sv + j) % YW;
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wh[w], acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wl[w], acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wh[w], acc[j], 0, 0, 0);
+                // tile 0 holds lag 0, where lo*lo' = lo^2 >= 0 for a == b
+                // does not average out over t0: add it there
+                if (LAG0 && j == 0)
+                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wl[w], acc[0], 0, 0, 0);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Operand layouts of v_mfma_f32_32x32x16_bf16: lane l feeds row l % 32 of A
 // and column l % 32 of B, each with the 8 k-values of slot l / 32 (the same k
 // slots in A and B); result item e of lane l is row 8(e/4) + 4(l/32) + e%4,
-// column l % 32.  Grid: x = tile group, y = current pair, z = (block, chunk).
-__global__ __launch_bounds__(64, 2) void gkm_tile_kernel(const GkmBatch a) {
-    __shared__ double tile[32][33];
-    const uint32_t g = blockIdx.x, pair = blockIdx.y, z = blockIdx.z;
+// column l % 32.  Grid: x = GKM_WG consecutive tile groups (one per
+// wavefront), y = current pair, z = (block, chunk).  The 63 diagonal sums of
+// a tile are pairwise FP64 sums in a fixed order, so results do not depend
+// on the launch.
+__global__ __launch_bounds__(64 * GKM_WG, 2) void gkm_tile_kernel(const GkmBatch a) {
+    // X blocks during the K loop, then one 32 x 33 FP64 tile per wavefront
+    __shared__ double smem[GKM_WG * 32 * 33];
+    static_assert(sizeof(smem) >= 2 * GKM_CS * 64 * sizeof(uint4), "X block does not fit");
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * GKM_WG + wv, pair = blockIdx.y, z = blockIdx.z;
     uint32_t b = 0;
     while (z >= a.z0[b + 1]) b++;  // uniform
     const uint32_t chunk = z - a.z0[b], nchunks = a.z0[b + 1] - a.z0[b];
@@ -171,50 +265,24 @@ __global__ __launch_bounds__(64, 2) void gkm_tile_kernel(const GkmBatch a) {
     const unsigned short *yh = (const unsigned short *)(reg + off.yh), *yl = (const unsigned short *)(reg + off.yl);
     const uint32_t qbase = (g / 16) * 16 * GKM_Q, q0 = g % 16;
     const uint32_t ca = pair / a.ncur, cb = pair % a.ncur, ncx = a.ncx;
-    const uint32_t lane = threadIdx.x, r = lane & 31u, h = lane >> 5;
+    const uint32_t lane = threadIdx.x & 63u, r = lane & 31u, h = lane >> 5;
     const uint32_t c8 = q0 & 7u, qoff = qbase + (q0 & ~7u);
     const uint32_t st0 = chunk * GKM_CS;
+    uint4 *xs = reinterpret_cast<uint4 *>(smem);
     f32x16 acc[GKM_Q];
 #pragma unroll
     for (uint32_t j = 0; j < GKM_Q; j++) acc[j] = f32x16{};
-    for (uint32_t x = 0; x < 3; x++) {
-        const uint64_t xo = ((uint64_t)(3 * ca + x) * 32 + r) * PP + 8 * h;
-        const uint64_t yo = (((uint64_t)c8 * ncx + 3 * cb + x) * 32 + r) * YP + qoff + 8 * h;
-        const unsigned short *XH = xh + xo, *XL = xl + xo, *YH = yh + yo, *YL = yl + yo;
-        bf16x8 wh[GKM_Q], wl[GKM_Q];
-#pragma unroll
-        for (uint32_t j = 0; j + 1 < GKM_Q; j++) {
-            wh[j] = ld8(YH + 16 * (st0 + j));
-            wl[j] = ld8(YL + 16 * (st0 + j));
-        }
-        // X operands of the next step are loaded while this step's MFMAs run
-        bf16x8 ahc = ld8(XH + 16 * st0), alc = ld8(XL + 16 * st0);
-        for (uint32_t st = st0; st < st0 + GKM_CS; st += GKM_Q) {
-#pragma unroll
-            for (uint32_t u = 0; u < GKM_Q; u++) {
-                const uint32_t sn = st + u;
-                wh[(u + GKM_Q - 1) % GKM_Q] = ld8(YH + 16 * (sn + GKM_Q - 1));
-                wl[(u + GKM_Q - 1) % GKM_Q] = ld8(YL + 16 * (sn + GKM_Q - 1));
-                const bf16x8 ah = ahc, al = alc;
-                if (u + 1 < GKM_Q || st + GKM_Q < st0 + GKM_CS) {  // next step inside this chunk
-                    ahc = ld8(XH + 16 * (sn + 1));
-                    alc = ld8(XL + 16 * (sn + 1));
-                }
-#pragma unroll
-                for (uint32_t j = 0; j < GKM_Q; j++) {
-                    const uint32_t w = (u + j) % GKM_Q;
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wh[w], acc[j], 0, 0, 0);
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wl[w], acc[j], 0, 0, 0);
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wh[w], acc[j], 0, 0, 0);
-                    // tile 0 holds lag 0, where lo*lo' = lo^2 >= 0 for a == b
-                    // does not average out over t0: add it there
-                    if (j == 0 && g == 0)
-                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wl[w], acc[0], 0, 0, 0);
-                }
-            }
-        }
-    }
-    // diagonal sums of each tile, in FP64, into this chunk's partial
+    // the wavefront holding tile 0 (lag 0) adds the fourth product; the branch
+    // sits outside the MFMA loop so that its loads schedule freely (both
+    // paths pass the same workgroup barriers)
+    if (g == 0)
+        gkm_accumulate<true>(acc, xs, xh, xl, yh, yl, PP, YP, ca, cb, ncx, r, h, c8, qoff, st0);
+    else
+        gkm_accumulate<false>(acc, xs, xh, xl, yh, yl, PP, YP, ca, cb, ncx, r, h, c8, qoff, st0);
+    __syncthreads();  // X blocks are dead: the tiles below reuse their LDS
+    // diagonal sums of each tile, in FP64, into this chunk's partial; each
+    // wavefront works in its own tile, so only wavefront-level syncs
+    double(*tile)[33] = reinterpret_cast<double(*)[33]>(smem + wv * 32 * 33);
     double *part = a.partial + a.poff[b];
 #pragma unroll
     for (uint32_t j = 0; j < GKM_Q; j++) {
@@ -222,15 +290,25 @@ __global__ __launch_bounds__(64, 2) void gkm_tile_kernel(const GkmBatch a) {
         if (q >= a.ntiles) break;  // uniform across the wavefront
 #pragma unroll
         for (int e = 0; e < 16; e++) tile[8 * (e / 4) + 4 * h + (e % 4)][r] = (double)acc[j][e];
-        __syncthreads();
+        wave_sync_lds();
         if (lane < 63) {
-            const int d = (int)lane - 31;  // s - r
-            double sum = 0.0;
-            const int r0 = d < 0 ? -d : 0, r1 = d < 0 ? 32 : 32 - d;
-            for (int rr = r0; rr < r1; rr++) sum += tile[rr][rr + d];
-            part[(((uint64_t)pair * a.ntiles + q) * nchunks + chunk) * 63 + lane] = sum;
+            // diagonal d = s - r of this lane: all 32 rows read at once (rows
+            // off the tile read a clamped cell and add 0), pairwise sum
+            const int d = (int)lane - 31;
+            double x[32];
+#pragma unroll
+            for (int rr = 0; rr < 32; rr++) {
+                const int c = rr + d;
+                const double v = tile[rr][min(max(c, 0), 31)];
+                x[rr] = (c >= 0 && c < 32) ? v : 0.0;
+            }
+#pragma unroll
+            for (int w = 16; w > 0; w >>= 1)
+#pragma unroll
+                for (int i = 0; i < w; i++) x[i] += x[i + w];
+            part[(((uint64_t)pair * a.ntiles + q) * nchunks + chunk) * 63 + lane] = x[0];
         }
-        __syncthreads();
+        wave_sync_lds();
     }
 }
 
@@ -373,7 +451,7 @@ void gkm_enqueue_batch(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uin
     for (uint32_t iseg = 0; iseg < n_seg; iseg++) {
         hipLaunchKernelGGL(gkm_split_kernel<false>, dim3(maxPP / GKM_SP, 1, a.nblk), dim3(256), lds_x, s, a, iseg);
         ANL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gkm_tile_kernel, dim3(ngroups, ncorr, a.z0[a.nblk]), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(gkm_tile_kernel, dim3(ngroups / GKM_WG, ncorr, a.z0[a.nblk]), dim3(64 * GKM_WG), 0, s, a);
         ANL_HIP(hipGetLastError());
         hipLaunchKernelGGL(gkm_reduce_kernel, rgrid, dim3(256), 0, s, a, iseg);
         ANL_HIP(hipGetLastError());
