@@ -242,21 +242,10 @@ __global__ __launch_bounds__(256) void gk_integral_kernel(double *__restrict__ o
 
 // Parallel forms of the two scans, for the MFMA form (whose C(l) already
 // differs from the FP64 one by ~1e-7 of its scale): one workgroup of 1024
-// threads per block, each thread a contiguous run of lags, partial sums
-// combined in a fixed order, so the result is deterministic but rounds
-// differently from the serial reference order above.
+// threads per block, partial sums combined in a fixed order, so the result
+// is deterministic but rounds differently from the serial reference order
+// above.
 constexpr uint32_t GK_PAR = 1024;
-
-__device__ __forceinline__ void block_inclusive_scan2(double *a, double *b, uint32_t t) {
-    for (uint32_t off = 1; off < GK_PAR; off <<= 1) {
-        const double xa = a[t], xb = b[t];
-        const double ya = t >= off ? a[t - off] : 0.0, yb = t >= off ? b[t - off] : 0.0;
-        __syncthreads();
-        a[t] = ya + xa;
-        b[t] = yb + xb;
-        __syncthreads();
-    }
-}
 
 // :259-277 mean of C over lags [start_mean, leff), subtracted from every lag
 __global__ __launch_bounds__(GK_PAR) void gk_mean_par_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncorr,
@@ -264,11 +253,10 @@ __global__ __launch_bounds__(GK_PAR) void gk_mean_par_kernel(double *__restrict_
     __shared__ double red[GK_PAR];
     if (start_mean >= leff) return;
     out += blockIdx.y * stride;
-    const uint32_t t = threadIdx.x, n = leff - start_mean, per = (n + GK_PAR - 1) / GK_PAR;
-    const uint32_t k0 = start_mean + min(n, t * per), k1 = start_mean + min(n, (t + 1) * per);
+    const uint32_t t = threadIdx.x, n = leff - start_mean;
     for (uint32_t j = 0; j < ncorr; j++) {
         double s = 0.0;
-        for (uint32_t k = k0; k < k1; k++) s += out[(uint64_t)k * narr + j];
+        for (uint32_t k = start_mean + t; k < leff; k += GK_PAR) s += out[(uint64_t)k * narr + j];  // coalesced
         red[t] = s;
         __syncthreads();
         for (uint32_t w = GK_PAR / 2; w > 0; w >>= 1) {
@@ -281,42 +269,111 @@ __global__ __launch_bounds__(GK_PAR) void gk_mean_par_kernel(double *__restrict_
     }
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront (Hillis-Steele,
+// fixed order).
+__device__ __forceinline__ double wave_inclusive_scan(double v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const double u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
 // :279-288 I(it) = I(0) + sum_{k<=it} (C(k-1)/2 + C(k)/2), E likewise with
-// weights k/2, as two prefix sums over the lags
+// weights k/2, as prefix sums over the lags.  Each of the 16 wavefronts owns
+// a contiguous range of lags and reads it 64 consecutive lags at a time, all
+// NC columns of a lag from one lane (coalesced): pass 1 sums its range, the
+// ranges' offsets come from an exclusive scan of those sums, pass 2 scans
+// each 64-lag batch across the lanes and carries between batches.
+template <int NC>
 __global__ __launch_bounds__(GK_PAR) void gk_integral_par_kernel(double *__restrict__ out, uint32_t narr,
-                                                                 uint32_t ncorr, uint32_t leff, uint64_t stride) {
-    __shared__ double sa[GK_PAR], sb[GK_PAR];
+                                                                 uint32_t leff, uint64_t stride) {
+    constexpr uint32_t NW = GK_PAR / 64;
+    __shared__ double tot[2 * NC][NW];
     if (leff < 2) return;
     out += blockIdx.y * stride;
-    const uint32_t t = threadIdx.x, n = leff - 1, per = (n + GK_PAR - 1) / GK_PAR;
-    const uint32_t k0 = 1 + min(n, t * per), k1 = 1 + min(n, (t + 1) * per);
-    for (uint32_t j = 0; j < ncorr; j++) {
-        double a = 0.0, b = 0.0;
-        double cprev = k0 < k1 ? out[(uint64_t)(k0 - 1) * narr + j] : 0.0;
-#pragma unroll 8
-        for (uint32_t k = k0; k < k1; k++) {
-            const double v = out[(uint64_t)k * narr + j], dk = (double)k;
-            a += cprev / 2.0 + v / 2.0;
-            b += cprev * dk / 2.0 + v * dk / 2.0;
-            cprev = v;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t n = leff - 1, per = ((n + NW - 1) / NW + 63) / 64 * 64;
+    const uint32_t k0 = 1 + min(n, wv * per), k1 = 1 + min(n, (wv + 1) * per);
+    // the two trapezoid terms of lag k for every column
+    auto terms = [&](uint32_t k, double (&a)[NC], double (&b)[NC]) {
+        const double *rp = out + (uint64_t)(k - 1) * narr, *rv = out + (uint64_t)k * narr;
+        const double dk = (double)k;
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            const double cp = rp[j], v = rv[j];
+            a[j] = cp / 2.0 + v / 2.0;
+            b[j] = cp * dk / 2.0 + v * dk / 2.0;
         }
-        sa[t] = a;
-        sb[t] = b;
-        __syncthreads();
-        block_inclusive_scan2(sa, sb, t);
-        double I = out[ncorr + 2 * j] + (t > 0 ? sa[t - 1] : 0.0);
-        double E = out[ncorr + 2 * j + 1] + (t > 0 ? sb[t - 1] : 0.0);
-        cprev = k0 < k1 ? out[(uint64_t)(k0 - 1) * narr + j] : 0.0;
-#pragma unroll 8
-        for (uint32_t k = k0; k < k1; k++) {
-            const double v = out[(uint64_t)k * narr + j], dk = (double)k;
-            I += cprev / 2.0 + v / 2.0;
-            E += cprev * dk / 2.0 + v * dk / 2.0;
-            cprev = v;
-            out[(uint64_t)k * narr + ncorr + 2 * j] = I;
-            out[(uint64_t)k * narr + ncorr + 2 * j + 1] = E;
+    };
+    double sa[NC], sb[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) sa[j] = sb[j] = 0.0;
+    for (uint32_t kb = k0; kb < k1; kb += 64) {
+        const uint32_t k = kb + lane;
+        if (k < k1) {
+            double a[NC], b[NC];
+            terms(k, a, b);
+#pragma unroll
+            for (int j = 0; j < NC; j++) {
+                sa[j] += a[j];
+                sb[j] += b[j];
+            }
         }
-        __syncthreads();  // sa/sb are reused by the next column
+    }
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        const double ta = wave_inclusive_scan(sa[j]), tb = wave_inclusive_scan(sb[j]);
+        if (lane == 63) {
+            tot[2 * j][wv] = ta;
+            tot[2 * j + 1][wv] = tb;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * NC) {  // exclusive scan of the ranges' sums, in range order
+        double run = 0.0;
+        for (uint32_t w = 0; w < NW; w++) {
+            const double t = tot[threadIdx.x][w];
+            tot[threadIdx.x][w] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    double ci[NC], ce[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        ci[j] = out[NC + 2 * j] + tot[2 * j][wv];
+        ce[j] = out[NC + 2 * j + 1] + tot[2 * j + 1][wv];
+    }
+    for (uint32_t kb = k0; kb < k1; kb += 64) {
+        const uint32_t k = kb + lane;
+        double a[NC], b[NC];
+        if (k < k1) {
+            terms(k, a, b);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NC; j++) a[j] = b[j] = 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            a[j] = wave_inclusive_scan(a[j]);
+            b[j] = wave_inclusive_scan(b[j]);
+        }
+        if (k < k1) {
+            double *row = out + (uint64_t)k * narr + NC;
+#pragma unroll
+            for (int j = 0; j < NC; j++) {
+                row[2 * j] = ci[j] + a[j];
+                row[2 * j + 1] = ce[j] + b[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            ci[j] += __shfl(a[j], 63, 64);
+            ce[j] += __shfl(b[j], 63, 64);
+        }
     }
 }
 
@@ -416,7 +473,19 @@ void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int sub
                                start_mean, stride);
             ANL_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(gk_integral_par_kernel, dim3(1, nblocks), dim3(GK_PAR), 0, s, d, narr, ncorr, leff, stride);
+        switch (ncorr) {
+#define ANL_GK_PAR(NCV)                                                                                          \
+    case NCV:                                                                                                    \
+        hipLaunchKernelGGL(gk_integral_par_kernel<NCV>, dim3(1, nblocks), dim3(GK_PAR), 0, s, d, narr, leff, stride); \
+        break;
+            ANL_GK_PAR(1)
+            ANL_GK_PAR(4)
+            ANL_GK_PAR(9)
+            ANL_GK_PAR(16)
+#undef ANL_GK_PAR
+            default:
+                fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
+        }
         ANL_HIP(hipGetLastError());
         hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256, nblocks), dim3(256), 0, s, d, narr, ncur, leff,
                            stride);
