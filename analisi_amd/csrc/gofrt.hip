@@ -666,6 +666,12 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     clustered = true;
                     ncl += lane == 0 ? 1 : 0;
                     const float thc = cluster_threshold(rho, a.rmaxf, a.dq);
+                    // j with |q| > rho + rmax + dq is farther than rmax + dq from
+                    // every active i (|q - p| >= |q| - |p|), so none of its pairs
+                    // can be counted; the bound carries a margin for the FP32
+                    // rounding of |q|^2 and rho
+                    const float rs = (rho + a.rmaxf + a.dq) * (1.0f + 0x1p-12f) + 1e-3f;
+                    const float rs2 = rs * rs;
                     // rows: (-2p, |p|^2, 1) for active lanes, (0, 0, 0, 60000, 1) otherwise
                     f16x8_t row = {};
                     if (active) {
@@ -715,6 +721,11 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             qz = a.hz * gc;
                         }
                         const float q2 = fmaf(qz, qz, fmaf(qy, qy, qx * qx));
+                        // 32-column halves with no j near the cluster skip their
+                        // MFMAs and compares (most of them: the j slots are in k-d
+                        // order, so a half is a compact group of atoms too)
+                        const unsigned long long nearm = __ballot(jin && q2 <= rs2);
+                        if (nearm == 0ull) continue;
                         // column (q, 1, |q|^2); past N: (0, 0, 0, 1, 60000)
                         f16x8_t col = {};
                         col[0] = (_Float16)(jin ? qx : 0.0f);
@@ -733,6 +744,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                                __builtin_bit_cast(f16x8_t, colane ? cdn : z4)};
 #pragma unroll
                         for (int jb = 0; jb < 2; jb++) {
+                            if ((uint32_t)(nearm >> (32 * jb)) == 0u) continue;  // uniform
                             // entry = j << 8 | local i; item e of tile T is row
                             // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
                             const uint32_t eb = ((j0 + 32u * (uint32_t)jb + (lane & 31u)) << 8) | (wbase + half4);
