@@ -120,6 +120,8 @@ class Trajectory {
         try {
             lammps_bin::Header h0;
             const size_t size0 = read_step(0, h0, nullptr);
+            if (h0.natoms <= 0 || h0.natoms > (int64_t)INT32_MAX)
+                throw std::runtime_error("Error: invalid number of atoms in the first timestep");
             natoms = (size_t)h0.natoms;
             init_types();
             n_timesteps = fsize / size0;
@@ -265,7 +267,7 @@ class Trajectory {
 
     // Window offset of a calculation on timesteps [primo, primo + need): the
     // device mirror holds the loaded window only, so the window is moved to
-    // primo when it does not cover the range and is large enough.
+    // cover the range when it does not and is large enough.
     size_t window_offset(size_t primo, size_t need) {
         if (!(dati_caricati && primo >= current_timestep && primo + need <= current_timestep + loaded_timesteps)) {
             if (need > loaded_timesteps) {
@@ -274,7 +276,11 @@ class Trajectory {
                    << " timesteps to do the requested calculation";
                 throw std::runtime_error(ss.str());
             }
-            if (set_access_at(primo) != Ok) throw std::runtime_error("Error loading the file\n");
+            // the window starts at primo, or earlier when it would run past
+            // the last timestep
+            size_t start = primo;
+            if (n_timesteps >= loaded_timesteps) start = std::min(primo, n_timesteps - loaded_timesteps);
+            if (set_access_at(start) != Ok) throw std::runtime_error("Error loading the file\n");
         }
         return primo - current_timestep;
     }

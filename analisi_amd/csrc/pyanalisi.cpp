@@ -182,6 +182,35 @@ void define_gk(py::module &m, const std::string &suffix) {
         .def_buffer([](G &g) { return vbuffer(g); });
 }
 
+// BlockAverageG + MediaVar of Gofrt / MSD (blockaverage.h:194-203), the
+// CLI's -g and -Q paths: (mean, variance) in the calculation's layout.  Over
+// a Traj, each block's calculation moves the window to the block (the window
+// must hold a block and its lags).
+template <class T>
+void define_block_averages(py::module &m) {
+    using G = Gofrt<double, T>;
+    using M = MSD<T>;
+    m.def("gofrt_block_average",
+          [](T &t, unsigned int nblocks, double rmin, double rmax, unsigned int nbin, unsigned int tmax,
+             unsigned int skip, unsigned int every) {
+              BlockAverageG<T, G, double, double, unsigned int, unsigned int, unsigned int, unsigned int, unsigned int,
+                            bool, int>
+                  ba(&t, nblocks);
+              ba.calculate(rmin, rmax, nbin, tmax, 1u, skip, every, false, (int)ANL_GOFRT_AUTO);
+              auto shape = ba.media()->get_shape();
+              return py::make_tuple(copy_array(ba.media()->access_vdata(), shape),
+                                    copy_array(ba.varianza()->access_vdata(), shape));
+          });
+    m.def("msd_block_average", [](T &t, unsigned int nblocks, unsigned int skip, unsigned int tmax, bool cm,
+                                  bool cm_self) {
+        BlockAverageG<T, M, unsigned int, unsigned int, unsigned int, bool, bool, bool> ba(&t, nblocks);
+        ba.calculate(skip, tmax, 1u, cm, cm_self, false);
+        auto shape = ba.media()->get_shape();
+        std::vector<ssize_t> sh(shape.begin(), shape.end());
+        return py::make_tuple(copy_array(ba.media()->access_vdata(), sh), copy_array(ba.varianza()->access_vdata(), sh));
+    });
+}
+
 }  // namespace
 
 using GofrtT = Gofrt<double, Trajectory_numpy>;
@@ -283,27 +312,8 @@ PYBIND11_MODULE(_pyanalisi, m) {
           py::arg("subtract_mean"), py::arg("start_mean"), py::arg("mfma") = false);
     // BlockAverageG + MediaVar of Gofrt / MSD (blockaverage.h:194-203), the
     // CLI's -g and -Q paths: returns (mean, variance) in the calculation's layout.
-    m.def("gofrt_block_average",
-          [](Trajectory_numpy &t, unsigned int nblocks, double rmin, double rmax, unsigned int nbin, unsigned int tmax,
-             unsigned int skip, unsigned int every) {
-              BlockAverageG<Trajectory_numpy, GofrtT, double, double, unsigned int, unsigned int, unsigned int,
-                            unsigned int, unsigned int, bool, int>
-                  ba(&t, nblocks);
-              ba.calculate(rmin, rmax, nbin, tmax, 1u, skip, every, false, (int)ANL_GOFRT_AUTO);
-              auto shape = ba.media()->get_shape();
-              return py::make_tuple(copy_array(ba.media()->access_vdata(), shape),
-                                    copy_array(ba.varianza()->access_vdata(), shape));
-          });
-    m.def("msd_block_average",
-          [](Trajectory_numpy &t, unsigned int nblocks, unsigned int skip, unsigned int tmax, bool cm, bool cm_self) {
-              BlockAverageG<Trajectory_numpy, MSDT, unsigned int, unsigned int, unsigned int, bool, bool, bool> ba(
-                  &t, nblocks);
-              ba.calculate(skip, tmax, 1u, cm, cm_self, false);
-              auto shape = ba.media()->get_shape();
-              std::vector<ssize_t> sh(shape.begin(), shape.end());
-              return py::make_tuple(copy_array(ba.media()->access_vdata(), sh),
-                                    copy_array(ba.varianza()->access_vdata(), sh));
-          });
+    define_block_averages<Trajectory_numpy>(m);
+    define_block_averages<Trajectory>(m);
 
     m.def("info", []() { return std::string(anl_info()); });
     m.def("has_mmap", []() { return true; });

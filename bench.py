@@ -316,9 +316,12 @@ def run_gofrt(w, args, hip, res_extra=None):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic_from_profile("gofrt_screened", w["name"]),
-                     "note": "VALU-bound pair kernel (no MFMA work): %d algorithmic FLOP per pair-lag update "
-                             "(3 sub, 6 min image, %s5 d2, 2 gates) over the event-timed kernel; peak = MI355X "
-                             "vector FP32 157.3 TFLOP/s" % (fl, "3 tilt, " if w["triclinic"] else "")},
+                     "note": "%d algorithmic FLOP per pair-lag update (3 sub, 6 min image, %s5 d2, 2 gates: the "
+                             "reference's brute-force evaluation of every pair) over the event-timed kernel; peak = "
+                             "MI355X vector FP32 157.3 TFLOP/s.  The kernel does not execute that work: pairs are "
+                             "screened with 16-bit and FP16-MFMA bounds and far column halves are skipped, so this is "
+                             "an effective rate and can exceed 1 on sparse workloads; the executed instruction mix is "
+                             "in profiles/gofrt_c2_kernel_trace.md" % (fl, "3 tilt, " if w["triclinic"] else "")},
         "fp64_plain": {"value": rate64, "unit": "pair-lag updates/s",
                        "sample": "same workload, reset(%d): %d t0 x %d lags" % (n_sl, n_sl, leff_sl), "ms": ms64,
                        "roofline_frac": upd64 * fl / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,

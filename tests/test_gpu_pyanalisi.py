@@ -265,3 +265,26 @@ def test_greenkubo_file_matches_greenkubo_on_the_same_log(pa, gk_ref, tmp_path):
     c = np.array(gk, copy=True)
     ref = np.array([np.mean(np.sum(J[:F - 50] * J[l:l + F - 50], axis=1)) / 3.0 for l in range(50)])
     np.testing.assert_allclose(c[:, 0], ref, rtol=1e-12, atol=1e-15)
+
+
+def test_lammps_binary_block_averages_match_the_numpy_trajectory(pa, tmp_path):
+    """gofrt_block_average / msd_block_average over a Traj (the window moves
+    from block to block) equal the in-memory trajectory's: g(r,t) mean and
+    variance bit for bit, MSD to is_same."""
+    N, F = 200, 30
+    box_l = synth.lammps_box(12.0)
+    pos = synth.random_walk(43, N, F, box_l, 0.3)
+    vel = np.zeros((F, N, 3))
+    types = synth.types_by_fraction(N, [0.5, 0.5]).astype(np.int32)
+    tnp = pa.Trajectory(pos, vel, types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False)
+    path = str(tmp_path / "traj.bin")
+    tnp.write_lammps_binary(path, 0, -1)
+    tl = pa.Traj(path)
+    tl.setAccessWindowSize(20)
+    tl.setAccessStart(0)
+    m0, v0 = pa.gofrt_block_average(tnp, 2, 0.0, 5.0, 30, 6, 1, 1)
+    m1, v1 = pa.gofrt_block_average(tl, 2, 0.0, 5.0, 30, 6, 1, 1)
+    assert np.array_equal(m0, m1) and np.array_equal(np.nan_to_num(v0), np.nan_to_num(v1))
+    m0, v0 = pa.msd_block_average(tnp, 2, 1, 5, False, False)
+    m1, v1 = pa.msd_block_average(tl, 2, 1, 5, False, False)
+    assert is_same(m0, m1).all() and is_same(v0, v1).all()
