@@ -74,3 +74,37 @@ def test_fullsize_invariants_forms_and_oracle(c2):
     O.gofrt_single(ot, RMIN, RMAX, NBIN, leff, 16, 3, 0, N, primo, ref)
     got = g.calc_single_th(16, 3, 0, N, primo, None)
     assert first_diff(got, ref) is None, first_diff(got, ref)
+
+
+def test_fullsize_green_kubo_blocks():
+    """The Green-Kubo block workload at its full size: 2 vector currents of
+    2,000,000 AR(2) steps (the bench's seed), 32 uneven blocks x 20,000
+    lags in one call.  The split-BF16 MFMA form stays within 1e-6 of each
+    column's scale sqrt(C_aa(0) C_bb(0)) of the FP64 form for C(l) and within
+    1e-6 (l + 1) of it for the running integral; each block of the batched
+    FP64 call equals its own single call bit for bit (blocks 0, 13 and 31);
+    the block mean is the plain mean of the blocks to rounding."""
+    T, lmax, nb, seed = 2_000_000, 20_000, 32, 55
+    J = synth.ar2_currents(seed, T, ncur=2, coupling=0.3)
+    span = T - lmax
+    u = synth.uniform(seed, 31, nb)
+    lens = lmax + np.floor((span - nb * lmax) * u / u.sum()).astype(np.int64)
+    lens[-1] += span - lens.sum()
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    s = capi.DeviceSeries(J)
+    f64, mean64, _ = s.gk_blocks([0, 3], 1, lmax, 1, False, 0, starts, lens, mfma=False)
+    mf, _, _ = s.gk_blocks([0, 3], 1, lmax, 1, False, 0, starts, lens, mfma=True)
+    assert f64.shape == (nb, lmax, 14)
+    for b in range(nb):
+        c0 = np.sqrt(np.abs(np.diag(f64[b, 0, :4].reshape(2, 2))))
+        sig = np.outer(c0, c0).reshape(-1)
+        dC = np.abs(mf[b, :, :4] - f64[b, :, :4]) / sig
+        assert dC.max() <= 1e-6, (b, dC.max())
+        dI = np.abs(mf[b, :, 4:12:2] - f64[b, :, 4:12:2]) / sig / (np.arange(lmax)[:, None] + 1.0)
+        assert dI.max() <= 1e-6, (b, dI.max())
+    for b in (0, 13, 31):
+        one = s.gk([0, 3], 1, lmax, int(lens[b]), 1, False, 0, int(starts[b]))
+        same = np.array_equal(np.nan_to_num(one, nan=0.25), np.nan_to_num(f64[b], nan=0.25))
+        assert same, b
+    fin = np.isfinite(mean64[:, :4])
+    np.testing.assert_allclose(mean64[:, :4][fin], f64[:, :, :4].mean(axis=0)[fin], rtol=1e-9, atol=1e-12)
