@@ -27,6 +27,16 @@
  *                          translation with cm_self, cm channel), thread
  *                          invariance.
  *   orc_type_ids, orc_cm   pinned: closed forms.
+ *   orc_sph_ylm            pinned: scipy.special.sph_harm_y (complex Y_l^m,
+ *                          Condon-Shortley phase) turned into the real form.
+ *   orc_sph_density        pinned: an independent numpy brute force (lattice
+ *                          minimum image, scipy harmonics) on small frames.
+ *   orc_sph_correlations   pinned: the addition theorem, sum_m Y_lm(a)Y_lm(b)
+ *                          = (2l+1)/(4 pi) P_l(a.b), evaluated by brute force
+ *                          over neighbour pairs with scipy's Legendre P_l.
+ *                          The reference's t1 visiting order depends on its
+ *                          thread split; this restatement visits t1 in
+ *                          ascending order (agreement to rounding).
  */
 #include <math.h>
 #include <pthread.h>
@@ -640,4 +650,159 @@ void orc_mvc_end(double *var, double *cov, uint64_t size_block, unsigned n_colum
         for (unsigned ic = 0; ic < n_columns; ic++) var[(uint64_t)ic * size_block + ib] /= d;
         for (unsigned k = 0; k < ncv; k++) cov[(uint64_t)k * size_block + ib] /= d;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Spherical-harmonic density correlations (SphericalCorrelations)          */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_PI 3.14159265358979323846264338327950288419716939937510
+#define ORC_SH_LMAX 10
+
+/* specialfunctions.h:33-49 -- coefficient of the real harmonic (l, +-m):
+ * sqrt((2l+1)/(4 pi)) for m = 0, else sqrt((2l+1)/(4 pi) * 2 / v) (-1)^m with
+ * v = (l-m+1)(l-m+2)...(l+m) multiplied up in that order from 1. */
+static double sh_coeff(int l, int m)
+{
+    if (m < 0) m = -m;
+    if (m == 0) return sqrt((double)(2 * l + 1) / (4 * ORC_PI));
+    double value = 1.0;
+    for (int v = l - m + 1; v <= l + m; v++) value = value * v;
+    return sqrt((double)(2 * l + 1) / (4 * ORC_PI) * 2 / value) * pow(-1.0, m);
+}
+
+/* specialfunctions.h:279-360 (SphericalHarmonics<lmax,double,true,true>) --
+ * real spherical harmonics of the direction of (x, y, z):
+ *   cos(theta) = z/r, cos(phi) = x/rxy, sin(phi) = y/rxy;
+ *   P_l^l = -(2l-1) sqrt(1-c^2) P_{l-1}^{l-1}, P_{m+1}^m = P_m^m c (2m+1),
+ *   P_l^m = (P_{l-1}^m c (2l-1) - P_{l-2}^m (l+m-1)) / (l-m)   (:262-318);
+ *   cos/sin(m phi) by the Chebyshev recursion (:11-22);
+ *   Y_lm = P_l^|m| coeff(l,m) cos(m phi) for m >= 0, sin(|m| phi) for m < 0.
+ * Output in the reference's memory order (MultiValDynamic::init, :149-154):
+ * for l = lmax down to 0: m = -1, -2, ..., -l, then m = 0, 1, ..., l. */
+void orc_sph_ylm(double x, double y, double z, int lmax, double *out)
+{
+    double P[ORC_SH_LMAX + 1][ORC_SH_LMAX + 1];
+    double cs[2 * (ORC_SH_LMAX + 1) + 2];
+    double rxy = sqrt(x * x + y * y);
+    double r = sqrt(x * x + y * y + z * z);
+    double ct = z / r, sp = y / rxy, cp = x / rxy;
+    P[0][0] = 1.0;
+    for (int l = 1; l <= lmax; l++) P[l][l] = (double)(-(2 * l - 1)) * sqrt(1 - ct * ct) * P[l - 1][l - 1];
+    for (int m = 0; m < lmax; m++) {
+        P[m + 1][m] = P[m][m] * ct * (2 * m + 1);
+        for (int l = m + 2; l <= lmax; l++)
+            P[l][m] = (P[l - 1][m] * ct * (2 * l - 1) - P[l - 2][m] * (l + m - 1)) / (l - m);
+    }
+    cs[0] = 1.0;
+    cs[1] = 0.0;
+    cs[2] = cp;
+    cs[3] = sp;
+    for (int i = 2; i <= lmax; i++) {
+        cs[2 * i] = 2 * cp * cs[2 * (i - 1)] - cs[2 * (i - 2)];
+        cs[2 * i + 1] = 2 * cp * cs[2 * (i - 1) + 1] - cs[2 * (i - 2) + 1];
+    }
+    int off = 0;
+    for (int l = lmax; l >= 0; l--) {
+        for (int m = 1; m <= l; m++) out[off + m - 1] = P[l][m] * sh_coeff(l, m) * cs[2 * m + 1];
+        for (int m = 0; m <= l; m++) out[off + l + m] = P[l][m] * sh_coeff(l, m) * cs[2 * m];
+        off += 2 * l + 1;
+    }
+}
+
+/* sphericalbase.cpp:19-66 (no neighbour list) -- result[i][type_j][idx][:] is
+ * the sum, over the other atoms j in index order, of the harmonics of the
+ * minimum image x_i - x_j (frame ts for both) whose distance d falls in bin
+ * idx = (int)floorf((d - rmin_p) / dr_p), 0 <= idx < nbin, of the range
+ * p = ntypes * type_i + type_j (dr_p = (rmax_p - rmin_p) / nbin, :5-14). */
+void orc_sph_density(const orc_traj *tr, uint64_t ts, const double *rminmax, unsigned nbin, int lmax,
+                     double *res)
+{
+    const uint64_t N = tr->natoms, nt = tr->ntypes;
+    const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
+    double y[(ORC_SH_LMAX + 1) * (ORC_SH_LMAX + 1)];
+    double *dr = malloc(nt * nt * sizeof(double));
+    for (uint64_t p = 0; p < nt * nt; p++) dr[p] = (rminmax[2 * p + 1] - rminmax[2 * p]) / (double)nbin;
+    memset(res, 0, N * nt * nbin * L2 * sizeof(double));
+    const double *box = tr->box + ts * tr->box_stride;
+    for (uint64_t i = 0; i < N; i++) {
+        const unsigned ti = (unsigned)tr->type_id[i];
+        for (uint64_t j = 0; j < N; j++) {
+            const unsigned tj = (unsigned)tr->type_id[j];
+            const uint64_t p = nt * ti + tj;
+            if (i == j) continue;
+            double x[3];
+            double d = sqrt(orc_d2_min_image(tr->pos + (ts * N + i) * 3, tr->pos + (ts * N + j) * 3, box,
+                                             tr->triclinic, x));
+            int idx = (int)floorf((float)((d - rminmax[2 * p]) / dr[p]));
+            if (idx < (int)nbin && idx >= 0) {
+                orc_sph_ylm(x[0], x[1], x[2], lmax, y);
+                double *dst = res + L2 * (nbin * (nt * i + tj) + (uint64_t)idx);
+                for (size_t k = 0; k < L2; k++) dst[k] += y[k];
+            }
+        }
+    }
+    free(dr);
+}
+
+/* sphericalcorrelations.cpp:62-236 and corr_sh_calc :238-270, one thread --
+ * for every lag dt < leff and t1 = 0, skip, ... < n (ascending): the product
+ * of the densities of frames primo+t1 and primo+t1+dt, element by element;
+ * per (atom, type_j, bin) and l the sum over m (the m = 0 term, then
+ * m = -1..-l, then m = 1..l; MultiVal::sum_in_m_zero); the running mean over
+ * the central atoms of each type (in index order); then the running mean over
+ * t1 per lag.  out[dt][type_i][type_j][bin][k] holds l = lmax - k
+ * (running_average_m_zero_vector walks l from lmax down).  leff = min(n, tmax)
+ * (tmax = 0: n).  Returns -1 when leff + n + primo > nframes (:63-65). */
+int orc_sph_correlations(const orc_traj *tr, const double *rminmax, unsigned nbin, int lmax, unsigned tmax,
+                         unsigned skip, unsigned n, uint64_t primo, double *out)
+{
+    const uint64_t N = tr->natoms, nt = tr->ntypes;
+    const size_t L2 = (size_t)(lmax + 1) * (lmax + 1), L1 = (size_t)lmax + 1;
+    const unsigned leff = (n < tmax || tmax == 0) ? n : tmax;
+    if ((uint64_t)leff + n + primo > tr->nframes) return -1;
+    if (skip == 0) skip = 1;
+    const uint64_t nfr = (uint64_t)n + leff - 1, snap = N * nt * nbin * L2;
+    double *dens = malloc(nfr * snap * sizeof(double));
+    for (uint64_t f = 0; f < nfr; f++) orc_sph_density(tr, primo + f, rminmax, nbin, lmax, dens + f * snap);
+    const uint64_t fin = nt * nt * nbin * L1;
+    double *ave = malloc(fin * sizeof(double));
+    double *prod = malloc(L2 * sizeof(double));
+    unsigned *cnt = calloc(leff, sizeof(unsigned));
+    int *acnt = malloc(nt * sizeof(int));
+    memset(out, 0, (size_t)leff * fin * sizeof(double));
+    for (unsigned t1 = 0; t1 < n; t1 += skip) {
+        for (unsigned dt = 0; dt < leff; dt++) {
+            const double *s1 = dens + (uint64_t)t1 * snap, *s2 = dens + (uint64_t)(t1 + dt) * snap;
+            memset(ave, 0, fin * sizeof(double));
+            for (uint64_t t = 0; t < nt; t++) acnt[t] = 0;
+            for (uint64_t i = 0; i < N; i++) {
+                const unsigned ti = (unsigned)tr->type_id[i];
+                acnt[ti]++;
+                for (uint64_t tj = 0; tj < nt; tj++)
+                    for (unsigned b = 0; b < nbin; b++) {
+                        const uint64_t base = L2 * (nbin * (nt * i + tj) + b);
+                        for (size_t k = 0; k < L2; k++) prod[k] = s1[base + k] * s2[base + k];
+                        double *av = ave + L1 * (nbin * (nt * ti + tj) + b);
+                        size_t off = 0;
+                        for (int l = lmax, k = 0; l >= 0; l--, k++) {
+                            double v = prod[off + l];
+                            for (int m = 0; m < l; m++) v += prod[off + m];
+                            for (int m = 0; m < l; m++) v += prod[off + l + 1 + m];
+                            av[k] += (v - av[k]) / (double)acnt[ti];
+                            off += 2 * (size_t)l + 1;
+                        }
+                    }
+            }
+            cnt[dt]++;
+            double *o = out + (uint64_t)dt * fin;
+            for (uint64_t k = 0; k < fin; k++) o[k] += (ave[k] - o[k]) / (double)cnt[dt];
+        }
+    }
+    free(dens);
+    free(ave);
+    free(prod);
+    free(cnt);
+    free(acnt);
+    return 0;
 }

@@ -74,6 +74,11 @@ def lib():
         L.orc_mvc_update.argtypes = [_dp, _dp, _dp, _dp, _u64, ctypes.c_uint, _up, ctypes.c_uint,
                                      ctypes.c_uint, _dp, _dp]
         L.orc_mvc_end.argtypes = [_dp, _dp, _u64, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]
+        L.orc_sph_ylm.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _dp]
+        L.orc_sph_density.argtypes = [ctypes.POINTER(OrcTraj), _u64, _dp, ctypes.c_uint, ctypes.c_int, _dp]
+        L.orc_sph_correlations.restype = ctypes.c_int
+        L.orc_sph_correlations.argtypes = [ctypes.POINTER(OrcTraj), _dp, ctypes.c_uint, ctypes.c_int,
+                                           ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, _dp]
         _lib = L
     return _lib
 
@@ -160,6 +165,39 @@ def gk(data, idx_j, skip, lmax, subtract_mean, start_mean, n_seg, n, primo):
     out = np.zeros((leff, 3 * ncur * ncur + 2), dtype=np.float64)
     rc = L.orc_gk_calculate(_p(data), T, ncols, _p(idx, _up), ncur, skip, lmax, int(bool(subtract_mean)),
                             start_mean, n_seg, n, primo, _p(out))
+    if rc != 0:
+        raise RuntimeError("trajectory is too short for this kind of calculation")
+    return out
+
+
+def sph_ylm(v, lmax=10):
+    """Real harmonics of the direction of v in the reference memory order."""
+    out = np.empty((lmax + 1) ** 2, dtype=np.float64)
+    lib().orc_sph_ylm(float(v[0]), float(v[1]), float(v[2]), lmax, _p(out))
+    return out
+
+
+def _rminmax(rminmax, nt):
+    rm = np.ascontiguousarray(np.asarray(rminmax, dtype=np.float64).reshape(-1, 2))
+    assert rm.shape[0] == nt * nt
+    return rm
+
+
+def sph_density(tr, frame, rminmax, nbin, lmax=10):
+    """SphericalBase::calc of one frame: [N][nt][nbin][(lmax+1)^2]."""
+    rm = _rminmax(rminmax, tr.ntypes)
+    out = np.empty((tr.natoms, tr.ntypes, nbin, (lmax + 1) ** 2), dtype=np.float64)
+    lib().orc_sph_density(tr.ref, frame, _p(rm), nbin, lmax, _p(out))
+    return out
+
+
+def sph_correlations(tr, rminmax, nbin, tmax, skip, n, primo, lmax=10):
+    """SphericalCorrelations reset(n) + calculate(primo): [leff][nt][nt][nbin][lmax+1]."""
+    rm = _rminmax(rminmax, tr.ntypes)
+    leff = n if (n < tmax or tmax == 0) else tmax
+    nt = tr.ntypes
+    out = np.zeros((leff, nt, nt, nbin, lmax + 1), dtype=np.float64)
+    rc = lib().orc_sph_correlations(tr.ref, _p(rm), nbin, lmax, tmax, skip, n, primo, _p(out))
     if rc != 0:
         raise RuntimeError("trajectory is too short for this kind of calculation")
     return out

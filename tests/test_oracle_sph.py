@@ -1,0 +1,154 @@
+"""Pins the oracle's spherical-harmonic correlation path
+(specialfunctions.h:11-360, sphericalbase.cpp:19-66,
+sphericalcorrelations.cpp:62-270).
+
+The reference's goldens for it (tests/test_sh/test_sh.csv,
+cpp_regression_data/sh*) come from trajectories that are not in the snapshot,
+so the oracle is pinned by independent mathematics:
+  * the real harmonics against scipy's complex Y_l^m (Condon-Shortley phase),
+  * the densities against a numpy brute force (nearest image, scipy harmonics),
+  * the correlations against the addition theorem
+      sum_m Y_lm(a) Y_lm(b) = (2l+1)/(4 pi) P_l(a.b)
+    summed by brute force over neighbour pairs with scipy's Legendre P_l, which
+    fixes the m-sum, the normalisation, the l order of the output (index k
+    holds l = lmax - k), the per-type averages and the t0/skip loops without
+    using any harmonic at all."""
+import numpy as np
+import pytest
+from scipy import special
+
+from analisi_amd import synth
+from oracle import oracle as O
+
+LMAX = 10
+
+
+def _real_ylm_scipy(v, lmax):
+    """Real harmonics in the reference memory order from scipy's complex ones:
+    Y(l, m>0) = sqrt2 (-1)^m Re Y_l^m, Y(l, -m) = sqrt2 (-1)^m Im Y_l^m."""
+    r = np.linalg.norm(v)
+    theta = np.arccos(v[2] / r)
+    phi = np.arctan2(v[1], v[0])
+    out = []
+    for l in range(lmax, -1, -1):
+        neg = [np.sqrt(2) * (-1) ** m * special.sph_harm_y(l, m, theta, phi).imag for m in range(1, l + 1)]
+        pos = [special.sph_harm_y(l, 0, theta, phi).real] + \
+              [np.sqrt(2) * (-1) ** m * special.sph_harm_y(l, m, theta, phi).real for m in range(1, l + 1)]
+        out += neg + pos
+    return np.array(out)
+
+
+def _blocks(y, lmax):
+    """Split a reference-order vector into {l: values over its 2l+1 m's}."""
+    out, off = {}, 0
+    for l in range(lmax, -1, -1):
+        out[l] = y[off:off + 2 * l + 1]
+        off += 2 * l + 1
+    return out
+
+
+def test_ylm_matches_scipy():
+    rng = np.random.default_rng(11)
+    for _ in range(60):
+        v = rng.normal(size=3) * rng.uniform(0.1, 5.0)
+        np.testing.assert_allclose(O.sph_ylm(v, LMAX), _real_ylm_scipy(v, LMAX), rtol=0, atol=2e-12)
+    for lmax in (1, 2, 6):
+        v = np.array([0.3, -1.2, 0.7])
+        np.testing.assert_allclose(O.sph_ylm(v, lmax), _real_ylm_scipy(v, lmax), rtol=0, atol=2e-13)
+
+
+def test_addition_theorem_per_l():
+    rng = np.random.default_rng(12)
+    for _ in range(40):
+        a, b = rng.normal(size=3), rng.normal(size=3)
+        c = a @ b / np.linalg.norm(a) / np.linalg.norm(b)
+        ya, yb = _blocks(O.sph_ylm(a), LMAX), _blocks(O.sph_ylm(b), LMAX)
+        for l in range(LMAX + 1):
+            want = (2 * l + 1) / (4 * np.pi) * special.eval_legendre(l, c)
+            assert abs(ya[l] @ yb[l] - want) <= 1e-12, (l, ya[l] @ yb[l], want)
+
+
+def _small_traj(seed, N, F, L, types_frac):
+    box_l = synth.lammps_box(L)
+    pos = synth.random_walk(seed, N, F, box_l, 0.3)
+    pos = np.mod(pos, L)      # inside the cell: nearest image = lattice minimum
+    types = synth.types_by_fraction(N, types_frac)
+    return O.Traj(pos, types, np.tile(box_l, (F, 1)), False), L
+
+
+def _neigh(tr, L, frame, i, rminmax, nbin):
+    """Nearest-image neighbours of atom i in frame: {(type_j, bin): [vectors]}."""
+    nt = tr.ntypes
+    ti = tr.type_id[i]
+    out = {}
+    for j in range(tr.natoms):
+        if j == i:
+            continue
+        x = tr.pos[frame, i] - tr.pos[frame, j]
+        x = x - L * np.round(x / L)
+        tj = tr.type_id[j]
+        rmin, rmax = rminmax[nt * ti + tj]
+        dr = (rmax - rmin) / nbin
+        b = int(np.floor(np.float32((np.sqrt(x @ x) - rmin) / dr)))
+        if 0 <= b < nbin:
+            out.setdefault((tj, b), []).append(x)
+    return out
+
+
+RMM2 = [(0.5, 3.0), (1.0, 3.4), (1.0, 3.4), (0.8, 2.9)]
+
+
+def test_density_bruteforce():
+    tr, L = _small_traj(5, 40, 3, 8.0, [0.6, 0.4])
+    nbin = 3
+    for frame in (0, 2):
+        got = O.sph_density(tr, frame, RMM2, nbin)
+        want = np.zeros_like(got)
+        for i in range(tr.natoms):
+            for (tj, b), xs in _neigh(tr, L, frame, i, RMM2, nbin).items():
+                want[i, tj, b] = sum(_real_ylm_scipy(x, LMAX) for x in xs)
+        assert np.count_nonzero(want) > 0
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("skip,tmax,n,primo", [(1, 3, 5, 0), (2, 4, 6, 1), (3, 0, 4, 2)])
+def test_correlations_addition_theorem(skip, tmax, n, primo):
+    tr, L = _small_traj(7, 30, 12, 7.5, [0.5, 0.5])
+    nbin, nt = 2, tr.ntypes
+    got = O.sph_correlations(tr, RMM2, nbin, tmax, skip, n, primo)
+    leff = n if (n < tmax or tmax == 0) else tmax
+    assert got.shape == (leff, nt, nt, nbin, LMAX + 1)
+    ls = np.arange(LMAX + 1)
+    want = np.zeros_like(got)
+    t1s = list(range(0, n, skip))
+    ntype = np.bincount(tr.type_id, minlength=nt)
+    neigh = {}
+    for f in range(primo, primo + n + leff - 1):
+        for i in range(tr.natoms):
+            neigh[f, i] = _neigh(tr, L, f, i, RMM2, nbin)
+    for dt in range(leff):
+        for t1 in t1s:
+            for i in range(tr.natoms):
+                ti = tr.type_id[i]
+                n1, n2 = neigh[primo + t1, i], neigh[primo + t1 + dt, i]
+                for key, xs in n1.items():
+                    if key not in n2:
+                        continue
+                    u = np.array(xs) / np.linalg.norm(xs, axis=1)[:, None]
+                    w = np.array(n2[key]) / np.linalg.norm(n2[key], axis=1)[:, None]
+                    c = np.clip(u @ w.T, -1, 1)
+                    s = [(2 * l + 1) / (4 * np.pi) * special.eval_legendre(l, c).sum() for l in ls]
+                    want[dt, ti, key[0], key[1], LMAX - ls] += np.array(s) / ntype[ti]
+        want[dt] /= len(t1s)
+    scale = np.abs(want).max()
+    assert scale > 0.1
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * scale * 100)
+    # lag 0, l = 0: |sum of Y_00 over the bin|^2 = n_bin^2 / (4 pi) >= 0
+    assert np.all(got[0, ..., LMAX] >= 0)
+
+
+def test_correlations_length_check():
+    tr, _ = _small_traj(7, 10, 8, 7.5, [1.0])
+    with pytest.raises(RuntimeError, match="too short"):
+        O.sph_correlations(tr, [(0.5, 3.0)], 2, 3, 1, 5, 1)     # 3 + 5 + 1 > 8
+    O.sph_correlations(tr, [(0.5, 3.0)], 2, 3, 1, 5, 0)         # 3 + 5 + 0 = 8 fits
