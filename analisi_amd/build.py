@@ -17,7 +17,7 @@ import sysconfig
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
-HIP_SOURCES = ["runtime.hip", "gofrt.hip", "msd.hip", "gk.hip", "gk_mfma.hip", "blocks.hip"]
+HIP_SOURCES = ["runtime.hip", "gofrt.hip", "msd.hip", "gk.hip", "gk_mfma.hip", "blocks.hip", "sph.hip"]
 LIB = os.path.join(PKG, "libanalisi_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -70,6 +70,11 @@ SIGNATURES = [
     ("anl_mediavar_update_async", _i, [_vp, _vp, _vp, _u64, _u32, _vp]),
     ("anl_mediavar_end_async", _i, [_vp, _u64, _u32, _vp]),
     ("anl_counts_to_double_async", _i, [_vp, _d, _u64, _vp, _vp]),
+    ("anl_sph_create", _i, [_vp, _u32, _vp, _u32, _pp]),
+    ("anl_sph_destroy", _i, [_vp]),
+    ("anl_sph_calculate", _i, [_vp, _u64, _u32, _u32, _u32, _vp]),
+    ("anl_sph_snapshot", _i, [_vp, _u64, _vp]),
+    ("anl_sph_set_budget", _i, [_vp, _u64]),
 ]
 
 _lib = None
@@ -271,6 +276,48 @@ def msd(traj, skip, tmax, cm_msd, cm_self, n, primo):
     check(lib().anl_msd_calculate(traj.handle, skip, tmax, int(bool(cm_msd)), int(bool(cm_self)), n, primo,
                                   _ptr(out)))
     return out
+
+
+class DeviceSph:
+    """anl_sph: spherical-harmonic density correlations over a DeviceTrajectory;
+    rminmax is a list of ntypes^2 (rmin, rmax) pairs."""
+
+    def __init__(self, traj, rminmax, nbin, lmax=10):
+        self.traj = traj
+        self.nbin, self.lmax = int(nbin), int(lmax)
+        rm = np.ascontiguousarray(np.asarray(rminmax, dtype=np.float64).reshape(-1, 2))
+        if rm.shape[0] != traj.ntypes ** 2:
+            raise ValueError("rminmax must hold ntypes^2 (rmin, rmax) pairs")
+        h = ctypes.c_void_p()
+        check(lib().anl_sph_create(traj.handle, self.lmax, _ptr(rm), self.nbin, ctypes.byref(h)))
+        self.handle = h
+
+    def set_budget(self, nbytes):
+        check(lib().anl_sph_set_budget(self.handle, int(nbytes)))
+
+    def calculate(self, n, tmax, skip, primo):
+        leff = n if (n < tmax or tmax == 0) else tmax
+        nt = self.traj.ntypes
+        out = np.zeros((leff, nt, nt, self.nbin, self.lmax + 1), dtype=np.float64)
+        check(lib().anl_sph_calculate(self.handle, primo, n, tmax, skip, _ptr(out)))
+        return out
+
+    def snapshot(self, frame):
+        nt = self.traj.ntypes
+        out = np.zeros((self.traj.natoms, nt, self.nbin, (self.lmax + 1) ** 2), dtype=np.float64)
+        check(lib().anl_sph_snapshot(self.handle, frame, _ptr(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().anl_sph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:

@@ -104,6 +104,10 @@ struct anl_series {
 
 namespace anl {
 Cell cell_of(const double *box9);
+// T[k], k = 0..nbin: the smallest non-negative d2 whose bin
+// (int)floorf((float)((sqrt(d2) - rmin) / dr)) is >= k, computed on the host
+// with IEEE double arithmetic (gofrt.cpp:114-115, sphericalbase.cpp:52-53).
+std::vector<double> bin_edges(double rmin, double rmax, uint32_t nbin, double dr);
 void build_frac16(anl_traj *tr, hipStream_t s);
 
 // Prepared launches for the block engine: all host work (checks, plans,

@@ -1,0 +1,571 @@
+// Spherical-harmonic density correlations: SphericalCorrelations<lmax>
+// (sphericalcorrelations.cpp:62-270) over SphericalBase::calc
+// (sphericalbase.cpp:19-66, no neighbour list), FP64 throughout.
+//
+// For every frame f and atom i the reference builds the density
+//   S_f[i][type_j][bin][l,m] = sum over the other atoms j in the bin of the
+//   pair range (type_i, type_j) of Y_lm(x_i - x_j)   (real harmonics),
+// and the output is, per lag dt, the mean over t1 = 0, skip, .. < n of the
+// mean over the atoms i of each type of sum_m S_{t1}[i] S_{t1+dt}[i].
+//
+// Per chunk of t1 values (chunks bound the HBM held by the densities):
+//  1. sph_density_kernel: one wavefront per (frame, central atom).  The lanes
+//     take 64 other atoms at a time: FP64 minimum image in the reference's
+//     loop order, distance bin from exact host-computed edges of the pair's
+//     range, ballot of the atoms in range.  Each such atom, in slot order, is
+//     expanded by lanes m = 0..lmax (lane m runs the reference's Legendre
+//     recursion of column m and the Chebyshev recursion of cos/sin(m phi), in
+//     the same operation order) into the wave's density block in LDS.  The
+//     block is written once, as S[frame][type_j][bin][l][atom by type][m].
+//  2. sph_gram_kernel: for every problem (type_i, type_j, bin, l) the lag
+//     products sum_{i of type_i} sum_m S_{t1} S_{t2} are a banded Gram matrix
+//     over frames with K = (atoms of type_i) x (2l+1) contiguous in HBM.  A
+//     256-thread workgroup computes a 64 (t1) x 64 (t2) tile, 4x4 FP64 FMA per
+//     thread, K staged through LDS 16 at a time (double-buffered); entries
+//     with 0 <= t2 - t1 < leff go to G[problem][t1 index][t2 - t1].
+//  3. sph_reduce_kernel: R[problem][dt] += sum over t1 of G, in t1 order.
+// The host divides R by (number of t1) x (atoms of type_i) and writes the
+// reference layout [dt][type_i][type_j][bin][k] with k holding l = lmax - k.
+// Every sum has a fixed order: results are deterministic.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "anl_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace anl {
+
+#include "min_image.h"
+
+constexpr int SPH_WAVES = 4;           // wavefronts (central atoms) per density workgroup
+constexpr uint32_t SPH_WAVE_LDS = 1024;  // doubles of density block per wavefront (8 KiB)
+constexpr int GT = 64;                 // Gram tile edge (t1 rows, t2 columns)
+constexpr int GKC = 16;                // K values staged per step
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct SphDensArgs {
+    const double4 *pos;        // [F][N] slots
+    const double *box;         // [F][9]
+    const int32_t *type_slot;  // [N]
+    const int32_t *aprime;     // [N] index of the slot among the atoms sorted by type
+    const double *thr;         // [nt*nt][nbin+1] bin edges of each pair range (in d2)
+    const double *coef;        // [(lmax+1)^2]: coefficient of (l, |m|) at l*(lmax+1)+|m|
+    double *S;                 // [nfr][FS]
+    uint64_t N, FS, f0;        // atoms, doubles per frame, first frame
+    uint32_t nt, nbin, lmax;
+    uint32_t nbc;              // bins per launch column (blockIdx.y)
+    uint32_t ngroups;          // ceil(N / SPH_WAVES)
+};
+
+// Lane m's share of the real harmonics of direction (x, y, z)
+// (specialfunctions.h:11-22, 262-360): P_l^m for l = m..lmax and
+// cos/sin(m phi), added into acc[l*l + l + m] (cos) and acc[l*l + l - m] (sin).
+__device__ __forceinline__ void add_harmonics(double *acc, double x, double y, double z, int m, int lmax,
+                                              const double *coef) {
+    const double rxy = sqrt(x * x + y * y);
+    const double r = sqrt(x * x + y * y + z * z);
+    const double ct = z / r, sp = y / rxy, cp = x / rxy;
+    double pmm = 1.0;
+    if (m > 0) {
+        const double st = sqrt(1 - ct * ct);
+        for (int l = 1; l <= m; ++l) pmm = (double)(-(2 * l - 1)) * st * pmm;
+    }
+    double cm = 1.0, sm = 0.0;
+    if (m > 0) {
+        double c2 = 1.0, s2 = 0.0, c1 = cp, s1 = sp;
+        for (int k = 2; k <= m; ++k) {
+            const double cn = 2 * cp * c1 - c2;
+            const double sn = 2 * cp * s1 - s2;
+            c2 = c1;
+            s2 = s1;
+            c1 = cn;
+            s1 = sn;
+        }
+        cm = c1;
+        sm = s1;
+    }
+    const int L1 = lmax + 1;
+    {
+        const double v = pmm * coef[m * L1 + m];
+        acc[m * m + m + m] += v * cm;
+        if (m > 0) acc[m * m] += v * sm;
+    }
+    if (m < lmax) {
+        double p2 = pmm, p1 = pmm * ct * (2 * m + 1);
+        for (int l = m + 1; l <= lmax; ++l) {
+            if (l > m + 1) {
+                const double p = (p1 * ct * (2 * l - 1) - p2 * (l + m - 1)) / (l - m);
+                p2 = p1;
+                p1 = p;
+            }
+            const double v = p1 * coef[l * L1 + m];
+            acc[l * l + l + m] += v * cm;
+            if (m > 0) acc[l * l + l - m] += v * sm;
+        }
+    }
+}
+
+template <bool TRI>
+__global__ __launch_bounds__(256) void sph_density_kernel(SphDensArgs a) {
+    extern __shared__ double sph_lds[];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t L2 = (a.lmax + 1) * (a.lmax + 1);
+    const uint32_t accn = a.nt * a.nbc * L2;
+    double *acc = sph_lds + wv * accn;
+    const uint32_t fl = blockIdx.x / a.ngroups;
+    const uint64_t i = (uint64_t)(blockIdx.x % a.ngroups) * SPH_WAVES + wv;
+    if (i >= a.N) return;  // the whole wavefront; no workgroup barrier below
+    const uint32_t b0 = blockIdx.y * a.nbc;
+    const uint32_t b1 = min(b0 + a.nbc, a.nbin);
+    for (uint32_t k = lane; k < accn; k += 64) acc[k] = 0.0;
+    wave_sync_lds();
+
+    const uint64_t f = a.f0 + fl;
+    const double4 *P = a.pos + f * a.N;
+    const DevCell c = load_cell(a.box + f * 9);
+    const double4 pi = P[i];
+    const int ti = a.type_slot[i];
+    const double *thr_i = a.thr + (size_t)ti * a.nt * (a.nbin + 1);
+    for (uint64_t jb = 0; jb < a.N; jb += 64) {
+        const uint64_t j = jb + lane;
+        bool in = false;
+        double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+        int tj = 0, bin = 0;
+        if (j < a.N && j != i) {
+            const double4 pj = P[j];
+            x0 = pi.x - pj.x;
+            x1 = pi.y - pj.y;
+            x2 = pi.z - pj.z;
+            min_image_ref<TRI>(x0, x1, x2, c);
+            const double d2 = d2_of(x0, x1, x2);
+            tj = a.type_slot[j];
+            const double *T = thr_i + (size_t)tj * (a.nbin + 1);
+            if (d2 >= T[b0] && d2 < T[b1]) {
+                in = true;
+                uint32_t b = b0;
+                while (b + 1 < b1 && d2 >= T[b + 1]) ++b;
+                bin = (int)b;
+            }
+        }
+        unsigned long long mask = __ballot(in);
+        while (mask) {
+            const int src = __ffsll(mask) - 1;  // lowest lane first: slot order
+            mask &= mask - 1;
+            const double y0 = __shfl(x0, src), y1 = __shfl(x1, src), y2 = __shfl(x2, src);
+            const int jt = __shfl(tj, src), bb = __shfl(bin, src);
+            // lane m owns the entries (l, +-m): no two lanes touch one address
+            if (lane <= a.lmax)
+                add_harmonics(acc + ((uint32_t)jt * a.nbc + (uint32_t)bb - b0) * L2, y0, y1, y2, (int)lane,
+                              (int)a.lmax, a.coef);
+        }
+    }
+    wave_sync_lds();
+    double *Sf = a.S + (uint64_t)fl * a.FS;
+    const uint64_t ap = (uint64_t)a.aprime[i];
+    for (uint32_t k = lane; k < accn; k += 64) {
+        const uint32_t blk = k / L2, e = k - blk * L2;
+        const uint32_t jt = blk / a.nbc, bl = blk - jt * a.nbc;
+        if (b0 + bl >= a.nbin) continue;
+        uint32_t l = (uint32_t)sqrtf((float)e);
+        while (l * l > e) --l;
+        while ((l + 1) * (l + 1) <= e) ++l;
+        const uint64_t blk_off = ((uint64_t)(jt * a.nbin + b0 + bl) * L2 + l * l) * a.N;
+        Sf[blk_off + ap * (2 * l + 1) + (e - l * l)] = acc[k];
+    }
+}
+
+struct SphGramArgs {
+    const double *S;          // chunk densities [nfr][FS]
+    double *G;                // [nprob][n0c][leff]
+    const uint32_t *tstart;   // [nt+1] first type-sorted index of each type
+    uint64_t N, FS;
+    uint32_t nt, nbin, lmax;
+    uint32_t n0c, skip, leff, nfr;
+    uint32_t ncol;            // column tiles per row tile
+};
+
+__global__ __launch_bounds__(256) void sph_gram_kernel(SphGramArgs a) {
+    __shared__ double As[2][GKC][GT];
+    __shared__ double Bs[2][GKC][GT];
+    const uint32_t L1 = a.lmax + 1, L2 = L1 * L1;
+    uint32_t y = blockIdx.y;
+    const uint32_t l = y % L1;
+    y /= L1;
+    const uint32_t bin = y % a.nbin;
+    y /= a.nbin;
+    const uint32_t jt = y % a.nt, ti = y / a.nt;
+    const uint32_t rt = blockIdx.x / a.ncol, ct = blockIdx.x - rt * a.ncol;
+    const uint32_t e0 = rt * GT;
+    const uint32_t rows = min((uint32_t)GT, a.n0c - e0);
+    const uint64_t t1_first = (uint64_t)e0 * a.skip;
+    const uint64_t t1_last = (uint64_t)(e0 + rows - 1) * a.skip;
+    const uint64_t c0 = t1_first + (uint64_t)ct * GT;
+    if (c0 >= a.nfr || c0 > t1_last + a.leff - 1) return;  // the whole workgroup
+
+    const uint32_t w = 2 * l + 1;
+    const uint64_t K = (uint64_t)(a.tstart[ti + 1] - a.tstart[ti]) * w;
+    const uint64_t base = ((uint64_t)(jt * a.nbin + bin) * L2 + l * l) * a.N + (uint64_t)a.tstart[ti] * w;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lr = tid >> 2, kq = (tid & 3) * 4;  // loader: row/column lr, k offset kq
+    const uint32_t ea = e0 + lr;
+    const bool va = ea < a.n0c;
+    const uint64_t fb = c0 + lr;
+    const bool vb = fb < a.nfr;
+    const double *Ap = a.S + (va ? (uint64_t)ea * a.skip : 0) * a.FS + base;
+    const double *Bp = a.S + (vb ? fb : 0) * a.FS + base;
+    const uint32_t tr = tid >> 4, tc = tid & 15;  // compute: rows 4tr.., columns 4tc..
+
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    double ra[4], rb[4];
+    auto load = [&](uint64_t kc) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t k = kc + kq + u;
+            ra[u] = (va && k < K) ? Ap[k] : 0.0;
+            rb[u] = (vb && k < K) ? Bp[k] : 0.0;
+        }
+    };
+    load(0);
+    uint32_t buf = 0;
+    for (uint64_t kc = 0; kc < K; kc += GKC) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            As[buf][kq + u][lr] = ra[u];
+            Bs[buf][kq + u][lr] = rb[u];
+        }
+        __syncthreads();
+        if (kc + GKC < K) load(kc + GKC);
+#pragma unroll
+        for (int kk = 0; kk < GKC; ++kk) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[buf][kk][tr * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][kk][tc * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_fma(av[i], bv[j], acc[i][j]);
+        }
+        // the next step fills the other buffer; the one read here is refilled
+        // only after the next step's barrier, which every thread reaches after
+        // finishing these reads
+        buf ^= 1u;
+    }
+    double *Gp = a.G + (uint64_t)blockIdx.y * a.n0c * a.leff;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t e = e0 + tr * 4 + i;
+        if (e >= a.n0c) continue;
+        const uint64_t t1 = (uint64_t)e * a.skip;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t t2 = c0 + tc * 4 + j;
+            if (t2 >= a.nfr || t2 < t1 || t2 - t1 >= a.leff) continue;
+            Gp[(uint64_t)e * a.leff + (t2 - t1)] = acc[i][j];
+        }
+    }
+}
+
+// R[p][dt] += sum over the chunk's t1 (in order) of G[p][t1][dt]
+__global__ __launch_bounds__(256) void sph_reduce_kernel(const double *G, double *R, uint32_t nprob, uint32_t n0c,
+                                                         uint32_t leff) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (uint64_t)nprob * leff) return;
+    const uint64_t p = k / leff, dt = k - p * leff;
+    const double *g = G + p * n0c * leff + dt;
+    double s = 0.0;
+    for (uint32_t e = 0; e < n0c; ++e) s += g[(uint64_t)e * leff];
+    R[k] = R[k] + s;
+}
+
+__global__ __launch_bounds__(256) void sph_zero_kernel(double *p, uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}
+
+// specialfunctions.h:33-49: sqrt((2l+1)/(4 pi)) for m = 0, else
+// sqrt((2l+1)/(4 pi) * 2 / v) (-1)^m, v = (l-m+1)(l-m+2)...(l+m) multiplied
+// up in that order from 1.
+static double sh_coeff(int l, int m) {
+    const double pi = 3.14159265358979323846264338327950288419716939937510;
+    if (m == 0) return std::sqrt((double)(2 * l + 1) / (4 * pi));
+    double value = 1.0;
+    for (int v = l - m + 1; v <= l + m; v++) value = value * v;
+    return std::sqrt((double)(2 * l + 1) / (4 * pi) * 2 / value) * std::pow(-1.0, m);
+}
+
+}  // namespace anl
+
+using namespace anl;
+
+struct anl_sph {
+    anl_traj *tr = nullptr;
+    uint32_t lmax = 0, nbin = 0, nt = 0;
+    std::vector<double> rminmax;      // [nt*nt][2]
+    std::vector<int32_t> aprime;      // [N] type-sorted index of each slot
+    std::vector<uint32_t> tstart;     // [nt+1]
+    double *d_thr = nullptr;          // [nt*nt][nbin+1]
+    double *d_coef = nullptr;         // [(lmax+1)^2]
+    int32_t *d_aprime = nullptr;
+    uint32_t *d_tstart = nullptr;
+    uint64_t budget = 8ull << 30;     // bytes of densities held at once
+    void *work = nullptr;             // densities, G, R (grow-only)
+    size_t work_cap = 0;
+};
+
+namespace {
+
+uint64_t frame_doubles(const anl_sph *h) {
+    const uint64_t L2 = (uint64_t)(h->lmax + 1) * (h->lmax + 1);
+    return h->tr->N * h->nt * h->nbin * L2;
+}
+
+uint64_t a256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+void *workspace(anl_sph *h, size_t bytes) {
+    if (bytes <= h->work_cap) return h->work;
+    ANL_HIP(hipDeviceSynchronize());
+    if (h->work) ANL_HIP(hipFree(h->work));
+    h->work = nullptr;
+    h->work_cap = 0;
+    ANL_HIP(hipMalloc(&h->work, bytes));
+    h->work_cap = bytes;
+    return h->work;
+}
+
+// bins per density launch column so that a wavefront's block fits its LDS
+uint32_t bins_per_launch(const anl_sph *h) {
+    const uint32_t per_bin = h->nt * (h->lmax + 1) * (h->lmax + 1);
+    const uint32_t cap = per_bin > SPH_WAVE_LDS ? 2 * SPH_WAVE_LDS : SPH_WAVE_LDS;
+    return std::min(h->nbin, std::max(1u, cap / per_bin));
+}
+
+// densities of frames [f0, f0 + nfr) of the device trajectory into S
+void launch_density(anl_sph *h, uint64_t f0, uint32_t nfr, double *S, hipStream_t s) {
+    SphDensArgs a{};
+    a.pos = h->tr->pos;
+    a.box = h->tr->box;
+    a.type_slot = h->tr->type_slot;
+    a.aprime = h->d_aprime;
+    a.thr = h->d_thr;
+    a.coef = h->d_coef;
+    a.S = S;
+    a.N = h->tr->N;
+    a.FS = frame_doubles(h);
+    a.f0 = f0;
+    a.nt = h->nt;
+    a.nbin = h->nbin;
+    a.lmax = h->lmax;
+    a.nbc = bins_per_launch(h);
+    a.ngroups = (uint32_t)((a.N + SPH_WAVES - 1) / SPH_WAVES);
+    const uint32_t accn = a.nt * a.nbc * (a.lmax + 1) * (a.lmax + 1);
+    const size_t lds = (size_t)SPH_WAVES * accn * sizeof(double);
+    const uint64_t gx = (uint64_t)nfr * a.ngroups;
+    if (gx > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom) pairs in one density launch");
+    if ((h->nbin + a.nbc - 1) / a.nbc > 65535) fail(ANL_ERR_ARG, "too many distance bins");
+    dim3 grid((uint32_t)gx, (h->nbin + a.nbc - 1) / a.nbc);
+    if (h->tr->triclinic)
+        hipLaunchKernelGGL(sph_density_kernel<true>, grid, dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL(sph_density_kernel<false>, grid, dim3(256), lds, s, a);
+    ANL_HIP(hipGetLastError());
+}
+
+void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32_t skip, double *out_host) {
+    anl_traj *tr = h->tr;
+    if (skip == 0) skip = 1;
+    const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;
+    if ((uint64_t)leff + n + primo > tr->F)
+        fail(ANL_ERR_TOO_SHORT,
+             "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
+             "the size of the average or the lenght of the correlation function");
+    const uint32_t L1 = h->lmax + 1, nt = h->nt;
+    const size_t fin = (size_t)nt * nt * h->nbin * L1;
+    if (n == 0 || leff == 0) return;
+    const uint64_t n0 = (n + skip - 1) / skip;  // t1 = 0, skip, .. < n
+    const uint64_t fbytes = frame_doubles(h) * sizeof(double);
+    const uint64_t maxfr = std::max<uint64_t>(1, h->budget / std::max<uint64_t>(fbytes, 1));
+    if (leff > maxfr) fail(ANL_ERR_ARG, "the densities of one lag window do not fit the density budget");
+    uint64_t rc = std::min<uint64_t>(n0, (maxfr - leff) / skip + 1);
+    if (rc >= (uint64_t)GT) rc = rc / GT * GT;
+    const uint32_t nprob = (uint32_t)fin;
+    if (nprob > 65535) fail(ANL_ERR_ARG, "too many (type, type, bin, l) channels for one launch (at most 65535)");
+    const uint64_t nfr_max = (rc - 1) * skip + leff;
+    const uint64_t offS = 0, offG = a256(nfr_max * fbytes), offR = offG + a256((uint64_t)nprob * rc * leff * 8);
+    char *w = (char *)workspace(h, offR + a256((uint64_t)nprob * leff * 8));
+    double *S = (double *)(w + offS), *G = (double *)(w + offG), *R = (double *)(w + offR);
+    hipStream_t s = nullptr;
+    const uint64_t nR = (uint64_t)nprob * leff;
+    hipLaunchKernelGGL(sph_zero_kernel, dim3((uint32_t)((nR + 255) / 256)), dim3(256), 0, s, R, nR);
+    ANL_HIP(hipGetLastError());
+    for (uint64_t e0 = 0; e0 < n0; e0 += rc) {
+        const uint32_t n0c = (uint32_t)std::min<uint64_t>(rc, n0 - e0);
+        const uint64_t t1_first = e0 * skip;
+        const uint32_t nfr = (uint32_t)((uint64_t)(n0c - 1) * skip + leff);
+        launch_density(h, primo + t1_first, nfr, S, s);
+        SphGramArgs g{};
+        g.S = S;
+        g.G = G;
+        g.tstart = h->d_tstart;
+        g.N = tr->N;
+        g.FS = frame_doubles(h);
+        g.nt = nt;
+        g.nbin = h->nbin;
+        g.lmax = h->lmax;
+        g.n0c = n0c;
+        g.skip = skip;
+        g.leff = leff;
+        g.nfr = nfr;
+        const uint64_t span = std::min<uint64_t>((uint64_t)(std::min<uint32_t>(n0c, GT) - 1) * skip + leff, nfr);
+        g.ncol = (uint32_t)((span + GT - 1) / GT);
+        const uint32_t nrt = (n0c + GT - 1) / GT;
+        hipLaunchKernelGGL(sph_gram_kernel, dim3(nrt * g.ncol, nprob), dim3(256), 0, s, g);
+        ANL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(sph_reduce_kernel, dim3((uint32_t)((nR + 255) / 256)), dim3(256), 0, s, G, R, nprob,
+                           n0c, leff);
+        ANL_HIP(hipGetLastError());
+    }
+    std::vector<double> Rh(nR);
+    ANL_HIP(hipMemcpy(Rh.data(), R, nR * sizeof(double), hipMemcpyDeviceToHost));
+    // R[((ti*nt + tj)*nbin + b)*L1 + l][dt] -> out[dt][ti][tj][b][lmax - l]
+    for (uint32_t dt = 0; dt < leff; dt++)
+        for (uint32_t p = 0; p < nprob; p++) {
+            const uint32_t l = p % L1, q = p / L1, ti = q / (nt * h->nbin);
+            const double na = (double)(h->tstart[ti + 1] - h->tstart[ti]);
+            out_host[(size_t)dt * fin + (size_t)q * L1 + (h->lmax - l)] = Rh[(size_t)p * leff + dt] / na / (double)n0;
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t nbin, anl_sph **out) {
+    return guard([&] {
+        if (!tr || !rminmax || !out) fail(ANL_ERR_ARG, "null pointer passed to anl_sph_create");
+        if (lmax > ANL_SPH_LMAX_MAX) fail(ANL_ERR_ARG, "lmax must be at most 10");
+        if (nbin == 0) fail(ANL_ERR_ARG, "nbin must be positive");
+        const uint32_t nt = tr->nt;
+        for (uint32_t p = 0; p < nt * nt; p++) {
+            const double lo = rminmax[2 * p], hi = rminmax[2 * p + 1];
+            if (!std::isfinite(lo) || !std::isfinite(hi) || !(hi > lo))
+                fail(ANL_ERR_ARG, "every radial range must be finite with rmax > rmin");
+        }
+        if (nt * (lmax + 1) * (lmax + 1) > 2 * SPH_WAVE_LDS)
+            fail(ANL_ERR_ARG, "too many atom types for the per-atom density block");
+        ensure_device();
+        anl_sph *h = new anl_sph();
+        h->tr = tr;
+        h->lmax = lmax;
+        h->nbin = nbin;
+        h->nt = nt;
+        h->rminmax.assign(rminmax, rminmax + 2 * nt * nt);
+        std::vector<double> thr((size_t)nt * nt * (nbin + 1));
+        for (uint32_t p = 0; p < nt * nt; p++) {
+            const double lo = rminmax[2 * p], hi = rminmax[2 * p + 1];
+            const double dr = (hi - lo) / (double)nbin;  // sphericalbase.h:43-47
+            std::vector<double> T = bin_edges(lo, hi, nbin, dr);
+            std::copy(T.begin(), T.end(), thr.begin() + (size_t)p * (nbin + 1));
+        }
+        const uint32_t L1 = lmax + 1;
+        std::vector<double> coef((size_t)L1 * L1, 0.0);
+        for (uint32_t l = 0; l <= lmax; l++)
+            for (uint32_t m = 0; m <= l; m++) coef[l * L1 + m] = sh_coeff((int)l, (int)m);
+        h->aprime.assign(tr->N, 0);
+        h->tstart.assign(nt + 1, 0);
+        uint32_t k = 0;
+        for (uint32_t t = 0; t < nt; t++) {
+            h->tstart[t] = k;
+            for (uint64_t sl = 0; sl < tr->N; sl++)
+                if ((uint32_t)tr->type_of_slot[sl] == t) h->aprime[sl] = (int32_t)k++;
+        }
+        h->tstart[nt] = k;
+        try {
+            ANL_HIP(hipMalloc(&h->d_thr, thr.size() * sizeof(double)));
+            ANL_HIP(hipMemcpy(h->d_thr, thr.data(), thr.size() * sizeof(double), hipMemcpyHostToDevice));
+            ANL_HIP(hipMalloc(&h->d_coef, coef.size() * sizeof(double)));
+            ANL_HIP(hipMemcpy(h->d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+            ANL_HIP(hipMalloc(&h->d_aprime, tr->N * sizeof(int32_t)));
+            ANL_HIP(hipMemcpy(h->d_aprime, h->aprime.data(), tr->N * sizeof(int32_t), hipMemcpyHostToDevice));
+            ANL_HIP(hipMalloc(&h->d_tstart, (nt + 1) * sizeof(uint32_t)));
+            ANL_HIP(hipMemcpy(h->d_tstart, h->tstart.data(), (nt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+        } catch (...) {
+            anl_sph_destroy(h);
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int anl_sph_destroy(anl_sph *h) {
+    if (!h) return ANL_OK;
+    (void)hipDeviceSynchronize();
+    if (h->d_thr) (void)hipFree(h->d_thr);
+    if (h->d_coef) (void)hipFree(h->d_coef);
+    if (h->d_aprime) (void)hipFree(h->d_aprime);
+    if (h->d_tstart) (void)hipFree(h->d_tstart);
+    if (h->work) (void)hipFree(h->work);
+    delete h;
+    return ANL_OK;
+}
+
+int anl_sph_set_budget(anl_sph *h, uint64_t bytes) {
+    return guard([&] {
+        if (!h) fail(ANL_ERR_ARG, "null pointer passed to anl_sph_set_budget");
+        if (bytes == 0) fail(ANL_ERR_ARG, "the density budget must be positive");
+        h->budget = bytes;
+    });
+}
+
+int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host) {
+    return guard([&] {
+        if (!h || !out_host) fail(ANL_ERR_ARG, "null pointer passed to anl_sph_snapshot");
+        if (frame >= h->tr->F) fail(ANL_ERR_ARG, "frame index out of range");
+        const uint64_t FS = frame_doubles(h);
+        double *S = (double *)workspace(h, FS * sizeof(double));
+        launch_density(h, frame, 1, S, nullptr);
+        std::vector<double> Sh(FS);
+        ANL_HIP(hipMemcpy(Sh.data(), S, FS * sizeof(double), hipMemcpyDeviceToHost));
+        // S[tj][b][l][a'][m + l] -> out[atom][tj][b][reference (l, m) order]
+        const uint32_t L1 = h->lmax + 1, L2 = L1 * L1, nt = h->nt, nbin = h->nbin;
+        const uint64_t N = h->tr->N;
+        std::vector<int32_t> orig(N);
+        ANL_HIP(hipMemcpy(orig.data(), h->tr->orig_slot, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (uint64_t sl = 0; sl < N; sl++) {
+            const uint64_t ap = (uint64_t)h->aprime[sl], atom = (uint64_t)orig[sl];
+            for (uint32_t tj = 0; tj < nt; tj++)
+                for (uint32_t b = 0; b < nbin; b++) {
+                    double *dst = out_host + ((atom * nt + tj) * nbin + b) * L2;
+                    for (uint32_t l = 0; l <= h->lmax; l++) {
+                        const double *src = Sh.data() + ((uint64_t)(tj * nbin + b) * L2 + l * l) * N + ap * (2 * l + 1);
+                        const uint32_t off = L2 - (l + 1) * (l + 1);  // l = lmax first
+                        for (uint32_t m = 1; m <= l; m++) dst[off + m - 1] = src[l - m];
+                        for (uint32_t m = 0; m <= l; m++) dst[off + l + m] = src[l + m];
+                    }
+                }
+        }
+    });
+}
+
+int anl_sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32_t skip, double *out_host) {
+    return guard([&] {
+        if (!h || !out_host) fail(ANL_ERR_ARG, "null pointer passed to anl_sph_calculate");
+        sph_calculate(h, primo, n, tmax, skip, out_host);
+    });
+}
+
+}  // extern "C"

@@ -143,6 +143,31 @@ int anl_blocks_gofrt_results(anl_blocks *b, double *mean, double *var, uint64_t 
 int anl_blocks_msd_results(anl_blocks *b, double *mean, double *var, double *per_block, uint32_t *leff);
 int anl_blocks_destroy(anl_blocks *b);
 
+/* ---- spherical-harmonic density correlations -------------------------- */
+/* Replaces SphericalCorrelations<lmax,double,T> (sphericalcorrelations.h:9-40,
+ * sphericalcorrelations.cpp:10-270) and SphericalBase::calc
+ * (sphericalbase.cpp:19-66) without neighbour lists.  rminmax holds
+ * ntypes*ntypes (rmin, rmax) pairs indexed ntypes*type_i + type_j
+ * (sphericalbase.cpp:7-16); lmax <= ANL_SPH_LMAX_MAX. */
+#define ANL_SPH_LMAX_MAX 10
+typedef struct anl_sph anl_sph;
+int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t nbin, anl_sph **out);
+int anl_sph_destroy(anl_sph *h);
+/* reset(n) + calculate(primo) (sphericalcorrelations.cpp:40-236): out_host
+ * [leff][ntypes][ntypes][nbin][lmax+1] FP64, leff = min(n, tmax) (tmax = 0:
+ * n); index k of the last axis holds l = lmax - k, as in the reference.
+ * ANL_ERR_TOO_SHORT when leff + n + primo exceeds the frames (:63-65). */
+int anl_sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32_t skip, double *out_host);
+/* SphericalBase::calc(frame) (sphericalbase.cpp:19-66): out_host
+ * [natoms][ntypes][nbin][(lmax+1)^2] in original atom order and the
+ * reference's (l, m) order (l = lmax first; m = -1..-l, then 0..l). */
+int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host);
+/* Bytes of per-frame densities held on the device at once (default 8 GiB);
+ * a smaller budget processes the t0 values in more chunks, same results up
+ * to the order of the chunk sums.  No counterpart in the reference (its
+ * CalcBuffer size, buffer_size, plays the same role on the host). */
+int anl_sph_set_budget(anl_sph *h, uint64_t bytes);
+
 /* ---- mean square displacement ----------------------------------------- */
 /* Replaces MSD<T> (msd.h:30-47, msd.cpp:40-146): out_host [leff][f_cm][ntypes]
  * FP64, f_cm = 2 when cm_msd, per-type mean over (t0 in 0,skip,..<n, atoms);

@@ -1,0 +1,129 @@
+"""Spherical-harmonic density correlations on the GPU (anl_sph_*) against the
+oracle (sphericalbase.cpp:19-66, sphericalcorrelations.cpp:62-270).
+
+FP64 throughout; the device sums in a different order than the reference
+(densities in slot order, products as a Gram matrix over (atom, m), t1 sums
+in chunks), so values agree to rounding: the tolerance is 1e-12 of each
+array's largest magnitude.  Densities and correlations are checked against
+the oracle at small sizes (orthorhombic and triclinic cells, 1-3 types,
+skip/tmax/primo variations, several density chunks), and the full
+2,048-atom workload through its own snapshots: every lag equals the numpy
+contraction of the device densities of its frames."""
+import numpy as np
+import pytest
+
+from analisi_amd import capi, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _close(got, want, tol=TOL):
+    scale = max(np.abs(want).max(), 1e-300)
+    err = np.abs(got - want).max() / scale
+    assert err <= tol, (err, scale)
+
+
+def _case(seed, N, F, L, frac, tri, sigma=0.25):
+    tilt = (2.0 * synth.uniform(seed, 7, 3) - 1.0) * 0.15 * L if tri else None
+    box_l = synth.lammps_box(L, tilt) if tri else synth.lammps_box(L)
+    pos = synth.random_walk(seed, N, F, box_l, sigma)
+    types = synth.types_by_fraction(N, frac)
+    ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), tri)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    return ot, dt
+
+
+def _ranges(nt, seed):
+    u = synth.uniform(seed, 3, 2 * nt * nt).reshape(nt * nt, 2)
+    lo = 0.4 + 0.6 * u[:, 0]
+    return np.stack([lo, lo + 2.2 + 0.8 * u[:, 1]], axis=1)
+
+
+@pytest.mark.parametrize("tri,frac", [(False, [1.0]), (False, [0.5, 0.5]), (True, [0.6, 0.3, 0.1])])
+def test_snapshot_matches_oracle(tri, frac):
+    ot, dt = _case(31, 70, 4, 8.5, frac, tri)
+    rm = _ranges(ot.ntypes, 5)
+    s = capi.DeviceSph(dt, rm, 3)
+    for f in (0, 3):
+        got = s.snapshot(f)
+        want = O.sph_density(ot, f, rm, 3)
+        assert np.count_nonzero(want) > 0
+        _close(got, want)
+
+
+@pytest.mark.parametrize("tri,frac,skip,tmax,n,primo", [
+    (False, [0.5, 0.5], 1, 4, 6, 0),
+    (False, [0.5, 0.5], 2, 3, 7, 2),
+    (True, [0.6, 0.3, 0.1], 3, 0, 5, 1),
+    (True, [1.0], 1, 1, 4, 3),
+])
+def test_correlations_match_oracle(tri, frac, skip, tmax, n, primo):
+    ot, dt = _case(41, 60, 14, 8.0, frac, tri)
+    rm = _ranges(ot.ntypes, 9)
+    s = capi.DeviceSph(dt, rm, 3)
+    got = s.calculate(n, tmax, skip, primo)
+    want = O.sph_correlations(ot, rm, 3, tmax, skip, n, primo)
+    assert got.shape == want.shape
+    _close(got, want)
+
+
+def test_chunked_densities_and_lower_lmax():
+    """A budget of a few frames splits the t1 values into many chunks; lmax 6
+    (the other instantiation of the reference) uses the same kernels."""
+    ot, dt = _case(43, 50, 40, 7.5, [0.7, 0.3], False)
+    rm = _ranges(ot.ntypes, 4)
+    for lmax in (10, 6):
+        s = capi.DeviceSph(dt, rm, 2, lmax)
+        whole = s.calculate(30, 6, 1, 2)
+        fbytes = 50 * 2 * 2 * (lmax + 1) ** 2 * 8
+        s.set_budget(9 * fbytes)          # 6 lag frames + 4 t1 values per chunk
+        chunked = s.calculate(30, 6, 1, 2)
+        _close(chunked, whole, 1e-13)
+        want = O.sph_correlations(ot, rm, 2, 6, 1, 30, 2, lmax)
+        _close(chunked, want)
+
+
+def test_errors():
+    ot, dt = _case(47, 20, 8, 7.0, [1.0], False)
+    s = capi.DeviceSph(dt, [(0.5, 3.0)], 2)
+    with pytest.raises(capi.AnlError, match="too short") as e:
+        s.calculate(5, 3, 1, 1)           # 3 + 5 + 1 > 8
+    assert e.value.code == capi.ANL_ERR_TOO_SHORT
+    s.calculate(5, 3, 1, 0)
+    with pytest.raises(capi.AnlError, match="rmax > rmin"):
+        capi.DeviceSph(dt, [(3.0, 0.5)], 2)
+    with pytest.raises(capi.AnlError, match="lmax"):
+        capi.DeviceSph(dt, [(0.5, 3.0)], 2, 11)
+
+
+def test_fullsize_lags_equal_snapshot_contractions():
+    """2,048 atoms (2 types), 0.1 atoms/A^3, 4 bins in [0.7, 3.5) A, lmax 10:
+    C[dt][ti][tj][b][lmax-l] = mean over t1 and over atoms of type ti of
+    sum_m S_t1 S_t1+dt, with S the device's own snapshots."""
+    N, L = 2048, (2048 / 0.1) ** (1 / 3)
+    ot, dt = _case(53, N, 40, L, [0.5, 0.5], False, 0.1)
+    rm = [(0.7, 3.5)] * 4
+    s = capi.DeviceSph(dt, rm, 4)
+    n, tmax, skip, primo = 20, 8, 3, 5
+    got = s.calculate(n, tmax, skip, primo)
+    snaps = {f: s.snapshot(f) for f in range(primo, primo + n + tmax - 1)}
+    # neighbours per atom near 0.1 * 4/3 pi (3.5^3 - 0.7^3): the bins are populated
+    counts = snaps[primo][:, :, :, -1] * np.sqrt(4 * np.pi)
+    assert 12 < counts.sum(axis=(1, 2)).mean() < 24
+    t1s = list(range(0, n, skip))
+    ti = ot.type_id
+    lsz = [2 * l + 1 for l in range(10, -1, -1)]
+    edges = np.concatenate([[0], np.cumsum(lsz)])
+    want = np.zeros_like(got)
+    for d in range(tmax):
+        for t1 in t1s:
+            p = snaps[primo + t1] * snaps[primo + t1 + d]
+            msum = np.stack([p[..., edges[k]:edges[k + 1]].sum(axis=-1) for k in range(11)], axis=-1)
+            for t in range(2):
+                want[d, t] += msum[ti == t].mean(axis=0)
+        want[d] /= len(t1s)
+    _close(got, want, 1e-11)
+    assert np.all(got[0, ..., 10] > 0)
