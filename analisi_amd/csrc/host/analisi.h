@@ -12,6 +12,7 @@
 //   ReadLog_numpy<TFLOAT>             readlog_numpy.{h,cpp}
 //   Gofrt<TFLOAT,T>                   gofrt.{h,cpp}
 //   MSD<T,FPE>                        msd.{h,cpp}
+//   SphericalCorrelations<T>          sphericalcorrelations.{h,cpp}, sphericalbase.{h,cpp}
 //   GreenKuboNComponentIonicFluid<..> greenkuboNcomponentionicfluid.{h,cpp}
 //   MediaVar<T>, MediaVarCovar<T>     calcoliblocchi.h
 //   BlockAverageG<TR,T,Args...>       blockaverage.h:83-218
@@ -636,6 +637,131 @@ class MSD : public VectorOp<MSD<T, FPE>> {
     size_t lmax, f_cm, ntypes = 0, leff = 0, ntimesteps = 0;
     unsigned int skip;
     bool cm_msd, cm_self, debug;
+};
+
+// ---------------------------------------------------------------------------
+// SphericalCorrelations (sphericalcorrelations.h:9-78, sphericalcorrelations.cpp)
+// over SphericalBase (sphericalbase.h:6-60): the time correlation of the
+// per-atom real spherical-harmonic densities of each neighbour type and
+// distance bin.  vdata [leff][ntypes][ntypes][nbin][lmax+1], index k of the
+// last axis holding l = lmax - k.  Neighbour lists (the SANN variant) are not
+// supported: a non-empty neighList is an error.
+// ---------------------------------------------------------------------------
+template <class T>
+class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
+    using VOp = VectorOp<SphericalCorrelations<T>>;
+    using VOp::data_length;
+    using VOp::vdata;
+
+   public:
+    using rminmax_t = std::vector<std::pair<double, double>>;
+    using NeighListSpec = std::vector<std::tuple<size_t, double, double>>;  // neighbour.h:13
+    SphericalCorrelations(T *t, const rminmax_t rminmax, size_t nbin, size_t tmax = 0, size_t nthreads = 0,
+                          size_t skip = 1, size_t buffer_size = 10, bool debug = false,
+                          const NeighListSpec neighList = {}, unsigned int lmax = ANL_SPH_LMAX_MAX)
+        : traiettoria(t), rminmax(rminmax), nbin(nbin), tmax(tmax), nthreads(nthreads), skip(skip == 0 ? 1 : skip),
+          buffer_size(buffer_size), lmax(lmax), ntypes(t->get_ntypes()) {
+        (void)debug;
+        if ((size_t)ntypes * ntypes != rminmax.size()) {  // sphericalbase.cpp:7-16
+            std::stringstream ss;
+            ss << "you must provide a radial range for each pair of atomic types, in total ntypes*ntypes pair of "
+                  "numbers. You provided "
+               << rminmax.size() << " elements while ntypes is " << ntypes << " .";
+            throw std::runtime_error(ss.str());
+        }
+        if (!neighList.empty())
+            throw std::runtime_error("neighbour lists (the SANN variant of SphericalCorrelations) are not supported");
+        if (t->get_nloaded_timesteps() > 0) h();  // argument errors surface here
+    }
+    ~SphericalCorrelations() {
+        if (hs) anl_sph_destroy(hs);
+    }
+    SphericalCorrelations &operator=(const SphericalCorrelations &destra) {
+        VOp::operator=(destra);
+        return *this;
+    }
+    unsigned int nExtraTimesteps(unsigned int n_b) {  // sphericalcorrelations.cpp:32-35
+        size_t v = traiettoria->get_ntimesteps() / (n_b + 1) + 1;
+        return (unsigned int)((v < tmax || tmax == 0) ? v : tmax);
+    }
+    void reset(const unsigned int n) {  // sphericalcorrelations.cpp:37-59
+        ntimesteps = n;
+        leff = (n < tmax || tmax == 0) ? n : tmax;
+        data_length = leff * ntypes * ntypes * nbin * (lmax + 1);
+        delete[] vdata;
+        vdata = new double[data_length];
+    }
+    // sphericalcorrelations.cpp:62-236: the argument checks of the reference,
+    // then the whole (t0, lag) nest in one device call
+    void calculate(unsigned int primo) {
+        if (leff + ntimesteps + primo > traiettoria->get_ntimesteps())
+            throw std::runtime_error(
+                "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
+                "the size of the average or the lenght of the correlation function");
+        if (nthreads <= 1) {
+            std::cerr << "Warning: I'm using a single thread.\n";
+            nthreads = 1;
+        }
+        if (buffer_size < 2) {
+            std::cerr << "Warning: buffer size cannot be less than 2, setting it to 2" << std::endl;
+            buffer_size = 2;
+        }
+        if (leff / nthreads == 0)
+            throw std::runtime_error(
+                "Don't have enough timesteps to parallelize the calculation. Try again with a smaller number of "
+                "threads. ");
+        const size_t off = traiettoria->window_offset(primo, ntimesteps + leff - 1);
+        anl_check(anl_sph_calculate(h(), off, (uint32_t)ntimesteps, (uint32_t)tmax, (uint32_t)skip, vdata));
+    }
+    std::vector<ssize_t> get_shape() const {
+        return {(ssize_t)leff, (ssize_t)ntypes, (ssize_t)ntypes, (ssize_t)nbin, (ssize_t)(lmax + 1)};
+    }
+    std::vector<ssize_t> get_stride() const {  // sphericalcorrelations.h:40-49: C order
+        auto s = get_shape();
+        std::vector<ssize_t> st(s.size());
+        ssize_t acc = sizeof(double);
+        for (int i = (int)s.size() - 1; i >= 0; --i) {
+            st[i] = acc;
+            acc *= s[i];
+        }
+        return st;
+    }
+    std::string get_columns_description() { return "#TODO\n"; }
+    // SphericalBase::calc(timestep) without counters (sphericalbase.cpp:19-66):
+    // [natoms][ntypes][nbin][(lmax+1)^2], the reference's (l, m) order
+    std::vector<double> calc(size_t timestep) {
+        const size_t off = traiettoria->window_offset(timestep, 1);
+        std::vector<double> out((size_t)traiettoria->get_natoms() * ntypes * nbin * (lmax + 1) * (lmax + 1));
+        anl_check(anl_sph_snapshot(h(), off, out.data()));
+        return out;
+    }
+    size_t get_snap_size() const { return (size_t)traiettoria->get_natoms() * ntypes * nbin * (lmax + 1) * (lmax + 1); }
+
+   private:
+    // device handle over the trajectory's current device mirror; rebuilt when
+    // a windowed trajectory has moved its window
+    anl_sph *h() {
+        if (!hs || gen != traiettoria->generation()) {
+            anl_traj *d = traiettoria->device();
+            if (hs) anl_sph_destroy(hs);
+            hs = nullptr;
+            std::vector<double> rm;
+            for (const auto &p : rminmax) {
+                rm.push_back(p.first);
+                rm.push_back(p.second);
+            }
+            anl_check(anl_sph_create(d, lmax, rm.data(), (uint32_t)nbin, &hs));
+            gen = traiettoria->generation();
+        }
+        return hs;
+    }
+    T *traiettoria;
+    rminmax_t rminmax;
+    size_t nbin, tmax, nthreads, skip, buffer_size;
+    unsigned int lmax;
+    size_t ntypes, leff = 0, ntimesteps = 0;
+    anl_sph *hs = nullptr;
+    uint64_t gen = 0;
 };
 
 // ---------------------------------------------------------------------------

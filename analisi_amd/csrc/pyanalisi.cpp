@@ -162,6 +162,25 @@ void define_atomic_traj(py::module &m, const std::string &suffix) {
         .def_buffer([](M &x) { return vbuffer(x); })
         .def("__enter__", [](M &x) -> M & { return x; })
         .def("__exit__", [](M &, py::object, py::object, py::object) {});
+
+    // SphericalCorrelations (pyanalisi.cpp:85-111), lmax 10 by default
+    using S = SphericalCorrelations<T>;
+    py::class_<S>(m, ("SphericalCorrelations" + suffix).c_str(), py::buffer_protocol())
+        .def(py::init<T *, typename S::rminmax_t, size_t, size_t, size_t, size_t, size_t, bool,
+                      typename S::NeighListSpec, unsigned int>(),
+             py::arg("trajectory"), py::arg("rminmax"), py::arg("nbin"), py::arg("tmax") = 0, py::arg("nthreads") = 0,
+             py::arg("skip") = 1, py::arg("buffer_size") = 10, py::arg("debug") = false,
+             py::arg("neighList") = typename S::NeighListSpec{}, py::arg("lmax") = (unsigned int)ANL_SPH_LMAX_MAX)
+        .def("reset", &S::reset)
+        .def("calculate", &S::calculate)
+        .def("getNumberOfExtraTimestepsNeeded", &S::nExtraTimesteps)
+        .def("get_columns_description", &S::get_columns_description)
+        .def("calc",
+             [](S &s, size_t timestep) {
+                 std::vector<double> v = s.calc(timestep);
+                 return copy_array(v.data(), {(ssize_t)v.size()});
+             })
+        .def_buffer([](S &s) { return vbuffer(s); });
 }
 
 // gk (pyanalisi.cpp:241-279): GreenKubo over one log class

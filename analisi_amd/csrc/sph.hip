@@ -41,7 +41,7 @@ namespace anl {
 #include "min_image.h"
 
 constexpr int SPH_WAVES = 4;           // wavefronts (central atoms) per density workgroup
-constexpr uint32_t SPH_WAVE_LDS = 1024;  // doubles of density block per wavefront (8 KiB)
+constexpr uint32_t SPH_WAVE_LDS = 2048;  // doubles of LDS per wavefront (16 KiB): density block + staging
 constexpr int GT = 64;                 // Gram tile edge (t1 rows, t2 columns)
 constexpr int GKC = 16;                // K values staged per step
 
@@ -63,52 +63,95 @@ struct SphDensArgs {
     uint32_t nt, nbin, lmax;
     uint32_t nbc;              // bins per launch column (blockIdx.y)
     uint32_t ngroups;          // ceil(N / SPH_WAVES)
+    double d2lo, d2hi;         // min over type pairs of T[0], max of T[nbin]
+    double rcut;               // > sqrt(d2hi), with a rounding margin
+    const double4 *block_rad;  // [nfr][ceil(N/64)] {centre, radius} of each 64-slot block, or null (triclinic)
 };
+
+// bounding sphere of each 64-slot block of each frame (orthorhombic cells):
+// centre = first atom + mean minimum-image offset of the block's atoms, radius
+// = largest minimum-image distance from the centre, rounded up by a relative
+// 1e-12; one wavefront per (frame, block), out[.] = {centre, radius}
+__global__ __launch_bounds__(64) void sph_block_radius_kernel(const double4 *pos, const double *box, uint64_t N,
+                                                              uint64_t f0, double4 *out) {
+    const uint64_t nblk = (N + 63) / 64;
+    const uint64_t fl = blockIdx.x / nblk, blk = blockIdx.x - fl * nblk;
+    const uint64_t f = f0 + fl;
+    const double4 *P = pos + f * N;
+    const DevCell c = load_cell(box + f * 9);
+    const uint64_t j = blk * 64 + threadIdx.x;
+    const uint64_t nin = min((uint64_t)64, N - blk * 64);
+    const double4 r = P[blk * 64];
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    if (j < N) {
+        const double4 pj = P[j];
+        x0 = pj.x - r.x;
+        x1 = pj.y - r.y;
+        x2 = pj.z - r.z;
+        min_image_ref<false>(x0, x1, x2, c);
+    }
+    double s0 = x0, s1 = x1, s2 = x2;
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+    }
+    const double c0 = r.x + s0 / (double)nin, c1 = r.y + s1 / (double)nin, c2 = r.z + s2 / (double)nin;
+    double d = 0.0;
+    if (j < N) {
+        const double4 pj = P[j];
+        double y0 = pj.x - c0, y1 = pj.y - c1, y2 = pj.z - c2;
+        min_image_ref<false>(y0, y1, y2, c);
+        d = sqrt(d2_of(y0, y1, y2));
+    }
+    for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o));
+    if (threadIdx.x == 0) out[blockIdx.x] = make_double4(c0, c1, c2, d * (1.0 + 1e-12) + 1e-12);
+}
 
 // Lane m's share of the real harmonics of direction (x, y, z)
 // (specialfunctions.h:11-22, 262-360): P_l^m for l = m..lmax and
-// cos/sin(m phi), added into acc[l*l + l + m] (cos) and acc[l*l + l - m] (sin).
-__device__ __forceinline__ void add_harmonics(double *acc, double x, double y, double z, int m, int lmax,
-                                              const double *coef) {
-    const double rxy = sqrt(x * x + y * y);
-    const double r = sqrt(x * x + y * y + z * z);
-    const double ct = z / r, sp = y / rxy, cp = x / rxy;
+// cos/sin(m phi), written to y[l*l + l + m] (cos) and y[l*l + l - m] (sin);
+// cf[l] is the coefficient of (l, m).  The loops run to the largest lmax with
+// the lane's own range as a predicate, so cf stays in registers.
+__device__ __forceinline__ void put_harmonics(double *y, double x0, double x1, double x2, int m, int lmax,
+                                              const double (&cf)[ANL_SPH_LMAX_MAX + 1]) {
+    const double rxy = sqrt(x0 * x0 + x1 * x1);
+    const double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+    const double ct = x2 / r, sp = x1 / rxy, cp = x0 / rxy;
+    const double st = sqrt(1 - ct * ct);
     double pmm = 1.0;
-    if (m > 0) {
-        const double st = sqrt(1 - ct * ct);
-        for (int l = 1; l <= m; ++l) pmm = (double)(-(2 * l - 1)) * st * pmm;
-    }
-    double cm = 1.0, sm = 0.0;
-    if (m > 0) {
-        double c2 = 1.0, s2 = 0.0, c1 = cp, s1 = sp;
-        for (int k = 2; k <= m; ++k) {
-            const double cn = 2 * cp * c1 - c2;
-            const double sn = 2 * cp * s1 - s2;
-            c2 = c1;
-            s2 = s1;
-            c1 = cn;
-            s1 = sn;
-        }
-        cm = c1;
-        sm = s1;
-    }
-    const int L1 = lmax + 1;
+#pragma unroll
+    for (int l = 1; l <= ANL_SPH_LMAX_MAX; ++l)
+        if (l <= m) pmm = (double)(-(2 * l - 1)) * st * pmm;
+    double cm = m > 0 ? cp : 1.0, sm = m > 0 ? sp : 0.0;
     {
-        const double v = pmm * coef[m * L1 + m];
-        acc[m * m + m + m] += v * cm;
-        if (m > 0) acc[m * m] += v * sm;
-    }
-    if (m < lmax) {
-        double p2 = pmm, p1 = pmm * ct * (2 * m + 1);
-        for (int l = m + 1; l <= lmax; ++l) {
-            if (l > m + 1) {
-                const double p = (p1 * ct * (2 * l - 1) - p2 * (l + m - 1)) / (l - m);
-                p2 = p1;
-                p1 = p;
+        double c2 = 1.0, s2 = 0.0, c1 = cp, s1 = sp;
+#pragma unroll
+        for (int k = 2; k <= ANL_SPH_LMAX_MAX; ++k)
+            if (k <= m) {
+                const double cn = 2 * cp * c1 - c2;
+                const double sn = 2 * cp * s1 - s2;
+                c2 = c1;
+                s2 = s1;
+                c1 = cn;
+                s1 = sn;
+                cm = c1;
+                sm = s1;
             }
-            const double v = p1 * coef[l * L1 + m];
-            acc[l * l + l + m] += v * cm;
-            if (m > 0) acc[l * l + l - m] += v * sm;
+    }
+    double p1 = 0.0, p2 = 0.0;  // P_{l-1}^m, P_{l-2}^m
+#pragma unroll
+    for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l) {
+        if (l >= m && l <= lmax) {
+            double p;
+            if (l == m) p = pmm;
+            else if (l == m + 1) p = pmm * ct * (2 * m + 1);
+            else p = (p1 * ct * (2 * l - 1) - p2 * (l + m - 1)) / (l - m);
+            const double v = p * cf[l];
+            y[l * l + l + m] = v * cm;
+            if (m > 0) y[l * l + l - m] = v * sm;
+            p2 = p1;
+            p1 = p;
         }
     }
 }
@@ -117,9 +160,16 @@ template <bool TRI>
 __global__ __launch_bounds__(256) void sph_density_kernel(SphDensArgs a) {
     extern __shared__ double sph_lds[];
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t L2 = (a.lmax + 1) * (a.lmax + 1);
+    const uint32_t L1 = a.lmax + 1, L2 = L1 * L1;
     const uint32_t accn = a.nt * a.nbc * L2;
-    double *acc = sph_lds + wv * accn;
+    const uint32_t nc = 64 / L1;                       // candidates expanded at once
+    const uint32_t slot = lane / L1, m = lane - slot * L1;
+    double *acc = sph_lds + wv * (accn + nc * L2);
+    double *stage = acc + accn;                        // [nc][L2]
+    double cf[ANL_SPH_LMAX_MAX + 1];                   // coefficients of (l, m) of this lane's m
+#pragma unroll
+    for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l)
+        cf[l] = (l >= (int)m && l <= (int)a.lmax) ? a.coef[l * L1 + m] : 0.0;
     const uint32_t fl = blockIdx.x / a.ngroups;
     const uint64_t i = (uint64_t)(blockIdx.x % a.ngroups) * SPH_WAVES + wv;
     if (i >= a.N) return;  // the whole wavefront; no workgroup barrier below
@@ -134,37 +184,70 @@ __global__ __launch_bounds__(256) void sph_density_kernel(SphDensArgs a) {
     const double4 pi = P[i];
     const int ti = a.type_slot[i];
     const double *thr_i = a.thr + (size_t)ti * a.nt * (a.nbin + 1);
-    for (uint64_t jb = 0; jb < a.N; jb += 64) {
-        const uint64_t j = jb + lane;
-        bool in = false;
-        double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-        int tj = 0, bin = 0;
-        if (j < a.N && j != i) {
-            const double4 pj = P[j];
-            x0 = pi.x - pj.x;
-            x1 = pi.y - pj.y;
-            x2 = pi.z - pj.z;
-            min_image_ref<TRI>(x0, x1, x2, c);
-            const double d2 = d2_of(x0, x1, x2);
-            tj = a.type_slot[j];
-            const double *T = thr_i + (size_t)tj * (a.nbin + 1);
-            if (d2 >= T[b0] && d2 < T[b1]) {
-                in = true;
-                uint32_t b = b0;
-                while (b + 1 < b1 && d2 >= T[b + 1]) ++b;
-                bin = (int)b;
-            }
+    const double4 *rad = a.block_rad ? a.block_rad + (uint64_t)fl * ((a.N + 63) / 64) : nullptr;
+    const uint64_t nblk = (a.N + 63) / 64;
+    // blocks of 64 slots, 64 blocks at a time: lane b decides whether block b
+    // can hold an atom within rcut of atom i.  In orthorhombic cells the
+    // minimum image is the nearest one, a metric, so a block whose bounding
+    // sphere's centre is farther than rcut + its radius cannot.
+    for (uint64_t g = 0; g < nblk; g += 64) {
+        bool near = g + lane < nblk;
+        if (rad && near) {
+            const double4 bc = rad[g + lane];
+            double x0 = pi.x - bc.x, x1 = pi.y - bc.y, x2 = pi.z - bc.z;
+            min_image_ref<false>(x0, x1, x2, c);
+            near = !(sqrt(d2_of(x0, x1, x2)) > a.rcut + bc.w);
         }
-        unsigned long long mask = __ballot(in);
-        while (mask) {
-            const int src = __ffsll(mask) - 1;  // lowest lane first: slot order
-            mask &= mask - 1;
-            const double y0 = __shfl(x0, src), y1 = __shfl(x1, src), y2 = __shfl(x2, src);
-            const int jt = __shfl(tj, src), bb = __shfl(bin, src);
-            // lane m owns the entries (l, +-m): no two lanes touch one address
-            if (lane <= a.lmax)
-                add_harmonics(acc + ((uint32_t)jt * a.nbc + (uint32_t)bb - b0) * L2, y0, y1, y2, (int)lane,
-                              (int)a.lmax, a.coef);
+        unsigned long long blocks = __ballot(near);
+        while (blocks) {
+            const uint64_t jb = (g + (uint64_t)(__ffsll(blocks) - 1)) * 64;
+            blocks &= blocks - 1;
+            const uint64_t j = jb + lane;
+            bool in = false;
+            double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+            int tj = 0, bin = 0;
+            if (j < a.N && j != i) {
+                const double4 pj = P[j];
+                x0 = pi.x - pj.x;
+                x1 = pi.y - pj.y;
+                x2 = pi.z - pj.z;
+                min_image_ref<TRI>(x0, x1, x2, c);
+                const double d2 = d2_of(x0, x1, x2);
+                if (d2 >= a.d2lo && d2 < a.d2hi) {  // outer bounds over every type pair
+                    tj = a.type_slot[j];
+                    const double *T = thr_i + (size_t)tj * (a.nbin + 1);
+                    if (d2 >= T[b0] && d2 < T[b1]) {
+                        in = true;
+                        uint32_t b = b0;
+                        while (b + 1 < b1 && d2 >= T[b + 1]) ++b;
+                        bin = (int)b;
+                    }
+                }
+            }
+            unsigned long long mask = __ballot(in);
+            // candidates in slot order, nc at a time: lane (slot, m) expands the
+            // slot-th one into its staging block, then the blocks are added into
+            // the density in slot order (each entry by one lane, in order)
+            while (mask) {
+                int src = -1;
+                uint32_t nb = 0;
+                for (; nb < nc && mask; ++nb) {
+                    const int k = __ffsll(mask) - 1;
+                    mask &= mask - 1;
+                    if (slot == nb) src = k;
+                }
+                const int sl = src < 0 ? 0 : src;
+                const double y0 = __shfl(x0, sl), y1 = __shfl(x1, sl), y2 = __shfl(x2, sl);
+                const int jt = __shfl(tj, sl), bb = __shfl(bin, sl);
+                if (src >= 0) put_harmonics(stage + slot * L2, y0, y1, y2, (int)m, (int)a.lmax, cf);
+                wave_sync_lds();
+                for (uint32_t k = 0; k < nb; ++k) {
+                    const uint32_t jk = (uint32_t)__shfl(jt, (int)(k * L1)), bk = (uint32_t)__shfl(bb, (int)(k * L1));
+                    double *blk = acc + (jk * a.nbc + bk - b0) * L2;
+                    for (uint32_t e = lane; e < L2; e += 64) blk[e] += stage[k * L2 + e];
+                }
+                wave_sync_lds();
+            }
         }
     }
     wave_sync_lds();
@@ -192,9 +275,14 @@ struct SphGramArgs {
     uint32_t ncol;            // column tiles per row tile
 };
 
+// RT t1 rows x CT = 4096/RT t2 columns per workgroup, 4x4 per thread
+template <int RT>
 __global__ __launch_bounds__(256) void sph_gram_kernel(SphGramArgs a) {
-    __shared__ double As[2][GKC][GT];
-    __shared__ double Bs[2][GKC][GT];
+    constexpr int CT = 4096 / RT;
+    constexpr int TC = CT / 4;                      // threads along the columns
+    constexpr int VA = RT / 16, VB = CT / 16;       // K values each thread stages per step
+    __shared__ double As[2][GKC][RT];
+    __shared__ double Bs[2][GKC][CT];
     const uint32_t L1 = a.lmax + 1, L2 = L1 * L1;
     uint32_t y = blockIdx.y;
     const uint32_t l = y % L1;
@@ -203,11 +291,11 @@ __global__ __launch_bounds__(256) void sph_gram_kernel(SphGramArgs a) {
     y /= a.nbin;
     const uint32_t jt = y % a.nt, ti = y / a.nt;
     const uint32_t rt = blockIdx.x / a.ncol, ct = blockIdx.x - rt * a.ncol;
-    const uint32_t e0 = rt * GT;
-    const uint32_t rows = min((uint32_t)GT, a.n0c - e0);
+    const uint32_t e0 = rt * RT;
+    const uint32_t rows = min((uint32_t)RT, a.n0c - e0);
     const uint64_t t1_first = (uint64_t)e0 * a.skip;
     const uint64_t t1_last = (uint64_t)(e0 + rows - 1) * a.skip;
-    const uint64_t c0 = t1_first + (uint64_t)ct * GT;
+    const uint64_t c0 = t1_first + (uint64_t)ct * CT;
     if (c0 >= a.nfr || c0 > t1_last + a.leff - 1) return;  // the whole workgroup
 
     const uint32_t w = 2 * l + 1;
@@ -215,26 +303,32 @@ __global__ __launch_bounds__(256) void sph_gram_kernel(SphGramArgs a) {
     const uint64_t base = ((uint64_t)(jt * a.nbin + bin) * L2 + l * l) * a.N + (uint64_t)a.tstart[ti] * w;
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t lr = tid >> 2, kq = (tid & 3) * 4;  // loader: row/column lr, k offset kq
-    const uint32_t ea = e0 + lr;
+    // loaders: thread stages VA consecutive K values of row ar and VB of column bc
+    const uint32_t ar = tid / (16 / VA), ak = (tid % (16 / VA)) * VA;
+    const uint32_t bc = tid / (16 / VB), bk = (tid % (16 / VB)) * VB;
+    const uint32_t ea = e0 + ar;
     const bool va = ea < a.n0c;
-    const uint64_t fb = c0 + lr;
+    const uint64_t fb = c0 + bc;
     const bool vb = fb < a.nfr;
     const double *Ap = a.S + (va ? (uint64_t)ea * a.skip : 0) * a.FS + base;
     const double *Bp = a.S + (vb ? fb : 0) * a.FS + base;
-    const uint32_t tr = tid >> 4, tc = tid & 15;  // compute: rows 4tr.., columns 4tc..
+    const uint32_t tr = tid / TC, tc = tid % TC;   // compute: rows 4tr.., columns 4tc..
 
     double acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    double ra[4], rb[4];
+    double ra[VA], rb[VB];
     auto load = [&](uint64_t kc) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint64_t k = kc + kq + u;
+        for (int u = 0; u < VA; ++u) {
+            const uint64_t k = kc + ak + u;
             ra[u] = (va && k < K) ? Ap[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < VB; ++u) {
+            const uint64_t k = kc + bk + u;
             rb[u] = (vb && k < K) ? Bp[k] : 0.0;
         }
     };
@@ -242,10 +336,9 @@ __global__ __launch_bounds__(256) void sph_gram_kernel(SphGramArgs a) {
     uint32_t buf = 0;
     for (uint64_t kc = 0; kc < K; kc += GKC) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            As[buf][kq + u][lr] = ra[u];
-            Bs[buf][kq + u][lr] = rb[u];
-        }
+        for (int u = 0; u < VA; ++u) As[buf][ak + u][ar] = ra[u];
+#pragma unroll
+        for (int u = 0; u < VB; ++u) Bs[buf][bk + u][bc] = rb[u];
         __syncthreads();
         if (kc + GKC < K) load(kc + GKC);
 #pragma unroll
@@ -322,7 +415,10 @@ struct anl_sph {
     double *d_coef = nullptr;         // [(lmax+1)^2]
     int32_t *d_aprime = nullptr;
     uint32_t *d_tstart = nullptr;
-    uint64_t budget = 8ull << 30;     // bytes of densities held at once
+    uint64_t budget = 0;              // bytes of densities held at once (set at creation)
+    double d2lo = 0.0, d2hi = 0.0;    // min over type pairs of T[0], max of T[nbin]
+    double rcut = 0.0;                // distance bound of the block screen
+    bool block_screen = false;        // orthorhombic cell and finite rcut
     void *work = nullptr;             // densities, G, R (grow-only)
     size_t work_cap = 0;
 };
@@ -347,16 +443,36 @@ void *workspace(anl_sph *h, size_t bytes) {
     return h->work;
 }
 
-// bins per density launch column so that a wavefront's block fits its LDS
-uint32_t bins_per_launch(const anl_sph *h) {
-    const uint32_t per_bin = h->nt * (h->lmax + 1) * (h->lmax + 1);
-    const uint32_t cap = per_bin > SPH_WAVE_LDS ? 2 * SPH_WAVE_LDS : SPH_WAVE_LDS;
-    return std::min(h->nbin, std::max(1u, cap / per_bin));
+// bins per density launch column: a wavefront's density block for them plus
+// the staging of 64/(lmax+1) harmonic vectors fits its share of LDS
+uint32_t stage_doubles(uint32_t lmax) { return 64 / (lmax + 1) * (lmax + 1) * (lmax + 1); }
+uint32_t bins_per_launch(uint32_t nt, uint32_t nbin, uint32_t lmax) {
+    const uint32_t per_bin = nt * (lmax + 1) * (lmax + 1);
+    const uint32_t room = SPH_WAVE_LDS - stage_doubles(lmax);
+    return std::min(nbin, room / per_bin);
 }
 
 // densities of frames [f0, f0 + nfr) of the device trajectory into S
-void launch_density(anl_sph *h, uint64_t f0, uint32_t nfr, double *S, hipStream_t s) {
+// bytes of block bounding spheres for nfr frames (none for triclinic cells)
+uint64_t radius_bytes(const anl_sph *h, uint64_t nfr) {
+    return h->block_screen ? nfr * ((h->tr->N + 63) / 64) * sizeof(double4) : 0;
+}
+
+// densities of frames [f0, f0 + nfr) of the device trajectory into S; rad
+// holds radius_bytes(h, nfr)
+void launch_density(anl_sph *h, uint64_t f0, uint32_t nfr, double *S, double4 *rad, hipStream_t s) {
     SphDensArgs a{};
+    a.d2lo = h->d2lo;
+    a.d2hi = h->d2hi;
+    a.rcut = h->rcut;
+    a.block_rad = h->block_screen ? rad : nullptr;
+    if (a.block_rad) {
+        const uint64_t nb = (uint64_t)nfr * ((h->tr->N + 63) / 64);
+        if (nb > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom block) pairs in one launch");
+        hipLaunchKernelGGL(sph_block_radius_kernel, dim3((uint32_t)nb), dim3(64), 0, s, h->tr->pos, h->tr->box,
+                           h->tr->N, f0, rad);
+        ANL_HIP(hipGetLastError());
+    }
     a.pos = h->tr->pos;
     a.box = h->tr->box;
     a.type_slot = h->tr->type_slot;
@@ -370,10 +486,10 @@ void launch_density(anl_sph *h, uint64_t f0, uint32_t nfr, double *S, hipStream_
     a.nt = h->nt;
     a.nbin = h->nbin;
     a.lmax = h->lmax;
-    a.nbc = bins_per_launch(h);
+    a.nbc = bins_per_launch(h->nt, h->nbin, h->lmax);
     a.ngroups = (uint32_t)((a.N + SPH_WAVES - 1) / SPH_WAVES);
     const uint32_t accn = a.nt * a.nbc * (a.lmax + 1) * (a.lmax + 1);
-    const size_t lds = (size_t)SPH_WAVES * accn * sizeof(double);
+    const size_t lds = (size_t)SPH_WAVES * (accn + stage_doubles(a.lmax)) * sizeof(double);
     const uint64_t gx = (uint64_t)nfr * a.ngroups;
     if (gx > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom) pairs in one density launch");
     if ((h->nbin + a.nbc - 1) / a.nbc > 65535) fail(ANL_ERR_ARG, "too many distance bins");
@@ -400,14 +516,20 @@ void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32
     const uint64_t fbytes = frame_doubles(h) * sizeof(double);
     const uint64_t maxfr = std::max<uint64_t>(1, h->budget / std::max<uint64_t>(fbytes, 1));
     if (leff > maxfr) fail(ANL_ERR_ARG, "the densities of one lag window do not fit the density budget");
-    uint64_t rc = std::min<uint64_t>(n0, (maxfr - leff) / skip + 1);
-    if (rc >= (uint64_t)GT) rc = rc / GT * GT;
+    // t1 values per chunk: as many as the budget holds, spread evenly over
+    // the chunks, in whole row tiles where that still fits
+    const uint64_t rc_max = std::min<uint64_t>(n0, (maxfr - leff) / skip + 1);
+    const uint64_t nchunks = (n0 + rc_max - 1) / rc_max;
+    uint64_t rc = (n0 + nchunks - 1) / nchunks;
+    if (rc > (uint64_t)GT) rc = std::min<uint64_t>(rc_max, (rc + GT - 1) / GT * GT);
     const uint32_t nprob = (uint32_t)fin;
     if (nprob > 65535) fail(ANL_ERR_ARG, "too many (type, type, bin, l) channels for one launch (at most 65535)");
     const uint64_t nfr_max = (rc - 1) * skip + leff;
     const uint64_t offS = 0, offG = a256(nfr_max * fbytes), offR = offG + a256((uint64_t)nprob * rc * leff * 8);
-    char *w = (char *)workspace(h, offR + a256((uint64_t)nprob * leff * 8));
+    const uint64_t offQ = offR + a256((uint64_t)nprob * leff * 8);
+    char *w = (char *)workspace(h, offQ + a256(radius_bytes(h, nfr_max)));
     double *S = (double *)(w + offS), *G = (double *)(w + offG), *R = (double *)(w + offR);
+    double4 *rad = (double4 *)(w + offQ);
     hipStream_t s = nullptr;
     const uint64_t nR = (uint64_t)nprob * leff;
     hipLaunchKernelGGL(sph_zero_kernel, dim3((uint32_t)((nR + 255) / 256)), dim3(256), 0, s, R, nR);
@@ -416,7 +538,7 @@ void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32
         const uint32_t n0c = (uint32_t)std::min<uint64_t>(rc, n0 - e0);
         const uint64_t t1_first = e0 * skip;
         const uint32_t nfr = (uint32_t)((uint64_t)(n0c - 1) * skip + leff);
-        launch_density(h, primo + t1_first, nfr, S, s);
+        launch_density(h, primo + t1_first, nfr, S, rad, s);
         SphGramArgs g{};
         g.S = S;
         g.G = G;
@@ -430,10 +552,14 @@ void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32
         g.skip = skip;
         g.leff = leff;
         g.nfr = nfr;
-        const uint64_t span = std::min<uint64_t>((uint64_t)(std::min<uint32_t>(n0c, GT) - 1) * skip + leff, nfr);
-        g.ncol = (uint32_t)((span + GT - 1) / GT);
-        const uint32_t nrt = (n0c + GT - 1) / GT;
-        hipLaunchKernelGGL(sph_gram_kernel, dim3(nrt * g.ncol, nprob), dim3(256), 0, s, g);
+        // 64 x 64 tiles.  32 x 128 tiles compute fewer entries outside the
+        // lag band at t0 stride > 1, but ran 1.4x slower per launch on the
+        // 2,048-atom stride-10 workload (26.1 vs 18.8 ms)
+        constexpr uint32_t RT = 64;
+        const uint64_t span = std::min<uint64_t>((std::min<uint64_t>(n0c, RT) - 1) * skip + leff, nfr);
+        g.ncol = (uint32_t)((span + (4096 / RT) - 1) / (4096 / RT));
+        const uint32_t nrt = (n0c + RT - 1) / RT;
+        hipLaunchKernelGGL(sph_gram_kernel<RT>, dim3(nrt * g.ncol, nprob), dim3(256), 0, s, g);
         ANL_HIP(hipGetLastError());
         hipLaunchKernelGGL(sph_reduce_kernel, dim3((uint32_t)((nR + 255) / 256)), dim3(256), 0, s, G, R, nprob,
                            n0c, leff);
@@ -465,7 +591,7 @@ int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t 
             if (!std::isfinite(lo) || !std::isfinite(hi) || !(hi > lo))
                 fail(ANL_ERR_ARG, "every radial range must be finite with rmax > rmin");
         }
-        if (nt * (lmax + 1) * (lmax + 1) > 2 * SPH_WAVE_LDS)
+        if (bins_per_launch(nt, nbin, lmax) == 0)
             fail(ANL_ERR_ARG, "too many atom types for the per-atom density block");
         ensure_device();
         anl_sph *h = new anl_sph();
@@ -481,6 +607,14 @@ int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t 
             std::vector<double> T = bin_edges(lo, hi, nbin, dr);
             std::copy(T.begin(), T.end(), thr.begin() + (size_t)p * (nbin + 1));
         }
+        h->d2lo = HUGE_VAL;
+        h->d2hi = 0.0;
+        for (uint32_t p = 0; p < nt * nt; p++) {
+            h->d2lo = std::min(h->d2lo, thr[(size_t)p * (nbin + 1)]);
+            h->d2hi = std::max(h->d2hi, thr[(size_t)p * (nbin + 1) + nbin]);
+        }
+        h->rcut = std::sqrt(h->d2hi) * (1.0 + 1e-12) + 1e-9;
+        h->block_screen = !tr->triclinic && std::isfinite(h->rcut);
         const uint32_t L1 = lmax + 1;
         std::vector<double> coef((size_t)L1 * L1, 0.0);
         for (uint32_t l = 0; l <= lmax; l++)
@@ -494,6 +628,9 @@ int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t 
                 if ((uint32_t)tr->type_of_slot[sl] == t) h->aprime[sl] = (int32_t)k++;
         }
         h->tstart[nt] = k;
+        size_t free_b = 0, total_b = 0;
+        ANL_HIP(hipMemGetInfo(&free_b, &total_b));
+        h->budget = std::min<uint64_t>(32ull << 30, std::max<uint64_t>(free_b / 2, 1ull << 30));
         try {
             ANL_HIP(hipMalloc(&h->d_thr, thr.size() * sizeof(double)));
             ANL_HIP(hipMemcpy(h->d_thr, thr.data(), thr.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -536,8 +673,10 @@ int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host) {
         if (!h || !out_host) fail(ANL_ERR_ARG, "null pointer passed to anl_sph_snapshot");
         if (frame >= h->tr->F) fail(ANL_ERR_ARG, "frame index out of range");
         const uint64_t FS = frame_doubles(h);
-        double *S = (double *)workspace(h, FS * sizeof(double));
-        launch_density(h, frame, 1, S, nullptr);
+        char *w = (char *)workspace(h, a256(FS * sizeof(double)) + a256(radius_bytes(h, 1)));
+        double *S = (double *)w;
+        double4 *rad = (double4 *)(w + a256(FS * sizeof(double)));
+        launch_density(h, frame, 1, S, rad, nullptr);
         std::vector<double> Sh(FS);
         ANL_HIP(hipMemcpy(Sh.data(), S, FS * sizeof(double), hipMemcpyDeviceToHost));
         // S[tj][b][l][a'][m + l] -> out[atom][tj][b][reference (l, m) order]

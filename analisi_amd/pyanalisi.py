@@ -1,7 +1,8 @@
 """The pyanalisi surface of the pair-correlation path (drop-in for
 ``import pyanalisi`` in code that uses Trajectory, Traj, ReadLog,
-ReadLog_file, BoxFormat, Gofrt, MeanSquareDisplacement, their ``_lammps``
-forms over Traj, GreenKubo and GreenKubo_file)::
+ReadLog_file, BoxFormat, Gofrt, MeanSquareDisplacement,
+SphericalCorrelations, their ``_lammps`` forms over Traj, GreenKubo and
+GreenKubo_file)::
 
     from analisi_amd import pyanalisi
     traj = pyanalisi.Trajectory(pos, vel, types, box, pyanalisi.BoxFormat.CellVectors, False, False)
@@ -9,5 +10,6 @@ forms over Traj, GreenKubo and GreenKubo_file)::
     g.reset(700); g.calculate(0); gr = np.array(g, copy=True)
 """
 from ._pyanalisi import (BoxFormat, Gofrt, Gofrt_lammps, GreenKubo, GreenKubo_file,  # noqa: F401
-                         MeanSquareDisplacement, MeanSquareDisplacement_lammps, ReadLog, ReadLog_file, Traj,
-                         Trajectory, gk_block_average, gofrt_block_average, has_mmap, info, msd_block_average)
+                         MeanSquareDisplacement, MeanSquareDisplacement_lammps, ReadLog, ReadLog_file,
+                         SphericalCorrelations, SphericalCorrelations_lammps, Traj, Trajectory, gk_block_average,
+                         gofrt_block_average, has_mmap, info, msd_block_average)

@@ -29,6 +29,9 @@ Other workloads (--config), each printing one JSON line of its own:
       seeded AR(2) processes with cross-coupling 0.3, 32 uneven blocks
       partitioning t0 in [0, 1,980,000), 20,000 lags per block, lag products
       on the matrix cores (split-BF16 MFMA) vs the FP64 form
+  sh  SphericalCorrelations (lmax 10): 2,048 atoms 50/50 at 0.1 atoms/A^3
+      (cubic 27.4 A), 2,200 frames, sigma 0.05 A; ranges (0.7, 3.5) A for every
+      type pair, 4 bins, lags 0..199, t0 stride 10, reset(2000), calculate(0)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config c2]
 """
@@ -613,12 +616,83 @@ def run_gk(args, hip):
     return res
 
 
+SH_WORKLOAD = {"N": 2048, "F": 2200, "rho": 0.1, "sigma": 0.05, "seed": 71, "fractions": [0.5, 0.5],
+               "rminmax": [(0.7, 3.5)] * 4, "nbin": 4, "tmax": 200, "skip": 10, "n": 2000, "lmax": 10}
+
+
+def run_sh(args, hip):
+    """SphericalCorrelations: per-atom spherical-harmonic densities of every
+    frame, then their lag products.  Algorithmic FLOP = 2 * (number of t0) *
+    lags * N * ntypes * nbin * (lmax+1)^2 (one multiply-add per harmonic
+    component of every atom for every (t0, lag) pair), priced against the FP64
+    peak."""
+    from analisi_amd import capi, synth
+    w = SH_WORKLOAD
+    N, F, lmax, nbin = w["N"], w["F"], w["lmax"], w["nbin"]
+    Lb = (N / w["rho"]) ** (1.0 / 3.0)
+    box_l = synth.lammps_box(Lb)
+    log("generating %d atoms x %d frames" % (N, F))
+    pos = synth.random_walk(w["seed"], N, F, box_l, w["sigma"])
+    types = synth.types_by_fraction(N, w["fractions"])
+    tid, nt = type_ids(types)
+    traj = capi.DeviceTrajectory(pos, internal_box(box_l, F), tid, nt)
+    s = capi.DeviceSph(traj, w["rminmax"], nbin, lmax)
+    n, tmax, skip = w["n"], w["tmax"], w["skip"]
+    leff = min(n, tmax) if tmax else n
+    n0 = (n + skip - 1) // skip
+    out = s.calculate(n, tmax, skip, 0)             # untimed: workspace, code objects
+    hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    reps = max(1, args.steps)
+    hip.record(e0, None)
+    for _ in range(reps):
+        out = s.calculate(n, tmax, skip, 0)
+    hip.record(e1, None)
+    ms = hip.elapsed_ms(e0, e1) / reps
+    flop = 2.0 * n0 * leff * N * nt * nbin * (lmax + 1) ** 2
+    # lag-0 check: every l=0 entry is mean_i (count_i / sqrt(4 pi))^2 > 0
+    assert np.all(out[0, ..., lmax] > 0)
+    res = {"metric": "spherical-harmonic density lag products FLOP/s (algorithmic 2*n0*lags*N*ntypes*nbin*(lmax+1)^2)",
+           "value": flop / (ms * 1e-3), "unit": "FLOP/s", "n_gpus": 1, "steps": reps, "ms_per_step": ms,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "SphericalCorrelations lmax %d: %d atoms 50/50, cubic %.2f A (0.1 atoms/A^3), %d "
+                                  "frames, sigma %g A (seed %d); ranges (0.7, 3.5) A, %d bins, %d lags, t0 stride %d, "
+                                  "reset(%d), calculate(0)" % (lmax, N, Lb, F, w["sigma"], w["seed"], nbin, leff, skip,
+                                                               n),
+                      "algorithmic_flop": flop},
+           "roofline": {"bound": "fp64", "achieved": flop / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": flop / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                        "note": "event-timed over whole calculate() calls (densities of every frame, lag-product "
+                                "tiles, t0 reduction, copy of the result)"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), False)
+        nf = 2
+        t = time.perf_counter()
+        dens = np.stack([O.sph_density(ot, f, w["rminmax"], nbin, lmax) for f in range(nf + 1)])
+        t_frame = (time.perf_counter() - t) / (nf + 1)
+        t = time.perf_counter()
+        O.sph_corr_from_densities(ot, dens, nbin, 2, 1, 2, lmax)
+        t_pair = (time.perf_counter() - t) / 4
+        nframes = n + leff - 1
+        est = nframes * t_frame + n0 * leff * t_pair
+        res["cpu_baseline"] = {"value": flop / est, "unit": "FLOP/s", "cores": 1, "kind": "port",
+                               "sample": "oracle on the same trajectory: density of one frame %.3f s (mean of %d), one "
+                                         "(t0, lag) product %.4f s (mean of 4); the whole workload's %d frames and "
+                                         "%d (t0, lag) pairs extrapolate to %.0f s" %
+                                         (t_frame, nf + 1, t_pair, nframes, n0 * leff, est)}
+    log("sh: %.2f ms per calculate" % ms)
+    s.close()
+    traj.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "sh"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-lags", type=int, default=4, help="lags per autotune probe launch")
@@ -650,6 +724,8 @@ def main():
     capi.check(capi.lib().anl_init(0))
     if args.config == "c5":
         res = run_gk(args, hip)
+    elif args.config == "sh":
+        res = run_sh(args, hip)
     else:
         w = gofrt_workload(args.config)
         if args.n:

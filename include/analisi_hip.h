@@ -1,7 +1,7 @@
 /*
  * analisi_hip.h -- C ABI of the MI355X (gfx950) time-lagged pair-correlation
  * path: g(r,t) histograms, mean square displacement, Green-Kubo lag
- * correlations and block statistics.
+ * correlations, spherical-harmonic density correlations and block statistics.
  *
  * Plain C types only (pointers, sizes, status codes).  Every call returns an
  * ANL_* status; on failure anl_last_error() returns the message, which the
@@ -162,7 +162,8 @@ int anl_sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uin
  * [natoms][ntypes][nbin][(lmax+1)^2] in original atom order and the
  * reference's (l, m) order (l = lmax first; m = -1..-l, then 0..l). */
 int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host);
-/* Bytes of per-frame densities held on the device at once (default 8 GiB);
+/* Bytes of per-frame densities held on the device at once (default: half the
+ * free device memory at creation, at most 32 GiB);
  * a smaller budget processes the t0 values in more chunks, same results up
  * to the order of the chunk sums.  No counterpart in the reference (its
  * CalcBuffer size, buffer_size, plays the same role on the host). */

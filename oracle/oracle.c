@@ -754,17 +754,15 @@ void orc_sph_density(const orc_traj *tr, uint64_t ts, const double *rminmax, uns
  * t1 per lag.  out[dt][type_i][type_j][bin][k] holds l = lmax - k
  * (running_average_m_zero_vector walks l from lmax down).  leff = min(n, tmax)
  * (tmax = 0: n).  Returns -1 when leff + n + primo > nframes (:63-65). */
-int orc_sph_correlations(const orc_traj *tr, const double *rminmax, unsigned nbin, int lmax, unsigned tmax,
-                         unsigned skip, unsigned n, uint64_t primo, double *out)
+/* The t1/lag loops of orc_sph_correlations over densities already computed:
+ * dens[f] is the density of frame primo + f, f < n + leff - 1. */
+void orc_sph_corr_from_densities(const orc_traj *tr, const double *dens, unsigned nbin, int lmax, unsigned leff,
+                                 unsigned skip, unsigned n, double *out)
 {
     const uint64_t N = tr->natoms, nt = tr->ntypes;
     const size_t L2 = (size_t)(lmax + 1) * (lmax + 1), L1 = (size_t)lmax + 1;
-    const unsigned leff = (n < tmax || tmax == 0) ? n : tmax;
-    if ((uint64_t)leff + n + primo > tr->nframes) return -1;
     if (skip == 0) skip = 1;
-    const uint64_t nfr = (uint64_t)n + leff - 1, snap = N * nt * nbin * L2;
-    double *dens = malloc(nfr * snap * sizeof(double));
-    for (uint64_t f = 0; f < nfr; f++) orc_sph_density(tr, primo + f, rminmax, nbin, lmax, dens + f * snap);
+    const uint64_t snap = N * nt * nbin * L2;
     const uint64_t fin = nt * nt * nbin * L1;
     double *ave = malloc(fin * sizeof(double));
     double *prod = malloc(L2 * sizeof(double));
@@ -799,10 +797,32 @@ int orc_sph_correlations(const orc_traj *tr, const double *rminmax, unsigned nbi
             for (uint64_t k = 0; k < fin; k++) o[k] += (ave[k] - o[k]) / (double)cnt[dt];
         }
     }
-    free(dens);
     free(ave);
     free(prod);
     free(cnt);
     free(acnt);
+}
+
+/* sphericalcorrelations.cpp:62-236 and corr_sh_calc :238-270, one thread --
+ * for every lag dt < leff and t1 = 0, skip, ... < n (ascending): the product
+ * of the densities of frames primo+t1 and primo+t1+dt, element by element;
+ * per (atom, type_j, bin) and l the sum over m (the m = 0 term, then
+ * m = -1..-l, then m = 1..l; MultiVal::sum_in_m_zero); the running mean over
+ * the central atoms of each type (in index order); then the running mean over
+ * t1 per lag.  out[dt][type_i][type_j][bin][k] holds l = lmax - k
+ * (running_average_m_zero_vector walks l from lmax down).  leff = min(n, tmax)
+ * (tmax = 0: n).  Returns -1 when leff + n + primo > nframes (:63-65). */
+int orc_sph_correlations(const orc_traj *tr, const double *rminmax, unsigned nbin, int lmax, unsigned tmax,
+                         unsigned skip, unsigned n, uint64_t primo, double *out)
+{
+    const uint64_t N = tr->natoms, nt = tr->ntypes;
+    const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
+    const unsigned leff = (n < tmax || tmax == 0) ? n : tmax;
+    if ((uint64_t)leff + n + primo > tr->nframes) return -1;
+    const uint64_t nfr = (uint64_t)n + leff - 1, snap = N * nt * nbin * L2;
+    double *dens = malloc(nfr * snap * sizeof(double));
+    for (uint64_t f = 0; f < nfr; f++) orc_sph_density(tr, primo + f, rminmax, nbin, lmax, dens + f * snap);
+    orc_sph_corr_from_densities(tr, dens, nbin, lmax, leff, skip, n, out);
+    free(dens);
     return 0;
 }

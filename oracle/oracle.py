@@ -79,6 +79,8 @@ def lib():
         L.orc_sph_correlations.restype = ctypes.c_int
         L.orc_sph_correlations.argtypes = [ctypes.POINTER(OrcTraj), _dp, ctypes.c_uint, ctypes.c_int,
                                            ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, _dp]
+        L.orc_sph_corr_from_densities.argtypes = [ctypes.POINTER(OrcTraj), _dp, ctypes.c_uint, ctypes.c_int,
+                                                  ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _dp]
         _lib = L
     return _lib
 
@@ -200,6 +202,16 @@ def sph_correlations(tr, rminmax, nbin, tmax, skip, n, primo, lmax=10):
     rc = lib().orc_sph_correlations(tr.ref, _p(rm), nbin, lmax, tmax, skip, n, primo, _p(out))
     if rc != 0:
         raise RuntimeError("trajectory is too short for this kind of calculation")
+    return out
+
+
+def sph_corr_from_densities(tr, dens, nbin, leff, skip, n, lmax=10):
+    """The t1/lag loops of sph_correlations over densities dens[f] of frames
+    primo + f (f < n + leff - 1), as sph_density returns them."""
+    dens = np.ascontiguousarray(dens, dtype=np.float64)
+    nt = tr.ntypes
+    out = np.zeros((leff, nt, nt, nbin, lmax + 1), dtype=np.float64)
+    lib().orc_sph_corr_from_densities(tr.ref, _p(dens), nbin, lmax, leff, skip, n, _p(out))
     return out
 
 

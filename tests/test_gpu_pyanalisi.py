@@ -288,3 +288,71 @@ def test_lammps_binary_block_averages_match_the_numpy_trajectory(pa, tmp_path):
     m0, v0 = pa.msd_block_average(tnp, 2, 1, 5, False, False)
     m1, v1 = pa.msd_block_average(tl, 2, 1, 5, False, False)
     assert is_same(m0, m1).all() and is_same(v0, v1).all()
+
+
+def _sh_close(got, want, tol=1e-12):
+    scale = np.abs(want).max()
+    assert scale > 0 and np.abs(got - want).max() <= tol * scale, np.abs(got - want).max() / scale
+
+
+def test_spherical_correlations_reference_test_call(pa):
+    """tests/test_sh.py:4-22 on a synthetic trajectory of the reference's test
+    layout (types 0, then 8 atoms of type 1 and 8 of type 2): [(0.7, 3.5)]*9,
+    4 bins, tmax 200, 4 threads, skip 90, buffer 10, debug, no neighbour list;
+    reset(700), calculate(0).  The values equal the oracle's to 1e-12 of the
+    largest (FP64, different summation order); calc(t), SphericalBase::calc,
+    equals the oracle's density."""
+    F, N = 905, 64
+    box_l = synth.lammps_box(8.0)
+    pos = synth.random_walk(61, N, F, box_l, 0.05)
+    vel = np.zeros_like(pos)
+    types = np.zeros(N, dtype=np.int32)
+    types[-16:-8] = 1
+    types[-8:] = 2
+    t = pa.Trajectory(pos, vel, types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False)
+    rm = [(0.7, 3.5)] * 9
+    sh = pa.SphericalCorrelations(t, rm, 4, 200, 4, 90, 10, True, [])
+    sh.reset(700)
+    sh.calculate(0)
+    res = np.array(sh, copy=True)
+    assert res.shape == (200, 3, 3, 4, 11)
+    ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), False)
+    _sh_close(res, O.sph_correlations(ot, rm, 4, 200, 90, 700, 0))
+    _sh_close(sh.calc(7).reshape(N, 3, 4, 121), O.sph_density(ot, 7, rm, 4))
+    assert sh.getNumberOfExtraTimestepsNeeded(10) == min(F // 11 + 1, 200)
+    with pytest.raises(RuntimeError, match="too short"):
+        sh.calculate(6)                                   # 200 + 700 + 6 > 905
+    with pytest.raises(RuntimeError, match="radial range for each pair"):
+        pa.SphericalCorrelations(t, rm[:4], 4, 200, 4, 90, 10, True, [])
+    with pytest.raises(RuntimeError, match="not supported"):
+        pa.SphericalCorrelations(t, rm, 1, 200, 4, 90, 10, True, [(19, 16.0, 16.0)])
+    few = pa.SphericalCorrelations(t, rm, 4, 3, 4, 90, 10, False, [])
+    few.reset(10)
+    with pytest.raises(RuntimeError, match="enough timesteps to parallelize"):
+        few.calculate(0)                                  # leff 3 < 4 threads
+
+
+def test_spherical_correlations_lammps_binary_window(pa, tmp_path):
+    """SphericalCorrelations_lammps over a Traj window equals
+    SphericalCorrelations over the in-memory Trajectory of the same data, for
+    a start inside the window and one that moves it."""
+    N, F = 90, 40
+    box_l = synth.lammps_box(9.0, [0.8, -0.5, 0.3])
+    pos = synth.random_walk(67, N, F, box_l, 0.2)
+    vel = np.zeros_like(pos)
+    types = synth.types_by_fraction(N, [0.7, 0.3]).astype(np.int32)
+    tnp = pa.Trajectory(pos, vel, types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsTriclinic, False, False)
+    path = str(tmp_path / "sh.bin")
+    tnp.write_lammps_binary(path, 0, -1)
+    tl = pa.Traj(path)
+    tl.setAccessWindowSize(30)
+    tl.setAccessStart(0)
+    rm = [(0.5, 3.2), (0.9, 3.6), (0.9, 3.6), (0.6, 3.0)]
+    s0 = pa.SphericalCorrelations(tnp, rm, 3, 6, 1, 3, 10, False, [])
+    s1 = pa.SphericalCorrelations_lammps(tl, rm, 3, 6, 1, 3, 10, False, [])
+    s0.reset(14)
+    s1.reset(14)
+    for primo in (4, 17):
+        s0.calculate(primo)
+        s1.calculate(primo)
+        _sh_close(np.array(s1), np.array(s0))
