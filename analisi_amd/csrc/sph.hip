@@ -630,7 +630,7 @@ int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t 
         h->tstart[nt] = k;
         size_t free_b = 0, total_b = 0;
         ANL_HIP(hipMemGetInfo(&free_b, &total_b));
-        h->budget = std::min<uint64_t>(32ull << 30, std::max<uint64_t>(free_b / 2, 1ull << 30));
+        h->budget = std::min<uint64_t>(64ull << 30, std::max<uint64_t>(free_b / 2, 1ull << 30));
         try {
             ANL_HIP(hipMalloc(&h->d_thr, thr.size() * sizeof(double)));
             ANL_HIP(hipMemcpy(h->d_thr, thr.data(), thr.size() * sizeof(double), hipMemcpyHostToDevice));

@@ -163,7 +163,7 @@ int anl_sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uin
  * reference's (l, m) order (l = lmax first; m = -1..-l, then 0..l). */
 int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host);
 /* Bytes of per-frame densities held on the device at once (default: half the
- * free device memory at creation, at most 32 GiB);
+ * free device memory at creation, at most 64 GiB);
  * a smaller budget processes the t0 values in more chunks, same results up
  * to the order of the chunk sums.  No counterpart in the reference (its
  * CalcBuffer size, buffer_size, plays the same role on the host). */
