@@ -13,6 +13,7 @@
 //   Gofrt<TFLOAT,T>                   gofrt.{h,cpp}
 //   MSD<T,FPE>                        msd.{h,cpp}
 //   SphericalCorrelations<T>          sphericalcorrelations.{h,cpp}, sphericalbase.{h,cpp}
+//   Steinhardt<T>                     steinhardt.{h,cpp}
 //   GreenKuboNComponentIonicFluid<..> greenkuboNcomponentionicfluid.{h,cpp}
 //   MediaVar<T>, MediaVarCovar<T>     calcoliblocchi.h
 //   BlockAverageG<TR,T,Args...>       blockaverage.h:83-218
@@ -640,6 +641,56 @@ class MSD : public VectorOp<MSD<T, FPE>> {
 };
 
 // ---------------------------------------------------------------------------
+// The device handle of SphericalBase (sphericalbase.h:6-60): ranges, bins and
+// lmax over the trajectory's current device mirror, rebuilt when a windowed
+// trajectory has moved its window.  The constructor checks the ranges as
+// sphericalbase.cpp:7-16 does.
+// ---------------------------------------------------------------------------
+template <class T>
+class SphericalHandle {
+   public:
+    using rminmax_t = std::vector<std::pair<double, double>>;
+    SphericalHandle(T *t, const rminmax_t &rminmax, size_t nbin, unsigned int lmax)
+        : traiettoria(t), nbin(nbin), lmax(lmax) {
+        const size_t ntypes = t->get_ntypes();
+        if (ntypes * ntypes != rminmax.size()) {
+            std::stringstream ss;
+            ss << "you must provide a radial range for each pair of atomic types, in total ntypes*ntypes pair of "
+                  "numbers. You provided "
+               << rminmax.size() << " elements while ntypes is " << ntypes << " .";
+            throw std::runtime_error(ss.str());
+        }
+        for (const auto &p : rminmax) {
+            rm.push_back(p.first);
+            rm.push_back(p.second);
+        }
+    }
+    ~SphericalHandle() {
+        if (hs) anl_sph_destroy(hs);
+    }
+    SphericalHandle(const SphericalHandle &) = delete;
+    SphericalHandle &operator=(const SphericalHandle &) = delete;
+    anl_sph *get() {
+        if (!hs || gen != traiettoria->generation()) {
+            anl_traj *d = traiettoria->device();
+            if (hs) anl_sph_destroy(hs);
+            hs = nullptr;
+            anl_check(anl_sph_create(d, lmax, rm.data(), (uint32_t)nbin, &hs));
+            gen = traiettoria->generation();
+        }
+        return hs;
+    }
+
+   private:
+    T *traiettoria;
+    std::vector<double> rm;
+    size_t nbin;
+    unsigned int lmax;
+    anl_sph *hs = nullptr;
+    uint64_t gen = 0;
+};
+
+// ---------------------------------------------------------------------------
 // SphericalCorrelations (sphericalcorrelations.h:9-78, sphericalcorrelations.cpp)
 // over SphericalBase (sphericalbase.h:6-60): the time correlation of the
 // per-atom real spherical-harmonic densities of each neighbour type and
@@ -659,22 +710,12 @@ class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
     SphericalCorrelations(T *t, const rminmax_t rminmax, size_t nbin, size_t tmax = 0, size_t nthreads = 0,
                           size_t skip = 1, size_t buffer_size = 10, bool debug = false,
                           const NeighListSpec neighList = {}, unsigned int lmax = ANL_SPH_LMAX_MAX)
-        : traiettoria(t), rminmax(rminmax), nbin(nbin), tmax(tmax), nthreads(nthreads), skip(skip == 0 ? 1 : skip),
-          buffer_size(buffer_size), lmax(lmax), ntypes(t->get_ntypes()) {
+        : traiettoria(t), handle(t, rminmax, nbin, lmax), nbin(nbin), tmax(tmax), nthreads(nthreads),
+          skip(skip == 0 ? 1 : skip), buffer_size(buffer_size), lmax(lmax), ntypes(t->get_ntypes()) {
         (void)debug;
-        if ((size_t)ntypes * ntypes != rminmax.size()) {  // sphericalbase.cpp:7-16
-            std::stringstream ss;
-            ss << "you must provide a radial range for each pair of atomic types, in total ntypes*ntypes pair of "
-                  "numbers. You provided "
-               << rminmax.size() << " elements while ntypes is " << ntypes << " .";
-            throw std::runtime_error(ss.str());
-        }
         if (!neighList.empty())
             throw std::runtime_error("neighbour lists (the SANN variant of SphericalCorrelations) are not supported");
-        if (t->get_nloaded_timesteps() > 0) h();  // argument errors surface here
-    }
-    ~SphericalCorrelations() {
-        if (hs) anl_sph_destroy(hs);
+        if (t->get_nloaded_timesteps() > 0) handle.get();  // argument errors surface here
     }
     SphericalCorrelations &operator=(const SphericalCorrelations &destra) {
         VOp::operator=(destra);
@@ -711,7 +752,7 @@ class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
                 "Don't have enough timesteps to parallelize the calculation. Try again with a smaller number of "
                 "threads. ");
         const size_t off = traiettoria->window_offset(primo, ntimesteps + leff - 1);
-        anl_check(anl_sph_calculate(h(), off, (uint32_t)ntimesteps, (uint32_t)tmax, (uint32_t)skip, vdata));
+        anl_check(anl_sph_calculate(handle.get(), off, (uint32_t)ntimesteps, (uint32_t)tmax, (uint32_t)skip, vdata));
     }
     std::vector<ssize_t> get_shape() const {
         return {(ssize_t)leff, (ssize_t)ntypes, (ssize_t)ntypes, (ssize_t)nbin, (ssize_t)(lmax + 1)};
@@ -732,36 +773,104 @@ class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
     std::vector<double> calc(size_t timestep) {
         const size_t off = traiettoria->window_offset(timestep, 1);
         std::vector<double> out((size_t)traiettoria->get_natoms() * ntypes * nbin * (lmax + 1) * (lmax + 1));
-        anl_check(anl_sph_snapshot(h(), off, out.data()));
+        anl_check(anl_sph_snapshot(handle.get(), off, out.data()));
         return out;
     }
     size_t get_snap_size() const { return (size_t)traiettoria->get_natoms() * ntypes * nbin * (lmax + 1) * (lmax + 1); }
 
    private:
-    // device handle over the trajectory's current device mirror; rebuilt when
-    // a windowed trajectory has moved its window
-    anl_sph *h() {
-        if (!hs || gen != traiettoria->generation()) {
-            anl_traj *d = traiettoria->device();
-            if (hs) anl_sph_destroy(hs);
-            hs = nullptr;
-            std::vector<double> rm;
-            for (const auto &p : rminmax) {
-                rm.push_back(p.first);
-                rm.push_back(p.second);
-            }
-            anl_check(anl_sph_create(d, lmax, rm.data(), (uint32_t)nbin, &hs));
-            gen = traiettoria->generation();
-        }
-        return hs;
-    }
     T *traiettoria;
-    rminmax_t rminmax;
+    SphericalHandle<T> handle;
     size_t nbin, tmax, nthreads, skip, buffer_size;
     unsigned int lmax;
     size_t ntypes, leff = 0, ntimesteps = 0;
-    anl_sph *hs = nullptr;
-    uint64_t gen = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Steinhardt<l> (steinhardt.h:17-120, steinhardt.cpp:7-236): per frame, the
+// order parameters q_l of every (atom, neighbour type, bin), as a histogram
+// over the listed l values or as a trajectory of q_1..q_lmax.  Neighbour lists
+// (SANN) and the averaged order parameter are not supported.
+// ---------------------------------------------------------------------------
+template <class T>
+class Steinhardt : public VectorOp<Steinhardt<T>> {
+    using VOp = VectorOp<Steinhardt<T>>;
+    using VOp::data_length;
+    using VOp::vdata;
+
+   public:
+    using Rminmax_t = std::vector<std::pair<double, double>>;
+    using NeighListSpec = std::vector<std::tuple<size_t, double, double>>;  // neighbour.h:13
+    Steinhardt(T *t, const Rminmax_t rminmax, unsigned int nbin, unsigned int steinhardt_histogram_size,
+               std::vector<unsigned int> steinhardt_l_histogram, unsigned int nthreads = 2, unsigned int skip = 1,
+               bool do_histogram = true, const NeighListSpec nls = {}, const bool averaged_order = false,
+               unsigned int lmax = ANL_SPH_LMAX_MAX)
+        : traiettoria(t), handle(t, rminmax, nbin, lmax), l_hist(std::move(steinhardt_l_histogram)), nbin(nbin),
+          nbin_st(steinhardt_histogram_size), skip(skip == 0 ? 1 : skip), lmax(lmax), ntypes(t->get_ntypes()),
+          do_histogram(do_histogram) {
+        (void)nthreads;
+        if (nls.empty() && averaged_order)  // steinhardt.cpp:30-35
+            throw std::runtime_error(
+                "You must use a neighbour list when computing a averaged steinhardt order parameter! ");
+        if (!nls.empty() && nbin > 1)
+            throw std::runtime_error("When using a neighbour list to compute steinhardt please set nbin to 1. ");
+        if (!nls.empty())
+            throw std::runtime_error("neighbour lists (the SANN variant of Steinhardt) are not supported");
+        hist_size = 1;
+        for (size_t i = 0; i < l_hist.size(); ++i) hist_size *= nbin_st;
+        if (t->get_nloaded_timesteps() > 0) handle.get();
+    }
+    Steinhardt &operator=(const Steinhardt &destra) {
+        VOp::operator=(destra);
+        return *this;
+    }
+    unsigned int nExtraTimesteps(unsigned int) { return 0; }
+    void reset(const unsigned int n) {  // steinhardt.cpp:47-80
+        ntimesteps = n;
+        const size_t len = do_histogram ? hist_size * nbin * ntypes * ntypes
+                                        : (size_t)nbin * (n / skip) * traiettoria->get_natoms() * ntypes * lmax;
+        if (len != data_length) {
+            delete[] vdata;
+            data_length = (unsigned int)len;
+            vdata = new double[data_length];
+        }
+    }
+    void calculate(size_t primo) {  // calculatemultithread.h:106-162 with calc_init, steinhardt.cpp:80-94
+        if (ntimesteps + primo > traiettoria->get_ntimesteps())
+            throw std::runtime_error(
+                "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
+                "the size of the average ");
+        const size_t off = traiettoria->window_offset(primo, ntimesteps);
+        anl_check(anl_steinhardt_calculate(handle.get(), off, (uint32_t)ntimesteps, skip, l_hist.data(),
+                                           (uint32_t)l_hist.size(), nbin_st, do_histogram ? 1 : 0, vdata));
+    }
+    std::vector<ssize_t> get_shape() const {  // steinhardt.h:76-87
+        if (do_histogram) {
+            std::vector<ssize_t> r{(ssize_t)nbin, (ssize_t)ntypes, (ssize_t)ntypes};
+            for (size_t i = 0; i < l_hist.size(); ++i) r.push_back(nbin_st);
+            return r;
+        }
+        return {(ssize_t)nbin, (ssize_t)(ntimesteps / skip), (ssize_t)traiettoria->get_natoms(), (ssize_t)ntypes,
+                (ssize_t)lmax};
+    }
+    std::vector<ssize_t> get_stride() const {  // steinhardt.h:48-74: C order
+        auto s = get_shape();
+        std::vector<ssize_t> st(s.size());
+        ssize_t acc = sizeof(double);
+        for (int i = (int)s.size() - 1; i >= 0; --i) {
+            st[i] = acc;
+            acc *= s[i];
+        }
+        return st;
+    }
+
+   private:
+    T *traiettoria;
+    SphericalHandle<T> handle;
+    std::vector<unsigned int> l_hist;
+    unsigned int nbin, nbin_st, skip, lmax;
+    size_t ntypes, hist_size = 1, ntimesteps = 0;
+    bool do_histogram;
 };
 
 // ---------------------------------------------------------------------------

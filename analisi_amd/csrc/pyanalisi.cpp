@@ -127,6 +127,31 @@ C &common_interfaces(C &c) {
     return c;
 }
 
+// Steinhardt with its lmax fixed, one Python class per lmax as in the reference
+template <class T, int L>
+struct SteinhardtL : Steinhardt<T> {
+    SteinhardtL(T *t, typename Steinhardt<T>::Rminmax_t rminmax, unsigned int nbin, unsigned int nbin_st,
+                std::vector<unsigned int> l_hist, unsigned int nthreads, unsigned int skip, bool do_histogram,
+                typename Steinhardt<T>::NeighListSpec nls, bool averaged)
+        : Steinhardt<T>(t, rminmax, nbin, nbin_st, std::move(l_hist), nthreads, skip, do_histogram, nls, averaged,
+                        (unsigned int)L) {}
+};
+
+template <class T, int L>
+void define_steinhardt(py::module &m, const std::string &suffix) {
+    using SL = SteinhardtL<T, L>;
+    py::class_<SL>(m, ("SteinhardtOrderParameterHistogram_" + std::to_string(L) + suffix).c_str(),
+                   py::buffer_protocol())
+        .def(py::init<T *, typename SL::Rminmax_t, unsigned int, unsigned int, std::vector<unsigned int>, unsigned int,
+                      unsigned int, bool, typename SL::NeighListSpec, bool>(),
+             py::arg("trajectory"), py::arg("rminmax"), py::arg("nbin"), py::arg("histogram_size"),
+             py::arg("l_list"), py::arg("nthreads") = 2, py::arg("skip") = 1, py::arg("do_histogram") = true,
+             py::arg("neighList") = typename SL::NeighListSpec{}, py::arg("averaged_order") = false)
+        .def("reset", &SL::reset)
+        .def("calculate", &SL::calculate)
+        .def_buffer([](SL &s) { return vbuffer(s); });
+}
+
 // define_atomic_traj (pyanalisi.cpp:65-153): the pair-correlation operators
 // over one trajectory class, named with its suffix
 template <class T>
@@ -181,6 +206,17 @@ void define_atomic_traj(py::module &m, const std::string &suffix) {
                  return copy_array(v.data(), {(ssize_t)v.size()});
              })
         .def_buffer([](S &s) { return vbuffer(s); });
+
+    // SteinhardtOrderParameterHistogram_<l> (pyanalisi.cpp:22-60, 114-122), l = 2..10
+    define_steinhardt<T, 2>(m, suffix);
+    define_steinhardt<T, 3>(m, suffix);
+    define_steinhardt<T, 4>(m, suffix);
+    define_steinhardt<T, 5>(m, suffix);
+    define_steinhardt<T, 6>(m, suffix);
+    define_steinhardt<T, 7>(m, suffix);
+    define_steinhardt<T, 8>(m, suffix);
+    define_steinhardt<T, 9>(m, suffix);
+    define_steinhardt<T, 10>(m, suffix);
 }
 
 // gk (pyanalisi.cpp:241-279): GreenKubo over one log class

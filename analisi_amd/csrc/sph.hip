@@ -60,6 +60,7 @@ struct SphDensArgs {
     const double *coef;        // [(lmax+1)^2]: coefficient of (l, |m|) at l*(lmax+1)+|m|
     double *S;                 // [nfr][FS]
     uint64_t N, FS, f0;        // atoms, doubles per frame, first frame
+    uint64_t fstride;          // frame fl of the launch is f0 + fl * fstride
     uint32_t nt, nbin, lmax;
     uint32_t nbc;              // bins per launch column (blockIdx.y)
     uint32_t ngroups;          // ceil(N / SPH_WAVES)
@@ -73,10 +74,10 @@ struct SphDensArgs {
 // = largest minimum-image distance from the centre, rounded up by a relative
 // 1e-12; one wavefront per (frame, block), out[.] = {centre, radius}
 __global__ __launch_bounds__(64) void sph_block_radius_kernel(const double4 *pos, const double *box, uint64_t N,
-                                                              uint64_t f0, double4 *out) {
+                                                              uint64_t f0, uint64_t fstride, double4 *out) {
     const uint64_t nblk = (N + 63) / 64;
     const uint64_t fl = blockIdx.x / nblk, blk = blockIdx.x - fl * nblk;
-    const uint64_t f = f0 + fl;
+    const uint64_t f = f0 + fl * fstride;
     const double4 *P = pos + f * N;
     const DevCell c = load_cell(box + f * 9);
     const uint64_t j = blk * 64 + threadIdx.x;
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(256) void sph_density_kernel(SphDensArgs a) {
     for (uint32_t k = lane; k < accn; k += 64) acc[k] = 0.0;
     wave_sync_lds();
 
-    const uint64_t f = a.f0 + fl;
+    const uint64_t f = a.f0 + fl * a.fstride;
     const double4 *P = a.pos + f * a.N;
     const DevCell c = load_cell(a.box + f * 9);
     const double4 pi = P[i];
@@ -385,6 +386,72 @@ __global__ __launch_bounds__(256) void sph_reduce_kernel(const double *G, double
     R[k] = R[k] + s;
 }
 
+constexpr uint32_t STEIN_MAXL = 16;  // l values of one histogram
+
+struct SteinArgs {
+    const double *S;             // chunk densities [nfr][FS]
+    const int32_t *type_ap;      // [N] type of each atom in type-sorted order
+    const int32_t *orig_ap;      // [N] original index of each atom in type-sorted order
+    unsigned long long *hist;    // [nbin][nt][nt][hs] counts, or null
+    double *q;                   // [nbin][nfs][N][nt][lmax] q_l, l = 1..lmax, or null
+    uint64_t N, FS, hs;
+    uint32_t nt, nbin, lmax, nl, nbin_st, nfr, fidx0, nfs;
+    double c00;                  // Y_00 of any direction: the density's l = 0 entry is (count) * c00
+    uint32_t l_list[STEIN_MAXL];
+};
+
+// steinhardt.cpp:166-225: one thread per (frame of the chunk, type_j, bin,
+// atom), atoms fastest.  q_l = sqrt(sum_m S_lm^2 / n / n * 4 pi / (2l+1)), the
+// m sum in the reference's order (m = 0, -1..-l, 1..l), n the neighbour count
+// recovered exactly from the l = 0 entry
+__global__ __launch_bounds__(256) void sph_steinhardt_kernel(SteinArgs a) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (uint64_t)a.nfr * a.nt * a.nbin * a.N) return;
+    uint64_t r = k;
+    const uint64_t ap = r % a.N;
+    r /= a.N;
+    const uint32_t b = (uint32_t)(r % a.nbin);
+    r /= a.nbin;
+    const uint32_t jt = (uint32_t)(r % a.nt), fl = (uint32_t)(r / a.nt);
+    const uint32_t L1 = a.lmax + 1, L2 = L1 * L1;
+    const double *Sb = a.S + (uint64_t)fl * a.FS + (uint64_t)(jt * a.nbin + b) * L2 * a.N;
+    const double na = rint(Sb[ap] / a.c00);
+    const double pi = 3.14159265358979323846264338327950288419716939937510;
+    double q[ANL_SPH_LMAX_MAX + 1];
+#pragma unroll
+    for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l) {
+        q[l] = 0.0;
+        if (l <= (int)a.lmax && na > 0) {
+            const double *s = Sb + (uint64_t)(l * l) * a.N + ap * (2 * l + 1);
+            double v = s[l] * s[l];
+            for (int mm = 1; mm <= l; ++mm) v += s[l - mm] * s[l - mm];
+            for (int mm = 1; mm <= l; ++mm) v += s[l + mm] * s[l + mm];
+            q[l] = sqrt(v / na / na * 4 * pi / (2 * l + 1));
+        }
+    }
+    if (a.hist) {
+        if (!(na > 0)) return;
+        uint64_t hidx = 0, stride = a.hs;
+        for (uint32_t kk = 0; kk < a.nl; ++kk) {
+            double qk = 0.0;
+#pragma unroll
+            for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l)
+                if ((uint32_t)l == a.l_list[kk]) qk = q[l];
+            stride /= a.nbin_st;
+            uint64_t jh = (uint64_t)floorf((float)(qk * a.nbin_st));
+            if (jh >= a.nbin_st) jh = a.nbin_st - 1;
+            hidx += jh * stride;
+        }
+        const uint32_t ti = (uint32_t)a.type_ap[ap];
+        atomicAdd(&a.hist[((uint64_t)(b * a.nt + ti) * a.nt + jt) * a.hs + hidx], 1ull);
+    } else {
+        double *o = a.q + ((((uint64_t)b * a.nfs + a.fidx0 + fl) * a.N + (uint64_t)a.orig_ap[ap]) * a.nt + jt) * a.lmax;
+#pragma unroll
+        for (int l = 1; l <= ANL_SPH_LMAX_MAX; ++l)
+            if (l <= (int)a.lmax) o[l - 1] = q[l];
+    }
+}
+
 __global__ __launch_bounds__(256) void sph_zero_kernel(double *p, uint64_t n) {
     const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (k < n) p[k] = 0.0;
@@ -415,6 +482,9 @@ struct anl_sph {
     double *d_coef = nullptr;         // [(lmax+1)^2]
     int32_t *d_aprime = nullptr;
     uint32_t *d_tstart = nullptr;
+    int32_t *d_type_ap = nullptr;     // [N] type of each type-sorted atom
+    int32_t *d_orig_ap = nullptr;     // [N] original index of each type-sorted atom
+    double c00 = 0.0;                 // Y_00
     uint64_t budget = 0;              // bytes of densities held at once (set at creation)
     double d2lo = 0.0, d2hi = 0.0;    // min over type pairs of T[0], max of T[nbin]
     double rcut = 0.0;                // distance bound of the block screen
@@ -458,10 +528,12 @@ uint64_t radius_bytes(const anl_sph *h, uint64_t nfr) {
     return h->block_screen ? nfr * ((h->tr->N + 63) / 64) * sizeof(double4) : 0;
 }
 
-// densities of frames [f0, f0 + nfr) of the device trajectory into S; rad
-// holds radius_bytes(h, nfr)
-void launch_density(anl_sph *h, uint64_t f0, uint32_t nfr, double *S, double4 *rad, hipStream_t s) {
+// densities of frames f0 + k * fstride, k < nfr, of the device trajectory into
+// S; rad holds radius_bytes(h, nfr)
+void launch_density(anl_sph *h, uint64_t f0, uint64_t fstride, uint32_t nfr, double *S, double4 *rad,
+                    hipStream_t s) {
     SphDensArgs a{};
+    a.fstride = fstride;
     a.d2lo = h->d2lo;
     a.d2hi = h->d2hi;
     a.rcut = h->rcut;
@@ -470,7 +542,7 @@ void launch_density(anl_sph *h, uint64_t f0, uint32_t nfr, double *S, double4 *r
         const uint64_t nb = (uint64_t)nfr * ((h->tr->N + 63) / 64);
         if (nb > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom block) pairs in one launch");
         hipLaunchKernelGGL(sph_block_radius_kernel, dim3((uint32_t)nb), dim3(64), 0, s, h->tr->pos, h->tr->box,
-                           h->tr->N, f0, rad);
+                           h->tr->N, f0, fstride, rad);
         ANL_HIP(hipGetLastError());
     }
     a.pos = h->tr->pos;
@@ -538,7 +610,7 @@ void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32
         const uint32_t n0c = (uint32_t)std::min<uint64_t>(rc, n0 - e0);
         const uint64_t t1_first = e0 * skip;
         const uint32_t nfr = (uint32_t)((uint64_t)(n0c - 1) * skip + leff);
-        launch_density(h, primo + t1_first, nfr, S, rad, s);
+        launch_density(h, primo + t1_first, 1, nfr, S, rad, s);
         SphGramArgs g{};
         g.S = S;
         g.G = G;
@@ -574,6 +646,77 @@ void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32
             const double na = (double)(h->tstart[ti + 1] - h->tstart[ti]);
             out_host[(size_t)dt * fin + (size_t)q * L1 + (h->lmax - l)] = Rh[(size_t)p * leff + dt] / na / (double)n0;
         }
+}
+
+// steinhardt.cpp:47-236 (no neighbour list, not averaged): frames primo + i,
+// i = 0, skip, .. < n - n % skip, in chunks of frames that fit the budget
+void steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip, const uint32_t *l_list, uint32_t nl,
+                          uint32_t nbin_st, int do_hist, double *out_host) {
+    anl_traj *tr = h->tr;
+    if (skip == 0) skip = 1;
+    if ((uint64_t)n + primo > tr->F)
+        fail(ANL_ERR_TOO_SHORT,
+             "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
+             "the size of the average ");
+    if (nl > STEIN_MAXL) fail(ANL_ERR_ARG, "at most 16 l values per histogram");
+    for (uint32_t k = 0; k < nl; k++)
+        if (l_list[k] > h->lmax) fail(ANL_ERR_ARG, "Wrong value of l!");
+    if (nl > 0 && nbin_st == 0) fail(ANL_ERR_ARG, "the histogram size must be positive");
+    uint64_t hs = 1;
+    for (uint32_t k = 0; k < nl; k++) {
+        hs *= nbin_st;
+        if (hs > (1ull << 31)) fail(ANL_ERR_ARG, "the histogram has more than 2^31 cells per (bin, type, type)");
+    }
+    const uint32_t nt = h->nt, nbin = h->nbin;
+    const uint64_t N = tr->N, nfs = n / skip;
+    const uint64_t len = do_hist ? (uint64_t)nbin * nt * nt * hs : (uint64_t)nbin * nfs * N * nt * h->lmax;
+    std::fill(out_host, out_host + len, 0.0);
+    if (nfs == 0 || len == 0) return;
+    const uint64_t fbytes = frame_doubles(h) * sizeof(double);
+    const uint64_t fpc = std::min<uint64_t>(nfs, std::max<uint64_t>(1, h->budget / std::max<uint64_t>(fbytes, 1)));
+    const uint64_t offR = a256(fpc * fbytes), offO = offR + a256(radius_bytes(h, fpc));
+    char *w = (char *)workspace(h, offO + a256(len * 8));
+    double *S = (double *)w;
+    double4 *rad = (double4 *)(w + offR);
+    double *outd = (double *)(w + offO);
+    hipStream_t s = nullptr;
+    hipLaunchKernelGGL(sph_zero_kernel, dim3((uint32_t)((len + 255) / 256)), dim3(256), 0, s, outd, len);
+    ANL_HIP(hipGetLastError());
+    SteinArgs a{};
+    a.S = S;
+    a.type_ap = h->d_type_ap;
+    a.orig_ap = h->d_orig_ap;
+    a.hist = do_hist ? (unsigned long long *)outd : nullptr;  // zero bits are the integer 0
+    a.q = do_hist ? nullptr : outd;
+    a.N = N;
+    a.FS = frame_doubles(h);
+    a.hs = hs;
+    a.nt = nt;
+    a.nbin = nbin;
+    a.lmax = h->lmax;
+    a.nl = nl;
+    a.nbin_st = nbin_st;
+    a.nfs = (uint32_t)nfs;
+    a.c00 = h->c00;
+    for (uint32_t k = 0; k < nl; k++) a.l_list[k] = l_list[k];
+    for (uint64_t f0 = 0; f0 < nfs; f0 += fpc) {
+        const uint32_t nfc = (uint32_t)std::min<uint64_t>(fpc, nfs - f0);
+        launch_density(h, primo + f0 * skip, skip, nfc, S, rad, s);
+        a.nfr = nfc;
+        a.fidx0 = (uint32_t)f0;
+        const uint64_t nth = (uint64_t)nfc * nt * nbin * N;
+        if ((nth + 255) / 256 > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom) pairs in one launch");
+        hipLaunchKernelGGL(sph_steinhardt_kernel, dim3((uint32_t)((nth + 255) / 256)), dim3(256), 0, s, a);
+        ANL_HIP(hipGetLastError());
+    }
+    if (do_hist) {
+        std::vector<unsigned long long> c(len);
+        ANL_HIP(hipMemcpy(c.data(), outd, len * 8, hipMemcpyDeviceToHost));
+        const double incr = 1.0 / (double)(int)nfs;  // steinhardt.cpp:92-93
+        for (uint64_t k = 0; k < len; k++) out_host[k] = (double)c[k] * incr;
+    } else {
+        ANL_HIP(hipMemcpy(out_host, outd, len * 8, hipMemcpyDeviceToHost));
+    }
 }
 
 }  // namespace
@@ -619,6 +762,7 @@ int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t 
         std::vector<double> coef((size_t)L1 * L1, 0.0);
         for (uint32_t l = 0; l <= lmax; l++)
             for (uint32_t m = 0; m <= l; m++) coef[l * L1 + m] = sh_coeff((int)l, (int)m);
+        h->c00 = coef[0];
         h->aprime.assign(tr->N, 0);
         h->tstart.assign(nt + 1, 0);
         uint32_t k = 0;
@@ -640,6 +784,16 @@ int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t 
             ANL_HIP(hipMemcpy(h->d_aprime, h->aprime.data(), tr->N * sizeof(int32_t), hipMemcpyHostToDevice));
             ANL_HIP(hipMalloc(&h->d_tstart, (nt + 1) * sizeof(uint32_t)));
             ANL_HIP(hipMemcpy(h->d_tstart, h->tstart.data(), (nt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+            std::vector<int32_t> orig(tr->N), type_ap(tr->N), orig_ap(tr->N);
+            ANL_HIP(hipMemcpy(orig.data(), tr->orig_slot, tr->N * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (uint64_t sl = 0; sl < tr->N; sl++) {
+                type_ap[h->aprime[sl]] = tr->type_of_slot[sl];
+                orig_ap[h->aprime[sl]] = orig[sl];
+            }
+            ANL_HIP(hipMalloc(&h->d_type_ap, tr->N * sizeof(int32_t)));
+            ANL_HIP(hipMemcpy(h->d_type_ap, type_ap.data(), tr->N * sizeof(int32_t), hipMemcpyHostToDevice));
+            ANL_HIP(hipMalloc(&h->d_orig_ap, tr->N * sizeof(int32_t)));
+            ANL_HIP(hipMemcpy(h->d_orig_ap, orig_ap.data(), tr->N * sizeof(int32_t), hipMemcpyHostToDevice));
         } catch (...) {
             anl_sph_destroy(h);
             throw;
@@ -655,6 +809,8 @@ int anl_sph_destroy(anl_sph *h) {
     if (h->d_coef) (void)hipFree(h->d_coef);
     if (h->d_aprime) (void)hipFree(h->d_aprime);
     if (h->d_tstart) (void)hipFree(h->d_tstart);
+    if (h->d_type_ap) (void)hipFree(h->d_type_ap);
+    if (h->d_orig_ap) (void)hipFree(h->d_orig_ap);
     if (h->work) (void)hipFree(h->work);
     delete h;
     return ANL_OK;
@@ -676,7 +832,7 @@ int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host) {
         char *w = (char *)workspace(h, a256(FS * sizeof(double)) + a256(radius_bytes(h, 1)));
         double *S = (double *)w;
         double4 *rad = (double4 *)(w + a256(FS * sizeof(double)));
-        launch_density(h, frame, 1, S, rad, nullptr);
+        launch_density(h, frame, 1, 1, S, rad, nullptr);
         std::vector<double> Sh(FS);
         ANL_HIP(hipMemcpy(Sh.data(), S, FS * sizeof(double), hipMemcpyDeviceToHost));
         // S[tj][b][l][a'][m + l] -> out[atom][tj][b][reference (l, m) order]
@@ -697,6 +853,14 @@ int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host) {
                     }
                 }
         }
+    });
+}
+
+int anl_steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip, const uint32_t *l_list,
+                             uint32_t nl, uint32_t nbin_st, int do_histogram, double *out_host) {
+    return guard([&] {
+        if (!h || !out_host || (nl > 0 && !l_list)) fail(ANL_ERR_ARG, "null pointer passed to anl_steinhardt_calculate");
+        steinhardt_calculate(h, primo, n, skip, l_list, nl, nbin_st, do_histogram, out_host);
     });
 }
 

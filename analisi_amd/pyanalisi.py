@@ -1,8 +1,8 @@
 """The pyanalisi surface of the pair-correlation path (drop-in for
 ``import pyanalisi`` in code that uses Trajectory, Traj, ReadLog,
 ReadLog_file, BoxFormat, Gofrt, MeanSquareDisplacement,
-SphericalCorrelations, their ``_lammps`` forms over Traj, GreenKubo and
-GreenKubo_file)::
+SphericalCorrelations, SteinhardtOrderParameterHistogram_<l>, their
+``_lammps`` forms over Traj, GreenKubo and GreenKubo_file)::
 
     from analisi_amd import pyanalisi
     traj = pyanalisi.Trajectory(pos, vel, types, box, pyanalisi.BoxFormat.CellVectors, False, False)
@@ -13,3 +13,10 @@ from ._pyanalisi import (BoxFormat, Gofrt, Gofrt_lammps, GreenKubo, GreenKubo_fi
                          MeanSquareDisplacement, MeanSquareDisplacement_lammps, ReadLog, ReadLog_file,
                          SphericalCorrelations, SphericalCorrelations_lammps, Traj, Trajectory, gk_block_average,
                          gofrt_block_average, has_mmap, info, msd_block_average)
+from . import _pyanalisi as _native
+
+# SteinhardtOrderParameterHistogram_<l> and _<l>_lammps, l = 2..10
+for _name in dir(_native):
+    if _name.startswith("SteinhardtOrderParameterHistogram_"):
+        globals()[_name] = getattr(_native, _name)
+del _name

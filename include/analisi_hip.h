@@ -162,6 +162,16 @@ int anl_sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uin
  * [natoms][ntypes][nbin][(lmax+1)^2] in original atom order and the
  * reference's (l, m) order (l = lmax first; m = -1..-l, then 0..l). */
 int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host);
+/* Steinhardt<lmax> (steinhardt.h:17-120, steinhardt.cpp:7-236) without
+ * neighbour lists, not averaged, over the handle's ranges and bins: frames
+ * primo + i, i = 0, skip, .. < n - n % skip; q_l = sqrt(4 pi/(2l+1) sum_m
+ * |S_lm|^2) / (neighbour count).  do_histogram: out_host [nbin][ntypes]
+ * [ntypes][nbin_st]^nl (one axis per entry of l_list, the first slowest),
+ * count * incr with incr = 1/int(n/skip); else out_host [nbin][n/skip]
+ * [natoms][ntypes][lmax] holding q_1..q_lmax (0 without neighbours).
+ * ANL_ERR_TOO_SHORT when n + primo exceeds the frames (:80-83). */
+int anl_steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip, const uint32_t *l_list,
+                             uint32_t nl, uint32_t nbin_st, int do_histogram, double *out_host);
 /* Bytes of per-frame densities held on the device at once (default: half the
  * free device memory at creation, at most 64 GiB);
  * a smaller budget processes the t0 values in more chunks, same results up

@@ -37,6 +37,10 @@
  *                          The reference's t1 visiting order depends on its
  *                          thread split; this restatement visits t1 in
  *                          ascending order (agreement to rounding).
+ *   orc_steinhardt         pinned: sc / bcc / fcc crystals (every atom's q_l
+ *                          equals scipy's from its neighbour vectors; q4, q6
+ *                          equal the tabulated values), q_l = 1 for a lone
+ *                          neighbour, the histogram equal to the binned q_l.
  */
 #include <math.h>
 #include <pthread.h>
@@ -824,5 +828,88 @@ int orc_sph_correlations(const orc_traj *tr, const double *rminmax, unsigned nbi
     for (uint64_t f = 0; f < nfr; f++) orc_sph_density(tr, primo + f, rminmax, nbin, lmax, dens + f * snap);
     orc_sph_corr_from_densities(tr, dens, nbin, lmax, leff, skip, n, out);
     free(dens);
+    return 0;
+}
+
+/* steinhardt.cpp:90-236 (no neighbour list, not averaged), one thread -- for
+ * frames primo + i, i = 0, skip, ... < n - n % skip (the split of
+ * calculatemultithread.h:50-79): the densities (with the neighbour count n_a
+ * of every (atom, type_j, bin)), q_l = sqrt(sum_m S_lm^2 / n_a / n_a * 4 * pi
+ * / (2l+1)) with the m sum in the order m = 0, -1..-l, 1..l.
+ *   do_hist: for n_a > 0, bin index floorf(q_l * nbin_st) (clamped to
+ *   nbin_st - 1) of every l in l_list, row-major with the first l slowest;
+ *   out[bin][type_i][type_j][hist] += incr, incr = 1/int(n/skip) (1 if 0).
+ *   else: out[bin][i/skip][atom][type_j][l-1] = q_l for l = 1..lmax (0 if
+ *   n_a = 0).
+ * Returns -1 when n + primo > nframes (:80-83). */
+int orc_steinhardt(const orc_traj *tr, const double *rminmax, unsigned nbin, int lmax, unsigned nbin_st,
+                   const unsigned *l_list, unsigned nl, unsigned skip, unsigned n, uint64_t primo, int do_hist,
+                   double *out)
+{
+    const uint64_t N = tr->natoms, nt = tr->ntypes;
+    const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
+    if ((uint64_t)n + primo > tr->nframes) return -1;
+    if (skip == 0) skip = 1;
+    const double incr = (n / skip > 0) ? 1.0 / (int)(n / skip) : 1.0;
+    size_t hs = 1;
+    for (unsigned k = 0; k < nl; k++) hs *= nbin_st;
+    const uint64_t nfs = n / skip;
+    const size_t len = do_hist ? nbin * nt * nt * hs : nbin * nfs * N * nt * (size_t)lmax;
+    memset(out, 0, len * sizeof(double));
+    double *dens = malloc(N * nt * nbin * L2 * sizeof(double));
+    int *cnt = malloc(N * nt * nbin * sizeof(int));
+    double *dr = malloc(nt * nt * sizeof(double));
+    for (uint64_t p = 0; p < nt * nt; p++) dr[p] = (rminmax[2 * p + 1] - rminmax[2 * p]) / (double)nbin;
+    for (unsigned i = 0; i < n - n % skip; i += skip) {
+        const uint64_t ts = primo + i;
+        orc_sph_density(tr, ts, rminmax, nbin, lmax, dens);
+        /* sphericalbase.cpp:33-60: the counter of the same loop */
+        memset(cnt, 0, N * nt * nbin * sizeof(int));
+        const double *box = tr->box + ts * tr->box_stride;
+        for (uint64_t a = 0; a < N; a++)
+            for (uint64_t b = 0; b < N; b++) {
+                if (a == b) continue;
+                const uint64_t p = nt * (uint64_t)tr->type_id[a] + (uint64_t)tr->type_id[b];
+                double x[3];
+                double d = sqrt(orc_d2_min_image(tr->pos + (ts * N + a) * 3, tr->pos + (ts * N + b) * 3, box,
+                                                 tr->triclinic, x));
+                int idx = (int)floorf((float)((d - rminmax[2 * p]) / dr[p]));
+                if (idx < (int)nbin && idx >= 0) cnt[nbin * (nt * a + (uint64_t)tr->type_id[b]) + (uint64_t)idx]++;
+            }
+        for (uint64_t a = 0; a < N; a++) {
+            const unsigned ti = (unsigned)tr->type_id[a];
+            for (uint64_t tj = 0; tj < nt; tj++)
+                for (unsigned b = 0; b < nbin; b++) {
+                    const double *s = dens + L2 * (nbin * (nt * a + tj) + b);
+                    const double na = (double)cnt[nbin * (nt * a + tj) + b];
+                    double q[ORC_SH_LMAX + 1];
+                    size_t off = 0;
+                    for (int l = lmax; l >= 0; l--) {
+                        double v = s[off + l] * s[off + l];
+                        for (int m = 0; m < l; m++) v += s[off + m] * s[off + m];
+                        for (int m = 0; m < l; m++) v += s[off + l + 1 + m] * s[off + l + 1 + m];
+                        q[l] = na > 0 ? sqrt(v / na / na * 4 * ORC_PI / (2 * l + 1)) : 0.0;
+                        off += 2 * (size_t)l + 1;
+                    }
+                    if (do_hist) {
+                        if (!(na > 0)) continue;
+                        size_t hidx = 0, stride = hs;
+                        for (unsigned k = 0; k < nl; k++) {
+                            stride /= nbin_st;
+                            size_t jh = (size_t)floorf((float)(q[l_list[k]] * nbin_st));
+                            if (jh >= nbin_st) jh = nbin_st - 1;
+                            hidx += jh * stride;
+                        }
+                        out[((b * nt + ti) * nt + tj) * hs + hidx] += incr;
+                    } else {
+                        for (int l = 1; l <= lmax; l++)
+                            out[(((b * nfs + i / skip) * N + a) * nt + tj) * (size_t)lmax + (size_t)(l - 1)] = q[l];
+                    }
+                }
+        }
+    }
+    free(dens);
+    free(cnt);
+    free(dr);
     return 0;
 }

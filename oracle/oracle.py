@@ -81,6 +81,9 @@ def lib():
                                            ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, _dp]
         L.orc_sph_corr_from_densities.argtypes = [ctypes.POINTER(OrcTraj), _dp, ctypes.c_uint, ctypes.c_int,
                                                   ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _dp]
+        L.orc_steinhardt.restype = ctypes.c_int
+        L.orc_steinhardt.argtypes = [ctypes.POINTER(OrcTraj), _dp, ctypes.c_uint, ctypes.c_int, ctypes.c_uint, _up,
+                                     ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -212,6 +215,21 @@ def sph_corr_from_densities(tr, dens, nbin, leff, skip, n, lmax=10):
     nt = tr.ntypes
     out = np.zeros((leff, nt, nt, nbin, lmax + 1), dtype=np.float64)
     lib().orc_sph_corr_from_densities(tr.ref, _p(dens), nbin, lmax, leff, skip, n, _p(out))
+    return out
+
+
+def steinhardt(tr, rminmax, nbin, lmax, nbin_st, l_list, skip, n, primo, do_hist):
+    """Steinhardt reset(n) + calculate(primo): histogram [nbin][nt][nt][nbin_st]*len(l_list)
+    (values count * incr) or q_l [nbin][n/skip][N][nt][lmax] (l = 1..lmax)."""
+    rm = _rminmax(rminmax, tr.ntypes)
+    ll = np.ascontiguousarray(l_list, dtype=np.uint32)
+    nt, skip = tr.ntypes, max(skip, 1)
+    shape = (nbin, nt, nt) + (nbin_st,) * len(ll) if do_hist else (nbin, n // skip, tr.natoms, nt, lmax)
+    out = np.zeros(shape, dtype=np.float64)
+    rc = lib().orc_steinhardt(tr.ref, _p(rm), nbin, lmax, nbin_st, _p(ll, _up), len(ll), skip, n, primo,
+                              int(bool(do_hist)), _p(out))
+    if rc != 0:
+        raise RuntimeError("trajectory is too short for this kind of calculation")
     return out
 
 

@@ -356,3 +356,35 @@ def test_spherical_correlations_lammps_binary_window(pa, tmp_path):
         s0.calculate(primo)
         s1.calculate(primo)
         _sh_close(np.array(s1), np.array(s0))
+
+
+def test_steinhardt_reference_fixture_call(pa):
+    """SteinhardtFixture of tests/src/test_sh.cpp:37-48 through the Python
+    class: [(0.5, 3.0)]*4 for 2 types, 4 bins, histogram size 100, l = (4, 6),
+    skip 17, histogram on; reset(all loaded timesteps), calculate(0).  Equal to
+    the oracle; the q trajectory form (do_histogram False) too."""
+    F, N = 40, 90
+    box_l = synth.lammps_box(8.0)
+    pos = synth.random_walk(71, N, F, box_l, 0.1)
+    types = synth.types_by_fraction(N, [0.7, 0.3]).astype(np.int32)
+    t = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False)
+    rm = [(0.5, 3.0)] * 4
+    st = pa.SteinhardtOrderParameterHistogram_6(t, rm, 4, 100, [4, 6], 2, 17, True, [], False)
+    st.reset(F)
+    st.calculate(0)
+    h = np.array(st, copy=True)
+    assert h.shape == (4, 2, 2, 100, 100)
+    ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), False)
+    np.testing.assert_allclose(h, O.steinhardt(ot, rm, 4, 6, 100, [4, 6], 17, F, 0, True), rtol=1e-13, atol=0)
+    tq = pa.SteinhardtOrderParameterHistogram_6(t, rm, 4, 100, [4, 6], 2, 17, False, [], False)
+    tq.reset(F)
+    tq.calculate(0)
+    q = np.array(tq, copy=True)
+    assert q.shape == (4, F // 17, N, 2, 6)
+    assert np.abs(q - O.steinhardt(ot, rm, 4, 6, 100, [4, 6], 17, F, 0, False)).max() <= 1e-12
+    with pytest.raises(RuntimeError, match="averaged steinhardt"):
+        pa.SteinhardtOrderParameterHistogram_6(t, rm, 4, 100, [4, 6], 2, 17, True, [], True)
+    with pytest.raises(RuntimeError, match="set nbin to 1"):
+        pa.SteinhardtOrderParameterHistogram_6(t, rm, 4, 100, [4, 6], 2, 17, True, [(19, 16.0, 16.0)], False)
+    with pytest.raises(RuntimeError, match="too short"):
+        st.calculate(1)

@@ -1,5 +1,7 @@
-"""Spherical-harmonic density correlations on the GPU (anl_sph_*) against the
-oracle (sphericalbase.cpp:19-66, sphericalcorrelations.cpp:62-270).
+"""Spherical-harmonic density correlations and Steinhardt order parameters on
+the GPU (anl_sph_*, anl_steinhardt_calculate) against the oracle
+(sphericalbase.cpp:19-66, sphericalcorrelations.cpp:62-270,
+steinhardt.cpp:90-236).
 
 FP64 throughout; the device sums in a different order than the reference
 (densities in slot order, products as a Gram matrix over (atom, m), t1 sums
@@ -8,7 +10,8 @@ array's largest magnitude.  Densities and correlations are checked against
 the oracle at small sizes (orthorhombic and triclinic cells, 1-3 types,
 skip/tmax/primo variations, several density chunks), and the full
 2,048-atom workload through its own snapshots: every lag equals the numpy
-contraction of the device densities of its frames."""
+contraction of the device densities of its frames.  Steinhardt q values agree
+to 1e-12 and their histograms equal the oracle's."""
 import numpy as np
 import pytest
 
@@ -127,3 +130,49 @@ def test_fullsize_lags_equal_snapshot_contractions():
         want[d] /= len(t1s)
     _close(got, want, 1e-11)
     assert np.all(got[0, ..., 10] > 0)
+
+
+# ---------------------------------------------------------------------------
+# Steinhardt order parameters (steinhardt.cpp:90-236)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("tri,frac,skip,n,primo", [(False, [0.5, 0.5], 2, 7, 1), (True, [0.6, 0.3, 0.1], 3, 9, 0),
+                                                  (False, [1.0], 1, 3, 4)])
+def test_steinhardt_matches_oracle(tri, frac, skip, n, primo):
+    """q_1..q_lmax of every (bin, frame, atom, type_j) within 1e-12 (q is at
+    most 1); the (q4, q6) and (q2, q4, q6) histograms equal the oracle's."""
+    ot, dt = _case(59, 80, 12, 8.5, frac, tri)
+    rm = _ranges(ot.ntypes, 13)
+    for lmax in (10, 6):
+        s = capi.DeviceSph(dt, rm, 3, lmax)
+        got = s.steinhardt(n, skip, primo, [4, 6], 16, do_histogram=False)
+        want = O.steinhardt(ot, rm, 3, lmax, 16, [4, 6], skip, n, primo, False)
+        assert got.shape == want.shape
+        assert np.abs(got - want).max() <= 1e-12
+        for ll, nb in (([4, 6], 16), ([2, 4, 6], 7), ([], 5)):
+            h = s.steinhardt(n, skip, primo, ll, nb, do_histogram=True)
+            hw = O.steinhardt(ot, rm, 3, lmax, nb, ll, skip, n, primo, True)
+            assert h.shape == hw.shape and h.sum() > 0
+            np.testing.assert_allclose(h, hw, rtol=1e-13, atol=0)
+
+
+def test_steinhardt_fcc_and_errors():
+    """A jittered fcc crystal (12 neighbours in (1.2, 1.6) A): q4 and q6 of
+    every atom are the tabulated 0.191 and 0.575; errors as the reference."""
+    g = np.array([[i, j, k] for i in range(3) for j in range(3) for k in range(3)], dtype=float)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    pos = (np.concatenate([(g + b) * 2.0 for b in basis]) + 0.1)[None]
+    pos = pos + 1e-5 * synth.normal(3, 5, pos.size).reshape(pos.shape)
+    pos = np.repeat(pos, 3, axis=0)
+    N = pos.shape[1]
+    ot = O.Traj(pos, np.zeros(N, dtype=np.int32), np.tile(synth.lammps_box(6.0), (3, 1)), False)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    s = capi.DeviceSph(dt, [(1.2, 1.6)], 1, 6)
+    q = s.steinhardt(3, 1, 0, [4, 6], 10, do_histogram=False)
+    assert np.all(np.abs(q[..., 3] - 0.191) < 1e-3) and np.all(np.abs(q[..., 5] - 0.575) < 1e-3)
+    h = s.steinhardt(3, 1, 0, [4, 6], 10)
+    assert h[0, 0, 0, 1, 5] == pytest.approx(N * 1.0)   # floor(0.191*10) = 1, floor(0.575*10) = 5; 3 frames x incr 1/3
+    with pytest.raises(capi.AnlError, match="too short"):
+        s.steinhardt(3, 1, 1, [4, 6], 10)
+    with pytest.raises(capi.AnlError, match="Wrong value of l"):
+        s.steinhardt(3, 1, 0, [4, 8], 10)

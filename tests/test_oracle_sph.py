@@ -152,3 +152,90 @@ def test_correlations_length_check():
     with pytest.raises(RuntimeError, match="too short"):
         O.sph_correlations(tr, [(0.5, 3.0)], 2, 3, 1, 5, 1)     # 3 + 5 + 1 > 8
     O.sph_correlations(tr, [(0.5, 3.0)], 2, 3, 1, 5, 0)         # 3 + 5 + 0 = 8 fits
+
+
+# ---------------------------------------------------------------------------
+# Steinhardt order parameters (steinhardt.cpp:90-236)
+# ---------------------------------------------------------------------------
+
+def _crystal(kind, a, cells, shift=0.1):
+    """Positions of a periodic sc / bcc / fcc crystal, one frame."""
+    basis = {"sc": [[0, 0, 0]],
+             "bcc": [[0, 0, 0], [0.5, 0.5, 0.5]],
+             "fcc": [[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]}[kind]
+    g = np.array([[i, j, k] for i in range(cells) for j in range(cells) for k in range(cells)], dtype=float)
+    pos = np.concatenate([(g + np.array(b)) * a for b in basis]) + shift
+    return pos[None, :, :], cells * a
+
+
+def _q_scipy(vectors, l):
+    """q_l of a set of neighbour vectors from scipy's complex harmonics."""
+    v = np.asarray(vectors, dtype=float)
+    r = np.linalg.norm(v, axis=1)
+    theta, phi = np.arccos(v[:, 2] / r), np.arctan2(v[:, 1], v[:, 0])
+    qlm = np.array([special.sph_harm_y(l, m, theta, phi).sum() for m in range(-l, l + 1)]) / len(v)
+    return np.sqrt(4 * np.pi / (2 * l + 1) * np.sum(np.abs(qlm) ** 2))
+
+
+# (kind, nearest-neighbour shell, q4, q6 as tabulated by Steinhardt, Nelson and Ronchetti, PRB 28, 784 (1983))
+CRYSTALS = [("sc", (1.5, 2.5), 6, 0.764, 0.354), ("bcc", (1.5, 1.9), 8, 0.509, 0.629),
+            ("fcc", (1.2, 1.6), 12, 0.191, 0.575)]
+
+
+@pytest.mark.parametrize("kind,rng,nn,q4,q6", CRYSTALS)
+def test_steinhardt_crystals(kind, rng, nn, q4, q6):
+    """Perfect crystals, jittered by 1e-5 A (a neighbour exactly on the z
+    axis has rxy = 0 and the reference's cos(phi) = x/rxy is NaN there too):
+    every atom's q_l equals scipy's on its own neighbour vectors, the ideal
+    crystal's value to 1e-4 (odd l are first order in the jitter), and the
+    tabulated q4, q6 to their 3 digits."""
+    pos, L = _crystal(kind, 2.0, 3 if kind != "sc" else 4)
+    N = pos.shape[1]
+    pos = pos + 1e-5 * synth.normal(3, 5, pos.size).reshape(pos.shape)
+    tr = O.Traj(np.repeat(pos, 2, axis=0), np.zeros(N, dtype=np.int32), np.tile(synth.lammps_box(L), (2, 1)), False)
+    q = O.steinhardt(tr, [rng], 1, 6, 10, [4, 6], 1, 2, 0, False)
+    assert q.shape == (1, 2, N, 1, 6)
+    unit = {"sc": np.concatenate([np.eye(3), -np.eye(3)]),
+            "bcc": np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)]),
+            "fcc": np.array([[x, y, 0] for x in (-1, 1) for y in (-1, 1)] + [[x, 0, z] for x in (-1, 1)
+                            for z in (-1, 1)] + [[0, y, z] for y in (-1, 1) for z in (-1, 1)])}[kind]
+    assert len(unit) == nn
+    for i in (0, 1, N // 2, N - 1):
+        vecs = _neigh(tr, L, 0, i, [rng], 1)[(0, 0)]
+        assert len(vecs) == nn
+        for l in range(1, 7):
+            assert abs(q[0, 0, i, 0, l - 1] - _q_scipy(vecs, l)) < 1e-12
+    for l in range(1, 7):
+        np.testing.assert_allclose(q[0, :, :, 0, l - 1], _q_scipy(unit, l), rtol=0, atol=1e-4)
+    assert abs(q[0, 0, 0, 0, 3] - q4) < 1e-3 and abs(q[0, 0, 0, 0, 5] - q6) < 1e-3
+
+
+def test_steinhardt_histogram_layout_and_single_neighbour():
+    """The histogram is the (l4, l6) joint histogram of the per-atom q values
+    (floorf of the float product, clamped), row-major with the first l
+    slowest, weighted by incr = 1/int(n/skip); a lone neighbour has q_l = 1."""
+    tr, _ = _small_traj(19, 60, 10, 8.0, [0.5, 0.5])
+    rm = RMM2
+    n, skip, nb = 7, 2, 12
+    q = O.steinhardt(tr, rm, 2, 8, nb, [4, 6], skip, n, 1, False)     # [bin][frame][atom][type_j][l-1]
+    h = O.steinhardt(tr, rm, 2, 8, nb, [4, 6], skip, n, 1, True)      # [bin][type_i][type_j][q4][q6]
+    assert q.shape == (2, 3, 60, 2, 8) and h.shape == (2, 2, 2, nb, nb)
+    want = np.zeros_like(h)
+    incr = 1.0 / (n // skip)
+    counts = np.zeros(h.shape, dtype=np.int64)
+    for b in range(2):
+        for f in range(3):
+            for i in range(60):
+                for tj in range(2):
+                    nbrs = _neigh(tr, 8.0, 1 + 2 * f, i, rm, 2).get((tj, b), [])
+                    if not nbrs:
+                        assert np.all(q[b, f, i, tj] == 0)
+                        continue
+                    if len(nbrs) == 1:
+                        np.testing.assert_allclose(q[b, f, i, tj], 1.0, rtol=0, atol=1e-13)
+                    k = [min(int(np.floor(np.float32(q[b, f, i, tj, l - 1] * nb))), nb - 1) for l in (4, 6)]
+                    counts[b, tr.type_id[i], tj, k[0], k[1]] += 1
+    assert counts.sum() > 100
+    np.testing.assert_allclose(h, counts * incr, rtol=1e-13, atol=0)
+    with pytest.raises(RuntimeError, match="too short"):
+        O.steinhardt(tr, rm, 2, 8, nb, [4, 6], skip, 10, 1, True)
