@@ -30,6 +30,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -1071,6 +1072,12 @@ inline unsigned int n_timesteps_of<ReadLog_numpy<double>>(ReadLog_numpy<double> 
 
 template <class TR, class T, typename... Args>
 class BlockAverageG {
+    // trajectories that can read the next block's window ahead
+    template <class X, class = void>
+    struct reads_ahead : std::false_type {};
+    template <class X>
+    struct reads_ahead<X, std::void_t<decltype(std::declval<X &>().set_next_window(size_t()))>> : std::true_type {};
+
    public:
     BlockAverageG(TR *t, const unsigned int &numero_blocchi) : traiettoria(t), n_b(numero_blocchi) {}
     ~BlockAverageG() {
@@ -1091,6 +1098,8 @@ class BlockAverageG {
         calc->calcola_begin(s, calcolo);
         for (unsigned int iblock = 0; iblock < n_b; iblock++) {
             calcolo->reset(s);
+            if constexpr (reads_ahead<TR>::value)
+                if (iblock + 1 < n_b) traiettoria->set_next_window((size_t)(iblock + 1) * s);
             calcolo->calculate(iblock * s);
             calc->calculate(calcolo);
         }

@@ -7,6 +7,7 @@
 #pragma once
 
 #include <fcntl.h>
+#include <future>
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -123,6 +124,8 @@ class Trajectory {
             if (h0.natoms <= 0 || h0.natoms > (int64_t)INT32_MAX)
                 throw std::runtime_error("Error: invalid number of atoms in the first timestep");
             natoms = (size_t)h0.natoms;
+            triclinic = h0.triclinic != 0;  // the first timestep's cell shape holds for the file
+            box_stride = triclinic ? 9 : 6;
             init_types();
             n_timesteps = fsize / size0;
             offsets.assign(1, 0);
@@ -134,6 +137,7 @@ class Trajectory {
         }
     }
     ~Trajectory() {
+        cancel_prefetch();
         if (dev) anl_traj_destroy(dev);
         if (file) munmap((void *)file, fsize);
         if (fd != -1) close(fd);
@@ -145,6 +149,7 @@ class Trajectory {
     Errori set_data_access_block_size(size_t n) {
         if (!ok) return non_inizializzato;
         if (loaded_timesteps == n) return Ok;
+        cancel_prefetch();
         dati_caricati = false;
         loaded_timesteps = n;
         pos_.assign(n * natoms * 3, 0.0);
@@ -158,54 +163,68 @@ class Trajectory {
     // trajectory.cpp:458-650: reads timesteps [ts, ts + window) into the
     // window: boxes to the internal form, positions and velocities by atom id,
     // per-type centre of mass (running mean of the raw values, in file order),
-    // then the optional pbc wrap.
+    // then the optional pbc wrap.  A window read ahead by prefetch(ts) is
+    // adopted instead of reading it again.
     Errori set_access_at(size_t ts) {
         if (!ok) return non_inizializzato;
         if (ts == current_timestep && dati_caricati) return Ok;
-        dati_caricati = false;
-        std::vector<size_t> cont(ntypes);
-        for (size_t k = 0; k < loaded_timesteps; k++) {
-            const size_t off = offset_of(ts + k);
-            lammps_bin::Header h;
-            std::vector<std::pair<const char *, int>> chunks;
-            read_step(off, h, &chunks);
-            double *c = &boxes[k * box_stride];
-            for (int i = 0; i < 6; i++) c[i] = h.box[i];
-            if (triclinic)
-                for (int i = 0; i < 3; i++) c[6 + i] = h.tilt[i];
-            Trajectory_numpy::lammps_to_internal(c);
-            double *pc = &pcm_[k * ntypes * 3], *vc = &vcm_[k * ntypes * 3];
-            std::fill(pc, pc + ntypes * 3, 0.0);
-            std::fill(vc, vc + ntypes * 3, 0.0);
-            std::fill(cont.begin(), cont.end(), 0);
-            for (auto &ch : chunks)
-                for (int ia = 0; ia < ch.second; ia++) {
-                    double row[lammps_bin::NDOUBLE_ATOMO];
-                    std::memcpy(row, ch.first + (size_t)ia * sizeof(row), sizeof(row));
-                    auto it = id_map.find((int)std::round(row[0]));
-                    if (it == id_map.end()) throw std::out_of_range("map::at");
-                    const size_t id = it->second;
-                    double *x = &pos_[(k * natoms + id) * 3], *v = &vel_[(k * natoms + id) * 3];
-                    for (int q = 0; q < 3; q++) {
-                        x[q] = row[2 + q];
-                        v[q] = row[5 + q];
-                    }
-                    const unsigned int t = type_id[id];
-                    cont[t]++;
-                    for (int q = 0; q < 3; q++) {
-                        pc[3 * t + q] += (x[q] - pc[3 * t + q]) / (double)cont[t];
-                        vc[3 * t + q] += (v[q] - vc[3 * t + q]) / (double)cont[t];
-                    }
-                }
-            if (wrap_pbc) pbc_wrap(k);
+        if (ahead.valid()) {
+            Window w = ahead.get();  // rethrows the reader's error
+            if (w.start == ts && w.pos.size() == loaded_timesteps * natoms * 3) {
+                adopt(w);
+                return Ok;
+            }
+            if (w.dev) anl_traj_destroy(w.dev);
         }
-        dati_caricati = true;
-        current_timestep = ts;
-        generation_++;
+        dati_caricati = false;
+        Window w;
+        w.start = ts;
+        load_window(window_offsets(ts), w);
+        adopt(w);
         return Ok;
     }
 
-    void set_pbc_wrap(bool p) { wrap_pbc = p; }
+    // Starts reading the window at ts, and building its device mirror, on a
+    // host thread: the reads, the host-side ordering and the uploads (on a
+    // stream of their own) overlap the work on the current window.  The next
+    // set_access_at(ts) adopts it; another prefetch or a window size change
+    // discards it.  A hint only: results never depend on it.
+    void prefetch(size_t ts) {
+        if (!ok || loaded_timesteps == 0 || !prefetching) return;
+        if (ts + loaded_timesteps > n_timesteps) return;
+        if (ahead.valid() && ahead_start == ts) return;
+        cancel_prefetch();
+        ahead_start = ts;
+        std::vector<size_t> offs = window_offsets(ts);  // indexes the file here, not on the reader
+        ahead = std::async(std::launch::async, [this, ts, offs]() {
+            Window w;
+            w.start = ts;
+            load_window(offs, w);
+            std::vector<int32_t> tid(type_id.begin(), type_id.end());
+            anl_check(anl_traj_create(w.pos.data(), w.boxes.data(), (uint32_t)box_stride, tid.data(),
+                                      (uint32_t)ntypes, w.pcm.data(), loaded_timesteps, natoms, &w.dev));
+            return w;
+        });
+    }
+    void cancel_prefetch() {
+        if (!ahead.valid()) return;
+        try {
+            Window w = ahead.get();
+            if (w.dev) anl_traj_destroy(w.dev);
+        } catch (...) {
+            // a discarded window's error does not concern the caller
+        }
+    }
+    // prefetch() on or off (on by default)
+    void set_prefetch(bool on) {
+        prefetching = on;
+        if (!on) cancel_prefetch();
+    }
+
+    void set_pbc_wrap(bool p) {
+        cancel_prefetch();
+        wrap_pbc = p;
+    }
     bool get_pbc_wrap() const { return wrap_pbc; }
 
     // trajectory.h:44-104: absolute timesteps; a timestep outside the window
@@ -282,7 +301,19 @@ class Trajectory {
             if (n_timesteps >= loaded_timesteps) start = std::min(primo, n_timesteps - loaded_timesteps);
             if (set_access_at(start) != Ok) throw std::runtime_error("Error loading the file\n");
         }
+        if (next_set) {  // the caller's next calculation: read its window ahead
+            next_set = false;
+            if (!(dati_caricati && next_start == current_timestep)) prefetch(next_start);
+        }
         return primo - current_timestep;
+    }
+    // The first timestep of the calculation after the current one (block
+    // loops know it): once the current calculation's window is in place, the
+    // window that one will need is read ahead with prefetch().
+    void set_next_window(size_t primo) {
+        if (loaded_timesteps == 0 || loaded_timesteps > n_timesteps) return;
+        next_start = std::min(primo, n_timesteps - loaded_timesteps);  // window_offset's rule
+        next_set = true;
     }
     // changes whenever the window's contents change; operators rebuild their
     // device handles when it does
@@ -304,9 +335,80 @@ class Trajectory {
     }
 
    private:
+    // one window's host arrays (and, when read ahead, its device mirror)
+    struct Window {
+        size_t start = 0;
+        std::vector<double> pos, vel, pcm, vcm, boxes;
+        anl_traj *dev = nullptr;
+    };
+    std::vector<size_t> window_offsets(size_t ts) {
+        std::vector<size_t> offs(loaded_timesteps);
+        for (size_t k = 0; k < loaded_timesteps; k++) offs[k] = offset_of(ts + k);
+        return offs;
+    }
+    // reads the timesteps at offs into w; touches no member (safe on the
+    // prefetch thread: the file, the atom map and the sizes are read-only
+    // once the window size is set)
+    void load_window(const std::vector<size_t> &offs, Window &w) const {
+        const size_t nl = offs.size();
+        w.pos.assign(nl * natoms * 3, 0.0);
+        w.vel.assign(nl * natoms * 3, 0.0);
+        w.pcm.assign(nl * ntypes * 3, 0.0);
+        w.vcm.assign(nl * ntypes * 3, 0.0);
+        w.boxes.assign(nl * box_stride, 0.0);
+        std::vector<size_t> cont(ntypes);
+        for (size_t k = 0; k < nl; k++) {
+            lammps_bin::Header h;
+            std::vector<std::pair<const char *, int>> chunks;
+            read_step(offs[k], h, &chunks);
+            double *c = &w.boxes[k * box_stride];
+            for (int i = 0; i < 6; i++) c[i] = h.box[i];
+            if (triclinic)
+                for (int i = 0; i < 3; i++) c[6 + i] = h.tilt[i];
+            Trajectory_numpy::lammps_to_internal(c);
+            double *pc = &w.pcm[k * ntypes * 3], *vc = &w.vcm[k * ntypes * 3];
+            std::fill(cont.begin(), cont.end(), 0);
+            for (auto &ch : chunks)
+                for (int ia = 0; ia < ch.second; ia++) {
+                    double row[lammps_bin::NDOUBLE_ATOMO];
+                    std::memcpy(row, ch.first + (size_t)ia * sizeof(row), sizeof(row));
+                    auto it = id_map.find((int)std::round(row[0]));
+                    if (it == id_map.end()) throw std::out_of_range("map::at");
+                    const size_t id = it->second;
+                    double *x = &w.pos[(k * natoms + id) * 3], *v = &w.vel[(k * natoms + id) * 3];
+                    for (int q = 0; q < 3; q++) {
+                        x[q] = row[2 + q];
+                        v[q] = row[5 + q];
+                    }
+                    const unsigned int t = type_id[id];
+                    cont[t]++;
+                    for (int q = 0; q < 3; q++) {
+                        pc[3 * t + q] += (x[q] - pc[3 * t + q]) / (double)cont[t];
+                        vc[3 * t + q] += (v[q] - vc[3 * t + q]) / (double)cont[t];
+                    }
+                }
+            if (wrap_pbc) pbc_wrap(&w.pos[k * natoms * 3], c);
+        }
+    }
+    void adopt(Window &w) {
+        pos_.swap(w.pos);
+        vel_.swap(w.vel);
+        pcm_.swap(w.pcm);
+        vcm_.swap(w.vcm);
+        boxes.swap(w.boxes);
+        dati_caricati = true;
+        current_timestep = w.start;
+        generation_++;
+        if (w.dev) {
+            if (dev) anl_traj_destroy(dev);
+            dev = w.dev;
+            dev_generation = generation_;
+        }
+    }
+
     // trajectory.cpp:237-290: header and chunk table of the timestep at off;
     // returns its size in bytes
-    size_t read_step(size_t off, lammps_bin::Header &h, std::vector<std::pair<const char *, int>> *chunks) {
+    size_t read_step(size_t off, lammps_bin::Header &h, std::vector<std::pair<const char *, int>> *chunks) const {
         if (off + lammps_bin::HEADER_GUARD > fsize) throw std::runtime_error("Error: trying to read beyond the end of file");
         const char *end = file + fsize;
         size_t n = h.read(file + off, end);
@@ -314,10 +416,6 @@ class Trajectory {
             std::stringstream ss;
             ss << "Error: at the LAMMPS timestep " << h.timestep << " the number of atoms has changed \n";
             throw std::runtime_error(ss.str());
-        }
-        if (off == 0) {
-            triclinic = h.triclinic != 0;
-            box_stride = triclinic ? 9 : 6;
         }
         if (h.nchunk <= 0) throw std::runtime_error("Error: the number of chunks of the timestep cannot be <= 0");
         for (int i = 0; i < h.nchunk; i++) {
@@ -405,12 +503,11 @@ class Trajectory {
             return base + (t - current_timestep) * s1 + i * s2;
         throw std::runtime_error("requested timestep is out of range");
     }
-    // basetrajectory.h:145-161
-    void pbc_wrap(size_t k) {
-        double *c = &boxes[k * box_stride];
+    // basetrajectory.h:145-161: one frame's positions p around the cell c
+    void pbc_wrap(double *p, const double *c) const {
         const double tilt0[3] = {0, 0, 0};
         for (size_t ia = 0; ia < natoms; ++ia) {
-            double *xa = &pos_[(k * natoms + ia) * 3];
+            double *xa = &p[ia * 3];
             for (int q = 0; q < 3; ++q) xa[q] = xa[q] - c[3 + q];
             Trajectory_numpy::minImage_triclinic(xa, c + 3, triclinic ? c + 6 : tilt0, triclinic);
             for (int q = 0; q < 3; ++q) xa[q] = xa[q] + c[3 + q];
@@ -430,6 +527,9 @@ class Trajectory {
     std::vector<double> pos_, vel_, pcm_, vcm_, boxes;
     uint64_t generation_ = 0, dev_generation = 0;
     anl_traj *dev = nullptr;
+    std::future<Window> ahead;  // window being read ahead by prefetch()
+    size_t ahead_start = 0, next_start = 0;
+    bool prefetching = true, next_set = false;
 };
 
 }  // namespace analisi

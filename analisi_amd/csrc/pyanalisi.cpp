@@ -287,6 +287,8 @@ PYBIND11_MODULE(_pyanalisi, m) {
         .def("setWrapPbc", &Trajectory::set_pbc_wrap)
         .def("setAccessWindowSize", [](Trajectory &t, int ts) { return (int)t.set_data_access_block_size((size_t)ts); })
         .def("setAccessStart", [](Trajectory &t, int ts) { return (int)t.set_access_at((size_t)ts); })
+        .def("setPrefetch", &Trajectory::set_prefetch, py::arg("on"))
+        .def("prefetch", [](Trajectory &t, size_t ts) { t.prefetch(ts); }, py::arg("timestep"))
         .def("serving_pos", &Trajectory::serving_pos)
         .def("toggle_pos_vel", &Trajectory::toggle_pos_vel)
         .def_buffer([](Trajectory &t) {

@@ -197,6 +197,62 @@ const char *anl_info(void) {
            "restates rikigigi/analisi v0.5.2 behaviour (Gofrt, MSD, GreenKuboNComponentIonicFluid, MediaVar)";
 }
 
+}  // extern "C"
+
+namespace anl {
+
+// Host-to-device copies of anl_traj_create: a non-blocking stream of its own,
+// so that they do not wait for kernels on the default stream (a trajectory
+// window prepared on a host thread while the current window's kernels run),
+// and two pinned staging buffers filled and copied alternately.
+struct Uploader {
+    hipStream_t s = nullptr;
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    size_t cap;
+    int k = 0;
+    explicit Uploader(size_t bytes) : cap(std::max<size_t>(bytes, 1 << 16)) {
+        ANL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        for (int i = 0; i < 2; i++) {
+            ANL_HIP(hipHostMalloc(&stage[i], cap, hipHostMallocDefault));
+            ANL_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+        }
+    }
+    ~Uploader() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (int i = 0; i < 2; i++) {
+            if (done[i]) (void)hipEventDestroy(done[i]);
+            if (stage[i]) (void)hipHostFree(stage[i]);
+        }
+        if (s) (void)hipStreamDestroy(s);
+    }
+    Uploader(const Uploader &) = delete;
+    Uploader &operator=(const Uploader &) = delete;
+    // the staging buffer of the next copy, once its previous copy has ended
+    void *next() {
+        ANL_HIP(hipEventSynchronize(done[k]));
+        return stage[k];
+    }
+    // copies the first bytes of that buffer to dst
+    void send(void *dst, size_t bytes) {
+        ANL_HIP(hipMemcpyAsync(dst, stage[k], bytes, hipMemcpyHostToDevice, s));
+        ANL_HIP(hipEventRecord(done[k], s));
+        k ^= 1;
+    }
+    void put(void *dst, const void *src, size_t bytes) {
+        for (size_t off = 0; off < bytes; off += cap) {
+            const size_t n = std::min(cap, bytes - off);
+            std::memcpy(next(), (const char *)src + off, n);
+            send((char *)dst + off, n);
+        }
+    }
+    void finish() { ANL_HIP(hipStreamSynchronize(s)); }
+};
+
+}  // namespace anl
+
+extern "C" {
+
 int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, const int32_t *type_id,
                     uint32_t ntypes, const double *pos_cm, uint64_t nframes, uint64_t natoms, anl_traj **out) {
     return guard([&] {
@@ -257,23 +313,24 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
             tr->type_of_slot[s] = type_id[order[s]];
             tr->type_count[type_id[order[s]]]++;
         }
-        // permuted, padded positions, uploaded frame by frame
+        // permuted, padded positions, staged frame by frame in pinned memory
+        Uploader up(natoms * sizeof(double4));
         ANL_HIP(hipMalloc(&tr->pos, nframes * natoms * sizeof(double4)));
-        std::vector<double4> buf(natoms);
         for (uint64_t f = 0; f < nframes; f++) {
             const double *p = pos + f * natoms * 3;
+            double4 *buf = (double4 *)up.next();
             for (uint64_t s = 0; s < natoms; s++) {
                 const double *x = p + (uint64_t)order[s] * 3;
                 buf[s] = make_double4(x[0], x[1], x[2], 0.0);
             }
-            ANL_HIP(hipMemcpy(tr->pos + f * natoms, buf.data(), natoms * sizeof(double4), hipMemcpyHostToDevice));
+            up.send(tr->pos + f * natoms, natoms * sizeof(double4));
         }
         ANL_HIP(hipMalloc(&tr->box, nframes * 9 * sizeof(double)));
-        ANL_HIP(hipMemcpy(tr->box, tr->box_host.data(), nframes * 9 * sizeof(double), hipMemcpyHostToDevice));
+        up.put(tr->box, tr->box_host.data(), nframes * 9 * sizeof(double));
         ANL_HIP(hipMalloc(&tr->type_slot, natoms * sizeof(int32_t)));
-        ANL_HIP(hipMemcpy(tr->type_slot, tr->type_of_slot.data(), natoms * sizeof(int32_t), hipMemcpyHostToDevice));
+        up.put(tr->type_slot, tr->type_of_slot.data(), natoms * sizeof(int32_t));
         ANL_HIP(hipMalloc(&tr->orig_slot, natoms * sizeof(int32_t)));
-        ANL_HIP(hipMemcpy(tr->orig_slot, order.data(), natoms * sizeof(int32_t), hipMemcpyHostToDevice));
+        up.put(tr->orig_slot, order.data(), natoms * sizeof(int32_t));
         // screen order, per frame (the screened g(r,t) needs a constant
         // cell): the same bisection of that frame's positions, so that the
         // atoms of one wavefront stay compact in every frame while they
@@ -300,12 +357,13 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
                 });
             for (auto &t : pool) t.join();
             ANL_HIP(hipMalloc(&tr->frac_order, nframes * natoms * sizeof(int32_t)));
-            ANL_HIP(hipMemcpy(tr->frac_order, ford.data(), nframes * natoms * sizeof(int32_t), hipMemcpyHostToDevice));
+            up.put(tr->frac_order, ford.data(), nframes * natoms * sizeof(int32_t));
         }
         if (pos_cm) {
             ANL_HIP(hipMalloc(&tr->cm, nframes * ntypes * 3 * sizeof(double)));
-            ANL_HIP(hipMemcpy(tr->cm, pos_cm, nframes * ntypes * 3 * sizeof(double), hipMemcpyHostToDevice));
+            up.put(tr->cm, pos_cm, nframes * ntypes * 3 * sizeof(double));
         }
+        up.finish();
         *out = tr;
     });
 }

@@ -31,7 +31,8 @@ Other workloads (--config), each printing one JSON line of its own:
       on the matrix cores (split-BF16 MFMA) vs the FP64 form
   sh  SphericalCorrelations (lmax 10): 2,048 atoms 50/50 at 0.1 atoms/A^3
       (cubic 27.4 A), 2,200 frames, sigma 0.05 A; ranges (0.7, 3.5) A for every
-      type pair, 4 bins, lags 0..199, t0 stride 10, reset(2000), calculate(0)
+      type pair, 4 bins, lags 0..199, t0 stride 10, reset(2000), calculate(0);
+      also the Steinhardt (q4, q6) histogram of every frame of that trajectory
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config c2]
 """
@@ -681,7 +682,27 @@ def run_sh(args, hip):
                                          "(t0, lag) product %.4f s (mean of 4); the whole workload's %d frames and "
                                          "%d (t0, lag) pairs extrapolate to %.0f s" %
                                          (t_frame, nf + 1, t_pair, nframes, n0 * leff, est)}
-    log("sh: %.2f ms per calculate" % ms)
+    # Steinhardt histogram of (q4, q6), 100 bins each, over every frame
+    ll, nbs = [4, 6], 100
+    s.steinhardt(F, 1, 0, ll, nbs)                  # untimed
+    hip.sync()
+    hip.record(e0, None)
+    for _ in range(reps):
+        hst = s.steinhardt(F, 1, 0, ll, nbs)
+    hip.record(e1, None)
+    ms_st = hip.elapsed_ms(e0, e1) / reps
+    assert hst.sum() > 0
+    res["steinhardt"] = {"workload": "SteinhardtOrderParameterHistogram_10 of the same trajectory: (q4, q6) "
+                                     "histogram, 100 x 100 cells, every one of the %d frames, 4 bins" % F,
+                         "ms_per_step": ms_st, "frames_per_s": F / (ms_st * 1e-3)}
+    if not args.no_cpu:
+        t = time.perf_counter()
+        O.steinhardt(ot, w["rminmax"], nbin, lmax, nbs, ll, 1, 2, 0, True)
+        t_st = (time.perf_counter() - t) / 2
+        res["steinhardt"]["cpu_baseline"] = {"value": 1.0 / t_st, "unit": "frames/s", "cores": 1, "kind": "port",
+                                             "sample": "oracle, 2 frames of the same trajectory, %.3f s per frame"
+                                                       % t_st}
+    log("sh: %.2f ms per calculate, steinhardt %.2f ms for %d frames" % (ms, ms_st, F))
     s.close()
     traj.close()
     return res

@@ -388,3 +388,44 @@ def test_steinhardt_reference_fixture_call(pa):
         pa.SteinhardtOrderParameterHistogram_6(t, rm, 4, 100, [4, 6], 2, 17, True, [(19, 16.0, 16.0)], False)
     with pytest.raises(RuntimeError, match="too short"):
         st.calculate(1)
+
+
+def test_lammps_window_read_ahead_changes_nothing(pa, tmp_path):
+    """Block averages over a Traj with the next block's window read ahead on
+    a host thread (the default: its reads, ordering and uploads overlap the
+    current block's kernels) equal those with read-ahead off, bit for bit;
+    prefetch(ts) then setAccessStart(ts) serves what a direct read serves."""
+    N, F = 300, 60
+    box_l = synth.lammps_box(12.0, [0.9, -0.6, 0.4])
+    pos = synth.random_walk(47, N, F, box_l, 0.3)
+    vel = np.random.default_rng(5).normal(size=(F, N, 3))
+    types = synth.types_by_fraction(N, [0.6, 0.4]).astype(np.int32)
+    tnp = pa.Trajectory(pos, vel, types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsTriclinic, False, False)
+    path = str(tmp_path / "ahead.bin")
+    tnp.write_lammps_binary(path, 0, -1)
+    tl = pa.Traj(path)
+    tl.setAccessWindowSize(20)
+    tl.setAccessStart(0)
+    on = pa.gofrt_block_average(tl, 4, 0.0, 5.0, 30, 4, 1, 1)
+    mon = pa.msd_block_average(tl, 4, 1, 4, True, True)
+    tl.setPrefetch(False)
+    off = pa.gofrt_block_average(tl, 4, 0.0, 5.0, 30, 4, 1, 1)
+    moff = pa.msd_block_average(tl, 4, 1, 4, True, True)
+    for a, b in zip(on + mon, off + moff):
+        assert np.array_equal(np.nan_to_num(a), np.nan_to_num(b))
+    ref = pa.gofrt_block_average(tnp, 4, 0.0, 5.0, 30, 4, 1, 1)
+    assert np.array_equal(on[0], ref[0])
+    tl.setPrefetch(True)
+    tl.prefetch(33)
+    tl.setAccessStart(33)
+    direct = pa.Traj(path)
+    direct.setAccessWindowSize(20)
+    direct.setAccessStart(33)
+    assert np.array_equal(np.array(tl), np.array(direct))
+    g = pa.Gofrt_lammps(tl, 0.0, 5.0, 30, 3, 1, 1, 1, False)
+    h = pa.Gofrt_lammps(direct, 0.0, 5.0, 30, 3, 1, 1, 1, False)
+    g.reset(10)
+    h.reset(10)
+    g.calculate(35)
+    h.calculate(35)
+    assert np.array_equal(np.array(g), np.array(h))
