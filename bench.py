@@ -33,6 +33,9 @@ Other workloads (--config), each printing one JSON line of its own:
       (cubic 27.4 A), 2,200 frames, sigma 0.05 A; ranges (0.7, 3.5) A for every
       type pair, 4 bins, lags 0..199, t0 stride 10, reset(2000), calculate(0);
       also the Steinhardt (q4, q6) histogram of every frame of that trajectory
+  lammps  g(r,t) block average (6 blocks, lags 0..40, 256 bins to 12 A) over a
+      LAMMPS-binary file of 8,192 atoms x 600 frames read in windows, with the
+      next block's window read ahead during the current block, and without
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config c2]
 """
@@ -708,12 +711,53 @@ def run_sh(args, hip):
     return res
 
 
+def run_lammps_feed(args, hip):
+    """gofrt_block_average over a memory-mapped LAMMPS-binary trajectory
+    (Traj): each block moves the window; with read-ahead the next block's
+    window is read, ordered and uploaded on a host thread while the current
+    block's kernels run.  Both modes give bit-identical means and variances."""
+    import tempfile
+    from analisi_amd import pyanalisi as pa, synth
+    N, F, L, nb, tmax = 8192, 600, 45.0, 6, 41
+    box_l = synth.lammps_box(L)
+    pos = synth.random_walk(5, N, F, box_l, 0.05)
+    types = synth.types_by_fraction(N, [0.8, 0.1, 0.1]).astype(np.int32)
+    t = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False)
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    os.close(fd)
+    try:
+        t.write_lammps_binary(path, 0, -1)
+        del t
+        tl = pa.Traj(path)
+        s = (F - min(F // (nb + 1) + 1, tmax)) // nb
+        tl.setAccessWindowSize(s + tmax - 1)
+        tl.setAccessStart(0)
+        times, outs = {True: [], False: []}, {}
+        for mode in [True, False] * max(1, args.steps):
+            tl.setPrefetch(mode)
+            t0 = time.perf_counter()
+            outs[mode] = pa.gofrt_block_average(tl, nb, 0.0, 12.0, 256, tmax, 1, 1)
+            times[mode].append(time.perf_counter() - t0)
+        same = all(np.array_equal(np.nan_to_num(a), np.nan_to_num(b)) for a, b in zip(outs[True], outs[False]))
+        size = os.path.getsize(path)
+    finally:
+        os.remove(path)
+    on, off = min(times[True]), min(times[False])
+    log("lammps feed: read-ahead %.3f s, without %.3f s" % (on, off))
+    return {"metric": "g(r,t) block average over a LAMMPS-binary file, wall time", "value": on, "unit": "s",
+            "n_gpus": 1, "steps": len(times[True]), "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "gofrt_block_average(Traj, %d blocks of %d t0, lags 0..%d, 256 bins to 12 A): "
+                                   "%d atoms 80/10/10, %d frames, cubic %g A, sigma 0.05 A (seed 5), %d MiB file, "
+                                   "window %d frames" % (nb, s, tmax - 1, N, F, L, size >> 20, s + tmax - 1)},
+            "without_read_ahead_s": off, "speedup": off / on, "bit_identical": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "sh"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "sh", "lammps"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-lags", type=int, default=4, help="lags per autotune probe launch")
@@ -747,6 +791,8 @@ def main():
         res = run_gk(args, hip)
     elif args.config == "sh":
         res = run_sh(args, hip)
+    elif args.config == "lammps":
+        res = run_lammps_feed(args, hip)
     else:
         w = gofrt_workload(args.config)
         if args.n:
