@@ -10,13 +10,16 @@
 //
 // Per chunk of t1 values (chunks bound the HBM held by the densities):
 //  1. sph_density_kernel: one wavefront per (frame, central atom).  The lanes
-//     take 64 other atoms at a time: FP64 minimum image in the reference's
-//     loop order, distance bin from exact host-computed edges of the pair's
-//     range, ballot of the atoms in range.  Each such atom, in slot order, is
-//     expanded by lanes m = 0..lmax (lane m runs the reference's Legendre
-//     recursion of column m and the Chebyshev recursion of cos/sin(m phi), in
-//     the same operation order) into the wave's density block in LDS.  The
-//     block is written once, as S[frame][type_j][bin][l][atom by type][m].
+//     take 64 other atoms at a time (whole 64-slot blocks are skipped by a
+//     bounding-sphere test in orthorhombic cells): FP64 minimum image in the
+//     reference's loop order, distance bin from exact host-computed edges of
+//     the pair's range, ballot of the atoms in range.  The atoms in range
+//     join a queue held one per lane, in slot order; 64/(lmax+1) at a time
+//     they are expanded by lanes (slot, m), lane m running the reference's
+//     Legendre recursion of column m and the Chebyshev recursion of
+//     cos/sin(m phi), and added into the wave's density block in LDS in queue
+//     order.  The block is written once, as
+//     S[frame][type_j][bin][l][atom by type][m].
 //  2. sph_gram_kernel: for every problem (type_i, type_j, bin, l) the lag
 //     products sum_{i of type_i} sum_m S_{t1} S_{t2} are a banded Gram matrix
 //     over frames with K = (atoms of type_i) x (2l+1) contiguous in HBM.  A
@@ -44,6 +47,13 @@ constexpr int SPH_WAVES = 4;           // wavefronts (central atoms) per density
 constexpr uint32_t SPH_WAVE_LDS = 2048;  // doubles of LDS per wavefront (16 KiB): density block + staging
 constexpr int GT = 64;                 // Gram tile edge (t1 rows, t2 columns)
 constexpr int GKC = 16;                // K values staged per step
+
+// doubles of a wavefront's staging area: the harmonic vectors of 64/(lmax+1)
+// atoms, and at least 64 queue entries of 4 doubles
+__host__ __device__ inline uint32_t stage_doubles(uint32_t lmax) {
+    const uint32_t h = 64 / (lmax + 1) * (lmax + 1) * (lmax + 1);
+    return h > 256 ? h : 256;
+}
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -113,7 +123,10 @@ __global__ __launch_bounds__(64) void sph_block_radius_kernel(const double4 *pos
 // (specialfunctions.h:11-22, 262-360): P_l^m for l = m..lmax and
 // cos/sin(m phi), written to y[l*l + l + m] (cos) and y[l*l + l - m] (sin);
 // cf[l] is the coefficient of (l, m).  The loops run to the largest lmax with
-// the lane's own range as a predicate, so cf stays in registers.
+// the lane's own range as a predicate, so cf stays in registers.  The
+// recursion multiplies by 1/(l-m), which depends on l and m only, where the
+// reference divides by l-m: the division leaves the dependency chain of the
+// recursion, and each step changes by at most one rounding.
 __device__ __forceinline__ void put_harmonics(double *y, double x0, double x1, double x2, int m, int lmax,
                                               const double (&cf)[ANL_SPH_LMAX_MAX + 1]) {
     const double rxy = sqrt(x0 * x0 + x1 * x1);
@@ -147,7 +160,7 @@ __device__ __forceinline__ void put_harmonics(double *y, double x0, double x1, d
             double p;
             if (l == m) p = pmm;
             else if (l == m + 1) p = pmm * ct * (2 * m + 1);
-            else p = (p1 * ct * (2 * l - 1) - p2 * (l + m - 1)) / (l - m);
+            else p = (p1 * ct * (2 * l - 1) - p2 * (l + m - 1)) * (1.0 / (double)(l - m));
             const double v = p * cf[l];
             y[l * l + l + m] = v * cm;
             if (m > 0) y[l * l + l - m] = v * sm;
@@ -158,15 +171,15 @@ __device__ __forceinline__ void put_harmonics(double *y, double x0, double x1, d
 }
 
 template <bool TRI>
-__global__ __launch_bounds__(256) void sph_density_kernel(SphDensArgs a) {
+__global__ __launch_bounds__(256, 3) void sph_density_kernel(SphDensArgs a) {
     extern __shared__ double sph_lds[];
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t L1 = a.lmax + 1, L2 = L1 * L1;
     const uint32_t accn = a.nt * a.nbc * L2;
     const uint32_t nc = 64 / L1;                       // candidates expanded at once
     const uint32_t slot = lane / L1, m = lane - slot * L1;
-    double *acc = sph_lds + wv * (accn + nc * L2);
-    double *stage = acc + accn;                        // [nc][L2]
+    double *acc = sph_lds + wv * (accn + stage_doubles(a.lmax));
+    double *stage = acc + accn;                        // [nc][L2] harmonics, or [64][4] new queue entries
     double cf[ANL_SPH_LMAX_MAX + 1];                   // coefficients of (l, m) of this lane's m
 #pragma unroll
     for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l)
@@ -186,70 +199,115 @@ __global__ __launch_bounds__(256) void sph_density_kernel(SphDensArgs a) {
     const int ti = a.type_slot[i];
     const double *thr_i = a.thr + (size_t)ti * a.nt * (a.nbin + 1);
     const double4 *rad = a.block_rad ? a.block_rad + (uint64_t)fl * ((a.N + 63) / 64) : nullptr;
+    // queue of atoms in range, in slot order: entry k in lane k (minimum-image
+    // vector and type << 16 | bin - b0); qn entries, wavefront-uniform
+    double qx0 = 0.0, qx1 = 0.0, qx2 = 0.0;
+    uint32_t qcode = 0, qn = 0;
+    // expand the nb oldest entries: lane (slot, m) takes entry slot into its
+    // staging block; the blocks are then added into the density in queue
+    // order (each entry by one lane, in order); the queue moves down by nb
+    auto expand = [&](uint32_t nb) {
+        const int src = slot < nb ? (int)slot : 0;
+        const double y0 = __shfl(qx0, src), y1 = __shfl(qx1, src), y2 = __shfl(qx2, src);
+        if (slot < nb) put_harmonics(stage + slot * L2, y0, y1, y2, (int)m, (int)a.lmax, cf);
+        wave_sync_lds();
+        for (uint32_t k = 0; k < nb; ++k) {
+            const uint32_t code = (uint32_t)__shfl((int)qcode, (int)k);
+            double *blk = acc + ((code >> 16) * a.nbc + (code & 0xFFFFu)) * L2;
+            for (uint32_t e = lane; e < L2; e += 64) blk[e] += stage[k * L2 + e];
+        }
+        wave_sync_lds();
+        const int up = (int)min(lane + nb, 63u);
+        qx0 = __shfl(qx0, up);
+        qx1 = __shfl(qx1, up);
+        qx2 = __shfl(qx2, up);
+        qcode = (uint32_t)__shfl((int)qcode, up);
+        qn -= nb;
+    };
     const uint64_t nblk = (a.N + 63) / 64;
-    // blocks of 64 slots, 64 blocks at a time: lane b decides whether block b
+    // Blocks of 64 slots, 64 blocks at a time: lane b decides whether block b
     // can hold an atom within rcut of atom i.  In orthorhombic cells the
     // minimum image is the nearest one, a metric, so a block whose bounding
-    // sphere's centre is farther than rcut + its radius cannot.
-    for (uint64_t g = 0; g < nblk; g += 64) {
-        bool near = g + lane < nblk;
-        if (rad && near) {
-            const double4 bc = rad[g + lane];
-            double x0 = pi.x - bc.x, x1 = pi.y - bc.y, x2 = pi.z - bc.z;
-            min_image_ref<false>(x0, x1, x2, c);
-            near = !(sqrt(d2_of(x0, x1, x2)) > a.rcut + bc.w);
-        }
-        unsigned long long blocks = __ballot(near);
-        while (blocks) {
-            const uint64_t jb = (g + (uint64_t)(__ffsll(blocks) - 1)) * 64;
-            blocks &= blocks - 1;
-            const uint64_t j = jb + lane;
-            bool in = false;
-            double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-            int tj = 0, bin = 0;
-            if (j < a.N && j != i) {
-                const double4 pj = P[j];
-                x0 = pi.x - pj.x;
-                x1 = pi.y - pj.y;
-                x2 = pi.z - pj.z;
-                min_image_ref<TRI>(x0, x1, x2, c);
-                const double d2 = d2_of(x0, x1, x2);
-                if (d2 >= a.d2lo && d2 < a.d2hi) {  // outer bounds over every type pair
-                    tj = a.type_slot[j];
-                    const double *T = thr_i + (size_t)tj * (a.nbin + 1);
-                    if (d2 >= T[b0] && d2 < T[b1]) {
-                        in = true;
-                        uint32_t b = b0;
-                        while (b + 1 < b1 && d2 >= T[b + 1]) ++b;
-                        bin = (int)b;
+    // sphere's centre is farther than rcut + its radius cannot.  The atoms in
+    // range of each near block join the queue (a block that does not fit
+    // waits until the queue is empty); the queue is expanded nc at a time,
+    // and emptied at the end.  One expansion site keeps the code compact.
+    uint64_t g = 0;
+    unsigned long long blocks = 0;
+    bool pending = false, finished = false, in = false;
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    uint32_t code = 0, cnt = 0;
+    unsigned long long mask = 0;
+    for (;;) {
+        if (!pending && !finished) {
+            while (blocks == 0 && g < nblk) {
+                bool near = g + lane < nblk;
+                if (rad && near) {
+                    const double4 bc = rad[g + lane];
+                    double y0 = pi.x - bc.x, y1 = pi.y - bc.y, y2 = pi.z - bc.z;
+                    min_image_ref<false>(y0, y1, y2, c);
+                    near = !(sqrt(d2_of(y0, y1, y2)) > a.rcut + bc.w);
+                }
+                blocks = __ballot(near);
+                if (blocks == 0) g += 64;
+            }
+            if (blocks == 0) {
+                finished = true;
+            } else {
+                const uint64_t jb = (g + (uint64_t)(__ffsll(blocks) - 1)) * 64;
+                blocks &= blocks - 1;
+                if (blocks == 0) g += 64;
+                const uint64_t j = jb + lane;
+                in = false;
+                x0 = x1 = x2 = 0.0;
+                code = 0;
+                if (j < a.N && j != i) {
+                    const double4 pj = P[j];
+                    x0 = pi.x - pj.x;
+                    x1 = pi.y - pj.y;
+                    x2 = pi.z - pj.z;
+                    min_image_ref<TRI>(x0, x1, x2, c);
+                    const double d2 = d2_of(x0, x1, x2);
+                    if (d2 >= a.d2lo && d2 < a.d2hi) {  // outer bounds over every type pair
+                        const uint32_t tj = (uint32_t)a.type_slot[j];
+                        const double *T = thr_i + (size_t)tj * (a.nbin + 1);
+                        if (d2 >= T[b0] && d2 < T[b1]) {
+                            in = true;
+                            uint32_t b = b0;
+                            while (b + 1 < b1 && d2 >= T[b + 1]) ++b;
+                            code = tj << 16 | (b - b0);  // b - b0 < nbc <= SPH_WAVE_LDS
+                        }
                     }
                 }
-            }
-            unsigned long long mask = __ballot(in);
-            // candidates in slot order, nc at a time: lane (slot, m) expands the
-            // slot-th one into its staging block, then the blocks are added into
-            // the density in slot order (each entry by one lane, in order)
-            while (mask) {
-                int src = -1;
-                uint32_t nb = 0;
-                for (; nb < nc && mask; ++nb) {
-                    const int k = __ffsll(mask) - 1;
-                    mask &= mask - 1;
-                    if (slot == nb) src = k;
-                }
-                const int sl = src < 0 ? 0 : src;
-                const double y0 = __shfl(x0, sl), y1 = __shfl(x1, sl), y2 = __shfl(x2, sl);
-                const int jt = __shfl(tj, sl), bb = __shfl(bin, sl);
-                if (src >= 0) put_harmonics(stage + slot * L2, y0, y1, y2, (int)m, (int)a.lmax, cf);
-                wave_sync_lds();
-                for (uint32_t k = 0; k < nb; ++k) {
-                    const uint32_t jk = (uint32_t)__shfl(jt, (int)(k * L1)), bk = (uint32_t)__shfl(bb, (int)(k * L1));
-                    double *blk = acc + (jk * a.nbc + bk - b0) * L2;
-                    for (uint32_t e = lane; e < L2; e += 64) blk[e] += stage[k * L2 + e];
-                }
-                wave_sync_lds();
+                mask = __ballot(in);
+                cnt = (uint32_t)__popcll(mask);
+                pending = cnt > 0;
             }
         }
+        if (pending && qn + cnt <= 64) {
+            // append in slot order: compact through LDS, lanes qn..qn+cnt-1 take them
+            if (in) {
+                double *e = stage + 4 * __popcll(mask & ((1ull << lane) - 1));
+                e[0] = x0;
+                e[1] = x1;
+                e[2] = x2;
+                e[3] = (double)code;
+            }
+            wave_sync_lds();
+            if (lane >= qn && lane < qn + cnt) {
+                const double *e = stage + 4 * (lane - qn);
+                qx0 = e[0];
+                qx1 = e[1];
+                qx2 = e[2];
+                qcode = (uint32_t)e[3];
+            }
+            wave_sync_lds();
+            qn += cnt;
+            pending = false;
+        }
+        const uint32_t keep = (pending || finished) ? 0 : nc - 1;
+        while (qn > keep) expand(min(qn, nc));
+        if (finished) break;
     }
     wave_sync_lds();
     double *Sf = a.S + (uint64_t)fl * a.FS;
@@ -515,7 +573,6 @@ void *workspace(anl_sph *h, size_t bytes) {
 
 // bins per density launch column: a wavefront's density block for them plus
 // the staging of 64/(lmax+1) harmonic vectors fits its share of LDS
-uint32_t stage_doubles(uint32_t lmax) { return 64 / (lmax + 1) * (lmax + 1) * (lmax + 1); }
 uint32_t bins_per_launch(uint32_t nt, uint32_t nbin, uint32_t lmax) {
     const uint32_t per_bin = nt * (lmax + 1) * (lmax + 1);
     const uint32_t room = SPH_WAVE_LDS - stage_doubles(lmax);
