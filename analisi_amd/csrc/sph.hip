@@ -30,6 +30,11 @@
 // The host divides R by (number of t1) x (atoms of type_i) and writes the
 // reference layout [dt][type_i][type_j][bin][k] with k holding l = lmax - k.
 // Every sum has a fixed order: results are deterministic.
+//
+// Steinhardt<lmax> (steinhardt.cpp:90-236) reuses step 1 on frames
+// primo, primo + skip, ..; sph_steinhardt_kernel then turns each (atom,
+// type_j, bin) density into q_l and either a histogram count (64-bit atomics,
+// exact) or the q_1..q_lmax trajectory.
 
 #include <algorithm>
 #include <cmath>
