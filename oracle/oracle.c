@@ -913,3 +913,168 @@ int orc_steinhardt(const orc_traj *tr, const double *rminmax, unsigned nbin, int
     free(dr);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* SANN neighbour lists (Neighbours, neighbour.h:10-104, neighbour.cpp)      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double r, x[3];
+    uint64_t j;
+} orc_neigh;
+
+static int cmp_neigh(const void *a, const void *b)
+{
+    const orc_neigh *p = a, *q = b;
+    if (p->r < q->r) return -1;
+    if (p->r > q->r) return 1;
+    return p->j < q->j ? -1 : (p->j > q->j);
+}
+
+/* neighbour.cpp:85-101 -- the SANN count of a list sorted by distance: 0 for
+ * fewer than 3 neighbours, else the smallest n >= 3 with
+ * r_0 + .. + r_{n-1} <= r_n (n - 2), or the whole list. */
+static unsigned sann_count(const orc_neigh *l, unsigned n)
+{
+    if (n < 3) return 0;
+    unsigned m = 3;
+    double s = 0.0;
+    for (unsigned i = 0; i < m; i++) s += l[i].r;
+    while (m < n) {
+        if (s <= l[m].r * (m - 2)) break;
+        s += l[m].r;
+        m++;
+    }
+    return m;
+}
+
+/* neighbour.cpp:8-82 and sphericalbase.cpp:67-93 -- for every atom i and type
+ * t, the atoms j != i of type t with d2(i, j) <= cut2[t] (x = x_i - x_j, the
+ * minimum image of frame ts), sorted by distance (ties by index), at most
+ * nneigh[t] of them (-1 beyond: "Too many neighbours in shell!"); the
+ * harmonics of the first SANN-count ones are summed into bin 0 of
+ * res[i][t][nbin][(lmax+1)^2], and the count goes to cnt[i][t] (may be NULL).
+ * The reference checks the capacity of both atoms' lists for every pair within
+ * either cutoff; this restatement checks each list against its own count. */
+int orc_sph_density_sann(const orc_traj *tr, uint64_t ts, const unsigned *nneigh, const double *cut2,
+                         unsigned nbin, int lmax, double *res, unsigned *cnt)
+{
+    const uint64_t N = tr->natoms, nt = tr->ntypes;
+    const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
+    double y[(ORC_SH_LMAX + 1) * (ORC_SH_LMAX + 1)];
+    unsigned cap = 0;
+    for (uint64_t t = 0; t < nt; t++) cap = nneigh[t] > cap ? nneigh[t] : cap;
+    orc_neigh *l = malloc((cap + 1) * sizeof(orc_neigh));
+    memset(res, 0, N * nt * nbin * L2 * sizeof(double));
+    const double *box = tr->box + ts * tr->box_stride;
+    int rc = 0;
+    for (uint64_t i = 0; i < N && rc == 0; i++)
+        for (uint64_t t = 0; t < nt && rc == 0; t++) {
+            unsigned n = 0;
+            for (uint64_t j = 0; j < N; j++) {
+                if (j == i || (uint64_t)tr->type_id[j] != t) continue;
+                double x[3];
+                const double d2 = orc_d2_min_image(tr->pos + (ts * N + i) * 3, tr->pos + (ts * N + j) * 3, box,
+                                                   tr->triclinic, x);
+                if (!(d2 <= cut2[t])) continue;
+                if (n >= nneigh[t]) { rc = -1; break; }
+                l[n].r = sqrt(d2);
+                l[n].x[0] = x[0];
+                l[n].x[1] = x[1];
+                l[n].x[2] = x[2];
+                l[n].j = j;
+                n++;
+            }
+            if (rc) break;
+            qsort(l, n, sizeof(orc_neigh), cmp_neigh);
+            const unsigned m = sann_count(l, n);
+            double *dst = res + L2 * (nbin * (nt * i + t));
+            for (unsigned k = 0; k < m; k++) {
+                orc_sph_ylm(l[k].x[0], l[k].x[1], l[k].x[2], lmax, y);
+                for (size_t e = 0; e < L2; e++) dst[e] += y[e];
+            }
+            if (cnt) cnt[nt * i + t] = m;
+        }
+    free(l);
+    return rc;
+}
+
+/* orc_sph_correlations with the SANN densities (sphericalcorrelations.cpp
+ * with a neighbour list, :115-117): returns -1 too short, -2 too many
+ * neighbours. */
+int orc_sph_correlations_sann(const orc_traj *tr, const unsigned *nneigh, const double *cut2, unsigned nbin,
+                              int lmax, unsigned tmax, unsigned skip, unsigned n, uint64_t primo, double *out)
+{
+    const uint64_t N = tr->natoms, nt = tr->ntypes;
+    const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
+    const unsigned leff = (n < tmax || tmax == 0) ? n : tmax;
+    if ((uint64_t)leff + n + primo > tr->nframes) return -1;
+    const uint64_t nfr = (uint64_t)n + leff - 1, snap = N * nt * nbin * L2;
+    double *dens = malloc(nfr * snap * sizeof(double));
+    for (uint64_t f = 0; f < nfr; f++)
+        if (orc_sph_density_sann(tr, primo + f, nneigh, cut2, nbin, lmax, dens + f * snap, NULL)) {
+            free(dens);
+            return -2;
+        }
+    orc_sph_corr_from_densities(tr, dens, nbin, lmax, leff, skip, n, out);
+    free(dens);
+    return 0;
+}
+
+/* orc_steinhardt with the SANN densities and counts (steinhardt.cpp with a
+ * neighbour list, not averaged; nbin is 1 there): returns -1 too short, -2
+ * too many neighbours. */
+int orc_steinhardt_sann(const orc_traj *tr, const unsigned *nneigh, const double *cut2, int lmax, unsigned nbin_st,
+                        const unsigned *l_list, unsigned nl, unsigned skip, unsigned n, uint64_t primo, int do_hist,
+                        double *out)
+{
+    const uint64_t N = tr->natoms, nt = tr->ntypes;
+    const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
+    if ((uint64_t)n + primo > tr->nframes) return -1;
+    if (skip == 0) skip = 1;
+    const double incr = (n / skip > 0) ? 1.0 / (int)(n / skip) : 1.0;
+    size_t hs = 1;
+    for (unsigned k = 0; k < nl; k++) hs *= nbin_st;
+    const uint64_t nfs = n / skip;
+    const size_t len = do_hist ? nt * nt * hs : nfs * N * nt * (size_t)lmax;
+    memset(out, 0, len * sizeof(double));
+    double *dens = malloc(N * nt * L2 * sizeof(double));
+    unsigned *cnt = malloc(N * nt * sizeof(unsigned));
+    int rc = 0;
+    for (unsigned i = 0; i < n - n % skip && rc == 0; i += skip) {
+        if (orc_sph_density_sann(tr, primo + i, nneigh, cut2, 1, lmax, dens, cnt)) { rc = -2; break; }
+        for (uint64_t a = 0; a < N; a++) {
+            const unsigned ti = (unsigned)tr->type_id[a];
+            for (uint64_t tj = 0; tj < nt; tj++) {
+                const double *s = dens + L2 * (nt * a + tj);
+                const double na = (double)cnt[nt * a + tj];
+                double q[ORC_SH_LMAX + 1];
+                size_t off = 0;
+                for (int l = lmax; l >= 0; l--) {
+                    double v = s[off + l] * s[off + l];
+                    for (int m = 0; m < l; m++) v += s[off + m] * s[off + m];
+                    for (int m = 0; m < l; m++) v += s[off + l + 1 + m] * s[off + l + 1 + m];
+                    q[l] = na > 0 ? sqrt(v / na / na * 4 * ORC_PI / (2 * l + 1)) : 0.0;
+                    off += 2 * (size_t)l + 1;
+                }
+                if (do_hist) {
+                    if (!(na > 0)) continue;
+                    size_t hidx = 0, stride = hs;
+                    for (unsigned k = 0; k < nl; k++) {
+                        stride /= nbin_st;
+                        size_t jh = (size_t)floorf((float)(q[l_list[k]] * nbin_st));
+                        if (jh >= nbin_st) jh = nbin_st - 1;
+                        hidx += jh * stride;
+                    }
+                    out[(ti * nt + tj) * hs + hidx] += incr;
+                } else {
+                    for (int l = 1; l <= lmax; l++)
+                        out[(((uint64_t)(i / skip) * N + a) * nt + tj) * (size_t)lmax + (size_t)(l - 1)] = q[l];
+                }
+            }
+        }
+    }
+    free(dens);
+    free(cnt);
+    return rc;
+}

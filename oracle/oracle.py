@@ -84,6 +84,15 @@ def lib():
         L.orc_steinhardt.restype = ctypes.c_int
         L.orc_steinhardt.argtypes = [ctypes.POINTER(OrcTraj), _dp, ctypes.c_uint, ctypes.c_int, ctypes.c_uint, _up,
                                      ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, ctypes.c_int, _dp]
+        L.orc_sph_density_sann.restype = ctypes.c_int
+        L.orc_sph_density_sann.argtypes = [ctypes.POINTER(OrcTraj), _u64, _up, _dp, ctypes.c_uint, ctypes.c_int, _dp,
+                                           _up]
+        L.orc_sph_correlations_sann.restype = ctypes.c_int
+        L.orc_sph_correlations_sann.argtypes = [ctypes.POINTER(OrcTraj), _up, _dp, ctypes.c_uint, ctypes.c_int,
+                                                ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, _dp]
+        L.orc_steinhardt_sann.restype = ctypes.c_int
+        L.orc_steinhardt_sann.argtypes = [ctypes.POINTER(OrcTraj), _up, _dp, ctypes.c_int, ctypes.c_uint, _up,
+                                          ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -230,6 +239,55 @@ def steinhardt(tr, rminmax, nbin, lmax, nbin_st, l_list, skip, n, primo, do_hist
                               int(bool(do_hist)), _p(out))
     if rc != 0:
         raise RuntimeError("trajectory is too short for this kind of calculation")
+    return out
+
+
+def _neigh_spec(spec, nt):
+    """[(max neighbours, cutoff^2, skin^2)] per type -> (uint32 counts, float64 cutoff^2)."""
+    assert len(spec) == nt
+    return (np.ascontiguousarray([s[0] for s in spec], dtype=np.uint32),
+            np.ascontiguousarray([s[1] for s in spec], dtype=np.float64))
+
+
+_TOO_MANY = "Too many neighbours in shell!"
+
+
+def sph_density_sann(tr, frame, spec, nbin, lmax=10):
+    """SphericalBase::calc with a SANN neighbour list: (density, counts[N][nt])."""
+    nn, c2 = _neigh_spec(spec, tr.ntypes)
+    out = np.empty((tr.natoms, tr.ntypes, nbin, (lmax + 1) ** 2), dtype=np.float64)
+    cnt = np.zeros((tr.natoms, tr.ntypes), dtype=np.uint32)
+    if lib().orc_sph_density_sann(tr.ref, frame, _p(nn, _up), _p(c2), nbin, lmax, _p(out), _p(cnt, _up)):
+        raise RuntimeError(_TOO_MANY)
+    return out, cnt
+
+
+def sph_correlations_sann(tr, spec, nbin, tmax, skip, n, primo, lmax=10):
+    nn, c2 = _neigh_spec(spec, tr.ntypes)
+    leff = n if (n < tmax or tmax == 0) else tmax
+    nt = tr.ntypes
+    out = np.zeros((leff, nt, nt, nbin, lmax + 1), dtype=np.float64)
+    rc = lib().orc_sph_correlations_sann(tr.ref, _p(nn, _up), _p(c2), nbin, lmax, tmax, skip, n, primo, _p(out))
+    if rc == -1:
+        raise RuntimeError("trajectory is too short for this kind of calculation")
+    if rc == -2:
+        raise RuntimeError(_TOO_MANY)
+    return out
+
+
+def steinhardt_sann(tr, spec, lmax, nbin_st, l_list, skip, n, primo, do_hist):
+    """Steinhardt with a SANN neighbour list (one radial bin)."""
+    nn, c2 = _neigh_spec(spec, tr.ntypes)
+    ll = np.ascontiguousarray(l_list, dtype=np.uint32)
+    nt, skip = tr.ntypes, max(skip, 1)
+    shape = (1, nt, nt) + (nbin_st,) * len(ll) if do_hist else (1, n // skip, tr.natoms, nt, lmax)
+    out = np.zeros(shape, dtype=np.float64)
+    rc = lib().orc_steinhardt_sann(tr.ref, _p(nn, _up), _p(c2), lmax, nbin_st, _p(ll, _up), len(ll), skip, n,
+                                   primo, int(bool(do_hist)), _p(out))
+    if rc == -1:
+        raise RuntimeError("trajectory is too short for this kind of calculation")
+    if rc == -2:
+        raise RuntimeError(_TOO_MANY)
     return out
 
 

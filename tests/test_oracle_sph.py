@@ -239,3 +239,88 @@ def test_steinhardt_histogram_layout_and_single_neighbour():
     np.testing.assert_allclose(h, counts * incr, rtol=1e-13, atol=0)
     with pytest.raises(RuntimeError, match="too short"):
         O.steinhardt(tr, rm, 2, 8, nb, [4, 6], skip, 10, 1, True)
+
+
+# ---------------------------------------------------------------------------
+# SANN neighbour lists (neighbour.cpp:8-101)
+# ---------------------------------------------------------------------------
+
+def _sann_bruteforce(tr, L, frame, i, spec):
+    """{type: neighbour vectors of atom i chosen by SANN}: the atoms of each
+    type within its cutoff, by distance; m >= 3 grows while the mean of the
+    first m distances times m/(m-2) exceeds the next distance (van Meel, Filion,
+    Valeriani and Frenkel, J. Chem. Phys. 136, 234107 (2012))."""
+    out = {}
+    for t in range(tr.ntypes):
+        v = []
+        for j in range(tr.natoms):
+            if j == i or tr.type_id[j] != t:
+                continue
+            x = tr.pos[frame, i] - tr.pos[frame, j]
+            x = x - L * np.round(x / L)
+            if x @ x <= spec[t][1]:
+                v.append(x)
+        v.sort(key=lambda a: np.sqrt(a @ a))
+        r = [np.sqrt(a @ a) for a in v]
+        m = 0
+        if len(r) >= 3:
+            m = 3
+            while m < len(r) and sum(r[:m]) / (m - 2) > r[m]:
+                m += 1
+        out[t] = v[:m]
+    return out
+
+
+def test_sann_density_bruteforce():
+    tr, L = _small_traj(23, 70, 3, 8.0, [0.6, 0.4])
+    spec = [(40, 3.2 ** 2, 0.0), (30, 3.6 ** 2, 0.0)]
+    for frame in (0, 2):
+        dens, cnt = O.sph_density_sann(tr, frame, spec, 2)
+        assert np.all(dens[:, :, 1] == 0)            # everything goes to bin 0
+        for i in range(0, 70, 7):
+            sel = _sann_bruteforce(tr, L, frame, i, spec)
+            for t in range(2):
+                assert cnt[i, t] == len(sel[t])
+                want = sum((_real_ylm_scipy(x, LMAX) for x in sel[t]), np.zeros((LMAX + 1) ** 2))
+                np.testing.assert_allclose(dens[i, t, 0], want, rtol=0, atol=1e-11)
+        assert cnt.max() >= 3
+    with pytest.raises(RuntimeError, match="Too many neighbours"):
+        O.sph_density_sann(tr, 0, [(4, 3.2 ** 2, 0.0), (30, 3.6 ** 2, 0.0)], 1)
+
+
+def test_sann_fcc_and_correlations():
+    """In a jittered fcc crystal SANN keeps the 12 nearest neighbours (q4, q6
+    as tabulated); the SANN correlations follow the addition theorem over the
+    selected neighbours."""
+    pos, L = _crystal("fcc", 2.0, 3)
+    N = pos.shape[1]
+    pos = pos + 1e-5 * synth.normal(3, 5, pos.size).reshape(pos.shape)
+    tr = O.Traj(np.repeat(pos, 2, axis=0), np.zeros(N, dtype=np.int32), np.tile(synth.lammps_box(L), (2, 1)), False)
+    spec = [(40, 2.3 ** 2, 0.0)]          # 12 at 1.41 A and 6 at 2.0 A: SANN keeps the 12
+    q = O.steinhardt_sann(tr, spec, 6, 10, [4, 6], 1, 2, 0, False)
+    _, cnt = O.sph_density_sann(tr, 0, spec, 1, 6)
+    assert np.all(cnt == 12)
+    assert np.all(np.abs(q[0, :, :, 0, 3] - 0.191) < 1e-3) and np.all(np.abs(q[0, :, :, 0, 5] - 0.575) < 1e-3)
+    tr2, L2 = _small_traj(29, 40, 10, 7.5, [0.5, 0.5])
+    spec2 = [(40, 3.0 ** 2, 0.0), (40, 3.3 ** 2, 0.0)]
+    got = O.sph_correlations_sann(tr2, spec2, 2, 3, 2, 5, 1)
+    ls = np.arange(LMAX + 1)
+    want = np.zeros_like(got)
+    t1s = list(range(0, 5, 2))
+    ntype = np.bincount(tr2.type_id, minlength=2)
+    sel = {(f, i): _sann_bruteforce(tr2, L2, f, i, spec2) for f in range(1, 8) for i in range(40)}
+    for dt in range(3):
+        for t1 in t1s:
+            for i in range(40):
+                for t in range(2):
+                    a, b = sel[1 + t1, i][t], sel[1 + t1 + dt, i][t]
+                    if not a or not b:
+                        continue
+                    u = np.array(a) / np.linalg.norm(a, axis=1)[:, None]
+                    w = np.array(b) / np.linalg.norm(b, axis=1)[:, None]
+                    c = np.clip(u @ w.T, -1, 1)
+                    s = [(2 * l + 1) / (4 * np.pi) * special.eval_legendre(l, c).sum() for l in ls]
+                    want[dt, tr2.type_id[i], t, 0, LMAX - ls] += np.array(s) / ntype[tr2.type_id[i]]
+        want[dt] /= len(t1s)
+    assert np.abs(want).max() > 0.1
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10 * np.abs(want).max())
