@@ -75,6 +75,7 @@ SIGNATURES = [
     ("anl_sph_calculate", _i, [_vp, _u64, _u32, _u32, _u32, _vp]),
     ("anl_sph_snapshot", _i, [_vp, _u64, _vp]),
     ("anl_sph_set_budget", _i, [_vp, _u64]),
+    ("anl_sph_set_neighbours", _i, [_vp, _vp, _vp]),
     ("anl_steinhardt_calculate", _i, [_vp, _u64, _u32, _u32, _vp, _u32, _u32, _i, _vp]),
 ]
 
@@ -292,6 +293,12 @@ class DeviceSph:
         h = ctypes.c_void_p()
         check(lib().anl_sph_create(traj.handle, self.lmax, _ptr(rm), self.nbin, ctypes.byref(h)))
         self.handle = h
+
+    def set_neighbours(self, spec):
+        """SANN neighbour lists: spec = [(max neighbours, cutoff^2, skin^2)] per type."""
+        nn = np.ascontiguousarray([x[0] for x in spec], dtype=np.uint32)
+        c2 = np.ascontiguousarray([x[1] for x in spec], dtype=np.float64)
+        check(lib().anl_sph_set_neighbours(self.handle, _ptr(nn), _ptr(c2)))
 
     def set_budget(self, nbytes):
         check(lib().anl_sph_set_budget(self.handle, int(nbytes)))

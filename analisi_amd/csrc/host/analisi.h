@@ -651,7 +651,8 @@ template <class T>
 class SphericalHandle {
    public:
     using rminmax_t = std::vector<std::pair<double, double>>;
-    SphericalHandle(T *t, const rminmax_t &rminmax, size_t nbin, unsigned int lmax)
+    using NeighListSpec = std::vector<std::tuple<size_t, double, double>>;  // neighbour.h:13
+    SphericalHandle(T *t, const rminmax_t &rminmax, size_t nbin, unsigned int lmax, const NeighListSpec &nls = {})
         : traiettoria(t), nbin(nbin), lmax(lmax) {
         const size_t ntypes = t->get_ntypes();
         if (ntypes * ntypes != rminmax.size()) {
@@ -665,6 +666,10 @@ class SphericalHandle {
             rm.push_back(p.first);
             rm.push_back(p.second);
         }
+        for (const auto &x : nls) {
+            nneigh.push_back((uint32_t)std::get<0>(x));
+            cut2.push_back(std::get<1>(x));
+        }
     }
     ~SphericalHandle() {
         if (hs) anl_sph_destroy(hs);
@@ -676,7 +681,12 @@ class SphericalHandle {
             anl_traj *d = traiettoria->device();
             if (hs) anl_sph_destroy(hs);
             hs = nullptr;
+            // neighbour.h:31-45: checked when the list is built, after the
+            // calculation's own argument checks
+            if (!nneigh.empty() && nneigh.size() != traiettoria->get_ntypes())
+                throw std::runtime_error("In neighbours list you must specify parameters for each atomic type");
             anl_check(anl_sph_create(d, lmax, rm.data(), (uint32_t)nbin, &hs));
+            if (!nneigh.empty()) anl_check(anl_sph_set_neighbours(hs, nneigh.data(), cut2.data()));
             gen = traiettoria->generation();
         }
         return hs;
@@ -684,7 +694,8 @@ class SphericalHandle {
 
    private:
     T *traiettoria;
-    std::vector<double> rm;
+    std::vector<double> rm, cut2;
+    std::vector<uint32_t> nneigh;
     size_t nbin;
     unsigned int lmax;
     anl_sph *hs = nullptr;
@@ -696,8 +707,9 @@ class SphericalHandle {
 // over SphericalBase (sphericalbase.h:6-60): the time correlation of the
 // per-atom real spherical-harmonic densities of each neighbour type and
 // distance bin.  vdata [leff][ntypes][ntypes][nbin][lmax+1], index k of the
-// last axis holding l = lmax - k.  Neighbour lists (the SANN variant) are not
-// supported: a non-empty neighList is an error.
+// last axis holding l = lmax - k.  With a neighbour list (the SANN variant)
+// the radial ranges are ignored and every atom's SANN neighbours of each type
+// fill bin 0.
 // ---------------------------------------------------------------------------
 template <class T>
 class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
@@ -711,11 +723,9 @@ class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
     SphericalCorrelations(T *t, const rminmax_t rminmax, size_t nbin, size_t tmax = 0, size_t nthreads = 0,
                           size_t skip = 1, size_t buffer_size = 10, bool debug = false,
                           const NeighListSpec neighList = {}, unsigned int lmax = ANL_SPH_LMAX_MAX)
-        : traiettoria(t), handle(t, rminmax, nbin, lmax), nbin(nbin), tmax(tmax), nthreads(nthreads),
+        : traiettoria(t), handle(t, rminmax, nbin, lmax, neighList), nbin(nbin), tmax(tmax), nthreads(nthreads),
           skip(skip == 0 ? 1 : skip), buffer_size(buffer_size), lmax(lmax), ntypes(t->get_ntypes()) {
         (void)debug;
-        if (!neighList.empty())
-            throw std::runtime_error("neighbour lists (the SANN variant of SphericalCorrelations) are not supported");
         if (t->get_nloaded_timesteps() > 0) handle.get();  // argument errors surface here
     }
     SphericalCorrelations &operator=(const SphericalCorrelations &destra) {
@@ -790,8 +800,9 @@ class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
 // ---------------------------------------------------------------------------
 // Steinhardt<l> (steinhardt.h:17-120, steinhardt.cpp:7-236): per frame, the
 // order parameters q_l of every (atom, neighbour type, bin), as a histogram
-// over the listed l values or as a trajectory of q_1..q_lmax.  Neighbour lists
-// (SANN) and the averaged order parameter are not supported.
+// over the listed l values or as a trajectory of q_1..q_lmax.  With a neighbour
+// list (SANN, one radial bin) the radial ranges are ignored; the averaged
+// order parameter is not supported.
 // ---------------------------------------------------------------------------
 template <class T>
 class Steinhardt : public VectorOp<Steinhardt<T>> {
@@ -806,7 +817,7 @@ class Steinhardt : public VectorOp<Steinhardt<T>> {
                std::vector<unsigned int> steinhardt_l_histogram, unsigned int nthreads = 2, unsigned int skip = 1,
                bool do_histogram = true, const NeighListSpec nls = {}, const bool averaged_order = false,
                unsigned int lmax = ANL_SPH_LMAX_MAX)
-        : traiettoria(t), handle(t, rminmax, nbin, lmax), l_hist(std::move(steinhardt_l_histogram)), nbin(nbin),
+        : traiettoria(t), handle(t, rminmax, nbin, lmax, nls), l_hist(std::move(steinhardt_l_histogram)), nbin(nbin),
           nbin_st(steinhardt_histogram_size), skip(skip == 0 ? 1 : skip), lmax(lmax), ntypes(t->get_ntypes()),
           do_histogram(do_histogram) {
         (void)nthreads;
@@ -815,8 +826,8 @@ class Steinhardt : public VectorOp<Steinhardt<T>> {
                 "You must use a neighbour list when computing a averaged steinhardt order parameter! ");
         if (!nls.empty() && nbin > 1)
             throw std::runtime_error("When using a neighbour list to compute steinhardt please set nbin to 1. ");
-        if (!nls.empty())
-            throw std::runtime_error("neighbour lists (the SANN variant of Steinhardt) are not supported");
+        if (averaged_order)
+            throw std::runtime_error("the averaged Steinhardt order parameter is not supported");
         hist_size = 1;
         for (size_t i = 0; i < l_hist.size(); ++i) hist_size *= nbin_st;
         if (t->get_nloaded_timesteps() > 0) handle.get();

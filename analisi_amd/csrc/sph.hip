@@ -175,6 +175,51 @@ __device__ __forceinline__ void put_harmonics(double *y, double x0, double x1, d
     }
 }
 
+// Expands the nb oldest queue entries (entry k in lane k: minimum-image vector
+// and type << 16 | bin - b0): lane (slot, m) puts entry slot's harmonics into
+// its staging block, the blocks are added into the density block acc in queue
+// order (each entry by one lane, in order), and the queue moves down by nb.
+__device__ __forceinline__ void expand_queue(uint32_t nb, double &qx0, double &qx1, double &qx2, uint32_t &qcode,
+                                             uint32_t &qn, double *acc, double *stage, uint32_t nbc, uint32_t L2,
+                                             uint32_t lane, uint32_t slot, uint32_t m, int lmax,
+                                             const double (&cf)[ANL_SPH_LMAX_MAX + 1]) {
+    const int src = slot < nb ? (int)slot : 0;
+    const double y0 = __shfl(qx0, src), y1 = __shfl(qx1, src), y2 = __shfl(qx2, src);
+    if (slot < nb) put_harmonics(stage + slot * L2, y0, y1, y2, (int)m, lmax, cf);
+    wave_sync_lds();
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint32_t code = (uint32_t)__shfl((int)qcode, (int)k);
+        double *blk = acc + ((code >> 16) * nbc + (code & 0xFFFFu)) * L2;
+        for (uint32_t e = lane; e < L2; e += 64) blk[e] += stage[k * L2 + e];
+    }
+    wave_sync_lds();
+    const int up = (int)min(lane + nb, 63u);
+    qx0 = __shfl(qx0, up);
+    qx1 = __shfl(qx1, up);
+    qx2 = __shfl(qx2, up);
+    qcode = (uint32_t)__shfl((int)qcode, up);
+    qn -= nb;
+}
+
+// Writes atom i's density block acc (bins b0 .. b0 + nbc - 1 of every type)
+// to S[frame fl][type_j][bin][l][atom by type][m].
+__device__ __forceinline__ void store_density(const SphDensArgs &a, const double *acc, uint32_t fl, uint64_t i,
+                                              uint32_t b0, uint32_t nbc, uint32_t lane) {
+    const uint32_t L2 = (a.lmax + 1) * (a.lmax + 1);
+    double *Sf = a.S + (uint64_t)fl * a.FS;
+    const uint64_t ap = (uint64_t)a.aprime[i];
+    for (uint32_t k = lane; k < a.nt * nbc * L2; k += 64) {
+        const uint32_t blk = k / L2, e = k - blk * L2;
+        const uint32_t jt = blk / nbc, bl = blk - jt * nbc;
+        if (b0 + bl >= a.nbin) continue;
+        uint32_t l = (uint32_t)sqrtf((float)e);
+        while (l * l > e) --l;
+        while ((l + 1) * (l + 1) <= e) ++l;
+        const uint64_t blk_off = ((uint64_t)(jt * a.nbin + b0 + bl) * L2 + l * l) * a.N;
+        Sf[blk_off + ap * (2 * l + 1) + (e - l * l)] = acc[k];
+    }
+}
+
 template <bool TRI>
 __global__ __launch_bounds__(256, 3) void sph_density_kernel(SphDensArgs a) {
     extern __shared__ double sph_lds[];
@@ -208,27 +253,6 @@ __global__ __launch_bounds__(256, 3) void sph_density_kernel(SphDensArgs a) {
     // vector and type << 16 | bin - b0); qn entries, wavefront-uniform
     double qx0 = 0.0, qx1 = 0.0, qx2 = 0.0;
     uint32_t qcode = 0, qn = 0;
-    // expand the nb oldest entries: lane (slot, m) takes entry slot into its
-    // staging block; the blocks are then added into the density in queue
-    // order (each entry by one lane, in order); the queue moves down by nb
-    auto expand = [&](uint32_t nb) {
-        const int src = slot < nb ? (int)slot : 0;
-        const double y0 = __shfl(qx0, src), y1 = __shfl(qx1, src), y2 = __shfl(qx2, src);
-        if (slot < nb) put_harmonics(stage + slot * L2, y0, y1, y2, (int)m, (int)a.lmax, cf);
-        wave_sync_lds();
-        for (uint32_t k = 0; k < nb; ++k) {
-            const uint32_t code = (uint32_t)__shfl((int)qcode, (int)k);
-            double *blk = acc + ((code >> 16) * a.nbc + (code & 0xFFFFu)) * L2;
-            for (uint32_t e = lane; e < L2; e += 64) blk[e] += stage[k * L2 + e];
-        }
-        wave_sync_lds();
-        const int up = (int)min(lane + nb, 63u);
-        qx0 = __shfl(qx0, up);
-        qx1 = __shfl(qx1, up);
-        qx2 = __shfl(qx2, up);
-        qcode = (uint32_t)__shfl((int)qcode, up);
-        qn -= nb;
-    };
     const uint64_t nblk = (a.N + 63) / 64;
     // Blocks of 64 slots, 64 blocks at a time: lane b decides whether block b
     // can hold an atom within rcut of atom i.  In orthorhombic cells the
@@ -311,22 +335,144 @@ __global__ __launch_bounds__(256, 3) void sph_density_kernel(SphDensArgs a) {
             pending = false;
         }
         const uint32_t keep = (pending || finished) ? 0 : nc - 1;
-        while (qn > keep) expand(min(qn, nc));
+        while (qn > keep)
+            expand_queue(min(qn, nc), qx0, qx1, qx2, qcode, qn, acc, stage, a.nbc, L2, lane, slot, m, (int)a.lmax, cf);
         if (finished) break;
     }
     wave_sync_lds();
+    store_density(a, acc, fl, i, b0, a.nbc, lane);
+}
+
+constexpr uint32_t SANN_MAXT = 16;      // atom types of a neighbour list
+constexpr uint32_t SANN_MAXTOT = 1024;  // sum over types of the list capacities
+
+struct SannArgs {
+    uint32_t nn[SANN_MAXT];     // capacity of the list of each neighbour type
+    uint32_t off[SANN_MAXT];    // its first entry
+    double cut2[SANN_MAXT];     // squared cutoff of each neighbour type
+    uint32_t total, capmax;     // sum and maximum of nn (total rounded up to even)
+    int *overflow;              // set when a list would exceed its capacity
+};
+
+// SphericalBase::calc with a neighbour list (neighbour.cpp:8-101,
+// sphericalbase.cpp:67-93): one wavefront per (frame, atom i).  For every
+// type t the atoms j != i of type t with d2 <= cut2[t] are listed in LDS
+// (distance, slot), sorted by (distance, slot) with a rank sort, and the
+// first SANN-count of them (3, then more while the sum of the first m
+// distances exceeds r_m (m - 2)) are expanded into bin 0; the other bins are
+// zero.  The neighbour count is the l = 0 entry over Y_00.
+template <bool TRI>
+__global__ __launch_bounds__(64) void sph_density_sann_kernel(SphDensArgs a, SannArgs s) {
+    extern __shared__ double sann_lds[];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t L1 = a.lmax + 1, L2 = L1 * L1;
+    const uint32_t nc = 64 / L1;
+    const uint32_t slot = lane / L1, m = lane - slot * L1;
+    double *acc = sann_lds;                           // [nt][L2], bin 0
+    double *stage = acc + a.nt * L2;
+    double *ru = stage + stage_doubles(a.lmax);       // [total] distances, in list order
+    double *rs = ru + s.total;                        // [capmax] sorted distances of one type
+    int *ju = (int *)(rs + s.capmax);                 // [total] slots, in list order
+    int *js = ju + s.total;                           // [capmax] sorted slots of one type
+    int *cnt = js + s.capmax;                         // [nt] list lengths (may exceed capacity)
+    double cf[ANL_SPH_LMAX_MAX + 1];
+#pragma unroll
+    for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l)
+        cf[l] = (l >= (int)m && l <= (int)a.lmax) ? a.coef[l * L1 + m] : 0.0;
+    const uint32_t fl = (uint32_t)(blockIdx.x / a.N);
+    const uint64_t i = blockIdx.x - (uint64_t)fl * a.N;
+    for (uint32_t k = lane; k < a.nt * L2; k += 64) acc[k] = 0.0;
+    if (lane < a.nt) cnt[lane] = 0;
+    wave_sync_lds();
+    const uint64_t f = a.f0 + fl * a.fstride;
+    const double4 *P = a.pos + f * a.N;
+    const DevCell c = load_cell(a.box + f * 9);
+    const double4 pi = P[i];
+    for (uint64_t jb = 0; jb < a.N; jb += 64) {
+        const uint64_t j = jb + lane;
+        bool ok = false;
+        uint32_t tj = 0;
+        double d2 = 0.0;
+        if (j < a.N && j != i) {
+            const double4 pj = P[j];
+            double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+            min_image_ref<TRI>(x0, x1, x2, c);
+            d2 = d2_of(x0, x1, x2);
+            tj = (uint32_t)a.type_slot[j];
+            ok = d2 <= s.cut2[tj];
+        }
+        for (uint32_t t = 0; t < a.nt; ++t) {
+            const unsigned long long mask = __ballot(ok && tj == t);
+            if (!mask) continue;
+            const uint32_t base = (uint32_t)cnt[t], add = (uint32_t)__popcll(mask);
+            if (ok && tj == t) {
+                const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+                if (pos < s.nn[t]) {
+                    ru[s.off[t] + pos] = sqrt(d2);
+                    ju[s.off[t] + pos] = (int)j;
+                }
+            }
+            wave_sync_lds();
+            if (lane == 0) {
+                cnt[t] = (int)(base + add);
+                if (base + add > s.nn[t]) atomicOr(s.overflow, 1);
+            }
+            wave_sync_lds();
+        }
+    }
+    double qx0 = 0.0, qx1 = 0.0, qx2 = 0.0;
+    uint32_t qcode = 0, qn = 0;
+    for (uint32_t t = 0; t < a.nt; ++t) {
+        const uint32_t n = min((uint32_t)cnt[t], s.nn[t]);
+        const double *r = ru + s.off[t];
+        const int *jj = ju + s.off[t];
+        for (uint32_t k = lane; k < n; k += 64) {
+            const double rk = r[k];
+            const int jk = jj[k];
+            uint32_t rank = 0;
+            for (uint32_t q = 0; q < n; ++q) rank += (r[q] < rk || (r[q] == rk && jj[q] < jk)) ? 1u : 0u;
+            rs[rank] = rk;
+            js[rank] = jk;
+        }
+        wave_sync_lds();
+        uint32_t ms = 0;  // neighbour.cpp:85-101
+        if (n >= 3) {
+            ms = 3;
+            double sum = rs[0] + rs[1] + rs[2];
+            while (ms < n) {
+                if (sum <= rs[ms] * (double)(ms - 2)) break;
+                sum += rs[ms];
+                ms++;
+            }
+        }
+        for (uint32_t c0 = 0; c0 < ms; c0 += 64) {
+            qn = min(64u, ms - c0);
+            if (lane < qn) {
+                const double4 pj = P[js[c0 + lane]];
+                qx0 = pi.x - pj.x;
+                qx1 = pi.y - pj.y;
+                qx2 = pi.z - pj.z;
+                min_image_ref<TRI>(qx0, qx1, qx2, c);
+                qcode = t << 16;
+            }
+            while (qn > 0)
+                expand_queue(min(qn, nc), qx0, qx1, qx2, qcode, qn, acc, stage, 1, L2, lane, slot, m, (int)a.lmax, cf);
+        }
+        wave_sync_lds();  // rs, js are reused by the next type
+    }
+    wave_sync_lds();
+    store_density(a, acc, fl, i, 0, 1, lane);
+    // bins 1.. hold nothing with a neighbour list
     double *Sf = a.S + (uint64_t)fl * a.FS;
     const uint64_t ap = (uint64_t)a.aprime[i];
-    for (uint32_t k = lane; k < accn; k += 64) {
-        const uint32_t blk = k / L2, e = k - blk * L2;
-        const uint32_t jt = blk / a.nbc, bl = blk - jt * a.nbc;
-        if (b0 + bl >= a.nbin) continue;
-        uint32_t l = (uint32_t)sqrtf((float)e);
-        while (l * l > e) --l;
-        while ((l + 1) * (l + 1) <= e) ++l;
-        const uint64_t blk_off = ((uint64_t)(jt * a.nbin + b0 + bl) * L2 + l * l) * a.N;
-        Sf[blk_off + ap * (2 * l + 1) + (e - l * l)] = acc[k];
-    }
+    for (uint32_t t = 0; t < a.nt; ++t)
+        for (uint32_t b = 1; b < a.nbin; ++b)
+            for (uint32_t e = lane; e < L2; e += 64) {
+                uint32_t l = (uint32_t)sqrtf((float)e);
+                while (l * l > e) --l;
+                while ((l + 1) * (l + 1) <= e) ++l;
+                Sf[((uint64_t)(t * a.nbin + b) * L2 + l * l) * a.N + ap * (2 * l + 1) + (e - l * l)] = 0.0;
+            }
 }
 
 struct SphGramArgs {
@@ -552,6 +698,9 @@ struct anl_sph {
     double d2lo = 0.0, d2hi = 0.0;    // min over type pairs of T[0], max of T[nbin]
     double rcut = 0.0;                // distance bound of the block screen
     bool block_screen = false;        // orthorhombic cell and finite rcut
+    bool sann = false;                // neighbour lists (SANN) instead of the radial ranges
+    SannArgs sann_args{};
+    int *d_overflow = nullptr;        // set by the list kernel when a list is full
     void *work = nullptr;             // densities, G, R (grow-only)
     size_t work_cap = 0;
 };
@@ -596,6 +745,38 @@ void launch_density(anl_sph *h, uint64_t f0, uint64_t fstride, uint32_t nfr, dou
                     hipStream_t s) {
     SphDensArgs a{};
     a.fstride = fstride;
+    if (h->sann) {
+        a.pos = h->tr->pos;
+        a.box = h->tr->box;
+        a.type_slot = h->tr->type_slot;
+        a.aprime = h->d_aprime;
+        a.coef = h->d_coef;
+        a.S = S;
+        a.N = h->tr->N;
+        a.FS = frame_doubles(h);
+        a.f0 = f0;
+        a.nt = h->nt;
+        a.nbin = h->nbin;
+        a.lmax = h->lmax;
+        a.nbc = 1;
+        const uint64_t gx = (uint64_t)nfr * a.N;
+        if (gx > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom) pairs in one density launch");
+        const SannArgs &sa = h->sann_args;
+        const size_t lds = (size_t)(h->nt * (h->lmax + 1) * (h->lmax + 1) + stage_doubles(h->lmax) + sa.total +
+                                    sa.capmax) * sizeof(double) +
+                           (size_t)(sa.total + sa.capmax + SANN_MAXT) * sizeof(int);
+        ANL_HIP(hipMemsetAsync(h->d_overflow, 0, sizeof(int), s));
+        if (h->tr->triclinic)
+            hipLaunchKernelGGL(sph_density_sann_kernel<true>, dim3((uint32_t)gx), dim3(64), lds, s, a, sa);
+        else
+            hipLaunchKernelGGL(sph_density_sann_kernel<false>, dim3((uint32_t)gx), dim3(64), lds, s, a, sa);
+        ANL_HIP(hipGetLastError());
+        int over = 0;
+        ANL_HIP(hipMemcpyAsync(&over, h->d_overflow, sizeof(int), hipMemcpyDeviceToHost, s));
+        ANL_HIP(hipStreamSynchronize(s));
+        if (over) fail(ANL_ERR_ARG, "Too many neighbours in shell!");  // neighbour.cpp:44-48
+        return;
+    }
     a.d2lo = h->d2lo;
     a.d2hi = h->d2hi;
     a.rcut = h->rcut;
@@ -873,9 +1054,34 @@ int anl_sph_destroy(anl_sph *h) {
     if (h->d_tstart) (void)hipFree(h->d_tstart);
     if (h->d_type_ap) (void)hipFree(h->d_type_ap);
     if (h->d_orig_ap) (void)hipFree(h->d_orig_ap);
+    if (h->d_overflow) (void)hipFree(h->d_overflow);
     if (h->work) (void)hipFree(h->work);
     delete h;
     return ANL_OK;
+}
+
+int anl_sph_set_neighbours(anl_sph *h, const uint32_t *nneigh, const double *cutoff2) {
+    return guard([&] {
+        if (!h || !nneigh || !cutoff2) fail(ANL_ERR_ARG, "null pointer passed to anl_sph_set_neighbours");
+        if (h->nt > SANN_MAXT) fail(ANL_ERR_ARG, "neighbour lists support at most 16 atom types");
+        SannArgs sa{};
+        uint32_t tot = 0;
+        for (uint32_t t = 0; t < h->nt; t++) {
+            if (nneigh[t] == 0) fail(ANL_ERR_ARG, "every neighbour list needs a positive capacity");
+            if (!std::isfinite(cutoff2[t])) fail(ANL_ERR_ARG, "neighbour cutoffs must be finite");
+            sa.nn[t] = nneigh[t];
+            sa.off[t] = tot;
+            sa.cut2[t] = cutoff2[t];
+            sa.capmax = std::max(sa.capmax, nneigh[t]);
+            tot += nneigh[t];
+            if (tot > SANN_MAXTOT) fail(ANL_ERR_ARG, "the neighbour lists hold at most 1024 atoms in total");
+        }
+        sa.total = (tot + 1) / 2 * 2;
+        if (!h->d_overflow) ANL_HIP(hipMalloc(&h->d_overflow, sizeof(int)));
+        sa.overflow = h->d_overflow;
+        h->sann_args = sa;
+        h->sann = true;
+    });
 }
 
 int anl_sph_set_budget(anl_sph *h, uint64_t bytes) {

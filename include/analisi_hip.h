@@ -172,6 +172,13 @@ int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host);
  * ANL_ERR_TOO_SHORT when n + primo exceeds the frames (:80-83). */
 int anl_steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip, const uint32_t *l_list,
                              uint32_t nl, uint32_t nbin_st, int do_histogram, double *out_host);
+/* Switches the handle to neighbour lists (Neighbours, neighbour.h:10-104,
+ * neighbour.cpp:8-101, the SANN variant of sphericalbase.cpp:67-93): for
+ * every atom and type t, the atoms of type t within sqrt(cutoff2[t]), at most
+ * nneigh[t] of them (else "Too many neighbours in shell!"), sorted by
+ * distance; the harmonics of the first SANN-count ones go to bin 0 and the
+ * radial ranges are ignored.  Sum of nneigh at most 1024, at most 16 types. */
+int anl_sph_set_neighbours(anl_sph *h, const uint32_t *nneigh, const double *cutoff2);
 /* Bytes of per-frame densities held on the device at once (default: half the
  * free device memory at creation, at most 64 GiB);
  * a smaller budget processes the t0 values in more chunks, same results up

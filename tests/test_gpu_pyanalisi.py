@@ -324,7 +324,7 @@ def test_spherical_correlations_reference_test_call(pa):
         sh.calculate(6)                                   # 200 + 700 + 6 > 905
     with pytest.raises(RuntimeError, match="radial range for each pair"):
         pa.SphericalCorrelations(t, rm[:4], 4, 200, 4, 90, 10, True, [])
-    with pytest.raises(RuntimeError, match="not supported"):
+    with pytest.raises(RuntimeError, match="parameters for each atomic type"):
         pa.SphericalCorrelations(t, rm, 1, 200, 4, 90, 10, True, [(19, 16.0, 16.0)])
     few = pa.SphericalCorrelations(t, rm, 4, 3, 4, 90, 10, False, [])
     few.reset(10)
@@ -429,3 +429,35 @@ def test_lammps_window_read_ahead_changes_nothing(pa, tmp_path):
     g.calculate(35)
     h.calculate(35)
     assert np.array_equal(np.array(g), np.array(h))
+
+
+def test_steinhardt_neighbour_list_fixture_call(pa):
+    """SteinhardtFixtureNeigh of tests/src/test_sh.cpp:50-62 through the
+    Python class: SANN lists of at most 19 and 17 neighbours within 4 A for the
+    two types, one bin, l = (4, 6), skip 13, histogram off and on; equal to the
+    oracle.  SphericalCorrelations with the same lists equals the oracle too."""
+    F, N = 30, 90
+    box_l = synth.lammps_box(16.0)      # ~6 atoms within 4 A: the lists of 19 and 17 hold them
+    pos = synth.random_walk(73, N, F, box_l, 0.1)
+    types = synth.types_by_fraction(N, [0.7, 0.3]).astype(np.int32)
+    t = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False)
+    rm = [(0.5, 3.0)] * 4
+    nls = [(19, 16.0, 16.0), (17, 16.0, 16.0)]
+    ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), False)
+    st = pa.SteinhardtOrderParameterHistogram_6(t, rm, 1, 100, [4, 6], 2, 13, False, nls, False)
+    st.reset(F)
+    st.calculate(0)
+    q = np.array(st, copy=True)
+    assert q.shape == (1, F // 13, N, 2, 6)
+    assert np.abs(q - O.steinhardt_sann(ot, nls, 6, 100, [4, 6], 13, F, 0, False)).max() <= 1e-12
+    sh = pa.SteinhardtOrderParameterHistogram_6(t, rm, 1, 100, [4, 6], 2, 13, True, nls, False)
+    sh.reset(F)
+    sh.calculate(0)
+    np.testing.assert_allclose(np.array(sh), O.steinhardt_sann(ot, nls, 6, 100, [4, 6], 13, F, 0, True),
+                               rtol=1e-13, atol=0)
+    sc = pa.SphericalCorrelations(t, rm, 2, 6, 1, 3, 10, False, nls)
+    sc.reset(12)
+    sc.calculate(2)
+    _sh_close(np.array(sc), O.sph_correlations_sann(ot, nls, 2, 6, 3, 12, 2))
+    with pytest.raises(RuntimeError, match="averaged Steinhardt order parameter is not supported"):
+        pa.SteinhardtOrderParameterHistogram_6(t, rm, 1, 100, [4, 6], 2, 13, True, nls, True)

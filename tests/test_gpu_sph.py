@@ -176,3 +176,32 @@ def test_steinhardt_fcc_and_errors():
         s.steinhardt(3, 1, 1, [4, 6], 10)
     with pytest.raises(capi.AnlError, match="Wrong value of l"):
         s.steinhardt(3, 1, 0, [4, 8], 10)
+
+
+# ---------------------------------------------------------------------------
+# SANN neighbour lists (neighbour.cpp:8-101)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("tri", [False, True])
+def test_sann_densities_correlations_steinhardt(tri):
+    """Densities, correlations and Steinhardt q / histograms with SANN
+    neighbour lists equal the oracle's (1e-12; histograms exactly)."""
+    ot, dt = _case(79, 90, 12, 8.0, [0.6, 0.4], tri)
+    spec = [(40, 3.3 ** 2, 0.0), (30, 3.6 ** 2, 0.0)]
+    s = capi.DeviceSph(dt, _ranges(2, 3), 2, 10)
+    s.set_neighbours(spec)
+    for f in (0, 7):
+        want, cnt = O.sph_density_sann(ot, f, spec, 2)
+        assert cnt.max() >= 3
+        _close(s.snapshot(f), want)
+    _close(s.calculate(6, 4, 2, 1), O.sph_correlations_sann(ot, spec, 2, 4, 2, 6, 1))
+    s1 = capi.DeviceSph(dt, _ranges(2, 3), 1, 6)
+    s1.set_neighbours(spec)
+    q = s1.steinhardt(9, 2, 2, [4, 6], 20, do_histogram=False)
+    assert np.abs(q - O.steinhardt_sann(ot, spec, 6, 20, [4, 6], 2, 9, 2, False)).max() <= 1e-12
+    h = s1.steinhardt(9, 2, 2, [4, 6], 20)
+    np.testing.assert_allclose(h, O.steinhardt_sann(ot, spec, 6, 20, [4, 6], 2, 9, 2, True), rtol=1e-13, atol=0)
+    small = capi.DeviceSph(dt, _ranges(2, 3), 1, 6)
+    small.set_neighbours([(3, 3.3 ** 2, 0.0), (30, 3.6 ** 2, 0.0)])
+    with pytest.raises(capi.AnlError, match="Too many neighbours"):
+        small.snapshot(0)
