@@ -41,6 +41,13 @@
  *                          equals scipy's from its neighbour vectors; q4, q6
  *                          equal the tabulated values), q_l = 1 for a lone
  *                          neighbour, the histogram equal to the binned q_l.
+ *   orc_*_sann             pinned: a numpy brute force of the published SANN
+ *                          rule (van Meel et al., J. Chem. Phys. 136, 234107,
+ *                          2012) with scipy harmonics; fcc keeps its 12
+ *                          nearest neighbours; addition theorem over the
+ *                          selected neighbours.  List capacity: each list is
+ *                          checked against its own count (the reference checks
+ *                          both atoms' lists for a pair within either cutoff).
  */
 #include <math.h>
 #include <pthread.h>
