@@ -903,8 +903,10 @@ int orc_steinhardt(const orc_traj *tr, const double *rminmax, unsigned nbin, int
                         size_t hidx = 0, stride = hs;
                         for (unsigned k = 0; k < nl; k++) {
                             stride /= nbin_st;
-                            size_t jh = (size_t)floorf((float)(q[l_list[k]] * nbin_st));
-                            if (jh >= nbin_st) jh = nbin_st - 1;
+                            const float x = floorf((float)(q[l_list[k]] * nbin_st));
+                            /* beyond the last cell, and NaN, go to the last cell: what the
+                             * size_t conversion and clamp of steinhardt.cpp:199-202 do on x86-64 */
+                            const size_t jh = x < (float)nbin_st ? (size_t)x : nbin_st - 1;
                             hidx += jh * stride;
                         }
                         out[((b * nt + ti) * nt + tj) * hs + hidx] += incr;
@@ -963,8 +965,8 @@ static unsigned sann_count(const orc_neigh *l, unsigned n)
  * res[i][t][nbin][(lmax+1)^2], and the count goes to cnt[i][t] (may be NULL).
  * The reference checks the capacity of both atoms' lists for every pair within
  * either cutoff; this restatement checks each list against its own count. */
-int orc_sph_density_sann(const orc_traj *tr, uint64_t ts, const unsigned *nneigh, const double *cut2,
-                         unsigned nbin, int lmax, double *res, unsigned *cnt)
+static int sph_density_sann(const orc_traj *tr, uint64_t ts, const unsigned *nneigh, const double *cut2,
+                            unsigned nbin, int lmax, double *res, unsigned *cnt, uint64_t *nbr, unsigned capmax)
 {
     const uint64_t N = tr->natoms, nt = tr->ntypes;
     const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
@@ -1001,9 +1003,17 @@ int orc_sph_density_sann(const orc_traj *tr, uint64_t ts, const unsigned *nneigh
                 for (size_t e = 0; e < L2; e++) dst[e] += y[e];
             }
             if (cnt) cnt[nt * i + t] = m;
+            if (nbr)
+                for (unsigned k = 0; k < m; k++) nbr[(nt * i + t) * capmax + k] = l[k].j;
         }
     free(l);
     return rc;
+}
+
+int orc_sph_density_sann(const orc_traj *tr, uint64_t ts, const unsigned *nneigh, const double *cut2,
+                         unsigned nbin, int lmax, double *res, unsigned *cnt)
+{
+    return sph_density_sann(tr, ts, nneigh, cut2, nbin, lmax, res, cnt, NULL, 0);
 }
 
 /* orc_sph_correlations with the SANN densities (sphericalcorrelations.cpp
@@ -1029,11 +1039,13 @@ int orc_sph_correlations_sann(const orc_traj *tr, const unsigned *nneigh, const 
 }
 
 /* orc_steinhardt with the SANN densities and counts (steinhardt.cpp with a
- * neighbour list, not averaged; nbin is 1 there): returns -1 too short, -2
- * too many neighbours. */
+ * neighbour list; nbin is 1 there).  averaged (:128-173): per atom and type
+ * t, qbar_lm = S_lm(i)/n(i) if type(i) = t, plus S_lm(k)/n(k) of every SANN
+ * neighbour k of type t, in list order; then q_l from qbar with n(i) as
+ * before.  Returns -1 too short, -2 too many neighbours. */
 int orc_steinhardt_sann(const orc_traj *tr, const unsigned *nneigh, const double *cut2, int lmax, unsigned nbin_st,
                         const unsigned *l_list, unsigned nl, unsigned skip, unsigned n, uint64_t primo, int do_hist,
-                        double *out)
+                        int averaged, double *out)
 {
     const uint64_t N = tr->natoms, nt = tr->ntypes;
     const size_t L2 = (size_t)(lmax + 1) * (lmax + 1);
@@ -1046,14 +1058,29 @@ int orc_steinhardt_sann(const orc_traj *tr, const unsigned *nneigh, const double
     const size_t len = do_hist ? nt * nt * hs : nfs * N * nt * (size_t)lmax;
     memset(out, 0, len * sizeof(double));
     double *dens = malloc(N * nt * L2 * sizeof(double));
+    double *avg = malloc(L2 * sizeof(double));
     unsigned *cnt = malloc(N * nt * sizeof(unsigned));
+    unsigned capmax = 1;
+    for (uint64_t t = 0; t < nt; t++) capmax = nneigh[t] > capmax ? nneigh[t] : capmax;
+    uint64_t *nbr = malloc(N * nt * capmax * sizeof(uint64_t));
     int rc = 0;
     for (unsigned i = 0; i < n - n % skip && rc == 0; i += skip) {
-        if (orc_sph_density_sann(tr, primo + i, nneigh, cut2, 1, lmax, dens, cnt)) { rc = -2; break; }
+        if (sph_density_sann(tr, primo + i, nneigh, cut2, 1, lmax, dens, cnt, nbr, capmax)) { rc = -2; break; }
         for (uint64_t a = 0; a < N; a++) {
             const unsigned ti = (unsigned)tr->type_id[a];
             for (uint64_t tj = 0; tj < nt; tj++) {
                 const double *s = dens + L2 * (nt * a + tj);
+                if (averaged) {
+                    const unsigned na_ = cnt[nt * a + tj];
+                    for (size_t e = 0; e < L2; e++) avg[e] = ti == tj ? s[e] / na_ : 0.0;
+                    for (unsigned k = 0; k < na_; k++) {
+                        const uint64_t b = nbr[(nt * a + tj) * capmax + k];
+                        const double nb = (double)cnt[nt * b + tj];
+                        const double *sb = dens + L2 * (nt * b + tj);
+                        for (size_t e = 0; e < L2; e++) avg[e] += sb[e] / nb;
+                    }
+                    s = avg;
+                }
                 const double na = (double)cnt[nt * a + tj];
                 double q[ORC_SH_LMAX + 1];
                 size_t off = 0;
@@ -1069,8 +1096,8 @@ int orc_steinhardt_sann(const orc_traj *tr, const unsigned *nneigh, const double
                     size_t hidx = 0, stride = hs;
                     for (unsigned k = 0; k < nl; k++) {
                         stride /= nbin_st;
-                        size_t jh = (size_t)floorf((float)(q[l_list[k]] * nbin_st));
-                        if (jh >= nbin_st) jh = nbin_st - 1;
+                        const float x = floorf((float)(q[l_list[k]] * nbin_st));
+                        const size_t jh = x < (float)nbin_st ? (size_t)x : nbin_st - 1;  /* NaN: last cell */
                         hidx += jh * stride;
                     }
                     out[(ti * nt + tj) * hs + hidx] += incr;
@@ -1082,6 +1109,8 @@ int orc_steinhardt_sann(const orc_traj *tr, const unsigned *nneigh, const double
         }
     }
     free(dens);
+    free(avg);
     free(cnt);
+    free(nbr);
     return rc;
 }

@@ -92,7 +92,8 @@ def lib():
                                                 ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, _dp]
         L.orc_steinhardt_sann.restype = ctypes.c_int
         L.orc_steinhardt_sann.argtypes = [ctypes.POINTER(OrcTraj), _up, _dp, ctypes.c_int, ctypes.c_uint, _up,
-                                          ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, ctypes.c_int, _dp]
+                                          ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u64, ctypes.c_int,
+                                          ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -275,15 +276,15 @@ def sph_correlations_sann(tr, spec, nbin, tmax, skip, n, primo, lmax=10):
     return out
 
 
-def steinhardt_sann(tr, spec, lmax, nbin_st, l_list, skip, n, primo, do_hist):
-    """Steinhardt with a SANN neighbour list (one radial bin)."""
+def steinhardt_sann(tr, spec, lmax, nbin_st, l_list, skip, n, primo, do_hist, averaged=False):
+    """Steinhardt with a SANN neighbour list (one radial bin), optionally averaged."""
     nn, c2 = _neigh_spec(spec, tr.ntypes)
     ll = np.ascontiguousarray(l_list, dtype=np.uint32)
     nt, skip = tr.ntypes, max(skip, 1)
     shape = (1, nt, nt) + (nbin_st,) * len(ll) if do_hist else (1, n // skip, tr.natoms, nt, lmax)
     out = np.zeros(shape, dtype=np.float64)
     rc = lib().orc_steinhardt_sann(tr.ref, _p(nn, _up), _p(c2), lmax, nbin_st, _p(ll, _up), len(ll), skip, n,
-                                   primo, int(bool(do_hist)), _p(out))
+                                   primo, int(bool(do_hist)), int(bool(averaged)), _p(out))
     if rc == -1:
         raise RuntimeError("trajectory is too short for this kind of calculation")
     if rc == -2:

@@ -324,3 +324,74 @@ def test_sann_fcc_and_correlations():
         want[dt] /= len(t1s)
     assert np.abs(want).max() > 0.1
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-10 * np.abs(want).max())
+
+
+def test_sann_averaged_order():
+    """steinhardt.cpp:128-173: qbar_lm(i) = S_lm(i)/n(i) (same type) plus
+    S_lm(k)/n(k) over the SANN neighbours k, normalised by n(i) as q is.  In a
+    crystal every atom has the same densities, so qbar = (1 + 12)/12 q in fcc;
+    on a random system the oracle equals a numpy brute force with scipy
+    harmonics and an independent neighbour selection."""
+    pos, L = _crystal("fcc", 2.0, 3)
+    N = pos.shape[1]
+    pos = pos + 1e-5 * synth.normal(3, 5, pos.size).reshape(pos.shape)
+    tr = O.Traj(np.repeat(pos, 2, axis=0), np.zeros(N, dtype=np.int32), np.tile(synth.lammps_box(L), (2, 1)), False)
+    spec = [(40, 2.3 ** 2, 0.0)]
+    q = O.steinhardt_sann(tr, spec, 6, 10, [4, 6], 1, 2, 0, False)
+    qa = O.steinhardt_sann(tr, spec, 6, 10, [4, 6], 1, 2, 0, False, averaged=True)
+    np.testing.assert_allclose(qa[..., 3:6:2], 13.0 / 12.0 * q[..., 3:6:2], rtol=1e-4)
+
+    tr2, L2 = _small_traj(31, 50, 3, 7.0, [0.6, 0.4])
+    spec2 = [(40, 3.2 ** 2, 0.0), (30, 3.6 ** 2, 0.0)]
+    got = O.steinhardt_sann(tr2, spec2, 6, 10, [4], 1, 1, 1, False, averaged=True)[0, 0]   # frame 1
+    sel, idx = {}, {}
+    for i in range(50):
+        for t in range(2):
+            vs = _sann_bruteforce(tr2, L2, 1, i, spec2)[t]
+            sel[i, t] = vs
+            # the indices of the chosen neighbours, matched by their vectors
+            idx[i, t] = [next(j for j in range(50) if j != i and tr2.type_id[j] == t and
+                              np.allclose((lambda d: d - L2 * np.round(d / L2))(tr2.pos[1, i] - tr2.pos[1, j]), v))
+                         for v in vs]
+    dens = {(i, t): sum((_real_ylm_scipy(x, 6) for x in sel[i, t]), np.zeros(49)) for i in range(50) for t in range(2)}
+    blocks = [(l, 49 - (l + 1) ** 2) for l in range(7)]
+    for i in range(0, 50, 5):
+        for t in range(2):
+            n = len(sel[i, t])
+            if n == 0:
+                assert np.all(got[i, t] == 0)
+                continue
+            qb = dens[i, t] / n if tr2.type_id[i] == t else np.zeros(49)
+            for k in idx[i, t]:
+                qb = qb + dens[k, t] / len(sel[k, t])
+            for l, off in blocks[1:]:
+                v = np.sum(qb[off:off + 2 * l + 1] ** 2)
+                assert abs(got[i, t, l - 1] - np.sqrt(v / n / n * 4 * np.pi / (2 * l + 1))) < 1e-11
+
+
+def test_averaged_histogram_nan_cells():
+    """A SANN neighbour k with no neighbours of type_j has n(k) = 0 and
+    contributes 0/0 to qbar (steinhardt.cpp:155-160), so q is NaN for the
+    atom; the histogram still counts it, in the last cell of every l, which
+    is where the size_t conversion and clamp of steinhardt.cpp:199-202 put a
+    NaN on x86-64.  Every (frame, atom, type_j) with n > 0 is counted once."""
+    F, N = 30, 90
+    box_l = synth.lammps_box(16.0)
+    pos = synth.random_walk(73, N, F, box_l, 0.1)
+    types = synth.types_by_fraction(N, [0.7, 0.3]).astype(np.int32)
+    tr = O.Traj(pos, types, np.tile(box_l, (F, 1)), False)
+    nls = [(19, 16.0, 16.0), (17, 16.0, 16.0)]
+    nb, skip = 100, 13
+    q = O.steinhardt_sann(tr, nls, 6, nb, [4, 6], skip, F, 0, False, averaged=True)[0]   # [frame][atom][tj][l-1]
+    h = O.steinhardt_sann(tr, nls, 6, nb, [4, 6], skip, F, 0, True, averaged=True)[0]    # [ti][tj][q4][q6]
+    plain = O.steinhardt_sann(tr, nls, 6, nb, [4, 6], skip, F, 0, False)[0]
+    nan = np.isnan(q[..., 3])
+    assert nan.sum() > 0 and np.array_equal(nan, np.isnan(q[..., 5]))
+    assert np.all(np.nan_to_num(q[..., 3:6:2], nan=0.0) < 1.0)      # finite values never reach the last cell
+    incr = 1.0 / (F // skip)
+    counted = np.any(plain != 0, axis=-1)                            # n(i, tj) > 0
+    for ti in range(2):
+        for tj in range(2):
+            rows = types == ti
+            assert h[ti, tj].sum() == pytest.approx(counted[:, rows, tj].sum() * incr)
+            assert h[ti, tj, nb - 1, nb - 1] == pytest.approx(nan[:, rows, tj].sum() * incr)
