@@ -76,7 +76,7 @@ SIGNATURES = [
     ("anl_sph_snapshot", _i, [_vp, _u64, _vp]),
     ("anl_sph_set_budget", _i, [_vp, _u64]),
     ("anl_sph_set_neighbours", _i, [_vp, _vp, _vp]),
-    ("anl_steinhardt_calculate", _i, [_vp, _u64, _u32, _u32, _vp, _u32, _u32, _i, _vp]),
+    ("anl_steinhardt_calculate", _i, [_vp, _u64, _u32, _u32, _vp, _u32, _u32, _i, _i, _vp]),
 ]
 
 _lib = None
@@ -310,7 +310,7 @@ class DeviceSph:
         check(lib().anl_sph_calculate(self.handle, primo, n, tmax, skip, _ptr(out)))
         return out
 
-    def steinhardt(self, n, skip, primo, l_list, nbin_st, do_histogram=True):
+    def steinhardt(self, n, skip, primo, l_list, nbin_st, do_histogram=True, averaged=False):
         """Steinhardt order parameters of frames primo + i, i = 0, skip, .. < n - n % skip."""
         nt, N = self.traj.ntypes, self.traj.natoms
         ll = np.ascontiguousarray(l_list, dtype=np.uint32)
@@ -319,7 +319,7 @@ class DeviceSph:
             (self.nbin, n // skip, N, nt, self.lmax)
         out = np.zeros(shape, dtype=np.float64)
         check(lib().anl_steinhardt_calculate(self.handle, primo, n, skip, _ptr(ll), len(ll), nbin_st,
-                                             int(bool(do_histogram)), _ptr(out)))
+                                             int(bool(do_histogram)), int(bool(averaged)), _ptr(out)))
         return out
 
     def snapshot(self, frame):

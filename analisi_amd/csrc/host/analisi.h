@@ -801,8 +801,8 @@ class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
 // Steinhardt<l> (steinhardt.h:17-120, steinhardt.cpp:7-236): per frame, the
 // order parameters q_l of every (atom, neighbour type, bin), as a histogram
 // over the listed l values or as a trajectory of q_1..q_lmax.  With a neighbour
-// list (SANN, one radial bin) the radial ranges are ignored; the averaged
-// order parameter is not supported.
+// list (SANN, one radial bin) the radial ranges are ignored, and the averaged
+// order parameter is available.
 // ---------------------------------------------------------------------------
 template <class T>
 class Steinhardt : public VectorOp<Steinhardt<T>> {
@@ -819,15 +819,13 @@ class Steinhardt : public VectorOp<Steinhardt<T>> {
                unsigned int lmax = ANL_SPH_LMAX_MAX)
         : traiettoria(t), handle(t, rminmax, nbin, lmax, nls), l_hist(std::move(steinhardt_l_histogram)), nbin(nbin),
           nbin_st(steinhardt_histogram_size), skip(skip == 0 ? 1 : skip), lmax(lmax), ntypes(t->get_ntypes()),
-          do_histogram(do_histogram) {
+          do_histogram(do_histogram), averaged(averaged_order) {
         (void)nthreads;
         if (nls.empty() && averaged_order)  // steinhardt.cpp:30-35
             throw std::runtime_error(
                 "You must use a neighbour list when computing a averaged steinhardt order parameter! ");
         if (!nls.empty() && nbin > 1)
             throw std::runtime_error("When using a neighbour list to compute steinhardt please set nbin to 1. ");
-        if (averaged_order)
-            throw std::runtime_error("the averaged Steinhardt order parameter is not supported");
         hist_size = 1;
         for (size_t i = 0; i < l_hist.size(); ++i) hist_size *= nbin_st;
         if (t->get_nloaded_timesteps() > 0) handle.get();
@@ -854,7 +852,8 @@ class Steinhardt : public VectorOp<Steinhardt<T>> {
                 "the size of the average ");
         const size_t off = traiettoria->window_offset(primo, ntimesteps);
         anl_check(anl_steinhardt_calculate(handle.get(), off, (uint32_t)ntimesteps, skip, l_hist.data(),
-                                           (uint32_t)l_hist.size(), nbin_st, do_histogram ? 1 : 0, vdata));
+                                           (uint32_t)l_hist.size(), nbin_st, do_histogram ? 1 : 0, averaged ? 1 : 0,
+                                           vdata));
     }
     std::vector<ssize_t> get_shape() const {  // steinhardt.h:76-87
         if (do_histogram) {
@@ -882,7 +881,7 @@ class Steinhardt : public VectorOp<Steinhardt<T>> {
     std::vector<unsigned int> l_hist;
     unsigned int nbin, nbin_st, skip, lmax;
     size_t ntypes, hist_size = 1, ntimesteps = 0;
-    bool do_histogram;
+    bool do_histogram, averaged;
 };
 
 // ---------------------------------------------------------------------------

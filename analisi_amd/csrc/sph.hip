@@ -352,6 +352,7 @@ struct SannArgs {
     double cut2[SANN_MAXT];     // squared cutoff of each neighbour type
     uint32_t total, capmax;     // sum and maximum of nn (total rounded up to even)
     int *overflow;              // set when a list would exceed its capacity
+    int32_t *nbr;               // [nfr][N][total] chosen neighbours (type-sorted index) from off[t], or null
 };
 
 // SphericalBase::calc with a neighbour list (neighbour.cpp:8-101,
@@ -444,6 +445,10 @@ __global__ __launch_bounds__(64) void sph_density_sann_kernel(SphDensArgs a, San
                 sum += rs[ms];
                 ms++;
             }
+        }
+        if (s.nbr) {
+            int32_t *o = s.nbr + ((uint64_t)fl * a.N + (uint64_t)a.aprime[i]) * s.total + s.off[t];
+            for (uint32_t k = lane; k < ms; k += 64) o[k] = a.aprime[js[k]];
         }
         for (uint32_t c0 = 0; c0 < ms; c0 += 64) {
             qn = min(64u, ms - c0);
@@ -607,6 +612,9 @@ struct SteinArgs {
     uint32_t nt, nbin, lmax, nl, nbin_st, nfr, fidx0, nfs;
     double c00;                  // Y_00 of any direction: the density's l = 0 entry is (count) * c00
     uint32_t l_list[STEIN_MAXL];
+    const int32_t *nbr;          // averaged order parameter: [nfr][N][nbr_stride] SANN neighbours, or null
+    uint32_t nbr_stride;
+    uint32_t nbr_off[SANN_MAXT];  // first neighbour of each type
 };
 
 // steinhardt.cpp:166-225: one thread per (frame of the chunk, type_j, bin,
@@ -626,15 +634,37 @@ __global__ __launch_bounds__(256) void sph_steinhardt_kernel(SteinArgs a) {
     const double *Sb = a.S + (uint64_t)fl * a.FS + (uint64_t)(jt * a.nbin + b) * L2 * a.N;
     const double na = rint(Sb[ap] / a.c00);
     const double pi = 3.14159265358979323846264338327950288419716939937510;
+    const uint32_t ti = (uint32_t)a.type_ap[ap];
+    // averaged order parameter (steinhardt.cpp:128-173): qbar = S(i)/n(i) for
+    // the atom's own type, plus S(k)/n(k) of its SANN neighbours k in order
+    const int32_t *nb = a.nbr ? a.nbr + ((uint64_t)fl * a.N + ap) * a.nbr_stride + a.nbr_off[jt] : nullptr;
+    const uint32_t nk = nb ? min((uint32_t)na, a.nbr_stride - a.nbr_off[jt]) : 0;
     double q[ANL_SPH_LMAX_MAX + 1];
 #pragma unroll
     for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l) {
         q[l] = 0.0;
         if (l <= (int)a.lmax && na > 0) {
-            const double *s = Sb + (uint64_t)(l * l) * a.N + ap * (2 * l + 1);
-            double v = s[l] * s[l];
-            for (int mm = 1; mm <= l; ++mm) v += s[l - mm] * s[l - mm];
-            for (int mm = 1; mm <= l; ++mm) v += s[l + mm] * s[l + mm];
+            const uint64_t lb = (uint64_t)(l * l) * a.N;
+            const double *s = Sb + lb + ap * (2 * l + 1);
+            double v;
+            if (!nb) {
+                v = s[l] * s[l];
+                for (int mm = 1; mm <= l; ++mm) v += s[l - mm] * s[l - mm];
+                for (int mm = 1; mm <= l; ++mm) v += s[l + mm] * s[l + mm];
+            } else {
+                // qbar of m = 0, then m = -1..-l, then m = 1..l, squared and summed in that order
+                v = 0.0;
+                for (int o = 0; o <= 2 * l; ++o) {
+                    const int mi = o == 0 ? l : (o <= l ? l - o : o);  // index of (l, m) in the m-run
+                    double qb = ti == jt ? s[mi] / na : 0.0;
+                    for (uint32_t k = 0; k < nk; ++k) {
+                        const uint64_t b2 = (uint64_t)nb[k];
+                        const double nn2 = rint(Sb[b2] / a.c00);
+                        qb += Sb[lb + b2 * (2 * l + 1) + mi] / nn2;
+                    }
+                    v = o == 0 ? qb * qb : v + qb * qb;
+                }
+            }
             q[l] = sqrt(v / na / na * 4 * pi / (2 * l + 1));
         }
     }
@@ -647,11 +677,13 @@ __global__ __launch_bounds__(256) void sph_steinhardt_kernel(SteinArgs a) {
             for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l)
                 if ((uint32_t)l == a.l_list[kk]) qk = q[l];
             stride /= a.nbin_st;
-            uint64_t jh = (uint64_t)floorf((float)(qk * a.nbin_st));
-            if (jh >= a.nbin_st) jh = a.nbin_st - 1;
+            // beyond the last cell, and NaN (a neighbour without neighbours of
+            // type_j in the averaged form), go to the last cell, as the size_t
+            // conversion and clamp of steinhardt.cpp:199-202 do on x86-64
+            const float x = floorf((float)(qk * a.nbin_st));
+            const uint64_t jh = x < (float)a.nbin_st ? (uint64_t)x : a.nbin_st - 1;
             hidx += jh * stride;
         }
-        const uint32_t ti = (uint32_t)a.type_ap[ap];
         atomicAdd(&a.hist[((uint64_t)(b * a.nt + ti) * a.nt + jt) * a.hs + hidx], 1ull);
     } else {
         double *o = a.q + ((((uint64_t)b * a.nfs + a.fidx0 + fl) * a.N + (uint64_t)a.orig_ap[ap]) * a.nt + jt) * a.lmax;
@@ -742,7 +774,7 @@ uint64_t radius_bytes(const anl_sph *h, uint64_t nfr) {
 // densities of frames f0 + k * fstride, k < nfr, of the device trajectory into
 // S; rad holds radius_bytes(h, nfr)
 void launch_density(anl_sph *h, uint64_t f0, uint64_t fstride, uint32_t nfr, double *S, double4 *rad,
-                    hipStream_t s) {
+                    hipStream_t s, int32_t *nbr = nullptr) {
     SphDensArgs a{};
     a.fstride = fstride;
     if (h->sann) {
@@ -761,7 +793,8 @@ void launch_density(anl_sph *h, uint64_t f0, uint64_t fstride, uint32_t nfr, dou
         a.nbc = 1;
         const uint64_t gx = (uint64_t)nfr * a.N;
         if (gx > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom) pairs in one density launch");
-        const SannArgs &sa = h->sann_args;
+        SannArgs sa = h->sann_args;
+        sa.nbr = nbr;  // the chosen neighbours, for the averaged order parameter
         const size_t lds = (size_t)(h->nt * (h->lmax + 1) * (h->lmax + 1) + stage_doubles(h->lmax) + sa.total +
                                     sa.capmax) * sizeof(double) +
                            (size_t)(sa.total + sa.capmax + SANN_MAXT) * sizeof(int);
@@ -894,9 +927,11 @@ void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32
 // steinhardt.cpp:47-236 (no neighbour list, not averaged): frames primo + i,
 // i = 0, skip, .. < n - n % skip, in chunks of frames that fit the budget
 void steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip, const uint32_t *l_list, uint32_t nl,
-                          uint32_t nbin_st, int do_hist, double *out_host) {
+                          uint32_t nbin_st, int do_hist, int averaged, double *out_host) {
     anl_traj *tr = h->tr;
     if (skip == 0) skip = 1;
+    if (averaged && !h->sann)  // steinhardt.cpp:30-32
+        fail(ANL_ERR_ARG, "You must use a neighbour list when computing a averaged steinhardt order parameter! ");
     if ((uint64_t)n + primo > tr->F)
         fail(ANL_ERR_TOO_SHORT,
              "trajectory is too short for this kind of calculation. Select a different starting timestep or lower "
@@ -917,11 +952,14 @@ void steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip,
     if (nfs == 0 || len == 0) return;
     const uint64_t fbytes = frame_doubles(h) * sizeof(double);
     const uint64_t fpc = std::min<uint64_t>(nfs, std::max<uint64_t>(1, h->budget / std::max<uint64_t>(fbytes, 1)));
+    const uint64_t nbr_bytes = averaged ? fpc * N * h->sann_args.total * sizeof(int32_t) : 0;
     const uint64_t offR = a256(fpc * fbytes), offO = offR + a256(radius_bytes(h, fpc));
-    char *w = (char *)workspace(h, offO + a256(len * 8));
+    const uint64_t offN = offO + a256(len * 8);
+    char *w = (char *)workspace(h, offN + a256(nbr_bytes));
     double *S = (double *)w;
     double4 *rad = (double4 *)(w + offR);
     double *outd = (double *)(w + offO);
+    int32_t *nbr = averaged ? (int32_t *)(w + offN) : nullptr;
     hipStream_t s = nullptr;
     hipLaunchKernelGGL(sph_zero_kernel, dim3((uint32_t)((len + 255) / 256)), dim3(256), 0, s, outd, len);
     ANL_HIP(hipGetLastError());
@@ -942,9 +980,12 @@ void steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip,
     a.nfs = (uint32_t)nfs;
     a.c00 = h->c00;
     for (uint32_t k = 0; k < nl; k++) a.l_list[k] = l_list[k];
+    a.nbr = nbr;
+    a.nbr_stride = h->sann_args.total;
+    for (uint32_t t = 0; t < nt && t < SANN_MAXT; t++) a.nbr_off[t] = h->sann_args.off[t];
     for (uint64_t f0 = 0; f0 < nfs; f0 += fpc) {
         const uint32_t nfc = (uint32_t)std::min<uint64_t>(fpc, nfs - f0);
-        launch_density(h, primo + f0 * skip, skip, nfc, S, rad, s);
+        launch_density(h, primo + f0 * skip, skip, nfc, S, rad, s, nbr);
         a.nfr = nfc;
         a.fidx0 = (uint32_t)f0;
         const uint64_t nth = (uint64_t)nfc * nt * nbin * N;
@@ -1125,10 +1166,10 @@ int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host) {
 }
 
 int anl_steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip, const uint32_t *l_list,
-                             uint32_t nl, uint32_t nbin_st, int do_histogram, double *out_host) {
+                             uint32_t nl, uint32_t nbin_st, int do_histogram, int averaged, double *out_host) {
     return guard([&] {
         if (!h || !out_host || (nl > 0 && !l_list)) fail(ANL_ERR_ARG, "null pointer passed to anl_steinhardt_calculate");
-        steinhardt_calculate(h, primo, n, skip, l_list, nl, nbin_st, do_histogram, out_host);
+        steinhardt_calculate(h, primo, n, skip, l_list, nl, nbin_st, do_histogram, averaged, out_host);
     });
 }
 

@@ -169,9 +169,11 @@ int anl_sph_snapshot(anl_sph *h, uint64_t frame, double *out_host);
  * [ntypes][nbin_st]^nl (one axis per entry of l_list, the first slowest),
  * count * incr with incr = 1/int(n/skip); else out_host [nbin][n/skip]
  * [natoms][ntypes][lmax] holding q_1..q_lmax (0 without neighbours).
+ * averaged (neighbour lists only, :128-173): q_l of qbar_lm = S_lm/n of the
+ * atom (for its own type) plus S_lm/n of each of its SANN neighbours.
  * ANL_ERR_TOO_SHORT when n + primo exceeds the frames (:80-83). */
 int anl_steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip, const uint32_t *l_list,
-                             uint32_t nl, uint32_t nbin_st, int do_histogram, double *out_host);
+                             uint32_t nl, uint32_t nbin_st, int do_histogram, int averaged, double *out_host);
 /* Switches the handle to neighbour lists (Neighbours, neighbour.h:10-104,
  * neighbour.cpp:8-101, the SANN variant of sphericalbase.cpp:67-93): for
  * every atom and type t, the atoms of type t within sqrt(cutoff2[t]), at most

@@ -434,8 +434,9 @@ def test_lammps_window_read_ahead_changes_nothing(pa, tmp_path):
 def test_steinhardt_neighbour_list_fixture_call(pa):
     """SteinhardtFixtureNeigh of tests/src/test_sh.cpp:50-62 through the
     Python class: SANN lists of at most 19 and 17 neighbours within 4 A for the
-    two types, one bin, l = (4, 6), skip 13, histogram off and on; equal to the
-    oracle.  SphericalCorrelations with the same lists equals the oracle too."""
+    two types, one bin, l = (4, 6), skip 13, histogram off and on, plain and
+    averaged order; equal to the oracle.  SphericalCorrelations with the same
+    lists equals the oracle too."""
     F, N = 30, 90
     box_l = synth.lammps_box(16.0)      # ~6 atoms within 4 A: the lists of 19 and 17 hold them
     pos = synth.random_walk(73, N, F, box_l, 0.1)
@@ -459,5 +460,13 @@ def test_steinhardt_neighbour_list_fixture_call(pa):
     sc.reset(12)
     sc.calculate(2)
     _sh_close(np.array(sc), O.sph_correlations_sann(ot, nls, 2, 6, 3, 12, 2))
-    with pytest.raises(RuntimeError, match="averaged Steinhardt order parameter is not supported"):
-        pa.SteinhardtOrderParameterHistogram_6(t, rm, 1, 100, [4, 6], 2, 13, True, nls, True)
+    for hist in (False, True):
+        av = pa.SteinhardtOrderParameterHistogram_6(t, rm, 1, 100, [4, 6], 2, 13, hist, nls, True)
+        av.reset(F)
+        av.calculate(0)
+        want = O.steinhardt_sann(ot, nls, 6, 100, [4, 6], 13, F, 0, hist, averaged=True)
+        if hist:
+            np.testing.assert_allclose(np.array(av), want, rtol=1e-13, atol=0)
+        else:   # NaN where a neighbour has no neighbours of type_j (0/0, as the reference)
+            assert np.isnan(want).any()
+            np.testing.assert_allclose(np.array(av), want, rtol=0, atol=1e-12, equal_nan=True)

@@ -185,7 +185,8 @@ def test_steinhardt_fcc_and_errors():
 @pytest.mark.parametrize("tri", [False, True])
 def test_sann_densities_correlations_steinhardt(tri):
     """Densities, correlations and Steinhardt q / histograms with SANN
-    neighbour lists equal the oracle's (1e-12; histograms exactly)."""
+    neighbour lists, plain and averaged (steinhardt.cpp:128-173), equal the
+    oracle's (1e-12; histograms exactly)."""
     ot, dt = _case(79, 90, 12, 8.0, [0.6, 0.4], tri)
     spec = [(40, 3.3 ** 2, 0.0), (30, 3.6 ** 2, 0.0)]
     s = capi.DeviceSph(dt, _ranges(2, 3), 2, 10)
@@ -201,6 +202,12 @@ def test_sann_densities_correlations_steinhardt(tri):
     assert np.abs(q - O.steinhardt_sann(ot, spec, 6, 20, [4, 6], 2, 9, 2, False)).max() <= 1e-12
     h = s1.steinhardt(9, 2, 2, [4, 6], 20)
     np.testing.assert_allclose(h, O.steinhardt_sann(ot, spec, 6, 20, [4, 6], 2, 9, 2, True), rtol=1e-13, atol=0)
+    for hist in (False, True):
+        qa = s1.steinhardt(9, 2, 2, [4, 6], 20, do_histogram=hist, averaged=True)
+        qw = O.steinhardt_sann(ot, spec, 6, 20, [4, 6], 2, 9, 2, hist, averaged=True)
+        np.testing.assert_allclose(qa, qw, rtol=1e-13 if hist else 0, atol=0 if hist else 1e-12)
+    with pytest.raises(capi.AnlError, match="neighbour list"):
+        capi.DeviceSph(dt, _ranges(2, 3), 1, 6).steinhardt(9, 2, 2, [4, 6], 20, averaged=True)
     small = capi.DeviceSph(dt, _ranges(2, 3), 1, 6)
     small.set_neighbours([(3, 3.3 ** 2, 0.0), (30, 3.6 ** 2, 0.0)])
     with pytest.raises(capi.AnlError, match="Too many neighbours"):
