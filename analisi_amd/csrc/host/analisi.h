@@ -466,6 +466,20 @@ class ReadLog_numpy {
 // ---------------------------------------------------------------------------
 // Gofrt (gofrt.h:28-107, gofrt.cpp)
 // ---------------------------------------------------------------------------
+// whether a trajectory holds frames a device handle can be built over now:
+// a windowed trajectory only once its window has been read
+template <class X, class = void>
+struct windowed_reader : std::false_type {};
+template <class X>
+struct windowed_reader<X, std::void_t<decltype(std::declval<const X &>().has_window_data())>> : std::true_type {};
+template <class TR>
+inline bool has_frames(TR *t) {
+    if constexpr (windowed_reader<TR>::value)
+        return t->has_window_data();
+    else
+        return t->get_nloaded_timesteps() > 0;
+}
+
 template <class TFLOAT, class T>
 class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
     using VOp = VectorOp<Gofrt<TFLOAT, T>, TFLOAT>;
@@ -477,7 +491,7 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
           unsigned int skip = 1, unsigned int every = 1, bool debug = false, int form = ANL_GOFRT_AUTO)
         : traiettoria(t), rmin(rmin), rmax(rmax), nbin(nbin), lmax(tmax), nthreads(nthreads == 0 ? 1 : nthreads),
           skip(skip == 0 ? 1 : skip), every(every == 0 ? 1 : every), debug(debug), form(form) {
-        if (t->get_nloaded_timesteps() > 0) h();  // argument errors surface here
+        if (has_frames(t)) h();  // argument errors surface here
     }
     ~Gofrt() {
         if (g) anl_gofrt_destroy(g);
@@ -491,6 +505,7 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
         return (v < lmax || lmax == 0) ? v : lmax;
     }
     void reset(const unsigned int n) {  // gofrt.cpp:41-63
+        c_descr = legend();
         leff = (n < lmax || lmax == 0) ? n : lmax;
         ntimesteps = n;
         data_length = leff * nch() * nbin;
@@ -503,6 +518,7 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
     std::vector<ssize_t> get_stride() {
         return {(ssize_t)(nch() * nbin * sizeof(TFLOAT)), (ssize_t)(nbin * sizeof(TFLOAT)), (ssize_t)sizeof(TFLOAT)};
     }
+    std::string get_columns_description() { return c_descr; }
     // CalculateMultiThread::calculate(primo) with Gofrt's flags: calc_init,
     // the whole (lag, t0, pair) nest in one device call, calc_end.
     void calculate(size_t primo) {
@@ -560,6 +576,31 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
         return g;
     }
     unsigned int nch() const { return (unsigned int)(traiettoria->get_ntypes() * (traiettoria->get_ntypes() + 1)); }
+    // gofrt.h:85-104: channel of the type pair (t1, t2), t1 <= t2, among the
+    // nt (nt+1) / 2 distinct-atom channels; counted from the last pair back
+    unsigned int pair_channel(unsigned int t1, unsigned int t2) const {
+        const unsigned int nt = (unsigned int)traiettoria->get_ntypes();
+        const unsigned int lo = std::min(t1, t2), hi = std::max(t1, t2);
+        return nt * (nt + 1) / 2 - (hi + 1) * (hi + 2) / 2 + lo;
+    }
+    // gofrt.cpp:43-53: the command line's legend of its g(r,t) columns (1-based:
+    // lag, bin, then a mean and a variance per channel)
+    std::string legend() const {
+        const unsigned int nt = (unsigned int)traiettoria->get_ntypes();
+        std::ostringstream d;
+        d << "# The first column is the time difference in timesteps, then you have the bin index. Every column "
+             "after is followed by the variance. Then you have the following: \n";
+        for (unsigned int a = 0; a < nt; a++)
+            for (unsigned int b = a; b < nt; b++) {
+                const unsigned int c = pair_channel(a, b);
+                d << "#g(" << a << ", " << b << "), different atom index: " << c * 2 + 3 << "\n";
+                d << "#g(" << a << ", " << b << "), same atom index: " << (c + nt * (nt + 1) / 2) * 2 + 3 << "\n";
+            }
+        d << "# same atom index means that the atom is tracked around and the average self-spread is shown with "
+             "larger time differences.\n";
+        d << "# different atom index is something that for t=0 is the traditional g(r) \n";
+        return d.str();
+    }
     T *traiettoria;
     TFLOAT rmin, rmax, incr = 1;
     unsigned int nbin, lmax, nthreads, skip, every;
@@ -569,6 +610,7 @@ class Gofrt : public VectorOp<Gofrt<TFLOAT, T>, TFLOAT> {
     bool reset_done = false;
     uint64_t gen = 0;
     anl_gofrt *g = nullptr;
+    std::string c_descr;
 };
 
 // ---------------------------------------------------------------------------
@@ -726,7 +768,7 @@ class SphericalCorrelations : public VectorOp<SphericalCorrelations<T>> {
         : traiettoria(t), handle(t, rminmax, nbin, lmax, neighList), nbin(nbin), tmax(tmax), nthreads(nthreads),
           skip(skip == 0 ? 1 : skip), buffer_size(buffer_size), lmax(lmax), ntypes(t->get_ntypes()) {
         (void)debug;
-        if (t->get_nloaded_timesteps() > 0) handle.get();  // argument errors surface here
+        if (has_frames(t)) handle.get();  // argument errors surface here
     }
     SphericalCorrelations &operator=(const SphericalCorrelations &destra) {
         VOp::operator=(destra);
@@ -828,7 +870,7 @@ class Steinhardt : public VectorOp<Steinhardt<T>> {
             throw std::runtime_error("When using a neighbour list to compute steinhardt please set nbin to 1. ");
         hist_size = 1;
         for (size_t i = 0; i < l_hist.size(); ++i) hist_size *= nbin_st;
-        if (t->get_nloaded_timesteps() > 0) handle.get();
+        if (has_frames(t)) handle.get();
     }
     Steinhardt &operator=(const Steinhardt &destra) {
         VOp::operator=(destra);
@@ -1071,13 +1113,16 @@ class MediaVarCovar {
 // ---------------------------------------------------------------------------
 // BlockAverageG (blockaverage.h:83-218), single process
 // ---------------------------------------------------------------------------
+template <class X, class = void>
+struct counts_lines : std::false_type {};  // a log (n_timestep) rather than a trajectory
+template <class X>
+struct counts_lines<X, std::void_t<decltype(std::declval<X &>().n_timestep())>> : std::true_type {};
 template <class TR>
 inline unsigned int n_timesteps_of(TR *t) {
-    return (unsigned int)t->get_ntimesteps();
-}
-template <>
-inline unsigned int n_timesteps_of<ReadLog_numpy<double>>(ReadLog_numpy<double> *t) {
-    return t->n_timestep();
+    if constexpr (counts_lines<TR>::value)
+        return (unsigned int)t->n_timestep();
+    else
+        return (unsigned int)t->get_ntimesteps();
 }
 
 template <class TR, class T, typename... Args>
@@ -1087,6 +1132,14 @@ class BlockAverageG {
     struct reads_ahead : std::false_type {};
     template <class X>
     struct reads_ahead<X, std::void_t<decltype(std::declval<X &>().set_next_window(size_t()))>> : std::true_type {};
+    // trajectories read through a window of timesteps (blockaverage.h:32-51:
+    // only the LAMMPS binary; logs and in-memory arrays hold everything)
+    template <class X, class = void>
+    struct windowed : std::false_type {};
+    template <class X>
+    struct windowed<X, std::void_t<decltype(std::declval<X &>().set_access_at(size_t())),
+                                   decltype(std::declval<X &>().set_data_access_block_size(size_t()))>>
+        : std::true_type {};
 
    public:
     BlockAverageG(TR *t, const unsigned int &numero_blocchi) : traiettoria(t), n_b(numero_blocchi) {}
@@ -1106,8 +1159,12 @@ class BlockAverageG {
         ok = true;
         calcolo->reset(s);
         calc->calcola_begin(s, calcolo);
+        if constexpr (windowed<TR>::value)  // a block and its lags (:123)
+            traiettoria->set_data_access_block_size((size_t)s + calcolo->nExtraTimesteps(n_b));
         for (unsigned int iblock = 0; iblock < n_b; iblock++) {
             calcolo->reset(s);
+            if constexpr (windowed<TR>::value)  // :135, adopting a window read ahead
+                traiettoria->set_access_at((size_t)iblock * s);
             if constexpr (reads_ahead<TR>::value)
                 if (iblock + 1 < n_b) traiettoria->set_next_window((size_t)(iblock + 1) * s);
             calcolo->calculate(iblock * s);

@@ -244,6 +244,8 @@ class Trajectory {
     size_t get_natoms() const { return natoms; }
     size_t get_ntimesteps() const { return n_timesteps; }
     size_t get_nloaded_timesteps() const { return loaded_timesteps; }
+    // a window is sized and read (not only sized)
+    bool has_window_data() const { return dati_caricati && loaded_timesteps > 0; }
     ssize_t get_current_timestep() const { return (ssize_t)current_timestep; }
     bool is_triclinic() const { return triclinic; }
     size_t get_box_stride() const { return box_stride; }

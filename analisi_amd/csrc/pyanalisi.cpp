@@ -11,6 +11,7 @@
 #include <pybind11/stl.h>
 
 #include "analisi.h"
+#include "cli.h"
 #include "lammps_traj.h"
 #include "readlog.h"
 
@@ -264,6 +265,42 @@ void define_block_averages(py::module &m) {
         std::vector<ssize_t> sh(shape.begin(), shape.end());
         return py::make_tuple(copy_array(ba.media()->access_vdata(), sh), copy_array(ba.varianza()->access_vdata(), sh));
     });
+    // the command line's stdout text of -q / -Q / --mean-square-displacement-self
+    // (main.cpp:520-551) and of -g (main.cpp:552-581)
+    m.def("cli_msd_text",
+          [](T &t, unsigned int blocknumber, unsigned int skip, unsigned int stop_acf, bool msd_cm, bool msd_self) {
+              std::ostringstream o;
+              cli::mean_square_displacement(o, t, blocknumber, skip, stop_acf, msd_cm, msd_self);
+              return o.str();
+          },
+          py::arg("traj"), py::arg("blocknumber") = 20, py::arg("skip") = 1, py::arg("stop_acf") = 0,
+          py::arg("msd_cm") = false, py::arg("msd_self") = false);
+    m.def("cli_gofrt_text",
+          [](T &t, unsigned int blocknumber, double rmin, double rmax, unsigned int nbin, unsigned int stop_acf,
+             unsigned int skip, unsigned int every) {
+              std::ostringstream o;
+              cli::gofrt(o, t, blocknumber, rmin, rmax, nbin, stop_acf, skip, every);
+              return o.str();
+          },
+          py::arg("traj"), py::arg("blocknumber"), py::arg("rmin"), py::arg("rmax"), py::arg("nbin"),
+          py::arg("stop_acf") = 0, py::arg("skip") = 1, py::arg("every") = 1);
+}
+
+// the command line's stdout text of -H (main.cpp:388-513)
+template <class L>
+void define_cli_gk(py::module &m) {
+    m.def("cli_gk_text",
+          [](L &log, std::vector<std::string> headers, unsigned int blocknumber, unsigned int skip,
+             unsigned int stop_acf, bool sub_mean, unsigned int sub_mean_start, unsigned int n_seg, double factor_conv,
+             double factor_intToCorr, bool mfma) {
+              std::ostringstream o;
+              cli::green_kubo(o, log, headers, blocknumber, skip, stop_acf, sub_mean, sub_mean_start, n_seg,
+                              factor_conv, factor_intToCorr, mfma);
+              return o.str();
+          },
+          py::arg("log"), py::arg("headers"), py::arg("blocknumber") = 20, py::arg("skip") = 1,
+          py::arg("stop_acf") = 0, py::arg("subtract_mean") = false, py::arg("start_mean") = 0, py::arg("n_seg") = 1,
+          py::arg("factor_conv") = 1.0, py::arg("factor_intToCorr") = 1.0, py::arg("mfma") = false);
 }
 
 }  // namespace
@@ -369,6 +406,8 @@ PYBIND11_MODULE(_pyanalisi, m) {
           py::arg("subtract_mean"), py::arg("start_mean"), py::arg("mfma") = false);
     // BlockAverageG + MediaVar of Gofrt / MSD (blockaverage.h:194-203), the
     // CLI's -g and -Q paths: returns (mean, variance) in the calculation's layout.
+    define_cli_gk<RL>(m);
+    define_cli_gk<ReadLog<double>>(m);
     define_block_averages<Trajectory_numpy>(m);
     define_block_averages<Trajectory>(m);
 

@@ -12,7 +12,8 @@ SphericalCorrelations, SteinhardtOrderParameterHistogram_<l>, their
 from ._pyanalisi import (BoxFormat, Gofrt, Gofrt_lammps, GreenKubo, GreenKubo_file,  # noqa: F401
                          MeanSquareDisplacement, MeanSquareDisplacement_lammps, ReadLog, ReadLog_file,
                          SphericalCorrelations, SphericalCorrelations_lammps, Traj, Trajectory, gk_block_average,
-                         gofrt_block_average, has_mmap, info, msd_block_average)
+                         gofrt_block_average, has_mmap, info, msd_block_average, cli_gk_text, cli_msd_text,
+                         cli_gofrt_text)
 from . import _pyanalisi as _native
 
 # SteinhardtOrderParameterHistogram_<l> and _<l>_lammps, l = 2..10

@@ -15,6 +15,11 @@ Contents
                  (test_gk.py:4-12: lmax 2000, n_seg 4, reset(T-2000))
   cli_GK_heat_single.txt, cli_GK_heat_double.txt
                  tests/data/cli/GK_heat_* (stdout of the CLI, test_cli.sh:31-32)
+  cli_MSD_normal.txt, cli_MSD_cm.txt
+                 tests/data/cli/MSD_normal (-Q -s 10 -S 50) and MSD_cm
+                 (-q -s 10 -S 50), test_cli.sh:25-26; their input
+                 lammps2020.bin is not in the snapshot, so they fix the
+                 layout only (the -Q columns start with the -q ones)
   gofr_reference.npz
     test_gofr    tests/test_gofrt/test_gofr.csv as float64 [10, 12, 200]
                  (test_gofrt.py:4-13; its input positions.npy is not in the
@@ -61,7 +66,7 @@ def main(ref):
     test_gk = read_csv_column(os.path.join(ref, "tests/test_gk/test_gk.csv")).reshape(2000, 14)
     np.savez_compressed(os.path.join(out, "gk_reference.npz"), log=log, headers=np.array(headers),
                         gk_2comp=gk_2comp, test_gk=test_gk)
-    for name in ("GK_heat_single", "GK_heat_double"):
+    for name in ("GK_heat_single", "GK_heat_double", "MSD_normal", "MSD_cm"):
         shutil.copyfile(os.path.join(ref, "tests/data/cli", name), os.path.join(out, "cli_" + name + ".txt"))
     gofr = read_csv_column(os.path.join(ref, "tests/test_gofrt/test_gofr.csv")).reshape(10, 12, 200)
     np.savez_compressed(os.path.join(out, "gofr_reference.npz"), test_gofr=gofr)

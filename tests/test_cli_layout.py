@@ -1,0 +1,30 @@
+"""The command line's MSD text layout, as its own output files fix it
+(tests/data/cli/MSD_normal and MSD_cm, test_cli.sh:25-26; their input
+lammps2020.bin is not in the snapshot, so the values cannot be recomputed).
+
+MSD_normal is -Q (atoms and centres of mass, 2 types) and MSD_cm is -q
+(atoms only) over the same trajectory with -s 10 -S 50: one line per lag,
+mean and variance of each column, each value followed by a space
+(main.cpp:537-544).  The -Q line starts with the -q line's columns, and lag 0
+is all zeros.  The GPU tests check that the printers of this build's
+command-line text keep both properties."""
+import os
+
+from conftest import GOLDEN
+
+
+def _lines(name):
+    with open(os.path.join(GOLDEN, "cli_%s.txt" % name)) as f:
+        return f.read().split("\n")[:-1]
+
+
+def test_msd_cli_files_layout():
+    q_atoms, q_cm = _lines("MSD_cm"), _lines("MSD_normal")
+    assert len(q_atoms) == len(q_cm) > 1
+    for a, c in zip(q_atoms, q_cm):
+        assert a.endswith(" ") and c.endswith(" ")
+        ta, tc = a.split(), c.split()
+        assert len(ta) == 4 and len(tc) == 8           # 2 types x (mean, variance), x 2 with -Q
+        assert tc[:4] == ta                             # [lag][f_cm][type]: the atoms' block first
+    assert q_atoms[0] == "0 " * 4 and q_cm[0] == "0 " * 8
+    assert all(float(x) >= 0 for l in q_cm for x in l.split())

@@ -416,6 +416,7 @@ def test_lammps_window_read_ahead_changes_nothing(pa, tmp_path):
     ref = pa.gofrt_block_average(tnp, 4, 0.0, 5.0, 30, 4, 1, 1)
     assert np.array_equal(on[0], ref[0])
     tl.setPrefetch(True)
+    tl.setAccessWindowSize(20)          # the block loops sized the window to their blocks
     tl.prefetch(33)
     tl.setAccessStart(33)
     direct = pa.Traj(path)
@@ -470,3 +471,104 @@ def test_steinhardt_neighbour_list_fixture_call(pa):
         else:   # NaN where a neighbour has no neighbours of type_j (0/0, as the reference)
             assert np.isnan(want).any()
             np.testing.assert_allclose(np.array(av), want, rtol=0, atol=1e-12, equal_nan=True)
+
+
+# ---------------------------------------------------------------------------
+# The command line's block-averaged text outputs (main.cpp:388-581)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name,heads", [("single", ["c_flux[1]"]), ("double", ["c_flux[1]", "c_vcm[1][1]"])])
+def test_cli_gk_text_equals_reference_files(pa, analisi_log, gk_ref, name, heads, tmp_path):
+    """-l gk_integral.dat -H -a HEADS with the defaults (20 blocks, factors 1):
+    the whole text the native printer writes over the GPU path (factor lines,
+    the columns' legend, the rows) equals the reference's
+    tests/data/cli/GK_heat_* line for line, a NaN's printed sign aside.  The
+    text log read from a file (as the command line reads it) prints the same."""
+    txt = pa.cli_gk_text(analisi_log, heads)
+    with open(os.path.join(GOLDEN, "cli_GK_heat_%s.txt" % name)) as f:
+        ref = f.read()
+    norm = lambda x: x.replace("-nan", "nan").split("\n")
+    assert norm(txt) == norm(ref)
+    if name == "single":
+        p = str(tmp_path / "gk_integral.dat")
+        with open(p, "w") as f:
+            f.write("    ".join(gk_ref["headers"]) + "\n")
+            for row in gk_ref["log"]:
+                f.write(repr(float(row[0])) + "\t" + " ".join(repr(float(x)) for x in row[1:]) + "\n")
+        assert pa.cli_gk_text(pa.ReadLog_file(p, None, 1, 1, 200, []), heads) == txt
+
+
+def _fmt_rows(mean, var, w):
+    m, v = mean.reshape(-1, w), var.reshape(-1, w)
+    return "".join("".join("%g %g " % (m[i, j], v[i, j]) for j in range(w)) + "\n" for i in range(m.shape[0]))
+
+
+def test_cli_msd_text(pa):
+    """-q, -Q and --mean-square-displacement-self with -s 10 -S 50 (the
+    options of the reference's MSD_cm / MSD_normal files): each line is a
+    lag's block-averaged mean and variance of every column, 6 significant
+    digits, each value followed by a space; the -Q lines start with the -q
+    lines' columns and lag 0 is all zeros, as in those files."""
+    N, F = 64, 1200
+    box_l = synth.lammps_box(10.0)
+    pos = synth.random_walk(91, N, F, box_l, 0.1)
+    types = synth.types_by_fraction(N, [0.5, 0.5]).astype(np.int32)
+    t = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False)
+    nb, skip, stop = 20, 10, 50
+    txt = {}
+    for cm, me in ((False, False), (True, False), (False, True)):
+        txt[cm, me] = pa.cli_msd_text(t, nb, skip, stop, cm, me)
+        mean, var = pa.msd_block_average(t, nb, skip, stop, cm, me)
+        assert txt[cm, me] == _fmt_rows(mean, var, 2 * (2 if cm else 1))
+    q, Q = txt[False, False].split("\n")[:-1], txt[True, False].split("\n")[:-1]
+    assert len(q) == len(Q) == stop
+    assert q[0] == "0 " * 4 and Q[0] == "0 " * 8
+    assert all(b.split()[:4] == a.split() for a, b in zip(q, Q))
+
+
+def test_cli_gofrt_text(pa, tmp_path):
+    """-g 12 -F 0 6 -S 6 -e 2 with 2 blocks: the columns' legend, then per
+    lag (step 2) and bin the line "t r" with every channel's mean and
+    variance, two empty lines after each lag.  The legend is pinned by
+    physics: rmax exceeds the largest minimum-image distance of the 6 A cell,
+    so every pair is counted, and the distinct column of g(a, b) sums over r
+    to the ordered pair count 2 N_a N_b (a != b) or N_a (N_a - 1), the
+    same-atom column of g(a, a) to N_a and of g(a, b != a) to 0; types of
+    30/20/10 atoms make every count different.  Over a LAMMPS binary (which
+    the command line wraps into the cell) the text is the same as over the
+    wrapped in-memory trajectory."""
+    N, F = 60, 40
+    box_l = synth.lammps_box(6.0)
+    pos = synth.random_walk(17, N, F, box_l, 0.2)
+    types = np.repeat([0, 1, 2], [30, 20, 10]).astype(np.int32)
+    sizes = [30, 20, 10]
+    t = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, False, False)
+    nbin, stop, every = 12, 6, 2
+    txt = pa.cli_gofrt_text(t, 2, 0.0, 6.0, nbin, stop, 1, every)
+    mean, var = pa.gofrt_block_average(t, 2, 0.0, 6.0, nbin, stop, 1, every)
+    lines = txt.split("\n")
+    legend = [l for l in lines if l.startswith("#")]
+    assert len(legend) == 2 * 6 + 3 and lines[:len(legend)] == legend
+    body = "\n".join(lines[len(legend):])
+    want = ""
+    for lag in range(0, mean.shape[0], every):
+        for r in range(nbin):
+            want += "%d %d" % (lag, r) + "".join(" %g %g" % (mean[lag, c, r], var[lag, c, r])
+                                                  for c in range(mean.shape[1])) + "\n"
+        want += "\n\n"
+    assert body == want
+    rows = [l.split() for l in lines[len(legend):] if l]
+    for entry in legend[1:-2]:
+        pair, col = entry[1:8], int(entry.rsplit(" ", 1)[1])   # "#g(a, b), ... index: col"
+        a, b = int(entry[3]), int(entry[6])
+        same = "same atom" in entry
+        want_sum = (sizes[a] if a == b else 0) if same else (sizes[a] * (sizes[a] - 1) if a == b else 2 * sizes[a] * sizes[b])
+        for lag in range(0, mean.shape[0], every):
+            got = sum(float(r[col - 1]) for r in rows if int(r[0]) == lag)
+            assert got == pytest.approx(want_sum, rel=1e-4, abs=1e-9), (pair, same, lag)
+    path = str(tmp_path / "g.bin")
+    t.write_lammps_binary(path, 0, -1)
+    tl = pa.Traj(path)
+    tl.setAccessWindowSize(F)          # nothing loaded yet: the printer turns the wrap on first, as the CLI does
+    tw = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, True, False)
+    assert pa.cli_gofrt_text(tl, 2, 0.0, 6.0, nbin, stop, 1, every) == pa.cli_gofrt_text(tw, 2, 0.0, 6.0, nbin, stop, 1, every)
