@@ -125,6 +125,18 @@ __global__ __launch_bounds__(64) void sph_block_radius_kernel(const double4 *pos
 }
 
 // Lane m's share of the real harmonics of direction (x, y, z)
+// The direction of x (specialfunctions.h:11-22): cos theta, cos phi and
+// sin phi replace x0, x1, x2, computed once per neighbour and shared by its m
+// lanes.
+__device__ __forceinline__ void direction(double &x0, double &x1, double &x2) {
+    const double rxy = sqrt(x0 * x0 + x1 * x1);
+    const double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+    const double ct = x2 / r, sp = x1 / rxy, cp = x0 / rxy;
+    x0 = ct;
+    x1 = cp;
+    x2 = sp;
+}
+
 // (specialfunctions.h:11-22, 262-360): P_l^m for l = m..lmax and
 // cos/sin(m phi), written to y[l*l + l + m] (cos) and y[l*l + l - m] (sin);
 // cf[l] is the coefficient of (l, m).  The loops run to the largest lmax with
@@ -132,11 +144,8 @@ __global__ __launch_bounds__(64) void sph_block_radius_kernel(const double4 *pos
 // recursion multiplies by 1/(l-m), which depends on l and m only, where the
 // reference divides by l-m: the division leaves the dependency chain of the
 // recursion, and each step changes by at most one rounding.
-__device__ __forceinline__ void put_harmonics(double *y, double x0, double x1, double x2, int m, int lmax,
+__device__ __forceinline__ void put_harmonics(double *y, double ct, double cp, double sp, int m, int lmax,
                                               const double (&cf)[ANL_SPH_LMAX_MAX + 1]) {
-    const double rxy = sqrt(x0 * x0 + x1 * x1);
-    const double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
-    const double ct = x2 / r, sp = x1 / rxy, cp = x0 / rxy;
     const double st = sqrt(1 - ct * ct);
     double pmm = 1.0;
 #pragma unroll
@@ -175,10 +184,11 @@ __device__ __forceinline__ void put_harmonics(double *y, double x0, double x1, d
     }
 }
 
-// Expands the nb oldest queue entries (entry k in lane k: minimum-image vector
-// and type << 16 | bin - b0): lane (slot, m) puts entry slot's harmonics into
-// its staging block, the blocks are added into the density block acc in queue
-// order (each entry by one lane, in order), and the queue moves down by nb.
+// Expands the nb oldest queue entries (entry k in lane k: direction, as
+// direction() leaves it, and type << 16 | bin - b0): lane (slot, m) puts entry
+// slot's harmonics into its staging block, the blocks are added into the
+// density block acc in queue order (each entry by one lane, in order), and the
+// queue moves down by nb.
 __device__ __forceinline__ void expand_queue(uint32_t nb, double &qx0, double &qx1, double &qx2, uint32_t &qcode,
                                              uint32_t &qn, double *acc, double *stage, uint32_t nbc, uint32_t L2,
                                              uint32_t lane, uint32_t slot, uint32_t m, int lmax,
@@ -258,9 +268,10 @@ __global__ __launch_bounds__(256, 3) void sph_density_kernel(SphDensArgs a) {
     // can hold an atom within rcut of atom i.  In orthorhombic cells the
     // minimum image is the nearest one, a metric, so a block whose bounding
     // sphere's centre is farther than rcut + its radius cannot.  The atoms in
-    // range of each near block join the queue (a block that does not fit
-    // waits until the queue is empty); the queue is expanded nc at a time,
-    // and emptied at the end.  One expansion site keeps the code compact.
+    // range of each near block join the queue.  The queue is drained when a
+    // block's atoms do not fit and at the end: every entry's direction is
+    // computed in one pass over the lanes, then the entries are expanded nc
+    // at a time.  One expansion site keeps the code compact.
     uint64_t g = 0;
     unsigned long long blocks = 0;
     bool pending = false, finished = false, in = false;
@@ -334,9 +345,12 @@ __global__ __launch_bounds__(256, 3) void sph_density_kernel(SphDensArgs a) {
             qn += cnt;
             pending = false;
         }
-        const uint32_t keep = (pending || finished) ? 0 : nc - 1;
-        while (qn > keep)
-            expand_queue(min(qn, nc), qx0, qx1, qx2, qcode, qn, acc, stage, a.nbc, L2, lane, slot, m, (int)a.lmax, cf);
+        if (qn > 0 && (pending || finished)) {
+            if (lane < qn) direction(qx0, qx1, qx2);
+            while (qn > 0)
+                expand_queue(min(qn, nc), qx0, qx1, qx2, qcode, qn, acc, stage, a.nbc, L2, lane, slot, m,
+                             (int)a.lmax, cf);
+        }
         if (finished) break;
     }
     wave_sync_lds();
@@ -460,8 +474,10 @@ __global__ __launch_bounds__(64) void sph_density_sann_kernel(SphDensArgs a, San
                 min_image_ref<TRI>(qx0, qx1, qx2, c);
                 qcode = t << 16;
             }
+            if (lane < qn) direction(qx0, qx1, qx2);
             while (qn > 0)
-                expand_queue(min(qn, nc), qx0, qx1, qx2, qcode, qn, acc, stage, 1, L2, lane, slot, m, (int)a.lmax, cf);
+                expand_queue(min(qn, nc), qx0, qx1, qx2, qcode, qn, acc, stage, 1, L2, lane, slot, m,
+                             (int)a.lmax, cf);
         }
         wave_sync_lds();  // rs, js are reused by the next type
     }
