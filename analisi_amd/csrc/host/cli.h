@@ -1,8 +1,8 @@
 // The analisi command line's block-averaged text outputs (analisi/main.cpp),
 // for the three paths of the pair-correlation family: Green-Kubo (-H), mean
 // square displacement (-q, -Q, --mean-square-displacement-self) and g(r,t)
-// (-g).  Each runs the same BlockAverageG loop as the command line over this
-// build's device path and writes what the command line writes to stdout,
+// (-g).  Each runs the same BlockAverageG loop as the command line over the
+// device path and writes what the command line writes to stdout,
 // with an ostream's default formatting (6 significant digits), so the text
 // can be compared with the command line's own output files.
 #pragma once

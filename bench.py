@@ -782,7 +782,7 @@ def main():
         print(json.dumps({"copy_bytes": nb, "copies": 3}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: analisi_amd has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     from analisi_amd import capi
     hip = Hip()

@@ -6,8 +6,8 @@ MSD_normal is -Q (atoms and centres of mass, 2 types) and MSD_cm is -q
 (atoms only) over the same trajectory with -s 10 -S 50: one line per lag,
 mean and variance of each column, each value followed by a space
 (main.cpp:537-544).  The -Q line starts with the -q line's columns, and lag 0
-is all zeros.  The GPU tests check that the printers of this build's
-command-line text keep both properties."""
+is all zeros.  The GPU tests check that the printers of
+analisi_amd/csrc/host/cli.h keep both properties."""
 import os
 
 from conftest import GOLDEN
