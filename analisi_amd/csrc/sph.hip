@@ -626,6 +626,7 @@ struct SteinArgs {
     double *q;                   // [nbin][nfs][N][nt][lmax] q_l, l = 1..lmax, or null
     uint64_t N, FS, hs;
     uint32_t nt, nbin, lmax, nl, nbin_st, nfr, fidx0, nfs;
+    uint32_t lmask;              // bit l set: q_l is needed
     double c00;                  // Y_00 of any direction: the density's l = 0 entry is (count) * c00
     uint32_t l_list[STEIN_MAXL];
     const int32_t *nbr;          // averaged order parameter: [nfr][N][nbr_stride] SANN neighbours, or null
@@ -659,7 +660,7 @@ __global__ __launch_bounds__(256) void sph_steinhardt_kernel(SteinArgs a) {
 #pragma unroll
     for (int l = 0; l <= ANL_SPH_LMAX_MAX; ++l) {
         q[l] = 0.0;
-        if (l <= (int)a.lmax && na > 0) {
+        if (l <= (int)a.lmax && na > 0 && ((a.lmask >> l) & 1u)) {
             const uint64_t lb = (uint64_t)(l * l) * a.N;
             const double *s = Sb + lb + ap * (2 * l + 1);
             double v;
@@ -736,7 +737,7 @@ struct anl_sph {
     std::vector<int32_t> aprime;      // [N] type-sorted index of each slot
     std::vector<uint32_t> tstart;     // [nt+1]
     double *d_thr = nullptr;          // [nt*nt][nbin+1]
-    double *d_coef = nullptr;         // [(lmax+1)^2]
+    double *d_coef = nullptr;         // tables [(l'+1)^2] for l' = 0..lmax, in that order (coef_offset)
     int32_t *d_aprime = nullptr;
     uint32_t *d_tstart = nullptr;
     int32_t *d_type_ap = nullptr;     // [N] type of each type-sorted atom
@@ -755,9 +756,18 @@ struct anl_sph {
 
 namespace {
 
-uint64_t frame_doubles(const anl_sph *h) {
-    const uint64_t L2 = (uint64_t)(h->lmax + 1) * (h->lmax + 1);
+// doubles of one frame's densities expanded up to l = lm (default lmax)
+uint64_t frame_doubles(const anl_sph *h, uint32_t lm = ~0u) {
+    if (lm > h->lmax) lm = h->lmax;
+    const uint64_t L2 = (uint64_t)(lm + 1) * (lm + 1);
     return h->tr->N * h->nt * h->nbin * L2;
+}
+
+// first coefficient of the table for expansions up to l = lm
+uint64_t coef_offset(uint32_t lm) {
+    uint64_t o = 0;
+    for (uint32_t k = 0; k < lm; k++) o += (uint64_t)(k + 1) * (k + 1);
+    return o;
 }
 
 uint64_t a256(uint64_t b) { return (b + 255) / 256 * 256; }
@@ -788,9 +798,11 @@ uint64_t radius_bytes(const anl_sph *h, uint64_t nfr) {
 }
 
 // densities of frames f0 + k * fstride, k < nfr, of the device trajectory into
-// S; rad holds radius_bytes(h, nfr)
+// S, expanded up to l = lm (default lmax; the values of l <= lm do not depend
+// on it); rad holds radius_bytes(h, nfr)
 void launch_density(anl_sph *h, uint64_t f0, uint64_t fstride, uint32_t nfr, double *S, double4 *rad,
-                    hipStream_t s, int32_t *nbr = nullptr) {
+                    hipStream_t s, int32_t *nbr = nullptr, uint32_t lm = ~0u) {
+    if (lm > h->lmax) lm = h->lmax;
     SphDensArgs a{};
     a.fstride = fstride;
     if (h->sann) {
@@ -798,20 +810,20 @@ void launch_density(anl_sph *h, uint64_t f0, uint64_t fstride, uint32_t nfr, dou
         a.box = h->tr->box;
         a.type_slot = h->tr->type_slot;
         a.aprime = h->d_aprime;
-        a.coef = h->d_coef;
+        a.coef = h->d_coef + coef_offset(lm);
         a.S = S;
         a.N = h->tr->N;
-        a.FS = frame_doubles(h);
+        a.FS = frame_doubles(h, lm);
         a.f0 = f0;
         a.nt = h->nt;
         a.nbin = h->nbin;
-        a.lmax = h->lmax;
+        a.lmax = lm;
         a.nbc = 1;
         const uint64_t gx = (uint64_t)nfr * a.N;
         if (gx > 0x7fffffffull) fail(ANL_ERR_ARG, "too many (frame, atom) pairs in one density launch");
         SannArgs sa = h->sann_args;
         sa.nbr = nbr;  // the chosen neighbours, for the averaged order parameter
-        const size_t lds = (size_t)(h->nt * (h->lmax + 1) * (h->lmax + 1) + stage_doubles(h->lmax) + sa.total +
+        const size_t lds = (size_t)(h->nt * (lm + 1) * (lm + 1) + stage_doubles(lm) + sa.total +
                                     sa.capmax) * sizeof(double) +
                            (size_t)(sa.total + sa.capmax + SANN_MAXT) * sizeof(int);
         ANL_HIP(hipMemsetAsync(h->d_overflow, 0, sizeof(int), s));
@@ -842,15 +854,15 @@ void launch_density(anl_sph *h, uint64_t f0, uint64_t fstride, uint32_t nfr, dou
     a.type_slot = h->tr->type_slot;
     a.aprime = h->d_aprime;
     a.thr = h->d_thr;
-    a.coef = h->d_coef;
+    a.coef = h->d_coef + coef_offset(lm);
     a.S = S;
     a.N = h->tr->N;
-    a.FS = frame_doubles(h);
+    a.FS = frame_doubles(h, lm);
     a.f0 = f0;
     a.nt = h->nt;
     a.nbin = h->nbin;
-    a.lmax = h->lmax;
-    a.nbc = bins_per_launch(h->nt, h->nbin, h->lmax);
+    a.lmax = lm;
+    a.nbc = bins_per_launch(h->nt, h->nbin, lm);
     a.ngroups = (uint32_t)((a.N + SPH_WAVES - 1) / SPH_WAVES);
     const uint32_t accn = a.nt * a.nbc * (a.lmax + 1) * (a.lmax + 1);
     const size_t lds = (size_t)SPH_WAVES * (accn + stage_doubles(a.lmax)) * sizeof(double);
@@ -966,7 +978,14 @@ void steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip,
     const uint64_t len = do_hist ? (uint64_t)nbin * nt * nt * hs : (uint64_t)nbin * nfs * N * nt * h->lmax;
     std::fill(out_host, out_host + len, 0.0);
     if (nfs == 0 || len == 0) return;
-    const uint64_t fbytes = frame_doubles(h) * sizeof(double);
+    // the histogram reads the listed l only (and l = 0 for the neighbour
+    // count): the densities are expanded up to the largest of them
+    uint32_t lm = h->lmax;
+    if (do_hist) {
+        lm = 0;
+        for (uint32_t k = 0; k < nl; k++) lm = std::max(lm, l_list[k]);
+    }
+    const uint64_t fbytes = frame_doubles(h, lm) * sizeof(double);
     const uint64_t fpc = std::min<uint64_t>(nfs, std::max<uint64_t>(1, h->budget / std::max<uint64_t>(fbytes, 1)));
     const uint64_t nbr_bytes = averaged ? fpc * N * h->sann_args.total * sizeof(int32_t) : 0;
     const uint64_t offR = a256(fpc * fbytes), offO = offR + a256(radius_bytes(h, fpc));
@@ -986,11 +1005,14 @@ void steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip,
     a.hist = do_hist ? (unsigned long long *)outd : nullptr;  // zero bits are the integer 0
     a.q = do_hist ? nullptr : outd;
     a.N = N;
-    a.FS = frame_doubles(h);
+    a.FS = frame_doubles(h, lm);
     a.hs = hs;
     a.nt = nt;
     a.nbin = nbin;
-    a.lmax = h->lmax;
+    a.lmax = lm;  // with the histogram off, lm = lmax: q_1..q_lmax are written
+    a.lmask = do_hist ? 0u : ~0u;
+    if (do_hist)
+        for (uint32_t k = 0; k < nl; k++) a.lmask |= 1u << l_list[k];
     a.nl = nl;
     a.nbin_st = nbin_st;
     a.nfs = (uint32_t)nfs;
@@ -1001,7 +1023,7 @@ void steinhardt_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t skip,
     for (uint32_t t = 0; t < nt && t < SANN_MAXT; t++) a.nbr_off[t] = h->sann_args.off[t];
     for (uint64_t f0 = 0; f0 < nfs; f0 += fpc) {
         const uint32_t nfc = (uint32_t)std::min<uint64_t>(fpc, nfs - f0);
-        launch_density(h, primo + f0 * skip, skip, nfc, S, rad, s, nbr);
+        launch_density(h, primo + f0 * skip, skip, nfc, S, rad, s, nbr, lm);
         a.nfr = nfc;
         a.fidx0 = (uint32_t)f0;
         const uint64_t nth = (uint64_t)nfc * nt * nbin * N;
@@ -1058,10 +1080,11 @@ int anl_sph_create(anl_traj *tr, uint32_t lmax, const double *rminmax, uint32_t 
         }
         h->rcut = std::sqrt(h->d2hi) * (1.0 + 1e-12) + 1e-9;
         h->block_screen = !tr->triclinic && std::isfinite(h->rcut);
-        const uint32_t L1 = lmax + 1;
-        std::vector<double> coef((size_t)L1 * L1, 0.0);
-        for (uint32_t l = 0; l <= lmax; l++)
-            for (uint32_t m = 0; m <= l; m++) coef[l * L1 + m] = sh_coeff((int)l, (int)m);
+        // coefficient tables [l * (l'+1) + m] for expansions up to every l' <= lmax
+        std::vector<double> coef(coef_offset(lmax + 1), 0.0);
+        for (uint32_t lp = 0; lp <= lmax; lp++)
+            for (uint32_t l = 0; l <= lp; l++)
+                for (uint32_t m = 0; m <= l; m++) coef[coef_offset(lp) + l * (lp + 1) + m] = sh_coeff((int)l, (int)m);
         h->c00 = coef[0];
         h->aprime.assign(tr->N, 0);
         h->tstart.assign(nt + 1, 0);
