@@ -8,6 +8,7 @@
 
 #include <fcntl.h>
 #include <future>
+#include <thread>
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -358,8 +359,23 @@ class Trajectory {
         w.pcm.assign(nl * ntypes * 3, 0.0);
         w.vcm.assign(nl * ntypes * 3, 0.0);
         w.boxes.assign(nl * box_stride, 0.0);
+        // timesteps are independent (their own slices and centre-of-mass
+        // counts): split them over up to 8 host threads
+        const size_t nth = std::min<size_t>(nl, std::max(1u, std::min(8u, std::thread::hardware_concurrency())));
+        if (nth <= 1) {
+            load_steps(offs, w, 0, nl);
+            return;
+        }
+        std::vector<std::future<void>> parts;
+        for (size_t p = 0; p < nth; p++)
+            parts.push_back(std::async(std::launch::async, [this, &offs, &w, p, nth, nl] {
+                load_steps(offs, w, nl * p / nth, nl * (p + 1) / nth);
+            }));
+        for (auto &f : parts) f.get();  // rethrows a step's error (the other parts are joined on the way out)
+    }
+    void load_steps(const std::vector<size_t> &offs, Window &w, size_t k0, size_t k1) const {
         std::vector<size_t> cont(ntypes);
-        for (size_t k = 0; k < nl; k++) {
+        for (size_t k = k0; k < k1; k++) {
             lammps_bin::Header h;
             std::vector<std::pair<const char *, int>> chunks;
             read_step(offs[k], h, &chunks);
