@@ -212,3 +212,26 @@ def test_sann_densities_correlations_steinhardt(tri):
     small.set_neighbours([(3, 3.3 ** 2, 0.0), (30, 3.6 ** 2, 0.0)])
     with pytest.raises(capi.AnlError, match="Too many neighbours"):
         small.snapshot(0)
+
+
+def test_steinhardt_histogram_equals_binned_full_order_q():
+    """The histogram expands the densities only up to its largest l; binning
+    the per-atom q (expanded to lmax) by the reference's rule
+    (steinhardt.cpp:194-210: floorf of the float product, clamped, first l
+    slowest, weight 1/int(n/skip), atoms with neighbours only) gives the same
+    histogram, at 2,048 atoms and for l lists ending below lmax."""
+    N, L = 2048, (2048 / 0.1) ** (1 / 3)
+    ot, dt = _case(61, N, 12, L, [0.5, 0.5], False, 0.1)
+    s = capi.DeviceSph(dt, [(0.7, 3.5)] * 4, 2, 10)
+    n, skip, primo, nb = 10, 3, 1, 40
+    q = s.steinhardt(n, skip, primo, [], nb, do_histogram=False)      # [bin][frame][atom][tj][l-1]
+    has = np.any(q != 0, axis=-1)
+    ti = ot.type_id
+    for ll in ([4, 6], [2, 5, 8], [3]):
+        h = s.steinhardt(n, skip, primo, ll, nb)
+        want = np.zeros_like(h)
+        idx = [np.minimum(np.floor((q[..., l - 1] * nb).astype(np.float32)).astype(np.int64), nb - 1) for l in ll]
+        for b, f, a, tj in zip(*np.nonzero(has)):
+            want[(b, ti[a], tj) + tuple(int(x[b, f, a, tj]) for x in idx)] += 1.0 / (n // skip)
+        assert h.sum() > 0
+        np.testing.assert_allclose(h, want, rtol=1e-12, atol=0)
