@@ -871,7 +871,8 @@ struct Launch {
 
 // Splits every block's t0 range into chunks: a chunk bounds the LDS counters
 // (each t0 adds at most 256*N to one bin: 256*N*chunk < 2^32) and sets the
-// work per workgroup (about 2^31 pair evaluations).
+// work per workgroup (the tuned number of pair evaluations), finer where the
+// launch would not fill the chip.
 Launch plan(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, int single_k) {
     Launch L;
     const uint64_t N = g->tr->N;
@@ -879,7 +880,12 @@ Launch plan(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nbl
     if (cap < 1) cap = 1;
     uint64_t target = g->tune.pairs_per_wg / ((uint64_t)WG * N);
     if (target < 1) target = 1;
-    uint64_t chunk = std::min(cap, target);
+    const uint64_t chunk = std::min(cap, target);
+    // a launch of fewer than 8 workgroups per CU (2,048 in all) leaves SIMDs
+    // idle: a small block splits its t0 range finer (the counts are sums of
+    // integers, so the split changes no result)
+    const uint64_t ntiles = (N + WG - 1) / WG;
+    const uint64_t want = (2048 + nblocks - 1) / nblocks;
     for (uint32_t bi = 0; bi < nblocks; bi++) {
         uint32_t leff = (n[bi] < g->lmax || g->lmax == 0) ? n[bi] : g->lmax;
         L.blocks.push_back(BlockInfo{primo[bi], leff, 0});
@@ -889,8 +895,14 @@ Launch plan(anl_gofrt *g, const uint64_t *primo, const uint32_t *n, uint32_t nbl
             continue;
         }
         uint32_t n0 = (n[bi] + g->skip - 1) / g->skip;  // t0 = 0, skip, ... < n
-        for (uint64_t k = 0; k < n0; k += chunk)
-            L.work.push_back(WorkItem{bi, (uint32_t)k, (uint32_t)std::min<uint64_t>(n0, k + chunk), 0});
+        const uint64_t units = ntiles * std::max<uint64_t>(1, (leff + g->every - 1) / g->every);
+        uint64_t chunk_b = chunk;
+        if (units * ((n0 + chunk - 1) / chunk) < want) {
+            const uint64_t items = (want + units - 1) / units;
+            chunk_b = std::min(chunk, std::max<uint64_t>(1, (n0 + items - 1) / items));
+        }
+        for (uint64_t k = 0; k < n0; k += chunk_b)
+            L.work.push_back(WorkItem{bi, (uint32_t)k, (uint32_t)std::min<uint64_t>(n0, k + chunk_b), 0});
     }
     return L;
 }
