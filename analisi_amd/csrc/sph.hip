@@ -701,7 +701,18 @@ __global__ __launch_bounds__(256) void sph_steinhardt_kernel(SteinArgs a) {
             const uint64_t jh = x < (float)a.nbin_st ? (uint64_t)x : a.nbin_st - 1;
             hidx += jh * stride;
         }
-        atomicAdd(&a.hist[((uint64_t)(b * a.nt + ti) * a.nt + jt) * a.hs + hidx], 1ull);
+        // the lanes of a wavefront that hit the same cell add once, with their
+        // number (q concentrates in few cells, where atomics would queue)
+        const unsigned long long key = ((uint64_t)(b * a.nt + ti) * a.nt + jt) * a.hs + hidx;
+        const int lane = (int)(threadIdx.x & 63);
+        unsigned long long left = __ballot(1);
+        while (left) {
+            const int leader = __ffsll(left) - 1;
+            const unsigned long long lk = __shfl(key, leader);
+            const unsigned long long same = __ballot(key == lk);
+            if (lane == leader) atomicAdd(&a.hist[lk], (unsigned long long)__popcll(same));
+            left &= ~same;
+        }
     } else {
         double *o = a.q + ((((uint64_t)b * a.nfs + a.fidx0 + fl) * a.N + (uint64_t)a.orig_ap[ap]) * a.nt + jt) * a.lmax;
 #pragma unroll
