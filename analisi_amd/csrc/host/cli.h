@@ -8,6 +8,7 @@
 #pragma once
 
 #include <ostream>
+#include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -27,19 +28,25 @@ struct has_pbc_wrap<X, std::void_t<decltype(std::declval<X &>().set_pbc_wrap(tru
 // "#Factors" line, the factor header, the columns' legend, then per lag the
 // mean and variance of every column times its factor (factor_conv *
 // factor_intToCorr for the n^2 correlations, factor_conv for the rest) and
-// no covariances.
+// the covariance of every column pair of -z (cvar_list: pairs of column
+// indices, main.cpp:266-274) times both factors.
 template <class RL>
 void green_kubo(std::ostream &o, RL &log, const std::vector<std::string> &headers, unsigned int blocknumber,
                 unsigned int skip, unsigned int stop_acf, bool sub_mean, unsigned int sub_mean_start,
-                unsigned int n_seg, double factor_conv = 1.0, double factor_intToCorr = 1.0, bool mfma = false) {
+                unsigned int n_seg, double factor_conv = 1.0, double factor_intToCorr = 1.0, bool mfma = false,
+                const std::vector<unsigned int> &cvar_list = {}) {
     using GK = GreenKuboNComponentIonicFluid<RL, double, double>;
+    if (cvar_list.size() % 2 != 0)
+        throw std::runtime_error("Error: covariance indexes list must contain an even number of elements\n");
+    std::vector<std::pair<unsigned int, unsigned int>> cvar;
+    for (size_t i = 0; i + 1 < cvar_list.size(); i += 2) cvar.emplace_back(cvar_list[i], cvar_list[i + 1]);
     o << "#Factors: " << factor_conv << ", " << factor_conv * factor_intToCorr << "\n";
     BlockAverageG<RL, GK, std::string, unsigned int, std::vector<std::string>, bool, unsigned int, unsigned int, bool,
                   unsigned int, unsigned int, bool, unsigned int, unsigned int, bool>
         ba(&log, blocknumber);
     const unsigned int nh = (unsigned int)headers.size();
     const unsigned int narr = nh * nh * 3 + 2;
-    MediaVarCovar<GK> mvc(narr, {});
+    MediaVarCovar<GK> mvc(narr, cvar);
     ba.calcola_custom(&mvc, std::string(""), skip, headers, false, stop_acf, 1u, sub_mean, sub_mean_start, n_seg,
                       false, 10u, 100u, mfma);
     std::vector<double> f(narr, factor_conv);
@@ -49,6 +56,8 @@ void green_kubo(std::ostream &o, RL &log, const std::vector<std::string> &header
     for (unsigned int i = 0; i < mvc.size(); i++) {
         for (unsigned int j = 0; j < narr; j++)
             o << mvc.media(j)[i] * f[j] << " " << mvc.varianza(j)[i] * f[j] * f[j] << " ";
+        for (unsigned int j = 0; j < mvc.n_cvar(); j++)
+            o << mvc.covarianza(j)[i] * f[cvar[j].first] * f[cvar[j].second] << " ";
         o << "\n";
     }
 }

@@ -292,15 +292,16 @@ void define_cli_gk(py::module &m) {
     m.def("cli_gk_text",
           [](L &log, std::vector<std::string> headers, unsigned int blocknumber, unsigned int skip,
              unsigned int stop_acf, bool sub_mean, unsigned int sub_mean_start, unsigned int n_seg, double factor_conv,
-             double factor_intToCorr, bool mfma) {
+             double factor_intToCorr, bool mfma, std::vector<unsigned int> covariance) {
               std::ostringstream o;
               cli::green_kubo(o, log, headers, blocknumber, skip, stop_acf, sub_mean, sub_mean_start, n_seg,
-                              factor_conv, factor_intToCorr, mfma);
+                              factor_conv, factor_intToCorr, mfma, covariance);
               return o.str();
           },
           py::arg("log"), py::arg("headers"), py::arg("blocknumber") = 20, py::arg("skip") = 1,
           py::arg("stop_acf") = 0, py::arg("subtract_mean") = false, py::arg("start_mean") = 0, py::arg("n_seg") = 1,
-          py::arg("factor_conv") = 1.0, py::arg("factor_intToCorr") = 1.0, py::arg("mfma") = false);
+          py::arg("factor_conv") = 1.0, py::arg("factor_intToCorr") = 1.0, py::arg("mfma") = false,
+          py::arg("covariance") = std::vector<unsigned int>());
 }
 
 }  // namespace

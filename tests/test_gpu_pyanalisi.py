@@ -572,3 +572,35 @@ def test_cli_gofrt_text(pa, tmp_path):
     tl.setAccessWindowSize(F)          # nothing loaded yet: the printer turns the wrap on first, as the CLI does
     tw = pa.Trajectory(pos, np.zeros_like(pos), types, np.tile(box_l, (F, 1)), pa.BoxFormat.LammpsOrtho, True, False)
     assert pa.cli_gofrt_text(tl, 2, 0.0, 6.0, nbin, stop, 1, every) == pa.cli_gofrt_text(tw, 2, 0.0, 6.0, nbin, stop, 1, every)
+
+
+def test_cli_gk_text_covariances_and_factors(pa, analisi_log, gk_ref):
+    """-H with -F 2 3 and -z 0 2 1 3 (main.cpp:266-274, 430-513): every column
+    times its factor (factor_conv * factor_intToCorr = 6 for the 4
+    correlations, factor_conv = 2 for the rest), variances times its square,
+    then the two covariances times both columns' factors, against the
+    oracle's block statistics (MediaVarCovar, calcoliblocchi.h:72-202)
+    printed with 6 significant digits; an odd -z list is an error."""
+    heads = ["c_flux[1]", "c_vcm[1][1]"]
+    txt = pa.cli_gk_text(analisi_log, heads, factor_conv=2.0, factor_intToCorr=3.0, covariance=[0, 2, 1, 3])
+    lines = txt.split("\n")
+    assert lines[0] == "#Factors: 2, 6" and lines[1] == "# factor integral, factor correlation" and lines[2] == "#2 3"
+    rows = [l for l in lines if l and not l.startswith("#")]
+    log = gk_ref["log"]
+    hd = list(gk_ref["headers"])
+    idx = [hd.index(h) for h in heads]
+    T, nb = log.shape[0], 20
+    s = (T - O.n_extra_timesteps(T, nb, 0)) // nb
+    narr = 3 * 4 + 2
+    mvc = O.MediaVarCovar(narr, s, [(0, 2), (1, 3)])
+    for b in range(nb):
+        mvc.add(O.gk(log, idx, 1, 0, False, 0, 1, s, b * s))
+    mvc.end()
+    f = np.array([6.0] * 4 + [2.0] * (narr - 4))
+    assert len(rows) == s
+    for i in (0, 1, 7, s // 2, s - 1):
+        want = " ".join(sum([["%g" % (mvc.mean[j, i] * f[j]), "%g" % (mvc.var[j, i] * f[j] * f[j])] for j in range(narr)], []))
+        want += " " + " ".join("%g" % (mvc.cov[k, i] * f[a] * f[b]) for k, (a, b) in enumerate([(0, 2), (1, 3)])) + " "
+        assert rows[i].replace("-nan", "nan") == want.replace("-nan", "nan"), i
+    with pytest.raises(RuntimeError, match="even number"):
+        pa.cli_gk_text(analisi_log, heads, covariance=[0, 2, 1])
