@@ -52,6 +52,8 @@ SIGNATURES = [
     ("anl_gofrt_get_tuning", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("anl_gofrt_autotune", _i, [_vp, _u64, _u32, _vp, _u32, _vp, _vp]),
     ("anl_gofrt_last_stats", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
+    ("anl_gofrt_pair_stats", _i, [_vp] + [ctypes.POINTER(_u64)] * 5),
+    ("anl_gofrt_screen_info", _i, [_vp, _d, ctypes.POINTER(_i), ctypes.POINTER(_u32)] + [ctypes.POINTER(_d)] * 4),
     ("anl_blocks_create", _i, [_vp, _vp, _vp, _u32, _vp, _i, _u32, _u32, _i, _i, _i, _pp]),
     ("anl_blocks_run", _i, [_vp, _vp]),
     ("anl_blocks_gofrt_results", _i, [_vp, _vp, _vp, _vp, ctypes.POINTER(_u32)]),
@@ -59,6 +61,9 @@ SIGNATURES = [
     ("anl_blocks_destroy", _i, [_vp]),
     ("anl_msd_calculate", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp]),
     ("anl_msd_calculate_async", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
+    ("anl_msd_set_tuning", _i, [_vp, _u32]),
+    ("anl_msd_get_tuning", _i, [_vp, ctypes.POINTER(_u32)]),
+    ("anl_msd_autotune", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _u32, _vp, ctypes.POINTER(_u32)]),
     ("anl_series_create", _i, [_vp, _u64, _u32, _pp]),
     ("anl_series_destroy", _i, [_vp]),
     ("anl_gk_calculate", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp]),
@@ -129,6 +134,26 @@ class DeviceTrajectory:
                                     self.ntypes, _ptr(self.cm), F, N, ctypes.byref(h)))
         self.handle = h
 
+    def set_msd_tile(self, lag_tile):
+        """Lag tile of the MSD kernel (8, 12, 16, 24; 0 = eight lags per thread)."""
+        check(lib().anl_msd_set_tuning(self.handle, lag_tile))
+
+    def msd_tile(self):
+        v = _u32()
+        check(lib().anl_msd_get_tuning(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def msd_autotune(self, skip, tmax, cm_msd, cm_self, n, primo, candidates=None):
+        """Times the MSD calculation with each lag tile and keeps the fastest.
+        Returns (times in ms per candidate, index of the best)."""
+        cand = None if candidates is None else np.ascontiguousarray(candidates, dtype=np.uint32)
+        nc = 5 if cand is None else cand.size
+        ms = np.zeros(nc, dtype=np.float32)
+        best = _u32()
+        check(lib().anl_msd_autotune(self.handle, skip, tmax, int(bool(cm_msd)), int(bool(cm_self)), n, primo,
+                                     _ptr(cand), 0 if cand is None else nc, _ptr(ms), ctypes.byref(best)))
+        return ms, best.value
+
     def close(self):
         if self.handle:
             lib().anl_traj_destroy(self.handle)
@@ -139,6 +164,19 @@ class DeviceTrajectory:
             self.close()
         except Exception:
             pass
+
+
+def screen_info(box9, rmax):
+    """Screened-form parameters of a constant cell (internal-format box of 9
+    values) computed on the host: dict with valid, threshold, rho_cap, dq,
+    q2_max, threshold_max (see anl_gofrt_screen_info)."""
+    box = np.ascontiguousarray(box9, dtype=np.float64).reshape(9)
+    valid, thr = _i(), _u32()
+    d = [_d() for _ in range(4)]
+    check(lib().anl_gofrt_screen_info(_ptr(box), rmax, ctypes.byref(valid), ctypes.byref(thr),
+                                      *[ctypes.byref(x) for x in d]))
+    return dict(valid=bool(valid.value), threshold=thr.value, rho_cap=d[0].value, dq=d[1].value,
+                q2_max=d[2].value, threshold_max=d[3].value)
 
 
 class DeviceGofrt:
@@ -175,6 +213,14 @@ class DeviceGofrt:
         r = _u64()
         check(lib().anl_gofrt_last_stats(self.handle, ctypes.byref(f), ctypes.byref(c), ctypes.byref(r)))
         return f.value, c.value, r.value
+
+    def pair_stats(self):
+        """Ordered pairs of the last calculate: dict with evaluated (some
+        distance test looked at the pair), all (the reference's count),
+        tile, screen16 and fp64 (evaluated by each test)."""
+        v = [_u64() for _ in range(5)]
+        check(lib().anl_gofrt_pair_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("evaluated", "all", "tile", "screen16", "fp64"), (x.value for x in v)))
 
     def cluster_stats(self):
         """(clustered, all) (wavefront, t0) units of the last screened calculate."""

@@ -125,17 +125,16 @@ static double min_eig_sym3(double G[3][3]) {
 //  (3) |A delta| >= sqrt(lambda_min(A^T A)) |delta|, and rounding each 16-bit
 //      coordinate moves each component of q by at most 1 unit, so
 //      d <= rmax  =>  |q| <= rmax 2^16 / sqrt(lambda_min) + sqrt(3).
-static Screen make_screen(const anl_traj *tr, double rmax) {
+static Screen make_screen(const Cell &c, bool box_const, uint32_t nt, double rmax) {
     Screen s{false, 0, "", 0.5, 0.0, 0.0, 0.0};
-    if (!tr->box_const) {
+    if (!box_const) {
         s.why_not = "the cell changes along the trajectory";
         return s;
     }
-    if (tr->nt > 255) {  // a candidate's type rides in 8 bits
+    if (nt > 255) {  // a candidate's type rides in 8 bits
         s.why_not = "more than 255 atom types";
         return s;
     }
-    const Cell &c = tr->cell0;
     double A[3][3] = {{c.L[0], c.xy, c.xz}, {0, c.L[1], c.yz}, {0, 0, c.L[2]}};
     // inverse of the upper-triangular A
     double Ai[3][3] = {{0}};
@@ -205,11 +204,77 @@ static float du_window(const Screen &s, double rmin, double rmax, double dr, uin
     return (float)(4.0 * du + 1e-6);
 }
 
+// Cluster screen.  A wavefront's 64 i atoms in frame fi get a common centre
+// c (16-bit fractional) and Cartesian offsets p_i = A wrap16(f_i - c); j atoms
+// get q_j = A wrap16(f_j - c).  If rho = max |p_i| satisfies
+//   rho + rmax + dq < w_min (1/2 - 2^-15)            (rho < rho_cap)
+// then for every pair the reference can count (d <= rmax) the vector
+// q_j - p_i is the reference's minimum-image vector up to the 16-bit
+// quantisation (|error| <= dq): the fractional components of
+// (f_j + n - c) stay strictly inside (-1/2, 1/2), so wrapping relative to c
+// picks the reference's image.  The squared distance
+//   |q - p|^2 = |p|^2 + |q|^2 - 2 p.q
+// of 32 x 32 pairs is then one MFMA with rows (-2p, |p|^2, 1) and columns
+// (q, 1, |q|^2) in FP16, accumulated in FP32.  Rounding p, q, |p|^2, |q|^2 to
+// FP16 (2^-11 relative) and the FP32 sum change the result by at most
+// 2^-10 (|p| + |q|)^2, and |q| <= rho + rmax + dq for counted pairs, so
+//   D < ((rmax + dq)^2 + 2^-10 (2 rho + rmax + dq)^2) (1 + 2^-12) + 1e-4
+// holds for every pair the reference counts.  Passing pairs go to the same
+// candidate queue as the 16-bit screen's.
+//
+// FP16 range.  Inactive rows carry |p|^2 = 60000 and columns past N carry
+// |q|^2 = 60000, so that their D >= 60000 never passes; and a counted pair's
+// |q|^2 must be finite in FP16 (< 65504).  Both hold when
+//   (rho + rmax + dq)^2 <= CLUSTER_Q2_MAX = 2^14,
+// since then the threshold is below 2^14 (1 + 2^-8)(1 + 2^-12) + 1e-4, far
+// under the sentinel.  rho_cap enforces it; a cell or rmax that leaves no room
+// turns the cluster screen off (rho_cap = 0) and the 16-bit screen runs.
+constexpr double CLUSTER_Q2_MAX = 16384.0;
+constexpr float CLUSTER_SENTINEL = 60000.0f;
+
+struct ClusterParams {
+    double dq;       // bound of a pair vector's error from 16-bit quantisation of both atoms
+    double rmaxf;    // rmax as the kernel compares it
+    double rho_cap;  // largest usable wavefront radius, 0 = cluster screen off
+};
+
+// dq: each fractional component within 2^-17 + 2^-33 per atom, times
+// |A|_2 <= |A|_F, sqrt(3) components.  rho_cap: rho + rmax + dq below
+// w_min (1/2 - 2^-15) (the image argument) and below sqrt(CLUSTER_Q2_MAX)
+// (the FP16 range), less 1e-3 A for the FP32 evaluation of rho.
+static ClusterParams cluster_params(const Screen &scr, double rmax) {
+    ClusterParams p;
+    p.dq = 2.0 * std::sqrt(3.0) * scr.fro * (std::ldexp(1.0, -17) + std::ldexp(1.0, -33)) * 1.001 * 1.0001;
+    p.rmaxf = std::fabs(rmax) * (1 + 1e-6);
+    const double image_cap = scr.wmin * (0.5 - std::ldexp(1.0, -15)) - p.rmaxf - p.dq - 1e-3;
+    const double range_cap = std::sqrt(CLUSTER_Q2_MAX) - p.rmaxf - p.dq - 1e-3;
+    const double cap = std::min(image_cap, range_cap);
+    p.rho_cap = scr.valid && cap > 0 ? cap : 0.0;
+    return p;
+}
+
 // ---------------------------------------------------------------------------
 // device
 // ---------------------------------------------------------------------------
 
 constexpr int WG = 256;       // threads per workgroup = i atoms per tile
+
+// Launch statistics (device counters, zeroed by every launch).  A pair is
+// "evaluated" when some distance test looks at that very (i, j) pair: the
+// FP64 sequence, the 16-bit screen or an FP16 tile of the cluster screen.
+// Pairs of 32-column halves that the cluster screen skips whole (no j of the
+// half near the wavefront) are pruned, not evaluated.
+enum Stat {
+    STAT_CANDIDATES = 0,  // pairs passed to the FP32 evaluation
+    STAT_RECHECKS,        // candidates re-evaluated in FP64 (near a bin edge, rmin or rmax)
+    STAT_CLUSTERED,       // (wavefront, t0) units that used the cluster screen
+    STAT_UNITS,           // all (wavefront, t0) units of a cluster-screen launch
+    STAT_PAIRS_TILE,      // ordered pairs evaluated by FP16 tiles
+    STAT_PAIRS_16,        // ordered pairs evaluated by the 16-bit screen
+    STAT_PAIRS_FP64,      // ordered pairs evaluated by the plain FP64 kernel
+    STAT_PAIRS_ALL,       // all ordered pairs (i active, any j) of the launch
+    NSTAT
+};
 constexpr int QCAP = 192;     // per-wavefront candidate queue entries
 
 struct BlockInfo {
@@ -231,7 +296,7 @@ struct GofrtArgs {
     const int32_t *type_slot;
     const int32_t *orig_slot;
     unsigned long long *out;     // [nblocks][leff_max][nch][nbin]
-    unsigned long long *ncand;   // screened: candidate count
+    unsigned long long *ncand;   // [NSTAT] launch statistics, see the Stat enum
     const double *thr;           // [nbin+1]
     const int32_t *chan;         // [nt*nt]
     const WorkItem *work;
@@ -330,6 +395,8 @@ __global__ __launch_bounds__(WG) void gofrt_fp64_kernel(const GofrtArgs a) {
         ti = a.type_slot[slot];
     }
     const int nbin = (int)a.nbin;
+    // every ordered pair of an active i is evaluated: N per t0
+    const unsigned long long npairs = (unsigned long long)__popcll(__ballot(active)) * N * (w.k1 - w.k0);
     for (uint32_t k = w.k0; k < w.k1; k++) {
         const uint64_t fi = b.primo + (uint64_t)k * a.skip;
         const uint64_t fj = fi + t;
@@ -348,6 +415,10 @@ __global__ __launch_bounds__(WG) void gofrt_fp64_kernel(const GofrtArgs a) {
                 }
             }
         }
+    }
+    if ((threadIdx.x & 63u) == 0 && a.ncand && npairs) {
+        atomicAdd(a.ncand + STAT_PAIRS_FP64, npairs);
+        atomicAdd(a.ncand + STAT_PAIRS_ALL, npairs);
     }
     __syncthreads();
     lds_flush(a, hist, w.block, t);
@@ -470,24 +541,9 @@ __device__ __forceinline__ float wave_max_f(float v) {
     return v;
 }
 
-// Cluster screen.  A wavefront's 64 i atoms in frame fi get a common centre
-// c (16-bit fractional) and Cartesian offsets p_i = A wrap16(f_i - c); j atoms
-// get q_j = A wrap16(f_j - c).  If rho = max |p_i| satisfies
-//   rho + rmax + dq < w_min (1/2 - 2^-15)            (rho < rho_cap)
-// then for every pair the reference can count (d <= rmax) the vector
-// q_j - p_i is the reference's minimum-image vector up to the 16-bit
-// quantisation (|error| <= dq): the fractional components of
-// (f_j + n - c) stay strictly inside (-1/2, 1/2), so wrapping relative to c
-// picks the reference's image.  The squared distance
-//   |q - p|^2 = |p|^2 + |q|^2 - 2 p.q
-// of 32 x 32 pairs is then one MFMA with rows (-2p, |p|^2, 1) and columns
-// (q, 1, |q|^2) in FP16, accumulated in FP32.  Rounding p, q, |p|^2, |q|^2 to
-// FP16 (2^-11 relative) and the FP32 sum change the result by at most
-// 2^-10 (|p| + |q|)^2, and |q| <= rho + rmax + dq for counted pairs, so
-//   D < ((rmax + dq)^2 + 2^-10 (2 rho + rmax + dq)^2) (1 + 2^-12) + 1e-4
-// holds for every pair the reference counts.  Passing pairs go to the same
-// candidate queue as the 16-bit screen's.
-__device__ __forceinline__ float cluster_threshold(float rho, float rmax, float dq) {
+// The cluster screen's distance threshold for a wavefront of radius rho (see
+// the kernel); shared by the kernel and the host-side range check below.
+__host__ __device__ __forceinline__ float cluster_threshold(float rho, float rmax, float dq) {
     const float a = rmax + dq, b = 2.0f * rho + rmax + dq;
     return (a * a + b * b * 0x1p-10f) * (1.0f + 0x1p-12f) + 1e-4f;
 }
@@ -536,6 +592,9 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     constexpr int UR = UNROLL;
     const uint32_t NU = (uint32_t)(N - N % UR);
     unsigned long long ncand = 0, nre = 0, ncl = 0, nwt = 0;
+    // wavefront-uniform pair counts: all ordered pairs of the active i atoms,
+    // pairs evaluated by the FP16 tile screen and by the 16-bit screen
+    unsigned long long nall = 0, ntile = 0, n16 = 0;
 
     for (uint32_t k = w.k0; k < w.k1; k++) {
         const uint64_t fi = b.primo + (uint64_t)k * a.skip;
@@ -564,6 +623,8 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         // strict comparison against threshold+1; inactive lanes compare
         // against 0 and never pass
         const uint32_t th = active ? a.threshold + 1u : 0u;
+        const unsigned long long nact = (unsigned long long)__popcll(__ballot(active));
+        nall += nact * N;
         __syncthreads();
         uint32_t qn = 0, rn = 0;
         // evaluate the queued entry of this lane; candidates inside the edge
@@ -621,7 +682,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     // rounding of |q|^2 and rho
                     const float rs = (rho + a.rmaxf + a.dq) * (1.0f + 0x1p-12f) + 1e-3f;
                     const float rs2 = rs * rs;
-                    // rows: (-2p, |p|^2, 1) for active lanes, (0, 0, 0, 60000, 1) otherwise
+                    // rows: (-2p, |p|^2, 1) for active lanes, (0, 0, 0, sentinel, 1) otherwise
                     f16x8_t row = {};
                     if (active) {
                         row[0] = (_Float16)(-2.0f * px);
@@ -629,7 +690,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         row[2] = (_Float16)(-2.0f * pz);
                         row[3] = (_Float16)p2;
                     } else {
-                        row[3] = (_Float16)60000.0f;
+                        row[3] = (_Float16)CLUSTER_SENTINEL;
                     }
                     row[4] = (_Float16)1.0f;
                     // A of i rows 0..31 (lanes 0-31 own them) and 32..63 (moved
@@ -675,13 +736,13 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         // order, so a half is a compact group of atoms too)
                         const unsigned long long nearm = __ballot(jin && q2 <= rs2);
                         if (nearm == 0ull) continue;
-                        // column (q, 1, |q|^2); past N: (0, 0, 0, 1, 60000)
+                        // column (q, 1, |q|^2); past N: (0, 0, 0, 1, sentinel)
                         f16x8_t col = {};
                         col[0] = (_Float16)(jin ? qx : 0.0f);
                         col[1] = (_Float16)(jin ? qy : 0.0f);
                         col[2] = (_Float16)(jin ? qz : 0.0f);
                         col[3] = (_Float16)1.0f;
-                        col[4] = (_Float16)(jin ? q2 : 60000.0f);
+                        col[4] = (_Float16)(jin ? q2 : CLUSTER_SENTINEL);
                         const uint4 cw = __builtin_bit_cast(uint4, col);
                         uint4 cdn;
                         cdn.x = __shfl(cw.x, (int)(lane ^ 32u), 64);
@@ -694,6 +755,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
 #pragma unroll
                         for (int jb = 0; jb < 2; jb++) {
                             if ((uint32_t)(nearm >> (32 * jb)) == 0u) continue;  // uniform
+                            ntile += nact * (unsigned long long)min(32u, (uint32_t)N - (j0 + 32u * (uint32_t)jb));
                             // entry = j << 8 | local i; item e of tile T is row
                             // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
                             const uint32_t eb = ((j0 + 32u * (uint32_t)jb + (lane & 31u)) << 8) | (wbase + half4);
@@ -742,6 +804,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
             }
         }
         if (!clustered) {
+            n16 += nact * N;
             for (uint32_t j = 0; j < NU; j += UR) {
                 unsigned long long g[UR];
                 bool p[UR];
@@ -799,12 +862,15 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         nre += __shfl_down(nre, off, 64);
     }
     if (lane == 0 && a.ncand) {
-        atomicAdd(a.ncand, ncand);
-        atomicAdd(a.nrecheck, nre);
+        atomicAdd(a.ncand + STAT_CANDIDATES, ncand);
+        atomicAdd(a.ncand + STAT_RECHECKS, nre);
         if (CLUSTER) {
-            atomicAdd(a.ncand + 2, ncl);
-            atomicAdd(a.ncand + 3, nwt);
+            atomicAdd(a.ncand + STAT_CLUSTERED, ncl);
+            atomicAdd(a.ncand + STAT_UNITS, nwt);
         }
+        if (nall) atomicAdd(a.ncand + STAT_PAIRS_ALL, nall);
+        if (ntile) atomicAdd(a.ncand + STAT_PAIRS_TILE, ntile);
+        if (n16) atomicAdd(a.ncand + STAT_PAIRS_16, n16);
     }
     __syncthreads();
     lds_flush(a, hist, w.block, t);
@@ -839,7 +905,7 @@ struct anl_gofrt {
     double lo = 0;
     double *d_thr = nullptr;
     int32_t *d_chan = nullptr;
-    unsigned long long *d_ncand = nullptr;   // [4]: candidates, FP64 rechecks, clustered and all wavefront-t0 units
+    unsigned long long *d_ncand = nullptr;   // [NSTAT] statistics of the last launch (Stat)
     int last_form = -1;
     // scratch for the work list
     BlockInfo *d_blocks = nullptr;
@@ -854,7 +920,7 @@ uint32_t nch_of(const anl_traj *tr) { return tr->nt * (tr->nt + 1); }
 int choose_form(anl_gofrt *g) {
     if (g->form == ANL_GOFRT_FP64) return ANL_GOFRT_FP64;
     if (g->tr->screen_rmax != g->rmax) {
-        g->tr->screen_cache = make_screen(g->tr, g->rmax);
+        g->tr->screen_cache = make_screen(g->tr->cell0, g->tr->box_const, g->tr->nt, g->rmax);
         g->tr->screen_rmax = g->rmax;
     }
     if (g->tr->screen_cache.valid) return ANL_GOFRT_SCREENED;
@@ -1022,7 +1088,7 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
     a.a0 = a0;
     a.a1 = a1;
     a.frac32 = tr->frac32;
-    a.nrecheck = g->d_ncand + 1;
+    a.nrecheck = g->d_ncand + STAT_RECHECKS;
     {
         const double sc = std::ldexp(1.0, -32);
         const Cell &cl = tr->cell0;
@@ -1033,11 +1099,8 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         a.cyz = (float)(cl.yz * sc);
         a.cz = (float)(cl.L[2] * sc);
         a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->rmax, g->dr, g->nbin) : 0.0f;
-        // cluster screen: cell entries times 2^-16; dq bounds the error of a
-        // pair vector from 16-bit quantisation of both atoms (each fractional
-        // component within 2^-17 + 2^-33, times |A|_2 <= |A|_F, sqrt(3)
-        // components); rho_cap from rho + rmax + dq < w_min (1/2 - 2^-15),
-        // less 1e-3 A for the FP32 evaluation of rho
+        // cluster screen: cell entries times 2^-16, quantisation bound,
+        // largest usable wavefront radius (cluster_params)
         const double sh = std::ldexp(1.0, -16);
         a.hx = (float)(cl.L[0] * sh);
         a.hxy = (float)(cl.xy * sh);
@@ -1045,12 +1108,10 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         a.hy = (float)(cl.L[1] * sh);
         a.hyz = (float)(cl.yz * sh);
         a.hz = (float)(cl.L[2] * sh);
-        const Screen &scr = tr->screen_cache;
-        const double dq = 2.0 * std::sqrt(3.0) * scr.fro * (std::ldexp(1.0, -17) + std::ldexp(1.0, -33)) * 1.001;
-        a.dq = (float)(dq * 1.0001);
-        a.rmaxf = (float)(std::fabs(g->rmax) * (1 + 1e-6));
-        const double cap = scr.wmin * (0.5 - std::ldexp(1.0, -15)) - std::fabs(g->rmax) - dq - 1e-3;
-        a.rho_cap = form == ANL_GOFRT_SCREENED && cap > 0 ? (float)cap : 0.0f;
+        const ClusterParams cp = cluster_params(tr->screen_cache, g->rmax);
+        a.dq = (float)cp.dq;
+        a.rmaxf = (float)cp.rmaxf;
+        a.rho_cap = form == ANL_GOFRT_SCREENED ? (float)cp.rho_cap : 0.0f;
     }
     const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
     const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;
@@ -1087,7 +1148,7 @@ void gofrt_enqueue(anl_gofrt *g, const GofrtPrepared &P, unsigned long long *out
     GofrtArgs a = P.a;
     a.out = out;
     zero_u64(out, P.out_len, s);
-    zero_u64(g->d_ncand, 4, s);
+    zero_u64(g->d_ncand, NSTAT, s);
     if (P.grid.x == 0) return;
     if (P.form == ANL_GOFRT_FP64) {
         if (P.triclinic) hipLaunchKernelGGL(gofrt_fp64_kernel<true>, P.grid, dim3(WG), P.lds, s, a);
@@ -1191,7 +1252,7 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
             }
         ANL_HIP(hipMalloc(&g->d_chan, chan.size() * sizeof(int32_t)));
         ANL_HIP(hipMemcpy(g->d_chan, chan.data(), chan.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        ANL_HIP(hipMalloc(&g->d_ncand, 4 * sizeof(unsigned long long)));
+        ANL_HIP(hipMalloc(&g->d_ncand, NSTAT * sizeof(unsigned long long)));
         *out = g;
     });
 }
@@ -1310,6 +1371,41 @@ int anl_gofrt_cluster_stats(anl_gofrt *g, uint64_t *clustered, uint64_t *units) 
         ANL_HIP(hipMemcpy(c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
         if (clustered) *clustered = c[2];
         if (units) *units = c[3];
+    });
+}
+
+int anl_gofrt_pair_stats(anl_gofrt *g, uint64_t *evaluated, uint64_t *all_pairs, uint64_t *tile, uint64_t *screen16,
+                         uint64_t *fp64) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        unsigned long long c[NSTAT] = {};
+        ANL_HIP(hipMemcpy(c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
+        if (evaluated) *evaluated = c[STAT_PAIRS_TILE] + c[STAT_PAIRS_16] + c[STAT_PAIRS_FP64];
+        if (all_pairs) *all_pairs = c[STAT_PAIRS_ALL];
+        if (tile) *tile = c[STAT_PAIRS_TILE];
+        if (screen16) *screen16 = c[STAT_PAIRS_16];
+        if (fp64) *fp64 = c[STAT_PAIRS_FP64];
+    });
+}
+
+int anl_gofrt_screen_info(const double *box9, double rmax, int *valid, uint32_t *threshold, double *rho_cap,
+                          double *dq, double *q2_max, double *threshold_max) {
+    return guard([&] {
+        if (!box9) fail(ANL_ERR_ARG, "null box");
+        const Cell c = cell_of(box9);
+        const Screen scr = make_screen(c, true, 1, rmax);
+        const ClusterParams cp = cluster_params(scr, rmax);
+        if (valid) *valid = scr.valid ? 1 : 0;
+        if (threshold) *threshold = scr.valid ? scr.threshold : 0;
+        if (rho_cap) *rho_cap = cp.rho_cap;
+        if (dq) *dq = cp.dq;
+        // what the kernel compares for the widest wavefront it clusters: the
+        // largest |q|^2 of a pair it must pass, and the distance threshold
+        const float rho = (float)cp.rho_cap;
+        const float rs = (rho + (float)cp.rmaxf + (float)cp.dq) * (1.0f + 0x1p-12f) + 1e-3f;
+        if (q2_max) *q2_max = cp.rho_cap > 0 ? (double)rs * (double)rs : 0.0;
+        if (threshold_max)
+            *threshold_max = cp.rho_cap > 0 ? (double)cluster_threshold(rho, (float)cp.rmaxf, (float)cp.dq) : 0.0;
     });
 }
 

@@ -103,6 +103,28 @@ int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uin
 /* Of the last screened calculate: how many (wavefront, t0) units used the
  * cluster (MFMA) screen, out of how many (0 of 0 when it was off). */
 int anl_gofrt_cluster_stats(anl_gofrt *g, uint64_t *clustered, uint64_t *units);
+/* Of the last calculate: ordered pairs whose distance some test looked at
+ * (evaluated = tile + screen16 + fp64: FP16 MFMA tiles of the cluster
+ * screen, the 16-bit screen, the plain FP64 form), and all ordered pairs of
+ * the calculation (the reference evaluates every one, calculatemultithread.h:
+ * 114-139 around gofrt.cpp:101-104).  all_pairs - evaluated were pruned: the
+ * cluster screen skips a 32-column half whose atoms are all farther than
+ * rmax from the wavefront's bounding sphere.  Diagnostics of this
+ * implementation; no counterpart in the reference. */
+int anl_gofrt_pair_stats(anl_gofrt *g, uint64_t *evaluated, uint64_t *all_pairs, uint64_t *tile, uint64_t *screen16,
+                         uint64_t *fp64);
+/* Screen parameters for a constant cell, on the host (no device needed):
+ * box9 is one frame's internal-format box [lo(3), half lengths(3), xy, xz,
+ * yz].  valid: the 16-bit screen applies (rmax below half of every cell
+ * width); threshold: its bound on |q|^2.  rho_cap: the largest wavefront
+ * radius the cluster (FP16 MFMA) screen accepts, 0 when it is off; dq: the
+ * 16-bit quantisation bound of a pair vector; q2_max and threshold_max: the
+ * largest |q|^2 of a pair the cluster screen must pass and its largest
+ * distance threshold, which stay far below both the FP16 maximum (65504) and
+ * the sentinel 60000 of inactive tile rows and columns.  No counterpart in
+ * the reference. */
+int anl_gofrt_screen_info(const double *box9, double rmax, int *valid, uint32_t *threshold, double *rho_cap,
+                          double *dq, double *q2_max, double *threshold_max);
 /* Launch shape of the pair kernels: unroll of the 16-bit screen loop (4 or
  * 8), XCD-aware workgroup order (0/1), pair evaluations per workgroup (sets
  * the t0 chunk of one workgroup), and the cluster screen (0/1: wavefronts
@@ -196,6 +218,21 @@ int anl_msd_calculate(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, in
                       uint32_t n, uint64_t primo, double *out_host);
 int anl_msd_calculate_async(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self,
                             uint32_t n, uint64_t primo, double *out_dev, void *stream);
+/* Launch shape of the MSD kernel for this trajectory: the lag tile R (8, 12,
+ * 16 or 24; default 16) of the t0-stride-1 kernel, which holds R frames of
+ * an atom in registers and loads two frames per t0 for R updates; 0 selects
+ * the eight-lags-per-thread kernel that every other stride uses.  Tiles 8
+ * to 24 give identical results (each (atom, lag) sum runs over t0 in order,
+ * the same operations); tile 0 agrees with them to rounding (both within
+ * 1e-10 of the reference).  anl_msd_autotune times each candidate tile
+ * (cand[c]; NULL = 8, 12, 16, 24, 0) on the calculation reset(n) at primo
+ * and keeps the fastest; ms_out[c] (may be NULL) receives the times.  The
+ * reference's equivalent knob is nthreads (msd.h:30-47,
+ * calculatemultithread.h:80-104); no counterpart for the tile. */
+int anl_msd_set_tuning(anl_traj *tr, uint32_t lag_tile);
+int anl_msd_get_tuning(anl_traj *tr, uint32_t *lag_tile);
+int anl_msd_autotune(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self, uint32_t n, uint64_t primo,
+                     const uint32_t *cand, uint32_t ncand, float *ms_out, uint32_t *best_index);
 
 /* ---- time series and Green-Kubo --------------------------------------- */
 /* Device mirror of ReadLog_numpy (readlog_numpy.h:6-26): row-major

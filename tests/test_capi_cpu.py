@@ -33,3 +33,41 @@ def test_no_device_is_an_error_not_a_fallback():
     rc = capi.lib().anl_init(0)
     assert rc == capi.ANL_ERR_NO_DEVICE
     assert "gfx950" in capi.lib().anl_last_error().decode()
+
+
+def _box9(L, tilt=(0.0, 0.0, 0.0)):
+    """Internal-format box of a cell with edge lengths L (origin 0)."""
+    return [0.0, 0.0, 0.0, L / 2, L / 2, L / 2, tilt[0], tilt[1], tilt[2]]
+
+
+def test_cluster_screen_stays_in_fp16_range():
+    """The cluster screen's tiles hold |p|^2 and |q|^2 in FP16 and mark
+    inactive rows and columns past N with 60000: a cell wide enough for a
+    large rmax must turn the cluster screen off instead of letting the
+    threshold reach the sentinel (rmax ~ 250 A in a 600 A cell gives a
+    threshold ~ 62,500 without the range cap) or |q|^2 overflow FP16."""
+    from analisi_amd import capi
+    # the bench's 90 A cubic cell, rmax 15: cluster screen on, far inside range
+    s = capi.screen_info(_box9(90.0), 15.0)
+    assert s["valid"] and s["rho_cap"] > 20.0
+    assert s["q2_max"] < 16400 and s["threshold_max"] < 16500
+    # 600 A cell, rmax 250: the 16-bit screen is valid, the cluster screen is off
+    s = capi.screen_info(_box9(600.0), 250.0)
+    assert s["valid"] and s["rho_cap"] == 0.0 and s["threshold_max"] == 0.0
+    # 520 A cell, rmax 100: only the FP16 range limits rho (128 - 100 - dq)
+    s = capi.screen_info(_box9(520.0), 100.0)
+    assert s["valid"] and 27.0 < s["rho_cap"] < 28.0
+    # rmax at half the cell width: no screen at all
+    s = capi.screen_info(_box9(20.0), 10.0)
+    assert not s["valid"] and s["rho_cap"] == 0.0
+    # sweep of cubic and triclinic cells: whenever the cluster screen is on,
+    # the largest tile value and threshold stay far below 65504 and 60000
+    for L in (12.0, 30.0, 56.0, 90.0, 200.0, 400.0, 700.0, 1500.0):
+        for frac in (0.05, 0.2, 0.35, 0.45, 0.49):
+            for tilt in ((0.0, 0.0, 0.0), (0.1 * L, -0.05 * L, 0.08 * L)):
+                s = capi.screen_info(_box9(L, tilt), frac * L)
+                if s["rho_cap"] > 0:
+                    assert s["valid"]
+                    assert s["rho_cap"] + frac * L + s["dq"] <= 128.0
+                    assert s["q2_max"] <= 16400.0, (L, frac, tilt, s)
+                    assert s["threshold_max"] <= 16500.0, (L, frac, tilt, s)
