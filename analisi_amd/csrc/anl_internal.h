@@ -89,6 +89,7 @@ struct anl_traj {
     anl::Cell cell0;                    // cell of frame 0
     anl::Screen screen_cache;
     double screen_rmax = -1.0;
+    uint32_t msd_tile = 16;             // lag tile of the MSD window kernel (anl_msd_set_tuning)
 };
 
 struct anl_series {

@@ -5,13 +5,22 @@
 // half vdata[t][nt + type] = mean over t0 of |cm(t0) - cm(t0+t)|^2.
 //
 // The atom part is a sum of non-negative terms: each thread accumulates one
-// atom over all t0 for a block of lags (FP64), a workgroup reduces its 256
-// atoms per type in slot order, and a second kernel adds the workgroup
-// partials in a fixed order and divides by the term count -- deterministic,
-// and within a few ulps of the reference's running mean (no cancellation).
-// The centre-of-mass channel replays the reference's running mean exactly.
-// Several blocks (primo_b, n_b) run in one launch (grid z = block).
+// atom over all t0 for a tile of lags (FP64), the workgroup sums its atoms per
+// type in slot order, and a second kernel adds the chunk partials in a fixed
+// order and divides by the term count -- deterministic, and within a few ulps
+// of the reference's running mean (no cancellation).  The centre-of-mass
+// channel replays the reference's running mean exactly.  Several blocks
+// (primo_b, n_b) run in one launch.
+//
+// t0 stride 1 (the usual case) runs msd_window_kernel: a thread holds the R
+// frames x(t0 + tb .. t0 + tb + R - 1) of its atom in registers; moving to t0+1
+// drops the first and loads one more, so each t0 costs two loads (x(t0) and
+// the new window frame) for R updates.  The t0 loop is unrolled R times so the
+// window rotates through fixed registers.  R (the lag tile) is tunable.
+// Other strides run msd_atoms_kernel (eight lags per thread, every frame
+// loaded).
 #include <algorithm>
+#include <type_traits>
 
 #include "anl_internal.h"
 
@@ -20,7 +29,17 @@
 namespace anl {
 
 constexpr int MSD_WG = 256;
-constexpr int MSD_TL = 8;  // lags per thread
+
+// f(std::integral_constant<int, 0>) .. f(std::integral_constant<int, N-1>),
+// unrolled at compile time.
+template <int N, int I = 0, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+constexpr int MSD_TL = 8;  // lags per thread of msd_atoms_kernel
 
 struct MsdBlock {
     uint64_t primo;
@@ -90,6 +109,99 @@ __global__ __launch_bounds__(MSD_WG) void msd_atoms_kernel(const double4 *__rest
     }
 }
 
+// t0 stride 1.  Workgroup = 64 atom slots (the lanes) x 4 wavefronts, each
+// wavefront a tile of R consecutive lags: lags (4 quad + wave) R .. + R - 1.
+// The 1-D grid is decoded so that the lag tiles of one atom chunk are
+// consecutive logical ids on one XCD (hardware id h runs on XCD h % 8), where
+// they share the x(t0) stream in L2.  y(f) = x(f) [- cm(f, type)]: with
+// drift removal the centre of mass is subtracted as each frame is loaded.
+template <int R, bool CMSELF>
+__global__ __launch_bounds__(MSD_WG) void msd_window_kernel(const double4 *__restrict__ pos,
+                                                           const double *__restrict__ cm,
+                                                           const int32_t *__restrict__ type_slot, uint64_t N,
+                                                           uint32_t nt, const MsdBlock *__restrict__ blocks,
+                                                           uint32_t leff, uint32_t nchunks, uint32_t nquads,
+                                                           uint32_t per_xcd, uint32_t total,
+                                                           double *__restrict__ partial) {
+    __shared__ double red[4][R][64];
+    __shared__ int type_lds[64];
+    const uint32_t h = blockIdx.x;
+    const uint32_t L = (h & 7u) * per_xcd + (h >> 3);
+    if (L >= total) return;  // whole workgroup: uniform
+    const uint32_t quad = L % nquads;
+    const uint32_t rest = L / nquads;
+    const uint32_t chunk = rest % nchunks;
+    const uint32_t blk = rest / nchunks;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t tb = (quad * 4u + wave) * (uint32_t)R;
+    const uint64_t slot = (uint64_t)chunk * 64u + lane;
+    const uint64_t sl = slot < N ? slot : N - 1;  // lanes past N load a real atom and are not summed
+    const MsdBlock b = blocks[blk];
+    const int ti = type_slot[sl];
+    // the last frame a counted (t0, lag) reads; window frames of lags past
+    // leff are clamped to it (their sums are never used)
+    const uint64_t flast = b.primo + (uint64_t)b.n - 1u + (uint64_t)leff - 1u;
+    auto load = [&](uint64_t f) -> double3 {
+        f = f < flast ? f : flast;
+        const double4 x = pos[f * N + sl];
+        double3 y = make_double3(x.x, x.y, x.z);
+        if (CMSELF) {
+            const double *c = cm + (f * nt + (uint32_t)ti) * 3u;
+            y.x = y.x - c[0];
+            y.y = y.y - c[1];
+            y.z = y.z - c[2];
+        }
+        return y;
+    };
+    double acc[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) acc[k] = 0.0;
+    if (tb < leff) {  // uniform per wavefront
+        double3 W[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) W[k] = load(b.primo + tb + (uint32_t)k);
+        // step s of a pass starting at t0: W[(s + k) % R] holds frame
+        // t0 + s + tb + k; afterwards W[s] is refilled with frame
+        // t0 + s + tb + R.  Every index is a compile-time constant once the
+        // pass is unrolled, so the window stays in registers.
+        auto step = [&](uint32_t t0, auto sc) {
+            constexpr int s = decltype(sc)::value;
+            const double3 ya = load(b.primo + t0 + (uint32_t)s);
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const double3 wv = W[(s + k) % R];
+                const double e0 = ya.x - wv.x, e1 = ya.y - wv.y, e2 = ya.z - wv.z;
+                acc[k] = fma(e0, e0, acc[k]);
+                acc[k] = fma(e1, e1, acc[k]);
+                acc[k] = fma(e2, e2, acc[k]);
+            }
+            W[s] = load(b.primo + t0 + (uint32_t)s + tb + (uint32_t)R);
+        };
+        uint32_t t0 = 0;
+        for (; t0 + R <= b.n; t0 += R)
+            static_for<R>([&](auto sc) { step(t0, sc); });
+        // last partial pass (uniform condition per step)
+        static_for<R>([&](auto sc) {
+            if (t0 + (uint32_t)decltype(sc)::value < b.n) step(t0, sc);
+        });
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) red[wave][k][lane] = acc[k];
+    if (threadIdx.x < 64) type_lds[lane] = slot < N ? ti : -1;
+    __syncthreads();
+    // thread (wave, lag, type) sums the 64 slots in order
+    for (uint32_t w = threadIdx.x; w < 4u * (uint32_t)R * nt; w += MSD_WG) {
+        const uint32_t wv = w / ((uint32_t)R * nt), r = w % ((uint32_t)R * nt);
+        const uint32_t k = r / nt, tt = r % nt;
+        const uint32_t t = (quad * 4u + wv) * (uint32_t)R + k;
+        if (t >= leff) continue;
+        double sum = 0.0;
+        for (int j = 0; j < 64; j++)
+            if (type_lds[j] == (int)tt) sum += red[wv][k][j];
+        partial[(((uint64_t)blk * leff + t) * nchunks + chunk) * nt + tt] = sum;
+    }
+}
+
 __global__ void msd_finish_kernel(const double *__restrict__ partial, uint32_t nchunks, uint32_t nt, uint32_t leff,
                                   uint32_t nblocks, const uint64_t *__restrict__ count, uint32_t fcm,
                                   double *__restrict__ out) {
@@ -126,6 +238,7 @@ __global__ void msd_cm_kernel(const double *__restrict__ cm, uint32_t nt, const 
 struct MsdPrepared {
     anl_traj *tr = nullptr;
     uint32_t skip = 1, leff = 0, nblocks = 0, nt = 0, fcm = 1, nchunks = 0;
+    uint32_t tile = 0;  // lag tile R of msd_window_kernel; 0: msd_atoms_kernel
     bool cm_msd = false, cm_self = false;
     MsdBlock *d_blocks = nullptr;
     uint64_t *d_count = nullptr;
@@ -167,7 +280,9 @@ void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n,
         P->fcm = cm_msd ? 2 : 1;
         P->cm_msd = cm_msd;
         P->cm_self = cm_self;
-        P->nchunks = (uint32_t)((tr->N + MSD_WG - 1) / MSD_WG);
+        // the window kernel needs t0 stride 1; its chunks are 64 atoms
+        P->tile = skip == 1 ? tr->msd_tile : 0;
+        P->nchunks = (uint32_t)(P->tile ? (tr->N + 63) / 64 : (tr->N + MSD_WG - 1) / MSD_WG);
         std::vector<MsdBlock> hb(nblocks);
         std::vector<uint64_t> hc((size_t)nblocks * tr->nt);
         for (uint32_t b = 0; b < nblocks; b++) {
@@ -193,14 +308,37 @@ void msd_enqueue_prepared(void *prep, double *out, hipStream_t s) {
     const MsdPrepared &P = *(const MsdPrepared *)prep;
     if (P.leff == 0) return;
     const anl_traj *tr = P.tr;
-    const uint32_t nlb = (P.leff + MSD_TL - 1) / MSD_TL;
-    dim3 grid(P.nchunks, nlb, P.nblocks);
-    if (P.cm_self)
-        hipLaunchKernelGGL(msd_atoms_kernel<true>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N,
-                           P.nt, P.d_blocks, P.skip, P.leff, P.nchunks, P.d_partial);
-    else
-        hipLaunchKernelGGL(msd_atoms_kernel<false>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N,
-                           P.nt, P.d_blocks, P.skip, P.leff, P.nchunks, P.d_partial);
+    if (P.tile) {
+        const uint32_t nquads = (P.leff + 4 * P.tile - 1) / (4 * P.tile);
+        const uint64_t total = (uint64_t)nquads * P.nchunks * P.nblocks;
+        const uint64_t per = (total + 7) / 8;
+        if (8 * per >= (1ull << 32)) fail(ANL_ERR_ARG, "MSD launch too large (reduce the number of blocks)");
+        const dim3 grid((uint32_t)(8 * per));
+#define ANL_MSD_WIN(R, C)                                                                                      \
+    hipLaunchKernelGGL((msd_window_kernel<R, C>), grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N, \
+                       P.nt, P.d_blocks, P.leff, P.nchunks, nquads, (uint32_t)per, (uint32_t)total, P.d_partial)
+        switch (P.tile * 2 + (P.cm_self ? 1 : 0)) {
+            case 16: ANL_MSD_WIN(8, false); break;
+            case 17: ANL_MSD_WIN(8, true); break;
+            case 24: ANL_MSD_WIN(12, false); break;
+            case 25: ANL_MSD_WIN(12, true); break;
+            case 32: ANL_MSD_WIN(16, false); break;
+            case 33: ANL_MSD_WIN(16, true); break;
+            case 48: ANL_MSD_WIN(24, false); break;
+            case 49: ANL_MSD_WIN(24, true); break;
+            default: fail(ANL_ERR_ARG, "MSD lag tile must be 0, 8, 12, 16 or 24");
+        }
+#undef ANL_MSD_WIN
+    } else {
+        const uint32_t nlb = (P.leff + MSD_TL - 1) / MSD_TL;
+        dim3 grid(P.nchunks, nlb, P.nblocks);
+        if (P.cm_self)
+            hipLaunchKernelGGL(msd_atoms_kernel<true>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N,
+                               P.nt, P.d_blocks, P.skip, P.leff, P.nchunks, P.d_partial);
+        else
+            hipLaunchKernelGGL(msd_atoms_kernel<false>, grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot,
+                               tr->N, P.nt, P.d_blocks, P.skip, P.leff, P.nchunks, P.d_partial);
+    }
     ANL_HIP(hipGetLastError());
     const uint32_t nw = P.nblocks * P.leff * P.nt;
     hipLaunchKernelGGL(msd_finish_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, P.d_partial, P.nchunks, P.nt,
@@ -255,6 +393,87 @@ int anl_msd_calculate(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, in
         hipError_t e = hipMemcpy(out_host, d, len * sizeof(double), hipMemcpyDeviceToHost);
         (void)hipFree(d);
         ANL_HIP(e);
+    });
+}
+
+int anl_msd_set_tuning(anl_traj *tr, uint32_t lag_tile) {
+    return guard([&] {
+        if (!tr) fail(ANL_ERR_ARG, "null trajectory");
+        if (lag_tile != 0 && lag_tile != 8 && lag_tile != 12 && lag_tile != 16 && lag_tile != 24)
+            fail(ANL_ERR_ARG, "MSD lag tile must be 0, 8, 12, 16 or 24");
+        tr->msd_tile = lag_tile;
+    });
+}
+
+int anl_msd_get_tuning(anl_traj *tr, uint32_t *lag_tile) {
+    return guard([&] {
+        if (!tr || !lag_tile) fail(ANL_ERR_ARG, "null pointer passed to anl_msd_get_tuning");
+        *lag_tile = tr->msd_tile;
+    });
+}
+
+// Times the MSD calculation (reset(n) at primo) with each candidate lag tile:
+// one untimed and one timed call each (HIP events on the null stream); keeps
+// the fastest in the trajectory.
+int anl_msd_autotune(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self, uint32_t n, uint64_t primo,
+                     const uint32_t *cand, uint32_t ncand, float *ms_out, uint32_t *best_out) {
+    return guard([&] {
+        if (!tr) fail(ANL_ERR_ARG, "null trajectory");
+        static const uint32_t defaults[] = {8, 12, 16, 24, 0};
+        if (!cand) {
+            cand = defaults;
+            ncand = sizeof(defaults) / sizeof(defaults[0]);
+        }
+        if (ncand == 0) fail(ANL_ERR_ARG, "no candidates");
+        for (uint32_t c = 0; c < ncand; c++)
+            if (cand[c] != 0 && cand[c] != 8 && cand[c] != 12 && cand[c] != 16 && cand[c] != 24)
+                fail(ANL_ERR_ARG, "candidate " + std::to_string(c) + ": MSD lag tile must be 0, 8, 12, 16 or 24");
+        ensure_device();
+        const uint32_t saved = tr->msd_tile;
+        const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;
+        double *d = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        void *P = nullptr;
+        float best = -1.0f;
+        uint32_t besti = 0;
+        try {
+            ANL_HIP(hipMalloc(&d, std::max<size_t>(1, (size_t)leff * tr->nt * 2) * sizeof(double)));
+            ANL_HIP(hipEventCreate(&e0));
+            ANL_HIP(hipEventCreate(&e1));
+            for (uint32_t c = 0; c < ncand; c++) {
+                tr->msd_tile = cand[c];
+                P = msd_prepare_blocks(tr, &primo, &n, 1, skip, tmax, cm_msd, cm_self, nullptr);
+                msd_enqueue_prepared(P, d, 0);
+                ANL_HIP(hipEventRecord(e0, 0));
+                msd_enqueue_prepared(P, d, 0);
+                ANL_HIP(hipEventRecord(e1, 0));
+                ANL_HIP(hipEventSynchronize(e1));
+                msd_free_prepared(P);
+                P = nullptr;
+                float ms = 0.0f;
+                ANL_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms_out) ms_out[c] = ms;
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    besti = c;
+                }
+            }
+        } catch (...) {
+            tr->msd_tile = saved;
+            if (P) {
+                (void)hipDeviceSynchronize();
+                msd_free_prepared(P);
+            }
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(d);
+        tr->msd_tile = cand[besti];
+        if (best_out) *best_out = besti;
     });
 }
 
