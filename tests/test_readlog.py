@@ -89,3 +89,26 @@ def test_traj_charge_current(tmp_path):
     bad = pa.ReadLog_file(lp, t, 1, 1, 200, ["#traj:JZ 3 1 2 3"])
     with pytest.raises(RuntimeError, match="number of coefficients must be equal"):
         bad.calc_currents(t, 2)
+
+
+def test_threads_and_blocks_do_not_change_the_table(tmp_path):
+    """Rows parsed in blocks of nthreads * nbatch lines over several threads
+    are the rows of a one-thread read, in file order, with the same skip
+    counting across block boundaries (bad and text lines interleaved)."""
+    rng = np.random.default_rng(5)
+    lines = ["some preamble", "Step a b\tc"]
+    for k in range(1000):
+        if k % 37 == 5:
+            lines.append("%d 1 2" % k)                  # too few fields
+        elif k % 53 == 7:
+            lines.append("text in between")
+        else:
+            lines.append("%d %r %r %r" % (k, *(float(x) for x in rng.normal(size=3))))
+    p = str(tmp_path / "log")
+    open(p, "w").write("\n".join(lines) + "\n")
+    one = pa.ReadLog_file(p, None, 3, 1, 200, [])
+    many = pa.ReadLog_file(p, None, 3, 3, 7, [])
+    assert one.getNtimesteps() == many.getNtimesteps() > 300
+    np.testing.assert_array_equal(one.get_data_copy(), many.get_data_copy())
+    assert [one.timestep(i) for i in range(one.getNtimesteps())] == \
+           [many.timestep(i) for i in range(many.getNtimesteps())]
