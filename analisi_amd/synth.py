@@ -60,9 +60,10 @@ def types_by_fraction(natoms, fractions):
     return np.repeat(np.arange(len(fractions), dtype=np.int32), counts)
 
 
-def random_walk(seed, natoms, nframes, box, sigma, dtype=np.float64):
+def random_walk(seed, natoms, nframes, box, sigma, dtype=np.float64, progress=None):
     """Positions [nframes, natoms, 3]: frame 0 uniform in the (possibly tilted)
-    cell, each later frame adds N(0, sigma^2) per axis."""
+    cell, each later frame adds N(0, sigma^2) per axis.  ``progress(f)`` is
+    called after every 1,000th frame (long trajectories take minutes)."""
     box = np.asarray(box, dtype=np.float64)
     lo = box[[0, 2, 4]]
     L = box[[1, 3, 5]] - lo
@@ -74,9 +75,37 @@ def random_walk(seed, natoms, nframes, box, sigma, dtype=np.float64):
     x0[:, 2] = lo[2] + s[:, 2] * L[2]
     pos = np.empty((nframes, natoms, 3), dtype=np.float64)
     pos[0] = x0
-    for f in range(1, nframes):
-        step = normal(seed, 2, natoms * 3, offset=(f - 1) * natoms * 3).reshape(natoms, 3)
-        pos[f] = pos[f - 1] + sigma * step
+    # the steps of a chunk of frames are generated on a few threads (numpy
+    # releases the GIL; the values depend only on their counters), then added
+    # frame after frame: pos[f] = pos[f-1] + sigma * step[f-1], in order
+    from concurrent.futures import ThreadPoolExecutor
+    chunk = max(1, (1 << 22) // max(natoms * 3, 1))
+    per = natoms * 3
+
+    def steps(f0, f1):
+        # frame f's step is normal(seed, 2, per, offset=(f - 1) * per): its
+        # uniforms start at counter 2 (f - 1) per
+        m = (per + 1) // 2
+        idx = (2 * per * np.arange(f0 - 1, f1 - 1, dtype=np.uint64))[:, None] + np.arange(2 * m, dtype=np.uint64)
+        u = ((splitmix64(seed, 2, idx) >> np.uint64(11)).astype(np.float64) + 1.0) * (2.0 ** -53)
+        r = np.sqrt(-2.0 * np.log(u[:, 0::2]))
+        th = 2.0 * np.pi * u[:, 1::2]
+        z = np.empty((f1 - f0, 2 * m))
+        z[:, 0::2] = r * np.cos(th)
+        z[:, 1::2] = r * np.sin(th)
+        return sigma * z[:, :per].reshape(f1 - f0, natoms, 3)
+
+    bounds = [(f0, min(f0 + chunk, nframes)) for f0 in range(1, nframes, chunk)]
+    done = 0
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        for k in range(0, len(bounds), 8):
+            group = bounds[k:k + 8]
+            for (f0, f1), st in zip(group, ex.map(lambda b: steps(*b), group)):
+                for i in range(f1 - f0):
+                    pos[f0 + i] = pos[f0 + i - 1] + st[i]
+                if progress is not None and f1 // 1000 > done // 1000:
+                    progress(f1 - 1)
+                done = f1
     return pos.astype(dtype, copy=False)
 
 

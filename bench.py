@@ -1,27 +1,28 @@
 """Benchmark of the time-lagged pair-correlation path on one MI355X.
 
-Default workload (the measured line): 16,384 atoms of 3 types (80/10/10 %),
-2,000 frames of a seeded Gaussian random walk (sigma 0.05 A per axis) in a
-triclinic cell with diagonal 56 A and tilts drawn uniformly in +-0.1 L (fixed
-per run); Gofrt with rmin 0, rmax 12 A, 256 bins, lags 0..100, t0 stride 1,
-reset(1900), calculate(0).  One step = one whole Gofrt::calculate: every
-ordered atom pair for every (t0, lag) = 16384^2 * 1900 * 101 = 5.15e13
-pair-lag updates.
-
-The timed value uses the FP32 + FP64-recheck form (16-bit fixed-point screen
-of every pair, FP32 evaluation of every candidate, FP64 re-evaluation near a
-bin edge); the plain FP64 form is timed on a t0 slice of the same workload and
-both forms' counts are compared on that slice.  Inputs are resident in HBM
-before timing starts; the launch shape is autotuned (untimed) first.  The CPU
-baseline is the oracle's restatement of the reference loop nest on the host
-cores, on a bounded t0 slice of the same workload.
+Default workload (the measured line, --config c3): 65,536 atoms of 2 types
+(50/50), 1,000 frames of a seeded Gaussian random walk (sigma 0.08 A per
+axis) in a cubic 90 A cell; Gofrt in the FP32 + FP64-recheck form with rmin
+0, rmax 15 A, 512 bins, lags 0..64, t0 stride 1, reset(936), calculate(0).
+One step = one whole Gofrt::calculate.  The reference evaluates every ordered
+atom pair for every (t0, lag), 65536^2 * 936 * 65 = 2.61e14 pair-lag updates
+per step; this build screens pairs and prunes whole groups of far pairs, so
+the reported value counts only the pairs some distance test evaluated (a
+device counter), and the all-pairs rate is reported beside it as the
+time-to-solution rate.  Both arithmetic forms are timed on one t0 slice of
+the same workload and their counts compared.  Inputs are resident in HBM
+before timing starts; the launch shape is autotuned (untimed) first.  The
+same line carries the MSD of that trajectory (lags 0..498 with the
+centre-of-mass channel and drift removal, reset(499)) and the Green-Kubo
+workload (c5) as sub-records.  The CPU baseline is the oracle's restatement
+of the reference loop nest on the host cores, on a bounded t0 slice.
 
 Other workloads (--config), each printing one JSON line of its own:
   c1  1,000 atoms 500/500, cubic 32.2 A, 200 frames: Gofrt rmax 10, 200 bins,
       lags 0..50, reset(150); the CPU baseline runs it whole
-  c3  65,536 atoms 50/50, cubic 90 A, sigma 0.08, 1,000 frames: Gofrt rmax 15,
-      512 bins, lags 0..64, reset(936); MSD lags 0..500 with the
-      centre-of-mass channel and drift removal, reset(499)
+  c2  16,384 atoms of 3 types (80/10/10 %), 2,000 frames, sigma 0.05 A,
+      triclinic cell with diagonal 56 A and tilts uniform in +-0.1 L (fixed
+      per run); Gofrt rmax 12 A, 256 bins, lags 0..100, reset(1900)
   c4  8,192 atoms 80/10/10, cubic 45 A, 60,000 frames; 64 seeded windows of
       500..4000 frames: Gofrt lags 0..40 and MSD lags 0..400 of every window in
       one graph launch, with the mean and variance over windows
@@ -37,7 +38,7 @@ Other workloads (--config), each printing one JSON line of its own:
       LAMMPS-binary file of 8,192 atoms x 600 frames read in windows, with the
       next block's window read ahead during the current block, and without
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config c2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config c3]
 """
 import argparse
 import ctypes
@@ -129,7 +130,10 @@ def gofrt_workload(name):
         raise ValueError(name)
     box_l = synth.lammps_box(L, tilt)
     log("generating %s: N=%d F=%d" % (name, N, F))
-    pos = synth.random_walk(seed, N, F, box_l, sigma)
+    t = time.perf_counter()
+    pos = synth.random_walk(seed, N, F, box_l, sigma,
+                            progress=lambda f: log("generated %d of %d frames (%.0f s)" % (f, F, time.perf_counter() - t)))
+    log("generated in %.1f s" % (time.perf_counter() - t))
     types = synth.types_by_fraction(N, fr)
     w = dict(name=name, pos=pos, types=types, box_l=box_l, tilt=tilt, N=N, F=F, L=L, sigma=sigma, seed=seed,
              triclinic=tilt is not None, skip=1, every=1, primo=0, fractions=fr)
@@ -226,11 +230,60 @@ def traffic_from_profile(kernel_prefix, workload):
 # g(r,t) (c1, c2, c3)
 # ---------------------------------------------------------------------------
 
-def run_gofrt(w, args, hip, res_extra=None):
+def profile_figure(key, workload):
+    """A figure measured by rocprofv3 counters and committed under profiles/
+    (lines 'key workload value' in profiles/counters.txt), or None."""
+    p = os.path.join(ROOT, "profiles", "counters.txt")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and not line.startswith("#") and parts[0] == key and parts[1] == workload:
+                return float(parts[2])
+    return None
+
+
+def host_cores():
+    """Threads the CPU baseline uses (at most 16, the box's share of one GPU)
+    and what the host reports."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    return min(aff, 16), nproc, aff
+
+
+def timed_calculates(L, g, w, d_out, stream, hip, nsteps, what):
+    """nsteps whole Gofrt::calculate calls on one stream, HIP events around
+    each; every call first waits for the device (it uploads its work list),
+    so a step's time includes the host's launch work.  A line per step."""
+    check = __import__("analisi_amd.capi", fromlist=["check"]).check
+    evs = [hip.event() for _ in range(nsteps + 1)]
+    hip.sync()
+    t_wall = time.perf_counter()
+    hip.record(evs[0], stream)
+    for i in range(nsteps):
+        check(L.anl_gofrt_calculate_async(g.handle, w["primo"], d_out, stream))
+        hip.record(evs[i + 1], stream)
+        if i > 0:
+            log("%s %d: %.1f ms" % (what, i - 1, hip.elapsed_ms(evs[i - 1], evs[i])))
+    hip.sync()
+    t_wall = time.perf_counter() - t_wall
+    step_ms = [hip.elapsed_ms(evs[i], evs[i + 1]) for i in range(nsteps)]
+    if nsteps:
+        log("%s %d: %.1f ms" % (what, nsteps - 1, step_ms[-1]))
+    return step_ms, t_wall
+
+
+def run_gofrt(w, args, hip):
     from analisi_amd import capi
     check = capi.check
     L = capi.lib()
+    t = time.perf_counter()
     traj, tid, nt = device_traj(w, with_cm=("msd_tmax" in w))
+    log("device trajectory ready in %.1f s" % (time.perf_counter() - t))
     N, F = w["N"], w["F"]
     g = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], w["skip"], w["every"], capi.GOFRT_SCREENED)
     leff = g.reset(w["n"])
@@ -246,37 +299,27 @@ def run_gofrt(w, args, hip, res_extra=None):
     d_out = hip.malloc(out_len * 8)
     stream = hip.stream()
     updates = n_updates(N, w["n"], w["tmax"], w["skip"], w["every"])
-    for i in range(args.warmup):
-        check(L.anl_gofrt_calculate_async(g.handle, w["primo"], d_out, stream))
-        hip.sync()
-        log("warmup %d done" % i)
-    evs = [hip.event() for _ in range(args.steps + 1)]
-    hip.sync()
-    t_wall = time.perf_counter()
-    hip.record(evs[0], stream)
-    for i in range(args.steps):
-        check(L.anl_gofrt_calculate_async(g.handle, w["primo"], d_out, stream))
-        hip.record(evs[i + 1], stream)
-    hip.sync()
-    t_wall = time.perf_counter() - t_wall
-    step_ms = [hip.elapsed_ms(evs[i], evs[i + 1]) for i in range(args.steps)]
-    ms = float(np.mean(step_ms))
-    log("steps ms:", ["%.1f" % x for x in step_ms])
+    timed_calculates(L, g, w, d_out, stream, hip, args.warmup, "warmup")
+    step_ms, t_wall = timed_calculates(L, g, w, d_out, stream, hip, args.steps, "step")
+    ms = float(np.mean(step_ms)) if step_ms else float("nan")
     counts = np.zeros((leff, nch, w["nbin"]), dtype=np.uint64)
     hip.d2h(counts, d_out, out_len * 8)
     _, ncand, nre = g.stats()
+    ps = g.pair_stats()
+    clustered, units = g.cluster_stats()
     # invariants of the full run: lag-0 self channels hold N_type per t0 in
     # bin 0, the pair channels are symmetric (even counts at lag 0)
     ntype = np.bincount(tid, minlength=nt)
     half = nt * (nt + 1) // 2
     inv_ok = True
-    for t in range(nt):
-        ch = self_channel(nt, t)
-        inv_ok &= int(counts[0, ch, 0]) == int(ntype[t]) * w["n"] and int(counts[0, ch, 1:].sum()) == 0
+    for t_ in range(nt):
+        ch = self_channel(nt, t_)
+        inv_ok &= int(counts[0, ch, 0]) == int(ntype[t_]) * w["n"] and int(counts[0, ch, 1:].sum()) == 0
     inv_ok &= bool(np.all(counts[0, :half] % 2 == 0))
 
-    # plain FP64 form on a t0 slice of the same workload + bit-exact check
-    n_sl = min(args.fp64_n, w["n"])
+    # both forms on one t0 slice of the same workload, timed alike; counts compared
+    n_sl = min(args.fp64_n or (101 if w["name"] == "c2" else 16), w["n"])
+    ws = dict(w, n=n_sl)
     g64 = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_FP64)
     g64.reset(n_sl)
     gsc = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_SCREENED)
@@ -284,28 +327,29 @@ def run_gofrt(w, args, hip, res_extra=None):
     gsc.reset(n_sl)
     leff_sl = g64.leff
     len_sl = leff_sl * nch * w["nbin"]
-    d64 = hip.malloc(len_sl * 8)
-    check(L.anl_gofrt_calculate_async(g64.handle, 0, d64, stream))   # warm
-    hip.sync()
-    e0, e1 = hip.event(), hip.event()
-    hip.record(e0, stream)
-    check(L.anl_gofrt_calculate_async(g64.handle, 0, d64, stream))
-    hip.record(e1, stream)
-    ms64 = hip.elapsed_ms(e0, e1)
+    d64, dsc = hip.malloc(len_sl * 8), hip.malloc(len_sl * 8)
+    timed_calculates(L, g64, ws, d64, stream, hip, 1, "fp64 slice warmup")
+    ms64 = timed_calculates(L, g64, ws, d64, stream, hip, 1, "fp64 slice")[0][0]
+    timed_calculates(L, gsc, ws, dsc, stream, hip, 1, "fp32+recheck slice warmup")
+    mssl = timed_calculates(L, gsc, ws, dsc, stream, hip, 1, "fp32+recheck slice")[0][0]
     c64 = np.zeros((leff_sl, nch, w["nbin"]), dtype=np.uint64)
+    csc = np.zeros_like(c64)
     hip.d2h(c64, d64, len_sl * 8)
-    csc = gsc.calculate(0)
-    slice_identical = bool(c64.shape == csc.shape and np.array_equal(c64, csc))
+    hip.d2h(csc, dsc, len_sl * 8)
+    slice_identical = bool(np.array_equal(c64, csc))
     upd64 = n_updates(N, n_sl, w["tmax"])
     rate64 = upd64 / (ms64 * 1e-3)
-    log("fp64 slice: %d t0, %.1f ms, %.3e upd/s, identical=%s" % (n_sl, ms64, rate64, slice_identical))
+    log("slice reset(%d): fp64 %.1f ms, fp32+recheck %.1f ms, identical=%s" % (n_sl, ms64, mssl, slice_identical))
 
-    rate = updates / (ms * 1e-3)
+    evaluated = ps["evaluated"]
     fl = FLOP_PER_UPDATE[w["triclinic"]]
-    achieved = updates * fl / (ms * 1e-3) / 1e12
+    sec = ms * 1e-3
+    achieved = evaluated * fl / sec / 1e12
+    cname = w["name"]
+    busy = profile_figure("gofrt_screened_valu_busy", cname)
     res = {
         "metric": "g(r,t) pair-lag updates/s, FP32+recheck vs plain FP64, bit-exact counts",
-        "value": rate,
+        "value": evaluated / sec,
         "unit": "pair-lag updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -314,37 +358,56 @@ def run_gofrt(w, args, hip, res_extra=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i16 screen + f32 candidates + f64 recheck",
+        "dtype": "i16/f16 screens, f32 candidates, f64 recheck",
         "data": "synthetic (seeded SplitMix64/Box-Muller random walk, seed %d)" % w["seed"],
-        "config": {"workload": "Gofrt g(r,t): %s; rmax %g A, %d bins, lags 0..%d, t0 stride 1, reset(%d)"
-                               % (describe(w), w["rmax"], w["nbin"], leff - 1, w["n"]),
+        "config": {"workload": "%s: Gofrt g(r,t) FP32+recheck, %s; rmax %g A, %d bins, lags 0..%d, t0 stride 1, "
+                               "reset(%d), calculate(0)" % (cname.upper(), describe(w), w["rmax"], w["nbin"],
+                                                            leff - 1, w["n"]),
                    "form": "fp32+recheck", "natoms": N, "frames": F, "lags": leff, "t0": w["n"], "nbin": w["nbin"],
-                   "updates_per_step": updates, "launch_shape": list(g.tuning())},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "launch_shape": list(g.tuning())},
+        "pairs": {"evaluated_per_step": int(evaluated), "all_per_step": int(ps["all"]),
+                  "reference_count_per_step": updates, "tile_screen": int(ps["tile"]),
+                  "screen16": int(ps["screen16"]), "pruned_fraction": 1.0 - evaluated / max(ps["all"], 1),
+                  "note": "value counts evaluated pairs only: a pair is evaluated when a distance test looks at that "
+                          "very pair (FP16 MFMA tile, 16-bit screen, FP32/FP64 evaluation); pairs of 32-column halves "
+                          "skipped whole by the cluster bounding-sphere test are pruned and not counted"},
+        "reference_equivalent": {"value": updates / sec, "unit": "pair-lag updates/s",
+                                 "note": "all N^2 n0 Lf ordered pairs the reference evaluates, per second of whole "
+                                         "calculate (time-to-solution rate, not work done)"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": traffic_from_profile("gofrt_screened", w["name"]),
-                     "note": "%d algorithmic FLOP per pair-lag update (3 sub, 6 min image, %s5 d2, 2 gates: the "
-                             "reference's brute-force evaluation of every pair) over the event-timed kernel; peak = "
-                             "MI355X vector FP32 157.3 TFLOP/s.  The kernel does not execute that work: pairs are "
-                             "screened with 16-bit and FP16-MFMA bounds and far column halves are skipped, so this is "
-                             "an effective rate and can exceed 1 on sparse workloads; the executed instruction mix is "
-                             "in profiles/gofrt_c2_kernel_trace.md" % (fl, "3 tilt, " if w["triclinic"] else "")},
-        "fp64_plain": {"value": rate64, "unit": "pair-lag updates/s",
-                       "sample": "same workload, reset(%d): %d t0 x %d lags" % (n_sl, n_sl, leff_sl), "ms": ms64,
-                       "roofline_frac": upd64 * fl / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                       "speedup_fp32_recheck_vs_fp64": rate / rate64, "counts_identical_on_slice": slice_identical},
-        "screen": {"candidates_per_step": int(ncand), "candidate_fraction": ncand / updates,
-                   "fp64_rechecks_per_step": int(nre), "recheck_fraction_of_candidates": nre / max(ncand, 1)},
-        "checks": {"lag0_self_channels_and_pair_symmetry": bool(inv_ok)},
+                     "traffic": traffic_from_profile("gofrt_screened", cname),
+                     "peak_nonpacked_fp32": FP32_PEAK_TFLOPS / 2,
+                     "frac_nonpacked": achieved / (FP32_PEAK_TFLOPS / 2),
+                     "valu_busy_measured": busy,
+                     "note": "%d algorithmic FLOP per evaluated pair-lag update (3 sub, 6 min image, %s5 d2, 2 gates) "
+                             "x evaluated pairs / event-timed whole calculate; peak = MI355X published FP32 157.3 "
+                             "TFLOP/s (packed); this build issues no packed FP32, whose half, 78.6, is the "
+                             "non-packed VALU figure.  The kernel is instruction-issue bound (VALU/SALU/LDS), so "
+                             "neither 'hbm' nor 'mfma' describes it; valu_busy_measured = SQ_ACTIVE_INST_VALU x 4 "
+                             "/ SQ_BUSY_CYCLES per SIMD from profiles/counters.txt" % (fl, "3 tilt, " if w["triclinic"] else "")},
+        "time_to_solution_vs_fp64": {
+            "slice": "reset(%d) of the same trajectory: %d t0 x %d lags, both forms timed alike" % (n_sl, n_sl, leff_sl),
+            "fp64_ms": ms64, "fp32_recheck_ms": mssl, "ratio_on_slice": ms64 / mssl,
+            "fp64_rate": rate64, "ratio_whole_run": (updates / sec) / rate64,
+            "counts_identical_on_slice": slice_identical,
+            "fp64_roofline_frac": upd64 * fl / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "note": "ratio_whole_run = all-pairs rate of the whole FP32+recheck calculate over the FP64 form's "
+                    "all-pairs rate on the slice (the FP64 form evaluates every pair; its rate does not depend "
+                    "on t0 or lag)"},
+        "screen": {"candidates_per_step": int(ncand), "candidate_fraction_of_all": ncand / updates,
+                   "fp64_rechecks_per_step": int(nre), "recheck_fraction_of_candidates": nre / max(ncand, 1),
+                   "clustered_units": int(clustered), "units": int(units)},
+        "checks": {"lag0_self_channels_and_pair_symmetry": bool(inv_ok), "all_pairs_equal_reference_count":
+                   int(ps["all"]) == int(updates)},
         "autotune": tune,
         "wall_s_timed": t_wall,
     }
-    if w["name"] != "c2":
-        res["metric"] = "g(r,t) pair-lag updates/s (%s)" % w["name"]
     if not args.no_cpu:
-        nthr = min(os.cpu_count() or 1, 16)
-        cn = w["n"] if w["name"] == "c1" else args.cpu_n
-        cr, cdt, ccounts = cpu_gofrt(w, cn, nthr, F_used=F if w["name"] == "c1" else None)
+        nthr, nproc, aff = host_cores()
+        cn = w["n"] if cname == "c1" else (args.cpu_n or (2 if cname == "c3" else 8))
+        log("cpu baseline: oracle on reset(%d), %d threads" % (cn, nthr))
+        cr, cdt, ccounts = cpu_gofrt(w, cn, nthr, F_used=F if cname == "c1" else None)
         if cn == w["n"]:
             same = bool(np.array_equal(counts, ccounts))
         else:   # the oracle sample must equal the device result on the same reset(n)
@@ -352,12 +415,14 @@ def run_gofrt(w, args, hip, res_extra=None):
             gsl.reset(cn)
             same = bool(np.array_equal(gsl.calculate(0), ccounts))
         res["cpu_baseline"] = {"value": cr, "unit": "pair-lag updates/s", "cores": nthr, "kind": "port",
-                               "sample": "same trajectory and histogram, reset(%d): %d t0 x %d lags, %.1f s"
-                                         % (cn, cn, min(cn, w["tmax"]), cdt),
+                               "host_cpus": nproc, "affinity_cpus": aff,
+                               "sample": "oracle (reference loop nest, one thread round per (lag, t0), every ordered "
+                                         "pair) on the same trajectory and histogram, reset(%d): %d t0 x %d lags, "
+                                         "%.1f s on %d threads" % (cn, cn, min(cn, w["tmax"]), cdt, nthr),
                                "counts_identical_to_gpu": same}
         log("cpu baseline %.3e upd/s on %d threads (%.1f s), identical=%s" % (cr, nthr, cdt, same))
-    for p in (d_out, d64):
-        hip.free(p)
+    for p_ in (d_out, d64, dsc):
+        hip.free(p_)
     if "msd_tmax" in w:
         res["msd"] = run_msd(w, args, hip, traj, nt)
     return res
@@ -397,15 +462,16 @@ def run_msd(w, args, hip, traj, nt):
         ns = args.msd_cpu_n
         F = ns + min(ns, tmax)
         tr = O.Traj(w["pos"][:F], w["types"], np.tile(w["box_l"], (F, 1)), w["triclinic"])
-        nthr = min(os.cpu_count() or 1, 16)
+        nthr, nproc, aff = host_cores()
         t = time.perf_counter()
         ref = O.msd(tr, 1, tmax, nthr, True, True, ns, 0)
         dt = time.perf_counter() - t
         got = capi.msd(traj, 1, tmax, True, True, ns, 0)
         err = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))
         res["cpu_baseline"] = {"value": float(w["N"]) * ns * min(ns, tmax) / dt, "unit": "atom-lag-t0 updates/s",
-                               "cores": nthr, "kind": "port",
-                               "sample": "same trajectory, reset(%d), %.1f s" % (ns, dt),
+                               "cores": nthr, "kind": "port", "host_cpus": nproc, "affinity_cpus": aff,
+                               "sample": "oracle on the same trajectory, reset(%d): %d t0 x %d lags, %.1f s"
+                                         % (ns, ns, min(ns, tmax), dt),
                                "max_rel_error_vs_fp64_oracle": err}
         log("msd: %.1f ms, oracle slice max rel error %.2e" % (ms, err))
     return res
@@ -757,16 +823,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "sh", "lammps"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "sh", "lammps"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-lags", type=int, default=4, help="lags per autotune probe launch")
-    ap.add_argument("--cpu-n", type=int, default=8, help="reset(n) of the CPU baseline sample (n t0 x n lags)")
+    ap.add_argument("--cpu-n", type=int, default=0,
+                    help="reset(n) of the CPU baseline sample (n t0 x n lags; 0 = 2 on c3, 8 on c2)")
     ap.add_argument("--c4-windows", type=int, default=0, help="number of c4 windows (0 = the workload's 64)")
     ap.add_argument("--c4-split", type=int, default=1, help="c4 windows cut into this many graphs per step")
     ap.add_argument("--gk-cpu-n", type=int, default=6000, help="reset(n), n lags, of the GK oracle sample")
     ap.add_argument("--msd-cpu-n", type=int, default=20, help="reset(n) of the MSD oracle sample")
-    ap.add_argument("--fp64-n", type=int, default=101, help="reset(n) of the plain-FP64 slice")
+    ap.add_argument("--fp64-n", type=int, default=0, help="reset(n) of the plain-FP64 slice (0 = 101 on c2, 16 else)")
+    ap.add_argument("--no-gk", action="store_true", help="c3: leave out the Green-Kubo (c5) sub-record")
     ap.add_argument("--n", type=int, default=0, help="reset(n) of the timed run (0 = the workload's own)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device-to-device (a plain copy kernel of known size, "
@@ -798,6 +866,8 @@ def main():
         if args.n:
             w["n"] = args.n
         res = run_blocks(w, args, hip) if args.config == "c4" else run_gofrt(w, args, hip)
+        if args.config == "c3" and not args.no_gk:
+            res["gk"] = run_gk(args, hip)
     print(json.dumps(res), flush=True)
 
 
