@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 
 #include "anl_internal.h"
 
@@ -1220,7 +1221,8 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
         if (!(rmax > rmin) || rmin < 0) fail(ANL_ERR_ARG, "need 0 <= rmin < rmax");
         if (form < ANL_GOFRT_FP64 || form > ANL_GOFRT_AUTO) fail(ANL_ERR_ARG, "unknown g(r,t) form");
         ensure_device();
-        auto *g = new anl_gofrt();
+        std::unique_ptr<anl_gofrt, int (*)(anl_gofrt *)> owner(new anl_gofrt(), anl_gofrt_destroy);
+        anl_gofrt *g = owner.get();
         g->tr = tr;
         g->rmin = rmin;
         g->rmax = rmax;
@@ -1253,7 +1255,7 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
         ANL_HIP(hipMalloc(&g->d_chan, chan.size() * sizeof(int32_t)));
         ANL_HIP(hipMemcpy(g->d_chan, chan.data(), chan.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         ANL_HIP(hipMalloc(&g->d_ncand, NSTAT * sizeof(unsigned long long)));
-        *out = g;
+        *out = owner.release();
     });
 }
 

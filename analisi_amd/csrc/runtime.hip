@@ -5,6 +5,7 @@
 #include <thread>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 
 #include "anl_internal.h"
@@ -279,7 +280,13 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
             if (!(std::fabs(pos[i]) <= bound))
                 fail(ANL_ERR_ARG, "positions must be finite and within 10^4 (shortest) cell lengths of the origin");
         ensure_device();
-        auto *tr = new anl_traj();
+        // on any failure below (a device allocation, an upload) everything
+        // allocated so far is released with the trajectory
+        std::unique_ptr<anl_traj, void (*)(anl_traj *)> owner(new anl_traj(), [](anl_traj *t) {
+            (void)hipDeviceSynchronize();
+            (void)anl_traj_destroy(t);
+        });
+        anl_traj *tr = owner.get();
         tr->F = nframes;
         tr->N = natoms;
         tr->nt = ntypes;
@@ -364,7 +371,7 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
             up.put(tr->cm, pos_cm, nframes * ntypes * 3 * sizeof(double));
         }
         up.finish();
-        *out = tr;
+        *out = owner.release();
     });
 }
 

@@ -365,8 +365,9 @@ struct SannArgs {
     uint32_t off[SANN_MAXT];    // its first entry
     double cut2[SANN_MAXT];     // squared cutoff of each neighbour type
     uint32_t total, capmax;     // sum and maximum of nn (total rounded up to even)
-    int *overflow;              // set when a list would exceed its capacity
+    int *overflow;              // set when the reference would throw "Too many neighbours in shell!"
     int32_t *nbr;               // [nfr][N][total] chosen neighbours (type-sorted index) from off[t], or null
+    const int32_t *orig;        // [N] original atom index of each slot
 };
 
 // SphericalBase::calc with a neighbour list (neighbour.cpp:8-101,
@@ -433,6 +434,34 @@ __global__ __launch_bounds__(64) void sph_density_sann_kernel(SphDensArgs a, San
                 if (base + add > s.nn[t]) atomicOr(s.overflow, 1);
             }
             wave_sync_lds();
+        }
+    }
+    // neighbour.cpp:19-44 meets the pairs of atom i in increasing order of
+    // the partner's index and throws when a pair within either type's cutoff
+    // finds i's list for the partner's type already full.  Beyond a list that
+    // overflows (above), that happens when the list of type t is exactly full
+    // and an atom j of type t with cut2[t] < d2 <= cut2[type i] comes after
+    // the list's last member in index order.  Only cutoffs that differ by
+    // type can give it, so this pass is rare.
+    {
+        const uint32_t ti = (uint32_t)a.type_slot[i];
+        for (uint32_t t = 0; t < a.nt; ++t) {
+            if (!((uint32_t)cnt[t] == s.nn[t] && s.cut2[ti] > s.cut2[t])) continue;  // uniform
+            int last = -1;
+            for (uint32_t k = lane; k < s.nn[t]; k += 64) last = max(last, s.orig[ju[s.off[t] + k]]);
+            for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o, 64));
+            bool late = false;
+            for (uint64_t jb = 0; jb < a.N; jb += 64) {
+                const uint64_t j = jb + lane;
+                if (j < a.N && j != i && (uint32_t)a.type_slot[j] == t && s.orig[j] > last) {
+                    const double4 pj = P[j];
+                    double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                    min_image_ref<TRI>(x0, x1, x2, c);
+                    const double d2 = d2_of(x0, x1, x2);
+                    late = late || (d2 > s.cut2[t] && d2 <= s.cut2[ti]);
+                }
+            }
+            if (__ballot(late) && lane == 0) atomicOr(s.overflow, 1);
         }
     }
     double qx0 = 0.0, qx1 = 0.0, qx2 = 0.0;
@@ -1170,6 +1199,7 @@ int anl_sph_set_neighbours(anl_sph *h, const uint32_t *nneigh, const double *cut
         sa.total = (tot + 1) / 2 * 2;
         if (!h->d_overflow) ANL_HIP(hipMalloc(&h->d_overflow, sizeof(int)));
         sa.overflow = h->d_overflow;
+        sa.orig = h->tr->orig_slot;
         h->sann_args = sa;
         h->sann = true;
     });

@@ -963,8 +963,10 @@ static unsigned sann_count(const orc_neigh *l, unsigned n)
  * nneigh[t] of them (-1 beyond: "Too many neighbours in shell!"); the
  * harmonics of the first SANN-count ones are summed into bin 0 of
  * res[i][t][nbin][(lmax+1)^2], and the count goes to cnt[i][t] (may be NULL).
- * The reference checks the capacity of both atoms' lists for every pair within
- * either cutoff; this restatement checks each list against its own count. */
+ * The capacity check is the reference's own pair loop (neighbour.cpp:15-47):
+ * pairs i < j in index order; a pair within either atom type's cutoff throws
+ * when i's list of j's type or j's list of i's type is already full; a list
+ * counts a partner only within the partner type's cutoff. */
 static int sph_density_sann(const orc_traj *tr, uint64_t ts, const unsigned *nneigh, const double *cut2,
                             unsigned nbin, int lmax, double *res, unsigned *cnt, uint64_t *nbr, unsigned capmax)
 {
@@ -977,6 +979,19 @@ static int sph_density_sann(const orc_traj *tr, uint64_t ts, const unsigned *nne
     memset(res, 0, N * nt * nbin * L2 * sizeof(double));
     const double *box = tr->box + ts * tr->box_stride;
     int rc = 0;
+    unsigned *size = calloc(N * nt, sizeof(unsigned)); /* [atom][type] list lengths */
+    for (uint64_t i = 0; i < N && rc == 0; i++)
+        for (uint64_t j = i + 1; j < N; j++) {
+            const int ti = tr->type_id[i], tj = tr->type_id[j];
+            double x[3];
+            const double d2 = orc_d2_min_image(tr->pos + (ts * N + i) * 3, tr->pos + (ts * N + j) * 3, box,
+                                               tr->triclinic, x);
+            if (!(d2 <= cut2[tj] || d2 <= cut2[ti])) continue;
+            if (!(size[i * nt + tj] < nneigh[tj] && size[j * nt + ti] < nneigh[ti])) { rc = -1; break; }
+            if (d2 <= cut2[tj]) size[i * nt + tj]++;
+            if (d2 <= cut2[ti]) size[j * nt + ti]++;
+        }
+    free(size);
     for (uint64_t i = 0; i < N && rc == 0; i++)
         for (uint64_t t = 0; t < nt && rc == 0; t++) {
             unsigned n = 0;

@@ -214,6 +214,25 @@ def test_sann_densities_correlations_steinhardt(tri):
         small.snapshot(0)
 
 
+@pytest.mark.parametrize("late_first", [False, True])
+def test_sann_capacity_follows_the_reference_pair_order(late_first):
+    """A list that is exactly full when a later atom of its type comes within
+    the other type's (larger) cutoff throws, as the reference's pair loop
+    does (neighbour.cpp:15-47); the same atoms in another index order do not
+    (the oracle test of the same case says why)."""
+    from test_oracle_sph import _exactly_full_case
+    pos, types, box_l = _exactly_full_case(late_first)
+    ot = O.Traj(pos, types, np.tile(box_l, (2, 1)), False)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    s = capi.DeviceSph(dt, _ranges(2, 3), 1, 6)
+    s.set_neighbours([(10, 2.0 ** 2, 0.0), (2, 1.2 ** 2, 0.0)])
+    if late_first:
+        _close(s.snapshot(0), O.sph_density_sann(ot, 0, [(10, 4.0, 0.0), (2, 1.44, 0.0)], 1, 6)[0])
+    else:
+        with pytest.raises(capi.AnlError, match="Too many neighbours"):
+            s.snapshot(0)
+
+
 def test_steinhardt_histogram_equals_binned_full_order_q():
     """The histogram expands the densities only up to its largest l; binning
     the per-atom q (expanded to lmax) by the reference's rule

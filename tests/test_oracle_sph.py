@@ -395,3 +395,35 @@ def test_averaged_histogram_nan_cells():
             rows = types == ti
             assert h[ti, tj].sum() == pytest.approx(counted[:, rows, tj].sum() * incr)
             assert h[ti, tj, nb - 1, nb - 1] == pytest.approx(nan[:, rows, tj].sum() * incr)
+
+
+def _exactly_full_case(late_first):
+    """Atom 0 (type 0, cutoff 2.0 A) with two type-1 atoms at 1.0 and 1.1 A
+    (inside type 1's cutoff of 1.2 A; type 1's lists hold 2) and one at 1.5 A
+    (outside 1.2, inside 2.0).  The reference meets atom 0's pairs in index
+    order (neighbour.cpp:15-47): with the 1.5 A atom last, pair (0, 3) is
+    within type 0's cutoff and finds atom 0's type-1 list full: it throws.
+    With the 1.5 A atom first, the list is empty then, and fills to exactly 2
+    later: no throw.  The type-1 atoms are 1.49 A or more apart."""
+    c = np.array([25.0, 25.0, 25.0])
+    near = [c + [1.0, 0, 0], c + [0, 1.1, 0]]
+    far = c + [0, 0, 1.5]
+    pts = [c] + ([far] + near if late_first else near + [far])
+    pos = np.repeat(np.array(pts)[None], 2, axis=0)
+    types = np.array([0, 1, 1, 1], dtype=np.int32)
+    return pos, types, synth.lammps_box(50.0)
+
+
+@pytest.mark.parametrize("late_first", [False, True])
+def test_sann_capacity_follows_the_reference_pair_order(late_first):
+    pos, types, box_l = _exactly_full_case(late_first)
+    tr = O.Traj(pos, types, np.tile(box_l, (2, 1)), False)
+    spec = [(10, 2.0 ** 2, 0.0), (2, 1.2 ** 2, 0.0)]
+    if late_first:
+        _, cnt = O.sph_density_sann(tr, 0, spec, 1)
+        assert cnt[0, 1] == 0            # two neighbours: fewer than SANN's three
+    else:
+        with pytest.raises(RuntimeError, match="Too many neighbours"):
+            O.sph_density_sann(tr, 0, spec, 1)
+    # equal cutoffs: the 1.5 A atom is outside both, no throw either way
+    O.sph_density_sann(tr, 0, [(10, 1.2 ** 2, 0.0), (2, 1.2 ** 2, 0.0)], 1)
