@@ -72,6 +72,9 @@ SIGNATURES = [
                                           _vp, _vp]),
     ("anl_gk_calculate_async", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp, _vp]),
     ("anl_gk_calculate_mfma_async", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _i, _u32, _u64, _vp, _vp]),
+    ("anl_gk_set_tuning", _i, [_vp, _u32, _u32, _u32]),
+    ("anl_gk_get_tuning", _i, [_vp] + [ctypes.POINTER(_u32)] * 3),
+    ("anl_gk_autotune", _i, [_vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _u32, _vp, _u32, _vp, ctypes.POINTER(_u32)]),
     ("anl_mediavar_update_async", _i, [_vp, _vp, _vp, _u64, _u32, _vp]),
     ("anl_mediavar_end_async", _i, [_vp, _u64, _u32, _vp]),
     ("anl_counts_to_double_async", _i, [_vp, _d, _u64, _vp, _vp]),
@@ -458,6 +461,29 @@ class DeviceSeries:
         if not stats:
             return res, None, None
         return res, mean.to_host((leff, narr)), var.to_host((leff, narr))
+
+    def set_gk_shape(self, tiles_per_wave, chunk_steps, waves_per_group):
+        check(lib().anl_gk_set_tuning(self.handle, tiles_per_wave, chunk_steps, waves_per_group))
+
+    def gk_shape(self):
+        v = [_u32() for _ in range(3)]
+        check(lib().anl_gk_get_tuning(self.handle, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def gk_autotune(self, idx_j, skip, tmax, n_seg, primo, n, candidates=None):
+        """Times the MFMA lag products of the blocks (primo[b], n[b]) with each
+        shape (tiles per wavefront, chunk steps, wavefronts per workgroup) and
+        keeps the fastest.  Returns (times in ms, index of the best)."""
+        idx = np.ascontiguousarray(idx_j, dtype=np.uint32)
+        pr = np.ascontiguousarray(primo, dtype=np.uint64).reshape(-1)
+        nn = np.ascontiguousarray(n, dtype=np.uint32).reshape(-1)
+        cand = None if candidates is None else np.ascontiguousarray(candidates, dtype=np.uint32).reshape(-1, 3)
+        nc = 6 if cand is None else cand.shape[0]
+        ms = np.zeros(nc, dtype=np.float32)
+        best = _u32()
+        check(lib().anl_gk_autotune(self.handle, _ptr(idx), len(idx), skip, tmax, n_seg, _ptr(pr), _ptr(nn), len(pr),
+                                    _ptr(cand), 0 if cand is None else nc, _ptr(ms), ctypes.byref(best)))
+        return ms, best.value
 
     def close(self):
         if self.handle:

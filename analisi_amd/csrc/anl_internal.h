@@ -101,6 +101,9 @@ struct anl_series {
     // calls on one series are stream-ordered on the caller's stream.
     void *work = nullptr;
     size_t work_cap = 0;
+    // launch shape of the MFMA lag products: tiles per wavefront, MFMA steps
+    // per FP32 chunk, wavefronts per workgroup (anl_gk_set_tuning)
+    uint32_t gk_q = 8, gk_cs = 16, gk_wg = 4;
 };
 
 namespace anl {
@@ -140,7 +143,8 @@ void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int sub
 void gk_mfma_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
                             uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, double *d,
                             hipStream_t st);
-size_t gk_mfma_scratch_bytes(uint32_t ncur, uint32_t leff, uint32_t n);
+size_t gk_mfma_scratch_bytes(const anl_series *s, uint32_t ncur, uint32_t leff, uint32_t n);
+bool gkm_shape_ok(uint32_t q, uint32_t cs, uint32_t wg);
 // MediaVar over blocks (calcoliblocchi.h:21-65) of x_b = counts_b * incr_b or
 // values_b, element-parallel, Welford steps in block order.
 void mediavar_blocks(const unsigned long long *counts, const double *values, const double *incr, uint32_t nblocks,

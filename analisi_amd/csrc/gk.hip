@@ -620,7 +620,7 @@ static int gk_calculate_host(anl_series *s, const uint32_t *idx_j, uint32_t ncur
         if (leff == 0) return;
         ensure_device();
         const size_t bytes = (size_t)leff * (3 * ncur * ncur + 2) * sizeof(double);
-        const size_t front = form == ANL_GK_MFMA ? gk_mfma_scratch_bytes(ncur, leff, n) : 0;
+        const size_t front = form == ANL_GK_MFMA ? gk_mfma_scratch_bytes(s, ncur, leff, n) : 0;
         char *w = (char *)series_workspace(s, front + bytes);
         double *d = (double *)(w + front);
         // the MFMA form takes its scratch from the front of the series scratch
@@ -628,6 +628,95 @@ static int gk_calculate_host(anl_series *s, const uint32_t *idx_j, uint32_t ncur
                                                &n, 1, form, d, nullptr, nullptr, 0);
         if (rc != ANL_OK) throw Error{rc, anl_last_error()};
         ANL_HIP(hipMemcpy(out_host, d, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int anl_gk_set_tuning(anl_series *s, uint32_t tiles_per_wave, uint32_t chunk_steps, uint32_t waves_per_group) {
+    return guard([&] {
+        if (!s) fail(ANL_ERR_ARG, "null series");
+        if (!gkm_shape_ok(tiles_per_wave, chunk_steps, waves_per_group))
+            fail(ANL_ERR_ARG, "Green-Kubo MFMA shape: tiles per wavefront 4|8, chunk steps 8|16, wavefronts 2|4");
+        s->gk_q = tiles_per_wave;
+        s->gk_cs = chunk_steps;
+        s->gk_wg = waves_per_group;
+    });
+}
+
+int anl_gk_get_tuning(anl_series *s, uint32_t *tiles_per_wave, uint32_t *chunk_steps, uint32_t *waves_per_group) {
+    return guard([&] {
+        if (!s) fail(ANL_ERR_ARG, "null series");
+        if (tiles_per_wave) *tiles_per_wave = s->gk_q;
+        if (chunk_steps) *chunk_steps = s->gk_cs;
+        if (waves_per_group) *waves_per_group = s->gk_wg;
+    });
+}
+
+// Times the MFMA lag products of the given blocks with each candidate shape
+// (one untimed and one timed enqueue each, HIP events on the null stream;
+// the epilogue does not depend on the shape and is left out) and keeps the
+// fastest in the series.
+int anl_gk_autotune(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax, uint32_t n_seg,
+                    const uint64_t *primo, const uint32_t *n, uint32_t nblocks, const uint32_t *cand, uint32_t ncand,
+                    float *ms_out, uint32_t *best_out) {
+    return guard([&] {
+        const uint32_t leff = gk_block_checks(s, idx_j, ncur, tmax, primo, n, nblocks);
+        static const uint32_t defaults[] = {8, 16, 4, 8, 16, 2, 4, 16, 4, 4, 16, 2, 8, 8, 4, 4, 8, 4};
+        if (!cand) {
+            cand = defaults;
+            ncand = sizeof(defaults) / sizeof(defaults[0]) / 3;
+        }
+        if (ncand == 0) fail(ANL_ERR_ARG, "no candidates");
+        for (uint32_t c = 0; c < ncand; c++)
+            if (!gkm_shape_ok(cand[3 * c], cand[3 * c + 1], cand[3 * c + 2]))
+                fail(ANL_ERR_ARG, "candidate " + std::to_string(c) +
+                                      ": tiles per wavefront 4|8, chunk steps 8|16, wavefronts 2|4");
+        if (n_seg < 1) n_seg = 1;
+        if (skip == 0) skip = 1;
+        if (leff == 0) return;
+        ensure_device();
+        const uint32_t sq = s->gk_q, sc = s->gk_cs, sw = s->gk_wg;
+        double *d = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        float best = -1.0f;
+        uint32_t besti = 0;
+        try {
+            ANL_HIP(hipMalloc(&d, (size_t)nblocks * leff * (3 * ncur * ncur + 2) * sizeof(double)));
+            ANL_HIP(hipEventCreate(&e0));
+            ANL_HIP(hipEventCreate(&e1));
+            for (uint32_t c = 0; c < ncand; c++) {
+                s->gk_q = cand[3 * c];
+                s->gk_cs = cand[3 * c + 1];
+                s->gk_wg = cand[3 * c + 2];
+                gk_mfma_enqueue_blocks(s, idx_j, ncur, skip, leff, n_seg, primo, n, nblocks, d, 0);
+                ANL_HIP(hipEventRecord(e0, 0));
+                gk_mfma_enqueue_blocks(s, idx_j, ncur, skip, leff, n_seg, primo, n, nblocks, d, 0);
+                ANL_HIP(hipEventRecord(e1, 0));
+                ANL_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                ANL_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms_out) ms_out[c] = ms;
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    besti = c;
+                }
+            }
+        } catch (...) {
+            s->gk_q = sq;
+            s->gk_cs = sc;
+            s->gk_wg = sw;
+            (void)hipDeviceSynchronize();
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(d);
+        s->gk_q = cand[3 * besti];
+        s->gk_cs = cand[3 * besti + 1];
+        s->gk_wg = cand[3 * besti + 2];
+        if (best_out) *best_out = besti;
     });
 }
 

@@ -23,10 +23,15 @@
 // load for every tile offset q (the eight c8 copies absorb q mod 8), and the
 // 32 rows of one k-block are 512 contiguous bytes: a wavefront's operand load
 // touches 8 whole cache lines.
-// A wavefront owns tiles q0 + 16j (j < GKM_Q) of one current pair and one K
-// chunk: tile q0 + 16j at step st reads the same Y vector as tile q0 at step
-// st + j, so Y operands live in a rolling register window and each step loads
-// one new Y vector and one X vector for GKM_Q tiles.
+// A wavefront owns tiles q0 + 16j (j < Q) of one current pair and one K
+// chunk of CS steps: tile q0 + 16j at step st reads the same Y vector as tile
+// q0 at step st + j, so Y operands live in a rolling register window and each
+// step loads one new Y vector and one X vector for Q tiles.  WG wavefronts of
+// consecutive tile groups form a workgroup and share the X operands.  The
+// shape (Q, CS, WG) is a launch parameter of the series, picked by
+// anl_gk_autotune: Q and WG never change results (every tile sums its K
+// steps in the same order); CS sets where the FP32 sums are handed to FP64,
+// so it changes the rounding within the MFMA form's error bound.
 #include <algorithm>
 #include <vector>
 
@@ -37,8 +42,6 @@ namespace anl {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t GKM_Q = 8;      // tiles per wavefront, 16 apart
-constexpr uint32_t GKM_CS = 16;    // MFMA steps (16 rows of 32 t0 each) per FP32 chunk
 constexpr uint32_t GKM_DY = 2;     // Y operands loaded this many steps before first use
 constexpr uint32_t GKM_DX = 1;     // X operands likewise
 constexpr uint32_t GKM_MAXB = 64;  // blocks per batch of launches
@@ -159,31 +162,29 @@ __device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
 }
 
-constexpr uint32_t GKM_WG = 4;  // wavefronts per workgroup: consecutive tile groups sharing X
-
-// The K loop of one wavefront: 3 components x GKM_CS steps of GKM_Q tiles,
+// The K loop of one wavefront: 3 components x CS steps of Q tiles,
 // three split products each (four on tile 0 when LAG0).  The workgroup's
 // wavefronts share one X operand block per component: it is copied into LDS
 // (xs: [hi/lo][step][lane] 16-byte vectors, contiguous in HBM) while the Y
 // window preloads, and read from LDS one step ahead.
-template <bool LAG0>
-__device__ __forceinline__ void gkm_accumulate(f32x16 (&acc)[GKM_Q], uint4 *xs, const unsigned short *__restrict__ xh,
+template <uint32_t Q, uint32_t CS, uint32_t WG, bool LAG0>
+__device__ __forceinline__ void gkm_accumulate(f32x16 (&acc)[Q], uint4 *xs, const unsigned short *__restrict__ xh,
                                                const unsigned short *__restrict__ xl,
                                                const unsigned short *__restrict__ yh,
                                                const unsigned short *__restrict__ yl, uint32_t PP, uint32_t YP,
                                                uint32_t ca, uint32_t cb, uint32_t ncx, uint32_t r, uint32_t h,
                                                uint32_t c8, uint32_t qoff, uint32_t st0) {
     const uint32_t lane = h * 32 + r, tid = threadIdx.x;
-    constexpr uint32_t XN = GKM_CS * 64;  // vectors of one X block (hi or lo)
+    constexpr uint32_t XN = CS * 64;  // vectors of one X block (hi or lo)
     for (uint32_t x = 0; x < 3; x++) {
         // k-block of step st: 2 st + h (X), 2 st + qoff / 8 + h (Y); 512 elements per step
         const uint64_t xb = ((uint64_t)(3 * ca + x) * (PP / 8) + 2 * st0) * 32 * 8;
         const uint64_t yo = ((((uint64_t)c8 * ncx + 3 * cb + x) * (YP / 8) + qoff / 8 + h) * 32 + r) * 8;
         const unsigned short *YH = yh + yo, *YL = yl + yo;
         // Y(v) (the operand vector of step v) lives in window slot v % YW and
-        // is loaded GKM_DY steps before its first use (by tile GKM_Q - 1);
+        // is loaded GKM_DY steps before its first use (by tile Q - 1);
         // X(v) in register slot v % XW, read from LDS GKM_DX steps ahead
-        constexpr uint32_t YW = GKM_Q + GKM_DY, XW = GKM_DX + 1;
+        constexpr uint32_t YW = Q + GKM_DY, XW = GKM_DX + 1;
         bf16x8 wh[YW], wl[YW], xhv[XW], xlv[XW];
 #pragma unroll
         for (uint32_t v = 0; v + 1 < YW; v++) {
@@ -194,10 +195,10 @@ __device__ __forceinline__ void gkm_accumulate(f32x16 (&acc)[GKM_Q], uint4 *xs, 
         {
             const uint4 *XHb = reinterpret_cast<const uint4 *>(xh + xb), *XLb = reinterpret_cast<const uint4 *>(xl + xb);
 #pragma unroll
-            for (uint32_t k = 0; k < XN / (64 * GKM_WG); k++) {
-                const uint4 vh = XHb[tid + 64 * GKM_WG * k], vl = XLb[tid + 64 * GKM_WG * k];
-                xs[tid + 64 * GKM_WG * k] = vh;
-                xs[XN + tid + 64 * GKM_WG * k] = vl;
+            for (uint32_t k = 0; k < XN / (64 * WG); k++) {
+                const uint4 vh = XHb[tid + 64 * WG * k], vl = XLb[tid + 64 * WG * k];
+                xs[tid + 64 * WG * k] = vh;
+                xs[XN + tid + 64 * WG * k] = vl;
             }
         }
         __syncthreads();
@@ -207,13 +208,13 @@ __device__ __forceinline__ void gkm_accumulate(f32x16 (&acc)[GKM_Q], uint4 *xs, 
             xlv[v] = __builtin_bit_cast(bf16x8, xs[XN + v * 64 + lane]);
         }
 #pragma unroll
-        for (uint32_t sv = 0; sv < GKM_CS; sv++) {  // unrolled: every slot index is a constant
+        for (uint32_t sv = 0; sv < CS; sv++) {  // unrolled: every slot index is a constant
             const uint32_t sn = st0 + sv;
-            if (sv + YW - 1 < GKM_CS + GKM_Q - 1) {  // Y rows up to step GKM_CS + GKM_Q - 2 are used
+            if (sv + YW - 1 < CS + Q - 1) {  // Y rows up to step CS + Q - 2 are used
                 wh[(sv + YW - 1) % YW] = ld8(YH + 512 * (sn + YW - 1));
                 wl[(sv + YW - 1) % YW] = ld8(YL + 512 * (sn + YW - 1));
             }
-            if (sv + XW - 1 < GKM_CS) {
+            if (sv + XW - 1 < CS) {
                 xhv[(sv + XW - 1) % XW] = __builtin_bit_cast(bf16x8, xs[(sv + XW - 1) * 64 + lane]);
                 xlv[(sv + XW - 1) % XW] = __builtin_bit_cast(bf16x8, xs[XN + (sv + XW - 1) * 64 + lane]);
             }
@@ -222,7 +223,7 @@ __device__ __forceinline__ void gkm_accumulate(f32x16 (&acc)[GKM_Q], uint4 *xs, 
             __builtin_amdgcn_sched_barrier(0);
             const bf16x8 ah = xhv[sv % XW], al = xlv[sv % XW];
 #pragma unroll
-            for (uint32_t j = 0; j < GKM_Q; j++) {
+            for (uint32_t j = 0; j < Q; j++) {
                 const uint32_t w = (sv + j) % YW;
                 acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wh[w], acc[j], 0, 0, 0);
                 acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wl[w], acc[j], 0, 0, 0);
@@ -245,16 +246,18 @@ __device__ __forceinline__ void wave_sync_lds() {
 // Operand layouts of v_mfma_f32_32x32x16_bf16: lane l feeds row l % 32 of A
 // and column l % 32 of B, each with the 8 k-values of slot l / 32 (the same k
 // slots in A and B); result item e of lane l is row 8(e/4) + 4(l/32) + e%4,
-// column l % 32.  Grid: x = GKM_WG consecutive tile groups (one per
+// column l % 32.  Grid: x = WG consecutive tile groups (one per
 // wavefront), y = current pair, z = (block, chunk).  The 63 diagonal sums of
 // a tile are pairwise FP64 sums in a fixed order, so results do not depend
-// on the launch.
-__global__ __launch_bounds__(64 * GKM_WG, 2) void gkm_tile_kernel(const GkmBatch a) {
+// on the launch.  At least 2 wavefronts per SIMD (up to 256 VGPRs each).
+template <uint32_t Q, uint32_t CS, uint32_t WG>
+__global__ __launch_bounds__(64 * WG, 2) void gkm_tile_kernel(const GkmBatch a) {
+    static_assert(CS % WG == 0, "the X block is copied in WG equal parts");
     // X blocks during the K loop, then one 32 x 33 FP64 tile per wavefront
-    __shared__ double smem[GKM_WG * 32 * 33];
-    static_assert(sizeof(smem) >= 2 * GKM_CS * 64 * sizeof(uint4), "X block does not fit");
+    constexpr uint32_t TILE_D = WG * 32 * 33, XBLK_D = 2 * CS * 64 * sizeof(uint4) / sizeof(double);
+    __shared__ double smem[TILE_D > XBLK_D ? TILE_D : XBLK_D];
     const uint32_t wv = threadIdx.x >> 6;
-    const uint32_t g = blockIdx.x * GKM_WG + wv, pair = blockIdx.y, z = blockIdx.z;
+    const uint32_t g = blockIdx.x * WG + wv, pair = blockIdx.y, z = blockIdx.z;
     uint32_t b = 0;
     while (z >= a.z0[b + 1]) b++;  // uniform
     const uint32_t chunk = z - a.z0[b], nchunks = a.z0[b + 1] - a.z0[b];
@@ -263,29 +266,29 @@ __global__ __launch_bounds__(64 * GKM_WG, 2) void gkm_tile_kernel(const GkmBatch
     const char *reg = a.work + a.base[b];
     const unsigned short *xh = (const unsigned short *)(reg + off.xh), *xl = (const unsigned short *)(reg + off.xl);
     const unsigned short *yh = (const unsigned short *)(reg + off.yh), *yl = (const unsigned short *)(reg + off.yl);
-    const uint32_t qbase = (g / 16) * 16 * GKM_Q, q0 = g % 16;
+    const uint32_t qbase = (g / 16) * 16 * Q, q0 = g % 16;
     const uint32_t ca = pair / a.ncur, cb = pair % a.ncur, ncx = a.ncx;
     const uint32_t lane = threadIdx.x & 63u, r = lane & 31u, h = lane >> 5;
     const uint32_t c8 = q0 & 7u, qoff = qbase + (q0 & ~7u);
-    const uint32_t st0 = chunk * GKM_CS;
+    const uint32_t st0 = chunk * CS;
     uint4 *xs = reinterpret_cast<uint4 *>(smem);
-    f32x16 acc[GKM_Q];
+    f32x16 acc[Q];
 #pragma unroll
-    for (uint32_t j = 0; j < GKM_Q; j++) acc[j] = f32x16{};
+    for (uint32_t j = 0; j < Q; j++) acc[j] = f32x16{};
     // the wavefront holding tile 0 (lag 0) adds the fourth product; the branch
     // sits outside the MFMA loop so that its loads schedule freely (both
     // paths pass the same workgroup barriers)
     if (g == 0)
-        gkm_accumulate<true>(acc, xs, xh, xl, yh, yl, PP, YP, ca, cb, ncx, r, h, c8, qoff, st0);
+        gkm_accumulate<Q, CS, WG, true>(acc, xs, xh, xl, yh, yl, PP, YP, ca, cb, ncx, r, h, c8, qoff, st0);
     else
-        gkm_accumulate<false>(acc, xs, xh, xl, yh, yl, PP, YP, ca, cb, ncx, r, h, c8, qoff, st0);
+        gkm_accumulate<Q, CS, WG, false>(acc, xs, xh, xl, yh, yl, PP, YP, ca, cb, ncx, r, h, c8, qoff, st0);
     __syncthreads();  // X blocks are dead: the tiles below reuse their LDS
     // diagonal sums of each tile, in FP64, into this chunk's partial; each
     // wavefront works in its own tile, so only wavefront-level syncs
     double(*tile)[33] = reinterpret_cast<double(*)[33]>(smem + wv * 32 * 33);
     double *part = a.partial + a.poff[b];
 #pragma unroll
-    for (uint32_t j = 0; j < GKM_Q; j++) {
+    for (uint32_t j = 0; j < Q; j++) {
         const uint32_t q = qbase + q0 + 16 * j;
         if (q >= a.ntiles) break;  // uniform across the wavefront
 #pragma unroll
@@ -355,19 +358,28 @@ struct GkmSize {
     uint64_t region, partial;  // bytes of the operand region, doubles of partial sums
 };
 
-GkmSize gkm_size(uint32_t ncur, uint32_t leff, uint32_t n) {
+// Launch shape of the tile kernel (see gkm_tile_kernel): tiles per
+// wavefront, MFMA steps per FP32 chunk, wavefronts per workgroup.
+struct GkmShape {
+    uint32_t q, cs, wg;
+};
+
+GkmShape gkm_shape_of(const anl_series *s) { return GkmShape{s->gk_q, s->gk_cs, s->gk_wg}; }
+
+GkmSize gkm_size(uint32_t ncur, uint32_t leff, uint32_t n, const GkmShape &sh) {
     GkmSize z{};
-    // t0 axis: rows of 32, MFMA steps of 16 rows, chunks of GKM_CS steps
+    const uint64_t Q = sh.q, CS = sh.cs;
+    // t0 axis: rows of 32, MFMA steps of 16 rows, chunks of CS steps
     const uint64_t nsteps = ((uint64_t)n + 511) / 512;
-    const uint64_t nchunks = (nsteps + GKM_CS - 1) / GKM_CS;
-    const uint64_t PP = nchunks * GKM_CS * 16;
+    const uint64_t nchunks = (nsteps + CS - 1) / CS;
+    const uint64_t PP = nchunks * CS * 16;
     // lag axis: tiles of 32 (plus one for the negative diagonals)
     const uint64_t ntiles = (leff + 30) / 32 + 1;
-    const uint64_t ngroups = (ntiles + 16 * GKM_Q - 1) / (16 * GKM_Q) * 16;
-    const uint64_t qmax = ngroups / 16 * 16 * GKM_Q;  // tiles covered, >= ntiles
+    const uint64_t ngroups = (ntiles + 16 * Q - 1) / (16 * Q) * 16;
+    const uint64_t qmax = ngroups / 16 * 16 * Q;  // tiles covered, >= ntiles
     // highest Y row read: 16 (st + Q - 1) + 8 + 7 + qoff, st < nchunks * CS, qoff <= qmax - 16 Q + 8;
     // rounded up to whole split workgroups
-    const uint64_t YP = (PP + 16 * GKM_Q + qmax + 16 + 7) / GKM_SP * GKM_SP;
+    const uint64_t YP = (PP + 16 * Q + qmax + 16 + 7) / GKM_SP * GKM_SP;
     if (YP >= (1ull << 31) || nchunks > 65535)
         fail(ANL_ERR_ARG, "series too long for the MFMA form (use the FP64 form)");
     z.nchunks = (uint32_t)nchunks;
@@ -388,11 +400,11 @@ uint64_t gkm_need(const GkmSize &z) { return z.region + align256(z.partial * siz
 // Blocks [b0, b1) of one batch: at most GKM_MAXB blocks, 65,535 chunks and
 // GKM_BATCH_BYTES of scratch (a single block may exceed the latter).
 uint32_t gkm_batch_end(uint32_t ncur, uint32_t leff, const uint32_t *n, uint32_t nblocks, uint32_t b0,
-                       uint64_t *bytes) {
+                       const GkmShape &sh, uint64_t *bytes) {
     uint64_t tot = 0, chunks = 0;
     uint32_t b = b0;
     for (; b < nblocks && b - b0 < GKM_MAXB; b++) {
-        const GkmSize z = gkm_size(ncur, leff, n[b]);
+        const GkmSize z = gkm_size(ncur, leff, n[b], sh);
         if (b > b0 && (tot + gkm_need(z) > GKM_BATCH_BYTES || chunks + z.nchunks > 65535)) break;
         tot += gkm_need(z);
         chunks += z.nchunks;
@@ -406,7 +418,7 @@ uint32_t gkm_batch_end(uint32_t ncur, uint32_t leff, const uint32_t *n, uint32_t
 // elsewhere.
 void gkm_enqueue_batch(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
                        uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t b0, uint32_t b1, char *work,
-                       double *d, uint64_t stride, hipStream_t s) {
+                       double *d, uint64_t stride, const GkmShape &sh, hipStream_t s) {
     GkmBatch a{};
     a.data = sr->data;
     a.work = work;
@@ -426,7 +438,7 @@ void gkm_enqueue_batch(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uin
     uint32_t maxPP = 0, maxYP = 0, ngroups = 0;
     a.z0[0] = 0;
     for (uint32_t k = 0; k < a.nblk; k++) {
-        const GkmSize z = gkm_size(ncur, leff, n[b0 + k]);
+        const GkmSize z = gkm_size(ncur, leff, n[b0 + k], sh);
         a.primo[k] = primo[b0 + k];
         a.n[k] = n[b0 + k];
         a.PP[k] = z.PP;
@@ -451,7 +463,20 @@ void gkm_enqueue_batch(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uin
     for (uint32_t iseg = 0; iseg < n_seg; iseg++) {
         hipLaunchKernelGGL(gkm_split_kernel<false>, dim3(maxPP / GKM_SP, 1, a.nblk), dim3(256), lds_x, s, a, iseg);
         ANL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gkm_tile_kernel, dim3(ngroups / GKM_WG, ncorr, a.z0[a.nblk]), dim3(64 * GKM_WG), 0, s, a);
+        const dim3 tgrid(ngroups / sh.wg, ncorr, a.z0[a.nblk]);
+#define ANL_GKM_TILE(Q, CS, WG) hipLaunchKernelGGL((gkm_tile_kernel<Q, CS, WG>), tgrid, dim3(64 * WG), 0, s, a)
+        switch (sh.q * 100 + sh.cs * 10 + sh.wg) {
+            case 4 * 100 + 8 * 10 + 2: ANL_GKM_TILE(4, 8, 2); break;
+            case 4 * 100 + 8 * 10 + 4: ANL_GKM_TILE(4, 8, 4); break;
+            case 4 * 100 + 16 * 10 + 2: ANL_GKM_TILE(4, 16, 2); break;
+            case 4 * 100 + 16 * 10 + 4: ANL_GKM_TILE(4, 16, 4); break;
+            case 8 * 100 + 8 * 10 + 2: ANL_GKM_TILE(8, 8, 2); break;
+            case 8 * 100 + 8 * 10 + 4: ANL_GKM_TILE(8, 8, 4); break;
+            case 8 * 100 + 16 * 10 + 2: ANL_GKM_TILE(8, 16, 2); break;
+            case 8 * 100 + 16 * 10 + 4: ANL_GKM_TILE(8, 16, 4); break;
+            default: fail(ANL_ERR_ARG, "unsupported Green-Kubo MFMA launch shape");
+        }
+#undef ANL_GKM_TILE
         ANL_HIP(hipGetLastError());
         hipLaunchKernelGGL(gkm_reduce_kernel, rgrid, dim3(256), 0, s, a, iseg);
         ANL_HIP(hipGetLastError());
@@ -462,19 +487,24 @@ void gkm_enqueue_batch(anl_series *sr, const uint32_t *idx_j, uint32_t ncur, uin
 
 namespace anl {
 
-size_t gk_mfma_scratch_bytes(uint32_t ncur, uint32_t leff, uint32_t n) {
+bool gkm_shape_ok(uint32_t q, uint32_t cs, uint32_t wg) {
+    return (q == 4 || q == 8) && (cs == 8 || cs == 16) && (wg == 2 || wg == 4);
+}
+
+size_t gk_mfma_scratch_bytes(const anl_series *s, uint32_t ncur, uint32_t leff, uint32_t n) {
     uint64_t bytes = 0;
-    gkm_batch_end(ncur, leff, &n, 1, 0, &bytes);
+    gkm_batch_end(ncur, leff, &n, 1, 0, gkm_shape_of(s), &bytes);
     return bytes;
 }
 
 void gk_mfma_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,
                             uint32_t n_seg, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, double *d,
                             hipStream_t st) {
+    const GkmShape sh = gkm_shape_of(s);
     uint64_t need = 0;
     for (uint32_t b0 = 0; b0 < nblocks;) {
         uint64_t bytes = 0;
-        b0 = gkm_batch_end(ncur, leff, n, nblocks, b0, &bytes);
+        b0 = gkm_batch_end(ncur, leff, n, nblocks, b0, sh, &bytes);
         need = std::max(need, bytes);
     }
     char *work = (char *)series_workspace(s, need);
@@ -482,8 +512,8 @@ void gk_mfma_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur,
     // batches run one after the other on st, sharing the scratch
     for (uint32_t b0 = 0; b0 < nblocks;) {
         uint64_t bytes = 0;
-        const uint32_t b1 = gkm_batch_end(ncur, leff, n, nblocks, b0, &bytes);
-        gkm_enqueue_batch(s, idx_j, ncur, skip, leff, n_seg, primo, n, b0, b1, work, d, stride, st);
+        const uint32_t b1 = gkm_batch_end(ncur, leff, n, nblocks, b0, sh, &bytes);
+        gkm_enqueue_batch(s, idx_j, ncur, skip, leff, n_seg, primo, n, b0, b1, work, d, stride, sh, st);
         b0 = b1;
     }
 }

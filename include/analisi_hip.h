@@ -282,6 +282,23 @@ int anl_gk_calculate_mfma_async(anl_series *s, const uint32_t *idx_j, uint32_t n
                                 uint32_t tmax, uint32_t n, uint32_t n_seg, int subtract_mean,
                                 uint32_t start_mean, uint64_t primo, double *out_dev, void *stream);
 
+/* Launch shape of the split-BF16 MFMA lag products of this series: tiles of
+ * 32 x 32 (t0 x lag) per wavefront (4 or 8; default 8), MFMA steps of 512 t0
+ * per FP32 chunk before the sums go to FP64 (8 or 16; default 16), and
+ * wavefronts per workgroup sharing one X operand block (2 or 4; default 4).
+ * Tiles per wavefront and wavefronts per workgroup never change results; the
+ * chunk length changes the FP32 rounding within the MFMA form's error bound.
+ * anl_gk_autotune times each candidate (cand[3c..3c+2]; NULL = a built-in
+ * set of 6) on the lag products of the given blocks (reset(n[b]) at primo[b],
+ * as anl_gk_calculate_blocks_async) and keeps the fastest; ms_out[c] (may be
+ * NULL) receives the times.  The reference's only knob is nthreads
+ * (greenkuboNcomponentionicfluid.h:26-51); no counterpart for the shape. */
+int anl_gk_set_tuning(anl_series *s, uint32_t tiles_per_wave, uint32_t chunk_steps, uint32_t waves_per_group);
+int anl_gk_get_tuning(anl_series *s, uint32_t *tiles_per_wave, uint32_t *chunk_steps, uint32_t *waves_per_group);
+int anl_gk_autotune(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t tmax, uint32_t n_seg,
+                    const uint64_t *primo, const uint32_t *n, uint32_t nblocks, const uint32_t *cand, uint32_t ncand,
+                    float *ms_out, uint32_t *best_index);
+
 /* ---- block statistics --------------------------------------------------- */
 /* MediaVar::calculate (calcoliblocchi.h:35-58) on device vectors of length
  * len: one Welford step with x = block iblock (0-based); mean/var updated in

@@ -147,3 +147,36 @@ def test_gk_blocks_more_than_one_batch(mfma):
     for b in (0, 5, 63, 64, 69):
         one = s.gk([0], 1, tmax, n[b], 1, False, 0, primo[b], mfma=mfma)
         assert np.array_equal(np.nan_to_num(per[b], nan=0.25), np.nan_to_num(one, nan=0.25)), b
+
+
+def test_gk_mfma_shapes_and_autotune():
+    """Every launch shape of the MFMA lag products (tiles per wavefront 4|8,
+    chunk steps 8|16, wavefronts per workgroup 2|4): shapes with the same
+    chunk length give the same bits; every shape is exact on integer currents
+    and within 1e-6 of the FP64 oracle on AR(2) currents; the autotuner keeps
+    one of its candidates and the result is that shape's."""
+    ncur, skip, n_seg, tmax, n, primo = 2, 1, 2, 1300, 20000, 3
+    idx = [1, 4]
+    shapes = [(q, cs, wg) for q in (4, 8) for cs in (8, 16) for wg in (2, 4)]
+    for kind in ("int", "ar2"):
+        data = _series(kind, ncur, tmax + primo + n, 41)
+        ref = O.gk(data, idx, skip, tmax, False, 0, n_seg, n, primo)
+        sig = _scales(ref, ncur)
+        s = capi.DeviceSeries(data)
+        outs = {}
+        for sh in shapes:
+            s.set_gk_shape(*sh)
+            assert s.gk_shape() == sh
+            outs[sh] = s.gk(idx, skip, tmax, n, n_seg, False, 0, primo, mfma=True)
+            dC = np.abs(outs[sh][:, :4] - ref[:, :4]) / sig
+            assert dC.max() <= (1e-12 if kind == "int" else 1e-6), (kind, sh, dC.max())
+        same = lambda a, b: np.array_equal(np.nan_to_num(a, nan=0.5), np.nan_to_num(b, nan=0.5))
+        for sh in shapes:
+            base = (4, sh[1], 2)
+            assert same(outs[sh], outs[base]), (kind, sh)
+        ms, best = s.gk_autotune(idx, skip, tmax, n_seg, [primo, primo + 10], [n - 10, n - 10])
+        assert len(ms) == 6 and np.all(ms > 0)
+        chosen = s.gk_shape()
+        assert chosen in shapes
+        assert same(s.gk(idx, skip, tmax, n, n_seg, False, 0, primo, mfma=True), outs[chosen])
+        s.close()
