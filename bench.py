@@ -439,6 +439,12 @@ def run_msd(w, args, hip, traj, nt):
     out_len = leff * 2 * nt
     d = hip.malloc(out_len * 8)
     stream = hip.stream()
+    tune = None
+    if not args.no_autotune:
+        ms_t, best = traj.msd_autotune(1, tmax, True, True, n, 0)
+        tune = {"lag_tiles": [8, 12, 16, 24, 0], "candidates_ms": [round(float(x), 3) for x in ms_t],
+                "chosen_lag_tile": traj.msd_tile()}
+        log("msd autotune: %s -> lag tile %d" % (tune["candidates_ms"], traj.msd_tile()))
     check(L.anl_msd_calculate_async(traj.handle, 1, tmax, 1, 1, n, 0, d, stream))
     hip.sync()
     e0, e1 = hip.event(), hip.event()
@@ -452,7 +458,8 @@ def run_msd(w, args, hip, traj, nt):
     flop = MSD_FLOP + 3
     res = {"metric": "MSD atom-lag-t0 updates/s", "value": upd / (ms * 1e-3), "ms_per_step": ms,
            "config": "MSD lags 0..%d, reset(%d), centre-of-mass channel + drift removal, skip 1" % (leff - 1, n),
-           "roofline": {"bound": "fp64 valu", "achieved": upd * flop / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+           "autotune": tune,
+           "roofline": {"bound": "valu", "achieved": upd * flop / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": upd * flop / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                         "note": "%d FLOP per atom-lag-t0 update (3 sub, 3 mul, 2 add, 1 accumulate, "
                                 "3 drift)" % flop}}
@@ -616,6 +623,13 @@ def run_gk(args, hip):
         check(L.anl_gk_calculate_blocks_async(s.handle, pidx, 2, 1, lmax, 1, 0, 0, ctypes.c_void_p(pr.ctypes.data),
                                               ctypes.c_void_p(nn.ctypes.data), nb, form, out, mean, var, stream))
 
+    tune = None
+    if not args.no_autotune:
+        ms_t, best = s.gk_autotune([0, 3], 1, lmax, 1, starts, lens)
+        tune = {"candidates": [[8, 16, 4], [8, 16, 2], [4, 16, 4], [4, 16, 2], [8, 8, 4], [4, 8, 4]],
+                "candidates_ms": [round(float(x), 3) for x in ms_t], "chosen_shape": list(s.gk_shape()),
+                "shape": "tiles per wavefront, MFMA steps per FP32 chunk, wavefronts per workgroup"}
+        log("gk autotune: %s -> %s" % (tune["candidates_ms"], tune["chosen_shape"]))
     enqueue(capi.GK_MFMA, out_m, mean_m, var_m)     # sizes the series scratch outside the timed region
     hip.sync()
     e0, e1, e2 = hip.event(), hip.event(), hip.event()
@@ -662,7 +676,7 @@ def run_gk(args, hip):
            "config": {"workload": "GK 2 vector currents (6 FP64 columns), %d AR(2) steps, coupling 0.3, %d uneven "
                                   "blocks, %d lags, skip 1, n_seg 1" % (T, nb, lmax),
                       "block_lengths_min_max": [int(lens.min()), int(lens.max())], "algorithmic_flop": flop},
-           "ms_all_blocks_fp64": ms_f, "speedup_mfma_vs_fp64": ms_f / ms_m,
+           "ms_all_blocks_fp64": ms_f, "speedup_mfma_vs_fp64": ms_f / ms_m, "autotune": tune,
            "roofline": {"bound": "mfma", "achieved": 3 * flop / (ms_m * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": 3 * flop / (ms_m * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
                         "note": "3 bf16 MFMA products per algorithmic product; event-timed over one call for all "
