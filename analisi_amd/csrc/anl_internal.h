@@ -83,6 +83,8 @@ struct anl_traj {
     uint2 *frac16 = nullptr;            // [F][N] {a | b<<16, c} 16-bit fractions, lazily built
     uint4 *frac32 = nullptr;            // [F][N] {a, b, c, type | slot << 8} 32-bit fractions, lazily built
     int32_t *frac_order = nullptr;      // [F][N] slot held by each screen slot, until the fractions are built
+    uint4 *jgroups = nullptr;           // [F][ceil(N/32)] bounding spheres of 32 consecutive screen slots:
+                                        // {centre a | b << 16, centre c, radius (float bits, A), 0}
     std::vector<double> box_host;       // [F][9]
     std::vector<int32_t> type_of_slot;  // host copy
     std::vector<uint64_t> type_count;   // atoms per type

@@ -320,6 +320,11 @@ struct GofrtArgs {
     // largest usable cluster radius, quantisation bound, rmax
     float hx, hxy, hxz, hy, hyz, hz;
     float rho_cap, dq, rmaxf;
+    // bounding spheres of 32-slot groups of each frame ([F][ngroups]) and
+    // 2^16 / (smallest cell width), the fractional bound of a group's extent
+    const uint4 *jgroups;
+    uint32_t ngroups;
+    float inv_w16;
 };
 
 // The grid is one-dimensional.  Workgroups are dispatched round-robin over
@@ -708,17 +713,16 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     const uint2 *__restrict__ fj16 = a.frac + fj * N;
                     const uint32_t wbase = threadIdx.x & ~63u;
                     const uint32_t half4 = 4u * (lane >> 5);
-                    // every lane builds the column of j = j0 + lane; lanes 0-31's
-                    // columns form the B operand of the first 32 j, lanes 32-63's
-                    // (moved down) that of the next 32.  The next 64 j's 16-bit
-                    // fractions are loaded one step ahead.
                     const bool colane = lane < 32;
-                    uint2 gn = make_uint2(0u, 0u);
-                    if (lane < (uint32_t)N) gn = fj16[lane];
-                    for (uint32_t j0 = 0; j0 < (uint32_t)N; j0 += 64) {
-                        const uint2 g = gn;
-                        const bool jin = j0 + lane < (uint32_t)N;
-                        if (j0 + 64 + lane < (uint32_t)N) gn = fj16[j0 + 64 + lane];
+                    // One step: lanes 0-31 build the columns of j = jA + lane, lanes
+                    // 32-63 those of j = jB + lane - 32 (when hasB).  Lanes 0-31's
+                    // columns form the B operand of half 0, lanes 32-63's (moved
+                    // down) that of half 1.
+                    auto step = [&](uint32_t jA, uint32_t jB, bool hasB) {
+                        const uint32_t jl = (colane ? jA : jB) + (lane & 31u);
+                        const bool jin = (colane || hasB) && jl < (uint32_t)N;
+                        uint2 g = make_uint2(0u, 0u);
+                        if (jin) g = fj16[jl];
                         const float ga = (float)s16(g.x - ca), gb = (float)s16((g.x >> 16) - cb);
                         const float gc = (float)s16(g.y - cc);
                         float qx, qy, qz;
@@ -732,11 +736,9 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             qz = a.hz * gc;
                         }
                         const float q2 = fmaf(qz, qz, fmaf(qy, qy, qx * qx));
-                        // 32-column halves with no j near the cluster skip their
-                        // MFMAs and compares (most of them: the j slots are in k-d
-                        // order, so a half is a compact group of atoms too)
+                        // a half with no j near the cluster skips its MFMAs and compares
                         const unsigned long long nearm = __ballot(jin && q2 <= rs2);
-                        if (nearm == 0ull) continue;
+                        if (nearm == 0ull) return;
                         // column (q, 1, |q|^2); past N: (0, 0, 0, 1, sentinel)
                         f16x8_t col = {};
                         col[0] = (_Float16)(jin ? qx : 0.0f);
@@ -756,10 +758,11 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
 #pragma unroll
                         for (int jb = 0; jb < 2; jb++) {
                             if ((uint32_t)(nearm >> (32 * jb)) == 0u) continue;  // uniform
-                            ntile += nact * (unsigned long long)min(32u, (uint32_t)N - (j0 + 32u * (uint32_t)jb));
+                            const uint32_t jfirst = jb ? jB : jA;
+                            ntile += nact * (unsigned long long)min(32u, (uint32_t)N - jfirst);
                             // entry = j << 8 | local i; item e of tile T is row
                             // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
-                            const uint32_t eb = ((j0 + 32u * (uint32_t)jb + (lane & 31u)) << 8) | (wbase + half4);
+                            const uint32_t eb = ((jfirst + (lane & 31u)) << 8) | (wbase + half4);
                             // both row tiles' products are issued before either is
                             // compared, so one MFMA's latency hides behind the other's
                             const f32x16_t DT[2] = {
@@ -799,6 +802,51 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                     wave_sync_lds();
                                 }
                             }
+                        }
+                    };
+                    // Groups of 32 screen slots (= the halves) with a bounding
+                    // sphere (centre c_g, radius R_g) per frame.  With e_g =
+                    // wrap16(c_g - c): when every component satisfies |e_k| +
+                    // R_g 2^16 / w_min < 2^15, each member's wrapped offset is
+                    // e_g + its offset from c_g, so |q_j| >= |A e_g| - R_g; a group
+                    // with |A e_g| > rs + R_g (plus rounding margins) has no j
+                    // that the per-j test above would keep, and is skipped whole.
+                    // The groups left are taken two per step, in order.
+                    const uint4 *__restrict__ gt = a.jgroups + (uint64_t)fj * a.ngroups;
+                    for (uint32_t gbase = 0; gbase < a.ngroups; gbase += 64) {
+                        const uint32_t gi = gbase + lane;
+                        bool maybe = false;
+                        if (gi < a.ngroups) {
+                            const uint4 gv = gt[gi];
+                            const int ea = s16(gv.x - ca), eb_ = s16((gv.x >> 16) - cb), ec = s16(gv.y - cc);
+                            const float R = __uint_as_float(gv.z);
+                            const float lim = 32766.0f - R * a.inv_w16;
+                            const bool inside = fabsf((float)ea) < lim && fabsf((float)eb_) < lim && fabsf((float)ec) < lim;
+                            float gx, gy, gz;
+                            if (TRI) {
+                                gz = a.hz * (float)ec;
+                                gy = fmaf(a.hyz, (float)ec, a.hy * (float)eb_);
+                                gx = fmaf(a.hxz, (float)ec, fmaf(a.hxy, (float)eb_, a.hx * (float)ea));
+                            } else {
+                                gx = a.hx * (float)ea;
+                                gy = a.hy * (float)eb_;
+                                gz = a.hz * (float)ec;
+                            }
+                            const float g2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+                            const float far_r = (rs + R) * (1.0f + 0x1p-12f) + 1e-3f;
+                            maybe = !(inside && g2 > far_r * far_r);
+                        }
+                        unsigned long long M = __ballot(maybe);
+                        while (M) {  // uniform
+                            const uint32_t g1 = gbase + (uint32_t)__builtin_ctzll(M);
+                            M &= M - 1ull;
+                            uint32_t g2i = 0;
+                            const bool hasB = M != 0ull;
+                            if (hasB) {
+                                g2i = gbase + (uint32_t)__builtin_ctzll(M);
+                                M &= M - 1ull;
+                            }
+                            step(32u * g1, 32u * g2i, hasB);
                         }
                     }
                 }
@@ -1113,6 +1161,9 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         a.dq = (float)cp.dq;
         a.rmaxf = (float)cp.rmaxf;
         a.rho_cap = form == ANL_GOFRT_SCREENED ? (float)cp.rho_cap : 0.0f;
+        a.jgroups = tr->jgroups;
+        a.ngroups = (uint32_t)((tr->N + 31) / 32);
+        a.inv_w16 = form == ANL_GOFRT_SCREENED ? (float)(65536.0 / tr->screen_cache.wmin * (1 + 1e-6)) : 0.0f;
     }
     const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
     const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;

@@ -96,14 +96,58 @@ __global__ void frac_kernel(const double4 *__restrict__ pos, const int32_t *__re
     out32[k] = make_uint4(a, b, cc, (uint32_t)type_slot[sl] | (sl << 8));
 }
 
+static __device__ __forceinline__ int wrap16(uint32_t x) { return (int)(short)(x & 0xFFFFu); }
+
+// Bounding sphere of each group of 32 consecutive screen slots of a frame
+// (the last group may be shorter): a 16-bit fractional centre c_g (the
+// first member plus the rounded mean wrapped offset of the members) and the
+// radius R_g = max |A wrap16(f_j - c_g)| 2^-16 over the members, in FP64,
+// rounded up to float.  Groups are compact because each frame's screen slots
+// follow that frame's k-d order.
+__global__ void jgroup_kernel(const uint2 *__restrict__ frac16, uint64_t natoms, uint32_t ngroups, uint64_t total,
+                              Cell c, uint4 *__restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const uint64_t f = k / ngroups;
+    const uint32_t g = (uint32_t)(k - f * ngroups);
+    const uint2 *p = frac16 + f * natoms + 32ull * g;
+    const uint32_t n = (uint32_t)min<uint64_t>(32, natoms - 32ull * g);
+    const uint2 r = p[0];
+    int sa = 0, sb = 0, sc = 0;
+    for (uint32_t m = 0; m < n; m++) {
+        sa += wrap16(p[m].x - r.x);
+        sb += wrap16((p[m].x >> 16) - (r.x >> 16));
+        sc += wrap16(p[m].y - r.y);
+    }
+    const uint32_t ca = (r.x + (uint32_t)__double2int_rn((double)sa / n)) & 0xFFFFu;
+    const uint32_t cb = ((r.x >> 16) + (uint32_t)__double2int_rn((double)sb / n)) & 0xFFFFu;
+    const uint32_t cc = (r.y + (uint32_t)__double2int_rn((double)sc / n)) & 0xFFFFu;
+    double R2 = 0.0;
+    for (uint32_t m = 0; m < n; m++) {
+        const double da = wrap16(p[m].x - ca), db = wrap16((p[m].x >> 16) - cb), dc = wrap16(p[m].y - cc);
+        const double x = (c.L[0] * da + c.xy * db + c.xz * dc) * 0x1p-16;
+        const double y = (c.L[1] * db + c.yz * dc) * 0x1p-16;
+        const double z = (c.L[2] * dc) * 0x1p-16;
+        R2 = fmax(R2, x * x + y * y + z * z);
+    }
+    const float R = __double2float_ru(sqrt(R2) * (1.0 + 1e-9));
+    out[k] = make_uint4(ca | (cb << 16), cc, __float_as_uint(R), 0u);
+}
+
 void build_frac16(anl_traj *tr, hipStream_t s) {
     if (tr->frac16) return;
     uint64_t total = tr->F * tr->N;
     ANL_HIP(hipMalloc(&tr->frac16, total * sizeof(uint2)));
     ANL_HIP(hipMalloc(&tr->frac32, total * sizeof(uint4)));
+    const uint32_t ngroups = (uint32_t)((tr->N + 31) / 32);
+    const uint64_t gtotal = tr->F * ngroups;
+    ANL_HIP(hipMalloc(&tr->jgroups, gtotal * sizeof(uint4)));
     unsigned blocks = (unsigned)((total + 255) / 256);
     hipLaunchKernelGGL(frac_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->type_slot, tr->frac_order, tr->N,
                        tr->frac16, tr->frac32, total, tr->cell0);
+    ANL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(jgroup_kernel, dim3((unsigned)((gtotal + 255) / 256)), dim3(256), 0, s, tr->frac16, tr->N,
+                       ngroups, gtotal, tr->cell0, tr->jgroups);
     ANL_HIP(hipGetLastError());
     ANL_HIP(hipStreamSynchronize(s));
     (void)hipFree(tr->frac_order);  // the fractions hold the order from now on
@@ -386,6 +430,7 @@ int anl_traj_destroy(anl_traj *tr) {
         (void)hipFree(tr->cm);
         (void)hipFree(tr->frac16);
         (void)hipFree(tr->frac32);
+        (void)hipFree(tr->jgroups);
         delete tr;
     });
 }

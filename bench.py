@@ -851,17 +851,21 @@ def main():
     ap.add_argument("--no-gk", action="store_true", help="c3: leave out the Green-Kubo (c5) sub-record")
     ap.add_argument("--n", type=int, default=0, help="reset(n) of the timed run (0 = the workload's own)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only copy MIB mebibytes device-to-device (a plain copy kernel of known size, "
-                         "to scale the FETCH_SIZE/WRITE_SIZE counters) and exit")
+                    help="only run a plain streaming kernel that reads and writes MIB mebibytes (to scale the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.calibrate_copy:
+        # a plain streaming kernel of known traffic: anl_counts_to_double reads
+        # one uint64 and writes one double per element (MIB MiB each way)
+        from analisi_amd import capi
         hip = Hip()
+        capi.check(capi.lib().anl_init(0))
         nb = args.calibrate_copy << 20
         src, dst = hip.malloc(nb), hip.malloc(nb)
         for _ in range(3):
-            hip.h.hipMemcpyDtoD(dst, src, ctypes.c_size_t(nb))
+            capi.check(capi.lib().anl_counts_to_double_async(src, 1.0, nb // 8, dst, None))
         hip.sync()
-        print(json.dumps({"copy_bytes": nb, "copies": 3}))
+        print(json.dumps({"kernel": "counts_to_double_kernel", "read_bytes": nb, "write_bytes": nb, "launches": 3}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: analisi_amd has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
