@@ -26,8 +26,25 @@ __device__ __forceinline__ DevCell load_cell(const double *b) {
 // then x; a z step also shifts y by yz and x by xz, a y step shifts x by xy.
 // A step "if (d<0) d += 2l else d -= 2l" is written d -= s*2l with s = +-1
 // (exact products; x - (-y) == x + y in IEEE arithmetic).
+// The first iteration of each loop is written as straight-line code whose
+// result each lane keeps only if it needs the step (the loop condition),
+// so a wavefront whose lanes disagree does not branch; the loop then runs
+// the further steps, which a component within 1.5 cell lengths never needs.
+// Every lane performs the reference's operations in the reference's order
+// either way.
 template <bool TRI>
 __device__ __forceinline__ void min_image_ref(double &x0, double &x1, double &x2, const DevCell &c) {
+    {
+        const bool t = fabs(x2) > c.lh2;
+        const double s = x2 < 0.0 ? -1.0 : 1.0;
+        const double n2 = x2 - s * c.L2;
+        x2 = t ? n2 : x2;
+        if (TRI) {
+            const double n1 = x1 - s * c.yz, n0 = x0 - s * c.xz;
+            x1 = t ? n1 : x1;
+            x0 = t ? n0 : x0;
+        }
+    }
     // Inputs are validated on the host (finite, within 10^4 cell lengths), so
     // each loop ends after a few steps; the counters only guard the device.
     int guard = 0;
@@ -39,10 +56,26 @@ __device__ __forceinline__ void min_image_ref(double &x0, double &x1, double &x2
             x0 = x0 - s * c.xz;
         }
     }
+    {
+        const bool t = fabs(x1) > c.lh1;
+        const double s = x1 < 0.0 ? -1.0 : 1.0;
+        const double n1 = x1 - s * c.L1;
+        x1 = t ? n1 : x1;
+        if (TRI) {
+            const double n0 = x0 - s * c.xy;
+            x0 = t ? n0 : x0;
+        }
+    }
     while (fabs(x1) > c.lh1 && ++guard < (2 << 20)) {
         double s = x1 < 0.0 ? -1.0 : 1.0;
         x1 = x1 - s * c.L1;
         if (TRI) x0 = x0 - s * c.xy;
+    }
+    {
+        const bool t = fabs(x0) > c.lh0;
+        const double s = x0 < 0.0 ? -1.0 : 1.0;
+        const double n0 = x0 - s * c.L0;
+        x0 = t ? n0 : x0;
     }
     while (fabs(x0) > c.lh0 && ++guard < (3 << 20)) {
         double s = x0 < 0.0 ? -1.0 : 1.0;
