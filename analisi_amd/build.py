@@ -17,7 +17,8 @@ import sysconfig
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
-HIP_SOURCES = ["runtime.hip", "gofrt.hip", "msd.hip", "gk.hip", "gk_mfma.hip", "blocks.hip", "sph.hip"]
+HIP_SOURCES = ["runtime.hip", "gofrt.hip", "msd.hip", "msd_noalloc.hip", "gk.hip", "gk_mfma.hip", "blocks.hip",
+               "sph.hip"]
 LIB = os.path.join(PKG, "libanalisi_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

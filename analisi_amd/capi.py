@@ -61,6 +61,7 @@ SIGNATURES = [
     ("anl_blocks_destroy", _i, [_vp]),
     ("anl_msd_calculate", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp]),
     ("anl_msd_calculate_async", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
+    ("anl_msd_calculate_noalloc", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
     ("anl_msd_set_tuning", _i, [_vp, _u32]),
     ("anl_msd_get_tuning", _i, [_vp, ctypes.POINTER(_u32)]),
     ("anl_msd_autotune", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _u32, _vp, ctypes.POINTER(_u32)]),

@@ -131,6 +131,12 @@ void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n,
                          uint32_t tmax, int cm_msd, int cm_self, uint32_t *leff_out);
 void msd_enqueue_prepared(void *prep, double *out, hipStream_t s);
 void msd_free_prepared(void *prep);
+// One MSD calculation (reset(n) + calculate(primo)) with its block table,
+// term counts and partial sums in caller-provided device memory; fails with
+// ANL_ERR_ARG when they do not fit in scratch_bytes.  Allocates nothing.
+void msd_calculate_in_scratch(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self, uint32_t n,
+                              uint64_t primo, double *out, unsigned char *scratch, size_t scratch_bytes,
+                              hipStream_t s);
 // Scratch of at least bytes in s->work (256-byte aligned sub-buffers are
 // carved by the caller); growing synchronises the device first.
 void *series_workspace(anl_series *s, size_t bytes);

@@ -240,19 +240,24 @@ struct MsdPrepared {
     uint32_t skip = 1, leff = 0, nblocks = 0, nt = 0, fcm = 1, nchunks = 0;
     uint32_t tile = 0;  // lag tile R of msd_window_kernel; 0: msd_atoms_kernel
     bool cm_msd = false, cm_self = false;
+    bool owns = true;   // the three buffers were allocated for this launch
     MsdBlock *d_blocks = nullptr;
     uint64_t *d_count = nullptr;
     double *d_partial = nullptr;
     ~MsdPrepared() {
+        if (!owns) return;
         (void)hipFree(d_blocks);
         (void)hipFree(d_count);
         (void)hipFree(d_partial);
     }
 };
 
-// msd.cpp:52-61 checks (leff is common to all blocks: min(n_b, tmax) must agree)
-void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, uint32_t skip,
-                         uint32_t tmax, int cm_msd, int cm_self, uint32_t *leff_out) {
+// msd.cpp:52-61 checks (leff is common to all blocks: min(n_b, tmax) must
+// agree); fills P's sizes and the host tables of the block list and the
+// term counts.  Touches no device memory.
+static void msd_plan(anl_traj *tr, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, uint32_t skip,
+                     uint32_t tmax, int cm_msd, int cm_self, MsdPrepared *P, std::vector<MsdBlock> *hb,
+                     std::vector<uint64_t> *hc) {
     if (nblocks == 0) fail(ANL_ERR_ARG, "no blocks");
     if (skip == 0) skip = 1;
     const uint32_t leff = (n[0] < tmax || tmax == 0) ? n[0] : tmax;  // msd.cpp:42
@@ -269,37 +274,49 @@ void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n,
                                          " timesteps to do the requested calculation");
     }
     if ((cm_msd || cm_self) && !tr->cm) fail(ANL_ERR_ARG, "centre-of-mass series not provided");
-    ensure_device();
+    P->tr = tr;
+    P->skip = skip;
+    P->leff = leff;
+    P->nblocks = nblocks;
+    P->nt = tr->nt;
+    P->fcm = cm_msd ? 2 : 1;
+    P->cm_msd = cm_msd;
+    P->cm_self = cm_self;
+    // the window kernel needs t0 stride 1; its chunks are 64 atoms
+    P->tile = skip == 1 ? tr->msd_tile : 0;
+    P->nchunks = (uint32_t)(P->tile ? (tr->N + 63) / 64 : (tr->N + MSD_WG - 1) / MSD_WG);
+    hb->assign(nblocks, MsdBlock{});
+    hc->assign((size_t)nblocks * tr->nt, 0);
+    for (uint32_t b = 0; b < nblocks; b++) {
+        (*hb)[b] = MsdBlock{primo[b], n[b], 0};
+        const uint64_t n0 = (n[b] + skip - 1) / skip;
+        for (uint32_t k = 0; k < tr->nt; k++) (*hc)[(size_t)b * tr->nt + k] = n0 * tr->type_count[k];
+    }
+}
+
+// partial sums of a planned launch: [nblocks][leff][nchunks][nt] doubles
+static size_t msd_partial_len(const MsdPrepared &P) {
+    return std::max<size_t>(1, (size_t)P.nblocks * P.leff * P.nchunks * P.nt);
+}
+
+void *msd_prepare_blocks(anl_traj *tr, const uint64_t *primo, const uint32_t *n, uint32_t nblocks, uint32_t skip,
+                         uint32_t tmax, int cm_msd, int cm_self, uint32_t *leff_out) {
     auto *P = new MsdPrepared();
     try {
-        P->tr = tr;
-        P->skip = skip;
-        P->leff = leff;
-        P->nblocks = nblocks;
-        P->nt = tr->nt;
-        P->fcm = cm_msd ? 2 : 1;
-        P->cm_msd = cm_msd;
-        P->cm_self = cm_self;
-        // the window kernel needs t0 stride 1; its chunks are 64 atoms
-        P->tile = skip == 1 ? tr->msd_tile : 0;
-        P->nchunks = (uint32_t)(P->tile ? (tr->N + 63) / 64 : (tr->N + MSD_WG - 1) / MSD_WG);
-        std::vector<MsdBlock> hb(nblocks);
-        std::vector<uint64_t> hc((size_t)nblocks * tr->nt);
-        for (uint32_t b = 0; b < nblocks; b++) {
-            hb[b] = MsdBlock{primo[b], n[b], 0};
-            const uint64_t n0 = (n[b] + skip - 1) / skip;
-            for (uint32_t k = 0; k < tr->nt; k++) hc[(size_t)b * tr->nt + k] = n0 * tr->type_count[k];
-        }
+        std::vector<MsdBlock> hb;
+        std::vector<uint64_t> hc;
+        msd_plan(tr, primo, n, nblocks, skip, tmax, cm_msd, cm_self, P, &hb, &hc);
+        ensure_device();
         ANL_HIP(hipMalloc(&P->d_blocks, nblocks * sizeof(MsdBlock)));
         ANL_HIP(hipMemcpy(P->d_blocks, hb.data(), nblocks * sizeof(MsdBlock), hipMemcpyHostToDevice));
         ANL_HIP(hipMalloc(&P->d_count, hc.size() * sizeof(uint64_t)));
         ANL_HIP(hipMemcpy(P->d_count, hc.data(), hc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-        ANL_HIP(hipMalloc(&P->d_partial, std::max<size_t>(1, (size_t)nblocks * leff * P->nchunks * P->nt) * sizeof(double)));
+        ANL_HIP(hipMalloc(&P->d_partial, msd_partial_len(*P) * sizeof(double)));
     } catch (...) {
         delete P;
         throw;
     }
-    if (leff_out) *leff_out = leff;
+    if (leff_out) *leff_out = P->leff;
     return P;
 }
 
@@ -352,6 +369,34 @@ void msd_enqueue_prepared(void *prep, double *out, hipStream_t s) {
 }
 
 void msd_free_prepared(void *prep) { delete (MsdPrepared *)prep; }
+
+// One MSD calculation whose block table, term counts and partial sums live
+// in caller-provided device memory (scratch, scratch_bytes, 256-byte
+// aligned); allocates nothing.  The tables are copied in on stream s, which
+// is synchronised before returning (the host tables are locals).
+void msd_calculate_in_scratch(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self, uint32_t n,
+                              uint64_t primo, double *out, unsigned char *scratch, size_t scratch_bytes,
+                              hipStream_t s) {
+    MsdPrepared P;
+    P.owns = false;
+    std::vector<MsdBlock> hb;
+    std::vector<uint64_t> hc;
+    msd_plan(tr, &primo, &n, 1, skip, tmax, cm_msd, cm_self, &P, &hb, &hc);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_blocks = al(sizeof(MsdBlock)), b_count = al(hc.size() * sizeof(uint64_t));
+    const size_t b_partial = al(msd_partial_len(P) * sizeof(double));
+    if (b_blocks + b_count + b_partial > scratch_bytes)
+        fail(ANL_ERR_ARG, "MSD of " + std::to_string(tr->N) + " atoms and " + std::to_string(P.leff) + " lags needs " +
+                              std::to_string(b_blocks + b_count + b_partial) + " bytes of scratch, more than the " +
+                              std::to_string(scratch_bytes) + " of the allocation-free entry point");
+    P.d_blocks = reinterpret_cast<MsdBlock *>(scratch);
+    P.d_count = reinterpret_cast<uint64_t *>(scratch + b_blocks);
+    P.d_partial = reinterpret_cast<double *>(scratch + b_blocks + b_count);
+    ANL_HIP(hipMemcpyAsync(P.d_blocks, hb.data(), sizeof(MsdBlock), hipMemcpyHostToDevice, s));
+    ANL_HIP(hipMemcpyAsync(P.d_count, hc.data(), hc.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    msd_enqueue_prepared(&P, out, s);
+    ANL_HIP(hipStreamSynchronize(s));
+}
 
 }  // namespace anl
 

@@ -218,6 +218,19 @@ int anl_msd_calculate(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, in
                       uint32_t n, uint64_t primo, double *out_host);
 int anl_msd_calculate_async(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self,
                             uint32_t n, uint64_t primo, double *out_dev, void *stream);
+/* Allocation-free variant of anl_msd_calculate_async, with the same
+ * arguments and results (bit for bit: the same kernels).  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its scratch (block table, term counts,
+ * partial sums: about 8 * leff * ceil(natoms/64) * ntypes bytes at t0
+ * stride 1, ceil(natoms/256) at other strides) lives in a fixed 16 MiB
+ * device array per GPU, held until the process ends; larger calculations
+ * fail with ANL_ERR_ARG before anything is launched.  The call synchronises
+ * stream before returning.  Calls that overlap on one GPU share the array
+ * and race. */
+int anl_msd_calculate_noalloc(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self, uint32_t n,
+                              uint64_t primo, double *out_dev, void *stream);
 /* Launch shape of the MSD kernel for this trajectory: the lag tile R (8, 12,
  * 16 or 24; default 16) of the t0-stride-1 kernel, which holds R frames of
  * an atom in registers and loads two frames per t0 for R updates; 0 selects
