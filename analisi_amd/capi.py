@@ -232,8 +232,10 @@ class DeviceGofrt:
         check(lib().anl_gofrt_cluster_stats(self.handle, ctypes.byref(c), ctypes.byref(t)))
         return c.value, t.value
 
-    def set_tuning(self, unroll, xcd, pairs_per_wg, cluster=True):
-        check(lib().anl_gofrt_set_tuning(self.handle, unroll, int(bool(xcd)), pairs_per_wg, int(bool(cluster))))
+    def set_tuning(self, unroll, xcd, pairs_per_wg, cluster=2):
+        """cluster: 0 off; 1 cluster screen over the groups the bounding-sphere table leaves; 2 the same with
+        the step's columns cached in LDS; 3 cluster screen over every j in order."""
+        check(lib().anl_gofrt_set_tuning(self.handle, unroll, int(bool(xcd)), pairs_per_wg, int(cluster)))
 
     def tuning(self):
         """(unroll, xcd, pairs per workgroup, cluster) of the pair kernel's launch shape."""

@@ -60,6 +60,7 @@ struct Screen {
     double row_sum;        // max over rows of the cell matrix of sum |A_ik|
     double wmin;           // smallest cell width (distance between opposite faces)
     double fro;            // Frobenius norm of the cell matrix (>= its spectral norm)
+    double smin;           // sqrt of the smallest eigenvalue of A^T A (|A x| >= smin |x|)
 };
 
 }  // namespace anl
@@ -83,6 +84,7 @@ struct anl_traj {
     uint2 *frac16 = nullptr;            // [F][N] {a | b<<16, c} 16-bit fractions, lazily built
     uint4 *frac32 = nullptr;            // [F][N] {a, b, c, type | slot << 8} 32-bit fractions, lazily built
     int32_t *frac_order = nullptr;      // [F][N] slot held by each screen slot, until the fractions are built
+    double4 *pos_scr = nullptr;         // [F][N] positions in each frame's screen order (with the fractions)
     uint4 *jgroups = nullptr;           // [F][ceil(N/32)] bounding spheres of 32 consecutive screen slots:
                                         // {centre a | b << 16, centre c, radius (float bits, A), 0}
     std::vector<double> box_host;       // [F][9]

@@ -127,7 +127,7 @@ static double min_eig_sym3(double G[3][3]) {
 //      coordinate moves each component of q by at most 1 unit, so
 //      d <= rmax  =>  |q| <= rmax 2^16 / sqrt(lambda_min) + sqrt(3).
 static Screen make_screen(const Cell &c, bool box_const, uint32_t nt, double rmax) {
-    Screen s{false, 0, "", 0.5, 0.0, 0.0, 0.0};
+    Screen s{false, 0, "", 0.5, 0.0, 0.0, 0.0, 0.0};
     if (!box_const) {
         s.why_not = "the cell changes along the trajectory";
         return s;
@@ -163,6 +163,7 @@ static Screen make_screen(const Cell &c, bool box_const, uint32_t nt, double rma
         s.why_not = "degenerate cell";
         return s;
     }
+    s.smin = std::sqrt(lam);
     double R = r * 65536.0 / std::sqrt(lam) + std::sqrt(3.0) * 1.0001 + 1e-3;
     double R2 = std::ceil(R * R * (1 + 1e-9));
     if (!(R2 < 4294967295.0)) {
@@ -293,6 +294,7 @@ struct WorkItem {
 struct GofrtArgs {
     const double4 *pos;
     const uint2 *frac;
+    const double4 *pos_scr;      // screened form: positions in each frame's screen order
     const double *box;
     const int32_t *type_slot;
     const int32_t *orig_slot;
@@ -325,6 +327,7 @@ struct GofrtArgs {
     const uint4 *jgroups;
     uint32_t ngroups;
     float inv_w16;
+    float smin16;  // sqrt(smallest eigenvalue of A^T A) times 2^-16
 };
 
 // The grid is one-dimensional.  Workgroups are dispatched round-robin over
@@ -366,10 +369,15 @@ __device__ __forceinline__ void lds_setup(const GofrtArgs &a, unsigned *hist, do
     for (int k = threadIdx.x; k < (int)(a.nt * a.nt); k += WG) chan[k] = a.chan[k];
 }
 
+// Workgroups of one (block, lag) flush into the same counters, often at the
+// same time (they start and end together): each starts at its own offset,
+// so that their atomics meet different addresses.
 __device__ __forceinline__ void lds_flush(const GofrtArgs &a, const unsigned *hist, uint32_t block, uint32_t t) {
     const int nh = (int)(a.nch * a.nbin);
     unsigned long long *o = a.out + (uint64_t)block * a.out_block_stride + (uint64_t)t * nh;
-    for (int k = threadIdx.x; k < nh; k += WG) {
+    const int rot = (int)((blockIdx.x * 977u) % (uint32_t)nh);
+    for (int k0 = threadIdx.x; k0 < nh; k0 += WG) {
+        const int k = k0 + rot < nh ? k0 + rot : k0 + rot - nh;
         unsigned v = hist[k];
         if (v) atomicAdd(o + k, (unsigned long long)v);
     }
@@ -468,12 +476,10 @@ __device__ __forceinline__ uint32_t lane_slot(unsigned long long m, uint32_t bas
 // du_margin is four times a worst-case bound of |u_fp32 - u_reference|, so
 // outside the window both floor to the same bin.
 template <bool TRI>
-__device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds,
-                                               const uint4 *__restrict__ fj32, const int *chan, unsigned *hist) {
+__device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 uj,
+                                               const int *chan, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
-    const uint32_t j = e >> 8;
-    const uint4 ui = fi_lds[il];  // .w = type | slot << 8 of i
-    const uint4 uj = fj32[j];     // .w = type | slot << 8 of j
+    const uint4 ui = fi_lds[il];  // .w = type | slot << 8 of i; uj likewise of j
     const float fa = (float)(int)(ui.x - uj.x);
     const float fb = (float)(int)(ui.y - uj.y);
     const float fc = (float)(int)(ui.z - uj.z);
@@ -499,7 +505,9 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
 }
 
 // Exact FP64 evaluation (the reference's sequence) of a candidate whose FP32
-// distance fell inside the window around a bin edge, rmin or rmax.
+// distance fell inside the window around a bin edge, rmin or rmax.  pi and pj
+// are the positions of the two frames in screen order (pi from the tile's
+// first slot), the order of the entry's indices.
 template <bool TRI>
 __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds,
                                                   const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
@@ -508,8 +516,8 @@ __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
     const uint32_t wi = fi_lds[il].w, wj = fj32[j].w;  // type | slot << 8
-    const double4 xi = pi[wi >> 8];
-    const double4 xj = pj[wj >> 8];
+    const double4 xi = pi[il];
+    const double4 xj = pj[j];
     // the t0 frame's cell is read here, on this rare path, rather than held
     // in registers across the whole pair loop
     const DevCell c = load_cell(cell);
@@ -523,6 +531,7 @@ __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e
 
 constexpr int SQCAP = 576;   // per-wavefront queue: < 64 left + up to 8 masks of 64
 constexpr int SQCAP_CL = 1088;  // cluster screen: < 64 left + 16 masks of 64 (one drain per 32 x 32 tile)
+constexpr int SQCAP_CLC = 576;  // cluster screen with the LDS column cache: < 64 left + 8 masks (a drain per 8)
 constexpr int RQCAP = 128;   // per-wavefront recheck queue: < 64 left + one batch
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -566,8 +575,13 @@ __host__ __device__ __forceinline__ float cluster_threshold(float rho, float rma
         qn += (uint32_t)__popcll(M);                                        \
     }
 
-template <bool TRI, int UNROLL, bool CLUSTER>
+// CMODE, the cluster screen's j loop: 1 the groups left by the bounding-
+// sphere test, two per step; 2 the same with each step's column fractions in
+// LDS for the candidates' evaluation; 3 every 64 consecutive j per step, the
+// next step's fractions loaded ahead (no group table).  0 without CLUSTER.
+template <bool TRI, int UNROLL, bool CLUSTER, int CMODE>
 __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(const GofrtArgs a) {
+    constexpr bool COLC = CMODE == 2;
     extern __shared__ unsigned char smem[];
     const WgId id = decode_wg(a.ntiles, a.nlag, a.nwork, a.per_xcd);
     if (!id.valid) return;
@@ -576,8 +590,11 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     const uint32_t t = a.lag0 + id.lagi * a.every;
     if (t >= b.leff) return;
     uint4 *fi_lds = (uint4 *)smem;                          // [WG]
-    constexpr int QC = CLUSTER ? SQCAP_CL : SQCAP;
-    uint32_t *queue = (uint32_t *)(fi_lds + WG);            // [WG/64][QC]
+    // cluster screen: the 32-bit fractions of the current step's 64 columns,
+    // per wavefront, read by the candidate evaluation instead of global memory
+    uint4 *colc_all = fi_lds + WG;                          // [WG/64][64] (COLC)
+    constexpr int QC = COLC ? SQCAP_CLC : (CLUSTER ? SQCAP_CL : SQCAP);
+    uint32_t *queue = (uint32_t *)(fi_lds + WG + (COLC ? WG : 0));  // [WG/64][QC]
     uint32_t *rqueue = queue + (WG / 64) * QC;              // [WG/64][RQCAP]
     double *thr = (double *)(rqueue + (WG / 64) * RQCAP);   // [nbin+1]
     int *chan = (int *)(thr + a.nbin + 1);                  // [nt*nt]
@@ -606,8 +623,8 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         const uint64_t fi = b.primo + (uint64_t)k * a.skip;
         const uint64_t fj = fi + t;
         const double *cellp = a.box + fi * 9;
-        const double4 *__restrict__ pi = a.pos + fi * N;
-        const double4 *__restrict__ pj = a.pos + fj * N;
+        const double4 *__restrict__ pi = a.pos_scr + fi * N + slot0;
+        const double4 *__restrict__ pj = a.pos_scr + fj * N;
         const uint4 *__restrict__ fj32 = a.frac32 + fj * N;
         cu64_t *fjc = (cu64_t *)(a.frac + fj * N);
         __syncthreads();  // previous t0's candidates are done with fi_lds
@@ -635,8 +652,8 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         uint32_t qn = 0, rn = 0;
         // evaluate the queued entry of this lane; candidates inside the edge
         // window go to the recheck queue, evaluated 64 at a time
-        auto eval_batch = [&](uint32_t e, bool valid) {
-            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, fj32, chan, hist);
+        auto eval_with = [&](uint32_t e, bool valid, const uint4 uj) {
+            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, uj, chan, hist);
             const unsigned long long mr = __ballot(need);
             if (mr) {
                 if (need) rq[rn + lane_prefix(mr)] = e;
@@ -649,6 +666,10 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     wave_sync_lds();
                 }
             }
+        };
+        // the j atom's fractions from global memory (16-bit screen, final drain)
+        auto eval_batch = [&](uint32_t e, bool valid) {
+            eval_with(e, valid, valid ? fj32[e >> 8] : make_uint4(0u, 0u, 0u, 0u));
         };
         bool clustered = false;
         if constexpr (CLUSTER) {
@@ -711,6 +732,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     const f16x8_t A0 = __builtin_bit_cast(f16x8_t, lane < 32 ? rw : z4);
                     const f16x8_t A1 = __builtin_bit_cast(f16x8_t, lane < 32 ? up : z4);
                     const uint2 *__restrict__ fj16 = a.frac + fj * N;
+                    uint4 *colc = colc_all + (threadIdx.x & ~63u);
                     const uint32_t wbase = threadIdx.x & ~63u;
                     const uint32_t half4 = 4u * (lane >> 5);
                     const bool colane = lane < 32;
@@ -718,11 +740,10 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     // 32-63 those of j = jB + lane - 32 (when hasB).  Lanes 0-31's
                     // columns form the B operand of half 0, lanes 32-63's (moved
                     // down) that of half 1.
-                    auto step = [&](uint32_t jA, uint32_t jB, bool hasB) {
+                    // g: the lane's column's 16-bit fractions (zero when !jin)
+                    auto step = [&](uint32_t jA, uint32_t jB, bool hasB, const uint2 g) {
                         const uint32_t jl = (colane ? jA : jB) + (lane & 31u);
                         const bool jin = (colane || hasB) && jl < (uint32_t)N;
-                        uint2 g = make_uint2(0u, 0u);
-                        if (jin) g = fj16[jl];
                         const float ga = (float)s16(g.x - ca), gb = (float)s16((g.x >> 16) - cb);
                         const float gc = (float)s16(g.y - cc);
                         float qx, qy, qz;
@@ -739,6 +760,23 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         // a half with no j near the cluster skips its MFMAs and compares
                         const unsigned long long nearm = __ballot(jin && q2 <= rs2);
                         if (nearm == 0ull) return;
+                        // COLC: this step's columns' 32-bit fractions go to LDS for
+                        // the candidate evaluation (the queue is empty here: each
+                        // step drains it); otherwise candidates load them from
+                        // global memory
+                        if constexpr (COLC) {
+                            colc[lane] = jin ? fj32[jl] : make_uint4(0u, 0u, 0u, 0u);
+                            wave_sync_lds();
+                        }
+                        auto eval_step = [&](uint32_t e, bool valid) {
+                            if constexpr (COLC) {
+                                const uint32_t jj = e >> 8;
+                                const uint32_t c = jj - jA < 32u ? jj - jA : 32u + (jj - jB);
+                                eval_with(e, valid, colc[valid ? c : 0u]);
+                            } else {
+                                eval_batch(e, valid);
+                            }
+                        };
                         // column (q, 1, |q|^2); past N: (0, 0, 0, 1, sentinel)
                         f16x8_t col = {};
                         col[0] = (_Float16)(jin ? qx : 0.0f);
@@ -790,18 +828,43 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                                                   (uint32_t)(e & 3);
                                             ANL_APPEND_E(pc[u], mk[u], eb + roff)
                                         }
+                                        // with the column cache, one drain site per 8 compares
+                                        // (the queue holds < 64 + 8 masks)
+                                        if (COLC) {
+                                            while (qn >= 64) {
+                                                wave_sync_lds();
+                                                qn -= 64;
+                                                const uint32_t e = q[qn + lane];
+                                                ncand += 1;
+                                                eval_step(e, true);
+                                                wave_sync_lds();
+                                            }
+                                        }
                                     }
                                 }
-                                // one drain site per 32 x 32 tile (queue holds < 64 + 16 masks)
-                                while (qn >= 64) {
-                                    wave_sync_lds();
-                                    qn -= 64;
-                                    const uint32_t e = q[qn + lane];
-                                    ncand += 1;
-                                    eval_batch(e, true);
-                                    wave_sync_lds();
+                                // otherwise one per 32 x 32 tile (< 64 + 16 masks)
+                                if (!COLC) {
+                                    while (qn >= 64) {
+                                        wave_sync_lds();
+                                        qn -= 64;
+                                        const uint32_t e = q[qn + lane];
+                                        ncand += 1;
+                                        eval_step(e, true);
+                                        wave_sync_lds();
+                                    }
                                 }
                             }
+                        }
+                        // the step's last entries (< 64), so that the next step
+                        // can replace the cached columns
+                        if (COLC && qn) {
+                            wave_sync_lds();
+                            const bool valid = lane < qn;
+                            const uint32_t e = valid ? q[lane] : 0u;
+                            ncand += valid ? 1 : 0;
+                            eval_step(e, valid);
+                            qn = 0;
+                            wave_sync_lds();
                         }
                     };
                     // Groups of 32 screen slots (= the halves) with a bounding
@@ -811,9 +874,16 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     // e_g + its offset from c_g, so |q_j| >= |A e_g| - R_g; a group
                     // with |A e_g| > rs + R_g (plus rounding margins) has no j
                     // that the per-j test above would keep, and is skipped whole.
+                    // For any group, with or without that condition, each
+                    // component of a member's wrapped offset is at least |e_k| -
+                    // R_g 2^16 / w_min in magnitude (a member that wraps to the
+                    // other image lies at least 2^15 - R_g 2^16 / w_min away), so
+                    // |q_j| >= sqrt(lambda_min(A^T A)) |w| with w those lower
+                    // bounds; a group whose bound exceeds rs is skipped too.
                     // The groups left are taken two per step, in order.
                     const uint4 *__restrict__ gt = a.jgroups + (uint64_t)fj * a.ngroups;
-                    for (uint32_t gbase = 0; gbase < a.ngroups; gbase += 64) {
+                    // near mask of the 64 groups from gbase (one lane per group)
+                    auto window_mask = [&](uint32_t gbase) -> unsigned long long {
                         const uint32_t gi = gbase + lane;
                         bool maybe = false;
                         if (gi < a.ngroups) {
@@ -834,19 +904,52 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             }
                             const float g2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
                             const float far_r = (rs + R) * (1.0f + 0x1p-12f) + 1e-3f;
-                            maybe = !(inside && g2 > far_r * far_r);
+                            // whatever image a member takes, each component of its
+                            // wrapped offset is at least |e_k| - (its offset from c_g)
+                            const float dl = R * a.inv_w16 + 2.0f;
+                            const float la = fmaxf(fabsf((float)ea) - dl, 0.0f), lb = fmaxf(fabsf((float)eb_) - dl, 0.0f);
+                            const float lc = fmaxf(fabsf((float)ec) - dl, 0.0f);
+                            const float w2 = fmaf(lc, lc, fmaf(lb, lb, la * la)) * (a.smin16 * a.smin16);
+                            const float far_w = rs * (1.0f + 0x1p-12f) + 1e-3f;
+                            maybe = !((inside && g2 > far_r * far_r) || w2 > far_w * far_w);
                         }
-                        unsigned long long M = __ballot(maybe);
-                        while (M) {  // uniform
-                            const uint32_t g1 = gbase + (uint32_t)__builtin_ctzll(M);
+                        return __ballot(maybe);
+                    };
+                    // the groups left, two per step in order (uniform state)
+                    uint32_t gbase = 0;
+                    unsigned long long M = CMODE != 3 ? window_mask(0) : 0ull;
+                    auto next_pair = [&](uint32_t &jA, uint32_t &jB, bool &hasB) -> bool {
+                        while (M == 0ull) {
+                            gbase += 64;
+                            if (gbase >= a.ngroups) return false;
+                            M = window_mask(gbase);
+                        }
+                        jA = 32u * (gbase + (uint32_t)__builtin_ctzll(M));
+                        M &= M - 1ull;
+                        hasB = M != 0ull;
+                        jB = 0;
+                        if (hasB) {
+                            jB = 32u * (gbase + (uint32_t)__builtin_ctzll(M));
                             M &= M - 1ull;
-                            uint32_t g2i = 0;
-                            const bool hasB = M != 0ull;
-                            if (hasB) {
-                                g2i = gbase + (uint32_t)__builtin_ctzll(M);
-                                M &= M - 1ull;
-                            }
-                            step(32u * g1, 32u * g2i, hasB);
+                        }
+                        return true;
+                    };
+                    if constexpr (CMODE != 3) {
+                        uint32_t jA = 0, jB = 0;
+                        bool hasB = false;
+                        while (next_pair(jA, jB, hasB)) {  // uniform
+                            const uint32_t jl = (colane ? jA : jB) + (lane & 31u);
+                            uint2 g = make_uint2(0u, 0u);
+                            if ((colane || hasB) && jl < (uint32_t)N) g = fj16[jl];
+                            step(jA, jB, hasB, g);
+                        }
+                    } else {
+                        uint2 gn = make_uint2(0u, 0u);
+                        if (lane < (uint32_t)N) gn = fj16[lane];
+                        for (uint32_t j0 = 0; j0 < (uint32_t)N; j0 += 64) {
+                            const uint2 g = gn;
+                            if (j0 + 64 + lane < (uint32_t)N) gn = fj16[j0 + 64 + lane];
+                            step(j0, j0 + 32, true, g);
                         }
                     }
                 }
@@ -940,7 +1043,11 @@ struct GofrtTuning {
     int unroll = 8;
     bool xcd = false;
     uint64_t pairs_per_wg = 1ull << 30;
-    bool cluster = true;  // cluster (MFMA) screen where a wavefront's atoms are compact enough
+    // cluster (MFMA) screen where a wavefront's atoms are compact enough: 0 off,
+    // 1 on (far groups skipped from the bounding-sphere table), 2 as 1 with
+    // each step's column fractions cached in LDS, 3 on with every j visited
+    // in order (no table)
+    int cluster = 2;
 };
 
 struct anl_gofrt {
@@ -1047,7 +1154,7 @@ struct GofrtPrepared {
     uint64_t out_len = 0;
     bool triclinic = false;
     int unroll = 4;
-    bool cluster = false;
+    int cluster = 0;  // as GofrtTuning::cluster, 0 when the cluster screen cannot be used
     // device work list and block table owned by this launch (block engine)
     BlockInfo *own_blocks = nullptr;
     WorkItem *own_work = nullptr;
@@ -1111,6 +1218,7 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
     GofrtArgs &a = P.a;
     a.pos = tr->pos;
     a.frac = tr->frac16;
+    a.pos_scr = tr->pos_scr;
     a.box = tr->box;
     a.type_slot = tr->type_slot;
     a.orig_slot = tr->orig_slot;
@@ -1164,6 +1272,7 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         a.jgroups = tr->jgroups;
         a.ngroups = (uint32_t)((tr->N + 31) / 32);
         a.inv_w16 = form == ANL_GOFRT_SCREENED ? (float)(65536.0 / tr->screen_cache.wmin * (1 + 1e-6)) : 0.0f;
+        a.smin16 = form == ANL_GOFRT_SCREENED ? (float)(tr->screen_cache.smin * std::ldexp(1.0, -16) * (1 - 1e-6)) : 0.0f;
     }
     const uint32_t ntiles = (uint32_t)((tr->N + WG - 1) / WG);
     const uint32_t nlag = nlag_force ? nlag_force : (L.leff_max + g->every - 1) / g->every;
@@ -1179,19 +1288,21 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
     P.grid = dim3((uint32_t)(remap ? 8 * per : total));
     const size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
                           (size_t)a.nch * g->nbin * sizeof(unsigned);
-    auto screened_lds = [&](int qcap) {
-        return WG * sizeof(uint4) + (WG / 64) * (qcap + RQCAP) * sizeof(uint32_t) + common;
+    auto screened_lds = [&](int qcap, bool cluster) {
+        return (cluster ? 2 : 1) * WG * sizeof(uint4) + (WG / 64) * (qcap + RQCAP) * sizeof(uint32_t) + common;
     };
     // the cluster screen needs the larger queue; without room it is not used
-    const bool cluster_ok = g->tune.cluster && form == ANL_GOFRT_SCREENED && a.rho_cap > 0 &&
-                            screened_lds(SQCAP_CL) <= 160 * 1024;
-    P.lds = form == ANL_GOFRT_FP64 ? common : screened_lds(cluster_ok ? SQCAP_CL : SQCAP);
+    const bool colc = g->tune.cluster == 2;
+    const int qcl = colc ? SQCAP_CLC : SQCAP_CL;
+    const bool cluster_ok = g->tune.cluster != 0 && form == ANL_GOFRT_SCREENED && a.rho_cap > 0 &&
+                            screened_lds(qcl, colc) <= 160 * 1024;
+    P.lds = form == ANL_GOFRT_FP64 ? common : (cluster_ok ? screened_lds(qcl, colc) : screened_lds(SQCAP, false));
     if (P.lds > 160 * 1024) fail(ANL_ERR_ARG, "histogram does not fit in LDS (reduce nbin or types)");
     P.form = form;
     P.out_len = out_len;
     P.triclinic = tr->triclinic;
     P.unroll = g->tune.unroll;
-    P.cluster = cluster_ok;
+    P.cluster = cluster_ok ? g->tune.cluster : 0;
     g->last_form = form;
     return P;
 }
@@ -1207,17 +1318,23 @@ void gofrt_enqueue(anl_gofrt *g, const GofrtPrepared &P, unsigned long long *out
         else hipLaunchKernelGGL(gofrt_fp64_kernel<false>, P.grid, dim3(WG), P.lds, s, a);
     } else {
         const bool u8 = P.unroll == 8;
-#define ANL_SCREENED(T, U, C) hipLaunchKernelGGL((gofrt_screened_kernel<T, U, C>), P.grid, dim3(WG), P.lds, s, a)
+#define ANL_SCREENED(T, U, C, K) hipLaunchKernelGGL((gofrt_screened_kernel<T, U, C, K>), P.grid, dim3(WG), P.lds, s, a)
         if (P.cluster) {  // the 16-bit fallback of non-compact wavefronts takes 8 j per step
-            if (P.triclinic) ANL_SCREENED(true, 8, true);
-            else ANL_SCREENED(false, 8, true);
+            switch (P.cluster * 2 + (P.triclinic ? 1 : 0)) {
+                case 2: ANL_SCREENED(false, 8, true, 1); break;
+                case 3: ANL_SCREENED(true, 8, true, 1); break;
+                case 4: ANL_SCREENED(false, 8, true, 2); break;
+                case 5: ANL_SCREENED(true, 8, true, 2); break;
+                case 6: ANL_SCREENED(false, 8, true, 3); break;
+                default: ANL_SCREENED(true, 8, true, 3); break;
+            }
         } else {
             if (P.triclinic) {
-                if (u8) ANL_SCREENED(true, 8, false);
-                else ANL_SCREENED(true, 4, false);
+                if (u8) ANL_SCREENED(true, 8, false, 0);
+                else ANL_SCREENED(true, 4, false, 0);
             } else {
-                if (u8) ANL_SCREENED(false, 8, false);
-                else ANL_SCREENED(false, 4, false);
+                if (u8) ANL_SCREENED(false, 8, false, 0);
+                else ANL_SCREENED(false, 4, false, 0);
             }
         }
 #undef ANL_SCREENED
@@ -1284,7 +1401,7 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
         g->form = form;
         if (const char *e = std::getenv("ANL_GOFRT_UNROLL")) g->tune.unroll = std::atoi(e) == 8 ? 8 : 4;
         if (const char *e = std::getenv("ANL_GOFRT_XCD")) g->tune.xcd = std::atoi(e) != 0;
-        if (const char *e = std::getenv("ANL_GOFRT_CLUSTER")) g->tune.cluster = std::atoi(e) != 0;
+        if (const char *e = std::getenv("ANL_GOFRT_CLUSTER")) g->tune.cluster = std::min(3, std::max(0, std::atoi(e)));
         if (const char *e = std::getenv("ANL_GOFRT_PAIRS_PER_WG")) {
             long long v = std::atoll(e);
             if (v > 0) g->tune.pairs_per_wg = (uint64_t)v;
@@ -1470,7 +1587,8 @@ int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_w
         g->tune.unroll = unroll;
         g->tune.xcd = xcd != 0;
         g->tune.pairs_per_wg = pairs_per_wg;
-        g->tune.cluster = cluster != 0;
+        if (cluster < 0 || cluster > 3) fail(ANL_ERR_ARG, "cluster must be 0, 1, 2 or 3");
+        g->tune.cluster = cluster;
     });
 }
 
@@ -1480,7 +1598,7 @@ int anl_gofrt_get_tuning(anl_gofrt *g, int *unroll, int *xcd, uint64_t *pairs_pe
         if (unroll) *unroll = g->tune.unroll;
         if (xcd) *xcd = g->tune.xcd ? 1 : 0;
         if (pairs_per_wg) *pairs_per_wg = g->tune.pairs_per_wg;
-        if (cluster) *cluster = g->tune.cluster ? 1 : 0;
+        if (cluster) *cluster = g->tune.cluster;
     });
 }
 
@@ -1494,16 +1612,18 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         ensure_device();
         check_lengths(g, &primo, &g->n, 1);
         // unroll, xcd, log2 pairs per workgroup, cluster
-        static const int32_t defaults[] = {8, 0, 30, 1, 8, 0, 31, 1, 4, 0, 30, 1, 8, 1, 30, 1,
-                                           8, 0, 29, 1, 8, 0, 30, 0, 4, 0, 31, 0, 8, 1, 31, 0};
+        static const int32_t defaults[] = {8, 0, 29, 2, 8, 0, 30, 2, 8, 0, 31, 2, 8, 0, 29, 1,
+                                           8, 0, 29, 3, 8, 0, 30, 3, 8, 1, 29, 2, 8, 0, 30, 0};
         if (!cand) {
             cand = defaults;
             ncand = sizeof(defaults) / sizeof(defaults[0]) / 4;
         }
         if (ncand == 0) fail(ANL_ERR_ARG, "no candidates");
         for (uint32_t c = 0; c < ncand; c++)
-            if ((cand[4 * c] != 4 && cand[4 * c] != 8) || cand[4 * c + 2] < 10 || cand[4 * c + 2] > 40)
-                fail(ANL_ERR_ARG, "candidate " + std::to_string(c) + ": unroll 4|8, log2(pairs per workgroup) in [10, 40]");
+            if ((cand[4 * c] != 4 && cand[4 * c] != 8) || cand[4 * c + 2] < 10 || cand[4 * c + 2] > 40 ||
+                cand[4 * c + 3] < 0 || cand[4 * c + 3] > 3)
+                fail(ANL_ERR_ARG, "candidate " + std::to_string(c) +
+                                      ": unroll 4|8, log2(pairs per workgroup) in [10, 40], cluster 0..3");
         // probe lags: at least nlag_probe, and enough for >= 4096 workgroups
         // (several waves of residency), so that the probe behaves like the run
         const uint32_t nlag_all = (g->leff + g->every - 1) / g->every;
@@ -1523,7 +1643,7 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
             g->tune.unroll = cand[4 * c];
             g->tune.xcd = cand[4 * c + 1] != 0;
             g->tune.pairs_per_wg = 1ull << cand[4 * c + 2];
-            g->tune.cluster = cand[4 * c + 3] != 0;
+            g->tune.cluster = cand[4 * c + 3];
         };
         try {
             ANL_HIP(hipEventCreate(&e0));

@@ -77,10 +77,12 @@ __device__ static inline uint32_t fix32(double s) {
 // (slot s itself without an order) as {a, b, c} 32-bit fixed-point fractions
 // (frac32) and the same rounded to 16 bits, packed {a | b << 16, c} (frac16).
 // out32.w = type | slot << 8, so that one 16-byte load gives a candidate's
-// position, type and identity.
+// position, type and identity.  outpos holds the FP64 positions in the same
+// order, for the exact re-evaluation of candidates near a bin edge (reads of
+// neighbouring screen slots then share cache lines).
 __global__ void frac_kernel(const double4 *__restrict__ pos, const int32_t *__restrict__ type_slot,
                             const int32_t *__restrict__ order, uint64_t natoms, uint2 *__restrict__ out16,
-                            uint4 *__restrict__ out32, uint64_t total, Cell c) {
+                            uint4 *__restrict__ out32, double4 *__restrict__ outpos, uint64_t total, Cell c) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
     const uint64_t f = k / natoms;
@@ -94,6 +96,7 @@ __global__ void frac_kernel(const double4 *__restrict__ pos, const int32_t *__re
     uint32_t c16 = ((cc + 0x8000u) >> 16) & 0xFFFFu;
     out16[k] = make_uint2(a16 | (b16 << 16), c16);
     out32[k] = make_uint4(a, b, cc, (uint32_t)type_slot[sl] | (sl << 8));
+    outpos[k] = x;
 }
 
 static __device__ __forceinline__ int wrap16(uint32_t x) { return (int)(short)(x & 0xFFFFu); }
@@ -139,12 +142,13 @@ void build_frac16(anl_traj *tr, hipStream_t s) {
     uint64_t total = tr->F * tr->N;
     ANL_HIP(hipMalloc(&tr->frac16, total * sizeof(uint2)));
     ANL_HIP(hipMalloc(&tr->frac32, total * sizeof(uint4)));
+    ANL_HIP(hipMalloc(&tr->pos_scr, total * sizeof(double4)));
     const uint32_t ngroups = (uint32_t)((tr->N + 31) / 32);
     const uint64_t gtotal = tr->F * ngroups;
     ANL_HIP(hipMalloc(&tr->jgroups, gtotal * sizeof(uint4)));
     unsigned blocks = (unsigned)((total + 255) / 256);
     hipLaunchKernelGGL(frac_kernel, dim3(blocks), dim3(256), 0, s, tr->pos, tr->type_slot, tr->frac_order, tr->N,
-                       tr->frac16, tr->frac32, total, tr->cell0);
+                       tr->frac16, tr->frac32, tr->pos_scr, total, tr->cell0);
     ANL_HIP(hipGetLastError());
     hipLaunchKernelGGL(jgroup_kernel, dim3((unsigned)((gtotal + 255) / 256)), dim3(256), 0, s, tr->frac16, tr->N,
                        ngroups, gtotal, tr->cell0, tr->jgroups);
@@ -431,6 +435,7 @@ int anl_traj_destroy(anl_traj *tr) {
         (void)hipFree(tr->frac16);
         (void)hipFree(tr->frac32);
         (void)hipFree(tr->jgroups);
+        (void)hipFree(tr->pos_scr);
         delete tr;
     });
 }

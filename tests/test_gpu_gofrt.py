@@ -143,7 +143,8 @@ def test_launch_shapes_and_autotune_keep_counts(form):
     ref, _ = O.gofrt(ot, 0.0, 6.0, 48, 9, 4, 1, 1, 20, 0, want_vdata=False)
     g = capi.DeviceGofrt(dt, 0.0, 6.0, 48, 9, 1, 1, form)
     g.reset(20)
-    for unroll, xcd, lp, cl in [(4, 0, 31, 1), (8, 1, 31, 0), (4, 1, 12, 1), (8, 0, 16, 0), (4, 0, 40, 1)]:
+    for unroll, xcd, lp, cl in [(4, 0, 31, 1), (8, 1, 31, 0), (4, 1, 12, 2), (8, 0, 16, 0), (4, 0, 40, 1),
+                                (8, 0, 30, 2), (8, 1, 29, 3)]:
         g.set_tuning(unroll, xcd, 1 << lp, cl)
         assert g.tuning() == (unroll, xcd, 1 << lp, cl)
         got = g.calculate(0)
@@ -156,6 +157,8 @@ def test_launch_shapes_and_autotune_keep_counts(form):
     assert first_diff(g.calculate(0), ref) is None
     with pytest.raises(capi.AnlError):
         g.autotune(0, 2, [(5, 0, 20, 1)])
+    with pytest.raises(capi.AnlError):
+        g.autotune(0, 2, [(8, 0, 20, 4)])
 
 
 # Boxes large against rmax, so that compact wavefronts take the cluster screen:
@@ -193,7 +196,7 @@ def test_cluster_screen_matches_oracle(case):
     ref, _ = O.gofrt(ot, 0.0, rmax, nbin, tmax, 8, 1, 1, n, 1, want_vdata=False)
     g = capi.DeviceGofrt(dt, 0.0, rmax, nbin, tmax, 1, 1, capi.GOFRT_SCREENED)
     g.reset(n)
-    for cl in (1, 0):
+    for cl in (3, 2, 1, 0):
         g.set_tuning(8, 0, 1 << 30, cl)
         got = g.calculate(1)
         assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
