@@ -288,7 +288,11 @@ def run_gofrt(w, args, hip):
     g = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], w["skip"], w["every"], capi.GOFRT_SCREENED)
     leff = g.reset(w["n"])
     tune = None
-    if not args.no_autotune:
+    if args.shape:
+        u, x, lp, cl = (int(v) for v in args.shape.split(","))
+        g.set_tuning(u, x, 1 << lp, cl)
+        tune = {"given": [u, x, 1 << lp, cl]}
+    elif not args.no_autotune:
         nl = max(1, min(leff, args.tune_lags))
         ms, best = g.autotune(w["primo"], nl)
         tune = {"probe_lags": nl, "candidates_ms": [round(float(x), 2) for x in ms], "best": int(best),
@@ -550,6 +554,14 @@ def run_blocks(w, args, hip):
     g64 = capi.DeviceGofrt(traj, w["rmin"], w["rmax"], w["nbin"], w["tmax"], 1, 1, capi.GOFRT_FP64)
     g64.reset(int(lens[b]))
     same_g = bool(np.array_equal(g64.calculate(int(starts[b])), gcounts[bl]))
+    # the plain FP64 form timed on that window (the comparator of the batched run)
+    nch = nt * (nt + 1)
+    leff_b = min(int(lens[b]), w["tmax"])
+    d64 = hip.malloc(leff_b * nch * w["nbin"] * 8)
+    wb = dict(w, primo=int(starts[b]), n=int(lens[b]))
+    ms64 = timed_calculates(capi.lib(), g64, wb, d64, stream, hip, 1, "fp64 window %d" % b)[0][0]
+    hip.free(d64)
+    rate64 = n_updates(w["N"], int(lens[b]), w["tmax"]) / (ms64 * 1e-3)
     m1 = capi.msd(traj, 1, w["msd_tmax"], False, False, int(lens[b]), int(starts[b]))
     same_m = bool(np.array_equal(m1, mper[bl]))
     res = {"metric": "block-averaged g(r,t) + MSD, pair-lag updates/s over all windows",
@@ -562,6 +574,10 @@ def run_blocks(w, args, hip):
                       "sum_window_frames": int(lens.sum()), "gofrt_updates": upd, "msd_updates": mupd,
                       "launch_shape": list(g.tuning())},
            "setup_s": t_create, "autotune": tune,
+           "time_to_solution_vs_fp64": {"window": b, "frames": int(lens[b]), "fp64_ms": ms64, "fp64_rate": rate64,
+                                        "ratio_whole_run": upd / (ms * 1e-3) / rate64,
+                                        "note": "plain FP64 form timed on one window's reset(n) calculate; ratio = "
+                                                "all-pairs rate of the whole batched g(r,t) + MSD step over that"},
            "checks": {"window_%d_gofrt_fp64_identical" % b: same_g, "window_%d_msd_identical" % b: same_m}}
     if not args.no_cpu:
         from oracle import oracle as O
@@ -841,6 +857,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-lags", type=int, default=4, help="lags per autotune probe launch")
+    ap.add_argument("--shape", default="", help="g(r,t) launch shape 'unroll,xcd,log2 pairs per workgroup,cluster' "
+                                                "instead of the autotuner (for reproducible profiles)")
     ap.add_argument("--cpu-n", type=int, default=0,
                     help="reset(n) of the CPU baseline sample (n t0 x n lags; 0 = 2 on c3, 8 on c2)")
     ap.add_argument("--c4-windows", type=int, default=0, help="number of c4 windows (0 = the workload's 64)")
