@@ -388,8 +388,8 @@ def run_gofrt(w, args, hip):
                              "x evaluated pairs / event-timed whole calculate; peak = MI355X published FP32 157.3 "
                              "TFLOP/s (packed); this build issues no packed FP32, whose half, 78.6, is the "
                              "non-packed VALU figure.  The kernel is instruction-issue bound (VALU/SALU/LDS), so "
-                             "neither 'hbm' nor 'mfma' describes it; valu_busy_measured = SQ_ACTIVE_INST_VALU x 4 "
-                             "/ SQ_BUSY_CYCLES per SIMD from profiles/counters.txt" % (fl, "3 tilt, " if w["triclinic"] else "")},
+                             "neither 'hbm' nor 'mfma' describes it; valu_busy_measured = SQ_INSTS_VALU x 4 / (1024 "
+                             "SIMDs x SQ_BUSY_CYCLES / 32), from profiles/counters.txt" % (fl, "3 tilt, " if w["triclinic"] else "")},
         "time_to_solution_vs_fp64": {
             "slice": "reset(%d) of the same trajectory: %d t0 x %d lags, both forms timed alike" % (n_sl, n_sl, leff_sl),
             "fp64_ms": ms64, "fp32_recheck_ms": mssl, "ratio_on_slice": ms64 / mssl,
