@@ -26,6 +26,7 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -1027,36 +1028,50 @@ class GreenKuboNComponentIonicFluid
 // ---------------------------------------------------------------------------
 // MediaVar / MediaVarCovar (calcoliblocchi.h:21-202)
 // ---------------------------------------------------------------------------
+// MediaVar: running mean and variance over blocks, element by element.  For
+// block b (0-based) and every element k of the block's result x:
+//     d = x[k] - m[k];   m[k] += d / (b + 1);   v[k] += (x[k] - m[k]) * d
+// and after the last of n blocks v[k] /= (n - 1) n.  Per element these are
+// the operations of the reference's whole-vector sequence
+// (calcoliblocchi.h:21-65), in its order and precision, so the mean and the
+// variance are bit-identical to it.
 template <class T>
 class MediaVar {
    public:
-    MediaVar(T *Tmedio, T *Tvar, T *delta, T *tmp) : Tmedio(Tmedio), Tvar(Tvar), delta(delta), tmp(tmp) {}
+    MediaVar(T *mean, T *var) : mean_(mean), var_(var) {}
     void calcola_begin(unsigned int s, T *) {
-        Tmedio->reset(s);
-        Tmedio->azzera();
-        Tvar->reset(s);
-        Tvar->azzera();
-        delta->reset(s);
-        tmp->reset(s);
-        iblock = 0;
+        for (T *acc : {mean_, var_}) {
+            acc->reset(s);
+            acc->azzera();
+        }
+        seen_ = 0;
     }
-    void calculate(T *calcolo) {
-        *delta = *calcolo;
-        *delta -= *Tmedio;
-        *tmp = *delta;
-        *tmp /= (iblock + 1);
-        *Tmedio += *tmp;
-        *tmp = *calcolo;
-        *tmp -= *Tmedio;
-        *tmp *= *delta;
-        *Tvar += *tmp;
-        iblock++;
+    void calculate(T *block) {
+        const unsigned int len = block->lunghezza();
+        if (mean_->lunghezza() != len || var_->lunghezza() != len)
+            throw std::runtime_error("Trying to operate on VectorOp of different sizes!");
+        auto *x = block->access_vdata();
+        auto *m = mean_->access_vdata();
+        auto *v = var_->access_vdata();
+        using F = std::remove_pointer_t<decltype(x)>;
+        const F weight = (F)(seen_ + 1);
+        for (unsigned int k = 0; k < len; k++) {
+            const F d = x[k] - m[k];
+            const F step = d / weight;
+            m[k] += step;
+            const F spread = x[k] - m[k];
+            v[k] += spread * d;
+        }
+        seen_++;
     }
-    void calcola_end(unsigned int n_b) { *Tvar /= ((n_b - 1) * n_b); }
+    void calcola_end(unsigned int n_b) {
+        using F = std::remove_pointer_t<decltype(var_->access_vdata())>;
+        *var_ /= (F)((n_b - 1) * n_b);
+    }
 
    private:
-    T *Tmedio, *Tvar, *delta, *tmp;
-    unsigned int iblock = 0;
+    T *mean_, *var_;
+    unsigned int seen_ = 0;
 };
 
 template <class T>
@@ -1125,15 +1140,22 @@ inline unsigned int n_timesteps_of(TR *t) {
         return (unsigned int)t->get_ntimesteps();
 }
 
+// BlockAverageG: the trajectory (or log) is cut into n_b consecutive blocks
+// of equal length, chosen so that the last block still has the lags its
+// calculation reads past its end; one calculation object is reset and run on
+// each block in turn and an accumulator (MediaVar by default) folds the
+// block results.  Over a windowed reader the window is sized once to a block
+// plus its lags and moved to each block before its calculation; a reader
+// that can read ahead starts on the following block's window meanwhile.
 template <class TR, class T, typename... Args>
 class BlockAverageG {
-    // trajectories that can read the next block's window ahead
+    // readers that can read the next block's window ahead
     template <class X, class = void>
     struct reads_ahead : std::false_type {};
     template <class X>
     struct reads_ahead<X, std::void_t<decltype(std::declval<X &>().set_next_window(size_t()))>> : std::true_type {};
-    // trajectories read through a window of timesteps (blockaverage.h:32-51:
-    // only the LAMMPS binary; logs and in-memory arrays hold everything)
+    // readers that hold only a window of timesteps (blockaverage.h:32-51:
+    // the LAMMPS binary; logs and in-memory arrays hold everything)
     template <class X, class = void>
     struct windowed : std::false_type {};
     template <class X>
@@ -1141,63 +1163,62 @@ class BlockAverageG {
                                    decltype(std::declval<X &>().set_data_access_block_size(size_t()))>>
         : std::true_type {};
 
-   public:
-    BlockAverageG(TR *t, const unsigned int &numero_blocchi) : traiettoria(t), n_b(numero_blocchi) {}
-    ~BlockAverageG() {
-        delete Tmedio;
-        delete Tvar;
-        delete delta;
-        delete tmp;
-        delete calcolo;
-    }
-    template <class Calcolo>
-    void calcola_custom(Calcolo *calc, Args... arg) {  // :107-190
-        calcolo = new T(traiettoria, arg...);
-        int per_block = ((int)n_timesteps_of(traiettoria) - (int)calcolo->nExtraTimesteps(n_b)) / (int)n_b;
-        if (per_block <= 0) throw std::runtime_error("Cannot divide the trajectory in the requested blocks");
-        s = (unsigned int)per_block;
-        ok = true;
-        calcolo->reset(s);
-        calc->calcola_begin(s, calcolo);
-        if constexpr (windowed<TR>::value)  // a block and its lags (:123)
-            traiettoria->set_data_access_block_size((size_t)s + calcolo->nExtraTimesteps(n_b));
-        for (unsigned int iblock = 0; iblock < n_b; iblock++) {
-            calcolo->reset(s);
-            if constexpr (windowed<TR>::value)  // :135, adopting a window read ahead
-                traiettoria->set_access_at((size_t)iblock * s);
-            if constexpr (reads_ahead<TR>::value)
-                if (iblock + 1 < n_b) traiettoria->set_next_window((size_t)(iblock + 1) * s);
-            calcolo->calculate(iblock * s);
-            calc->calculate(calcolo);
-        }
-        calc->calcola_end(n_b);
-    }
-    void calculate(Args... arg) {  // :194-203
-        Tmedio = new T(traiettoria, arg...);
-        Tvar = new T(traiettoria, arg...);
-        delta = new T(traiettoria, arg...);
-        tmp = new T(traiettoria, arg...);
-        MediaVar<T> media_var(Tmedio, Tvar, delta, tmp);
-        calcola_custom<MediaVar<T>>(&media_var, arg...);
-    }
-    T *media() {
-        if (ok) return Tmedio;
-        throw std::runtime_error("block average not computed");
-    }
-    T *varianza() {
-        if (ok) return Tvar;
-        throw std::runtime_error("block average not computed");
-    }
-    T *puntatoreCalcolo() {
-        if (ok) return calcolo;
-        throw std::runtime_error("block average not computed");
+    // Block length: the timesteps left after the lags of the whole run,
+    // shared evenly (blockaverage.h:113-121; the reference aborts where this
+    // throws).
+    unsigned int block_length(T &probe) const {
+        const long total = (long)n_timesteps_of(source_);
+        const long extra = (long)probe.nExtraTimesteps(n_blocks_);
+        const long len = (total - extra) / (long)n_blocks_;
+        if (len <= 0) throw std::runtime_error("Cannot divide the trajectory in the requested blocks");
+        return (unsigned int)len;
     }
 
+   public:
+    BlockAverageG(TR *t, const unsigned int &numero_blocchi) : source_(t), n_blocks_(numero_blocchi) {}
+
+    // blockaverage.h:107-190 (one process: MPI's round-robin over blocks is
+    // not part of this library)
+    template <class Calcolo>
+    void calcola_custom(Calcolo *acc, Args... arg) {
+        work_ = std::make_unique<T>(source_, arg...);
+        const unsigned int len = block_length(*work_);
+        work_->reset(len);
+        acc->calcola_begin(len, work_.get());
+        if constexpr (windowed<TR>::value)
+            source_->set_data_access_block_size((size_t)len + work_->nExtraTimesteps(n_blocks_));
+        ready_ = true;
+        for (unsigned int b = 0; b < n_blocks_; b++) {
+            const size_t first = (size_t)b * len;
+            work_->reset(len);
+            if constexpr (windowed<TR>::value) source_->set_access_at(first);
+            if constexpr (reads_ahead<TR>::value)
+                if (b + 1 < n_blocks_) source_->set_next_window(first + len);
+            work_->calculate(first);
+            acc->calculate(work_.get());
+        }
+        acc->calcola_end(n_blocks_);
+    }
+    // blockaverage.h:194-203: mean and variance over the blocks
+    void calculate(Args... arg) {
+        mean_ = std::make_unique<T>(source_, arg...);
+        var_ = std::make_unique<T>(source_, arg...);
+        MediaVar<T> folded(mean_.get(), var_.get());
+        calcola_custom(&folded, arg...);
+    }
+    T *media() { return checked(mean_.get()); }
+    T *varianza() { return checked(var_.get()); }
+    T *puntatoreCalcolo() { return checked(work_.get()); }
+
    private:
-    TR *traiettoria;
-    unsigned int n_b, s = 0;
-    T *Tmedio = nullptr, *Tvar = nullptr, *calcolo = nullptr, *delta = nullptr, *tmp = nullptr;
-    bool ok = false;
+    T *checked(T *p) const {
+        if (!ready_ || !p) throw std::runtime_error("block average not computed");
+        return p;
+    }
+    TR *source_;
+    unsigned int n_blocks_;
+    std::unique_ptr<T> work_, mean_, var_;
+    bool ready_ = false;
 };
 
 }  // namespace analisi
