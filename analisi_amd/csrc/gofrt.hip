@@ -369,6 +369,19 @@ __device__ __forceinline__ void lds_setup(const GofrtArgs &a, unsigned *hist, do
     for (int k = threadIdx.x; k < (int)(a.nt * a.nt); k += WG) chan[k] = a.chan[k];
 }
 
+// Screened form: the histogram offset of a pair, by (type pair, same atom):
+// hb[(ti nt + tj) 2 + self] = (channel + self * half) * nbin.
+__device__ __forceinline__ void lds_setup_hb(const GofrtArgs &a, unsigned *hist, double *thr, int *hb) {
+    const int nh = (int)(a.nch * a.nbin);
+    for (int k = threadIdx.x; k < nh; k += WG) hist[k] = 0u;
+    for (int k = threadIdx.x; k <= (int)a.nbin; k += WG) thr[k] = a.thr[k];
+    for (int k = threadIdx.x; k < (int)(2 * a.nt * a.nt); k += WG)
+        hb[k] = (a.chan[k >> 1] + ((k & 1) ? (int)a.half : 0)) * (int)a.nbin;
+}
+__device__ __forceinline__ uint32_t hb_key(uint32_t wi, uint32_t wj, uint32_t nt) {
+    return ((__umul24(wi & 0xFFu, nt) + (wj & 0xFFu)) << 1) | (wi == wj ? 1u : 0u);
+}
+
 // Workgroups of one (block, lag) flush into the same counters, often at the
 // same time (they start and end together): each starts at its own offset,
 // so that their atomics meet different addresses.
@@ -477,7 +490,7 @@ __device__ __forceinline__ uint32_t lane_slot(unsigned long long m, uint32_t bas
 // outside the window both floor to the same bin.
 template <bool TRI>
 __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 uj,
-                                               const int *chan, unsigned *hist) {
+                                               const int *hb, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint4 ui = fi_lds[il];  // .w = type | slot << 8 of i; uj likewise of j
     const float fa = (float)(int)(ui.x - uj.x);
@@ -497,10 +510,7 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
     if (fabsf(u - rintf(u)) < a.du_margin) return true;
     const int bb = (int)floorf(u);
-    if ((unsigned)bb < a.nbin) {
-        const int ch = chan[__umul24(ui.w & 0xFFu, a.nt) + (uj.w & 0xFFu)] + (ui.w == uj.w ? (int)a.half : 0);
-        atomicAdd(&hist[__umul24((unsigned)ch, a.nbin) + (unsigned)bb], 1u);
-    }
+    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hb[hb_key(ui.w, uj.w, a.nt)] + bb], 1u);
     return false;
 }
 
@@ -512,7 +522,7 @@ template <bool TRI>
 __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds,
                                                   const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
                                                   const double4 *__restrict__ pj, const double *cell,
-                                                  const double *thr, const int *chan, unsigned *hist) {
+                                                  const double *thr, const int *hb, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
     const uint32_t wi = fi_lds[il].w, wj = fj32[j].w;  // type | slot << 8
@@ -524,8 +534,7 @@ __device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e
     const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
     if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) {
         const int bb = bin_of(d2r, thr, (int)a.nbin, a.rminf, a.inv_drf);
-        const int ch = chan[(wi & 0xFFu) * a.nt + (wj & 0xFFu)] + (wi == wj ? (int)a.half : 0);
-        atomicAdd(&hist[__umul24((unsigned)ch, a.nbin) + (unsigned)bb], 1u);
+        atomicAdd(&hist[hb[hb_key(wi, wj, a.nt)] + bb], 1u);
     }
 }
 
@@ -597,9 +606,9 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     uint32_t *queue = (uint32_t *)(fi_lds + WG + (COLC ? WG : 0));  // [WG/64][QC]
     uint32_t *rqueue = queue + (WG / 64) * QC;              // [WG/64][RQCAP]
     double *thr = (double *)(rqueue + (WG / 64) * RQCAP);   // [nbin+1]
-    int *chan = (int *)(thr + a.nbin + 1);                  // [nt*nt]
-    unsigned *hist = (unsigned *)(chan + a.nt * a.nt);      // [nch*nbin]
-    lds_setup(a, hist, thr, chan);
+    int *hb = (int *)(thr + a.nbin + 1);                    // [nt*nt*2]
+    unsigned *hist = (unsigned *)(hb + 2 * a.nt * a.nt);    // [nch*nbin]
+    lds_setup_hb(a, hist, thr, hb);
 
     const uint64_t N = a.N;
     const uint32_t lane = threadIdx.x & 63u;
@@ -651,25 +660,26 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         __syncthreads();
         uint32_t qn = 0, rn = 0;
         // evaluate the queued entry of this lane; candidates inside the edge
-        // window go to the recheck queue, evaluated 64 at a time
-        auto eval_with = [&](uint32_t e, bool valid, const uint4 uj) {
-            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, uj, chan, hist);
+        // window go to the recheck queue (entries j << 8 | i_local, to_j
+        // converts this queue's entry), evaluated 64 at a time
+        auto eval_with = [&](uint32_t e, bool valid, const uint4 uj, auto to_j) {
+            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, uj, hb, hist);
             const unsigned long long mr = __ballot(need);
             if (mr) {
-                if (need) rq[rn + lane_prefix(mr)] = e;
+                if (need) rq[rn + lane_prefix(mr)] = to_j(e);
                 rn += (uint32_t)__popcll(mr);
                 if (rn >= 64) {
                     wave_sync_lds();
                     rn -= 64;
                     nre += 1;
-                    recheck_candidate<TRI>(a, rq[rn + lane], fi_lds, fj32, pi, pj, cellp, thr, chan, hist);
+                    recheck_candidate<TRI>(a, rq[rn + lane], fi_lds, fj32, pi, pj, cellp, thr, hb, hist);
                     wave_sync_lds();
                 }
             }
         };
         // the j atom's fractions from global memory (16-bit screen, final drain)
         auto eval_batch = [&](uint32_t e, bool valid) {
-            eval_with(e, valid, valid ? fj32[e >> 8] : make_uint4(0u, 0u, 0u, 0u));
+            eval_with(e, valid, valid ? fj32[e >> 8] : make_uint4(0u, 0u, 0u, 0u), [](uint32_t x) { return x; });
         };
         bool clustered = false;
         if constexpr (CLUSTER) {
@@ -768,11 +778,14 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             colc[lane] = jin ? fj32[jl] : make_uint4(0u, 0u, 0u, 0u);
                             wave_sync_lds();
                         }
+                        // COLC: entries hold the step's column c (0..63) instead of
+                        // j; a recheck entry gets j back
                         auto eval_step = [&](uint32_t e, bool valid) {
                             if constexpr (COLC) {
-                                const uint32_t jj = e >> 8;
-                                const uint32_t c = jj - jA < 32u ? jj - jA : 32u + (jj - jB);
-                                eval_with(e, valid, colc[valid ? c : 0u]);
+                                eval_with(e, valid, colc[valid ? e >> 8 : 0u], [&](uint32_t x) {
+                                    const uint32_t c = x >> 8;
+                                    return ((c < 32u ? jA + c : jB + (c - 32u)) << 8) | (x & 0xFFu);
+                                });
                             } else {
                                 eval_batch(e, valid);
                             }
@@ -798,9 +811,12 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             if ((uint32_t)(nearm >> (32 * jb)) == 0u) continue;  // uniform
                             const uint32_t jfirst = jb ? jB : jA;
                             ntile += nact * (unsigned long long)min(32u, (uint32_t)N - jfirst);
-                            // entry = j << 8 | local i; item e of tile T is row
+                            // entry = j << 8 | local i (COLC: the column, 32 jb + lane % 32,
+                            // instead of j); item e of tile T is row
                             // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
-                            const uint32_t eb = ((jfirst + (lane & 31u)) << 8) | (wbase + half4);
+                            uint32_t col = COLC ? 32u * (uint32_t)jb + (lane & 31u) : jfirst + (lane & 31u);
+                            if constexpr (COLC) asm volatile("" : "+v"(col));  // recomputed per step, not held
+                            const uint32_t eb = (col << 8) | (wbase + half4);
                             // both row tiles' products are issued before either is
                             // compared, so one MFMA's latency hides behind the other's
                             const f32x16_t DT[2] = {
@@ -1004,7 +1020,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         wave_sync_lds();
         if (lane < rn) {
             nre += 1;
-            recheck_candidate<TRI>(a, rq[lane], fi_lds, fj32, pi, pj, cellp, thr, chan, hist);
+            recheck_candidate<TRI>(a, rq[lane], fi_lds, fj32, pi, pj, cellp, thr, hb, hist);
         }
         wave_sync_lds();
     }
@@ -1286,7 +1302,10 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
     const bool remap = g->tune.xcd && form == ANL_GOFRT_SCREENED;
     a.per_xcd = remap ? (uint32_t)per : 0u;
     P.grid = dim3((uint32_t)(remap ? 8 * per : total));
-    const size_t common = (g->nbin + 1) * sizeof(double) + (size_t)tr->nt * tr->nt * sizeof(int) +
+    // channel table (FP64 form) or histogram-offset table (screened form, two
+    // entries per type pair), bin-edge table, histogram
+    const size_t common = (g->nbin + 1) * sizeof(double) +
+                          (size_t)(form == ANL_GOFRT_FP64 ? 1 : 2) * tr->nt * tr->nt * sizeof(int) +
                           (size_t)a.nch * g->nbin * sizeof(unsigned);
     auto screened_lds = [&](int qcap, bool cluster) {
         return (cluster ? 2 : 1) * WG * sizeof(uint4) + (WG / 64) * (qcap + RQCAP) * sizeof(uint32_t) + common;
