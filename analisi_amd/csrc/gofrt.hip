@@ -206,6 +206,30 @@ static float du_window(const Screen &s, double rmin, double rmax, double dr, uin
     return (float)(4.0 * du + 1e-6);
 }
 
+// Window of the mode-2 candidate evaluation, which subtracts FP32 Cartesian
+// offsets of both atoms from the wavefront's centre c: P = A' float(W_i),
+// Q = A' float(W_j), with W = 32-bit fraction - c (exact, wrapped) and A' the
+// FP32 cell constants times 2^-32.  Per component, the int->float conversion
+// and the up to three products and sums of a row cost <= 4u sum_m |A_km W_m|
+// for each atom (u = 2^-24), and sum_m |A_km W_m| <= K |offset| with
+// K = row_sum / smin; |P| <= rho + dq and |Q| <= rho + dq + d for the
+// wavefront radius rho and a pair at distance d <= dmax = rmax + 1.  The FP32
+// cell constants add u K d, the subtraction u d, the 32-bit quantisation
+// row_sum 2^-31.  So per component
+//   e_comp <= u K (8 (rho + dq) + 6 dmax) + row_sum 2^-31,
+// and e_d, du and the factor 4 follow du_window.  The window is
+// m0 + m1 (rho + dq) bins, computed per wavefront from its own radius.
+static void du_window_offsets(const Screen &s, double rmax, double dr, uint32_t nbin, float *m0, float *m1) {
+    const double e24 = std::ldexp(1.0, -24);
+    const double K = s.row_sum / s.smin;
+    const double dmax = std::fabs(rmax) + 1.0;
+    const double r3 = std::sqrt(3.0);
+    const double c0 = r3 * (e24 * K * 6.0 * dmax + s.row_sum * std::ldexp(1.0, -31)) + 8.0 * e24 * dmax;
+    const double c1 = r3 * e24 * K * 8.0;
+    *m0 = (float)((4.0 * (c0 / dr + 8.0 * e24 * (nbin + 1.0)) + 1e-6) * (1 + 1e-6));
+    *m1 = (float)(4.0 * c1 / dr * (1 + 1e-6));
+}
+
 // Cluster screen.  A wavefront's 64 i atoms in frame fi get a common centre
 // c (16-bit fractional) and Cartesian offsets p_i = A wrap16(f_i - c); j atoms
 // get q_j = A wrap16(f_j - c).  If rho = max |p_i| satisfies
@@ -318,6 +342,7 @@ struct GofrtArgs {
     unsigned long long *nrecheck;
     float cx, cxy, cxz, cy, cyz, cz;  // cell matrix entries times 2^-32
     float du_margin;                  // bins: |u - nearest integer| below this -> FP64
+    float du_m0, du_m1;               // mode 2: window m0 + m1 (rho + dq) of the offset evaluation
     // cluster screen (see cluster_threshold): cell entries times 2^-16,
     // largest usable cluster radius, quantisation bound, rmax
     float hx, hxy, hxz, hy, hyz, hz;
@@ -514,6 +539,47 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
     return false;
 }
 
+// FP32 Cartesian offset of an atom (32-bit fractions f) from a centre c
+// (32-bit fractional units): A' float(f - c), the wrapped difference taken
+// as a signed integer; .w keeps the atom's type | slot << 8.
+template <bool TRI>
+__device__ __forceinline__ uint4 offset32(const GofrtArgs &a, const uint4 f, uint32_t ca, uint32_t cb, uint32_t cc) {
+    const float fa = (float)(int)(f.x - ca);
+    const float fb = (float)(int)(f.y - cb);
+    const float fc = (float)(int)(f.z - cc);
+    float x, y, z;
+    if (TRI) {
+        z = a.cz * fc;
+        y = fmaf(a.cyz, fc, a.cy * fb);
+        x = fmaf(a.cxz, fc, fmaf(a.cxy, fb, a.cx * fa));
+    } else {
+        x = a.cx * fa;
+        y = a.cy * fb;
+        z = a.cz * fc;
+    }
+    return make_uint4(__float_as_uint(x), __float_as_uint(y), __float_as_uint(z), f.w);
+}
+
+// Mode 2: a candidate from the offsets of both atoms from the wavefront's
+// centre (pi from fi_lds, qj from the column cache).  For every pair the
+// reference can count, wrapping relative to the centre picks the
+// reference's image (the cluster screen's validity condition), so the pair
+// vector is P - Q; margin (bins) bounds the error of this form for the
+// wavefront's radius.
+__device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 qj,
+                                                   const int *hb, unsigned *hist, float margin) {
+    const uint4 pi = fi_lds[e & 0xFFu];
+    const float dx = __uint_as_float(pi.x) - __uint_as_float(qj.x);
+    const float dy = __uint_as_float(pi.y) - __uint_as_float(qj.y);
+    const float dz = __uint_as_float(pi.z) - __uint_as_float(qj.z);
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
+    if (fabsf(u - rintf(u)) < margin) return true;
+    const int bb = (int)floorf(u);
+    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hb[hb_key(pi.w, qj.w, a.nt)] + bb], 1u);
+    return false;
+}
+
 // Exact FP64 evaluation (the reference's sequence) of a candidate whose FP32
 // distance fell inside the window around a bin edge, rmin or rmax.  pi and pj
 // are the positions of the two frames in screen order (pi from the tile's
@@ -662,8 +728,8 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         // evaluate the queued entry of this lane; candidates inside the edge
         // window go to the recheck queue (entries j << 8 | i_local, to_j
         // converts this queue's entry), evaluated 64 at a time
-        auto eval_with = [&](uint32_t e, bool valid, const uint4 uj, auto to_j) {
-            const bool need = valid && eval_candidate<TRI>(a, e, fi_lds, uj, hb, hist);
+        auto eval_with = [&](uint32_t e, bool valid, auto ev, auto to_j) {
+            const bool need = valid && ev(e);
             const unsigned long long mr = __ballot(need);
             if (mr) {
                 if (need) rq[rn + lane_prefix(mr)] = to_j(e);
@@ -679,7 +745,9 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         };
         // the j atom's fractions from global memory (16-bit screen, final drain)
         auto eval_batch = [&](uint32_t e, bool valid) {
-            eval_with(e, valid, valid ? fj32[e >> 8] : make_uint4(0u, 0u, 0u, 0u), [](uint32_t x) { return x; });
+            eval_with(
+                e, valid, [&](uint32_t x) { return eval_candidate<TRI>(a, x, fi_lds, fj32[x >> 8], hb, hist); },
+                [](uint32_t x) { return x; });
         };
         bool clustered = false;
         if constexpr (CLUSTER) {
@@ -713,6 +781,16 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     clustered = true;
                     ncl += lane == 0 ? 1 : 0;
                     const float thc = cluster_threshold(rho, a.rmaxf, a.dq);
+                    // mode 2: candidates subtract FP32 offsets from the centre;
+                    // the rows' offsets replace their fractions in fi_lds, which
+                    // only this wavefront reads (the .w words stay)
+                    float margin_off = 0.0f;
+                    if constexpr (COLC) {
+                        margin_off = fmaf(a.du_m1, rho + a.dq, a.du_m0);
+                        if (slot < N)
+                            fi_lds[threadIdx.x] = offset32<TRI>(a, fi_lds[threadIdx.x], ca << 16, cb << 16, cc << 16);
+                        wave_sync_lds();
+                    }
                     // j with |q| > rho + rmax + dq is farther than rmax + dq from
                     // every active i (|q - p| >= |q| - |p|), so none of its pairs
                     // can be counted; the bound carries a margin for the FP32
@@ -775,17 +853,23 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         // step drains it); otherwise candidates load them from
                         // global memory
                         if constexpr (COLC) {
-                            colc[lane] = jin ? fj32[jl] : make_uint4(0u, 0u, 0u, 0u);
+                            colc[lane] = jin ? offset32<TRI>(a, fj32[jl], ca << 16, cb << 16, cc << 16)
+                                             : make_uint4(0u, 0u, 0u, 0u);
                             wave_sync_lds();
                         }
                         // COLC: entries hold the step's column c (0..63) instead of
                         // j; a recheck entry gets j back
                         auto eval_step = [&](uint32_t e, bool valid) {
                             if constexpr (COLC) {
-                                eval_with(e, valid, colc[valid ? e >> 8 : 0u], [&](uint32_t x) {
-                                    const uint32_t c = x >> 8;
-                                    return ((c < 32u ? jA + c : jB + (c - 32u)) << 8) | (x & 0xFFu);
-                                });
+                                eval_with(
+                                    e, valid,
+                                    [&](uint32_t x) {
+                                        return eval_candidate_off(a, x, fi_lds, colc[x >> 8], hb, hist, margin_off);
+                                    },
+                                    [&](uint32_t x) {
+                                        const uint32_t c = x >> 8;
+                                        return ((c < 32u ? jA + c : jB + (c - 32u)) << 8) | (x & 0xFFu);
+                                    });
                             } else {
                                 eval_batch(e, valid);
                             }
@@ -1272,6 +1356,9 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         a.cyz = (float)(cl.yz * sc);
         a.cz = (float)(cl.L[2] * sc);
         a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->rmax, g->dr, g->nbin) : 0.0f;
+        a.du_m0 = a.du_m1 = 0.0f;
+        if (form == ANL_GOFRT_SCREENED)
+            du_window_offsets(tr->screen_cache, g->rmax, g->dr, g->nbin, &a.du_m0, &a.du_m1);
         // cluster screen: cell entries times 2^-16, quantisation bound,
         // largest usable wavefront radius (cluster_params)
         const double sh = std::ldexp(1.0, -16);

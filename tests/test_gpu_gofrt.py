@@ -110,6 +110,47 @@ def test_bin_edges_adversarial():
         assert first_diff(got, ref) is None, (form, first_diff(got, ref))
 
 
+@pytest.mark.parametrize("tilt", [None, (2.5, -1.5, 1.0)], ids=["cubic", "tri"])
+def test_bin_edges_adversarial_cluster(tilt):
+    """Pairs within a few ulps of bin edges, rmin and rmax, inside one compact
+    cloud, so that wavefronts take the cluster screen: every mode (including
+    mode 2's offset-difference evaluation and its radius-dependent window)
+    must give the reference's float-rounded bins."""
+    from analisi_amd import synth
+    rmin, rmax, nbin = 0.5, 4.5, 40
+    dr = (rmax - rmin) / nbin
+    d = []
+    for k in range(0, nbin + 1):
+        e = rmin + k * dr
+        d += [np.nextafter(e, 0), e, np.nextafter(e, 10), e * (1 - 2e-8), e * (1 + 2e-8)]
+    d = np.array(d)
+    M = len(d)
+    L = 60.0
+    rng = np.random.default_rng(5)
+    base = 20.0 + 12.0 * rng.random((M, 3))
+    pos = np.zeros((2, 2 * M, 3))
+    pos[:, 0::2] = base
+    pos[:, 1::2] = base
+    pos[:, 1::2, 0] += d
+    types = (np.arange(2 * M) % 2).astype(np.int32)
+    box = np.tile(synth.lammps_box(L, None if tilt is None else np.array(tilt)), (2, 1))
+    ot = O.Traj(pos, types, box, tilt is not None)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    ref, _ = O.gofrt(ot, rmin, rmax, nbin, 1, 1, 1, 1, 1, 0, want_vdata=False)
+    assert ref.sum() >= M  # every planted pair is inside [rmin, rmax] or at its edges
+    g = capi.DeviceGofrt(dt, rmin, rmax, nbin, 1, 1, 1, capi.GOFRT_SCREENED)
+    g.reset(1)
+    for cl in (2, 1, 3, 0):
+        g.set_tuning(8, 0, 1 << 30, cl)
+        got = g.calculate(0)
+        assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
+        c, tot = g.cluster_stats()
+        if cl:
+            assert 0 < c <= tot, (cl, c, tot)
+        _, _, rechecks = g.stats()
+        assert rechecks > 0, cl
+
+
 def test_too_short_raises():
     ot, dt = make_case(2, 50, 10, 8.0, [1.0], 0.1)
     g = capi.DeviceGofrt(dt, 0.0, 3.0, 10, 3, 1, 1)
