@@ -62,8 +62,8 @@ SIGNATURES = [
     ("anl_msd_calculate", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp]),
     ("anl_msd_calculate_async", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
     ("anl_msd_calculate_noalloc", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _vp]),
-    ("anl_msd_set_tuning", _i, [_vp, _u32]),
-    ("anl_msd_get_tuning", _i, [_vp, ctypes.POINTER(_u32)]),
+    ("anl_msd_set_tuning", _i, [_vp, _u32, _u32]),
+    ("anl_msd_get_tuning", _i, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     ("anl_msd_autotune", _i, [_vp, _u32, _u32, _i, _i, _u32, _u64, _vp, _u32, _vp, ctypes.POINTER(_u32)]),
     ("anl_series_create", _i, [_vp, _u64, _u32, _pp]),
     ("anl_series_destroy", _i, [_vp]),
@@ -138,20 +138,29 @@ class DeviceTrajectory:
                                     self.ntypes, _ptr(self.cm), F, N, ctypes.byref(h)))
         self.handle = h
 
-    def set_msd_tile(self, lag_tile):
-        """Lag tile of the MSD kernel (8, 12, 16, 24; 0 = eight lags per thread)."""
-        check(lib().anl_msd_set_tuning(self.handle, lag_tile))
+    # (lag tile, wavefronts per workgroup) of the MSD window kernel, in the
+    # library's default candidate order (tile 0: eight lags per thread)
+    MSD_SHAPES = [(8, 4), (12, 4), (16, 4), (24, 4), (16, 8), (24, 8), (12, 12), (16, 12), (8, 16), (0, 4)]
+
+    def set_msd_tile(self, lag_tile, waves=4):
+        """Launch shape of the MSD kernel: lag tile (8, 12, 16, 24; 0 = eight
+        lags per thread) and wavefronts per workgroup (see MSD_SHAPES)."""
+        check(lib().anl_msd_set_tuning(self.handle, lag_tile, waves))
 
     def msd_tile(self):
-        v = _u32()
-        check(lib().anl_msd_get_tuning(self.handle, ctypes.byref(v)))
-        return v.value
+        return self.msd_shape()[0]
+
+    def msd_shape(self):
+        v, w = _u32(), _u32()
+        check(lib().anl_msd_get_tuning(self.handle, ctypes.byref(v), ctypes.byref(w)))
+        return v.value, w.value
 
     def msd_autotune(self, skip, tmax, cm_msd, cm_self, n, primo, candidates=None):
-        """Times the MSD calculation with each lag tile and keeps the fastest.
-        Returns (times in ms per candidate, index of the best)."""
-        cand = None if candidates is None else np.ascontiguousarray(candidates, dtype=np.uint32)
-        nc = 5 if cand is None else cand.size
+        """Times the MSD calculation with each launch shape (pairs (lag tile,
+        wavefronts); None = MSD_SHAPES) and keeps the fastest.  Returns
+        (times in ms per candidate, index of the best)."""
+        cand = None if candidates is None else np.ascontiguousarray(candidates, dtype=np.uint32).reshape(-1)
+        nc = len(self.MSD_SHAPES) if cand is None else cand.size // 2
         ms = np.zeros(nc, dtype=np.float32)
         best = _u32()
         check(lib().anl_msd_autotune(self.handle, skip, tmax, int(bool(cm_msd)), int(bool(cm_self)), n, primo,

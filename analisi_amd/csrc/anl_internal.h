@@ -94,6 +94,7 @@ struct anl_traj {
     anl::Screen screen_cache;
     double screen_rmax = -1.0;
     uint32_t msd_tile = 16;             // lag tile of the MSD window kernel (anl_msd_set_tuning)
+    uint32_t msd_waves = 4;             // its wavefronts per workgroup
 };
 
 struct anl_series {

@@ -115,15 +115,15 @@ __global__ __launch_bounds__(MSD_WG) void msd_atoms_kernel(const double4 *__rest
 // consecutive logical ids on one XCD (hardware id h runs on XCD h % 8), where
 // they share the x(t0) stream in L2.  y(f) = x(f) [- cm(f, type)]: with
 // drift removal the centre of mass is subtracted as each frame is loaded.
-template <int R, bool CMSELF>
-__global__ __launch_bounds__(MSD_WG) void msd_window_kernel(const double4 *__restrict__ pos,
+template <int R, int WV, bool CMSELF>
+__global__ __launch_bounds__(64 * WV) void msd_window_kernel(const double4 *__restrict__ pos,
                                                            const double *__restrict__ cm,
                                                            const int32_t *__restrict__ type_slot, uint64_t N,
                                                            uint32_t nt, const MsdBlock *__restrict__ blocks,
                                                            uint32_t leff, uint32_t nchunks, uint32_t nquads,
                                                            uint32_t per_xcd, uint32_t total,
                                                            double *__restrict__ partial) {
-    __shared__ double red[4][R][64];
+    __shared__ double red[WV][R][64];
     __shared__ int type_lds[64];
     const uint32_t h = blockIdx.x;
     const uint32_t L = (h & 7u) * per_xcd + (h >> 3);
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(MSD_WG) void msd_window_kernel(const double4 *__res
     const uint32_t chunk = rest % nchunks;
     const uint32_t blk = rest / nchunks;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t tb = (quad * 4u + wave) * (uint32_t)R;
+    const uint32_t tb = (quad * (uint32_t)WV + wave) * (uint32_t)R;
     const uint64_t slot = (uint64_t)chunk * 64u + lane;
     const uint64_t sl = slot < N ? slot : N - 1;  // lanes past N load a real atom and are not summed
     const MsdBlock b = blocks[blk];
@@ -164,9 +164,13 @@ __global__ __launch_bounds__(MSD_WG) void msd_window_kernel(const double4 *__res
         // t0 + s + tb + k; afterwards W[s] is refilled with frame
         // t0 + s + tb + R.  Every index is a compile-time constant once the
         // pass is unrolled, so the window stays in registers.
+        // x(t0) is loaded one step ahead (yn), so that its latency hides
+        // behind the previous step's R updates
+        double3 yn = load(b.primo);
         auto step = [&](uint32_t t0, auto sc) {
             constexpr int s = decltype(sc)::value;
-            const double3 ya = load(b.primo + t0 + (uint32_t)s);
+            const double3 ya = yn;
+            yn = load(b.primo + t0 + (uint32_t)s + 1u);
 #pragma unroll
             for (int k = 0; k < R; k++) {
                 const double3 wv = W[(s + k) % R];
@@ -190,10 +194,10 @@ __global__ __launch_bounds__(MSD_WG) void msd_window_kernel(const double4 *__res
     if (threadIdx.x < 64) type_lds[lane] = slot < N ? ti : -1;
     __syncthreads();
     // thread (wave, lag, type) sums the 64 slots in order
-    for (uint32_t w = threadIdx.x; w < 4u * (uint32_t)R * nt; w += MSD_WG) {
+    for (uint32_t w = threadIdx.x; w < (uint32_t)WV * R * nt; w += 64u * WV) {
         const uint32_t wv = w / ((uint32_t)R * nt), r = w % ((uint32_t)R * nt);
         const uint32_t k = r / nt, tt = r % nt;
-        const uint32_t t = (quad * 4u + wv) * (uint32_t)R + k;
+        const uint32_t t = (quad * (uint32_t)WV + wv) * (uint32_t)R + k;
         if (t >= leff) continue;
         double sum = 0.0;
         for (int j = 0; j < 64; j++)
@@ -235,10 +239,27 @@ __global__ void msd_cm_kernel(const double *__restrict__ cm, uint32_t nt, const 
     out[(uint64_t)b * leff * nt * 2 + (uint64_t)t * nt * 2 + nt + tt] = m;
 }
 
+// Launch shapes of msd_window_kernel: (lag tile R, wavefronts per workgroup).
+// A workgroup covers R * waves consecutive lags of 64 atoms and streams the
+// atoms' frames once for all of them, so wider workgroups read each frame
+// fewer times; the register window (R frames) and the workgroup size must
+// fit the VGPRs (R 24 needs 256: at most 8 wavefronts; R 16 at most 12).
+struct MsdShape {
+    uint32_t tile, waves;
+};
+constexpr MsdShape MSD_SHAPES[] = {{8, 4}, {12, 4}, {16, 4}, {24, 4}, {16, 8}, {24, 8}, {12, 12}, {16, 12}, {8, 16}};
+bool msd_shape_ok(uint32_t tile, uint32_t waves) {
+    if (tile == 0) return true;  // the eight-lags-per-thread kernel (waves ignored)
+    for (const MsdShape &m : MSD_SHAPES)
+        if (m.tile == tile && m.waves == waves) return true;
+    return false;
+}
+
 struct MsdPrepared {
     anl_traj *tr = nullptr;
     uint32_t skip = 1, leff = 0, nblocks = 0, nt = 0, fcm = 1, nchunks = 0;
     uint32_t tile = 0;  // lag tile R of msd_window_kernel; 0: msd_atoms_kernel
+    uint32_t waves = 4; // wavefronts per workgroup of msd_window_kernel
     bool cm_msd = false, cm_self = false;
     bool owns = true;   // the three buffers were allocated for this launch
     MsdBlock *d_blocks = nullptr;
@@ -284,6 +305,7 @@ static void msd_plan(anl_traj *tr, const uint64_t *primo, const uint32_t *n, uin
     P->cm_self = cm_self;
     // the window kernel needs t0 stride 1; its chunks are 64 atoms
     P->tile = skip == 1 ? tr->msd_tile : 0;
+    P->waves = tr->msd_waves;
     P->nchunks = (uint32_t)(P->tile ? (tr->N + 63) / 64 : (tr->N + MSD_WG - 1) / MSD_WG);
     hb->assign(nblocks, MsdBlock{});
     hc->assign((size_t)nblocks * tr->nt, 0);
@@ -326,25 +348,32 @@ void msd_enqueue_prepared(void *prep, double *out, hipStream_t s) {
     if (P.leff == 0) return;
     const anl_traj *tr = P.tr;
     if (P.tile) {
-        const uint32_t nquads = (P.leff + 4 * P.tile - 1) / (4 * P.tile);
+        const uint32_t span = P.waves * P.tile;  // lags per workgroup
+        const uint32_t nquads = (P.leff + span - 1) / span;
         const uint64_t total = (uint64_t)nquads * P.nchunks * P.nblocks;
         const uint64_t per = (total + 7) / 8;
         if (8 * per >= (1ull << 32)) fail(ANL_ERR_ARG, "MSD launch too large (reduce the number of blocks)");
         const dim3 grid((uint32_t)(8 * per));
-#define ANL_MSD_WIN(R, C)                                                                                      \
-    hipLaunchKernelGGL((msd_window_kernel<R, C>), grid, dim3(MSD_WG), 0, s, tr->pos, tr->cm, tr->type_slot, tr->N, \
-                       P.nt, P.d_blocks, P.leff, P.nchunks, nquads, (uint32_t)per, (uint32_t)total, P.d_partial)
-        switch (P.tile * 2 + (P.cm_self ? 1 : 0)) {
-            case 16: ANL_MSD_WIN(8, false); break;
-            case 17: ANL_MSD_WIN(8, true); break;
-            case 24: ANL_MSD_WIN(12, false); break;
-            case 25: ANL_MSD_WIN(12, true); break;
-            case 32: ANL_MSD_WIN(16, false); break;
-            case 33: ANL_MSD_WIN(16, true); break;
-            case 48: ANL_MSD_WIN(24, false); break;
-            case 49: ANL_MSD_WIN(24, true); break;
-            default: fail(ANL_ERR_ARG, "MSD lag tile must be 0, 8, 12, 16 or 24");
+#define ANL_MSD_WIN(R, W, C)                                                                             \
+    hipLaunchKernelGGL((msd_window_kernel<R, W, C>), grid, dim3(64 * W), 0, s, tr->pos, tr->cm, tr->type_slot, \
+                       tr->N, P.nt, P.d_blocks, P.leff, P.nchunks, nquads, (uint32_t)per, (uint32_t)total,       \
+                       P.d_partial)
+#define ANL_MSD_CASE(R, W)                          \
+    case (R) * 64 + (W) * 2: ANL_MSD_WIN(R, W, false); break; \
+    case (R) * 64 + (W) * 2 + 1: ANL_MSD_WIN(R, W, true); break;
+        switch (P.tile * 64 + P.waves * 2 + (P.cm_self ? 1 : 0)) {
+            ANL_MSD_CASE(8, 4)
+            ANL_MSD_CASE(12, 4)
+            ANL_MSD_CASE(16, 4)
+            ANL_MSD_CASE(24, 4)
+            ANL_MSD_CASE(16, 8)
+            ANL_MSD_CASE(24, 8)
+            ANL_MSD_CASE(12, 12)
+            ANL_MSD_CASE(16, 12)
+            ANL_MSD_CASE(8, 16)
+            default: fail(ANL_ERR_ARG, "unsupported MSD launch shape");
         }
+#undef ANL_MSD_CASE
 #undef ANL_MSD_WIN
     } else {
         const uint32_t nlb = (P.leff + MSD_TL - 1) / MSD_TL;
@@ -441,19 +470,24 @@ int anl_msd_calculate(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, in
     });
 }
 
-int anl_msd_set_tuning(anl_traj *tr, uint32_t lag_tile) {
+static const char *MSD_SHAPE_MSG =
+    "MSD launch shape must be lag tile 0, or (lag tile, wavefronts) one of (8,4) (12,4) (16,4) (24,4) (16,8) "
+    "(24,8) (12,12) (16,12) (8,16)";
+
+int anl_msd_set_tuning(anl_traj *tr, uint32_t lag_tile, uint32_t waves) {
     return guard([&] {
         if (!tr) fail(ANL_ERR_ARG, "null trajectory");
-        if (lag_tile != 0 && lag_tile != 8 && lag_tile != 12 && lag_tile != 16 && lag_tile != 24)
-            fail(ANL_ERR_ARG, "MSD lag tile must be 0, 8, 12, 16 or 24");
+        if (!msd_shape_ok(lag_tile, waves)) fail(ANL_ERR_ARG, MSD_SHAPE_MSG);
         tr->msd_tile = lag_tile;
+        tr->msd_waves = lag_tile ? waves : 4;
     });
 }
 
-int anl_msd_get_tuning(anl_traj *tr, uint32_t *lag_tile) {
+int anl_msd_get_tuning(anl_traj *tr, uint32_t *lag_tile, uint32_t *waves) {
     return guard([&] {
         if (!tr || !lag_tile) fail(ANL_ERR_ARG, "null pointer passed to anl_msd_get_tuning");
         *lag_tile = tr->msd_tile;
+        if (waves) *waves = tr->msd_waves;
     });
 }
 
@@ -464,17 +498,18 @@ int anl_msd_autotune(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int
                      const uint32_t *cand, uint32_t ncand, float *ms_out, uint32_t *best_out) {
     return guard([&] {
         if (!tr) fail(ANL_ERR_ARG, "null trajectory");
-        static const uint32_t defaults[] = {8, 12, 16, 24, 0};
+        // (lag tile, wavefronts) pairs
+        static const uint32_t defaults[] = {8, 4, 12, 4, 16, 4, 24, 4, 16, 8, 24, 8, 12, 12, 16, 12, 8, 16, 0, 4};
         if (!cand) {
             cand = defaults;
-            ncand = sizeof(defaults) / sizeof(defaults[0]);
+            ncand = sizeof(defaults) / sizeof(defaults[0]) / 2;
         }
         if (ncand == 0) fail(ANL_ERR_ARG, "no candidates");
         for (uint32_t c = 0; c < ncand; c++)
-            if (cand[c] != 0 && cand[c] != 8 && cand[c] != 12 && cand[c] != 16 && cand[c] != 24)
-                fail(ANL_ERR_ARG, "candidate " + std::to_string(c) + ": MSD lag tile must be 0, 8, 12, 16 or 24");
+            if (!msd_shape_ok(cand[2 * c], cand[2 * c + 1]))
+                fail(ANL_ERR_ARG, "candidate " + std::to_string(c) + ": " + MSD_SHAPE_MSG);
         ensure_device();
-        const uint32_t saved = tr->msd_tile;
+        const uint32_t saved = tr->msd_tile, saved_w = tr->msd_waves;
         const uint32_t leff = (n < tmax || tmax == 0) ? n : tmax;
         double *d = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -486,7 +521,8 @@ int anl_msd_autotune(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int
             ANL_HIP(hipEventCreate(&e0));
             ANL_HIP(hipEventCreate(&e1));
             for (uint32_t c = 0; c < ncand; c++) {
-                tr->msd_tile = cand[c];
+                tr->msd_tile = cand[2 * c];
+                tr->msd_waves = cand[2 * c] ? cand[2 * c + 1] : 4;
                 P = msd_prepare_blocks(tr, &primo, &n, 1, skip, tmax, cm_msd, cm_self, nullptr);
                 msd_enqueue_prepared(P, d, 0);
                 ANL_HIP(hipEventRecord(e0, 0));
@@ -505,6 +541,7 @@ int anl_msd_autotune(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int
             }
         } catch (...) {
             tr->msd_tile = saved;
+            tr->msd_waves = saved_w;
             if (P) {
                 (void)hipDeviceSynchronize();
                 msd_free_prepared(P);
@@ -517,7 +554,8 @@ int anl_msd_autotune(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipFree(d);
-        tr->msd_tile = cand[besti];
+        tr->msd_tile = cand[2 * besti];
+        tr->msd_waves = cand[2 * besti] ? cand[2 * besti + 1] : 4;
         if (best_out) *best_out = besti;
     });
 }

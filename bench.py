@@ -446,9 +446,10 @@ def run_msd(w, args, hip, traj, nt):
     tune = None
     if not args.no_autotune:
         ms_t, best = traj.msd_autotune(1, tmax, True, True, n, 0)
-        tune = {"lag_tiles": [8, 12, 16, 24, 0], "candidates_ms": [round(float(x), 3) for x in ms_t],
-                "chosen_lag_tile": traj.msd_tile()}
-        log("msd autotune: %s -> lag tile %d" % (tune["candidates_ms"], traj.msd_tile()))
+        tune = {"shapes": [list(x) for x in traj.MSD_SHAPES], "candidates_ms": [round(float(x), 3) for x in ms_t],
+                "chosen_shape": list(traj.msd_shape()),
+                "shape": "lag tile, wavefronts per workgroup (tile 0: eight lags per thread)"}
+        log("msd autotune: %s -> shape %s" % (tune["candidates_ms"], traj.msd_shape()))
     check(L.anl_msd_calculate_async(traj.handle, 1, tmax, 1, 1, n, 0, d, stream))
     hip.sync()
     e0, e1 = hip.event(), hip.event()

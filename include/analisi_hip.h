@@ -235,19 +235,24 @@ int anl_msd_calculate_async(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_m
  * and race. */
 int anl_msd_calculate_noalloc(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self, uint32_t n,
                               uint64_t primo, double *out_dev, void *stream);
-/* Launch shape of the MSD kernel for this trajectory: the lag tile R (8, 12,
- * 16 or 24; default 16) of the t0-stride-1 kernel, which holds R frames of
- * an atom in registers and loads two frames per t0 for R updates; 0 selects
- * the eight-lags-per-thread kernel that every other stride uses.  Tiles 8
- * to 24 give identical results (each (atom, lag) sum runs over t0 in order,
- * the same operations); tile 0 agrees with them to rounding (both within
- * 1e-10 of the reference).  anl_msd_autotune times each candidate tile
- * (cand[c]; NULL = 8, 12, 16, 24, 0) on the calculation reset(n) at primo
- * and keeps the fastest; ms_out[c] (may be NULL) receives the times.  The
- * reference's equivalent knob is nthreads (msd.h:30-47,
- * calculatemultithread.h:80-104); no counterpart for the tile. */
-int anl_msd_set_tuning(anl_traj *tr, uint32_t lag_tile);
-int anl_msd_get_tuning(anl_traj *tr, uint32_t *lag_tile);
+/* Launch shape of the MSD kernel for this trajectory: the lag tile R of the
+ * t0-stride-1 kernel, which holds R frames of an atom in registers and loads
+ * two frames per t0 for R updates, and its wavefronts per workgroup W; a
+ * workgroup covers R * W lags of 64 atoms, streaming their frames once for
+ * all of them.  (R, W) is one of (8,4) (12,4) (16,4) (24,4) (16,8) (24,8)
+ * (12,12) (16,12) (8,16); default (16,4).  Lag tile 0 selects the
+ * eight-lags-per-thread kernel that every other stride uses (W ignored).
+ * All shapes with R > 0 give identical results (each (atom, lag) sum runs
+ * over t0 in order with the same operations, and the workgroup sums its 64
+ * atoms in slot order); tile 0 agrees with them to rounding (both within
+ * 1e-10 of the reference).  anl_msd_autotune times each candidate shape
+ * (cand[2c], cand[2c+1]; NULL = the nine shapes above and tile 0) on the
+ * calculation reset(n) at primo and keeps the fastest; ms_out[c] (may be
+ * NULL) receives the times.  The reference's equivalent knob is nthreads
+ * (msd.h:30-47, calculatemultithread.h:80-104); no counterpart for the
+ * shape. */
+int anl_msd_set_tuning(anl_traj *tr, uint32_t lag_tile, uint32_t waves);
+int anl_msd_get_tuning(anl_traj *tr, uint32_t *lag_tile, uint32_t *waves);
 int anl_msd_autotune(anl_traj *tr, uint32_t skip, uint32_t tmax, int cm_msd, int cm_self, uint32_t n, uint64_t primo,
                      const uint32_t *cand, uint32_t ncand, float *ms_out, uint32_t *best_index);
 

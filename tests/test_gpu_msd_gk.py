@@ -26,27 +26,32 @@ def test_msd_matches_oracle(cm_msd, cm_self):
 
 @pytest.mark.parametrize("cm_self", [False, True])
 def test_msd_lag_tiles_and_autotune(cm_self):
-    """Every lag tile of the t0-stride-1 kernel (8, 12, 16, 24) gives the same
-    bits (each (atom, lag) sum runs over t0 in order); tile 0, the general
-    kernel, agrees to rounding; all are within 1e-10 of the oracle.  Lag
-    counts that are not multiples of the tile, an atom count that is not a
-    multiple of 64 and n not a multiple of the tile exercise the partial
-    tiles.  The autotuner keeps one of the candidates and the result stays
-    the same."""
+    """Every launch shape (lag tile, wavefronts per workgroup) of the
+    t0-stride-1 kernel gives the same bits (each (atom, lag) sum runs over t0
+    in order, the workgroup sums its atoms in slot order); tile 0, the
+    general kernel, agrees to rounding; all are within 1e-10 of the oracle.
+    Lag counts that are not multiples of a workgroup's lag span (from 32 to
+    192), an atom count that is not a multiple of 64 and n not a multiple of
+    the tile exercise the partial tiles.  The autotuner keeps one of the
+    candidates and the result stays the same."""
     ot, dt = make_case(21, 331, 90, 14.0, [0.6, 0.4], 0.12, unwrap_shift=-17.0)
+    shapes = dt.MSD_SHAPES
     for tmax, n, primo in [(37, 41, 5), (0, 29, 0), (50, 40, 0)]:
         ref = O.msd(ot, 1, tmax, 3, True, cm_self, n, primo)
         outs = {}
-        for tile in (8, 12, 16, 24, 0):
-            dt.set_msd_tile(tile)
-            assert dt.msd_tile() == tile
-            outs[tile] = capi.msd(dt, 1, tmax, True, cm_self, n, primo)
-            assert is_same(outs[tile], ref).all(), (tile, tmax, n)
-        for tile in (12, 16, 24):
-            assert np.array_equal(outs[tile], outs[8]), (tile, tmax, n)
+        for tile, waves in shapes:
+            dt.set_msd_tile(tile, waves)
+            assert dt.msd_shape() == (tile, waves)
+            outs[tile, waves] = capi.msd(dt, 1, tmax, True, cm_self, n, primo)
+            assert is_same(outs[tile, waves], ref).all(), (tile, waves, tmax, n)
+        for sh in shapes:
+            if sh[0]:
+                assert np.array_equal(outs[sh], outs[8, 4]), (sh, tmax, n)
         ms, best = dt.msd_autotune(1, tmax, True, cm_self, n, primo)
-        assert len(ms) == 5 and np.all(ms > 0) and dt.msd_tile() == [8, 12, 16, 24, 0][best]
-        assert np.array_equal(capi.msd(dt, 1, tmax, True, cm_self, n, primo), outs[dt.msd_tile()])
+        assert len(ms) == len(shapes) and np.all(ms > 0) and dt.msd_shape() == shapes[best]
+        assert np.array_equal(capi.msd(dt, 1, tmax, True, cm_self, n, primo), outs[dt.msd_shape()])
+    with pytest.raises(capi.AnlError):
+        dt.set_msd_tile(24, 12)     # 24 frames of registers do not fit 12 wavefronts
 
 
 def test_msd_errors():
