@@ -179,6 +179,10 @@ class DeviceTrajectory:
             pass
 
 
+# number of launch shapes anl_gofrt_autotune times when given no candidates
+GOFRT_DEFAULT_SHAPES = 9
+
+
 def screen_info(box9, rmax):
     """Screened-form parameters of a constant cell (internal-format box of 9
     values) computed on the host: dict with valid, threshold, rho_cap, dq,
@@ -243,7 +247,8 @@ class DeviceGofrt:
 
     def set_tuning(self, unroll, xcd, pairs_per_wg, cluster=2):
         """cluster: 0 off; 1 cluster screen over the groups the bounding-sphere table leaves; 2 the same with
-        the step's columns cached in LDS; 3 cluster screen over every j in order."""
+        the step's columns cached in LDS as FP32 offsets; 3 cluster screen over every j in order; 4 as 3 with
+        the column cache."""
         check(lib().anl_gofrt_set_tuning(self.handle, unroll, int(bool(xcd)), pairs_per_wg, int(cluster)))
 
     def tuning(self):
@@ -258,7 +263,7 @@ class DeviceGofrt:
         on the first nlag_probe lags of reset()'s calculation at primo; keeps
         the fastest.  Returns (times in ms per candidate, index of the best)."""
         if candidates is None:
-            cand, nc = None, 8
+            cand, nc = None, GOFRT_DEFAULT_SHAPES
         else:
             cand = np.ascontiguousarray(candidates, dtype=np.int32).reshape(-1, 4)
             nc = cand.shape[0]

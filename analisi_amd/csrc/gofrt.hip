@@ -651,12 +651,14 @@ __host__ __device__ __forceinline__ float cluster_threshold(float rho, float rma
     }
 
 // CMODE, the cluster screen's j loop: 1 the groups left by the bounding-
-// sphere test, two per step; 2 the same with each step's column fractions in
-// LDS for the candidates' evaluation; 3 every 64 consecutive j per step, the
-// next step's fractions loaded ahead (no group table).  0 without CLUSTER.
+// sphere test, two per step; 2 the same with each step's columns cached in
+// LDS as FP32 offsets for the candidates' evaluation; 3 every 64 consecutive
+// j per step, the next step's fractions loaded ahead (no group table); 4 as
+// 3 with the column cache of 2.  0 without CLUSTER.
 template <bool TRI, int UNROLL, bool CLUSTER, int CMODE>
 __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(const GofrtArgs a) {
-    constexpr bool COLC = CMODE == 2;
+    constexpr bool COLC = CMODE == 2 || CMODE == 4;    // column cache + offset evaluation
+    constexpr bool GROUPS = CMODE == 1 || CMODE == 2;  // j groups from the bounding-sphere table
     extern __shared__ unsigned char smem[];
     const WgId id = decode_wg(a.ntiles, a.nlag, a.nwork, a.per_xcd);
     if (!id.valid) return;
@@ -1017,7 +1019,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     };
                     // the groups left, two per step in order (uniform state)
                     uint32_t gbase = 0;
-                    unsigned long long M = CMODE != 3 ? window_mask(0) : 0ull;
+                    unsigned long long M = GROUPS ? window_mask(0) : 0ull;
                     auto next_pair = [&](uint32_t &jA, uint32_t &jB, bool &hasB) -> bool {
                         while (M == 0ull) {
                             gbase += 64;
@@ -1034,7 +1036,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         }
                         return true;
                     };
-                    if constexpr (CMODE != 3) {
+                    if constexpr (GROUPS) {
                         uint32_t jA = 0, jB = 0;
                         bool hasB = false;
                         while (next_pair(jA, jB, hasB)) {  // uniform
@@ -1145,8 +1147,8 @@ struct GofrtTuning {
     uint64_t pairs_per_wg = 1ull << 30;
     // cluster (MFMA) screen where a wavefront's atoms are compact enough: 0 off,
     // 1 on (far groups skipped from the bounding-sphere table), 2 as 1 with
-    // each step's column fractions cached in LDS, 3 on with every j visited
-    // in order (no table)
+    // each step's columns cached in LDS as FP32 offsets, 3 on with every j
+    // visited in order (no table), 4 as 3 with the column cache
     int cluster = 2;
 };
 
@@ -1398,7 +1400,7 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         return (cluster ? 2 : 1) * WG * sizeof(uint4) + (WG / 64) * (qcap + RQCAP) * sizeof(uint32_t) + common;
     };
     // the cluster screen needs the larger queue; without room it is not used
-    const bool colc = g->tune.cluster == 2;
+    const bool colc = g->tune.cluster == 2 || g->tune.cluster == 4;
     const int qcl = colc ? SQCAP_CLC : SQCAP_CL;
     const bool cluster_ok = g->tune.cluster != 0 && form == ANL_GOFRT_SCREENED && a.rho_cap > 0 &&
                             screened_lds(qcl, colc) <= 160 * 1024;
@@ -1432,7 +1434,9 @@ void gofrt_enqueue(anl_gofrt *g, const GofrtPrepared &P, unsigned long long *out
                 case 4: ANL_SCREENED(false, 8, true, 2); break;
                 case 5: ANL_SCREENED(true, 8, true, 2); break;
                 case 6: ANL_SCREENED(false, 8, true, 3); break;
-                default: ANL_SCREENED(true, 8, true, 3); break;
+                case 7: ANL_SCREENED(true, 8, true, 3); break;
+                case 8: ANL_SCREENED(false, 8, true, 4); break;
+                default: ANL_SCREENED(true, 8, true, 4); break;
             }
         } else {
             if (P.triclinic) {
@@ -1507,7 +1511,7 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
         g->form = form;
         if (const char *e = std::getenv("ANL_GOFRT_UNROLL")) g->tune.unroll = std::atoi(e) == 8 ? 8 : 4;
         if (const char *e = std::getenv("ANL_GOFRT_XCD")) g->tune.xcd = std::atoi(e) != 0;
-        if (const char *e = std::getenv("ANL_GOFRT_CLUSTER")) g->tune.cluster = std::min(3, std::max(0, std::atoi(e)));
+        if (const char *e = std::getenv("ANL_GOFRT_CLUSTER")) g->tune.cluster = std::min(4, std::max(0, std::atoi(e)));
         if (const char *e = std::getenv("ANL_GOFRT_PAIRS_PER_WG")) {
             long long v = std::atoll(e);
             if (v > 0) g->tune.pairs_per_wg = (uint64_t)v;
@@ -1693,7 +1697,7 @@ int anl_gofrt_set_tuning(anl_gofrt *g, int unroll, int xcd, uint64_t pairs_per_w
         g->tune.unroll = unroll;
         g->tune.xcd = xcd != 0;
         g->tune.pairs_per_wg = pairs_per_wg;
-        if (cluster < 0 || cluster > 3) fail(ANL_ERR_ARG, "cluster must be 0, 1, 2 or 3");
+        if (cluster < 0 || cluster > 4) fail(ANL_ERR_ARG, "cluster must be 0, 1, 2, 3 or 4");
         g->tune.cluster = cluster;
     });
 }
@@ -1718,8 +1722,8 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         ensure_device();
         check_lengths(g, &primo, &g->n, 1);
         // unroll, xcd, log2 pairs per workgroup, cluster
-        static const int32_t defaults[] = {8, 0, 29, 2, 8, 0, 30, 2, 8, 0, 31, 2, 8, 0, 29, 1,
-                                           8, 0, 29, 3, 8, 0, 30, 3, 8, 1, 29, 2, 8, 0, 30, 0};
+        static const int32_t defaults[] = {8, 0, 29, 2, 8, 0, 30, 2, 8, 0, 31, 2, 8, 0, 29, 1, 8, 0, 29, 3,
+                                           8, 0, 29, 4, 8, 0, 30, 4, 8, 1, 29, 2, 8, 0, 30, 0};
         if (!cand) {
             cand = defaults;
             ncand = sizeof(defaults) / sizeof(defaults[0]) / 4;
@@ -1727,9 +1731,9 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         if (ncand == 0) fail(ANL_ERR_ARG, "no candidates");
         for (uint32_t c = 0; c < ncand; c++)
             if ((cand[4 * c] != 4 && cand[4 * c] != 8) || cand[4 * c + 2] < 10 || cand[4 * c + 2] > 40 ||
-                cand[4 * c + 3] < 0 || cand[4 * c + 3] > 3)
+                cand[4 * c + 3] < 0 || cand[4 * c + 3] > 4)
                 fail(ANL_ERR_ARG, "candidate " + std::to_string(c) +
-                                      ": unroll 4|8, log2(pairs per workgroup) in [10, 40], cluster 0..3");
+                                      ": unroll 4|8, log2(pairs per workgroup) in [10, 40], cluster 0..4");
         // probe lags: at least nlag_probe, and enough for >= 4096 workgroups
         // (several waves of residency), so that the probe behaves like the run
         const uint32_t nlag_all = (g->leff + g->every - 1) / g->every;

@@ -130,13 +130,14 @@ int anl_gofrt_screen_info(const double *box9, double rmax, int *valid, uint32_t 
  * the t0 chunk of one workgroup), and the cluster screen (0: off; 1: on,
  * wavefronts whose 64 atoms are compact screen their pairs with FP16 MFMA
  * distance tiles, skipping groups of 32 atoms that a per-frame table of
- * bounding spheres shows far; 2: as 1, with each step's 64 column atoms'
- * 32-bit fractions cached in LDS for the candidates' evaluation; 3: on,
- * visiting every j in order without the table).  Shapes change speed only,
+ * bounding spheres shows far; 2: as 1, with each step's 64 column atoms
+ * cached in LDS as FP32 offsets from the wavefront's centre for the
+ * candidates' evaluation; 3: on, visiting every j in order without the
+ * table; 4: as 3, with the column cache of 2).  Shapes change speed only,
  * never results.  anl_gofrt_autotune
  * times each candidate (cand[4*c] = unroll, cand[4*c+1] = xcd, cand[4*c+2] =
- * log2 pairs per workgroup, cand[4*c+3] = cluster 0..3; NULL = a built-in
- * set of 8) on the first lags of the calculation set by reset() at primo (at least
+ * log2 pairs per workgroup, cand[4*c+3] = cluster 0..4; NULL = a built-in
+ * set of 9) on the first lags of the calculation set by reset() at primo (at least
  * nlag_probe, raised until the probe launch has >= 4096 workgroups), keeps
  * the fastest and reports ms_out[c] (may be NULL) and its index.  The reference's equivalent knob is nthreads
  * (calculatemultithread.h:80-104); no counterpart for the rest. */

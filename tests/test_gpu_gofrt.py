@@ -140,7 +140,7 @@ def test_bin_edges_adversarial_cluster(tilt):
     assert ref.sum() >= M  # every planted pair is inside [rmin, rmax] or at its edges
     g = capi.DeviceGofrt(dt, rmin, rmax, nbin, 1, 1, 1, capi.GOFRT_SCREENED)
     g.reset(1)
-    for cl in (2, 1, 3, 0):
+    for cl in (2, 1, 4, 3, 0):
         g.set_tuning(8, 0, 1 << 30, cl)
         got = g.calculate(0)
         assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
@@ -185,13 +185,13 @@ def test_launch_shapes_and_autotune_keep_counts(form):
     g = capi.DeviceGofrt(dt, 0.0, 6.0, 48, 9, 1, 1, form)
     g.reset(20)
     for unroll, xcd, lp, cl in [(4, 0, 31, 1), (8, 1, 31, 0), (4, 1, 12, 2), (8, 0, 16, 0), (4, 0, 40, 1),
-                                (8, 0, 30, 2), (8, 1, 29, 3)]:
+                                (8, 0, 30, 2), (8, 1, 29, 3), (8, 0, 18, 4)]:
         g.set_tuning(unroll, xcd, 1 << lp, cl)
         assert g.tuning() == (unroll, xcd, 1 << lp, cl)
         got = g.calculate(0)
         assert first_diff(got, ref) is None, (unroll, xcd, lp, cl, first_diff(got, ref))
     ms, best = g.autotune(0, 3)
-    assert len(ms) == 8 and (ms > 0).all() and 0 <= best < 8
+    assert len(ms) == 9 and (ms > 0).all() and 0 <= best < 9
     assert first_diff(g.calculate(0), ref) is None
     ms, best = g.autotune(0, 2, [(8, 1, 20, 1), (4, 0, 22, 0)])
     assert g.tuning() == [(8, 1, 1 << 20, 1), (4, 0, 1 << 22, 0)][best]
@@ -199,7 +199,7 @@ def test_launch_shapes_and_autotune_keep_counts(form):
     with pytest.raises(capi.AnlError):
         g.autotune(0, 2, [(5, 0, 20, 1)])
     with pytest.raises(capi.AnlError):
-        g.autotune(0, 2, [(8, 0, 20, 4)])
+        g.autotune(0, 2, [(8, 0, 20, 5)])
 
 
 # Boxes large against rmax, so that compact wavefronts take the cluster screen:
@@ -237,7 +237,7 @@ def test_cluster_screen_matches_oracle(case):
     ref, _ = O.gofrt(ot, 0.0, rmax, nbin, tmax, 8, 1, 1, n, 1, want_vdata=False)
     g = capi.DeviceGofrt(dt, 0.0, rmax, nbin, tmax, 1, 1, capi.GOFRT_SCREENED)
     g.reset(n)
-    for cl in (3, 2, 1, 0):
+    for cl in (4, 3, 2, 1, 0):
         g.set_tuning(8, 0, 1 << 30, cl)
         got = g.calculate(1)
         assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
