@@ -1722,7 +1722,7 @@ int anl_gofrt_autotune(anl_gofrt *g, uint64_t primo, uint32_t nlag_probe, const 
         ensure_device();
         check_lengths(g, &primo, &g->n, 1);
         // unroll, xcd, log2 pairs per workgroup, cluster
-        static const int32_t defaults[] = {8, 0, 29, 2, 8, 0, 30, 2, 8, 0, 27, 2, 8, 0, 29, 1, 8, 0, 29, 3,
+        static const int32_t defaults[] = {8, 0, 29, 2, 8, 0, 30, 2, 8, 0, 31, 2, 8, 0, 29, 1, 8, 0, 29, 3,
                                            8, 0, 29, 4, 8, 0, 30, 4, 8, 1, 29, 2, 8, 0, 30, 0};
         if (!cand) {
             cand = defaults;

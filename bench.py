@@ -603,7 +603,27 @@ def run_blocks(w, args, hip):
         for q in range(nw):
             mv2.add(mper[q].ravel())
         mv2.end(nw)
-        res["cpu_baseline"] = None
+        # the oracle's g(r,t) on a bounded slice of the same window, timed on
+        # the host cores, its counts checked against the device's
+        nthr, nproc, aff = host_cores()
+        cn = args.cpu_n or 16
+        Fs = min(cn, w["tmax"]) + cn
+        sl = w["pos"][int(starts[b]):int(starts[b]) + Fs]
+        ot = O.Traj(sl, w["types"], np.tile(w["box_l"], (Fs, 1)), w["triclinic"])
+        log("cpu baseline: oracle on window %d, reset(%d), %d threads" % (b, cn, nthr))
+        t = time.perf_counter()
+        cc, _ = O.gofrt(ot, w["rmin"], w["rmax"], w["nbin"], w["tmax"], nthr, 1, 1, cn, 0, want_vdata=False)
+        cdt = time.perf_counter() - t
+        g64.reset(cn)
+        same_cpu = bool(np.array_equal(g64.calculate(int(starts[b])), cc))
+        res["cpu_baseline"] = {"value": n_updates(w["N"], cn, w["tmax"]) / cdt, "unit": "pair-lag updates/s",
+                               "cores": nthr, "kind": "port", "host_cpus": nproc, "affinity_cpus": aff,
+                               "sample": "oracle (reference loop nest, every ordered pair) on window %d, reset(%d): "
+                                         "%d t0 x %d lags, %.1f s on %d threads" % (b, cn, cn, min(cn, w["tmax"]),
+                                                                                   cdt, nthr),
+                               "counts_identical_to_gpu": same_cpu}
+        log("cpu baseline %.3e upd/s on %d threads (%.1f s), identical=%s"
+            % (res["cpu_baseline"]["value"], nthr, cdt, same_cpu))
         res["checks"]["mediavar_identical_to_oracle"] = bool(
             np.array_equal(gmean.ravel(), mv.mean) and np.array_equal(gvar.ravel(), mv.var)
             and np.array_equal(mmean.ravel(), mv2.mean) and np.array_equal(mvar.ravel(), mv2.var))
@@ -873,7 +893,7 @@ def main():
     ap.add_argument("--shape", default="", help="g(r,t) launch shape 'unroll,xcd,log2 pairs per workgroup,cluster' "
                                                 "instead of the autotuner (for reproducible profiles)")
     ap.add_argument("--cpu-n", type=int, default=0,
-                    help="reset(n) of the CPU baseline sample (n t0 x n lags; 0 = 2 on c3, 8 on c2)")
+                    help="reset(n) of the CPU baseline sample (n t0 x n lags; 0 = 2 on c3, 8 on c2, 16 on a c4 window)")
     ap.add_argument("--c4-windows", type=int, default=0, help="number of c4 windows (0 = the workload's 64)")
     ap.add_argument("--c4-split", type=int, default=1, help="c4 windows cut into this many graphs per step")
     ap.add_argument("--gk-cpu-n", type=int, default=6000, help="reset(n), n lags, of the GK oracle sample")
