@@ -110,12 +110,15 @@ def test_bin_edges_adversarial():
         assert first_diff(got, ref) is None, (form, first_diff(got, ref))
 
 
+@pytest.mark.parametrize("origin", [20.0, -6.0], ids=["inside", "corner"])
 @pytest.mark.parametrize("tilt", [None, (2.5, -1.5, 1.0)], ids=["cubic", "tri"])
-def test_bin_edges_adversarial_cluster(tilt):
+def test_bin_edges_adversarial_cluster(tilt, origin):
     """Pairs within a few ulps of bin edges, rmin and rmax, inside one compact
     cloud, so that wavefronts take the cluster screen: every mode (including
-    mode 2's offset-difference evaluation and its radius-dependent window)
-    must give the reference's float-rounded bins."""
+    the offset-difference evaluation of modes 2 and 4 and its radius-dependent
+    window) must give the reference's float-rounded bins.  The corner cloud
+    straddles the cell's corner, so pairs and offsets cross the periodic
+    boundary in every direction."""
     from analisi_amd import synth
     rmin, rmax, nbin = 0.5, 4.5, 40
     dr = (rmax - rmin) / nbin
@@ -127,7 +130,7 @@ def test_bin_edges_adversarial_cluster(tilt):
     M = len(d)
     L = 60.0
     rng = np.random.default_rng(5)
-    base = 20.0 + 12.0 * rng.random((M, 3))
+    base = origin + 12.0 * rng.random((M, 3))
     pos = np.zeros((2, 2 * M, 3))
     pos[:, 0::2] = base
     pos[:, 1::2] = base
