@@ -20,12 +20,13 @@
 //     cos/sin(m phi), and added into the wave's density block in LDS in queue
 //     order.  The block is written once, as
 //     S[frame][type_j][bin][l][atom by type][m].
-//  2. sph_gram_kernel: for every problem (type_i, type_j, bin, l) the lag
-//     products sum_{i of type_i} sum_m S_{t1} S_{t2} are a banded Gram matrix
-//     over frames with K = (atoms of type_i) x (2l+1) contiguous in HBM.  A
-//     256-thread workgroup computes a 64 (t1) x 64 (t2) tile, 4x4 FP64 FMA per
-//     thread, K staged through LDS 16 at a time (double-buffered); entries
-//     with 0 <= t2 - t1 < leff go to G[problem][t1 index][t2 - t1].
+//  2. sph_gram_mfma_kernel: for every problem (type_i, type_j, bin, l) the
+//     lag products sum_{i of type_i} sum_m S_{t1} S_{t2} are a banded Gram
+//     matrix over frames with K = (atoms of type_i) x (2l+1) contiguous in
+//     HBM.  A 256-thread workgroup takes a block of 64 frames t2 and 32 of
+//     the t1 rows whose band reaches it, on FP64 MFMA (16 x 16 x 4), K staged
+//     through LDS 32 at a time; entries with 0 <= t2 - t1 < leff go to
+//     G[problem][t1 index][t2 - t1].
 //  3. sph_reduce_kernel: R[problem][dt] += sum over t1 of G, in t1 order.
 // The host divides R by (number of t1) x (atoms of type_i) and writes the
 // reference layout [dt][type_i][type_j][bin][k] with k holding l = lmax - k.
@@ -50,8 +51,7 @@ namespace anl {
 
 constexpr int SPH_WAVES = 4;           // wavefronts (central atoms) per density workgroup
 constexpr uint32_t SPH_WAVE_LDS = 2048;  // doubles of LDS per wavefront (16 KiB): density block + staging
-constexpr int GT = 64;                 // Gram tile edge (t1 rows, t2 columns)
-constexpr int GKC = 16;                // K values staged per step
+constexpr int GT = 64;                 // t1 values per chunk are rounded to multiples of this
 
 // doubles of a wavefront's staging area: the harmonic vectors of 64/(lmax+1)
 // atoms, and at least 64 queue entries of 4 doubles
@@ -532,17 +532,28 @@ struct SphGramArgs {
     uint64_t N, FS;
     uint32_t nt, nbin, lmax;
     uint32_t n0c, skip, leff, nfr;
-    uint32_t ncol;            // column tiles per row tile
+    uint32_t ncol;            // row sub-blocks per column block
 };
 
-// RT t1 rows x CT = 4096/RT t2 columns per workgroup, 4x4 per thread
-template <int RT>
-__global__ __launch_bounds__(256) void sph_gram_kernel(SphGramArgs a) {
-    constexpr int CT = 4096 / RT;
-    constexpr int TC = CT / 4;                      // threads along the columns
-    constexpr int VA = RT / 16, VB = CT / 16;       // K values each thread stages per step
-    __shared__ double As[2][GKC][RT];
-    __shared__ double Bs[2][GKC][CT];
+// Banded Gram tiles on the matrix cores (v_mfma_f64_16x16x4f64), by column
+// block: a workgroup takes CB = 64 consecutive frames t2 of one problem and
+// 32 of the t1 rows whose lag band [t1, t1 + leff) meets them, so each frame
+// of the chunk is read by ceil(rows / 32) workgroups and a t1 row by the
+// (leff + 63) / 64 column blocks it spans, instead of every row tile reading
+// every column of its band.  Four wavefronts each compute 16 rows x 32
+// columns: per step of 4 K values a lane feeds one A value and two B values
+// (A[row l%16][k l/16], B[k l/16][col l%16]) and the results land as
+// D[l/16 + 4i][l%16].  K is staged through LDS 32 values at a time,
+// row-major with one pad double per row.  Every entry's K sum runs in the
+// same order whatever the tiling, so chunking the t1 rows changes nothing.
+constexpr int GCB = 64;   // frames per column block
+constexpr int GRB = 32;   // t1 rows per workgroup
+constexpr int GKM = 32;   // K values per LDS stage
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void sph_gram_mfma_kernel(SphGramArgs a) {
+    __shared__ double As[GRB][GKM + 1];
+    __shared__ double Bs[GCB][GKM + 1];
     const uint32_t L1 = a.lmax + 1, L2 = L1 * L1;
     uint32_t y = blockIdx.y;
     const uint32_t l = y % L1;
@@ -550,85 +561,72 @@ __global__ __launch_bounds__(256) void sph_gram_kernel(SphGramArgs a) {
     const uint32_t bin = y % a.nbin;
     y /= a.nbin;
     const uint32_t jt = y % a.nt, ti = y / a.nt;
-    const uint32_t rt = blockIdx.x / a.ncol, ct = blockIdx.x - rt * a.ncol;
-    const uint32_t e0 = rt * RT;
-    const uint32_t rows = min((uint32_t)RT, a.n0c - e0);
-    const uint64_t t1_first = (uint64_t)e0 * a.skip;
-    const uint64_t t1_last = (uint64_t)(e0 + rows - 1) * a.skip;
-    const uint64_t c0 = t1_first + (uint64_t)ct * CT;
-    if (c0 >= a.nfr || c0 > t1_last + a.leff - 1) return;  // the whole workgroup
+    const uint32_t cb = blockIdx.x / a.ncol, rs = blockIdx.x - cb * a.ncol;  // ncol: row sub-blocks per column block
+    const uint64_t c0 = (uint64_t)cb * GCB;
+    if (c0 >= a.nfr) return;  // the whole workgroup
+    // rows e whose band meets [c0, c0 + GCB): e skip <= c0 + GCB - 1 and e skip + leff - 1 >= c0
+    const uint64_t e_lo = c0 + 1 > a.leff ? (c0 + 1 - a.leff + a.skip - 1) / a.skip : 0;
+    const uint64_t e_hi = std::min<uint64_t>(a.n0c - 1, (c0 + GCB - 1) / a.skip);
+    const uint64_t e0 = e_lo + (uint64_t)rs * GRB;
+    if (e0 > e_hi) return;
 
     const uint32_t w = 2 * l + 1;
     const uint64_t K = (uint64_t)(a.tstart[ti + 1] - a.tstart[ti]) * w;
     const uint64_t base = ((uint64_t)(jt * a.nbin + bin) * L2 + l * l) * a.N + (uint64_t)a.tstart[ti] * w;
 
-    const uint32_t tid = threadIdx.x;
-    // loaders: thread stages VA consecutive K values of row ar and VB of column bc
-    const uint32_t ar = tid / (16 / VA), ak = (tid % (16 / VA)) * VA;
-    const uint32_t bc = tid / (16 / VB), bk = (tid % (16 / VB)) * VB;
-    const uint32_t ea = e0 + ar;
-    const bool va = ea < a.n0c;
-    const uint64_t fb = c0 + bc;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // loaders: A row tid / 8, 4 K values; B frame tid / 4, 8 K values
+    const uint32_t ar = tid >> 3, ak = (tid & 7u) * 4u;
+    const uint32_t bf = tid >> 2, bk = (tid & 3u) * 8u;
+    const uint64_t ea = e0 + ar;
+    const bool va = ea <= e_hi;
+    const uint64_t fb = c0 + bf;
     const bool vb = fb < a.nfr;
-    const double *Ap = a.S + (va ? (uint64_t)ea * a.skip : 0) * a.FS + base;
+    const double *Ap = a.S + (va ? ea * a.skip : 0) * a.FS + base;
     const double *Bp = a.S + (vb ? fb : 0) * a.FS + base;
-    const uint32_t tr = tid / TC, tc = tid % TC;   // compute: rows 4tr.., columns 4tc..
+    // compute: wavefront (wr, wc) holds rows 16 wr.., columns 32 wc..
+    const uint32_t wr = wave >> 1, wc = wave & 1u;
+    const uint32_t lr = lane & 15u, lk = lane >> 4;
 
-    double acc[4][4];
+    f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t kc = 0; kc < K; kc += GKM) {
+        double ra[4], rb[8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    double ra[VA], rb[VB];
-    auto load = [&](uint64_t kc) {
-#pragma unroll
-        for (int u = 0; u < VA; ++u) {
+        for (int u = 0; u < 4; ++u) {
             const uint64_t k = kc + ak + u;
             ra[u] = (va && k < K) ? Ap[k] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < VB; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const uint64_t k = kc + bk + u;
             rb[u] = (vb && k < K) ? Bp[k] : 0.0;
         }
-    };
-    load(0);
-    uint32_t buf = 0;
-    for (uint64_t kc = 0; kc < K; kc += GKC) {
+        __syncthreads();  // the previous stage's reads are done
 #pragma unroll
-        for (int u = 0; u < VA; ++u) As[buf][ak + u][ar] = ra[u];
+        for (int u = 0; u < 4; ++u) As[ar][ak + u] = ra[u];
 #pragma unroll
-        for (int u = 0; u < VB; ++u) Bs[buf][bk + u][bc] = rb[u];
+        for (int u = 0; u < 8; ++u) Bs[bf][bk + u] = rb[u];
         __syncthreads();
-        if (kc + GKC < K) load(kc + GKC);
 #pragma unroll
-        for (int kk = 0; kk < GKC; ++kk) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[buf][kk][tr * 4 + i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][kk][tc * 4 + j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_fma(av[i], bv[j], acc[i][j]);
+        for (int kk = 0; kk < GKM; kk += 4) {
+            const double av = As[16 * wr + lr][kk + lk];
+            const double b0 = Bs[32 * wc + lr][kk + lk];
+            const double b1 = Bs[32 * wc + 16 + lr][kk + lk];
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1, acc1, 0, 0, 0);
         }
-        // the next step fills the other buffer; the one read here is refilled
-        // only after the next step's barrier, which every thread reaches after
-        // finishing these reads
-        buf ^= 1u;
     }
     double *Gp = a.G + (uint64_t)blockIdx.y * a.n0c * a.leff;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const uint32_t e = e0 + tr * 4 + i;
-        if (e >= a.n0c) continue;
-        const uint64_t t1 = (uint64_t)e * a.skip;
+        const uint64_t e = e0 + 16 * wr + lk + 4 * i;
+        if (e > e_hi) continue;
+        const uint64_t t1 = e * a.skip;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint64_t t2 = c0 + tc * 4 + j;
+        for (int h = 0; h < 2; ++h) {
+            const uint64_t t2 = c0 + 32 * wc + 16 * h + lr;
             if (t2 >= a.nfr || t2 < t1 || t2 - t1 >= a.leff) continue;
-            Gp[(uint64_t)e * a.leff + (t2 - t1)] = acc[i][j];
+            Gp[e * a.leff + (t2 - t1)] = h ? acc1[i] : acc0[i];
         }
     }
 }
@@ -968,14 +966,13 @@ void sph_calculate(anl_sph *h, uint64_t primo, uint32_t n, uint32_t tmax, uint32
         g.skip = skip;
         g.leff = leff;
         g.nfr = nfr;
-        // 64 x 64 tiles.  32 x 128 tiles compute fewer entries outside the
-        // lag band at t0 stride > 1, but ran 1.4x slower per launch on the
-        // 2,048-atom stride-10 workload (26.1 vs 18.8 ms)
-        constexpr uint32_t RT = 64;
-        const uint64_t span = std::min<uint64_t>((std::min<uint64_t>(n0c, RT) - 1) * skip + leff, nfr);
-        g.ncol = (uint32_t)((span + (4096 / RT) - 1) / (4096 / RT));
-        const uint32_t nrt = (n0c + RT - 1) / RT;
-        hipLaunchKernelGGL(sph_gram_kernel<RT>, dim3(nrt * g.ncol, nprob), dim3(256), 0, s, g);
+        // column blocks of 64 frames, each with the row sub-blocks of 32 t1
+        // values whose lag band meets it
+        const uint64_t rows_max = std::min<uint64_t>(n0c, (uint64_t)(GCB - 1 + leff - 1) / skip + 1);
+        g.ncol = (uint32_t)((rows_max + GRB - 1) / GRB);
+        const uint64_t ncb = (nfr + GCB - 1) / GCB;
+        if (ncb * g.ncol > 0x7fffffffull) fail(ANL_ERR_ARG, "too many Gram tiles in one launch");
+        hipLaunchKernelGGL(sph_gram_mfma_kernel, dim3((uint32_t)(ncb * g.ncol), nprob), dim3(256), 0, s, g);
         ANL_HIP(hipGetLastError());
         hipLaunchKernelGGL(sph_reduce_kernel, dim3((uint32_t)((nR + 255) / 256)), dim3(256), 0, s, G, R, nprob,
                            n0c, leff);
