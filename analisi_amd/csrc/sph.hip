@@ -544,7 +544,8 @@ struct SphGramArgs {
 // columns: per step of 4 K values a lane feeds one A value and two B values
 // (A[row l%16][k l/16], B[k l/16][col l%16]) and the results land as
 // D[l/16 + 4i][l%16].  K is staged through LDS 32 values at a time,
-// row-major with one pad double per row.  Every entry's K sum runs in the
+// row-major with one pad double per row, the next stage's values loaded into
+// registers during this stage's MFMAs.  Every entry's K sum runs in the
 // same order whatever the tiling, so chunking the t1 rows changes nothing.
 constexpr int GCB = 64;   // frames per column block
 constexpr int GRB = 32;   // t1 rows per workgroup
@@ -589,8 +590,10 @@ __global__ __launch_bounds__(256) void sph_gram_mfma_kernel(SphGramArgs a) {
     const uint32_t lr = lane & 15u, lk = lane >> 4;
 
     f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    for (uint64_t kc = 0; kc < K; kc += GKM) {
-        double ra[4], rb[8];
+    // the next stage's K values are loaded into registers while this stage's
+    // MFMAs run
+    double ra[4], rb[8];
+    auto load = [&](uint64_t kc) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint64_t k = kc + ak + u;
@@ -601,12 +604,16 @@ __global__ __launch_bounds__(256) void sph_gram_mfma_kernel(SphGramArgs a) {
             const uint64_t k = kc + bk + u;
             rb[u] = (vb && k < K) ? Bp[k] : 0.0;
         }
+    };
+    load(0);
+    for (uint64_t kc = 0; kc < K; kc += GKM) {
         __syncthreads();  // the previous stage's reads are done
 #pragma unroll
         for (int u = 0; u < 4; ++u) As[ar][ak + u] = ra[u];
 #pragma unroll
         for (int u = 0; u < 8; ++u) Bs[bf][bk + u] = rb[u];
         __syncthreads();
+        if (kc + GKM < K) load(kc + GKM);
 #pragma unroll
         for (int kk = 0; kk < GKM; kk += 4) {
             const double av = As[16 * wr + lr][kk + lk];
