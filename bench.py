@@ -245,7 +245,7 @@ def profile_figure(key, workload):
 
 
 def host_cores():
-    """Threads the CPU baseline uses (at most 16, the box's share of one GPU)
+    """Threads the CPU baseline uses (at most 16, one GPU's share of the host's cores)
     and what the host reports."""
     nproc = os.cpu_count() or 1
     try:
