@@ -218,16 +218,19 @@ __device__ __forceinline__ void store_density(const SphDensArgs &a, const double
     const uint32_t L2 = (a.lmax + 1) * (a.lmax + 1);
     double *Sf = a.S + (uint64_t)fl * a.FS;
     const uint64_t ap = (uint64_t)a.aprime[i];
-    for (uint32_t k = lane; k < a.nt * nbc * L2; k += 64) {
-        const uint32_t blk = k / L2, e = k - blk * L2;
-        const uint32_t jt = blk / nbc, bl = blk - jt * nbc;
-        if (b0 + bl >= a.nbin) continue;
-        uint32_t l = (uint32_t)sqrtf((float)e);
-        while (l * l > e) --l;
-        while ((l + 1) * (l + 1) <= e) ++l;
-        const uint64_t blk_off = ((uint64_t)(jt * a.nbin + b0 + bl) * L2 + l * l) * a.N;
-        Sf[blk_off + ap * (2 * l + 1) + (e - l * l)] = acc[k];
-    }
+    // blocks (type_j, bin) in a uniform loop, the (l, m) entries of a block
+    // across the lanes: no per-element division
+    for (uint32_t jt = 0; jt < a.nt; ++jt)
+        for (uint32_t bl = 0; bl < nbc && b0 + bl < a.nbin; ++bl) {
+            const double *src = acc + (jt * nbc + bl) * L2;
+            const uint64_t boff = (uint64_t)(jt * a.nbin + b0 + bl) * L2;
+            for (uint32_t e = lane; e < L2; e += 64) {
+                uint32_t l = (uint32_t)sqrtf((float)e);
+                while (l * l > e) --l;
+                while ((l + 1) * (l + 1) <= e) ++l;
+                Sf[(boff + l * l) * a.N + ap * (2 * l + 1) + (e - l * l)] = src[e];
+            }
+        }
 }
 
 template <bool TRI>
