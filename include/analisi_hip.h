@@ -313,8 +313,9 @@ int anl_gk_calculate_mfma_async(anl_series *s, const uint32_t *idx_j, uint32_t n
  * chunk length changes the FP32 rounding within the MFMA form's error bound.
  * anl_gk_autotune times each candidate (cand[3c..3c+2]; NULL = a built-in
  * set of 6) on the lag products of the given blocks (reset(n[b]) at primo[b],
- * as anl_gk_calculate_blocks_async) and keeps the fastest; ms_out[c] (may be
- * NULL) receives the times.  The reference's only knob is nthreads
+ * as anl_gk_calculate_blocks_async: one untimed call, then the fastest of
+ * three timed ones) and keeps the fastest; ms_out[c] (may be NULL) receives
+ * the times.  The reference's only knob is nthreads
  * (greenkuboNcomponentionicfluid.h:26-51); no counterpart for the shape. */
 int anl_gk_set_tuning(anl_series *s, uint32_t tiles_per_wave, uint32_t chunk_steps, uint32_t waves_per_group);
 int anl_gk_get_tuning(anl_series *s, uint32_t *tiles_per_wave, uint32_t *chunk_steps, uint32_t *waves_per_group);
