@@ -48,6 +48,7 @@ SIGNATURES = [
     ("anl_gofrt_calc_single_th", _i, [_vp, _i, _i, _i, _i, _u64, _vp]),
     ("anl_gofrt_calculate_blocks_async", _i, [_vp, _vp, _vp, _u32, _vp, _vp]),
     ("anl_gofrt_cluster_stats", _i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
+    ("anl_gofrt_recheck_stats", _i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     ("anl_gofrt_set_tuning", _i, [_vp, _i, _i, _u64, _i]),
     ("anl_gofrt_get_tuning", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("anl_gofrt_autotune", _i, [_vp, _u64, _u32, _vp, _u32, _vp, _vp]),
@@ -238,6 +239,14 @@ class DeviceGofrt:
         v = [_u64() for _ in range(5)]
         check(lib().anl_gofrt_pair_stats(self.handle, *[ctypes.byref(x) for x in v]))
         return dict(zip(("evaluated", "all", "tile", "screen16", "fp64"), (x.value for x in v)))
+
+    def recheck_stats(self):
+        """(window, exact): candidates re-evaluated in FP64 from their fractions (FP32 distance near an edge),
+        and those of them that took the reference's FP64 sequence from the positions (still within the FP64
+        bound of an edge), in the last screened calculate."""
+        w, x = _u64(), _u64()
+        check(lib().anl_gofrt_recheck_stats(self.handle, ctypes.byref(w), ctypes.byref(x)))
+        return w.value, x.value
 
     def cluster_stats(self):
         """(clustered, all) (wavefront, t0) units of the last screened calculate."""

@@ -76,6 +76,7 @@ struct anl_traj {
     uint32_t stride = 6;
     bool triclinic = false;
     bool box_const = false;             // every frame has bitwise the same box
+    double pos_max = 0.0;               // largest |position component| of the trajectory
     double4 *pos = nullptr;             // [F][N] slots, w = 0
     double *box = nullptr;              // [F][9] internal format, tilts 0 if ortho
     int32_t *type_slot = nullptr;       // [N]

@@ -230,6 +230,50 @@ static void du_window_offsets(const Screen &s, double rmax, double dr, uint32_t 
     *m1 = (float)(4.0 * c1 / dr * (1 + 1e-6));
 }
 
+// Bound (A, per Cartesian component) of the rounding in the reference's FP64
+// pair vector (basetrajectory.h:200-268): the difference of two positions
+// and every minimum-image step (z, then y, then x) round once, each by at
+// most 2^-53 of a magnitude below M, and a component sees at most
+// 1 + nz + ny + nx of them.  pmax bounds |position components|, so
+// |xi - xj| <= 2 pmax; a z step moves y by yz and x by xz, a y step x by xy.
+static double ref_vector_rounding(const Cell &c, double pmax) {
+    const double u = std::ldexp(1.0, -53);
+    const double d0 = 2.0 * pmax;
+    const double nz = std::ceil(d0 / c.L[2]) + 1.0;
+    const double m1 = d0 + nz * std::fabs(c.yz);
+    const double ny = std::ceil(m1 / c.L[1]) + 1.0;
+    const double m0 = d0 + nz * std::fabs(c.xz) + ny * std::fabs(c.xy);
+    const double nx = std::ceil(m0 / c.L[0]) + 1.0;
+    const double M = std::max(m0, m1) + std::max(c.L[0], std::max(c.L[1], c.L[2]));
+    return u * M * (1.0 + nz + ny + nx);
+}
+
+// Margin (A^2) of the FP64 evaluation of a window candidate from its wrapped
+// 32-bit fractional difference (recheck_candidate), with u = 2^-53.  Per
+// Cartesian component of the pair vector:
+//  - quantisation: an atom's fraction is rint(2^32 frac(s)) of its FP64
+//    fractional coordinate s, off by <= q = 2^-33 + e_s, e_s bounding the
+//    rounding of s (a few operations on magnitudes up to (pmax + |lo|) over a
+//    cell length, tilts included); two atoms, row_sum 2 q;
+//  - this evaluation: three roundings of sums of terms <= row_sum / 2;
+//  - the reference's own sequence (ref_vector_rounding).
+// With e = sqrt(3) e_comp the norm of the vector error and d <= dmax =
+// rmax + 1, |d2 - d2_ref| <= (2 dmax + e) e + 8 u dmax^2, the last term the
+// rounding of both squared sums.  Pairs beyond dmax are rejected by both
+// (e is far below 1 A).  The margin is four times the bound.
+static double d2_window_fp64(const Screen &s, const Cell &c, double pmax, double rmax) {
+    const double u = std::ldexp(1.0, -53);
+    const double lmin = std::min(c.L[0], std::min(c.L[1], c.L[2]));
+    const double tilt = std::max(std::fabs(c.xy), std::max(std::fabs(c.xz), std::fabs(c.yz))) / lmin;
+    const double lo = std::max(std::fabs(c.lo[0]), std::max(std::fabs(c.lo[1]), std::fabs(c.lo[2])));
+    const double e_s = 16.0 * u * (1.0 + tilt) * (1.0 + tilt) * ((pmax + lo) / lmin + 1.0);
+    const double q = std::ldexp(1.0, -33) + e_s;
+    const double e_comp = s.row_sum * 2.0 * q + 4.0 * u * s.row_sum + ref_vector_rounding(c, pmax);
+    const double e = std::sqrt(3.0) * e_comp;
+    const double dmax = std::fabs(rmax) + 1.0;
+    return 4.0 * ((2.0 * dmax + e) * e + 8.0 * u * dmax * dmax) * (1 + 1e-6);
+}
+
 // Cluster screen.  A wavefront's 64 i atoms in frame fi get a common centre
 // c (16-bit fractional) and Cartesian offsets p_i = A wrap16(f_i - c); j atoms
 // get q_j = A wrap16(f_j - c).  If rho = max |p_i| satisfies
@@ -292,15 +336,19 @@ constexpr int WG = 256;       // threads per workgroup = i atoms per tile
 // half near the wavefront) are pruned, not evaluated.
 enum Stat {
     STAT_CANDIDATES = 0,  // pairs passed to the FP32 evaluation
-    STAT_RECHECKS,        // candidates re-evaluated in FP64 (near a bin edge, rmin or rmax)
+    STAT_RECHECKS,        // candidates re-evaluated in FP64 from their fractions (FP32 result near a bin edge, rmin or rmax)
     STAT_CLUSTERED,       // (wavefront, t0) units that used the cluster screen
     STAT_UNITS,           // all (wavefront, t0) units of a cluster-screen launch
     STAT_PAIRS_TILE,      // ordered pairs evaluated by FP16 tiles
     STAT_PAIRS_16,        // ordered pairs evaluated by the 16-bit screen
     STAT_PAIRS_FP64,      // ordered pairs evaluated by the plain FP64 kernel
     STAT_PAIRS_ALL,       // all ordered pairs (i active, any j) of the launch
+    STAT_EXACT,           // rechecks evaluated with the reference's FP64 sequence (still near an edge)
     NSTAT
 };
+// FP64 window constants stored after the bin edges: the cell matrix entries
+// times 2^-32 and the margin eps_d2 (d2_window_fp64)
+enum Fp64K { FP64K_X = 0, FP64K_XY, FP64K_XZ, FP64K_Y, FP64K_YZ, FP64K_Z, FP64K_EPS, NFP64K };
 constexpr int QCAP = 192;     // per-wavefront candidate queue entries
 
 struct BlockInfo {
@@ -324,7 +372,7 @@ struct GofrtArgs {
     const int32_t *orig_slot;
     unsigned long long *out;     // [nblocks][leff_max][nch][nbin]
     unsigned long long *ncand;   // [NSTAT] launch statistics, see the Stat enum
-    const double *thr;           // [nbin+1]
+    const double *thr;           // [nbin+1] bin edges, then the FP64 window constants (FP64K_*)
     const int32_t *chan;         // [nt*nt]
     const WorkItem *work;
     const BlockInfo *blocks;
@@ -580,34 +628,92 @@ __device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t 
     return false;
 }
 
-// Exact FP64 evaluation (the reference's sequence) of a candidate whose FP32
-// distance fell inside the window around a bin edge, rmin or rmax.  pi and pj
-// are the positions of the two frames in screen order (pi from the tile's
-// first slot), the order of the entry's indices.
+// A candidate whose FP32 distance fell inside the window around a bin edge,
+// rmin or rmax, in two FP64 stages.  First the pair vector from the same
+// wrapped 32-bit fractional difference in FP64: its squared distance is
+// within eps_d2 of the reference's (the 32-bit quantisation of both atoms
+// and the rounding of both FP64 sequences; see d2_window_fp64), so if every
+// value within eps_d2 of it falls on the same side of rmin, rmax and the
+// bin edges, that decision is the reference's.  The rest -- pairs within
+// about 1e-7 A of an edge -- take the reference's exact FP64 sequence from
+// the positions.  pi and pj are the positions of the two frames in screen
+// order and fi32 the t0 frame's fractions from the tile's first slot (the
+// order of the entry's indices).  Returns whether the exact sequence ran.
 template <bool TRI>
-__device__ __forceinline__ void recheck_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds,
+__device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, uint32_t e, const uint4 *__restrict__ fi32,
                                                   const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
                                                   const double4 *__restrict__ pj, const double *cell,
                                                   const double *thr, const int *hb, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
-    const uint32_t wi = fi_lds[il].w, wj = fj32[j].w;  // type | slot << 8
-    const double4 xi = pi[il];
-    const double4 xj = pj[j];
-    // the t0 frame's cell is read here, on this rare path, rather than held
-    // in registers across the whole pair loop
-    const DevCell c = load_cell(cell);
-    const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
-    if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) {
-        const int bb = bin_of(d2r, thr, (int)a.nbin, a.rminf, a.inv_drf);
-        atomicAdd(&hist[hb[hb_key(wi, wj, a.nt)] + bb], 1u);
+    const uint4 ui = fi32[il], uj = fj32[j];  // .w = type | slot << 8
+    const double fa = (double)(int)(ui.x - uj.x);
+    const double fb = (double)(int)(ui.y - uj.y);
+    const double fc = (double)(int)(ui.z - uj.z);
+    // the constants are read here, on this rare path, rather than held in
+    // scalar registers across the whole pair loop (the empty asm keeps the
+    // compiler from hoisting the loads)
+    const double *k = a.thr + a.nbin + 1;
+    asm volatile("" : "+s"(k));
+    double x, y, z;
+    if (TRI) {
+        z = k[FP64K_Z] * fc;
+        y = fma(k[FP64K_YZ], fc, k[FP64K_Y] * fb);
+        x = fma(k[FP64K_XZ], fc, fma(k[FP64K_XY], fb, k[FP64K_X] * fa));
+    } else {
+        x = k[FP64K_X] * fa;
+        y = k[FP64K_Y] * fb;
+        z = k[FP64K_Z] * fc;
     }
+    const double d2 = fma(z, z, fma(y, y, x * x));
+    const double eps = k[FP64K_EPS];
+    const double dlo = d2 - eps, dhi = d2 + eps;
+    const int nb = (int)a.nbin;
+    int bb = -1;  // bin of the pair, -1: not counted
+    bool exact = true;
+    if (dhi < a.lo || dlo > a.rmax2 || dlo >= a.Tn) {
+        exact = false;  // below rmin or beyond rmax for every value in the margin
+    } else if (dlo >= a.lo && dhi <= a.rmax2 && dhi < a.Tn) {
+        const int b = bin_of(d2, thr, nb, a.rminf, a.inv_drf);
+        if ((b == 0 || dlo >= thr[b]) && (b == nb - 1 || dhi < thr[b + 1])) {
+            bb = b;
+            exact = false;
+        }
+    }
+    if (exact) {
+        const double4 xi = pi[il];
+        const double4 xj = pj[j];
+        // the t0 frame's cell is read here, on this rare path, rather than
+        // held in registers across the whole pair loop
+        asm volatile("" : "+s"(cell));
+        const DevCell c = load_cell(cell);
+        const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
+        if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) bb = bin_of(d2r, thr, nb, a.rminf, a.inv_drf);
+    }
+    if (bb >= 0) atomicAdd(&hist[hb[hb_key(ui.w, uj.w, a.nt)] + bb], 1u);
+    return exact;
 }
 
 constexpr int SQCAP = 576;   // per-wavefront queue: < 64 left + up to 8 masks of 64
 constexpr int SQCAP_CL = 1088;  // cluster screen: < 64 left + 16 masks of 64 (one drain per 32 x 32 tile)
 constexpr int SQCAP_CLC = 576;  // cluster screen with the LDS column cache: < 64 left + 8 masks (a drain per 8)
 constexpr int RQCAP = 128;   // per-wavefront recheck queue: < 64 left + one batch
+
+// c += v for a wavefront-uniform count.  PIN keeps it in scalar registers
+// (readfirstlane): that lowers the vector-register spills of the triclinic
+// screened kernels and speeds them up (3 % on the 16,384-atom triclinic
+// workload), while in the cubic ones it costs time (5 % on the 65,536-atom
+// cubic workload), where the compiler is left to place the counts.
+template <bool PIN>
+__device__ __forceinline__ void uadd(unsigned long long &c, unsigned long long v) {
+    if constexpr (PIN) {
+        const unsigned long long r = c + v;
+        c = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(r >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)r);
+    } else {
+        c += v;
+    }
+}
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -691,9 +797,11 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     const uint32_t il = threadIdx.x;
     constexpr int UR = UNROLL;
     const uint32_t NU = (uint32_t)(N - N % UR);
-    unsigned long long ncand = 0, nre = 0, ncl = 0, nwt = 0;
-    // wavefront-uniform pair counts: all ordered pairs of the active i atoms,
-    // pairs evaluated by the FP16 tile screen and by the 16-bit screen
+    // wavefront-uniform counts: candidates,
+    // window rechecks and their exact evaluations, clustered and all units;
+    // all ordered pairs of the active i atoms, pairs evaluated by the FP16
+    // tile screen and by the 16-bit screen
+    unsigned long long ncand = 0, nre = 0, nex = 0, ncl = 0, nwt = 0;
     unsigned long long nall = 0, ntile = 0, n16 = 0;
 
     for (uint32_t k = w.k0; k < w.k1; k++) {
@@ -701,6 +809,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         const uint64_t fj = fi + t;
         const double *cellp = a.box + fi * 9;
         const double4 *__restrict__ pi = a.pos_scr + fi * N + slot0;
+        const uint4 *__restrict__ fi32 = a.frac32 + fi * N + slot0;
         const double4 *__restrict__ pj = a.pos_scr + fj * N;
         const uint4 *__restrict__ fj32 = a.frac32 + fj * N;
         cu64_t *fjc = (cu64_t *)(a.frac + fj * N);
@@ -724,7 +833,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         // against 0 and never pass
         const uint32_t th = active ? a.threshold + 1u : 0u;
         const unsigned long long nact = (unsigned long long)__popcll(__ballot(active));
-        nall += nact * N;
+        uadd<TRI>(nall, nact * N);
         __syncthreads();
         uint32_t qn = 0, rn = 0;
         // evaluate the queued entry of this lane; candidates inside the edge
@@ -739,8 +848,9 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                 if (rn >= 64) {
                     wave_sync_lds();
                     rn -= 64;
-                    nre += 1;
-                    recheck_candidate<TRI>(a, rq[rn + lane], fi_lds, fj32, pi, pj, cellp, thr, hb, hist);
+                    uadd<TRI>(nre, 64);
+                    const bool ex = recheck_candidate<TRI>(a, rq[rn + lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
+                    uadd<TRI>(nex, (uint32_t)__popcll(__ballot(ex)));
                     wave_sync_lds();
                 }
             }
@@ -753,7 +863,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         };
         bool clustered = false;
         if constexpr (CLUSTER) {
-            nwt += lane == 0 ? 1 : 0;
+            uadd<TRI>(nwt, 1);
             const unsigned long long act = __ballot(active);
             if (act) {
                 // centre: reference atom + mean wrapped offset of the active lanes
@@ -781,7 +891,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                 const float rho = sqrtf(wave_max_f(active ? p2 : 0.0f));
                 if (rho < a.rho_cap) {
                     clustered = true;
-                    ncl += lane == 0 ? 1 : 0;
+                    uadd<TRI>(ncl, 1);
                     const float thc = cluster_threshold(rho, a.rmaxf, a.dq);
                     // mode 2: candidates subtract FP32 offsets from the centre;
                     // the rows' offsets replace their fractions in fi_lds, which
@@ -896,7 +1006,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         for (int jb = 0; jb < 2; jb++) {
                             if ((uint32_t)(nearm >> (32 * jb)) == 0u) continue;  // uniform
                             const uint32_t jfirst = jb ? jB : jA;
-                            ntile += nact * (unsigned long long)min(32u, (uint32_t)N - jfirst);
+                            uadd<TRI>(ntile, nact * (unsigned long long)min(32u, (uint32_t)N - jfirst));
                             // entry = j << 8 | local i (COLC: the column, 32 jb + lane % 32,
                             // instead of j); item e of tile T is row
                             // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
@@ -937,7 +1047,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                                 wave_sync_lds();
                                                 qn -= 64;
                                                 const uint32_t e = q[qn + lane];
-                                                ncand += 1;
+                                                uadd<TRI>(ncand, 64);
                                                 eval_step(e, true);
                                                 wave_sync_lds();
                                             }
@@ -950,7 +1060,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                         wave_sync_lds();
                                         qn -= 64;
                                         const uint32_t e = q[qn + lane];
-                                        ncand += 1;
+                                        uadd<TRI>(ncand, 64);
                                         eval_step(e, true);
                                         wave_sync_lds();
                                     }
@@ -963,7 +1073,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             wave_sync_lds();
                             const bool valid = lane < qn;
                             const uint32_t e = valid ? q[lane] : 0u;
-                            ncand += valid ? 1 : 0;
+                            uadd<TRI>(ncand, qn);
                             eval_step(e, valid);
                             qn = 0;
                             wave_sync_lds();
@@ -1058,7 +1168,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
             }
         }
         if (!clustered) {
-            n16 += nact * N;
+            uadd<TRI>(n16, nact * N);
             for (uint32_t j = 0; j < NU; j += UR) {
                 unsigned long long g[UR];
                 bool p[UR];
@@ -1082,7 +1192,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         wave_sync_lds();
                         qn -= 64;
                         const uint32_t e = q[qn + lane];
-                        ncand += 1;
+                        uadd<TRI>(ncand, 64);
                         eval_batch(e, true);
                         wave_sync_lds();
                     }
@@ -1100,24 +1210,21 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         for (uint32_t base = 0; base < qn; base += 64) {
             const bool valid = base + lane < qn;
             const uint32_t e = valid ? q[base + lane] : 0u;
-            ncand += valid ? 1 : 0;
+            uadd<TRI>(ncand, min(qn - base, 64u));
             eval_batch(e, valid);
         }
         wave_sync_lds();
-        if (lane < rn) {
-            nre += 1;
-            recheck_candidate<TRI>(a, rq[lane], fi_lds, fj32, pi, pj, cellp, thr, hb, hist);
-        }
+        bool ex = false;
+        if (lane < rn) ex = recheck_candidate<TRI>(a, rq[lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
+        uadd<TRI>(nex, (uint32_t)__popcll(__ballot(ex)));
+        uadd<TRI>(nre, rn);
         wave_sync_lds();
     }
     // per-wavefront totals, one atomic each per wavefront
-    for (int off = 32; off > 0; off >>= 1) {
-        ncand += __shfl_down(ncand, off, 64);
-        nre += __shfl_down(nre, off, 64);
-    }
     if (lane == 0 && a.ncand) {
         atomicAdd(a.ncand + STAT_CANDIDATES, ncand);
         atomicAdd(a.ncand + STAT_RECHECKS, nre);
+        if (nex) atomicAdd(a.ncand + STAT_EXACT, nex);
         if (CLUSTER) {
             atomicAdd(a.ncand + STAT_CLUSTERED, ncl);
             atomicAdd(a.ncand + STAT_UNITS, nwt);
@@ -1521,8 +1628,27 @@ int anl_gofrt_create(anl_traj *tr, double rmin, double rmax, uint32_t nbin, uint
         g->rmin2 = rmin * rmin;
         g->thr = bin_edges(rmin, rmax, nbin, g->dr);
         g->lo = std::max(g->rmin2, g->thr[0]);
-        ANL_HIP(hipMalloc(&g->d_thr, (nbin + 1) * sizeof(double)));
-        ANL_HIP(hipMemcpy(g->d_thr, g->thr.data(), (nbin + 1) * sizeof(double), hipMemcpyHostToDevice));
+        {
+            // bin edges, then the FP64 window constants of the screened form
+            // (zero when the screen does not apply to this cell and rmax)
+            std::vector<double> t(g->thr);
+            t.resize(nbin + 1 + NFP64K, 0.0);
+            const Screen sc = make_screen(tr->cell0, tr->box_const, tr->nt, rmax);
+            if (sc.valid) {
+                const Cell &cl = tr->cell0;
+                const double f = std::ldexp(1.0, -32);
+                double *k = t.data() + nbin + 1;
+                k[FP64K_X] = cl.L[0] * f;
+                k[FP64K_XY] = cl.xy * f;
+                k[FP64K_XZ] = cl.xz * f;
+                k[FP64K_Y] = cl.L[1] * f;
+                k[FP64K_YZ] = cl.yz * f;
+                k[FP64K_Z] = cl.L[2] * f;
+                k[FP64K_EPS] = d2_window_fp64(sc, cl, tr->pos_max, rmax);
+            }
+            ANL_HIP(hipMalloc(&g->d_thr, t.size() * sizeof(double)));
+            ANL_HIP(hipMemcpy(g->d_thr, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
         // gofrt.h:86-104: channel of (type1, type2)
         std::vector<int32_t> chan(tr->nt * tr->nt);
         for (uint32_t a = 0; a < tr->nt; a++)
@@ -1641,6 +1767,16 @@ int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uin
         ANL_HIP(hipMemcpy(c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
         if (candidates) *candidates = c[0];
         if (rechecks) *rechecks = c[1];
+    });
+}
+
+int anl_gofrt_recheck_stats(anl_gofrt *g, uint64_t *window, uint64_t *exact) {
+    return guard([&] {
+        if (!g) fail(ANL_ERR_ARG, "null handle");
+        unsigned long long c[NSTAT] = {};
+        ANL_HIP(hipMemcpy(c, g->d_ncand, sizeof(c), hipMemcpyDeviceToHost));
+        if (window) *window = c[STAT_RECHECKS];
+        if (exact) *exact = c[STAT_EXACT];
     });
 }
 

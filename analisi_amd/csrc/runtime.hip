@@ -324,9 +324,13 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
             for (uint32_t k = 6; k < box_stride; k++)
                 if (!std::isfinite(box[f * box_stride + k])) fail(ANL_ERR_ARG, "cell tilts must be finite");
         const double bound = 1e4 * 2 * lmin;
-        for (uint64_t i = 0; i < nframes * natoms * 3; i++)
-            if (!(std::fabs(pos[i]) <= bound))
+        double pos_max = 0.0;
+        for (uint64_t i = 0; i < nframes * natoms * 3; i++) {
+            const double v = std::fabs(pos[i]);
+            if (!(v <= bound))
                 fail(ANL_ERR_ARG, "positions must be finite and within 10^4 (shortest) cell lengths of the origin");
+            pos_max = std::max(pos_max, v);
+        }
         ensure_device();
         // on any failure below (a device allocation, an upload) everything
         // allocated so far is released with the trajectory
@@ -340,6 +344,7 @@ int anl_traj_create(const double *pos, const double *box, uint32_t box_stride, c
         tr->nt = ntypes;
         tr->stride = box_stride;
         tr->triclinic = box_stride == 9;
+        tr->pos_max = pos_max;
         tr->box_host.assign(nframes * 9, 0.0);
         for (uint64_t f = 0; f < nframes; f++)
             for (uint32_t k = 0; k < box_stride; k++) tr->box_host[f * 9 + k] = box[f * box_stride + k];

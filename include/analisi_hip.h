@@ -100,6 +100,14 @@ int anl_gofrt_calculate_blocks_async(anl_gofrt *g, const uint64_t *primo, const 
  * many of those were re-evaluated in FP64 (near a bin edge, rmin or rmax).
  * Diagnostics of this implementation; no counterpart in the reference. */
 int anl_gofrt_last_stats(anl_gofrt *g, int *form_used, uint64_t *candidates, uint64_t *rechecks);
+/* Of the last screened calculate: the candidates whose FP32 distance fell in
+ * the window around a bin edge, rmin or rmax (window, the rechecks of
+ * anl_gofrt_last_stats), re-evaluated in FP64 from their 32-bit fractions;
+ * and how many of those were still within that evaluation's error bound of
+ * an edge (about 1e-7 A) and took the reference's FP64 sequence from the
+ * positions (exact).  Diagnostics of this implementation; no counterpart in
+ * the reference. */
+int anl_gofrt_recheck_stats(anl_gofrt *g, uint64_t *window, uint64_t *exact);
 /* Of the last screened calculate: how many (wavefront, t0) units used the
  * cluster (MFMA) screen, out of how many (0 of 0 when it was off). */
 int anl_gofrt_cluster_stats(anl_gofrt *g, uint64_t *clustered, uint64_t *units);

@@ -154,6 +154,49 @@ def test_bin_edges_adversarial_cluster(tilt, origin):
         assert rechecks > 0, cl
 
 
+@pytest.mark.parametrize("tilt,origin", [(None, 20.0), ((2.5, -1.5, 1.0), -6.0)], ids=["cubic", "tri-corner"])
+def test_recheck_two_stages(tilt, origin):
+    """Pairs near every bin edge, rmin and rmax at two scales: 1e-6 and 1e-5
+    of the distance (inside the FP32 window; mostly outside the error bound
+    of the FP64 evaluation from the 32-bit fractions, which then decides
+    them) and a few ulps to 2e-8 (inside that bound, so the reference's FP64
+    sequence from the positions decides them).  Every cluster mode must give
+    the reference's bins, and both stages must have run."""
+    from analisi_amd import synth
+    rmin, rmax, nbin = 0.5, 4.5, 40
+    dr = (rmax - rmin) / nbin
+    d = []
+    for k in range(0, nbin + 1):
+        e = rmin + k * dr
+        d += [e * (1 - 1e-5), e * (1 - 1e-6), e * (1 + 1e-6), e * (1 + 1e-5)]
+        d += [np.nextafter(e, 0), e, np.nextafter(e, 10), e * (1 - 2e-8), e * (1 + 2e-8)]
+    d = np.array(d)
+    M = len(d)
+    L = 60.0
+    rng = np.random.default_rng(11)
+    base = origin + 12.0 * rng.random((M, 3))
+    pos = np.zeros((2, 2 * M, 3))
+    pos[:, 0::2] = base
+    pos[:, 1::2] = base
+    pos[:, 1::2, 0] += d
+    types = (np.arange(2 * M) % 2).astype(np.int32)
+    box = np.tile(synth.lammps_box(L, None if tilt is None else np.array(tilt)), (2, 1))
+    ot = O.Traj(pos, types, box, tilt is not None)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    ref, _ = O.gofrt(ot, rmin, rmax, nbin, 1, 1, 1, 1, 1, 0, want_vdata=False)
+    g = capi.DeviceGofrt(dt, rmin, rmax, nbin, 1, 1, 1, capi.GOFRT_SCREENED)
+    g.reset(1)
+    for cl in (2, 1, 4, 3, 0):
+        g.set_tuning(8, 0, 1 << 30, cl)
+        got = g.calculate(0)
+        assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
+        _, _, rechecks = g.stats()
+        window, exact = g.recheck_stats()
+        assert window == rechecks, (cl, window, rechecks)
+        # the ulp-level pairs (three per edge, both orders) take the exact sequence
+        assert 6 * (nbin + 1) <= exact < window, (cl, exact, window)
+
+
 def test_too_short_raises():
     ot, dt = make_case(2, 50, 10, 8.0, [1.0], 0.1)
     g = capi.DeviceGofrt(dt, 0.0, 3.0, 10, 3, 1, 1)
