@@ -427,7 +427,8 @@ __device__ __forceinline__ WgId decode_wg(uint32_t ntiles, uint32_t nlag, uint32
 #include "min_image.h"
 
 __device__ __forceinline__ int bin_of(double d2, const double *T, int nbin, float rminf, float inv_drf) {
-    float u = (sqrtf((float)d2) - rminf) * inv_drf;
+    // a first guess (any rounding of it is corrected by the edge table)
+    float u = (__builtin_amdgcn_sqrtf((float)d2) - rminf) * inv_drf;
     int b = (int)u;
     b = b < 0 ? 0 : (b > nbin - 1 ? nbin - 1 : b);
     while (b > 0 && d2 < T[b]) --b;
@@ -628,6 +629,30 @@ __device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t 
     return false;
 }
 
+// A pointer into read-only device memory for the recheck's rare path, opaque
+// to the optimiser, so that loads through it stay where they are written
+// instead of being hoisted out of the pair loop, where their results would
+// hold registers across it.  SCALAR: wavefront-uniform scalar loads (constant
+// address space); otherwise vector loads from global memory.  Measured on
+// 128/300-t0 slices: the triclinic kernels run best with scalar loads (their
+// spill fell from 44 to 8 B/lane; 987 -> 931 ms), the cubic ones with vector
+// loads (1,663 ms scalar, 1,614 ms vector).
+typedef const __attribute__((address_space(4))) double *rare_cptr;
+typedef const __attribute__((address_space(1))) double *rare_gptr;
+template <bool SCALAR>
+__device__ __forceinline__ auto rare_const(const double *p) {
+    uint64_t v = (uint64_t)p;
+    if constexpr (SCALAR) {
+        asm volatile("" : "+s"(v));
+        v = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+        return (rare_cptr)v;
+    } else {
+        asm volatile("" : "+v"(v));
+        return (rare_gptr)v;
+    }
+}
+
 // A candidate whose FP32 distance fell inside the window around a bin edge,
 // rmin or rmax, in two FP64 stages.  First the pair vector from the same
 // wrapped 32-bit fractional difference in FP64: its squared distance is
@@ -651,10 +676,8 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, uint32_t e
     const double fb = (double)(int)(ui.y - uj.y);
     const double fc = (double)(int)(ui.z - uj.z);
     // the constants are read here, on this rare path, rather than held in
-    // scalar registers across the whole pair loop (the empty asm keeps the
-    // compiler from hoisting the loads)
-    const double *k = a.thr + a.nbin + 1;
-    asm volatile("" : "+s"(k));
+    // scalar registers across the whole pair loop
+    const auto k = rare_const<TRI>(a.thr + a.nbin + 1);
     double x, y, z;
     if (TRI) {
         z = k[FP64K_Z] * fc;
@@ -685,8 +708,7 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, uint32_t e
         const double4 xj = pj[j];
         // the t0 frame's cell is read here, on this rare path, rather than
         // held in registers across the whole pair loop
-        asm volatile("" : "+s"(cell));
-        const DevCell c = load_cell(cell);
+        const DevCell c = load_cell(rare_const<TRI>(cell));
         const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
         if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) bb = bin_of(d2r, thr, nb, a.rminf, a.inv_drf);
     }

@@ -7,8 +7,10 @@ struct DevCell {
     double lh0, lh1, lh2, L0, L1, L2, xy, xz, yz;
 };
 
-// Internal-format box [xlo, ylo, zlo, lx/2, ly/2, lz/2, xy, xz, yz] of a frame.
-__device__ __forceinline__ DevCell load_cell(const double *b) {
+// Internal-format box [xlo, ylo, zlo, lx/2, ly/2, lz/2, xy, xz, yz] of a frame
+// (b: a pointer to double in any address space).
+template <class P>
+__device__ __forceinline__ DevCell load_cell(P b) {
     DevCell c;
     c.lh0 = b[3];
     c.lh1 = b[4];

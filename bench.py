@@ -309,6 +309,7 @@ def run_gofrt(w, args, hip):
     counts = np.zeros((leff, nch, w["nbin"]), dtype=np.uint64)
     hip.d2h(counts, d_out, out_len * 8)
     _, ncand, nre = g.stats()
+    _, nexact = g.recheck_stats()
     ps = g.pair_stats()
     clustered, units = g.cluster_stats()
     # invariants of the full run: lag-0 self channels hold N_type per t0 in
@@ -401,6 +402,11 @@ def run_gofrt(w, args, hip):
                     "on t0 or lag)"},
         "screen": {"candidates_per_step": int(ncand), "candidate_fraction_of_all": ncand / updates,
                    "fp64_rechecks_per_step": int(nre), "recheck_fraction_of_candidates": nre / max(ncand, 1),
+                   "exact_rechecks_per_step": int(nexact), "exact_fraction_of_rechecks": nexact / max(nre, 1),
+                   "note": "rechecks: candidates whose FP32 distance fell in the window around an edge, "
+                           "re-evaluated in FP64 from their 32-bit fractions; exact: those still within that "
+                           "evaluation's error bound of an edge, evaluated with the reference's FP64 sequence "
+                           "from the positions",
                    "clustered_units": int(clustered), "units": int(units)},
         "checks": {"lag0_self_channels_and_pair_symmetry": bool(inv_ok), "all_pairs_equal_reference_count":
                    int(ps["all"]) == int(updates)},
