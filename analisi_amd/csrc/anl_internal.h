@@ -56,11 +56,11 @@ struct Screen {
     bool valid;            // preconditions hold for the whole trajectory
     uint32_t threshold;    // |q|^2 <= threshold passes
     std::string why_not;   // reason when !valid
-    double frac_max;       // bound of |fractional component| of a candidate
     double row_sum;        // max over rows of the cell matrix of sum |A_ik|
     double wmin;           // smallest cell width (distance between opposite faces)
     double fro;            // Frobenius norm of the cell matrix (>= its spectral norm)
     double smin;           // sqrt of the smallest eigenvalue of A^T A (|A x| >= smin |x|)
+    double anorm;          // spectral norm of |A| (entry-wise absolute values), rounded up
 };
 
 }  // namespace anl

@@ -89,7 +89,7 @@ std::vector<double> bin_edges(double rmin, double rmax, uint32_t nbin, double dr
 // ---------------------------------------------------------------------------
 
 // Smallest eigenvalue of the symmetric 3x3 metric G = A^T A (cyclic Jacobi).
-static double min_eig_sym3(double G[3][3]) {
+static void eig_sym3(const double G[3][3], double *lmin, double *lmax) {
     double a[3][3];
     std::memcpy(a, G, sizeof(a));
     for (int sweep = 0; sweep < 60; sweep++) {
@@ -113,7 +113,8 @@ static double min_eig_sym3(double G[3][3]) {
                 }
             }
     }
-    return std::min(a[0][0], std::min(a[1][1], a[2][2]));
+    *lmin = std::min(a[0][0], std::min(a[1][1], a[2][2]));
+    *lmax = std::max(a[0][0], std::max(a[1][1], a[2][2]));
 }
 
 // The screen works on q = 16-bit wrapped differences of fractional
@@ -158,7 +159,9 @@ static Screen make_screen(const Cell &c, bool box_const, uint32_t nt, double rma
     double G[3][3];
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) G[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
-    double lam = min_eig_sym3(G) * (1 - 1e-9);
+    double lam, lam_max;
+    eig_sym3(G, &lam, &lam_max);
+    lam *= 1 - 1e-9;
     if (!(lam > 0)) {
         s.why_not = "degenerate cell";
         return s;
@@ -172,7 +175,6 @@ static Screen make_screen(const Cell &c, bool box_const, uint32_t nt, double rma
     }
     s.valid = true;
     s.threshold = (uint32_t)R2;
-    s.frac_max = std::min(0.5, std::sqrt(R2) / 65536.0 + 2.0 / 65536.0);
     s.row_sum = 0;
     for (int i = 0; i < 3; i++)
         s.row_sum = std::max(s.row_sum, std::fabs(A[i][0]) + std::fabs(A[i][1]) + std::fabs(A[i][2]));
@@ -181,52 +183,74 @@ static Screen make_screen(const Cell &c, bool box_const, uint32_t nt, double rma
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) s.fro += A[i][j] * A[i][j];
     s.fro = std::sqrt(s.fro);
+    // |A|^T |A| and its largest eigenvalue (rounded up): the spectral norm of
+    // |A| bounds || |A| |x| || by anorm |x|
+    double H[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            H[i][j] = std::fabs(A[0][i]) * std::fabs(A[0][j]) + std::fabs(A[1][i]) * std::fabs(A[1][j]) +
+                      std::fabs(A[2][i]) * std::fabs(A[2][j]);
+    double hmin, hmax;
+    eig_sym3(H, &hmin, &hmax);
+    s.anorm = std::sqrt(hmax * (1 + 1e-9));
     return s;
 }
 
-// Window (in bins) of the FP32 candidate evaluation.  Error sources, per
-// Cartesian component of a candidate (fractional components <= frac_max):
-// 32-bit quantisation of both atoms (2^-32 of each cell row), int->float
-// conversion, float cell constants and two FMAs (<= 2^-24 relative each, five
-// in all, bounded by 10 ulps of the largest term).  On d: sqrt(3) times the
-// component error plus the rounding of the three-term sum and the square
-// root.  On u = (d - rmin)/dr: that over dr plus the subtraction/product and
-// the reference's own double->float rounding of u.  The window is four times
-// the bound.
-static float du_window(const Screen &s, double rmin, double rmax, double dr, uint32_t nbin) {
+// Window (in bins) of the FP32 candidate evaluation (eval_candidate), with
+// u = 2^-24.  The pair vector is A' float(D), D the wrapped 32-bit integer
+// difference of the fractions and A' the FP32 cell constants times 2^-32.
+// Per Cartesian component k the int->float conversions, the rounded cell
+// constants, the product and the FMAs of a row round the terms A_km D_m at
+// most five times (three in the orthogonal form), so the computed vector is
+// off by e_k <= c u (|A| |D|)_k with c = 6 (4) covering the second-order
+// terms.  In the 2-norm, || |A| |D| || <= anorm |D| <= anorm |A D| / smin
+// = K' |A D| (anorm: spectral norm of |A|; smin: smallest singular value of
+// A), and |A D| <= dmax = rmax + 1 for the pairs whose bin matters, so
+// |e| <= c u K' dmax.  The 32-bit quantisation of both atoms adds
+// sqrt(3) row_sum 2^-31.  On d the three-term sum of squares (3u on d^2) and
+// v_sqrt_f32 (1 ulp, 2u) add 3.5u d, taken as 4u dmax.  On u =
+// (d - rmin)/dr: that over dr, plus the FP32 rmin (u rmin/dr), the
+// subtraction, the FP32 1/dr, the product and the reference's own rounding
+// of u to float (floorf of a double, gofrt.cpp:114-115), u umax each (5u
+// umax, umax = nbin + 1).  Beyond the window around nbin every pair is
+// rejected, as the reference does.  The window is four times the bound.
+static double du_tail(double rmin, double dr, uint32_t nbin) {
     const double e24 = std::ldexp(1.0, -24);
-    const double term = s.row_sum * s.frac_max;
-    const double e_comp = s.row_sum * std::ldexp(1.0, -31) + 10.0 * e24 * term;
-    // only distances up to rmax (u up to nbin) need the bound: beyond the
-    // window around nbin every pair is rejected, as the reference does
+    return 5.0 * e24 * (nbin + 1.0) + e24 * std::fabs(rmin) / dr;
+}
+static float du_window(const Screen &s, bool tri, double rmin, double rmax, double dr, uint32_t nbin) {
+    const double e24 = std::ldexp(1.0, -24);
     const double dmax = std::fabs(rmax) + 1.0;
-    const double e_d = std::sqrt(3.0) * e_comp + 8.0 * e24 * dmax;
-    const double umax = nbin + 1.0;
-    const double du = e_d / dr + 8.0 * e24 * umax;
-    return (float)(4.0 * du + 1e-6);
+    const double Kp = s.anorm / s.smin;
+    const double e_d = (tri ? 6.0 : 4.0) * e24 * Kp * dmax + std::sqrt(3.0) * s.row_sum * std::ldexp(1.0, -31) +
+                       4.0 * e24 * dmax;
+    const double du = e_d / dr + du_tail(rmin, dr, nbin);
+    return (float)(4.0 * du * (1 + 1e-6) + 1e-6);
 }
 
-// Window of the mode-2 candidate evaluation, which subtracts FP32 Cartesian
-// offsets of both atoms from the wavefront's centre c: P = A' float(W_i),
-// Q = A' float(W_j), with W = 32-bit fraction - c (exact, wrapped) and A' the
-// FP32 cell constants times 2^-32.  Per component, the int->float conversion
-// and the up to three products and sums of a row cost <= 4u sum_m |A_km W_m|
-// for each atom (u = 2^-24), and sum_m |A_km W_m| <= K |offset| with
-// K = row_sum / smin; |P| <= rho + dq and |Q| <= rho + dq + d for the
-// wavefront radius rho and a pair at distance d <= dmax = rmax + 1.  The FP32
-// cell constants add u K d, the subtraction u d, the 32-bit quantisation
-// row_sum 2^-31.  So per component
-//   e_comp <= u K (8 (rho + dq) + 6 dmax) + row_sum 2^-31,
-// and e_d, du and the factor 4 follow du_window.  The window is
-// m0 + m1 (rho + dq) bins, computed per wavefront from its own radius.
-static void du_window_offsets(const Screen &s, double rmax, double dr, uint32_t nbin, float *m0, float *m1) {
+// Window of the offset evaluation (modes 2 and 4, eval_candidate_off), which
+// subtracts FP32 Cartesian offsets of both atoms from the wavefront's centre
+// c: P = A' float(W_i), Q = A' float(W_j), W = 32-bit fraction - c (exact,
+// wrapped).  Per component, the conversions and the up to three products and
+// sums of a row cost <= 4u (|A| |W|)_k for each atom (u = 2^-24); the FP32
+// cell constants are common to P and Q, so their rounding acts on the
+// difference: <= u (|A| |D|)_k; the subtraction adds u |d_k|.  In the 2-norm
+// (K' = anorm / smin as in du_window), with |P| <= rho + dq and
+// |Q| <= rho + dq + d for the wavefront radius rho and a pair at distance
+// d <= dmax = rmax + 1:
+//   |e| <= u K' (8 (rho + dq) + 5 dmax) + u dmax + sqrt(3) row_sum 2^-31,
+// the last term the 32-bit quantisation; the sum, square root, u and the
+// factor 4 follow du_window.  The window is m0 + m1 (rho + dq) bins,
+// computed per wavefront from its own radius.
+static void du_window_offsets(const Screen &s, double rmin, double rmax, double dr, uint32_t nbin, float *m0,
+                              float *m1) {
     const double e24 = std::ldexp(1.0, -24);
-    const double K = s.row_sum / s.smin;
+    const double Kp = s.anorm / s.smin;
     const double dmax = std::fabs(rmax) + 1.0;
-    const double r3 = std::sqrt(3.0);
-    const double c0 = r3 * (e24 * K * 6.0 * dmax + s.row_sum * std::ldexp(1.0, -31)) + 8.0 * e24 * dmax;
-    const double c1 = r3 * e24 * K * 8.0;
-    *m0 = (float)((4.0 * (c0 / dr + 8.0 * e24 * (nbin + 1.0)) + 1e-6) * (1 + 1e-6));
+    const double c0 = e24 * Kp * 5.0 * dmax + e24 * dmax + std::sqrt(3.0) * s.row_sum * std::ldexp(1.0, -31) +
+                      4.0 * e24 * dmax;
+    const double c1 = e24 * Kp * 8.0;
+    *m0 = (float)((4.0 * (c0 / dr + du_tail(rmin, dr, nbin)) + 1e-6) * (1 + 1e-6));
     *m1 = (float)(4.0 * c1 / dr * (1 + 1e-6));
 }
 
@@ -663,9 +687,12 @@ __device__ __forceinline__ auto rare_const(const double *p) {
 // about 1e-7 A of an edge -- take the reference's exact FP64 sequence from
 // the positions.  pi and pj are the positions of the two frames in screen
 // order and fi32 the t0 frame's fractions from the tile's first slot (the
-// order of the entry's indices).  Returns whether the exact sequence ran.
+// order of the entry's indices).  At lag 0 (same_frame) an atom paired with
+// itself has the reference's d2 = 0 exactly, as the first stage computes it,
+// so its margin is 0.  Returns whether the exact sequence ran.
 template <bool TRI>
-__device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, uint32_t e, const uint4 *__restrict__ fi32,
+__device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, bool same_frame, uint32_t e,
+                                                  const uint4 *__restrict__ fi32,
                                                   const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
                                                   const double4 *__restrict__ pj, const double *cell,
                                                   const double *thr, const int *hb, unsigned *hist) {
@@ -689,7 +716,7 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, uint32_t e
         z = k[FP64K_Z] * fc;
     }
     const double d2 = fma(z, z, fma(y, y, x * x));
-    const double eps = k[FP64K_EPS];
+    const double eps = same_frame && ui.w == uj.w ? 0.0 : k[FP64K_EPS];
     const double dlo = d2 - eps, dhi = d2 + eps;
     const int nb = (int)a.nbin;
     int bb = -1;  // bin of the pair, -1: not counted
@@ -871,7 +898,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     wave_sync_lds();
                     rn -= 64;
                     uadd<TRI>(nre, 64);
-                    const bool ex = recheck_candidate<TRI>(a, rq[rn + lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
+                    const bool ex = recheck_candidate<TRI>(a, t == 0, rq[rn + lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
                     uadd<TRI>(nex, (uint32_t)__popcll(__ballot(ex)));
                     wave_sync_lds();
                 }
@@ -1237,7 +1264,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         }
         wave_sync_lds();
         bool ex = false;
-        if (lane < rn) ex = recheck_candidate<TRI>(a, rq[lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
+        if (lane < rn) ex = recheck_candidate<TRI>(a, t == 0, rq[lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
         uadd<TRI>(nex, (uint32_t)__popcll(__ballot(ex)));
         uadd<TRI>(nre, rn);
         wave_sync_lds();
@@ -1486,10 +1513,10 @@ GofrtPrepared gofrt_prepare(anl_gofrt *g, const Launch &L, uint64_t out_len, int
         a.cy = (float)(cl.L[1] * sc);
         a.cyz = (float)(cl.yz * sc);
         a.cz = (float)(cl.L[2] * sc);
-        a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, g->rmin, g->rmax, g->dr, g->nbin) : 0.0f;
+        a.du_margin = form == ANL_GOFRT_SCREENED ? du_window(tr->screen_cache, tr->triclinic, g->rmin, g->rmax, g->dr, g->nbin) : 0.0f;
         a.du_m0 = a.du_m1 = 0.0f;
         if (form == ANL_GOFRT_SCREENED)
-            du_window_offsets(tr->screen_cache, g->rmax, g->dr, g->nbin, &a.du_m0, &a.du_m1);
+            du_window_offsets(tr->screen_cache, g->rmin, g->rmax, g->dr, g->nbin, &a.du_m0, &a.du_m1);
         // cluster screen: cell entries times 2^-16, quantisation bound,
         // largest usable wavefront radius (cluster_params)
         const double sh = std::ldexp(1.0, -16);

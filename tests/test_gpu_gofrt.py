@@ -197,6 +197,46 @@ def test_recheck_two_stages(tilt, origin):
         assert 6 * (nbin + 1) <= exact < window, (cl, exact, window)
 
 
+@pytest.mark.parametrize("tilt,origin", [(None, 20.0), ((2.5, -1.5, 1.0), -6.0)], ids=["cubic", "tri-corner"])
+def test_fp32_window_edges(tilt, origin):
+    """Pairs along random directions at 1e-6 to 1e-4 A on either side of
+    every bin edge, rmin and rmax: around the FP32 window's width (1e-5 to
+    1e-4 A here), so that some are decided by the FP32 evaluation alone,
+    just outside the window, and some by the rechecks.  Every cluster mode
+    and the 16-bit screen must give the reference's bins."""
+    from analisi_amd import synth
+    rmin, rmax, nbin = 0.5, 4.5, 40
+    dr = (rmax - rmin) / nbin
+    offs = np.array([1e-6, 3e-6, 1e-5, 3e-5, 1e-4])
+    d = []
+    for k in range(0, nbin + 1):
+        e = rmin + k * dr
+        d += list(e - offs) + list(e + offs)
+    d = np.array(d)
+    M = len(d)
+    L = 60.0
+    rng = np.random.default_rng(17)
+    base = origin + 12.0 * rng.random((M, 3))
+    n = rng.normal(size=(M, 3))
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    pos = np.zeros((3, 2 * M, 3))
+    pos[:, 0::2] = base
+    pos[:, 1::2] = base + d[:, None] * n
+    types = (np.arange(2 * M) % 2).astype(np.int32)
+    box = np.tile(synth.lammps_box(L, None if tilt is None else np.array(tilt)), (3, 1))
+    ot = O.Traj(pos, types, box, tilt is not None)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    ref, _ = O.gofrt(ot, rmin, rmax, nbin, 2, 1, 1, 1, 2, 0, want_vdata=False)
+    g = capi.DeviceGofrt(dt, rmin, rmax, nbin, 2, 1, 1, capi.GOFRT_SCREENED)
+    g.reset(2)
+    for cl in (2, 1, 4, 3, 0):
+        g.set_tuning(8, 0, 1 << 30, cl)
+        got = g.calculate(0)
+        assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
+        _, cand, rechecks = g.stats()
+        assert 0 < rechecks < cand, (cl, rechecks, cand)
+
+
 def test_too_short_raises():
     ot, dt = make_case(2, 50, 10, 8.0, [1.0], 0.1)
     g = capi.DeviceGofrt(dt, 0.0, 3.0, 10, 3, 1, 1)
