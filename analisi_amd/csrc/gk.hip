@@ -282,9 +282,12 @@ __device__ __forceinline__ double wave_inclusive_scan(double v) {
 }
 
 // :279-288 I(it) = I(0) + sum_{k<=it} (C(k-1)/2 + C(k)/2), E likewise with
-// weights k/2, as prefix sums over the lags.  Each of the 16 wavefronts owns
-// a contiguous range of lags and reads it 64 consecutive lags at a time, all
-// NC columns of a lag from one lane (coalesced): pass 1 sums its range, the
+// weights k/2, as prefix sums over the lags.  The lags 1 .. leff-1 of a block
+// are cut into gridDim.x segments, one workgroup each.  A workgroup first sums
+// the terms of every lag before its segment (all threads, strided, then a
+// fixed-order reduction), then each of its 16 wavefronts owns a contiguous
+// range of the segment and reads it 64 consecutive lags at a time, all NC
+// columns of a lag from one lane (coalesced): pass 1 sums its range, the
 // ranges' offsets come from an exclusive scan of those sums, pass 2 scans
 // each 64-lag batch across the lanes and carries between batches.
 template <int NC>
@@ -292,11 +295,15 @@ __global__ __launch_bounds__(GK_PAR) void gk_integral_par_kernel(double *__restr
                                                                  uint32_t leff, uint64_t stride) {
     constexpr uint32_t NW = GK_PAR / 64;
     __shared__ double tot[2 * NC][NW];
+    __shared__ double pre[2 * NC][NW];
     if (leff < 2) return;
     out += blockIdx.y * stride;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t n = leff - 1, per = ((n + NW - 1) / NW + 63) / 64 * 64;
-    const uint32_t k0 = 1 + min(n, wv * per), k1 = 1 + min(n, (wv + 1) * per);
+    const uint32_t n = leff - 1;
+    const uint32_t seg = ((n + gridDim.x - 1) / gridDim.x + 63) / 64 * 64;
+    const uint32_t s0 = 1 + min(n, blockIdx.x * seg), s1 = 1 + min(n, (blockIdx.x + 1) * seg);
+    const uint32_t per = ((s1 - s0 + NW - 1) / NW + 63) / 64 * 64;
+    const uint32_t k0 = s0 + min(s1 - s0, wv * per), k1 = s0 + min(s1 - s0, (wv + 1) * per);
     // the two trapezoid terms of lag k for every column
     auto terms = [&](uint32_t k, double (&a)[NC], double (&b)[NC]) {
         const double *rp = out + (uint64_t)(k - 1) * narr, *rv = out + (uint64_t)k * narr;
@@ -308,7 +315,28 @@ __global__ __launch_bounds__(GK_PAR) void gk_integral_par_kernel(double *__restr
             b[j] = cp * dk / 2.0 + v * dk / 2.0;
         }
     };
+    // sums of the terms of lags 1 .. s0-1 (the segments before this one)
     double sa[NC], sb[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) sa[j] = sb[j] = 0.0;
+    for (uint32_t k = 1 + threadIdx.x; k < s0; k += GK_PAR) {
+        double a[NC], b[NC];
+        terms(k, a, b);
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            sa[j] += a[j];
+            sb[j] += b[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        const double ta = wave_inclusive_scan(sa[j]), tb = wave_inclusive_scan(sb[j]);
+        if (lane == 63) {
+            pre[2 * j][wv] = ta;
+            pre[2 * j + 1][wv] = tb;
+        }
+    }
+    // sums of each wavefront's range of the segment
 #pragma unroll
     for (int j = 0; j < NC; j++) sa[j] = sb[j] = 0.0;
     for (uint32_t kb = k0; kb < k1; kb += 64) {
@@ -332,8 +360,9 @@ __global__ __launch_bounds__(GK_PAR) void gk_integral_par_kernel(double *__restr
         }
     }
     __syncthreads();
-    if (threadIdx.x < 2 * NC) {  // exclusive scan of the ranges' sums, in range order
+    if (threadIdx.x < 2 * NC) {  // earlier segments, then an exclusive scan of the ranges' sums, in order
         double run = 0.0;
+        for (uint32_t w = 0; w < NW; w++) run += pre[threadIdx.x][w];
         for (uint32_t w = 0; w < NW; w++) {
             const double t = tot[threadIdx.x][w];
             tot[threadIdx.x][w] = run;
@@ -379,66 +408,71 @@ __global__ __launch_bounds__(GK_PAR) void gk_integral_par_kernel(double *__restr
 
 // k = c00 - c(0,1:) inv(c(1:,1:)) c(1:,0), c column-major n x n; the inverse
 // by LU with partial pivoting (Eigen's dynamic-size inverse); for n == 2 this
-// is c00 - (c01 * (1/c11)) * c10.
-__device__ double schur00(const double *c, int n) {
-    if (n <= 1) return c[0];
-    const int m = n - 1;
-    double lu[(GK_MAXC - 1) * (GK_MAXC - 1)], inv[(GK_MAXC - 1) * (GK_MAXC - 1)], tmp[GK_MAXC - 1];
-    int perm[GK_MAXC - 1];
-    for (int cc = 0; cc < m; cc++)
-        for (int r = 0; r < m; r++) lu[cc * m + r] = c[(cc + 1) * n + (r + 1)];
-    for (int i = 0; i < m; i++) perm[i] = i;
-    for (int k = 0; k < m; k++) {
-        int p = k;
-        double best = fabs(lu[k * m + k]);
-        for (int r = k + 1; r < m; r++)
-            if (fabs(lu[k * m + r]) > best) {
-                best = fabs(lu[k * m + r]);
-                p = r;
+// is c00 - (c01 * (1/c11)) * c10.  n is a template argument so that every
+// loop has constant bounds and the small arrays stay in registers.
+template <int n>
+__device__ __forceinline__ double schur00(const double *c) {
+    if constexpr (n <= 1) {
+        return c[0];
+    } else {
+        constexpr int m = n - 1;
+        double lu[m * m], inv[m * m], tmp[m];
+        int perm[m];
+        for (int cc = 0; cc < m; cc++)
+            for (int r = 0; r < m; r++) lu[cc * m + r] = c[(cc + 1) * n + (r + 1)];
+        for (int i = 0; i < m; i++) perm[i] = i;
+        for (int k = 0; k < m; k++) {
+            int p = k;
+            double best = fabs(lu[k * m + k]);
+            for (int r = k + 1; r < m; r++)
+                if (fabs(lu[k * m + r]) > best) {
+                    best = fabs(lu[k * m + r]);
+                    p = r;
+                }
+            if (p != k) {
+                for (int cc = 0; cc < m; cc++) {
+                    double t = lu[cc * m + k];
+                    lu[cc * m + k] = lu[cc * m + p];
+                    lu[cc * m + p] = t;
+                }
+                int t = perm[k];
+                perm[k] = perm[p];
+                perm[p] = t;
             }
-        if (p != k) {
-            for (int cc = 0; cc < m; cc++) {
-                double t = lu[cc * m + k];
-                lu[cc * m + k] = lu[cc * m + p];
-                lu[cc * m + p] = t;
+            for (int r = k + 1; r < m; r++) lu[k * m + r] /= lu[k * m + k];
+            for (int cc = k + 1; cc < m; cc++)
+                for (int r = k + 1; r < m; r++) lu[cc * m + r] -= lu[k * m + r] * lu[cc * m + k];
+        }
+        for (int col = 0; col < m; col++) {
+            double *x = inv + col * m;
+            for (int r = 0; r < m; r++) x[r] = (perm[r] == col) ? 1.0 : 0.0;
+            for (int r = 0; r < m; r++)
+                for (int k = 0; k < r; k++) x[r] -= lu[k * m + r] * x[k];
+            for (int r = m - 1; r >= 0; r--) {
+                for (int k = r + 1; k < m; k++) x[r] -= lu[k * m + r] * x[k];
+                x[r] = x[r] * (1.0 / lu[r * m + r]);
             }
-            int t = perm[k];
-            perm[k] = perm[p];
-            perm[p] = t;
         }
-        for (int r = k + 1; r < m; r++) lu[k * m + r] /= lu[k * m + k];
-        for (int cc = k + 1; cc < m; cc++)
-            for (int r = k + 1; r < m; r++) lu[cc * m + r] -= lu[k * m + r] * lu[cc * m + k];
-    }
-    for (int col = 0; col < m; col++) {
-        double *x = inv + col * m;
-        for (int r = 0; r < m; r++) x[r] = (perm[r] == col) ? 1.0 : 0.0;
-        for (int r = 0; r < m; r++)
-            for (int k = 0; k < r; k++) x[r] -= lu[k * m + r] * x[k];
-        for (int r = m - 1; r >= 0; r--) {
-            for (int k = r + 1; k < m; k++) x[r] -= lu[k * m + r] * x[k];
-            x[r] = x[r] * (1.0 / lu[r * m + r]);
+        for (int cc = 0; cc < m; cc++) {
+            double s = c[1 * n + 0] * inv[cc * m + 0];
+            for (int k = 1; k < m; k++) s += c[(k + 1) * n + 0] * inv[cc * m + k];
+            tmp[cc] = s;
         }
+        double s = tmp[0] * c[0 * n + 1];
+        for (int cc = 1; cc < m; cc++) s += tmp[cc] * c[0 * n + (cc + 1)];
+        return c[0] - s;
     }
-    for (int cc = 0; cc < m; cc++) {
-        double s = c[1 * n + 0] * inv[cc * m + 0];
-        for (int k = 1; k < m; k++) s += c[(k + 1) * n + 0] * inv[cc * m + k];
-        tmp[cc] = s;
-    }
-    double s = tmp[0] * c[0 * n + 1];
-    for (int cc = 1; cc < m; cc++) s += tmp[cc] * c[0 * n + (cc + 1)];
-    return c[0] - s;
 }
 
 // :289-346 k_gk and k_einst per lag, then :352-356 E /= l
-__global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_t ncur, uint32_t leff,
-                                uint64_t stride) {
+template <int NCUR>
+__global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_t leff, uint64_t stride) {
+    constexpr uint32_t ncur = NCUR, ncorr = NCUR * NCUR;
     const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it == 0 || it >= leff) return;
     out += blockIdx.y * stride;
-    const uint32_t ncorr = ncur * ncur;
     double *cur = out + (uint64_t)it * narr;
-    double matr[GK_MAXC * GK_MAXC];
+    double matr[ncorr];
     for (uint32_t j = 0; j < ncorr; j++) matr[j] = 0.0;
     uint32_t idx = 0;
     for (uint32_t j1 = 0; j1 < ncur; j1++)
@@ -447,7 +481,7 @@ __global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_
             matr[j1 * ncur + j2] += cur[ncorr + 2 * idx] / 2.0;
             idx++;
         }
-    const double kgk = schur00(matr, (int)ncur);
+    const double kgk = schur00<NCUR>(matr);
     for (uint32_t j = 0; j < ncorr; j++) matr[j] = 0.0;
     idx = 0;
     const double dit = (double)it;
@@ -458,10 +492,23 @@ __global__ void gk_schur_kernel(double *__restrict__ out, uint32_t narr, uint32_
             matr[j1 * ncur + j2] += v / 2.0;
             idx++;
         }
-    const double kein = schur00(matr, (int)ncur);
+    const double kein = schur00<NCUR>(matr);
     cur[3 * ncorr + 0] = kgk;
     cur[3 * ncorr + 1] = kein;
     for (uint32_t j = 0; j < ncorr; j++) cur[ncorr + 2 * j + 1] = cur[ncorr + 2 * j + 1] / dit;
+}
+
+static void gk_schur_launch(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, hipStream_t s, uint32_t nblocks,
+                            uint64_t stride) {
+    const dim3 grid((leff + 255) / 256, nblocks);
+    switch (ncur) {
+        case 1: hipLaunchKernelGGL(gk_schur_kernel<1>, grid, dim3(256), 0, s, d, narr, leff, stride); break;
+        case 2: hipLaunchKernelGGL(gk_schur_kernel<2>, grid, dim3(256), 0, s, d, narr, leff, stride); break;
+        case 3: hipLaunchKernelGGL(gk_schur_kernel<3>, grid, dim3(256), 0, s, d, narr, leff, stride); break;
+        case 4: hipLaunchKernelGGL(gk_schur_kernel<4>, grid, dim3(256), 0, s, d, narr, leff, stride); break;
+        default: fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
+    }
+    ANL_HIP(hipGetLastError());
 }
 
 void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int subtract_mean, uint32_t start_mean,
@@ -473,10 +520,13 @@ void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int sub
                                start_mean, stride);
             ANL_HIP(hipGetLastError());
         }
+        // segments of about 2,048 lags per workgroup, at most 16 per block
+        const uint32_t nseg = std::max(1u, std::min(16u, (leff + 2047) / 2048));
         switch (ncorr) {
 #define ANL_GK_PAR(NCV)                                                                                          \
     case NCV:                                                                                                    \
-        hipLaunchKernelGGL(gk_integral_par_kernel<NCV>, dim3(1, nblocks), dim3(GK_PAR), 0, s, d, narr, leff, stride); \
+        hipLaunchKernelGGL(gk_integral_par_kernel<NCV>, dim3(nseg, nblocks), dim3(GK_PAR), 0, s, d, narr, leff,    \
+                           stride);                                                                            \
         break;
             ANL_GK_PAR(1)
             ANL_GK_PAR(4)
@@ -487,9 +537,7 @@ void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int sub
                 fail(ANL_ERR_ARG, "between 1 and 4 currents are supported");
         }
         ANL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256, nblocks), dim3(256), 0, s, d, narr, ncur, leff,
-                           stride);
-        ANL_HIP(hipGetLastError());
+        gk_schur_launch(d, narr, ncur, leff, s, nblocks, stride);
         return;
     }
     // lag tile of the scans: two buffers of tl x ncorr doubles, <= 64 KiB
@@ -505,9 +553,7 @@ void gk_epilogue(double *d, uint32_t narr, uint32_t ncur, uint32_t leff, int sub
     const size_t ldsi = 6 * (size_t)tli * ncorr * sizeof(double);
     hipLaunchKernelGGL(gk_integral_kernel, dim3(1, nblocks), dim3(256), ldsi, s, d, narr, ncorr, leff, stride, tli);
     ANL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gk_schur_kernel, dim3((leff + 255) / 256, nblocks), dim3(256), 0, s, d, narr, ncur, leff,
-                       stride);
-    ANL_HIP(hipGetLastError());
+    gk_schur_launch(d, narr, ncur, leff, s, nblocks, stride);
 }
 
 void gk_fp64_enqueue_blocks(anl_series *s, const uint32_t *idx_j, uint32_t ncur, uint32_t skip, uint32_t leff,

@@ -318,17 +318,25 @@ __global__ __launch_bounds__(64 * WG, 2) void gkm_tile_kernel(const GkmBatch a) 
 // Segment mean of product/3 for lag l and pair (diagonal l - 32q of tiles
 // q = floor(l/32), floor(l/32) + 1; chunks in order), folded into the running
 // mean over segments (:230-237); the first segment also clears the other
-// columns of the row.  Grid: x = (lag, column), y = block.
+// columns of the row.  Grid: x = work item, y = block.  The first
+// npairs * leff items are (pair column, lag), consecutive threads taking
+// consecutive lags of one column, so that their partial-sum reads (63
+// diagonals per tile and chunk) are contiguous; the rest clear the other
+// columns.
 __global__ __launch_bounds__(256) void gkm_reduce_kernel(const GkmBatch a, uint32_t iseg) {
     const uint32_t b = blockIdx.y;
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.leff * a.narr) return;
-    const uint32_t l = w / a.narr, col = w % a.narr, npairs = a.ncur * a.ncur;
-    double *v = a.out + b * a.stride + (uint64_t)l * a.narr + col;
-    if (col >= npairs) {
-        if (iseg == 0) *v = 0.0;
+    const uint32_t npairs = a.ncur * a.ncur, npw = npairs * a.leff;
+    if (w >= npw) {
+        if (iseg == 0) {
+            const uint32_t r = w - npw, nother = a.narr - npairs;
+            a.out[b * a.stride + (uint64_t)(r / nother) * a.narr + npairs + r % nother] = 0.0;
+        }
         return;
     }
+    const uint32_t col = w / a.leff, l = w % a.leff;
+    double *v = a.out + b * a.stride + (uint64_t)l * a.narr + col;
     const uint32_t nchunks = a.z0[b + 1] - a.z0[b];
     double s = 0.0;
     for (uint32_t q = l / 32; q <= l / 32 + 1 && q < a.ntiles; q++) {
