@@ -68,3 +68,33 @@ def test_screened_matches_oracle_on_random_cases(seed):
         c_units, units = g.cluster_stats()
         print("  mode %d: %d candidates, %d rechecks (%d exact), %d of %d units clustered"
               % (cl, cand, rechecks, exact, c_units, units), flush=True)
+
+
+@pytest.mark.parametrize("tilt", [None, (3.0, -2.0, 1.5)], ids=["cubic", "tri"])
+def test_screened_far_from_origin(tilt):
+    """Unwrapped positions thousands of cell lengths from the origin: the
+    reference's FP64 minimum image then takes thousands of steps per pair and
+    its rounding (on coordinates near 1e5 A) is no longer negligible next to
+    the 32-bit quantisation; the FP64 evaluation from the fractions bounds it
+    from the largest position (ref_vector_rounding), so the counts must stay
+    the reference's in every mode."""
+    L, N, F, n, tmax = 36.0, 1500, 5, 2, 2
+    rmin, rmax, nbin = 0.4, 7.5, 300
+    box_l = synth.lammps_box(L, tilt)
+    pos = synth.random_walk(91, N, F, box_l, 0.3)
+    rng = np.random.default_rng(92)
+    shift = rng.integers(-3000, 3000, size=(N, 3)) * L  # whole cell lengths, as unwrapped runs accumulate
+    pos += shift[None, :, :]
+    types = synth.types_by_fraction(N, [0.5, 0.5])
+    ot = O.Traj(pos, types, np.tile(box_l, (F, 1)), tilt is not None)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    ref, _ = O.gofrt(ot, rmin, rmax, nbin, tmax, 16, 1, 1, n, 0, want_vdata=False)
+    assert ref.sum() > 0
+    g = capi.DeviceGofrt(dt, rmin, rmax, nbin, tmax, 1, 1, capi.GOFRT_SCREENED)
+    g.reset(n)
+    for cl in (2, 1, 4, 3, 0):
+        g.set_tuning(8, 0, 1 << 30, cl)
+        got = g.calculate(0)
+        assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
+        window, exact = g.recheck_stats()
+        print("  mode %d: %d rechecks, %d exact" % (cl, window, exact), flush=True)
