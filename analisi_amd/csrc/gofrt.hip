@@ -467,17 +467,39 @@ __device__ __forceinline__ void lds_setup(const GofrtArgs &a, unsigned *hist, do
     for (int k = threadIdx.x; k < (int)(a.nt * a.nt); k += WG) chan[k] = a.chan[k];
 }
 
-// Screened form: the histogram offset of a pair, by (type pair, same atom):
-// hb[(ti nt + tj) 2 + self] = (channel + self * half) * nbin.
-__device__ __forceinline__ void lds_setup_hb(const GofrtArgs &a, unsigned *hist, double *thr, int *hb) {
+// Screened form: the workgroup's histogram rows are the ordered type pairs of
+// distinct atoms (row ti nt + tj) and the types of an atom paired with itself
+// (row nt^2 + ti): nt (nt + 1) rows, as many as the reference's channels, so
+// that a candidate's row is arithmetic (hrow) and the flush folds the rows
+// into the channels: rowch[row] = (channel) * nbin, the rows (ti, tj) and
+// (tj, ti) sharing the channel of the pair and row nt^2 + t the self channel
+// half + channel(t, t).
+__device__ __forceinline__ void lds_setup_rows(const GofrtArgs &a, unsigned *hist, double *thr, int *rowch) {
     const int nh = (int)(a.nch * a.nbin);
     for (int k = threadIdx.x; k < nh; k += WG) hist[k] = 0u;
     for (int k = threadIdx.x; k <= (int)a.nbin; k += WG) thr[k] = a.thr[k];
-    for (int k = threadIdx.x; k < (int)(2 * a.nt * a.nt); k += WG)
-        hb[k] = (a.chan[k >> 1] + ((k & 1) ? (int)a.half : 0)) * (int)a.nbin;
+    const int nt2 = (int)(a.nt * a.nt);
+    for (int k = threadIdx.x; k < nt2 + (int)a.nt; k += WG)
+        rowch[k] = (k < nt2 ? a.chan[k] : a.chan[(k - nt2) * ((int)a.nt + 1)] + (int)a.half) * (int)a.nbin;
 }
-__device__ __forceinline__ uint32_t hb_key(uint32_t wi, uint32_t wj, uint32_t nt) {
-    return ((__umul24(wi & 0xFFu, nt) + (wj & 0xFFu)) << 1) | (wi == wj ? 1u : 0u);
+__device__ __forceinline__ uint32_t hrow(uint32_t wi, uint32_t wj, uint32_t nt) {
+    const uint32_t ti = wi & 0xFFu, tj = wj & 0xFFu;
+    return wi == wj ? nt * nt + ti : __umul24(ti, nt) + tj;
+}
+
+// The screened form's flush: row r of the workgroup's histogram adds to its
+// channel rowch[r] (see lds_setup_rows), from a rotated start as lds_flush.
+__device__ __forceinline__ void lds_flush_rows(const GofrtArgs &a, const unsigned *hist, const int *rowch,
+                                               uint32_t block, uint32_t t) {
+    const int nh = (int)(a.nch * a.nbin);
+    unsigned long long *o = a.out + (uint64_t)block * a.out_block_stride + (uint64_t)t * nh;
+    const int rot = (int)((blockIdx.x * 977u) % (uint32_t)nh);
+    for (int k0 = threadIdx.x; k0 < nh; k0 += WG) {
+        const int k = k0 + rot < nh ? k0 + rot : k0 + rot - nh;
+        const unsigned v = hist[k];
+        const int r = k / (int)a.nbin;
+        if (v) atomicAdd(o + rowch[r] + (k - r * (int)a.nbin), (unsigned long long)v);
+    }
 }
 
 // Workgroups of one (block, lag) flush into the same counters, often at the
@@ -588,7 +610,7 @@ __device__ __forceinline__ uint32_t lane_slot(unsigned long long m, uint32_t bas
 // outside the window both floor to the same bin.
 template <bool TRI>
 __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 uj,
-                                               const int *hb, unsigned *hist) {
+                                               unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint4 ui = fi_lds[il];  // .w = type | slot << 8 of i; uj likewise of j
     const float fa = (float)(int)(ui.x - uj.x);
@@ -608,7 +630,7 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
     if (fabsf(u - rintf(u)) < a.du_margin) return true;
     const int bb = (int)floorf(u);
-    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hb[hb_key(ui.w, uj.w, a.nt)] + bb], 1u);
+    if ((unsigned)bb < a.nbin) atomicAdd(&hist[__umul24(hrow(ui.w, uj.w, a.nt), a.nbin) + bb], 1u);
     return false;
 }
 
@@ -640,7 +662,7 @@ __device__ __forceinline__ uint4 offset32(const GofrtArgs &a, const uint4 f, uin
 // vector is P - Q; margin (bins) bounds the error of this form for the
 // wavefront's radius.
 __device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 qj,
-                                                   const int *hb, unsigned *hist, float margin) {
+                                                   unsigned *hist, float margin) {
     const uint4 pi = fi_lds[e & 0xFFu];
     const float dx = __uint_as_float(pi.x) - __uint_as_float(qj.x);
     const float dy = __uint_as_float(pi.y) - __uint_as_float(qj.y);
@@ -649,7 +671,7 @@ __device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t 
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
     if (fabsf(u - rintf(u)) < margin) return true;
     const int bb = (int)floorf(u);
-    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hb[hb_key(pi.w, qj.w, a.nt)] + bb], 1u);
+    if ((unsigned)bb < a.nbin) atomicAdd(&hist[__umul24(hrow(pi.w, qj.w, a.nt), a.nbin) + bb], 1u);
     return false;
 }
 
@@ -695,7 +717,7 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, bool same_
                                                   const uint4 *__restrict__ fi32,
                                                   const uint4 *__restrict__ fj32, const double4 *__restrict__ pi,
                                                   const double4 *__restrict__ pj, const double *cell,
-                                                  const double *thr, const int *hb, unsigned *hist) {
+                                                  const double *thr, unsigned *hist) {
     const uint32_t il = e & 0xFFu;
     const uint32_t j = e >> 8;
     const uint4 ui = fi32[il], uj = fj32[j];  // .w = type | slot << 8
@@ -739,7 +761,7 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, bool same_
         const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
         if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) bb = bin_of(d2r, thr, nb, a.rminf, a.inv_drf);
     }
-    if (bb >= 0) atomicAdd(&hist[hb[hb_key(ui.w, uj.w, a.nt)] + bb], 1u);
+    if (bb >= 0) atomicAdd(&hist[__umul24(hrow(ui.w, uj.w, a.nt), a.nbin) + bb], 1u);
     return exact;
 }
 
@@ -829,9 +851,9 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     uint32_t *queue = (uint32_t *)(fi_lds + WG + (COLC ? WG : 0));  // [WG/64][QC]
     uint32_t *rqueue = queue + (WG / 64) * QC;              // [WG/64][RQCAP]
     double *thr = (double *)(rqueue + (WG / 64) * RQCAP);   // [nbin+1]
-    int *hb = (int *)(thr + a.nbin + 1);                    // [nt*nt*2]
-    unsigned *hist = (unsigned *)(hb + 2 * a.nt * a.nt);    // [nch*nbin]
-    lds_setup_hb(a, hist, thr, hb);
+    int *rowch = (int *)(thr + a.nbin + 1);                 // [nt*nt*2], the first nt*(nt+1) used
+    unsigned *hist = (unsigned *)(rowch + 2 * a.nt * a.nt); // [nch*nbin]
+    lds_setup_rows(a, hist, thr, rowch);
 
     const uint64_t N = a.N;
     const uint32_t lane = threadIdx.x & 63u;
@@ -898,7 +920,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     wave_sync_lds();
                     rn -= 64;
                     uadd<TRI>(nre, 64);
-                    const bool ex = recheck_candidate<TRI>(a, t == 0, rq[rn + lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
+                    const bool ex = recheck_candidate<TRI>(a, t == 0, rq[rn + lane], fi32, fj32, pi, pj, cellp, thr, hist);
                     uadd<TRI>(nex, (uint32_t)__popcll(__ballot(ex)));
                     wave_sync_lds();
                 }
@@ -907,7 +929,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         // the j atom's fractions from global memory (16-bit screen, final drain)
         auto eval_batch = [&](uint32_t e, bool valid) {
             eval_with(
-                e, valid, [&](uint32_t x) { return eval_candidate<TRI>(a, x, fi_lds, fj32[x >> 8], hb, hist); },
+                e, valid, [&](uint32_t x) { return eval_candidate<TRI>(a, x, fi_lds, fj32[x >> 8], hist); },
                 [](uint32_t x) { return x; });
         };
         bool clustered = false;
@@ -1025,7 +1047,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                 eval_with(
                                     e, valid,
                                     [&](uint32_t x) {
-                                        return eval_candidate_off(a, x, fi_lds, colc[x >> 8], hb, hist, margin_off);
+                                        return eval_candidate_off(a, x, fi_lds, colc[x >> 8], hist, margin_off);
                                     },
                                     [&](uint32_t x) {
                                         const uint32_t c = x >> 8;
@@ -1264,7 +1286,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         }
         wave_sync_lds();
         bool ex = false;
-        if (lane < rn) ex = recheck_candidate<TRI>(a, t == 0, rq[lane], fi32, fj32, pi, pj, cellp, thr, hb, hist);
+        if (lane < rn) ex = recheck_candidate<TRI>(a, t == 0, rq[lane], fi32, fj32, pi, pj, cellp, thr, hist);
         uadd<TRI>(nex, (uint32_t)__popcll(__ballot(ex)));
         uadd<TRI>(nre, rn);
         wave_sync_lds();
@@ -1283,7 +1305,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         if (n16) atomicAdd(a.ncand + STAT_PAIRS_16, n16);
     }
     __syncthreads();
-    lds_flush(a, hist, w.block, t);
+    lds_flush_rows(a, hist, rowch, w.block, t);
 }
 #undef ANL_APPEND
 
