@@ -338,13 +338,21 @@ __global__ __launch_bounds__(256) void gkm_reduce_kernel(const GkmBatch a, uint3
     const uint32_t col = w / a.leff, l = w % a.leff;
     double *v = a.out + b * a.stride + (uint64_t)l * a.narr + col;
     const uint32_t nchunks = a.z0[b + 1] - a.z0[b];
-    double s = 0.0;
+    // four running sums over the chunks (c mod 4), added in a fixed order:
+    // independent chains keep four loads in flight
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t q = l / 32; q <= l / 32 + 1 && q < a.ntiles; q++) {
         const int d = (int)l - 32 * (int)q;
         if (d < -31 || d > 31) continue;
         const double *p = a.partial + a.poff[b] + ((uint64_t)col * a.ntiles + q) * nchunks * 63 + (d + 31);
-        for (uint32_t ch = 0; ch < nchunks; ch++) s += p[(uint64_t)ch * 63];
+        uint32_t ch = 0;
+        for (; ch + 4 <= nchunks; ch += 4) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) s4[k] += p[(uint64_t)(ch + k) * 63];
+        }
+        for (; ch < nchunks; ch++) s4[ch & 3u] += p[(uint64_t)ch * 63];
     }
+    const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
     const uint32_t seg = a.n[b] / a.n_seg;
     const uint32_t lo = seg * iseg, hi = seg * (iseg + 1);
     const uint32_t first = lo + lo % a.skip;  // greenkubo...cpp:208
