@@ -765,10 +765,15 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, bool same_
     return exact;
 }
 
-constexpr int SQCAP = 576;   // per-wavefront queue: < 64 left + up to 8 masks of 64
-constexpr int SQCAP_CL = 1088;  // cluster screen: < 64 left + 16 masks of 64 (one drain per 32 x 32 tile)
-constexpr int SQCAP_CLC = 576;  // cluster screen with the LDS column cache: < 64 left + 8 masks (a drain per 8)
-constexpr int RQCAP = 128;   // per-wavefront recheck queue: < 64 left + one batch
+// Per-wavefront queues.  Each ends with QSINK = 64 entries that only take
+// the stores of lanes whose test failed (entry sink + lane), so that an
+// append is one store by every lane, with no change of the execution mask;
+// nothing reads them.
+constexpr int QSINK = 64;
+constexpr int SQCAP = 576 + QSINK;       // 16-bit screen: < 64 left + up to 8 masks of 64
+constexpr int SQCAP_CL = 1088 + QSINK;   // cluster screen: < 64 left + 16 masks of 64 (one drain per 32 x 32 tile)
+constexpr int SQCAP_CLC = 576 + QSINK;   // cluster screen with the LDS column cache: < 64 left + 8 masks (a drain per 8)
+constexpr int RQCAP = 128 + QSINK;       // recheck queue: < 64 left + one batch
 
 // c += v for a wavefront-uniform count.  PIN keeps it in scalar registers
 // (readfirstlane): that lowers the vector-register spills of the triclinic
@@ -815,15 +820,16 @@ __host__ __device__ __forceinline__ float cluster_threshold(float rho, float rma
     return (a * a + b * b * 0x1p-10f) * (1.0f + 0x1p-12f) + 1e-4f;
 }
 
+// appends: lanes whose test failed store into the queue's sink
 #define ANL_APPEND_E(P, M, ENTRY)                                           \
     if (M) {                                                                \
-        if (P) q[qn + lane_prefix(M)] = (ENTRY);                            \
+        q[(P) ? qn + lane_prefix(M) : (uint32_t)(QC - QSINK) + lane] = (ENTRY); \
         qn += (uint32_t)__popcll(M);                                        \
     }
 
 #define ANL_APPEND(P, M, J)                                                 \
     if (M) {                                                                \
-        if (P) q[lane_slot(M, qn)] = ((uint32_t)(J) << 8) | il;             \
+        q[(P) ? lane_slot(M, qn) : (uint32_t)(QC - QSINK) + lane] = ((uint32_t)(J) << 8) | il; \
         qn += (uint32_t)__popcll(M);                                        \
     }
 
@@ -914,7 +920,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
             const bool need = valid && ev(e);
             const unsigned long long mr = __ballot(need);
             if (mr) {
-                if (need) rq[rn + lane_prefix(mr)] = to_j(e);
+                rq[need ? rn + lane_prefix(mr) : (uint32_t)(RQCAP - QSINK) + lane] = to_j(e);
                 rn += (uint32_t)__popcll(mr);
                 if (rn >= 64) {
                     wave_sync_lds();
