@@ -470,7 +470,7 @@ __device__ __forceinline__ void lds_setup(const GofrtArgs &a, unsigned *hist, do
 // Screened form: the workgroup's histogram rows are the ordered type pairs of
 // distinct atoms (row ti nt + tj) and the types of an atom paired with itself
 // (row nt^2 + ti): nt (nt + 1) rows, as many as the reference's channels, so
-// that a candidate's row is arithmetic (hrow) and the flush folds the rows
+// that a candidate's row is arithmetic (hidx) and the flush folds the rows
 // into the channels: rowch[row] = (channel) * nbin, the rows (ti, tj) and
 // (tj, ti) sharing the channel of the pair and row nt^2 + t the self channel
 // half + channel(t, t).
@@ -482,9 +482,19 @@ __device__ __forceinline__ void lds_setup_rows(const GofrtArgs &a, unsigned *his
     for (int k = threadIdx.x; k < nt2 + (int)a.nt; k += WG)
         rowch[k] = (k < nt2 ? a.chan[k] : a.chan[(k - nt2) * ((int)a.nt + 1)] + (int)a.half) * (int)a.nbin;
 }
-__device__ __forceinline__ uint32_t hrow(uint32_t wi, uint32_t wj, uint32_t nt) {
-    const uint32_t ti = wi & 0xFFu, tj = wj & 0xFFu;
-    return wi == wj ? nt * nt + ti : __umul24(ti, nt) + tj;
+// x y + z in one 24-bit multiply-add (x, y < 2^24; the compiler otherwise
+// picks a 64-bit multiply-add or a multiply, shifts and a three-way add)
+__device__ __forceinline__ uint32_t mad24(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(x), "s"(y), "v"(z));
+    return r;
+}
+// the histogram index of a pair: its row (see lds_setup_rows) times nbin
+// plus its bin bb; both rows are formed and one selected, without a branch
+__device__ __forceinline__ uint32_t hidx(uint32_t wi, uint32_t wj, uint32_t nt, uint32_t nbin, uint32_t bb) {
+    const uint32_t ti = wi & 0xFFu;
+    const uint32_t distinct = mad24(ti, nt, wj & 0xFFu), self = nt * nt + ti;
+    return mad24(wi == wj ? self : distinct, nbin, bb);
 }
 
 // The screened form's flush: row r of the workgroup's histogram adds to its
@@ -630,7 +640,7 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
     if (fabsf(u - rintf(u)) < a.du_margin) return true;
     const int bb = (int)floorf(u);
-    if ((unsigned)bb < a.nbin) atomicAdd(&hist[__umul24(hrow(ui.w, uj.w, a.nt), a.nbin) + bb], 1u);
+    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hidx(ui.w, uj.w, a.nt, a.nbin, (uint32_t)bb)], 1u);
     return false;
 }
 
@@ -671,7 +681,7 @@ __device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t 
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
     if (fabsf(u - rintf(u)) < margin) return true;
     const int bb = (int)floorf(u);
-    if ((unsigned)bb < a.nbin) atomicAdd(&hist[__umul24(hrow(pi.w, qj.w, a.nt), a.nbin) + bb], 1u);
+    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hidx(pi.w, qj.w, a.nt, a.nbin, (uint32_t)bb)], 1u);
     return false;
 }
 
@@ -761,7 +771,7 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, bool same_
         const double d2r = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
         if (d2r >= a.lo && d2r <= a.rmax2 && d2r < a.Tn) bb = bin_of(d2r, thr, nb, a.rminf, a.inv_drf);
     }
-    if (bb >= 0) atomicAdd(&hist[__umul24(hrow(ui.w, uj.w, a.nt), a.nbin) + bb], 1u);
+    if (bb >= 0) atomicAdd(&hist[hidx(ui.w, uj.w, a.nt, a.nbin, (uint32_t)bb)], 1u);
     return exact;
 }
 
