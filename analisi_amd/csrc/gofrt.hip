@@ -489,12 +489,21 @@ __device__ __forceinline__ uint32_t mad24(uint32_t x, uint32_t y, uint32_t z) {
     asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(x), "s"(y), "v"(z));
     return r;
 }
-// the histogram index of a pair: its row (see lds_setup_rows) times nbin
-// plus its bin bb; both rows are formed and one selected, without a branch
+// the histogram index of a pair from the .w words (type | slot << 8) of its
+// atoms: row ti nt + tj (see lds_setup_rows) times nbin plus its bin bb.
+// Every pair takes its distinct row, an atom paired with itself included:
+// the screened kernel moves each self pair to its self row afterwards
+// (move_self_pair), which keeps the per-candidate index free of the test.
 __device__ __forceinline__ uint32_t hidx(uint32_t wi, uint32_t wj, uint32_t nt, uint32_t nbin, uint32_t bb) {
-    const uint32_t ti = wi & 0xFFu;
-    const uint32_t distinct = mad24(ti, nt, wj & 0xFFu), self = nt * nt + ti;
-    return mad24(wi == wj ? self : distinct, nbin, bb);
+    return mad24(mad24(wi & 0xFFu, nt, wj & 0xFFu), nbin, bb);
+}
+
+// v_cvt_flr_i32_f32: (int)floor(x) in one instruction (the compiler emits a
+// floor and a conversion)
+__device__ __forceinline__ int floor_i32(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 // The screened form's flush: row r of the workgroup's histogram adds to its
@@ -620,7 +629,7 @@ __device__ __forceinline__ uint32_t lane_slot(unsigned long long m, uint32_t bas
 // outside the window both floor to the same bin.
 template <bool TRI>
 __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 uj,
-                                               unsigned *hist) {
+                                               unsigned *hist, unsigned *sink) {
     const uint32_t il = e & 0xFFu;
     const uint4 ui = fi_lds[il];  // .w = type | slot << 8 of i; uj likewise of j
     const float fa = (float)(int)(ui.x - uj.x);
@@ -638,17 +647,20 @@ __device__ __forceinline__ bool eval_candidate(const GofrtArgs &a, uint32_t e, c
     }
     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
-    if (fabsf(u - rintf(u)) < a.du_margin) return true;
-    const int bb = (int)floorf(u);
-    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hidx(ui.w, uj.w, a.nt, a.nbin, (uint32_t)bb)], 1u);
-    return false;
+    const bool need = fabsf(u - rintf(u)) < a.du_margin;
+    const int bb = floor_i32(u);
+    const bool add = !need && (unsigned)bb < a.nbin;
+    atomicAdd(add ? &hist[hidx(ui.w, uj.w, a.nt, a.nbin, (uint32_t)bb)] : sink, 1u);
+    return need;
 }
 
 // FP32 Cartesian offset of an atom (32-bit fractions f) from a centre c
 // (32-bit fractional units): A' float(f - c), the wrapped difference taken
-// as a signed integer; .w keeps the atom's type | slot << 8.
+// as a signed integer; .w = w, the atom's part of its pairs' histogram index
+// (eval_candidate_off).
 template <bool TRI>
-__device__ __forceinline__ uint4 offset32(const GofrtArgs &a, const uint4 f, uint32_t ca, uint32_t cb, uint32_t cc) {
+__device__ __forceinline__ uint4 offset32(const GofrtArgs &a, const uint4 f, uint32_t ca, uint32_t cb, uint32_t cc,
+                                          uint32_t w) {
     const float fa = (float)(int)(f.x - ca);
     const float fb = (float)(int)(f.y - cb);
     const float fc = (float)(int)(f.z - cc);
@@ -662,7 +674,7 @@ __device__ __forceinline__ uint4 offset32(const GofrtArgs &a, const uint4 f, uin
         y = a.cy * fb;
         z = a.cz * fc;
     }
-    return make_uint4(__float_as_uint(x), __float_as_uint(y), __float_as_uint(z), f.w);
+    return make_uint4(__float_as_uint(x), __float_as_uint(y), __float_as_uint(z), w);
 }
 
 // Mode 2: a candidate from the offsets of both atoms from the wavefront's
@@ -670,19 +682,21 @@ __device__ __forceinline__ uint4 offset32(const GofrtArgs &a, const uint4 f, uin
 // reference can count, wrapping relative to the centre picks the
 // reference's image (the cluster screen's validity condition), so the pair
 // vector is P - Q; margin (bins) bounds the error of this form for the
-// wavefront's radius.
+// wavefront's radius.  The .w words hold 4 ti nt nbin (row) and 4 tj nbin
+// (column), so that the histogram's byte offset is their sum plus 4 bins.
 __device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 qj,
-                                                   unsigned *hist, float margin) {
+                                                   unsigned *hist, unsigned *sink, float margin) {
     const uint4 pi = fi_lds[e & 0xFFu];
     const float dx = __uint_as_float(pi.x) - __uint_as_float(qj.x);
     const float dy = __uint_as_float(pi.y) - __uint_as_float(qj.y);
     const float dz = __uint_as_float(pi.z) - __uint_as_float(qj.z);
     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float u = (__builtin_amdgcn_sqrtf(d2) - a.rminf) * a.inv_drf;
-    if (fabsf(u - rintf(u)) < margin) return true;
-    const int bb = (int)floorf(u);
-    if ((unsigned)bb < a.nbin) atomicAdd(&hist[hidx(pi.w, qj.w, a.nt, a.nbin, (uint32_t)bb)], 1u);
-    return false;
+    const bool need = fabsf(u - rintf(u)) < margin;
+    const int bb = floor_i32(u);
+    const bool add = !need && (unsigned)bb < a.nbin;
+    atomicAdd(add ? (unsigned *)((unsigned char *)hist + (pi.w + qj.w + ((uint32_t)bb << 2))) : sink, 1u);
+    return need;
 }
 
 // A pointer into read-only device memory for the recheck's rare path, opaque
@@ -775,10 +789,33 @@ __device__ __forceinline__ bool recheck_candidate(const GofrtArgs &a, bool same_
     return exact;
 }
 
+// An atom paired with itself, at the t0 frame fi and the frame fj: every
+// candidate path above bins it like any pair, into the distinct row (ti, ti)
+// of its type and with the reference's bin (or not at all).  This moves it
+// to the self row nt^2 + ti: the bin is that of the reference's FP64
+// sequence from the positions (pos, in slot order), the same bin, so the
+// count taken from the distinct row is the one the pair put there.  w is
+// the atom's .w word (type | slot << 8), cell the t0 frame's cell.
+template <bool TRI>
+__device__ __forceinline__ void move_self_pair(const GofrtArgs &a, uint64_t fi, uint64_t fj, uint32_t w,
+                                               const double *cell, const double *thr, unsigned *hist) {
+    const uint32_t sl = w >> 8, ti = w & 0xFFu;
+    const double4 xi = a.pos[fi * a.N + sl];
+    const double4 xj = a.pos[fj * a.N + sl];
+    const DevCell c = load_cell(rare_const<TRI>(cell));
+    const double d2 = d2_ref<TRI>(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, c);
+    if (d2 >= a.lo && d2 <= a.rmax2 && d2 < a.Tn) {
+        const uint32_t bb = (uint32_t)bin_of(d2, thr, (int)a.nbin, a.rminf, a.inv_drf);
+        atomicSub(&hist[(ti * a.nt + ti) * a.nbin + bb], 1u);
+        atomicAdd(&hist[(a.nt * a.nt + ti) * a.nbin + bb], 1u);
+    }
+}
+
 // Per-wavefront queues.  Each ends with QSINK = 64 entries that only take
 // the stores of lanes whose test failed (entry sink + lane), so that an
 // append is one store by every lane, with no change of the execution mask;
-// nothing reads them.
+// nothing reads them.  The recheck queue's sink also takes the histogram adds
+// of candidates that are not counted.
 constexpr int QSINK = 64;
 constexpr int SQCAP = 576 + QSINK;       // 16-bit screen: < 64 left + up to 8 masks of 64
 constexpr int SQCAP_CL = 1088 + QSINK;   // cluster screen: < 64 left + 16 masks of 64 (one drain per 32 x 32 tile)
@@ -876,6 +913,9 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     // per-wavefront queue; its base is wavefront-uniform (kept scalar)
     uint32_t *q = queue + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * QC;
     uint32_t *rq = rqueue + (threadIdx.x >> 6) * RQCAP;
+    // this lane's sink entry of the recheck queue also takes the histogram
+    // add of a candidate that is not counted, so that the add needs no branch
+    unsigned *rsink = rq + (RQCAP - QSINK) + lane;
     const uint64_t slot0 = (uint64_t)id.tile * WG;
     const uint64_t slot = slot0 + threadIdx.x;
     // an atom range of calc_single_th narrower than all atoms is tested per
@@ -945,7 +985,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         // the j atom's fractions from global memory (16-bit screen, final drain)
         auto eval_batch = [&](uint32_t e, bool valid) {
             eval_with(
-                e, valid, [&](uint32_t x) { return eval_candidate<TRI>(a, x, fi_lds, fj32[x >> 8], hist); },
+                e, valid, [&](uint32_t x) { return eval_candidate<TRI>(a, x, fi_lds, fj32[x >> 8], hist, rsink); },
                 [](uint32_t x) { return x; });
         };
         bool clustered = false;
@@ -982,12 +1022,15 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     const float thc = cluster_threshold(rho, a.rmaxf, a.dq);
                     // mode 2: candidates subtract FP32 offsets from the centre;
                     // the rows' offsets replace their fractions in fi_lds, which
-                    // only this wavefront reads (the .w words stay)
+                    // only this wavefront reads, with .w = the row's 4 ti nt nbin
                     float margin_off = 0.0f;
                     if constexpr (COLC) {
                         margin_off = fmaf(a.du_m1, rho + a.dq, a.du_m0);
-                        if (slot < N)
-                            fi_lds[threadIdx.x] = offset32<TRI>(a, fi_lds[threadIdx.x], ca << 16, cb << 16, cc << 16);
+                        if (slot < N) {
+                            const uint4 f = fi_lds[threadIdx.x];
+                            fi_lds[threadIdx.x] =
+                                offset32<TRI>(a, f, ca << 16, cb << 16, cc << 16, (f.w & 0xFFu) * a.nt * a.nbin * 4u);
+                        }
                         wave_sync_lds();
                     }
                     // j with |q| > rho + rmax + dq is farther than rmax + dq from
@@ -1052,8 +1095,12 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                         // step drains it); otherwise candidates load them from
                         // global memory
                         if constexpr (COLC) {
-                            colc[lane] = jin ? offset32<TRI>(a, fj32[jl], ca << 16, cb << 16, cc << 16)
-                                             : make_uint4(0u, 0u, 0u, 0u);
+                            uint4 o = make_uint4(0u, 0u, 0u, 0u);
+                            if (jin) {
+                                const uint4 f = fj32[jl];
+                                o = offset32<TRI>(a, f, ca << 16, cb << 16, cc << 16, (f.w & 0xFFu) * a.nbin * 4u);
+                            }
+                            colc[lane] = o;
                             wave_sync_lds();
                         }
                         // COLC: entries hold the step's column c (0..63) instead of
@@ -1063,7 +1110,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                 eval_with(
                                     e, valid,
                                     [&](uint32_t x) {
-                                        return eval_candidate_off(a, x, fi_lds, colc[x >> 8], hist, margin_off);
+                                        return eval_candidate_off(a, x, fi_lds, colc[x >> 8], hist, rsink, margin_off);
                                     },
                                     [&](uint32_t x) {
                                         const uint32_t c = x >> 8;
@@ -1305,6 +1352,8 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
         if (lane < rn) ex = recheck_candidate<TRI>(a, t == 0, rq[lane], fi32, fj32, pi, pj, cellp, thr, hist);
         uadd<TRI>(nex, (uint32_t)__popcll(__ballot(ex)));
         uadd<TRI>(nre, rn);
+        // this lane's atom with itself, binned above in its distinct row
+        if (active) move_self_pair<TRI>(a, fi, fj, fi32[il].w, cellp, thr, hist);
         wave_sync_lds();
     }
     // per-wavefront totals, one atomic each per wavefront
