@@ -498,6 +498,15 @@ __device__ __forceinline__ uint32_t hidx(uint32_t wi, uint32_t wj, uint32_t nt, 
     return mad24(mad24(wi & 0xFFu, nt, wj & 0xFFu), nbin, bb);
 }
 
+// LDS (address space 3) pointers and the 32-bit LDS address of a pointer
+// into the workgroup's shared memory
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(3))) u32x4_t lds_u4_t;
+typedef __attribute__((address_space(3))) unsigned lds_u32_t;
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+    return (uint32_t)(size_t)(const __attribute__((address_space(3))) unsigned char *)p;
+}
+
 // v_cvt_flr_i32_f32: (int)floor(x) in one instruction (the compiler emits a
 // floor and a conversion)
 __device__ __forceinline__ int floor_i32(float x) {
@@ -682,11 +691,13 @@ __device__ __forceinline__ uint4 offset32(const GofrtArgs &a, const uint4 f, uin
 // reference can count, wrapping relative to the centre picks the
 // reference's image (the cluster screen's validity condition), so the pair
 // vector is P - Q; margin (bins) bounds the error of this form for the
-// wavefront's radius.  The .w words hold 4 ti nt nbin (row) and 4 tj nbin
-// (column), so that the histogram's byte offset is their sum plus 4 bins.
-__device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t e, const uint4 *fi_lds, const uint4 qj,
-                                                   unsigned *hist, unsigned *sink, float margin) {
-    const uint4 pi = fi_lds[e & 0xFFu];
+// wavefront's radius.  Byte offsets from the LDS base lds: the entry holds
+// the row's (low half) and the column's (high half); the .w words hold the
+// histogram's offset plus 4 ti nt nbin (row) and 4 tj nbin (column), so
+// that the count's offset is their sum plus 4 bins.
+__device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t e, uint32_t sink, float margin) {
+    const u32x4_t pi = *(const lds_u4_t *)(size_t)(e & 0xFFFFu);
+    const u32x4_t qj = *(const lds_u4_t *)(size_t)(e >> 16);
     const float dx = __uint_as_float(pi.x) - __uint_as_float(qj.x);
     const float dy = __uint_as_float(pi.y) - __uint_as_float(qj.y);
     const float dz = __uint_as_float(pi.z) - __uint_as_float(qj.z);
@@ -695,7 +706,8 @@ __device__ __forceinline__ bool eval_candidate_off(const GofrtArgs &a, uint32_t 
     const bool need = fabsf(u - rintf(u)) < margin;
     const int bb = floor_i32(u);
     const bool add = !need && (unsigned)bb < a.nbin;
-    atomicAdd(add ? (unsigned *)((unsigned char *)hist + (pi.w + qj.w + ((uint32_t)bb << 2))) : sink, 1u);
+    const uint32_t at = add ? pi.w + qj.w + ((uint32_t)bb << 2) : sink;
+    __hip_atomic_fetch_add((lds_u32_t *)(size_t)at, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     return need;
 }
 
@@ -916,6 +928,10 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
     // this lane's sink entry of the recheck queue also takes the histogram
     // add of a candidate that is not counted, so that the add needs no branch
     unsigned *rsink = rq + (RQCAP - QSINK) + lane;
+    // LDS address of fi_lds (the start of the shared memory); the column
+    // cache's queue entries hold LDS addresses below 2^16 (fi_lds and the
+    // column cache take the first 32 WG bytes)
+    const uint32_t lds0 = lds_addr(smem);
     const uint64_t slot0 = (uint64_t)id.tile * WG;
     const uint64_t slot = slot0 + threadIdx.x;
     // an atom range of calc_single_th narrower than all atoms is tested per
@@ -1022,14 +1038,16 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                     const float thc = cluster_threshold(rho, a.rmaxf, a.dq);
                     // mode 2: candidates subtract FP32 offsets from the centre;
                     // the rows' offsets replace their fractions in fi_lds, which
-                    // only this wavefront reads, with .w = the row's 4 ti nt nbin
+                    // only this wavefront reads, with .w = the byte offset of the
+                    // row's histogram rows (hist + 4 ti nt nbin)
                     float margin_off = 0.0f;
                     if constexpr (COLC) {
                         margin_off = fmaf(a.du_m1, rho + a.dq, a.du_m0);
                         if (slot < N) {
                             const uint4 f = fi_lds[threadIdx.x];
-                            fi_lds[threadIdx.x] =
-                                offset32<TRI>(a, f, ca << 16, cb << 16, cc << 16, (f.w & 0xFFu) * a.nt * a.nbin * 4u);
+                            const uint32_t hoff = lds_addr(hist);
+                            fi_lds[threadIdx.x] = offset32<TRI>(a, f, ca << 16, cb << 16, cc << 16,
+                                                                hoff + (f.w & 0xFFu) * a.nt * a.nbin * 4u);
                         }
                         wave_sync_lds();
                     }
@@ -1103,18 +1121,17 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             colc[lane] = o;
                             wave_sync_lds();
                         }
-                        // COLC: entries hold the step's column c (0..63) instead of
-                        // j; a recheck entry gets j back
+                        // COLC: entries hold the LDS byte offsets of the row's
+                        // offset (low half) and the column's (high half) instead
+                        // of j and i; a recheck entry gets j << 8 | i back
                         auto eval_step = [&](uint32_t e, bool valid) {
                             if constexpr (COLC) {
                                 eval_with(
                                     e, valid,
+                                    [&](uint32_t x) { return eval_candidate_off(a, x, lds_addr(rsink), margin_off); },
                                     [&](uint32_t x) {
-                                        return eval_candidate_off(a, x, fi_lds, colc[x >> 8], hist, rsink, margin_off);
-                                    },
-                                    [&](uint32_t x) {
-                                        const uint32_t c = x >> 8;
-                                        return ((c < 32u ? jA + c : jB + (c - 32u)) << 8) | (x & 0xFFu);
+                                        const uint32_t c = (((x >> 16) - lds0) >> 4) - (uint32_t)WG - wbase;
+                                        return ((c < 32u ? jA + c : jB + (c - 32u)) << 8) | (((x & 0xFFFFu) - lds0) >> 4);
                                     });
                             } else {
                                 eval_batch(e, valid);
@@ -1141,12 +1158,15 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                             if ((uint32_t)(nearm >> (32 * jb)) == 0u) continue;  // uniform
                             const uint32_t jfirst = jb ? jB : jA;
                             uadd<TRI>(ntile, nact * (unsigned long long)min(32u, (uint32_t)N - jfirst));
-                            // entry = j << 8 | local i (COLC: the column, 32 jb + lane % 32,
-                            // instead of j); item e of tile T is row
+                            // entry = j << 8 | local i (COLC: the LDS byte offsets
+                            // 16 (WG + wbase + column) << 16 | 16 i, column = 32 jb +
+                            // lane % 32); item e of tile T is row
                             // 32 T + 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
                             uint32_t col = COLC ? 32u * (uint32_t)jb + (lane & 31u) : jfirst + (lane & 31u);
                             if constexpr (COLC) asm volatile("" : "+v"(col));  // recomputed per step, not held
-                            const uint32_t eb = (col << 8) | (wbase + half4);
+                            const uint32_t eb = COLC ? ((lds0 + 16u * ((uint32_t)WG + wbase + col)) << 16) |
+                                                           (lds0 + 16u * (wbase + half4))
+                                                     : (col << 8) | (wbase + half4);
                             // both row tiles' products are issued before either is
                             // compared, so one MFMA's latency hides behind the other's
                             const f32x16_t DT[2] = {
@@ -1172,7 +1192,7 @@ __global__ __launch_bounds__(WG, CLUSTER ? 4 : 1) void gofrt_screened_kernel(con
                                             const int e = 8 * grp + u;
                                             const uint32_t roff = 32u * (uint32_t)T + 8u * (uint32_t)(e >> 2) +
                                                                   (uint32_t)(e & 3);
-                                            ANL_APPEND_E(pc[u], mk[u], eb + roff)
+                                            ANL_APPEND_E(pc[u], mk[u], eb + (COLC ? roff << 4 : roff))
                                         }
                                         // with the column cache, one drain site per 8 compares
                                         // (the queue holds < 64 + 8 masks)

@@ -237,6 +237,48 @@ def test_fp32_window_edges(tilt, origin):
         assert 0 < rechecks < cand, (cl, rechecks, cand)
 
 
+@pytest.mark.parametrize("tilt,origin", [(None, 20.0), ((2.5, -1.5, 1.0), -6.0)], ids=["cubic", "tri-corner"])
+def test_self_pairs_at_edges(tilt, origin):
+    """Each atom moves, frame to frame, by a distance at a bin edge, rmin or
+    rmax (ulps to 1e-4 A away, along a random direction), so that the pairs
+    of an atom with itself at lag 1 sit where the FP32 window, the FP64
+    stage and the exact sequence decide them.  The screened kernels bin
+    every pair, self pairs included, into the distinct rows and then move
+    each self pair to its self row with the reference's FP64 bin: the self
+    and distinct channels must both be the reference's in every mode."""
+    from analisi_amd import synth
+    rmin, rmax, nbin = 0.5, 4.5, 40
+    dr = (rmax - rmin) / nbin
+    offs = np.array([0.0, 2e-8, 1e-6, 1e-5, 1e-4])
+    d = []
+    for k in range(0, nbin + 1):
+        e = rmin + k * dr
+        d += list(e - offs) + list(e + offs[1:])
+    d = np.array(d)
+    M = len(d)
+    L = 60.0
+    rng = np.random.default_rng(23)
+    base = origin + 14.0 * rng.random((M, 3))
+    n = rng.normal(size=(M, 3))
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    pos = base[None, :, :] + np.arange(3)[:, None, None] * d[None, :, None] * n[None, :, :]
+    types = (np.arange(M) % 2).astype(np.int32)
+    box = np.tile(synth.lammps_box(L, None if tilt is None else np.array(tilt)), (3, 1))
+    ot = O.Traj(pos, types, box, tilt is not None)
+    dt = capi.DeviceTrajectory(ot.pos, ot.box, ot.type_id, ot.ntypes, ot.cm)
+    ref, _ = O.gofrt(ot, rmin, rmax, nbin, 2, 1, 1, 1, 2, 0, want_vdata=False)
+    half = ref.shape[1] // 2
+    assert ref[1, half:].sum() >= M // 2  # most planted self pairs are counted at lag 1
+    g = capi.DeviceGofrt(dt, rmin, rmax, nbin, 2, 1, 1, capi.GOFRT_SCREENED)
+    g.reset(2)
+    for cl in (2, 1, 4, 3, 0):
+        g.set_tuning(8, 0, 1 << 30, cl)
+        got = g.calculate(0)
+        assert first_diff(got, ref) is None, (cl, first_diff(got, ref))
+        window, exact = g.recheck_stats()
+        assert exact > 0 and window > exact, (cl, window, exact)
+
+
 def test_too_short_raises():
     ot, dt = make_case(2, 50, 10, 8.0, [1.0], 0.1)
     g = capi.DeviceGofrt(dt, 0.0, 3.0, 10, 3, 1, 1)
