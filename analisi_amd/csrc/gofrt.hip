@@ -879,16 +879,22 @@ __host__ __device__ __forceinline__ float cluster_threshold(float rho, float rma
     return (a * a + b * b * 0x1p-10f) * (1.0f + 0x1p-12f) + 1e-4f;
 }
 
-// appends: lanes whose test failed store into the queue's sink
+// appends: every lane forms its slot (mbcnt on the count) and lanes whose
+// test failed store into the queue's sink instead; the empty asm keeps the
+// compiler from turning the select back into an execution-mask branch
 #define ANL_APPEND_E(P, M, ENTRY)                                           \
     if (M) {                                                                \
-        q[(P) ? qn + lane_prefix(M) : (uint32_t)(QC - QSINK) + lane] = (ENTRY); \
+        uint32_t slot_ = lane_slot(M, qn);                                  \
+        asm volatile("" : "+v"(slot_)); /* every lane: a select, no branch */ \
+        q[(P) ? slot_ : (uint32_t)(QC - QSINK) + lane] = (ENTRY);           \
         qn += (uint32_t)__popcll(M);                                        \
     }
 
 #define ANL_APPEND(P, M, J)                                                 \
     if (M) {                                                                \
-        q[(P) ? lane_slot(M, qn) : (uint32_t)(QC - QSINK) + lane] = ((uint32_t)(J) << 8) | il; \
+        uint32_t slot_ = lane_slot(M, qn);                                  \
+        asm volatile("" : "+v"(slot_)); /* every lane: a select, no branch */ \
+        q[(P) ? slot_ : (uint32_t)(QC - QSINK) + lane] = ((uint32_t)(J) << 8) | il; \
         qn += (uint32_t)__popcll(M);                                        \
     }
 
