@@ -71,3 +71,17 @@ def test_cluster_screen_stays_in_fp16_range():
                     assert s["rho_cap"] + frac * L + s["dq"] <= 128.0
                     assert s["q2_max"] <= 16400.0, (L, frac, tilt, s)
                     assert s["threshold_max"] <= 16500.0, (L, frac, tilt, s)
+
+
+def test_gofrt_kernels_have_no_static_lds():
+    """The screened g(r,t) kernel's offset evaluation carries LDS byte
+    addresses in 16-bit halves of its queue entries (rows and cached columns
+    lie in the first 2 x 256 x 16 bytes of the dynamic shared memory), which
+    holds while the dynamic shared memory starts at address 0: no kernel of
+    the file may declare static shared arrays, only the one dynamic one."""
+    with open(os.path.join(ROOT, "analisi_amd", "csrc", "gofrt.hip")) as f:
+        src = f.read()
+    decls = re.findall(r"^[^/\n]*__shared__[^;]*;", src, flags=re.M)
+    assert decls, "no shared memory declaration found"
+    for d in decls:
+        assert re.fullmatch(r"\s*extern __shared__ unsigned char smem\[\];", d), d
