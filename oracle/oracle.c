@@ -15,8 +15,10 @@
  * Parity pinning (what checks each function; see tests/):
  *   orc_gk_calculate       pinned: reference goldens gk/gk_2comp, test_gk.csv,
  *                          CLI GK_heat_single/double (with orc_mvc_*).
- *                          Schur complement for >=3 currents: parity unpinned
- *                          (Eigen PartialPivLU order), checked vs numpy only.
+ *                          Schur complement (schur00): pinned against Eigen
+ *                          3.3.9's evaluation of the reference's expression
+ *                          (ref_schur.cpp, the reference's vendored headers):
+ *                          bit-identical up to 5 currents, 1e-13 beyond.
  *   orc_mvc_* / orc_mv_*   pinned: CLI GK_heat_* text outputs; closed form.
  *   orc_min_image          pinned: brute-force lattice search, |x_k|<=l_k/2
  *                          invariant, hand-derived iteration cases.
@@ -485,8 +487,11 @@ static void inverse_pplu(double *a, int m, double *inv)
         for (int r = 0; r < m; r++) x[r] = (perm[r] == col) ? 1.0 : 0.0;
         for (int r = 0; r < m; r++)
             for (int k = 0; k < r; k++) x[r] -= a[k * m + r] * x[k];
+        /* back substitution column by column from the last (Eigen's small-
+         * panel upper solve): each row's terms are subtracted in decreasing
+         * column order, then the row is scaled by the reciprocal pivot */
         for (int r = m - 1; r >= 0; r--) {
-            for (int k = r + 1; k < m; k++) x[r] -= a[k * m + r] * x[k];
+            for (int k = m - 1; k > r; k--) x[r] -= a[k * m + r] * x[k];
             x[r] = x[r] * (1.0 / a[r * m + r]);
         }
     }
@@ -515,6 +520,13 @@ static double schur00(const double *c, int n)
     double k = c[0] - s;
     free(sub); free(inv); free(tmp);
     return k;
+}
+
+/* schur00 exported for its own test (against Eigen's evaluation of the same
+ * expression, built by oracle/Makefile's ref target into oracle/_ref). */
+double orc_schur00(const double *c, int n)
+{
+    return schur00(c, n);
 }
 
 /* greenkuboNcomponentionicfluid.cpp:122-131 reset, :156-356 calculate.

@@ -65,6 +65,8 @@ def lib():
         L.orc_msd_calculate.argtypes = [ctypes.POINTER(OrcTraj), ctypes.c_uint, ctypes.c_uint,
                                         ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
                                         _u64, _dp]
+        L.orc_schur00.restype = ctypes.c_double
+        L.orc_schur00.argtypes = [_dp, ctypes.c_int]
         L.orc_gk_calculate.restype = ctypes.c_int
         L.orc_gk_calculate.argtypes = [_dp, _u64, _u64, _up, ctypes.c_uint, ctypes.c_uint,
                                        ctypes.c_uint, ctypes.c_int, ctypes.c_uint, ctypes.c_uint,
@@ -337,3 +339,30 @@ class MediaVarCovar:
 
     def end(self):
         lib().orc_mvc_end(_p(self.var), _p(self.cov), self.sb, self.nc, self.ncv, self.iblock)
+
+
+def schur00(M):
+    """The (0,0) Schur complement of the square matrix M, as the oracle's GK
+    epilogue computes it (greenkuboNcomponentionicfluid.cpp:309-316)."""
+    M = np.asfortranarray(np.asarray(M, dtype=np.float64))
+    col = np.ascontiguousarray(M.T).ravel()  # column-major order
+    return lib().orc_schur00(_p(col), M.shape[0])
+
+
+REF_SCHUR = os.path.join(_HERE, "_ref", "ref_schur")
+
+
+def ref_schur00(mats):
+    """The same complement of each matrix by Eigen's evaluation of the
+    reference's expression (oracle/_ref/ref_schur, built from ref_schur.cpp
+    against the reference's vendored Eigen); None when it is not built."""
+    if not os.path.exists(REF_SCHUR):
+        return None
+    import struct
+    parts = [struct.pack("<i", len(mats))]
+    for M in mats:
+        M = np.asarray(M, dtype=np.float64)
+        parts.append(struct.pack("<i", M.shape[0]))
+        parts.append(np.ascontiguousarray(M.T).tobytes())  # column-major
+    r = subprocess.run([REF_SCHUR], input=b"".join(parts), capture_output=True, check=True)
+    return np.frombuffer(r.stdout, dtype=np.float64)

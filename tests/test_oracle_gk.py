@@ -75,7 +75,8 @@ def test_gk_integrals_closed_form():
 
 def test_gk_schur_three_currents_vs_numpy():
     """n=3 currents: k = M00 - M01 inv(M11) M10 of the symmetrised integral
-    matrix (parity vs Eigen unpinned; checked against numpy.linalg)."""
+    matrix, against numpy.linalg (the bit-level pin against Eigen's own
+    evaluation is test_schur_matches_eigen below)."""
     from analisi_amd import synth
     J = synth.ar2_currents(3, 4000, ncur=3)
     out = O.gk(J, [0, 3, 6], 1, 200, False, 0, 1, 3000, 0)
@@ -96,3 +97,62 @@ def test_mediavar_closed_form():
     mv.end(7)
     np.testing.assert_allclose(mv.mean, blocks.mean(0), rtol=1e-13, atol=1e-15)
     np.testing.assert_allclose(mv.var, blocks.var(0, ddof=1) / 7, rtol=1e-12)
+
+
+def _eigen_or_skip(mats):
+    ref = O.ref_schur00(mats)
+    if ref is None:
+        pytest.skip("oracle/_ref/ref_schur not built (it needs the reference's vendored Eigen headers)")
+    return ref
+
+
+def test_schur_matches_eigen():
+    """The oracle's (0,0) Schur complement against Eigen 3.3.9's evaluation of
+    the reference's expression (greenkuboNcomponentionicfluid.cpp:309-316,
+    oracle/ref_schur.cpp): bit-identical up to 5 currents (a 4 x 4 inverse,
+    where Eigen's LU and triangular solves run element by element); from 6
+    currents Eigen's triangular solve updates through its packed block kernel,
+    whose sums are ordered differently, and the two agree to a few ulps."""
+    rng = np.random.default_rng(2024)
+    for n in range(1, 9):
+        mats = []
+        for t in range(300):
+            A = rng.normal(size=(n, n)) * 10.0 ** rng.uniform(-3, 3)
+            M = (A + A.T) / 2
+            if t % 3:
+                M += n * np.abs(M).max() * np.eye(n)   # well conditioned
+            mats.append(M)
+        ref = _eigen_or_skip(mats)
+        got = np.array([O.schur00(M) for M in mats])
+        if n <= 5:
+            assert (got == ref).all(), (n, int((got != ref).sum()))
+        else:
+            np.testing.assert_allclose(got, ref, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("ncur", [3, 4, 5])
+def test_gk_kappa_matches_eigen(ncur):
+    """The whole GK epilogue for 3 to 5 currents: the symmetrised integral
+    matrix built as the reference builds it (each product's integral added
+    as half to both mirrored entries, greenkuboNcomponentionicfluid.cpp:
+    293-306) and its complement by Eigen must equal the oracle's k_gk bit for
+    bit at every lag."""
+    from analisi_amd import synth
+    J = synth.ar2_currents(7 + ncur, 3000, ncur=ncur)
+    idx = [3 * k for k in range(ncur)]
+    out = O.gk(J, idx, 1, 120, False, 0, 1, 2500, 0)
+    N = ncur * ncur
+    mats = []
+    for l in range(1, out.shape[0]):
+        M = np.zeros((ncur, ncur))
+        i = 0
+        for j1 in range(ncur):
+            for j2 in range(ncur):
+                v = out[l, N + 2 * i]
+                M[j1, j2] += v / 2.0   # matr[j2 * n + j1], column-major (row j1, column j2)
+                M[j2, j1] += v / 2.0
+                i += 1
+        mats.append(M)
+    ref = _eigen_or_skip(mats)
+    got = out[1:, 3 * N]
+    assert (got == ref).all(), int((got != ref).sum())
