@@ -25,6 +25,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -308,9 +309,41 @@ class Trajectory_numpy {
         bool diagonal;
     };
     std::vector<Lcell> rotations;
-    // triclinic.h:10-40: cell is the row-major numpy [3][3] seen column-major,
-    // QR of its transpose by Householder reflections, then signs flipped so
-    // that the diagonal is positive.
+    // triclinic.h:10-40: cell is the row-major numpy [3][3] seen column-major;
+    // the QR of its transpose (the numpy matrix itself, rows = cell vectors)
+    // by Householder reflections, then the signs of R's diagonal made
+    // positive.  The reference calls Eigen's HouseholderQR on a fixed 3 x 3
+    // matrix, so the steps and their rounding follow that computation (Eigen
+    // 3.3.9, the version the reference vendors): per column k the reflector
+    // (makeHouseholder: beta = -sign(c0) |x|, essential = tail / (c0 - beta),
+    // tau = (beta - c0) / beta, a zero tail leaving the column as it is),
+    // applied to the columns on its right (tmp = essential . rows below +
+    // row k; row k -= tau tmp; rows below -= tmp (tau essential)); Q from the
+    // identity with the reflectors applied last to first; the sign flips as
+    // products with diag(.., -1, ..).  Sums here have at most two terms
+    // before the row term, so their order is Eigen's.
+    static void apply_reflector(double (&M)[3][3], int k, const double (&A)[3][3], double tau, int c0, int c1) {
+        // rows k..2 of columns c0..c1 of M, the reflector's essential part in A(k+1.., k)
+        for (int j = c0; j <= c1; j++) {
+            double t = 0;
+            if (k == 0)
+                t = A[1][0] * M[1][j] + A[2][0] * M[2][j];
+            else if (k == 1)
+                t = A[2][1] * M[2][j];
+            t += M[k][j];
+            M[k][j] -= tau * t;
+            for (int i = k + 1; i < 3; i++) M[i][j] -= t * (tau * A[i][k]);
+        }
+    }
+    // r = r diag(s) (s_i = -1 or 1) as a 3 x 3 product, sums in column order
+    static void times_sign(double (&r)[3][3], int i) {
+        double m[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        m[i][i] = -1;
+        double o[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) o[a][b] = r[a][0] * m[0][b] + r[a][1] * m[1][b] + r[a][2] * m[2][b];
+        std::memcpy(r, o, sizeof(o));
+    }
     static Lcell lammps_cell(const double *m) {
         Lcell L{};
         double cell[3][3];  // cell(i,j) = m[j*3+i]
@@ -327,46 +360,40 @@ class Trajectory_numpy {
             return L;
         }
         L.diagonal = false;
-        double A[3][3], Q[3][3];
+        double A[3][3], tau[3];
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                A[i][j] = cell[j][i];
-                Q[i][j] = i == j ? 1.0 : 0.0;
+            for (int j = 0; j < 3; j++) A[i][j] = cell[j][i];
+        for (int k = 0; k < 3; k++) {
+            const double tail = k == 0 ? A[1][0] * A[1][0] + A[2][0] * A[2][0] : (k == 1 ? A[2][1] * A[2][1] : 0.0);
+            const double c0 = A[k][k];
+            double beta;
+            if (tail <= std::numeric_limits<double>::min()) {
+                tau[k] = 0;
+                beta = c0;
+                for (int i = k + 1; i < 3; i++) A[i][k] = 0;
+            } else {
+                beta = std::sqrt(c0 * c0 + tail);
+                if (c0 >= 0) beta = -beta;
+                for (int i = k + 1; i < 3; i++) A[i][k] = A[i][k] / (c0 - beta);
+                tau[k] = (beta - c0) / beta;
             }
-        for (int k = 0; k < 2; k++) {
-            double norm = 0;
-            for (int i = k; i < 3; i++) norm += A[i][k] * A[i][k];
-            norm = std::sqrt(norm);
-            if (norm == 0) continue;
-            double alpha = A[k][k] > 0 ? -norm : norm;
-            double v[3] = {0, 0, 0};
-            for (int i = k; i < 3; i++) v[i] = A[i][k];
-            v[k] -= alpha;
-            double vv = 0;
-            for (int i = k; i < 3; i++) vv += v[i] * v[i];
-            if (vv == 0) continue;
-            for (int j = 0; j < 3; j++) {
-                double s = 0;
-                for (int i = k; i < 3; i++) s += v[i] * A[i][j];
-                for (int i = k; i < 3; i++) A[i][j] -= 2 * v[i] * s / vv;
-            }
-            for (int j = 0; j < 3; j++) {  // Q = Q H
-                double s = 0;
-                for (int i = k; i < 3; i++) s += Q[j][i] * v[i];
-                for (int i = k; i < 3; i++) Q[j][i] -= 2 * s * v[i] / vv;
-            }
+            A[k][k] = beta;
+            if (k < 2 && tau[k] != 0) apply_reflector(A, k, A, tau[k], k + 1, 2);
         }
+        double Q[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        for (int k = 1; k >= 0; k--)  // the last reflector is the identity (tau = 0)
+            if (tau[k] != 0) apply_reflector(Q, k, A, tau[k], k, 2);
+        double r[3][3];  // R's upper triangle, transposed (the lammps cell rows)
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < i; j++) A[i][j] = 0;
+            for (int j = 0; j < 3; j++) r[i][j] = j <= i ? A[j][i] : 0.0;
         for (int i = 0; i < 3; i++)  // triclinic.h:30-38 reflect
-            if (A[i][i] < 0)
-                for (int j = 0; j < 3; j++) {
-                    A[i][j] = -A[i][j];
-                    Q[j][i] = -Q[j][i];
-                }
+            if (r[i][i] < 0) {
+                times_sign(r, i);
+                times_sign(Q, i);
+            }
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
-                L.r[i][j] = A[j][i];
+                L.r[i][j] = r[i][j];
                 L.q[j * 3 + i] = Q[i][j];
             }
         return L;

@@ -366,3 +366,29 @@ def ref_schur00(mats):
         parts.append(np.ascontiguousarray(M.T).tobytes())  # column-major
     r = subprocess.run([REF_SCHUR], input=b"".join(parts), capture_output=True, check=True)
     return np.frombuffer(r.stdout, dtype=np.float64)
+
+
+REF_CELL = os.path.join(_HERE, "_ref", "ref_cell")
+
+
+def ref_cell(cells, vecs):
+    """The reference's lammps cell from cell vectors by Eigen
+    (triclinic.h:10-64; oracle/_ref/ref_cell, built from ref_cell.cpp against
+    the reference's vendored Eigen): for each frame's numpy cell [3][3] and
+    row vectors [n][3], the LAMMPS box (9 values), Q (column-major) and the
+    rotated vectors.  None when it is not built."""
+    if not os.path.exists(REF_CELL):
+        return None
+    import struct
+    parts = [struct.pack("<i", len(cells))]
+    for c, v in zip(cells, vecs):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        parts += [np.ascontiguousarray(c, dtype=np.float64).tobytes(), struct.pack("<i", len(v)), v.tobytes()]
+    r = subprocess.run([REF_CELL], input=b"".join(parts), capture_output=True, check=True)
+    flat = np.frombuffer(r.stdout, dtype=np.float64)
+    out, o = [], 0
+    for v in vecs:
+        n = len(np.asarray(v).reshape(-1, 3))
+        out.append((flat[o:o + 9], flat[o + 9:o + 18], flat[o + 18:o + 18 + 3 * n].reshape(n, 3)))
+        o += 18 + 3 * n
+    return out
