@@ -67,3 +67,29 @@ def test_cell_vectors_rotation_is_a_rotation():
         assert box[f][3] > 0 and box[f][4] > 0 and box[f][5] > 0
         L = np.array([[2 * box[f][3], 0, 0], [box[f][6], 2 * box[f][4], 0], [box[f][7], box[f][8], 2 * box[f][5]]])
         np.testing.assert_allclose(P[f], L, atol=1e-12 * np.abs(cells[f]).max())
+
+
+def test_degenerate_cells_match_eigen():
+    """Eigen's reflector skips a column whose tail is at most the smallest
+    normal double (tau = 0): a zero column, a denormal tilt, tiny pivots;
+    and a cell whose R needs all three reflections."""
+    from analisi_amd import pyanalisi as pa
+    cells = [np.array([[0, 1., 2.], [0, 3., 1.], [0, 0.5, 4.]]),
+             np.array([[5., 1e-320, 0], [0, 6., 2.], [0, 0, 7.]]),
+             np.array([[-5., 0, 0], [1., -6., 0], [2., 3., -7.]]),
+             np.array([[0, 0, 3.], [0, 4., 0], [5., 0, 0]]),
+             np.array([[1e-300, 2., 0], [3., 1e-300, 0], [0, 0, 1.]])]
+    N = 2
+    rng = np.random.default_rng(0)
+    pos = rng.normal(size=(len(cells), N, 3))
+    vel = rng.normal(size=(len(cells), N, 3))
+    ref = O.ref_cell(cells, [np.concatenate([pos[f], vel[f]]) for f in range(len(cells))])
+    if ref is None:
+        pytest.skip("oracle/_ref/ref_cell not built (it needs the reference's vendored Eigen headers)")
+    t = pa.Trajectory(pos, vel, np.zeros(N, np.int32), np.array(cells), pa.BoxFormat.CellVectors, False, True)
+    box, rot, P = t.get_box_copy(), t.get_rotation_matrix(), t.get_positions_copy()
+    for f in range(len(cells)):
+        b, q, v = ref[f]
+        np.testing.assert_array_equal(box[f], b, err_msg="box, cell %d" % f)
+        np.testing.assert_array_equal(rot[f].ravel(), q, err_msg="rotation, cell %d" % f)
+        np.testing.assert_array_equal(P[f], v[:N], err_msg="positions, cell %d" % f)
